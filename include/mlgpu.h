@@ -1,0 +1,235 @@
+/*
+ * mlgpu.h -- C ABI of the MI355X photon propagator for merlict sceneries.
+ *
+ * Every entry point is plain C: pointers, sizes and status codes, no C++ or
+ * torch types.  Device-side entry points take DEVICE pointers and a
+ * hipStream_t passed as void* (stream-ordered, asynchronous).  Host-side
+ * entry points (scene description, scene builders) work without a GPU.
+ *
+ * Units follow merlict: metres, seconds, wavelength in metres.
+ */
+#ifndef MLGPU_H_
+#define MLGPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLGPU_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- status */
+enum mlgpu_status {
+    MLGPU_OK = 0,
+    MLGPU_ERR_ARGUMENT = 1,    /* bad pointer / size / index              */
+    MLGPU_ERR_SCENE = 2,       /* inconsistent scene description           */
+    MLGPU_ERR_HIP = 3,         /* a HIP runtime call failed                */
+    MLGPU_ERR_NO_DEVICE = 4,   /* no usable GPU                            */
+    MLGPU_ERR_PHOTON = 5,      /* at least one photon carries an error flag */
+    MLGPU_ERR_UNSUPPORTED = 6
+};
+
+/* Text of the last error raised on the calling thread. */
+const char* mlgpu_last_error(void);
+int mlgpu_abi_version(void);
+
+/* ------------------------------------------------------ scene description
+ * A flat list of frames in creation order, mother before child, frame 0 is
+ * the root ("world").  It mirrors merlict's Frame tree before
+ * init_tree_based_on_mother_child_relations(): every frame carries its pose
+ * in its mother (position + Rot3), a surface kind with parameters, the eight
+ * surface functions (merlict/SurfaceEntity.h:21-28; colours are not needed)
+ * and an optional restriction frame (SurfaceEntity.h:13).
+ * A frame with children is never intersected itself (RayAndFrame.cpp:141-147).
+ */
+enum mlgpu_kind {
+    MLGPU_KIND_FRAME = 0,               /* no surface                       */
+    MLGPU_KIND_SPHERE_CAP_HEX = 1,      /* p0 curvature R, p1 outer hex r   */
+    MLGPU_KIND_SPHERE_CAP_CYL = 2,      /* p0 curvature R, p1 cap radius    */
+    MLGPU_KIND_SPHERE_CAP_RECT = 3,     /* p0 R, p1 x width, p2 y width     */
+    MLGPU_KIND_PLANE = 4,               /* p0 x width, p1 y width           */
+    MLGPU_KIND_DISC = 5,                /* p0 radius                        */
+    MLGPU_KIND_HEX_PLANE = 6,           /* p0 outer hex radius              */
+    MLGPU_KIND_ANNULUS = 7,             /* p0 outer radius, p1 inner radius */
+    MLGPU_KIND_PLANE_DUAL_SPHERICAL = 8,/* p0 x hight, p1 y width           */
+    MLGPU_KIND_CYLINDER = 9,            /* p0 radius, p1 length (along z)   */
+    MLGPU_KIND_TRIANGLE = 10,           /* p0..p5 Ax Ay Bx By Cx Cy         */
+    MLGPU_KIND_SPHERE = 11,             /* p0 radius                        */
+    MLGPU_NUM_KINDS = 12
+};
+
+enum mlgpu_rot_mode {
+    MLGPU_ROT_XYZ = 0,   /* rot[0..2] = Tait-Bryan x, y, z angles (Rot3)   */
+    MLGPU_ROT_AXIS = 1   /* rot[0..2] = axis, rot[3] = angle              */
+};
+
+/* surface slots, same order as the SurfaceEntity members */
+enum mlgpu_surface_slot {
+    MLGPU_OUTER_REFLECTION = 0,
+    MLGPU_INNER_REFLECTION = 1,
+    MLGPU_OUTER_REFRACTION = 2,
+    MLGPU_INNER_REFRACTION = 3,
+    MLGPU_OUTER_ABSORPTION = 4,
+    MLGPU_INNER_ABSORPTION = 5,
+    MLGPU_NUM_SURFACE_SLOTS = 6
+};
+
+#define MLGPU_NUM_PARAMS 6
+#define MLGPU_DEFAULT_FUNCTION (-1) /* slot uses merlict's DEFAULT_* table */
+#define MLGPU_NO_FRAME (-1)
+#define MLGPU_NO_SENSOR (-1)
+
+typedef struct mlgpu_scene_desc {
+    int32_t n_frames;
+    const int32_t* mother;        /* [n]   -1 for the root, else < own index */
+    const double* pos;            /* [n*3] position in mother               */
+    const int32_t* rot_mode;      /* [n]   mlgpu_rot_mode                   */
+    const double* rot;            /* [n*4]                                  */
+    const int32_t* kind;          /* [n]   mlgpu_kind                       */
+    const double* params;         /* [n*MLGPU_NUM_PARAMS]                   */
+    const int32_t* surface;       /* [n*6] function index or -1 (default)   */
+    const int32_t* allowed_frame; /* [n]   restriction frame or -1          */
+    const int32_t* sensor_id;     /* [n]   sensor id or -1                  */
+    int32_t n_functions;
+    const int32_t* func_start;    /* [n_functions+1] offsets into func_x/y  */
+    const double* func_x;         /* argument (wavelength), table order     */
+    const double* func_y;         /* value                                  */
+} mlgpu_scene_desc;
+
+/* Owned, growable scene description produced by the scene builders. */
+typedef struct mlgpu_desc mlgpu_desc;
+
+void mlgpu_desc_destroy(mlgpu_desc* d);
+/* View of an owned description; valid until the description is destroyed. */
+int mlgpu_desc_view(const mlgpu_desc* d, mlgpu_scene_desc* out);
+/* Write / read the binary description file ("MLGPUSC1"). */
+int mlgpu_desc_save(const mlgpu_desc* d, const char* path);
+int mlgpu_desc_load(const char* path, mlgpu_desc** out);
+
+/* Scene builders (host only).  Each returns a new owned description.
+ * Lens test bench: merlict BiConvexLens (lens_maker.cpp:15-34 curvature for
+ * focal length f, aperture radius r, index n) at z = 0 and a 19-cell hexagonal
+ * HexPlane flower at z = -f (pitch p).  */
+int mlgpu_build_lens_scene(double focal_length, double aperture_radius,
+                           double refractive_index, double cell_pitch,
+                           mlgpu_desc** out);
+/* FACT-like telescope: segmented_imaging_reflector (30 facets, f = 4.889 m),
+ * 1440 hexagonal pixels of 9.5 mm pitch at the focal plane and a shield disc;
+ * with_light_guides != 0 adds a hexagonal PMMA light guide above each pixel. */
+int mlgpu_build_fact_scene(int with_light_guides, mlgpu_desc** out);
+/* Plenoscope: segmented reflector (f = 106.5 m, 71 m aperture) and the
+ * light-field sensor of light_field_sensor/Factory.cpp (8,443 lenslets,
+ * 515,023 lixels for num_paxel_on_pixel_diagonal = 9). */
+int mlgpu_build_plenoscope_scene(double focal_length,
+                                 double max_outer_aperture_radius,
+                                 double min_inner_aperture_radius,
+                                 double facet_inner_hex_radius,
+                                 double gap_between_facets,
+                                 double max_FoV_diameter_deg,
+                                 double hex_pixel_FoV_flat2flat_deg,
+                                 int num_paxel_on_pixel_diagonal,
+                                 double housing_overhead,
+                                 double lens_refraction,
+                                 double mirror_reflection,
+                                 mlgpu_desc** out);
+
+/* --------------------------------------------------------------- photons
+ * Structure of arrays.  For mlgpu_propagate all pointers are device pointers.
+ * Direction need not be normalised (Ray.cpp:16-20 normalises). */
+typedef struct mlgpu_photons {
+    const double* support_x;
+    const double* support_y;
+    const double* support_z;
+    const double* direction_x;
+    const double* direction_y;
+    const double* direction_z;
+    const double* wavelength;
+} mlgpu_photons;
+
+/* merlict::Interaction (RayForPropagation.h:14-28) */
+enum mlgpu_interaction {
+    MLGPU_PRODUCTION = 0,
+    MLGPU_ABSORPTION_IN_VOID = 1,
+    MLGPU_ABSORPTION_IN_MEDIUM = 2,
+    MLGPU_ABSORPTION_ON_SURFACE = 3,
+    MLGPU_FRESNEL_REFLECTION_ON_SURFACE = 4,
+    MLGPU_REFLECTION_ON_SURFACE = 5,
+    MLGPU_REFRACTION_TO_OUTSIDE = 6,
+    MLGPU_REFRACTION_TO_INSIDE = 7
+};
+
+/* per-photon error flags */
+enum mlgpu_photon_error {
+    MLGPU_PHOTON_OK = 0,
+    MLGPU_PHOTON_BAD_WAVELENGTH = 1,     /* lambda <= 0 (Photon.cpp:20-28)   */
+    MLGPU_PHOTON_OUT_OF_TABLE = 2        /* Func1 range (Limits.cpp:37-46)   */
+};
+
+/* Results, one entry per photon (device pointers for mlgpu_propagate). */
+typedef struct mlgpu_results {
+    int32_t* final_frame;       /* frame index of final intersection, -1 void */
+    int32_t* sensor_id;         /* sensor of the final frame, -1 none        */
+    uint8_t* final_interaction; /* mlgpu_interaction                         */
+    uint8_t* num_interactions;  /* history length incl. production           */
+    uint8_t* error;             /* mlgpu_photon_error                        */
+    double* time_of_flight;     /* Photon::time_of_flight (Photon.cpp:44-56) */
+    float* x;                   /* final position, object frame              */
+    float* y;
+    float* cx;                  /* -incident direction, object frame         */
+    float* cy;
+} mlgpu_results;
+
+typedef struct mlgpu_config {
+    uint32_t max_num_interactions_per_photon; /* PropagationConfig.cpp:7 = 5 */
+    uint64_t seed;             /* Philox key                                 */
+    uint64_t photon_id_offset; /* Philox stream of photon i is offset + i    */
+} mlgpu_config;
+
+/* ------------------------------------------------------------- device scene */
+typedef struct mlgpu_scene mlgpu_scene;
+
+/* Build the traversal hierarchy on the host and upload it to the current
+ * device.  Replaces init_tree_based_on_mother_child_relations()
+ * (merlict/Frame.cpp:125-130) for the GPU. */
+int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out);
+void mlgpu_scene_destroy(mlgpu_scene* scene);
+/* hierarchy statistics: nodes, leaves, device bytes */
+int mlgpu_scene_stats(const mlgpu_scene* scene, int64_t* num_nodes,
+                      int64_t* num_leaves, int64_t* device_bytes);
+
+/* Propagate n photons, one Philox stream per photon.  Replaces
+ * merlict::propagate_photons_in_frame_with_config (merlict/Photons.cpp:9-22)
+ * and its multi-thread twin (merlict_multi_thread.cpp:28-55).  world_frame
+ * is the frame the propagation starts in (usually 0).  Asynchronous on
+ * `stream` (a hipStream_t, NULL = default stream). */
+int mlgpu_propagate(const mlgpu_scene* scene, int32_t world_frame,
+                    const mlgpu_photons* photons, int64_t n,
+                    const mlgpu_config* config, mlgpu_results* results,
+                    void* stream);
+
+/* Number of photons with a non-zero error flag in the last n results
+ * (synchronises `stream`). */
+int mlgpu_count_errors(const mlgpu_results* results, int64_t n, void* stream,
+                       int64_t* num_errors);
+
+/* First hit of probe rays.  Replaces
+ * merlict::rays_first_intersection_with_frame (RayAndFrame.cpp:118-124). */
+typedef struct mlgpu_hits {
+    int32_t* frame;      /* frame index, -1 = no hit (void)        */
+    double* distance;    /* distance to the ray support; 1e4 for void */
+    double* position_x;  /* hit position in the root frame          */
+    double* position_y;
+    double* position_z;
+} mlgpu_hits;
+
+int mlgpu_first_hit(const mlgpu_scene* scene, int32_t root_frame,
+                    const mlgpu_photons* rays, int64_t n, mlgpu_hits* hits,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* MLGPU_H_ */

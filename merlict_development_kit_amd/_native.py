@@ -1,0 +1,142 @@
+"""ctypes binding of ``libmlgpu.so`` (the C ABI declared in ``include/mlgpu.h``).
+
+The library is built in-tree by :mod:`merlict_development_kit_amd.build`.
+Loading fails loudly when it is missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmlgpu.so"
+
+MLGPU_OK = 0
+MLGPU_NUM_PARAMS = 6
+MLGPU_NUM_SURFACE_SLOTS = 6
+
+KIND = {
+    "Frame": 0,
+    "SphereCapWithHexagonalBound": 1,
+    "SphereCapWithCylinderBound": 2,
+    "SphereCapWithRectangularBound": 3,
+    "Plane": 4,
+    "Disc": 5,
+    "HexPlane": 6,
+    "Annulus": 7,
+    "PlaneDualSphericalBound": 8,
+    "Cylinder": 9,
+    "Triangle": 10,
+    "Sphere": 11,
+}
+
+# merlict::Interaction (merlict/RayForPropagation.h:14-28)
+PRODUCTION = 0
+ABSORPTION_IN_VOID = 1
+ABSORPTION_IN_MEDIUM = 2
+ABSORPTION_ON_SURFACE = 3
+FRESNEL_REFLECTION_ON_SURFACE = 4
+REFLECTION_ON_SURFACE = 5
+REFRACTION_TO_OUTSIDE = 6
+REFRACTION_TO_INSIDE = 7
+
+PHOTON_OK = 0
+PHOTON_BAD_WAVELENGTH = 1
+PHOTON_OUT_OF_TABLE = 2
+
+_P_I32 = C.POINTER(C.c_int32)
+_P_F64 = C.POINTER(C.c_double)
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("n_frames", C.c_int32),
+        ("mother", _P_I32),
+        ("pos", _P_F64),
+        ("rot_mode", _P_I32),
+        ("rot", _P_F64),
+        ("kind", _P_I32),
+        ("params", _P_F64),
+        ("surface", _P_I32),
+        ("allowed_frame", _P_I32),
+        ("sensor_id", _P_I32),
+        ("n_functions", C.c_int32),
+        ("func_start", _P_I32),
+        ("func_x", _P_F64),
+        ("func_y", _P_F64),
+    ]
+
+
+class Photons(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in (
+        "support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z", "wavelength")]
+
+
+class Results(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in (
+        "final_frame", "sensor_id", "final_interaction", "num_interactions", "error", "time_of_flight",
+        "x", "y", "cx", "cy")]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("max_num_interactions_per_photon", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("photon_id_offset", C.c_uint64),
+    ]
+
+
+class Hits(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in ("frame", "distance", "position_x", "position_y", "position_z")]
+
+
+# every function include/mlgpu.h declares, with its ctypes signature
+SIGNATURES = {
+    "mlgpu_last_error": (C.c_char_p, []),
+    "mlgpu_abi_version": (C.c_int, []),
+    "mlgpu_desc_destroy": (None, [C.c_void_p]),
+    "mlgpu_desc_view": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "mlgpu_desc_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "mlgpu_desc_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "mlgpu_build_lens_scene": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
+    "mlgpu_build_fact_scene": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "mlgpu_build_plenoscope_scene": (C.c_int, [C.c_double] * 7 + [C.c_int, C.c_double, C.c_double, C.c_double,
+                                                                   C.POINTER(C.c_void_p)]),
+    "mlgpu_scene_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    "mlgpu_scene_destroy": (None, [C.c_void_p]),
+    "mlgpu_scene_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mlgpu_propagate": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
+                                  C.POINTER(Results), C.c_void_p]),
+    "mlgpu_count_errors": (C.c_int, [C.POINTER(Results), C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
+    "mlgpu_first_hit": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Hits),
+                                  C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded native library (raises when it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing; build it with `python -m merlict_development_kit_amd.build`")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+class MlgpuError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        msg = lib().mlgpu_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed with status {status}: {msg}")
+        self.status = status
+
+
+def check(status: int, where: str) -> None:
+    if status != MLGPU_OK:
+        raise MlgpuError(status, where)

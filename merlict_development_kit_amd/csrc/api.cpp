@@ -1,0 +1,175 @@
+// C ABI of the device side (include/mlgpu.h): scene upload, propagation,
+// probe-ray first hits.  Never throws across the ABI; errors are status
+// codes plus mlgpu_last_error().
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "scene_build.hpp"
+
+struct mlgpu_scene {
+    mlgpu::DeviceScene dev;
+    std::vector<void*> buffers;
+    std::vector<int32_t> scope_node;
+    int64_t num_nodes = 0;
+    int64_t num_leaves = 0;
+    int64_t device_bytes = 0;
+    int device = 0;
+};
+
+namespace {
+
+using mlgpu::set_last_error;
+
+int hip_fail(hipError_t e, const char* what) {
+    set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+    return MLGPU_ERR_HIP;
+}
+
+template <typename T>
+int upload(mlgpu_scene* s, const std::vector<T>& v, const T** out) {
+    const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    s->buffers.push_back(p);
+    s->device_bytes += static_cast<int64_t>(bytes);
+    if (!v.empty()) {
+        e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+    }
+    *out = static_cast<const T*>(p);
+    return MLGPU_OK;
+}
+
+bool photons_ok(const mlgpu_photons* p) {
+    return p && p->support_x && p->support_y && p->support_z && p->direction_x && p->direction_y &&
+           p->direction_z && p->wavelength;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
+    if (!desc || !out) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+        set_last_error("no HIP device");
+        return MLGPU_ERR_NO_DEVICE;
+    }
+    mlgpu::HostScene hs;
+    try {
+        mlgpu::build_host_scene(*desc, &hs);
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return MLGPU_ERR_SCENE;
+    }
+    auto* s = new mlgpu_scene();
+    hipGetDevice(&s->device);
+    int rc = MLGPU_OK;
+    if ((rc = upload(s, hs.nodes, &s->dev.nodes)) || (rc = upload(s, hs.geo, &s->dev.geo)) ||
+        (rc = upload(s, hs.phys, &s->dev.phys)) || (rc = upload(s, hs.funcs, &s->dev.funcs)) ||
+        (rc = upload(s, hs.points, &s->dev.points))) {
+        mlgpu_scene_destroy(s);
+        return rc;
+    }
+    s->dev.num_nodes = static_cast<int32_t>(hs.nodes.size());
+    s->dev.num_leaves = static_cast<int32_t>(hs.geo.size());
+    s->dev.root_node = hs.root_node;
+    s->dev.default_refraction = hs.default_refraction;
+    s->dev.eps = static_cast<float>(hs.eps);
+    s->dev.root_r = hs.nodes[hs.root_node].r;
+    s->dev.root_cx = hs.nodes[hs.root_node].cx;
+    s->dev.root_cy = hs.nodes[hs.root_node].cy;
+    s->dev.root_cz = hs.nodes[hs.root_node].cz;
+    s->scope_node = hs.scope_node;
+    s->num_nodes = static_cast<int64_t>(hs.nodes.size());
+    s->num_leaves = static_cast<int64_t>(hs.geo.size());
+    *out = s;
+    return MLGPU_OK;
+}
+
+void mlgpu_scene_destroy(mlgpu_scene* s) {
+    if (!s) return;
+    for (void* p : s->buffers) hipFree(p);
+    delete s;
+}
+
+int mlgpu_scene_stats(const mlgpu_scene* s, int64_t* num_nodes, int64_t* num_leaves, int64_t* device_bytes) {
+    if (!s) { set_last_error("null scene"); return MLGPU_ERR_ARGUMENT; }
+    if (num_nodes) *num_nodes = s->num_nodes;
+    if (num_leaves) *num_leaves = s->num_leaves;
+    if (device_bytes) *device_bytes = s->device_bytes;
+    return MLGPU_OK;
+}
+
+int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photons* photons, int64_t n,
+                    const mlgpu_config* cfg, mlgpu_results* r, void* stream) {
+    if (!s || !cfg || !r || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (n == 0) return MLGPU_OK;
+    if (!photons_ok(photons)) { set_last_error("null photon array"); return MLGPU_ERR_ARGUMENT; }
+    if (!r->final_frame || !r->sensor_id || !r->final_interaction || !r->num_interactions || !r->error ||
+        !r->time_of_flight || !r->x || !r->y || !r->cx || !r->cy) {
+        set_last_error("null result array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (world_frame < 0 || world_frame >= static_cast<int32_t>(s->scope_node.size()) ||
+        s->scope_node[world_frame] < 0) {
+        set_last_error("world_frame must be frame 0 or a restriction frame");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (cfg->max_num_interactions_per_photon > 250) {
+        set_last_error("max_num_interactions_per_photon must be <= 250");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    hipError_t e = mlgpu::launch_propagate(s->dev, s->scope_node[world_frame], *photons, n,
+                                           cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset,
+                                           *r, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "propagate launch");
+    return MLGPU_OK;
+}
+
+int mlgpu_count_errors(const mlgpu_results* r, int64_t n, void* stream, int64_t* num_errors) {
+    if (!r || !r->error || !num_errors || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    *num_errors = 0;
+    if (n == 0) return MLGPU_OK;
+    unsigned long long* counter = nullptr;
+    hipError_t e = hipMalloc(&counter, sizeof(unsigned long long));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    e = hipMemsetAsync(counter, 0, sizeof(unsigned long long), st);
+    if (e == hipSuccess) e = mlgpu::launch_count_errors(r->error, n, counter, st);
+    unsigned long long host = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&host, counter, sizeof(host), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(counter);
+    if (e != hipSuccess) return hip_fail(e, "count errors");
+    *num_errors = static_cast<int64_t>(host);
+    return MLGPU_OK;
+}
+
+int mlgpu_first_hit(const mlgpu_scene* s, int32_t root_frame, const mlgpu_photons* rays, int64_t n,
+                    mlgpu_hits* hits, void* stream) {
+    if (!s || !hits || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (n == 0) return MLGPU_OK;
+    if (!photons_ok(rays) || !hits->frame || !hits->distance || !hits->position_x || !hits->position_y ||
+        !hits->position_z) {
+        set_last_error("null array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (root_frame < 0 || root_frame >= static_cast<int32_t>(s->scope_node.size()) ||
+        s->scope_node[root_frame] < 0) {
+        set_last_error("root_frame must be frame 0 or a restriction frame");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    hipError_t e = mlgpu::launch_first_hit(s->dev, s->scope_node[root_frame], *rays, n, *hits,
+                                           static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "first_hit launch");
+    return MLGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,408 @@
+// Host-side flattening of a scene description into the device layout of
+// device_scene.hpp.  Replaces merlict's post-initialisation
+// (Frame::init_tree_based_on_mother_child_relations, merlict/Frame.cpp:125-147)
+// for the GPU: frame2world poses, leaf geometry, surface tables and a
+// bounding-sphere hierarchy in depth-first order with skip indices.
+#include "scene_build.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+namespace mlgpu {
+namespace {
+
+struct M34 {
+    double T[3][4];
+};
+
+M34 unit() {
+    M34 m = {{{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}}};
+    return m;
+}
+
+// composition A*G of 3x4 homogeneous transforms (merlict/HomTra3.cpp:245-268)
+M34 compose(const M34& A, const M34& G) {
+    M34 r;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            r.T[i][j] = A.T[i][0] * G.T[0][j] + A.T[i][1] * G.T[1][j] + A.T[i][2] * G.T[2][j];
+        r.T[i][3] = A.T[i][0] * G.T[0][3] + A.T[i][1] * G.T[1][3] + A.T[i][2] * G.T[2][3] + A.T[i][3];
+    }
+    return r;
+}
+
+// pose in mother: translation * rotation, the rotation from Tait-Bryan
+// angles or axis-angle (merlict/HomTra3.cpp:43-112, merlict/Rot3.cpp:21-33)
+M34 pose(const double* pos, int mode, const double* rot) {
+    M34 R = unit();
+    bool xyz = mode == MLGPU_ROT_XYZ;
+    double rx = rot[0], ry = rot[1], rz = rot[2];
+    if (!xyz && rot[3] == 0.0) { xyz = true; rx = ry = rz = 0.0; }
+    if (xyz) {
+        const double cx = std::cos(rx), cy = std::cos(ry), cz = std::cos(rz);
+        const double sx = std::sin(rx), sy = std::sin(ry), sz = std::sin(rz);
+        R.T[0][0] = cy * cz;
+        R.T[0][1] = cx * sz + sx * sy * cz;
+        R.T[0][2] = sx * sz - cx * sy * cz;
+        R.T[1][0] = -cy * sz;
+        R.T[1][1] = cx * cz - sx * sy * sz;
+        R.T[1][2] = sx * cz + cx * sy * sz;
+        R.T[2][0] = sy;
+        R.T[2][1] = -sx * cy;
+        R.T[2][2] = cx * cy;
+    } else {
+        double ax = rot[0], ay = rot[1], az = rot[2];
+        for (int k = 0; k < 2; k++) {
+            const double nn = std::sqrt(ax * ax + ay * ay + az * az);
+            ax /= nn; ay /= nn; az /= nn;
+        }
+        const double s = std::sin(rot[3]), c = std::cos(rot[3]);
+        R.T[0][0] = c + ax * ax * (1. - c);
+        R.T[0][1] = ax * ay * (1. - c) - az * s;
+        R.T[0][2] = ax * az * (1. - c) + ay * s;
+        R.T[1][0] = ay * ax * (1. - c) + az * s;
+        R.T[1][1] = c + ay * ay * (1. - c);
+        R.T[1][2] = ay * az * (1. - c) - ax * s;
+        R.T[2][0] = az * ax * (1. - c) - ay * s;
+        R.T[2][1] = az * ay * (1. - c) + ax * s;
+        R.T[2][2] = c + az * az * (1. - c);
+    }
+    M34 Tr = unit();
+    Tr.T[0][3] = pos[0]; Tr.T[1][3] = pos[1]; Tr.T[2][3] = pos[2];
+    return compose(Tr, R);
+}
+
+// Bounding radius each primitive reports about its own origin and the
+// kernel parameters of the leaf (primitive post_initialize_* methods).
+double leaf_params(int kind, const double* p, double* out) {
+    for (int i = 0; i < 8; i++) out[i] = 0.0;
+    switch (kind) {
+    case MLGPU_KIND_SPHERE_CAP_HEX: {
+        const double R = p[0], r = p[1], rm = r > R ? R : r;
+        const double radius = std::hypot(rm, R - std::sqrt(R * R - rm * rm));
+        out[0] = R; out[1] = r * std::cos(M_PI / 6.0); out[7] = radius;
+        return radius;
+    }
+    case MLGPU_KIND_SPHERE_CAP_CYL: {
+        const double R = p[0], r = p[1], rm = r > R ? R : r;
+        const double radius = std::hypot(rm, R - std::sqrt(R * R - rm * rm));
+        out[0] = R; out[1] = r * r; out[7] = radius;
+        return radius;
+    }
+    case MLGPU_KIND_SPHERE_CAP_RECT: {
+        const double hx = 0.5 * p[1], hy = 0.5 * p[2];
+        const double radius = std::hypot(hx, hy);
+        out[0] = p[0]; out[1] = hx; out[2] = hy; out[7] = radius;
+        return radius;
+    }
+    case MLGPU_KIND_PLANE: {
+        const double hx = 0.5 * p[0], hy = 0.5 * p[1];
+        out[1] = hx; out[2] = hy;
+        return std::hypot(hx, hy);
+    }
+    case MLGPU_KIND_DISC:
+        out[1] = p[0] * p[0];
+        return std::sqrt(out[1]);
+    case MLGPU_KIND_HEX_PLANE:
+        out[1] = p[0] * std::cos(M_PI / 6.0);
+        return p[0];
+    case MLGPU_KIND_ANNULUS:
+        out[1] = p[0] * p[0]; out[2] = p[1] * p[1];
+        return std::sqrt(out[1]);
+    case MLGPU_KIND_PLANE_DUAL_SPHERICAL: {
+        const double h = p[0], w = p[1];
+        const double dist = (w * w - h * h) / (2.0 * h);
+        const double cr = dist / 2.0 + h / 2.0;
+        out[1] = dist / 2.0; out[2] = cr * cr;
+        return w * 0.5;
+    }
+    case MLGPU_KIND_CYLINDER:
+        out[0] = p[0]; out[1] = p[1] / 2.0;
+        return std::hypot(0.5 * p[1], p[0]);
+    case MLGPU_KIND_TRIANGLE: {
+        for (int i = 0; i < 6; i++) out[i] = p[i];
+        double m = 0.0;
+        for (int i = 0; i < 3; i++) m = std::max(m, std::sqrt(p[2 * i] * p[2 * i] + p[2 * i + 1] * p[2 * i + 1]));
+        return m;
+    }
+    case MLGPU_KIND_SPHERE:
+        out[0] = p[0];
+        return p[0];
+    default:
+        return 0.0;
+    }
+}
+
+struct BNode {
+    double c[3];
+    double r;
+    int32_t leaf_frame = -1;       // description frame of a leaf
+    std::vector<int32_t> kids;
+};
+
+// smallest sphere enclosing two spheres
+void enclose(const BNode& a, const BNode& b, BNode* out) {
+    const double dx = b.c[0] - a.c[0], dy = b.c[1] - a.c[1], dz = b.c[2] - a.c[2];
+    const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (d + b.r <= a.r) { for (int k = 0; k < 3; k++) out->c[k] = a.c[k]; out->r = a.r; return; }
+    if (d + a.r <= b.r) { for (int k = 0; k < 3; k++) out->c[k] = b.c[k]; out->r = b.r; return; }
+    const double R = 0.5 * (d + a.r + b.r);
+    const double w = (R - a.r) / d;
+    out->c[0] = a.c[0] + dx * w;
+    out->c[1] = a.c[1] + dy * w;
+    out->c[2] = a.c[2] + dz * w;
+    out->r = R * (1.0 + 1e-12);
+}
+
+class Builder {
+ public:
+    Builder(const mlgpu_scene_desc& d, HostScene* hs) : d_(d), hs_(hs) {}
+
+    void run() {
+        const int32_t n = d_.n_frames;
+        t2w_.resize(n);
+        children_.assign(n, {});
+        for (int32_t i = 0; i < n; i++) {
+            const M34 t2m = pose(d_.pos + 3 * i, d_.rot_mode[i], d_.rot + 4 * i);
+            t2w_[i] = d_.mother[i] >= 0 ? compose(t2w_[d_.mother[i]], t2m) : t2m;
+            if (d_.mother[i] >= 0) children_[d_.mother[i]].push_back(i);
+        }
+        scope_.assign(n, 0);
+        scope_[0] = 1;
+        for (int32_t i = 0; i < n; i++)
+            if (d_.allowed_frame && d_.allowed_frame[i] >= 0) scope_[d_.allowed_frame[i]] = 1;
+        build_functions();
+        bnodes_.reserve(2 * n);
+        const int32_t root = build_scope(0);
+        hs_->scope_node.assign(n, -1);
+        emit(root);
+        // scope frames -> emitted node index
+        for (int32_t f = 0; f < n; f++)
+            if (scope_bnode_.count(f)) hs_->scope_node[f] = emitted_[scope_bnode_[f]];
+        hs_->root_node = hs_->scope_node[0];
+        // leaf physics needs the scope nodes
+        for (size_t l = 0; l < hs_->phys.size(); l++) {
+            LeafPhys& ph = hs_->phys[l];
+            const int32_t fr = ph.frame;
+            const int32_t a = d_.allowed_frame ? d_.allowed_frame[fr] : -1;
+            ph.allowed_node = a >= 0 ? hs_->scope_node[a] : -1;
+            ph.root_node = hs_->root_node;
+        }
+        inflate();
+    }
+
+ private:
+    const mlgpu_scene_desc& d_;
+    HostScene* hs_;
+    std::vector<M34> t2w_;
+    std::vector<std::vector<int32_t>> children_;
+    std::vector<char> scope_;
+    std::vector<BNode> bnodes_;
+    std::unordered_map<int32_t, int32_t> scope_bnode_;
+    std::unordered_map<int32_t, int32_t> emitted_;
+
+    int32_t make_leaf(int32_t f) {
+        BNode b;
+        double prm[8];
+        b.r = leaf_params(d_.kind[f], d_.params + MLGPU_NUM_PARAMS * f, prm);
+        b.c[0] = t2w_[f].T[0][3]; b.c[1] = t2w_[f].T[1][3]; b.c[2] = t2w_[f].T[2][3];
+        b.leaf_frame = f;
+        bnodes_.push_back(b);
+        return static_cast<int32_t>(bnodes_.size()) - 1;
+    }
+
+    void collect(int32_t f, std::vector<int32_t>* items) {
+        for (int32_t c : children_[f]) {
+            if (scope_[c]) items->push_back(build_scope(c));
+            else if (children_[c].empty()) items->push_back(make_leaf(c));
+            else collect(c, items);
+        }
+    }
+
+    int32_t build_scope(int32_t f) {
+        std::vector<int32_t> items;
+        if (children_[f].empty()) {
+            items.push_back(make_leaf(f));
+        } else {
+            collect(f, &items);
+        }
+        int32_t root;
+        if (items.empty()) {
+            BNode b;
+            b.c[0] = t2w_[f].T[0][3]; b.c[1] = t2w_[f].T[1][3]; b.c[2] = t2w_[f].T[2][3];
+            b.r = 0.0;
+            bnodes_.push_back(b);
+            root = static_cast<int32_t>(bnodes_.size()) - 1;
+        } else {
+            root = bvh(items, 0, items.size());
+        }
+        scope_bnode_[f] = root;
+        return root;
+    }
+
+    // binary tree over items[lo, hi): median split along the widest axis
+    int32_t bvh(std::vector<int32_t>& items, size_t lo, size_t hi) {
+        if (hi - lo == 1) return items[lo];
+        double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+        for (size_t i = lo; i < hi; i++)
+            for (int k = 0; k < 3; k++) {
+                mn[k] = std::min(mn[k], bnodes_[items[i]].c[k]);
+                mx[k] = std::max(mx[k], bnodes_[items[i]].c[k]);
+            }
+        int axis = 0;
+        for (int k = 1; k < 3; k++)
+            if (mx[k] - mn[k] > mx[axis] - mn[axis]) axis = k;
+        const size_t mid = lo + (hi - lo) / 2;
+        std::nth_element(items.begin() + lo, items.begin() + mid, items.begin() + hi,
+                         [&](int32_t a, int32_t b) {
+                             const double ca = bnodes_[a].c[axis], cb = bnodes_[b].c[axis];
+                             return ca < cb || (ca == cb && a < b);
+                         });
+        const int32_t left = bvh(items, lo, mid);
+        const int32_t right = bvh(items, mid, hi);
+        BNode b;
+        enclose(bnodes_[left], bnodes_[right], &b);
+        b.kids = {left, right};
+        bnodes_.push_back(b);
+        return static_cast<int32_t>(bnodes_.size()) - 1;
+    }
+
+    // depth-first emission with skip indices
+    void emit(int32_t bi) {
+        const int32_t idx = static_cast<int32_t>(hs_->nodes.size());
+        emitted_[bi] = idx;
+        Node nd;
+        const BNode& b = bnodes_[bi];
+        nd.cx = static_cast<float>(b.c[0]);
+        nd.cy = static_cast<float>(b.c[1]);
+        nd.cz = static_cast<float>(b.c[2]);
+        nd.r = static_cast<float>(b.r);
+        nd.leaf = -1;
+        nd.pad0 = nd.pad1 = 0;
+        hs_->nodes.push_back(nd);
+        hs_->node_radius.push_back(b.r);
+        hs_->node_center.push_back({b.c[0], b.c[1], b.c[2]});
+        if (b.leaf_frame >= 0) {
+            hs_->nodes[idx].leaf = add_leaf(b.leaf_frame);
+        } else {
+            const std::vector<int32_t> kids = b.kids;
+            for (int32_t k : kids) emit(k);
+        }
+        hs_->nodes[idx].skip = static_cast<int32_t>(hs_->nodes.size());
+    }
+
+    int32_t add_leaf(int32_t f) {
+        LeafGeo g;
+        const M34& m = t2w_[f];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) g.rot[3 * i + j] = m.T[i][j];
+        g.tx = m.T[0][3]; g.ty = m.T[1][3]; g.tz = m.T[2][3];
+        leaf_params(d_.kind[f], d_.params + MLGPU_NUM_PARAMS * f, g.p);
+        g.kind = d_.kind[f];
+        g.pad0 = g.pad1 = g.pad2 = 0;
+        LeafPhys ph;
+        for (int s = 0; s < MLGPU_NUM_SURFACE_SLOTS; s++) {
+            int32_t fn = d_.surface[MLGPU_NUM_SURFACE_SLOTS * f + s];
+            if (fn < 0) {
+                if (s <= MLGPU_INNER_REFLECTION) fn = def_refl_;
+                else if (s <= MLGPU_INNER_REFRACTION) fn = def_refr_;
+                else fn = def_abs_;
+            }
+            ph.surf[s] = fn;
+        }
+        ph.allowed_node = -1;
+        ph.root_node = -1;
+        ph.frame = f;
+        ph.sensor = d_.sensor_id ? d_.sensor_id[f] : -1;
+        ph.pad0 = ph.pad1 = 0;
+        hs_->geo.push_back(g);
+        hs_->phys.push_back(ph);
+        hs_->leaf_frame.push_back(f);
+        return static_cast<int32_t>(hs_->geo.size()) - 1;
+    }
+
+    int32_t def_refl_ = -1, def_refr_ = -1, def_abs_ = -1;
+
+    void add_function(const double* x, const double* y, int n, int flags) {
+        struct P { double x, y, slope; };
+        std::vector<P> pts(n);
+        for (int i = 0; i < n; i++) {
+            pts[i].x = x[i];
+            pts[i].y = y[i];
+            pts[i].slope = (i != n - 1) ? (y[i] - y[i + 1]) / (x[i] - x[i + 1]) : 0.0;
+        }
+        std::sort(pts.begin(), pts.end(), [](const P& a, const P& b) { return a.x < b.x; });
+        for (int i = 0; i + 1 < n; i++)
+            if (pts[i].x == pts[i + 1].x) throw std::invalid_argument("function has a duplicate argument");
+        FuncHead h;
+        h.lower = pts.front().x;
+        h.upper = pts.back().x;
+        h.start = static_cast<int32_t>(hs_->points.size());
+        h.n = n;
+        h.flags = flags;
+        h.pad = 0;
+        for (const P& p : pts) {
+            FuncPoint fp;
+            fp.x = p.x; fp.y = p.y; fp.slope = p.slope; fp.pad = 0.0;
+            hs_->points.push_back(fp);
+        }
+        hs_->funcs.push_back(h);
+    }
+
+    void build_functions() {
+        for (int32_t f = 0; f < d_.n_functions; f++) {
+            const int32_t a = d_.func_start[f], b = d_.func_start[f + 1];
+            add_function(d_.func_x + a, d_.func_y + a, b - a, 0);
+        }
+        // merlict's defaults (merlict/SurfaceEntity.cpp:12-33)
+        const double x[2] = {200e-9, 1200e-9};
+        const double r0[2] = {0.0, 0.0}, r1[2] = {1.0, 1.0};
+        const double inf[2] = {INFINITY, INFINITY};
+        def_refl_ = d_.n_functions;
+        add_function(x, r0, 2, FUNC_DEFAULT_REFLECTION);
+        def_refr_ = d_.n_functions + 1;
+        add_function(x, r1, 2, FUNC_DEFAULT_REFRACTION);
+        def_abs_ = d_.n_functions + 2;
+        // Func1 slope of an infinite table is NaN, so merlict's default
+        // absorption evaluates to NaN and never absorbs; flagged explicitly.
+        add_function(x, inf, 2, FUNC_DEFAULT_ABSORPTION);
+        hs_->default_refraction = def_refr_;
+    }
+
+    // Inflate float spheres so the FP32 test stays conservative: centre
+    // rounding plus the test's own rounding, bounded by a few ulp of the
+    // scene's coordinate magnitude.
+    void inflate() {
+        double extent = 0.0;
+        for (size_t i = 0; i < hs_->nodes.size(); i++) {
+            const auto& c = hs_->node_center[i];
+            extent = std::max(extent, std::fabs(c[0]) + hs_->node_radius[i]);
+            extent = std::max(extent, std::fabs(c[1]) + hs_->node_radius[i]);
+            extent = std::max(extent, std::fabs(c[2]) + hs_->node_radius[i]);
+        }
+        const double eps = 2e-6 * std::max(extent, 1.0);
+        hs_->eps = eps;
+        for (size_t i = 0; i < hs_->nodes.size(); i++) {
+            const double r = hs_->node_radius[i] * (1.0 + 1e-6) + eps;
+            float rf = static_cast<float>(r);
+            if (static_cast<double>(rf) < r) rf = std::nextafter(rf, INFINITY);
+            hs_->nodes[i].r = rf;
+        }
+    }
+};
+
+}  // namespace
+
+void build_host_scene(const mlgpu_scene_desc& d, HostScene* hs) {
+    validate_desc(d);
+    Builder b(d, hs);
+    b.run();
+}
+
+}  // namespace mlgpu

@@ -1,0 +1,32 @@
+// Host representation of a flattened scene before upload.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <unordered_map>
+#include <vector>
+
+#include "device_scene.hpp"
+#include "scene.hpp"
+
+namespace mlgpu {
+
+struct HostScene {
+    std::vector<Node> nodes;
+    std::vector<double> node_radius;                 // exact (double) radii
+    std::vector<std::array<double, 3>> node_center;  // exact (double) centres
+    std::vector<LeafGeo> geo;
+    std::vector<LeafPhys> phys;
+    std::vector<int32_t> leaf_frame;
+    std::vector<FuncHead> funcs;
+    std::vector<FuncPoint> points;
+    std::vector<int32_t> scope_node;  // per description frame, -1 if not a scope
+    int32_t root_node = -1;
+    int32_t default_refraction = -1;
+    double eps = 0.0;
+};
+
+// Flatten a description (throws std::invalid_argument on bad input).
+void build_host_scene(const mlgpu_scene_desc& d, HostScene* hs);
+
+}  // namespace mlgpu

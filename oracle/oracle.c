@@ -1,0 +1,1067 @@
+/*
+ * oracle.c -- FP64 CPU restatement of merlict's photon propagation.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Every function names the
+ * reference file:line it restates.  Arithmetic is written in the reference's
+ * operation order and compiled without FMA contraction, so on the same libm
+ * it reproduces the reference's doubles.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+static __thread char g_err[512];
+static void set_err(const char* msg) { snprintf(g_err, sizeof(g_err), "%s", msg); }
+const char* oracle_last_error(void) { return g_err; }
+
+/* ------------------------------------------------------------ Vec3 math */
+typedef struct { double x, y, z; } v3;
+
+static v3 V(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+/* Vec3::operator*(v) = v.x*x + v.y*y + v.z*z  (merlict/Vec3.cpp:31-33) */
+static double dot(v3 a, v3 b) { return b.x * a.x + b.y * a.y + b.z * a.z; }
+static v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 mul(v3 a, double s) { return V(a.x * s, a.y * s, a.z * s); }
+static v3 dvd(v3 a, double s) { return V(a.x / s, a.y / s, a.z / s); }
+static double norm(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+static v3 normalized(v3 a) { return dvd(a, norm(a)); }
+/* Vec3 != : squared distance > 1e-14 (merlict/Vec3.cpp:121-127) */
+static int vec_neq(v3 a, v3 b) { v3 d = sub(a, b); return dot(d, d) > 1e-14; }
+/* Vec3::norm_is_less_equal_than (Vec3.cpp:146-149) */
+static int norm_le(v3 a, double r) { return dot(a, a) <= r * r; }
+/* Vec3::octant (Vec3.cpp:151-166) */
+static int octant(v3 a) { return 4 * (a.x >= 0.) + 2 * (a.y >= 0.) + 1 * (a.z >= 0.); }
+
+/* ---------------------------------------------------------------- HomTra3 */
+typedef struct { double T[3][4]; } homtra;
+
+static homtra ht_unit(void) {
+    homtra h = {{{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}}};
+    return h;
+}
+
+/* HomTra3::operator* (merlict/HomTra3.cpp:245-268) */
+static homtra ht_mul(const homtra* A, const homtra* G) {
+    homtra r;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            r.T[i][j] = A->T[i][0] * G->T[0][j] + A->T[i][1] * G->T[1][j] + A->T[i][2] * G->T[2][j];
+        r.T[i][3] = A->T[i][0] * G->T[0][3] + A->T[i][1] * G->T[1][3] + A->T[i][2] * G->T[2][3] + A->T[i][3];
+    }
+    return r;
+}
+
+/* Rot3 semantics (merlict/Rot3.cpp:10-33) + HomTra3 rotation components
+ * (HomTra3.cpp:67-112) + set_transformation = Translation*Rotation (:43-54) */
+static homtra ht_from_pos_rot(v3 pos, int rot_mode, const double* rot) {
+    homtra R = ht_unit();
+    int xyz = (rot_mode == MLGPU_ROT_XYZ);
+    double rx = rot[0], ry = rot[1], rz = rot[2];
+    v3 axis = V(rot[0], rot[1], rot[2]);
+    double angle = rot[3];
+    if (!xyz && angle == 0.0) { xyz = 1; rx = 0.0; ry = 0.0; rz = 0.0; }
+    if (xyz) {
+        const double cosRx = cos(rx), cosRy = cos(ry), cosRz = cos(rz);
+        const double sinRx = sin(rx), sinRy = sin(ry), sinRz = sin(rz);
+        R.T[0][0] = cosRy * cosRz;
+        R.T[0][1] = cosRx * sinRz + sinRx * sinRy * cosRz;
+        R.T[0][2] = sinRx * sinRz - cosRx * sinRy * cosRz;
+        R.T[1][0] = -cosRy * sinRz;
+        R.T[1][1] = cosRx * cosRz - sinRx * sinRy * sinRz;
+        R.T[1][2] = sinRx * cosRz + cosRx * sinRy * sinRz;
+        R.T[2][0] = sinRy;
+        R.T[2][1] = -sinRx * cosRy;
+        R.T[2][2] = cosRx * cosRy;
+    } else {
+        axis = dvd(axis, norm(axis));  /* Rot3::set normalises          */
+        axis = dvd(axis, norm(axis));  /* set_rotation_..._rot_axis again */
+        const double x = axis.x, y = axis.y, z = axis.z;
+        const double sinR = sin(angle), cosR = cos(angle);
+        R.T[0][0] = cosR + x * x * (1. - cosR);
+        R.T[0][1] = x * y * (1. - cosR) - z * sinR;
+        R.T[0][2] = x * z * (1. - cosR) + y * sinR;
+        R.T[1][0] = y * x * (1. - cosR) + z * sinR;
+        R.T[1][1] = cosR + y * y * (1. - cosR);
+        R.T[1][2] = y * z * (1. - cosR) - x * sinR;
+        R.T[2][0] = z * x * (1. - cosR) - y * sinR;
+        R.T[2][1] = z * y * (1. - cosR) + x * sinR;
+        R.T[2][2] = cosR + z * z * (1. - cosR);
+    }
+    homtra Tr = ht_unit();
+    Tr.T[0][3] = pos.x; Tr.T[1][3] = pos.y; Tr.T[2][3] = pos.z;
+    return ht_mul(&Tr, &R);
+}
+
+/* HomTra3::position / orientation / *_inverse (HomTra3.cpp:154-216) */
+static v3 ht_orient(const homtra* h, v3 o) {
+    const double (*T)[4] = h->T;
+    return V(o.x * T[0][0] + o.y * T[0][1] + o.z * T[0][2],
+             o.x * T[1][0] + o.y * T[1][1] + o.z * T[1][2],
+             o.x * T[2][0] + o.y * T[2][1] + o.z * T[2][2]);
+}
+static v3 ht_orient_inv(const homtra* h, v3 o) {
+    const double (*T)[4] = h->T;
+    return V(o.x * T[0][0] + o.y * T[1][0] + o.z * T[2][0],
+             o.x * T[0][1] + o.y * T[1][1] + o.z * T[2][1],
+             o.x * T[0][2] + o.y * T[1][2] + o.z * T[2][2]);
+}
+static v3 ht_pos(const homtra* h, v3 p) {
+    const double (*T)[4] = h->T;
+    return V(p.x * T[0][0] + p.y * T[0][1] + p.z * T[0][2] + T[0][3],
+             p.x * T[1][0] + p.y * T[1][1] + p.z * T[1][2] + T[1][3],
+             p.x * T[2][0] + p.y * T[2][1] + p.z * T[2][2] + T[2][3]);
+}
+static v3 ht_pos_inv(const homtra* h, v3 p) {
+    const double (*T)[4] = h->T;
+    return V(p.x * T[0][0] + p.y * T[1][0] + p.z * T[2][0] - (T[0][0] * T[0][3] + T[1][0] * T[1][3] + T[2][0] * T[2][3]),
+             p.x * T[0][1] + p.y * T[1][1] + p.z * T[2][1] - (T[0][1] * T[0][3] + T[1][1] * T[1][3] + T[2][1] * T[2][3]),
+             p.x * T[0][2] + p.y * T[1][2] + p.z * T[2][2] - (T[0][2] * T[0][3] + T[1][2] * T[1][3] + T[2][2] * T[2][3]));
+}
+
+/* ------------------------------------------------------------------ Func1 */
+typedef struct { double x, y, slope; } fpoint;
+typedef struct { fpoint* p; int n; double lower, upper; } func1;
+
+static int cmp_fpoint(const void* a, const void* b) {
+    double xa = ((const fpoint*)a)->x, xb = ((const fpoint*)b)->x;
+    return (xa < xb) ? -1 : (xa > xb) ? 1 : 0;
+}
+
+/* Func1::init: slopes from table order, then sort (Func1.cpp:22-50) */
+static int func1_init(func1* f, const double* x, const double* y, int n) {
+    if (n < 1) { set_err("Func1: empty table"); return 1; }
+    f->p = (fpoint*)malloc(sizeof(fpoint) * n);
+    f->n = n;
+    for (int i = 0; i < n; i++) {
+        f->p[i].x = x[i];
+        f->p[i].y = y[i];
+        f->p[i].slope = (i != n - 1) ? (y[i] - y[i + 1]) / (x[i] - x[i + 1]) : 0.0;
+    }
+    qsort(f->p, n, sizeof(fpoint), cmp_fpoint);
+    for (int i = 0; i + 1 < n; i++)
+        if (f->p[i].x == f->p[i + 1].x) { set_err("Func1: duplicate argument"); return 1; }
+    f->lower = f->p[0].x;
+    f->upper = f->p[n - 1].x;
+    return 0;
+}
+
+/* Func1::evaluate (Func1.cpp:66-71,118-130), Limits::assert_contains */
+static int func1_eval(const func1* f, double x, double* out) {
+    if (x < f->lower || x >= f->upper) return 1;
+    /* std::upper_bound: first point with P.x > x */
+    int lo = 0, hi = f->n;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (f->p[mid].x > x) hi = mid; else lo = mid + 1;
+    }
+    if (lo == 0 || (lo == f->n && x > f->p[f->n - 1].x)) return 1;
+    const fpoint p0 = f->p[lo - 1];
+    *out = p0.slope * x + p0.y - p0.slope * p0.x;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ scene */
+typedef struct {
+    int mother;
+    int* children;
+    int n_children, cap_children;
+    v3 pos;
+    int rot_mode;
+    double rot[4];
+    homtra t2m, t2w;
+    double radius;
+    int kind;
+    double p[MLGPU_NUM_PARAMS];
+    int surf[MLGPU_NUM_SURFACE_SLOTS];  /* resolved function index */
+    int allowed;
+    int sensor;
+    int root;
+    /* primitive constants */
+    double hex_inner;       /* HexagonalPrismZ inner radius          */
+    double cyl_r2;          /* CylinderPrismZ radius_square          */
+    double cyl_r2_inner;
+    double half_x, half_y;  /* RectangularPrismZ                     */
+    double ds_dist, ds_r;   /* DualSphericalPrismZ                   */
+} oframe;
+
+struct oracle_scene {
+    oframe* f;
+    int n, cap;
+    int n_desc;
+    func1* funcs;
+    int n_funcs;        /* desc functions; defaults follow */
+    int def_refl, def_refr, def_abs;
+};
+
+static const double UNIT_V_X_INIT = 0.0; /* placeholder for static init */
+static v3 UNIT_U, UNIT_V, UNIT_W;
+static int g_units_ready = 0;
+static void init_hex_units(void) {
+    if (g_units_ready) return;
+    /* HexagonalPrismZ.h:10-16 */
+    const double s = sin(2. / 3. * M_PI), c = cos(2. / 3. * M_PI);
+    UNIT_U = V(0., 1., 0.);
+    UNIT_V = add(mul(V(1., 0., 0.), +s), mul(V(0., 1., 0.), c));
+    UNIT_W = add(mul(V(1., 0., 0.), -s), mul(V(0., 1., 0.), c));
+    (void)UNIT_V_X_INIT;
+    g_units_ready = 1;
+}
+
+static int push_child(oframe* m, int c) {
+    if (m->n_children == m->cap_children) {
+        m->cap_children = m->cap_children ? 2 * m->cap_children : 4;
+        m->children = (int*)realloc(m->children, sizeof(int) * m->cap_children);
+    }
+    m->children[m->n_children++] = c;
+    return 0;
+}
+
+static int new_frame(oracle_scene* s) {
+    if (s->n == s->cap) {
+        s->cap = s->cap ? 2 * s->cap : 64;
+        s->f = (oframe*)realloc(s->f, sizeof(oframe) * s->cap);
+    }
+    memset(&s->f[s->n], 0, sizeof(oframe));
+    s->f[s->n].allowed = -1;
+    s->f[s->n].sensor = -1;
+    s->f[s->n].mother = -1;
+    return s->n++;
+}
+
+/* bounding radius a primitive sets on construction
+ * (post_initialize_radius_of_enclosing_sphere of each primitive). */
+static void primitive_init(oframe* f) {
+    const double* p = f->p;
+    switch (f->kind) {
+    case MLGPU_KIND_SPHERE_CAP_HEX: {     /* SphereCapWithHexagonalBound.cpp:8-43 */
+        double R = p[0], r = p[1];
+        double r_member = r > R ? R : r;
+        f->hex_inner = r * cos(M_PI / 6.0);  /* hexBounds gets the parameter */
+        double z = sqrt(R * R - r_member * r_member);
+        f->radius = hypot(r_member, R - z);
+        break;
+    }
+    case MLGPU_KIND_SPHERE_CAP_CYL: {     /* SphereCapWithCylinderBound.cpp:8-38 */
+        double R = p[0], r = p[1];
+        double r_member = r > R ? R : r;
+        f->cyl_r2 = r * r;
+        double z = sqrt(R * R - r_member * r_member);
+        f->radius = hypot(r_member, R - z);
+        break;
+    }
+    case MLGPU_KIND_SPHERE_CAP_RECT:      /* SphereCapWithRectangularBound.cpp:8-30 */
+        f->half_x = 0.5 * p[1];
+        f->half_y = 0.5 * p[2];
+        f->radius = hypot(f->half_x, f->half_y);
+        break;
+    case MLGPU_KIND_PLANE:                /* Plane.cpp:13-23 */
+        f->half_x = 0.5 * p[0];
+        f->half_y = 0.5 * p[1];
+        f->radius = hypot(f->half_x, f->half_y);
+        break;
+    case MLGPU_KIND_DISC:                 /* Disc.cpp:8-15, CylinderPrismZ.cpp */
+        f->cyl_r2 = p[0] * p[0];
+        f->radius = sqrt(f->cyl_r2);
+        break;
+    case MLGPU_KIND_HEX_PLANE:            /* HexPlane.cpp:9-16 */
+        f->hex_inner = p[0] * cos(M_PI / 6.0);
+        f->radius = p[0];
+        break;
+    case MLGPU_KIND_ANNULUS:              /* Annulus.cpp:8-19 */
+        f->cyl_r2 = p[0] * p[0];
+        f->cyl_r2_inner = p[1] * p[1];
+        f->radius = sqrt(f->cyl_r2);
+        break;
+    case MLGPU_KIND_PLANE_DUAL_SPHERICAL: { /* DualSphericalPrismZ.cpp:7-19 */
+        double h = p[0], w = p[1];
+        f->ds_dist = (w * w - h * h) / (2.0 * h);
+        f->ds_r = f->ds_dist / 2.0 + h / 2.0;
+        f->radius = w * 0.5;
+        break;
+    }
+    case MLGPU_KIND_CYLINDER:             /* Cylinder.cpp:104-107 */
+        f->radius = hypot(0.5 * p[1], p[0]);
+        break;
+    case MLGPU_KIND_TRIANGLE: {           /* Triangle.cpp:72-80 */
+        double a = sqrt(p[0] * p[0] + p[1] * p[1]);
+        double b = sqrt(p[2] * p[2] + p[3] * p[3]);
+        double c = sqrt(p[4] * p[4] + p[5] * p[5]);
+        double m = a;
+        if (b > m) m = b;
+        if (c > m) m = c;
+        f->radius = m;
+        break;
+    }
+    case MLGPU_KIND_SPHERE:               /* Sphere.cpp:13-21 */
+        f->radius = p[0];
+        break;
+    default:
+        f->radius = 0.0;
+    }
+}
+
+/* bound::position_mean_in_mother (Frames.cpp:92-108) */
+static v3 position_mean(const oracle_scene* s, const int* idx, int n) {
+    v3 sum = V(0., 0., 0.);
+    for (int i = 0; i < n; i++) sum = add(sum, s->f[idx[i]].pos);
+    return dvd(sum, (double)n);
+}
+
+/* bound::bounding_sphere_radius (Frames.cpp:137-206) */
+static double bounding_radius(const oracle_scene* s, const int* idx, int n, v3 center) {
+    double radius = 0.0;
+    for (int i = 0; i < n; i++) {
+        const oframe* c = &s->f[idx[i]];
+        double need = norm(sub(center, c->pos)) + c->radius;
+        if (need > radius) radius = need;
+    }
+    return radius;
+}
+
+/* bound::bounding_sphere_center_alternative (Frames.cpp:57-90) */
+static v3 center_alternative(const oracle_scene* s, const int* idx, int n) {
+    if (n == 1) return s->f[idx[0]].pos;
+    int start = -1, end = -1;
+    double maximum = 0.0;
+    for (int i = 0; i < n; i++) {
+        for (int j = i; j < n; j++) {
+            if (i != j) {
+                const oframe* a = &s->f[idx[i]];
+                const oframe* b = &s->f[idx[j]];
+                double dist = norm(sub(a->pos, b->pos)) + a->radius + b->radius;
+                if (dist > maximum) { maximum = dist; start = idx[i]; end = idx[j]; }
+            }
+        }
+    }
+    /* both VOID_FRAME (pos 0, radius 0) when all distances are 0 */
+    v3 sp = start >= 0 ? s->f[start].pos : V(0., 0., 0.);
+    v3 ep = end >= 0 ? s->f[end].pos : V(0., 0., 0.);
+    double sr = start >= 0 ? s->f[start].radius : 0.0;
+    v3 dir = sub(ep, sp);
+    dir = dvd(dir, norm(dir));
+    return add(sp, mul(dir, 0.5 * maximum - sr));
+}
+
+/* bound::bounding_sphere_center (Frames.cpp:39-55) */
+static v3 bounding_center(const oracle_scene* s, const int* idx, int n) {
+    v3 center = position_mean(s, idx, n);
+    if (n < 16) {
+        v3 alt = center_alternative(s, idx, n);
+        double m_r = bounding_radius(s, idx, n, center);
+        double a_r = bounding_radius(s, idx, n, alt);
+        if (m_r > a_r) center = alt;
+    }
+    return center;
+}
+
+/* bound::position_spread_in_mother (Frames.cpp:110-124) */
+static int too_close(const oracle_scene* s, const int* idx, int n) {
+    if (n < 2) return 0;
+    v3 mean = position_mean(s, idx, n);
+    v3 u = V(0., 0., 0.);
+    for (int i = 0; i < n; i++) {
+        v3 r = sub(s->f[idx[i]].pos, mean);
+        u = add(u, V(r.x * r.x, r.y * r.y, r.z * r.z));
+    }
+    u = dvd(u, (double)n);
+    return sqrt(norm(u)) < 1e-6;  /* FRAME_MIN_STRUCTURE_SIZE */
+}
+
+/* Frame::cluster_children (Frame.cpp:190-255) */
+static void cluster_children(oracle_scene* s, int fi) {
+    if (s->f[fi].n_children > 16) {
+        int n = s->f[fi].n_children;
+        int* old = (int*)malloc(sizeof(int) * n);
+        memcpy(old, s->f[fi].children, sizeof(int) * n);
+        int* oct[8];
+        int noct[8] = {0};
+        for (int k = 0; k < 8; k++) oct[k] = (int*)malloc(sizeof(int) * n);
+        for (int i = 0; i < n; i++) {
+            int c = old[i];
+            int k = octant(s->f[c].pos);
+            oct[k][noct[k]++] = c;
+        }
+        s->f[fi].n_children = 0;
+        for (int k = 0; k < 8; k++) {
+            if (too_close(s, oct[k], noct[k])) {
+                for (int i = 0; i < noct[k]; i++) {
+                    s->f[oct[k][i]].mother = fi;
+                    push_child(&s->f[fi], oct[k][i]);
+                }
+            } else if (noct[k] == 1) {
+                s->f[oct[k][0]].mother = fi;
+                push_child(&s->f[fi], oct[k][0]);
+            } else if (noct[k] > 1) {
+                v3 center = bounding_center(s, oct[k], noct[k]);
+                int o = new_frame(s);   /* may move s->f */
+                oframe* of = &s->f[o];
+                of->mother = fi;
+                of->kind = MLGPU_KIND_FRAME;
+                of->pos = center;
+                of->rot_mode = MLGPU_ROT_XYZ;
+                of->t2m = ht_from_pos_rot(center, MLGPU_ROT_XYZ, of->rot);
+                for (int q = 0; q < MLGPU_NUM_SURFACE_SLOTS; q++) of->surf[q] = -1;
+                push_child(&s->f[fi], o);
+                for (int i = 0; i < noct[k]; i++) {
+                    oframe* c = &s->f[oct[k][i]];
+                    c->pos = sub(c->pos, center);
+                    c->t2m = ht_from_pos_rot(c->pos, c->rot_mode, c->rot);
+                    c->mother = o;
+                    push_child(&s->f[o], oct[k][i]);
+                }
+            }
+        }
+        for (int k = 0; k < 8; k++) free(oct[k]);
+        free(old);
+    }
+    int nc = s->f[fi].n_children;
+    for (int i = 0; i < nc; i++) cluster_children(s, s->f[fi].children[i]);
+}
+
+static void init_root_and_t2w(oracle_scene* s, int fi, int root) {
+    oframe* f = &s->f[fi];
+    f->root = root;
+    if (f->mother >= 0) f->t2w = ht_mul(&s->f[f->mother].t2w, &f->t2m);
+    else f->t2w = f->t2m;
+    for (int i = 0; i < f->n_children; i++) init_root_and_t2w(s, f->children[i], root);
+}
+
+/* Frame::update_bounding_sphere (Frame.cpp:141-147) */
+static void update_radius(oracle_scene* s, int fi) {
+    oframe* f = &s->f[fi];
+    for (int i = 0; i < f->n_children; i++) update_radius(s, f->children[i]);
+    f = &s->f[fi];
+    if (f->n_children > 0)
+        f->radius = bounding_radius(s, f->children, f->n_children, V(0., 0., 0.));
+}
+
+static int resolve_func(const oracle_scene* s, int slot, int id) {
+    if (id >= 0) return id;
+    if (slot == MLGPU_OUTER_REFLECTION || slot == MLGPU_INNER_REFLECTION) return s->def_refl;
+    if (slot == MLGPU_OUTER_REFRACTION || slot == MLGPU_INNER_REFRACTION) return s->def_refr;
+    return s->def_abs;
+}
+
+oracle_scene* oracle_scene_create(const mlgpu_scene_desc* d) {
+    init_hex_units();
+    if (!d || d->n_frames < 1) { set_err("empty scene"); return NULL; }
+    oracle_scene* s = (oracle_scene*)calloc(1, sizeof(oracle_scene));
+    s->n_desc = d->n_frames;
+    s->n_funcs = d->n_functions;
+    s->funcs = (func1*)calloc(d->n_functions + 3, sizeof(func1));
+    for (int i = 0; i < d->n_functions; i++) {
+        int a = d->func_start[i], b = d->func_start[i + 1];
+        if (func1_init(&s->funcs[i], d->func_x + a, d->func_y + a, b - a)) {
+            oracle_scene_destroy(s);
+            return NULL;
+        }
+    }
+    /* SurfaceEntity.cpp:12-33 */
+    const double dx[2] = {200e-9, 1200e-9};
+    const double refl[2] = {0, 0}, refr[2] = {1, 1}, absr[2] = {INFINITY, INFINITY};
+    s->def_refl = d->n_functions;
+    s->def_refr = d->n_functions + 1;
+    s->def_abs = d->n_functions + 2;
+    func1_init(&s->funcs[s->def_refl], dx, refl, 2);
+    func1_init(&s->funcs[s->def_refr], dx, refr, 2);
+    func1_init(&s->funcs[s->def_abs], dx, absr, 2);
+
+    for (int i = 0; i < d->n_frames; i++) {
+        int fi = new_frame(s);
+        oframe* f = &s->f[fi];
+        f->mother = d->mother[i];
+        if (i == 0 && f->mother != -1) { set_err("frame 0 must be the root"); oracle_scene_destroy(s); return NULL; }
+        if (i > 0 && (f->mother < 0 || f->mother >= i)) { set_err("mother must precede child"); oracle_scene_destroy(s); return NULL; }
+        f->pos = V(d->pos[3 * i], d->pos[3 * i + 1], d->pos[3 * i + 2]);
+        f->rot_mode = d->rot_mode[i];
+        memcpy(f->rot, d->rot + 4 * i, sizeof(double) * 4);
+        f->t2m = ht_from_pos_rot(f->pos, f->rot_mode, f->rot);
+        f->kind = d->kind[i];
+        memcpy(f->p, d->params + MLGPU_NUM_PARAMS * i, sizeof(double) * MLGPU_NUM_PARAMS);
+        for (int q = 0; q < MLGPU_NUM_SURFACE_SLOTS; q++) {
+            int id = d->surface[MLGPU_NUM_SURFACE_SLOTS * i + q];
+            if (id >= d->n_functions) { set_err("function index out of range"); oracle_scene_destroy(s); return NULL; }
+            f->surf[q] = resolve_func(s, q, id);
+        }
+        f->allowed = d->allowed_frame ? d->allowed_frame[i] : -1;
+        f->sensor = d->sensor_id ? d->sensor_id[i] : -1;
+        primitive_init(f);
+        if (f->mother >= 0) push_child(&s->f[f->mother], fi);
+    }
+    /* Frame::init_tree_based_on_mother_child_relations (Frame.cpp:125-130) */
+    cluster_children(s, 0);
+    init_root_and_t2w(s, 0, 0);
+    update_radius(s, 0);
+    return s;
+}
+
+void oracle_scene_destroy(oracle_scene* s) {
+    if (!s) return;
+    for (int i = 0; i < s->n; i++) free(s->f[i].children);
+    free(s->f);
+    if (s->funcs)
+        for (int i = 0; i < s->n_funcs + 3; i++) free(s->funcs[i].p);
+    free(s->funcs);
+    free(s);
+}
+
+int oracle_num_frames(const oracle_scene* s) { return s->n; }
+
+static void dump_rec(const oracle_scene* s, int fi, int32_t* frame, int32_t* nch,
+                     double* t2w, double* radius, int cap, int* k) {
+    if (*k < cap) {
+        frame[*k] = fi;
+        nch[*k] = s->f[fi].n_children;
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 4; c++) t2w[12 * (*k) + 4 * r + c] = s->f[fi].t2w.T[r][c];
+        radius[*k] = s->f[fi].radius;
+    }
+    (*k)++;
+    for (int i = 0; i < s->f[fi].n_children; i++)
+        dump_rec(s, s->f[fi].children[i], frame, nch, t2w, radius, cap, k);
+}
+
+int oracle_tree_dump(const oracle_scene* s, int32_t* frame, int32_t* n_children,
+                     double* frame2world, double* radius, int capacity) {
+    int k = 0;
+    dump_rec(s, 0, frame, n_children, frame2world, radius, capacity, &k);
+    return k;
+}
+
+int oracle_func_evaluate(const oracle_scene* s, int32_t f, double x, double* y) {
+    int idx = f;
+    if (f == -1) idx = s->def_refl;
+    else if (f == -2) idx = s->def_refr;
+    else if (f == -3) idx = s->def_abs;
+    if (idx < 0 || idx >= s->n_funcs + 3) return 2;
+    return func1_eval(&s->funcs[idx], x, y);
+}
+
+/* ---------------------------------------------------------- intersections */
+typedef struct {
+    int frame;           /* -1: void */
+    v3 position;         /* object frame */
+    v3 normal;           /* object frame */
+    double distance;
+    int outside_in;
+} isect;
+
+static void isect_make(isect* is, int frame, v3 pos, v3 normal, double dist, v3 incident) {
+    is->frame = frame;
+    is->position = pos;
+    is->normal = normal;
+    is->distance = dist;
+    /* Intersection::ray_runs_from_outside_to_inside (Intersection.cpp:133-139) */
+    is->outside_in = dot(normal, incident) < 0.0;
+}
+
+static int hex_inside(double inner, v3 v) {
+    /* HexagonalPrismZ::is_inside (HexagonalPrismZ.cpp:14-29) */
+    const double u = dot(UNIT_U, v), vv = dot(UNIT_V, v), w = dot(UNIT_W, v);
+    return u < inner && u > -inner && vv < inner && vv > -inner && w < inner && w > -inner;
+}
+
+static int bound_inside(const oframe* f, v3 v) {
+    switch (f->kind) {
+    case MLGPU_KIND_SPHERE_CAP_HEX: return hex_inside(f->hex_inner, v);
+    case MLGPU_KIND_SPHERE_CAP_CYL: return v.x * v.x + v.y * v.y < f->cyl_r2;
+    case MLGPU_KIND_SPHERE_CAP_RECT: return fabs(v.x) < f->half_x && fabs(v.y) < f->half_y;
+    default: return 0;
+    }
+}
+
+/* SphericalCapRayIntersectionEquation (…Equation.cpp:6-40) +
+ * SurfaceWithOuterPrismBound::add_causeal_intersection (…Bound.cpp:7-70) */
+static int intersect_cap(const oframe* f, int fi, v3 s, v3 d, isect* out) {
+    const double R = f->p[0];
+    const double dd = dot(s, s), sd = dot(s, d), d2 = dot(d, d);
+    const double Rdz = R * d.z, Rsz = R * s.z;
+    const double P = 2.0 * (sd - Rdz) / d2;
+    const double Q = (dd - 2.0 * Rsz) / d2;
+    /* QuadraticEquation.cpp:12-30 */
+    const double p_over_2 = 0.5 * P;
+    const double inner = p_over_2 * p_over_2 - Q;
+    if (!(inner >= 0.0)) return 0;
+    const double sq = sqrt(inner);
+    const double minus = -p_over_2 - sq, plus = -p_over_2 + sq;
+    const v3 pi = add(s, mul(d, plus)), mi = add(s, mul(d, minus));
+    const int p_in = bound_inside(f, pi), m_in = bound_inside(f, mi);
+    const int p_c = plus > 0.0 && norm_le(pi, f->radius);
+    const int m_c = minus > 0.0 && norm_le(mi, f->radius);
+    double t = minus;
+    int ok = 0;
+    if (p_in && m_in) {
+        if (p_c && m_c) { t = (minus >= plus) ? plus : minus; ok = 1; }
+        else if (p_c) { t = plus; ok = 1; }
+        else if (m_c) { t = minus; ok = 1; }
+    } else if (p_in && p_c) { t = plus; ok = 1; }
+    else if (m_in && m_c) { t = minus; ok = 1; }
+    if (!ok) return 0;
+    const v3 p = add(s, mul(d, t));
+    if (!vec_neq(s, p)) return 0;
+    v3 n = sub(V(0.0, 0.0, R), p);
+    n = dvd(n, norm(n));
+    isect_make(out, fi, p, n, t, d);
+    return 1;
+}
+
+/* XyPlaneRayIntersectionEquation.cpp:6-41 + each planar primitive */
+static int intersect_planar(const oframe* f, int fi, v3 s, v3 d, isect* out) {
+    const int parallel = (d.z == 0. && (d.x != 0. || d.y != 0.));
+    if (parallel) return 0;
+    const double v = -s.z / d.z;
+    if (!(v > 0.0)) return 0;
+    const v3 p = add(s, mul(d, v));
+    int inside = 0;
+    switch (f->kind) {
+    case MLGPU_KIND_PLANE: inside = fabs(p.x) < f->half_x && fabs(p.y) < f->half_y; break;
+    case MLGPU_KIND_DISC: inside = p.x * p.x + p.y * p.y < f->cyl_r2; break;
+    case MLGPU_KIND_HEX_PLANE: inside = hex_inside(f->hex_inner, p); break;
+    case MLGPU_KIND_ANNULUS:
+        inside = (p.x * p.x + p.y * p.y < f->cyl_r2) && !(p.x * p.x + p.y * p.y < f->cyl_r2_inner);
+        break;
+    case MLGPU_KIND_PLANE_DUAL_SPHERICAL: {
+        /* DualSphericalPrismZ::is_inside (DualSphericalPrismZ.cpp:21-27) */
+        const double c1x = +f->ds_dist / 2.0, c2x = -f->ds_dist / 2.0;
+        const double ax = c1x - p.x, ay = 0.0 - p.y;
+        const double bx = c2x - p.x, by = 0.0 - p.y;
+        inside = (ax * ax + ay * ay <= f->ds_r * f->ds_r) && (bx * bx + by * by <= f->ds_r * f->ds_r);
+        break;
+    }
+    case MLGPU_KIND_TRIANGLE: {
+        /* Triangle::is_inside_triangle (Triangle.cpp:92-102) */
+        const double Ax = f->p[0], Ay = f->p[1], Bx = f->p[2], By = f->p[3], Cx = f->p[4], Cy = f->p[5];
+        const int bA = ((p.x - Bx) * (Ay - By) - (Ax - Bx) * (p.y - By)) < 0.0;
+        const int bB = ((p.x - Cx) * (By - Cy) - (Bx - Cx) * (p.y - Cy)) < 0.0;
+        const int bC = ((p.x - Ax) * (Cy - Ay) - (Cx - Ax) * (p.y - Ay)) < 0.0;
+        inside = (bA == bB) && (bB == bC);
+        break;
+    }
+    default: inside = 0;
+    }
+    if (!inside) return 0;
+    if (!vec_neq(s, p)) return 0;
+    isect_make(out, fi, p, V(0.0, 0.0, 1.0), v, d);
+    return 1;
+}
+
+/* ZaxisCylinderRayIntersectionEquation.cpp:6-57 + Cylinder.cpp:121-150 */
+static int intersect_cylinder(const oframe* f, int fi, v3 s, v3 d, isect* out) {
+    const double radius = f->p[0], length = f->p[1];
+    const double dx2 = d.x * d.x, dy2 = d.y * d.y;
+    const double sxdx = s.x * d.x, sydy = s.y * d.y;
+    const double sx2 = s.x * s.x, sy2 = s.y * s.y;
+    const double r2 = radius * radius;
+    const double dnom = dx2 + dy2;
+    const double p = 2.0 * (sxdx + sydy) / dnom;
+    const double q = (sx2 + sy2 - r2) / dnom;
+    const double p_over_2 = 0.5 * p;
+    const double inner = p_over_2 * p_over_2 - q;
+    if (!(inner >= 0.0)) return 0;
+    const double sq = sqrt(inner);
+    const double plus = -p_over_2 + sq, minus = -p_over_2 - sq;
+    int causal = 0;
+    double v = 0.0;
+    if (minus < 0.0 && plus >= 0.0) { causal = 1; v = plus; }
+    if (plus >= 0.0 && minus >= 0.0) { causal = 1; v = minus; }
+    if (!causal) return 0;
+    const v3 pt = add(s, mul(d, v));
+    if (!(fabs(pt.z) <= length / 2.0)) return 0;
+    if (!vec_neq(s, pt)) return 0;
+    v3 n = V(pt.x, pt.y, 0.0);
+    n = dvd(n, norm(n));
+    isect_make(out, fi, pt, n, v, d);
+    return 1;
+}
+
+/* Sphere.cpp:62-105 */
+static int intersect_sphere(const oframe* f, int fi, v3 s, v3 d, isect* out) {
+    const double sd = dot(s, d), d2 = dot(d, d), ss = dot(s, s);
+    const double r2 = f->p[0] * f->p[0];
+    const double P = 2 * (sd / d2), Q = ss / d2 - r2;
+    const double p_over_2 = 0.5 * P;
+    const double inner = p_over_2 * p_over_2 - Q;
+    if (!(inner >= 0.0)) return 0;
+    const double sq = sqrt(inner);
+    const double vP = -p_over_2 + sq, vM = -p_over_2 - sq;
+    double t;
+    if (vP >= 0.0 && vM >= 0.0) t = vM;
+    else if (!(vP < 0.0 && vM < 0.0)) t = vP;
+    else return 0;
+    const v3 p = add(s, mul(d, t));
+    const v3 n = dvd(p, norm(p));
+    if (!vec_neq(s, p)) return 0;
+    isect_make(out, fi, p, n, t, d);
+    return 1;
+}
+
+static int intersect_leaf(const oframe* f, int fi, v3 s, v3 d, isect* out) {
+    switch (f->kind) {
+    case MLGPU_KIND_SPHERE_CAP_HEX:
+    case MLGPU_KIND_SPHERE_CAP_CYL:
+    case MLGPU_KIND_SPHERE_CAP_RECT: return intersect_cap(f, fi, s, d, out);
+    case MLGPU_KIND_PLANE:
+    case MLGPU_KIND_DISC:
+    case MLGPU_KIND_HEX_PLANE:
+    case MLGPU_KIND_ANNULUS:
+    case MLGPU_KIND_PLANE_DUAL_SPHERICAL:
+    case MLGPU_KIND_TRIANGLE: return intersect_planar(f, fi, s, d, out);
+    case MLGPU_KIND_CYLINDER: return intersect_cylinder(f, fi, s, d, out);
+    case MLGPU_KIND_SPHERE: return intersect_sphere(f, fi, s, d, out);
+    default: return 0;
+    }
+}
+
+/* ray_intersects_frames_bounding_sphere (RayAndFrame.cpp:16-107) */
+static int sphere_hit(const oframe* f, v3 s, v3 d) {
+    const v3 c = V(f->t2w.T[0][3], f->t2w.T[1][3], f->t2w.T[2][3]);
+    const double alpha = dot(d, c) - dot(s, d);
+    const v3 q = add(s, mul(d, alpha));
+    const v3 sc = sub(c, q);
+    const double dist2 = dot(sc, sc);
+    if (dist2 > f->radius * f->radius) return 0;
+    if (alpha < 0.0) return norm_le(sub(c, s), f->radius);
+    return 1;
+}
+
+typedef struct {
+    const oracle_scene* s;
+    v3 sup, dir;
+    isect best;
+    int have;
+    int64_t n_sphere, n_leaf;
+} query;
+
+/* CausalIntersection (RayAndFrame.cpp:126-173): DFS in child order, then the
+ * first minimum distance among candidates. */
+static void dfs(query* q, int fi) {
+    const oframe* f = &q->s->f[fi];
+    q->n_sphere++;
+    if (!sphere_hit(f, q->sup, q->dir)) return;
+    if (f->n_children > 0) {
+        for (int i = 0; i < f->n_children; i++) dfs(q, f->children[i]);
+        return;
+    }
+    q->n_leaf++;
+    const v3 ls = ht_pos_inv(&f->t2w, q->sup);
+    const v3 ld = ht_orient_inv(&f->t2w, q->dir);
+    isect is;
+    if (intersect_leaf(f, fi, ls, ld, &is)) {
+        if (!q->have || is.distance < q->best.distance) { q->best = is; q->have = 1; }
+    }
+}
+
+static isect first_hit(const oracle_scene* s, int root, v3 sup, v3 dir, int64_t* ns, int64_t* nl) {
+    query q;
+    q.s = s; q.sup = sup; q.dir = dir; q.have = 0; q.n_sphere = 0; q.n_leaf = 0;
+    dfs(&q, root);
+    if (ns) *ns += q.n_sphere;
+    if (nl) *nl += q.n_leaf;
+    if (q.have) return q.best;
+    isect v;
+    /* void intersection at 1e4 m (RayAndFrame.cpp:161-167) */
+    isect_make(&v, -1, add(sup, mul(dir, 1e4)), dir, 1e4, dir);
+    return v;
+}
+
+int oracle_first_hit(const oracle_scene* s, int32_t root_frame, const double support[3],
+                     const double direction[3], int32_t* frame, double* distance,
+                     double position[3], int64_t* sphere_tests, int64_t* leaf_tests) {
+    if (root_frame < 0 || root_frame >= s->n) { set_err("bad root frame"); return 1; }
+    v3 sup = V(support[0], support[1], support[2]);
+    v3 dir = normalized(V(direction[0], direction[1], direction[2]));
+    isect is = first_hit(s, root_frame, sup, dir, sphere_tests, leaf_tests);
+    *frame = is.frame;
+    *distance = is.distance;
+    v3 p = is.frame >= 0 ? ht_pos(&s->f[is.frame].t2w, is.position) : is.position;
+    position[0] = p.x; position[1] = p.y; position[2] = p.z;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ Philox */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void oracle_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = counter[0], c1 = counter[1], c2 = counter[2], c3 = counter[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += PHILOX_W0; k1 += PHILOX_W1; }
+        const uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+        const uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+double oracle_uniform(uint64_t seed, uint32_t stream, uint64_t photon_id, uint32_t draw) {
+    const uint32_t ctr[4] = {draw >> 2, stream, (uint32_t)photon_id, (uint32_t)(photon_id >> 32)};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    oracle_philox4x32_10(ctr, key, o);
+    return (double)(o[draw & 3] >> 8) * (1.0 / 16777216.0);
+}
+
+typedef struct {
+    uint32_t ctr[4];
+    uint32_t key[2];
+    uint32_t buf[4];
+    uint32_t next;  /* draw index */
+} philox_gen;
+
+static void gen_init(philox_gen* g, uint64_t seed, uint64_t id) {
+    g->ctr[0] = 0; g->ctr[1] = 0;
+    g->ctr[2] = (uint32_t)id; g->ctr[3] = (uint32_t)(id >> 32);
+    g->key[0] = (uint32_t)seed; g->key[1] = (uint32_t)(seed >> 32);
+    g->next = 0;
+}
+
+static double gen_uniform(philox_gen* g) {
+    if ((g->next & 3) == 0) {
+        g->ctr[0] = g->next >> 2;
+        oracle_philox4x32_10(g->ctr, g->key, g->buf);
+    }
+    const uint32_t w = g->buf[g->next & 3];
+    g->next++;
+    return (double)(w >> 8) * (1.0 / 16777216.0);
+}
+
+/* -------------------------------------------------------------- propagator */
+#define C_VACUUM 299792458.0  /* PhysicalConstants.h:7 */
+
+typedef struct {
+    int frame;
+    v3 pos_obj;      /* object frame position */
+    v3 pos_world;
+    double distance;
+    int outside_in;
+} hist_entry;
+
+static int surf_func(const oracle_scene* s, int fi, int slot) {
+    if (fi < 0) return resolve_func(s, slot, -1);  /* VOID_SURFACE_ENTITY */
+    return s->f[fi].surf[slot];
+}
+
+/* Intersection property lookups (Intersection.cpp:83-121) */
+static int eval_coming_from_refraction(const oracle_scene* s, const isect* is, double lam, double* n) {
+    int f = surf_func(s, is->frame, is->outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION);
+    return func1_eval(&s->funcs[f], lam, n);
+}
+
+static void propagate_one(const oracle_scene* s, int world, v3 sup, v3 dir, double lam,
+                          uint32_t max_int, uint64_t seed, uint64_t id,
+                          int64_t k, oracle_results* out, int64_t* ns, int64_t* nl) {
+    philox_gen g;
+    gen_init(&g, seed, id);
+    int root = world;
+    int n_hist = 1;
+    int last_interaction = MLGPU_PRODUCTION;
+    isect last_is;
+    int have_last = 0;
+    v3 prev_world = sup, last_world = sup;
+    double tof = 0.0, acc = 0.0;
+    int err = MLGPU_PHOTON_OK;
+    dir = normalized(dir);                              /* Ray.cpp:16-20 */
+    if (!(lam > 0.0)) err = MLGPU_PHOTON_BAD_WAVELENGTH; /* Photon.cpp:20-28 */
+
+    /* Propagator::propagate loop (PhotonAndFrame.cpp:13-21) */
+    while (!err && (uint32_t)n_hist <= max_int) {
+        isect is = first_hit(s, root, sup, dir, ns, nl);
+        int interaction;
+        int stop = 1;
+        v3 new_sup = sup, new_dir = dir;
+        int new_root = root;
+        if (is.frame < 0) {
+            interaction = MLGPU_ABSORPTION_IN_VOID;
+        } else {
+            /* absorbed_in_medium_before_reaching_surface (:32-38) */
+            const oframe* f = &s->f[is.frame];
+            double ell;
+            int fa = f->surf[is.outside_in ? MLGPU_OUTER_ABSORPTION : MLGPU_INNER_ABSORPTION];
+            if (func1_eval(&s->funcs[fa], lam, &ell)) { err = MLGPU_PHOTON_OUT_OF_TABLE; break; }
+            const double survival = exp(-is.distance / ell);
+            if (gen_uniform(&g) > survival) {
+                interaction = MLGPU_ABSORPTION_IN_VOID;
+            } else {
+                /* interact_with_object (:40-48) */
+                double refl;
+                int fr = f->surf[is.outside_in ? MLGPU_OUTER_REFLECTION : MLGPU_INNER_REFLECTION];
+                if (func1_eval(&s->funcs[fr], lam, &refl)) { err = MLGPU_PHOTON_OUT_OF_TABLE; break; }
+                const v3 hit_world = ht_pos(&f->t2w, is.position);
+                if (refl >= gen_uniform(&g)) {
+                    /* reflect_on_surface_and_propagate_on (:56-66) */
+                    v3 d_obj = ht_orient_inv(&f->t2w, dir);
+                    const double x = is.normal.x, y = is.normal.y, z = is.normal.z;
+                    /* Vec3::mirror (Vec3.cpp:59-113) */
+                    v3 m = V((1. - 2. * x * x) * d_obj.x + -2. * x * y * d_obj.y + -2. * x * z * d_obj.z,
+                             -2. * x * y * d_obj.x + (1. - 2. * y * y) * d_obj.y + -2. * y * z * d_obj.z,
+                             -2. * x * z * d_obj.x + -2. * y * z * d_obj.y + (1. - 2. * z * z) * d_obj.z);
+                    new_sup = hit_world;
+                    new_dir = normalized(ht_orient(&f->t2w, m));
+                    interaction = MLGPU_REFLECTION_ON_SURFACE;
+                    stop = 0;
+                } else {
+                    const int transparent =
+                        f->surf[MLGPU_OUTER_REFRACTION] != s->def_refr ||
+                        f->surf[MLGPU_INNER_REFRACTION] != s->def_refr;
+                    if (!transparent) {
+                        interaction = MLGPU_ABSORPTION_ON_SURFACE;
+                    } else {
+                        /* fresnel_refraction_and_reflection (:75-87) */
+                        double n_from, n_to;
+                        int ff = f->surf[is.outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION];
+                        int ft = f->surf[is.outside_in ? MLGPU_INNER_REFRACTION : MLGPU_OUTER_REFRACTION];
+                        if (func1_eval(&s->funcs[ff], lam, &n_from) ||
+                            func1_eval(&s->funcs[ft], lam, &n_to)) { err = MLGPU_PHOTON_OUT_OF_TABLE; break; }
+                        const v3 inc = ht_orient_inv(&f->t2w, dir);
+                        const v3 nrm = is.outside_in ? is.normal : mul(is.normal, -1.0);
+                        /* Fresnel.cpp:7-38 */
+                        const double cosI = dot(nrm, inc) * (-1.0);
+                        const double eta = n_from / n_to;
+                        const double sinT2 = eta * eta * (1.0 - cosI * cosI);
+                        const double cosT = sqrt(1.0 - sinT2);
+                        double R;
+                        if (sinT2 > 1.0) {
+                            R = 1.0;
+                        } else {
+                            const double rOrth = (n_from * cosI - n_to * cosT) / (n_from * cosI + n_to * cosT);
+                            const double rPar = (n_to * cosI - n_from * cosT) / (n_to * cosI + n_from * cosT);
+                            R = (rOrth * rOrth + rPar * rPar) / 2.0;
+                        }
+                        if (R > gen_uniform(&g)) {
+                            v3 d_obj = ht_orient_inv(&f->t2w, dir);
+                            const double x = is.normal.x, y = is.normal.y, z = is.normal.z;
+                            v3 m = V((1. - 2. * x * x) * d_obj.x + -2. * x * y * d_obj.y + -2. * x * z * d_obj.z,
+                                     -2. * x * y * d_obj.x + (1. - 2. * y * y) * d_obj.y + -2. * y * z * d_obj.z,
+                                     -2. * x * z * d_obj.x + -2. * y * z * d_obj.y + (1. - 2. * z * z) * d_obj.z);
+                            new_sup = hit_world;
+                            new_dir = normalized(ht_orient(&f->t2w, m));
+                            interaction = MLGPU_FRESNEL_REFLECTION_ON_SURFACE;
+                            stop = 0;
+                        } else {
+                            /* pass_the_boundary_layer (:89-118) */
+                            interaction = is.outside_in ? MLGPU_REFRACTION_TO_INSIDE : MLGPU_REFRACTION_TO_OUTSIDE;
+                            const int going_to_default = is.outside_in
+                                ? f->surf[MLGPU_INNER_REFRACTION] == s->def_refr
+                                : f->surf[MLGPU_OUTER_REFRACTION] == s->def_refr;
+                            if (f->allowed >= 0 && !going_to_default) new_root = f->allowed;
+                            else new_root = f->root;
+                            const v3 refr = add(mul(inc, eta), mul(nrm, eta * cosI - cosT));
+                            new_sup = hit_world;
+                            new_dir = normalized(ht_orient(&f->t2w, refr));
+                            stop = 0;
+                        }
+                    }
+                }
+            }
+        }
+        /* push_back_intersection_and_interaction */
+        n_hist++;
+        last_interaction = interaction;
+        last_is = is;
+        have_last = 1;
+        prev_world = last_world;
+        last_world = is.frame >= 0 ? ht_pos(&s->f[is.frame].t2w, is.position) : is.position;
+        acc += is.distance;
+        {
+            double n;
+            if (eval_coming_from_refraction(s, &is, lam, &n)) { err = MLGPU_PHOTON_OUT_OF_TABLE; break; }
+            tof += n * is.distance / C_VACUUM;
+        }
+        sup = new_sup;
+        dir = new_dir;
+        root = new_root;
+        if (stop) break;
+    }
+
+    out->error[k] = (uint8_t)err;
+    out->num_interactions[k] = (uint8_t)n_hist;
+    out->final_interaction[k] = (uint8_t)last_interaction;
+    out->time_of_flight[k] = tof;
+    if (out->accumulated_distance) out->accumulated_distance[k] = acc;
+    if (!have_last) {
+        out->final_frame[k] = -1;
+        out->sensor_id[k] = -1;
+        out->x[k] = sup.x; out->y[k] = sup.y;
+        out->cx[k] = 0.0; out->cy[k] = 0.0;
+        return;
+    }
+    out->final_frame[k] = last_is.frame;
+    out->sensor_id[k] = last_is.frame >= 0 ? s->f[last_is.frame].sensor : -1;
+    out->x[k] = last_is.position.x;
+    out->y[k] = last_is.position.y;
+    /* final_intersection_incident_direction_wrt_frame (RayForPropagation.cpp:114-136) */
+    v3 inc = normalized(sub(last_world, prev_world));
+    v3 loc = last_is.frame >= 0 ? ht_orient_inv(&s->f[last_is.frame].t2w, inc) : inc;
+    out->cx[k] = -1.0 * loc.x;
+    out->cy[k] = -1.0 * loc.y;
+}
+
+typedef struct {
+    const oracle_scene* s;
+    int world;
+    const mlgpu_photons* ph;
+    int64_t begin, end;
+    uint32_t max_int;
+    uint64_t seed, id0;
+    oracle_results* out;
+    int64_t ns, nl;
+} work;
+
+static void* worker(void* arg) {
+    work* w = (work*)arg;
+    for (int64_t i = w->begin; i < w->end; i++) {
+        v3 sup = V(w->ph->support_x[i], w->ph->support_y[i], w->ph->support_z[i]);
+        v3 dir = V(w->ph->direction_x[i], w->ph->direction_y[i], w->ph->direction_z[i]);
+        propagate_one(w->s, w->world, sup, dir, w->ph->wavelength[i], w->max_int, w->seed,
+                      w->id0 + (uint64_t)i, i, w->out, &w->ns, &w->nl);
+    }
+    return NULL;
+}
+
+int oracle_propagate(const oracle_scene* s, int32_t world, const mlgpu_photons* ph, int64_t n,
+                     uint32_t max_int, uint64_t seed, uint64_t id0, oracle_results* out,
+                     int num_threads, int64_t* sphere_tests, int64_t* leaf_tests) {
+    if (!s || !ph || !out || n < 0) { set_err("bad argument"); return 1; }
+    if (world < 0 || world >= s->n) { set_err("bad world frame"); return 1; }
+    if (num_threads < 1) num_threads = 1;
+    if (num_threads > 256) num_threads = 256;
+    work* w = (work*)calloc(num_threads, sizeof(work));
+    pthread_t* th = (pthread_t*)calloc(num_threads, sizeof(pthread_t));
+    const int64_t chunk = (n + num_threads - 1) / num_threads;
+    for (int t = 0; t < num_threads; t++) {
+        w[t].s = s; w[t].world = world; w[t].ph = ph;
+        w[t].begin = t * chunk < n ? t * chunk : n;
+        w[t].end = (t + 1) * chunk < n ? (t + 1) * chunk : n;
+        w[t].max_int = max_int; w[t].seed = seed; w[t].id0 = id0; w[t].out = out;
+    }
+    if (num_threads == 1) {
+        worker(&w[0]);
+    } else {
+        for (int t = 0; t < num_threads; t++) pthread_create(&th[t], NULL, worker, &w[t]);
+        for (int t = 0; t < num_threads; t++) pthread_join(th[t], NULL);
+    }
+    int64_t ns = 0, nl = 0;
+    for (int t = 0; t < num_threads; t++) { ns += w[t].ns; nl += w[t].nl; }
+    if (sphere_tests) *sphere_tests = ns;
+    if (leaf_tests) *leaf_tests = nl;
+    free(w);
+    free(th);
+    return 0;
+}

@@ -1,0 +1,95 @@
+/*
+ * oracle.h -- FP64 CPU restatement of merlict's photon propagation.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the GPU propagator is
+ * compared against; only tests, the smoke check and the CPU-baseline leg of
+ * the benchmark may load it.  The product never links or calls it.
+ *
+ * It consumes the same scene description as the GPU library (include/mlgpu.h)
+ * and restates, in plain C and double precision, the reference algorithm:
+ *   tree build       merlict/Frame.cpp:125-147,190-255, merlict/Frames.cpp
+ *   DFS first hit    merlict/RayAndFrame.cpp:8-173
+ *   primitives       merlict/scenery/primitive/*.cpp, scenery/geometry/*.cpp
+ *   propagator       merlict/PhotonAndFrame.cpp:6-124
+ *   Fresnel          merlict/Fresnel.cpp:7-38
+ *   Func1            merlict/function/Func1.cpp:22-130, Limits.cpp:37-46
+ *   time of flight   merlict/Photon.cpp:44-56
+ *   sensor record    merlict/sensor/Sensor.cpp:14-30, RayForPropagation.cpp:114-136
+ * Random draws come from the Philox4x32-10 stream (Salmon et al., SC'11)
+ * keyed by (seed, photon id), the same stream the GPU kernel consumes.
+ */
+#ifndef MLGPU_ORACLE_H_
+#define MLGPU_ORACLE_H_
+
+#include <stdint.h>
+#include "../include/mlgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_scene oracle_scene;
+
+oracle_scene* oracle_scene_create(const mlgpu_scene_desc* desc);
+void oracle_scene_destroy(oracle_scene* s);
+const char* oracle_last_error(void);
+
+/* Number of frames after clustering (description frames + octant frames). */
+int oracle_num_frames(const oracle_scene* s);
+
+/* Pre-order walk of the initialised tree.  For position k writes the frame
+ * index (octant frames have index >= n_frames of the description), the
+ * number of children, frame2world (12 doubles, row major 3x4) and the
+ * bounding sphere radius.  Returns the number of frames written. */
+int oracle_tree_dump(const oracle_scene* s, int32_t* frame, int32_t* n_children,
+                     double* frame2world, double* radius, int capacity);
+
+/* First intersection of one ray with the frame tree below root_frame.
+ * frame = -1 for no hit.  position in the root frame.  Returns the number
+ * of bounding-sphere tests and leaf tests through the counters if given. */
+int oracle_first_hit(const oracle_scene* s, int32_t root_frame,
+                     const double support[3], const double direction[3],
+                     int32_t* frame, double* distance, double position[3],
+                     int64_t* sphere_tests, int64_t* leaf_tests);
+
+typedef struct oracle_results {
+    int32_t* final_frame;
+    int32_t* sensor_id;
+    uint8_t* final_interaction;
+    uint8_t* num_interactions;
+    uint8_t* error;
+    double* time_of_flight;
+    double* x;
+    double* y;
+    double* cx;
+    double* cy;
+    double* accumulated_distance;
+} oracle_results;
+
+/* Propagate photons (host arrays, double SoA).  One Philox generator per
+ * photon, as merlict_multi_thread.cpp:12-26 uses one Mt19937 per photon.
+ * num_threads <= 1 runs single threaded; otherwise contiguous chunks. */
+int oracle_propagate(const oracle_scene* s, int32_t world_frame,
+                     const mlgpu_photons* photons, int64_t n,
+                     uint32_t max_num_interactions, uint64_t seed,
+                     uint64_t photon_id_offset, oracle_results* out,
+                     int num_threads, int64_t* sphere_tests,
+                     int64_t* leaf_tests);
+
+/* Philox4x32-10 block and the derived 24-bit uniform draw. */
+void oracle_philox4x32_10(const uint32_t counter[4], const uint32_t key[2],
+                          uint32_t out[4]);
+double oracle_uniform(uint64_t seed, uint32_t stream, uint64_t photon_id,
+                      uint32_t draw_index);
+
+/* Func1 evaluation of description function f (-1..-3 = defaults
+ * reflection, refraction, absorption).  Returns 0, or 1 when x is outside
+ * the table. */
+int oracle_func_evaluate(const oracle_scene* s, int32_t f, double x,
+                         double* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* MLGPU_ORACLE_H_ */

@@ -1,0 +1,283 @@
+"""Python access to the oracle and to the reference harness.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline of bench.py, never by the product.
+
+* :class:`OracleScene` -- ctypes binding of ``liboracle.so`` (oracle.c), the
+  FP64 restatement of merlict's propagation.
+* :func:`ref_propagate` & co. -- run ``_ref/merlict_golden``, the harness
+  compiled against the reference's own sources, on files.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_BIN = HERE / "_ref" / "merlict_golden"
+
+_P_I32 = C.POINTER(C.c_int32)
+_P_F64 = C.POINTER(C.c_double)
+_P_U8 = C.POINTER(C.c_uint8)
+
+
+class _Results(C.Structure):
+    _fields_ = [
+        ("final_frame", _P_I32), ("sensor_id", _P_I32), ("final_interaction", _P_U8),
+        ("num_interactions", _P_U8), ("error", _P_U8), ("time_of_flight", _P_F64),
+        ("x", _P_F64), ("y", _P_F64), ("cx", _P_F64), ("cy", _P_F64), ("accumulated_distance", _P_F64),
+    ]
+
+
+class _Photons(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z", "wavelength")]
+
+
+PHOTON_FIELDS = ("support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z", "wavelength")
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing; run `make -C oracle oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        L.oracle_scene_create.restype = C.c_void_p
+        L.oracle_scene_create.argtypes = [C.c_void_p]
+        L.oracle_scene_destroy.argtypes = [C.c_void_p]
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_num_frames.argtypes = [C.c_void_p]
+        L.oracle_tree_dump.argtypes = [C.c_void_p, _P_I32, _P_I32, _P_F64, _P_F64, C.c_int]
+        L.oracle_first_hit.argtypes = [C.c_void_p, C.c_int32, _P_F64, _P_F64, _P_I32, _P_F64, _P_F64,
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.oracle_propagate.argtypes = [C.c_void_p, C.c_int32, C.POINTER(_Photons), C.c_int64, C.c_uint32,
+                                       C.c_uint64, C.c_uint64, C.POINTER(_Results), C.c_int,
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.oracle_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.oracle_uniform.restype = C.c_double
+        L.oracle_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32]
+        L.oracle_func_evaluate.argtypes = [C.c_void_p, C.c_int32, C.c_double, _P_F64]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+def photons_np(d: dict) -> dict:
+    return {k: np.ascontiguousarray(d[k], dtype=np.float64) for k in PHOTON_FIELDS}
+
+
+class OracleScene:
+    """The oracle's scene, built from a product SceneDescription's C view."""
+
+    def __init__(self, desc):
+        self.desc = desc  # keeps the arrays alive
+        self._h = lib().oracle_scene_create(C.addressof(desc.view))
+        if not self._h:
+            raise RuntimeError("oracle_scene_create: " + lib().oracle_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_scene_destroy(self._h)
+            self._h = None
+
+    @property
+    def num_frames(self) -> int:
+        return lib().oracle_num_frames(self._h)
+
+    def tree_dump(self) -> dict:
+        n = self.num_frames
+        frame = np.zeros(n, np.int32)
+        nch = np.zeros(n, np.int32)
+        t2w = np.zeros(12 * n, np.float64)
+        rad = np.zeros(n, np.float64)
+        k = lib().oracle_tree_dump(self._h, _ptr(frame, _P_I32), _ptr(nch, _P_I32), _ptr(t2w, _P_F64),
+                                   _ptr(rad, _P_F64), n)
+        assert k == n
+        return {"frame": frame, "n_children": nch, "frame2world": t2w.reshape(n, 12), "radius": rad}
+
+    def first_hit(self, rays: dict, root_frame: int = 0) -> dict:
+        p = photons_np(rays)
+        n = len(p["support_x"])
+        out = {"frame": np.zeros(n, np.int32), "distance": np.zeros(n), "position": np.zeros((n, 3))}
+        st, lt = C.c_int64(0), C.c_int64(0)
+        for i in range(n):
+            s = np.array([p["support_x"][i], p["support_y"][i], p["support_z"][i]])
+            d = np.array([p["direction_x"][i], p["direction_y"][i], p["direction_z"][i]])
+            f, dist, pos = C.c_int32(), C.c_double(), np.zeros(3)
+            rc = lib().oracle_first_hit(self._h, root_frame, _ptr(s, _P_F64), _ptr(d, _P_F64), C.byref(f),
+                                        C.byref(dist), _ptr(pos, _P_F64), C.byref(st), C.byref(lt))
+            if rc:
+                raise RuntimeError(lib().oracle_last_error().decode())
+            out["frame"][i] = f.value
+            out["distance"][i] = dist.value
+            out["position"][i] = pos
+        out["sphere_tests"], out["leaf_tests"] = st.value, lt.value
+        return out
+
+    def propagate(self, photons: dict, max_num_interactions: int = 5, seed: int = 0, photon_id_offset: int = 0,
+                  threads: int = 1, world_frame: int = 0) -> dict:
+        p = photons_np(photons)
+        n = len(p["support_x"])
+        ph = _Photons(*(p[k].ctypes.data for k in PHOTON_FIELDS))
+        out = {
+            "final_frame": np.zeros(n, np.int32), "sensor_id": np.zeros(n, np.int32),
+            "final_interaction": np.zeros(n, np.uint8), "num_interactions": np.zeros(n, np.uint8),
+            "error": np.zeros(n, np.uint8), "time_of_flight": np.zeros(n), "x": np.zeros(n), "y": np.zeros(n),
+            "cx": np.zeros(n), "cy": np.zeros(n), "accumulated_distance": np.zeros(n),
+        }
+        r = _Results(_ptr(out["final_frame"], _P_I32), _ptr(out["sensor_id"], _P_I32),
+                     _ptr(out["final_interaction"], _P_U8), _ptr(out["num_interactions"], _P_U8),
+                     _ptr(out["error"], _P_U8), _ptr(out["time_of_flight"], _P_F64), _ptr(out["x"], _P_F64),
+                     _ptr(out["y"], _P_F64), _ptr(out["cx"], _P_F64), _ptr(out["cy"], _P_F64),
+                     _ptr(out["accumulated_distance"], _P_F64))
+        st, lt = C.c_int64(0), C.c_int64(0)
+        rc = lib().oracle_propagate(self._h, world_frame, C.byref(ph), n, max_num_interactions, seed,
+                                    photon_id_offset, C.byref(r), threads, C.byref(st), C.byref(lt))
+        if rc:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        out["sphere_tests"], out["leaf_tests"] = st.value, lt.value
+        return out
+
+    def func_evaluate(self, f: int, x: float):
+        y = C.c_double()
+        rc = lib().oracle_func_evaluate(self._h, f, x, C.byref(y))
+        return (None if rc else y.value)
+
+
+def philox4x32_10(counter, key):
+    c = (C.c_uint32 * 4)(*counter)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def uniform(seed: int, photon_id: int, draw: int, stream: int = 0) -> float:
+    return lib().oracle_uniform(seed, stream, photon_id, draw)
+
+
+# --------------------------------------------------------------- file I/O
+def write_photons(path, photons: dict) -> None:
+    p = photons_np(photons)
+    n = len(p["support_x"])
+    with open(path, "wb") as f:
+        f.write(b"MLGPUPH1")
+        f.write(np.int64(n).tobytes())
+        for k in PHOTON_FIELDS:
+            f.write(p[k].tobytes())
+
+
+def read_photons(path) -> dict:
+    with open(path, "rb") as f:
+        assert f.read(8) == b"MLGPUPH1"
+        n = int(np.frombuffer(f.read(8), np.int64)[0])
+        return {k: np.frombuffer(f.read(8 * n), np.float64).copy() for k in PHOTON_FIELDS}
+
+
+def read_results(path) -> dict:
+    with open(path, "rb") as f:
+        assert f.read(8) == b"MLGPURS1"
+        n = int(np.frombuffer(f.read(8), np.int64)[0])
+        out = {}
+        for k, t in (("final_frame", np.int32), ("sensor_id", np.int32), ("final_interaction", np.uint8),
+                     ("num_interactions", np.uint8), ("error", np.uint8), ("time_of_flight", np.float64),
+                     ("x", np.float64), ("y", np.float64), ("cx", np.float64), ("cy", np.float64)):
+            out[k] = np.frombuffer(f.read(np.dtype(t).itemsize * n), t).copy()
+        return out
+
+
+def write_results(path, r: dict) -> None:
+    n = len(r["final_frame"])
+    with open(path, "wb") as f:
+        f.write(b"MLGPURS1")
+        f.write(np.int64(n).tobytes())
+        for k, t in (("final_frame", np.int32), ("sensor_id", np.int32), ("final_interaction", np.uint8),
+                     ("num_interactions", np.uint8), ("error", np.uint8), ("time_of_flight", np.float64),
+                     ("x", np.float64), ("y", np.float64), ("cx", np.float64), ("cy", np.float64)):
+            f.write(np.ascontiguousarray(r[k], dtype=t).tobytes())
+
+
+# ------------------------------------------------------- reference harness
+def ref_available() -> bool:
+    return REF_BIN.exists() and os.access(REF_BIN, os.X_OK)
+
+
+def _run_ref(args, capture=False):
+    proc = subprocess.run([str(REF_BIN), *map(str, args)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                          text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"merlict_golden {args[0]} failed: {proc.stderr.strip()}")
+    return proc.stdout
+
+
+def ref_propagate(desc, photons: dict, seed=0, max_num_interactions=5, photon_id_offset=0, threads=1) -> dict:
+    with tempfile.TemporaryDirectory() as d:
+        sp, pp, rp = Path(d) / "s.bin", Path(d) / "p.bin", Path(d) / "r.bin"
+        desc.save(str(sp))
+        write_photons(pp, photons)
+        _run_ref(["propagate", sp, pp, seed, max_num_interactions, photon_id_offset, rp, threads])
+        return read_results(rp)
+
+
+def ref_tree(desc) -> dict:
+    with tempfile.TemporaryDirectory() as d:
+        sp, tp = Path(d) / "s.bin", Path(d) / "t.bin"
+        desc.save(str(sp))
+        _run_ref(["tree", sp, tp])
+        with open(tp, "rb") as f:
+            assert f.read(8) == b"MLGPUTR1"
+            n = int(np.frombuffer(f.read(8), np.int64)[0])
+            frame = np.frombuffer(f.read(4 * n), np.int32).copy()
+            nch = np.frombuffer(f.read(4 * n), np.int32).copy()
+            t2w = np.frombuffer(f.read(8 * 12 * n), np.float64).copy().reshape(n, 12)
+            rad = np.frombuffer(f.read(8 * n), np.float64).copy()
+        return {"frame": frame, "n_children": nch, "frame2world": t2w, "radius": rad}
+
+
+def ref_first_hit(desc, rays: dict) -> dict:
+    with tempfile.TemporaryDirectory() as d:
+        sp, pp, hp = Path(d) / "s.bin", Path(d) / "p.bin", Path(d) / "h.bin"
+        desc.save(str(sp))
+        write_photons(pp, rays)
+        _run_ref(["firsthit", sp, pp, hp])
+        with open(hp, "rb") as f:
+            assert f.read(8) == b"MLGPUFH1"
+            n = int(np.frombuffer(f.read(8), np.int64)[0])
+            frame = np.frombuffer(f.read(4 * n), np.int32).copy()
+            dist = np.frombuffer(f.read(8 * n), np.float64).copy()
+            pos = np.stack([np.frombuffer(f.read(8 * n), np.float64).copy() for _ in range(3)], axis=1)
+        return {"frame": frame, "distance": dist, "position": pos}
+
+
+def ref_reflector(focal_length, dc, max_outer, min_inner, hex_shape, facet_inner, gap) -> np.ndarray:
+    out = _run_ref(["reflector", focal_length, dc, max_outer, min_inner, hex_shape, facet_inner, gap])
+    return np.array([[float(v) for v in line.split()] for line in out.strip().splitlines()])
+
+
+def ref_lens_maker(f, r, n) -> float:
+    return float(_run_ref(["lensmaker", f, r, n]).strip())
+
+
+def ref_lightfield(f, aperture, fov_deg, pixel_fov_deg, n_paxel, lens_n, bin_refl, housing) -> tuple:
+    out = _run_ref(["lightfield", f, aperture, fov_deg, pixel_fov_deg, n_paxel, lens_n, bin_refl, housing])
+    lines = out.strip().splitlines()
+    head = lines[0].split()
+    meta = {head[i]: float(head[i + 1]) for i in range(1, len(head) - 1, 2)}
+    rows = np.array([[float(v) for v in line.split()] for line in lines[1:]])
+    return meta, rows
+
+
+def ref_bench(scene_path, photons_path, threads, mode, seed=0) -> str:
+    return _run_ref(["bench", scene_path, photons_path, threads, mode, seed])

@@ -1,0 +1,510 @@
+// refharness.cpp -- golden-vector harness around the REFERENCE merlict code.
+//
+// TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile against the
+// reference's own sources (never copied), it builds a reference Frame tree
+// from a scene description file (include/mlgpu.h, "MLGPUSC1"), plugs a
+// Philox4x32-10 stream into merlict::random::Generator (merlict/random/
+// Generator.h:11-24) so the reference consumes the same draws as the GPU
+// kernel and the oracle, and writes what the reference computes:
+//
+//   propagate  per-photon results of merlict::Propagator (PhotonAndFrame.cpp)
+//   tree       pre-order dump of the initialised tree (Frame.cpp:125-147)
+//   firsthit   rays_first_intersection_with_frame (RayAndFrame.cpp:118-124)
+//   reflector  facet poses of segmented_imaging_reflector::Factory
+//   lensmaker  lens_maker::get_curvature_radius
+//   lightfield frames built by light_field_sensor::Factory
+//   bench      timing of the reference CPU path (chunked threads, or the
+//              reference's own merlict_multi_thread driver)
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "merlict/merlict.h"
+#include "merlict/scenery/primitive/primitive.h"
+#include "merlict/scenery/segmented_imaging_reflector/segmented_imaging_reflector.h"
+#include "merlict_multi_thread/merlict_multi_thread.h"
+#include "merlict_portal_plenoscope/light_field_sensor/LightFieldSensor.h"
+
+namespace ml = merlict;
+
+// ------------------------------------------------------------ Philox stream
+static void philox(uint32_t c[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+        const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = static_cast<uint32_t>(p1); c2 = n2; c3 = static_cast<uint32_t>(p0);
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+class PhiloxGenerator : public ml::random::Generator {
+    uint32_t key_[2];
+    uint32_t ctr_[4];
+    uint32_t buf_[4];
+    uint32_t next_ = 0;
+
+ public:
+    PhiloxGenerator(uint64_t seed, uint64_t photon_id) {
+        key_[0] = static_cast<uint32_t>(seed);
+        key_[1] = static_cast<uint32_t>(seed >> 32);
+        ctr_[0] = 0; ctr_[1] = 0;
+        ctr_[2] = static_cast<uint32_t>(photon_id);
+        ctr_[3] = static_cast<uint32_t>(photon_id >> 32);
+        seed_ = seed;
+    }
+    double uniform() override {
+        if ((next_ & 3u) == 0u) {
+            ctr_[0] = next_ >> 2;
+            philox(ctr_, key_, buf_);
+        }
+        const uint32_t w = buf_[next_ & 3u];
+        next_++;
+        return static_cast<double>(w >> 8) * (1.0 / 16777216.0);
+    }
+    uint64_t create_seed() override {
+        const uint64_t a = static_cast<uint64_t>(uniform() * 16777216.0);
+        const uint64_t b = static_cast<uint64_t>(uniform() * 16777216.0);
+        return (a << 24) | b;
+    }
+    double normal(const double mean, const double std_dev) override {
+        const double u1 = 1.0 - uniform(), u2 = uniform();
+        return mean + std_dev * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    }
+};
+
+// ----------------------------------------------------------- description file
+struct Desc {
+    int32_t n = 0, nf = 0, np = 0;
+    std::vector<int32_t> mother, rot_mode, kind, surface, allowed, sensor, func_start;
+    std::vector<double> pos, rot, params, fx, fy;
+};
+
+template <typename T>
+static void rd(std::ifstream& in, std::vector<T>* v, size_t n) {
+    v->resize(n);
+    in.read(reinterpret_cast<char*>(v->data()), static_cast<std::streamsize>(n * sizeof(T)));
+}
+
+static Desc load_desc(const char* path) {
+    std::ifstream in(path, std::ios::binary);
+    char magic[8];
+    int32_t hdr[4];
+    in.read(magic, 8);
+    in.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
+    if (!in || std::memcmp(magic, "MLGPUSC1", 8) != 0) throw std::runtime_error("bad scene file");
+    Desc d;
+    d.n = hdr[0]; d.nf = hdr[1]; d.np = hdr[2];
+    const size_t n = static_cast<size_t>(d.n);
+    rd(in, &d.mother, n); rd(in, &d.pos, 3 * n); rd(in, &d.rot_mode, n); rd(in, &d.rot, 4 * n);
+    rd(in, &d.kind, n); rd(in, &d.params, 6 * n); rd(in, &d.surface, 6 * n);
+    rd(in, &d.allowed, n); rd(in, &d.sensor, n);
+    rd(in, &d.func_start, static_cast<size_t>(d.nf) + 1);
+    rd(in, &d.fx, static_cast<size_t>(d.np)); rd(in, &d.fy, static_cast<size_t>(d.np));
+    if (!in) throw std::runtime_error("truncated scene file");
+    return d;
+}
+
+struct RefScene {
+    ml::Scenery scenery;
+    std::vector<ml::Frame*> frames;
+    std::map<const ml::Frame*, int32_t> index;
+    std::vector<ml::function::Func1*> funcs;
+    std::vector<ml::sensor::Sensor*> sensor_list;
+    ml::sensor::Sensors sensors;
+    ~RefScene() { for (auto* f : funcs) delete f; for (auto* s : sensor_list) delete s; }
+};
+
+template <typename T>
+static T* add_surface(ml::Frame* mother, const Desc& d, int32_t i, RefScene* rs) {
+    T* s = mother->add<T>();
+    const int32_t* sf = &d.surface[6 * i];
+    if (sf[0] >= 0) s->outer_reflection = rs->funcs[sf[0]];
+    if (sf[1] >= 0) s->inner_reflection = rs->funcs[sf[1]];
+    if (sf[2] >= 0) s->outer_refraction = rs->funcs[sf[2]];
+    if (sf[3] >= 0) s->inner_refraction = rs->funcs[sf[3]];
+    if (sf[4] >= 0) s->outer_absorption = rs->funcs[sf[4]];
+    if (sf[5] >= 0) s->inner_absorption = rs->funcs[sf[5]];
+    return s;
+}
+
+static void build(const Desc& d, RefScene* rs) {
+    for (int32_t f = 0; f < d.nf; f++) {
+        std::vector<std::vector<double>> xy;
+        for (int32_t k = d.func_start[f]; k < d.func_start[f + 1]; k++) xy.push_back({d.fx[k], d.fy[k]});
+        rs->funcs.push_back(new ml::function::Func1(xy));
+    }
+    std::vector<int> has_children(d.n, 0);
+    for (int32_t i = 1; i < d.n; i++) has_children[d.mother[i]] = 1;
+    rs->frames.resize(d.n);
+    rs->scenery.root.set_name_pos_rot("root", ml::Vec3(d.pos[0], d.pos[1], d.pos[2]),
+                                      d.rot_mode[0] == 0 ? ml::Rot3(d.rot[0], d.rot[1], d.rot[2])
+                                                         : ml::Rot3(ml::Vec3(d.rot[0], d.rot[1], d.rot[2]), d.rot[3]));
+    rs->frames[0] = &rs->scenery.root;
+    for (int32_t i = 1; i < d.n; i++) {
+        ml::Frame* m = rs->frames[d.mother[i]];
+        const double* p = &d.params[6 * i];
+        ml::Frame* f = nullptr;
+        switch (d.kind[i]) {
+        case 0: f = has_children[i] ? static_cast<ml::Frame*>(add_surface<ml::SurfaceEntity>(m, d, i, rs))
+                                    : m->add<ml::Frame>();
+                break;
+        case 1: { auto* s = add_surface<ml::SphereCapWithHexagonalBound>(m, d, i, rs); f = s; break; }
+        case 2: { auto* s = add_surface<ml::SphereCapWithCylinderBound>(m, d, i, rs); f = s; break; }
+        case 3: { auto* s = add_surface<ml::SphereCapWithRectangularBound>(m, d, i, rs); f = s; break; }
+        case 4: { auto* s = add_surface<ml::Plane>(m, d, i, rs); f = s; break; }
+        case 5: { auto* s = add_surface<ml::Disc>(m, d, i, rs); f = s; break; }
+        case 6: { auto* s = add_surface<ml::HexPlane>(m, d, i, rs); f = s; break; }
+        case 7: { auto* s = add_surface<ml::Annulus>(m, d, i, rs); f = s; break; }
+        case 8: { auto* s = add_surface<ml::PlaneDualSphericalBound>(m, d, i, rs); f = s; break; }
+        case 9: { auto* s = add_surface<ml::Cylinder>(m, d, i, rs); f = s; break; }
+        case 10: { auto* s = add_surface<ml::Triangle>(m, d, i, rs); f = s; break; }
+        case 11: { auto* s = add_surface<ml::Sphere>(m, d, i, rs); f = s; break; }
+        default: throw std::runtime_error("unknown kind");
+        }
+        const ml::Rot3 rot = d.rot_mode[i] == 0 ? ml::Rot3(d.rot[4 * i], d.rot[4 * i + 1], d.rot[4 * i + 2])
+                                                : ml::Rot3(ml::Vec3(d.rot[4 * i], d.rot[4 * i + 1], d.rot[4 * i + 2]),
+                                                           d.rot[4 * i + 3]);
+        f->set_name_pos_rot("f" + std::to_string(i), ml::Vec3(d.pos[3 * i], d.pos[3 * i + 1], d.pos[3 * i + 2]), rot);
+        switch (d.kind[i]) {
+        case 1: static_cast<ml::SphereCapWithHexagonalBound*>(f)->set_curvature_radius_and_outer_hex_radius(p[0], p[1]); break;
+        case 2: static_cast<ml::SphereCapWithCylinderBound*>(f)->set_curvature_radius_and_outer_radius(p[0], p[1]); break;
+        case 3: static_cast<ml::SphereCapWithRectangularBound*>(f)->set_curvature_radius_and_x_y_width(p[0], p[1], p[2]); break;
+        case 4: static_cast<ml::Plane*>(f)->set_x_y_width(p[0], p[1]); break;
+        case 5: static_cast<ml::Disc*>(f)->set_radius(p[0]); break;
+        case 6: static_cast<ml::HexPlane*>(f)->set_outer_hex_radius(p[0]); break;
+        case 7: static_cast<ml::Annulus*>(f)->set_outer_inner_radius(p[0], p[1]); break;
+        case 8: static_cast<ml::PlaneDualSphericalBound*>(f)->set_x_hight_and_y_width(p[0], p[1]); break;
+        case 9: static_cast<ml::Cylinder*>(f)->set_radius_and_length(p[0], p[1]); break;
+        case 10: static_cast<ml::Triangle*>(f)->set_corners_in_xy_plane(p[0], p[1], p[2], p[3], p[4], p[5]); break;
+        case 11: static_cast<ml::Sphere*>(f)->set_radius(p[0]); break;
+        default: break;
+        }
+        rs->frames[i] = f;
+    }
+    for (int32_t i = 0; i < d.n; i++) {
+        rs->index[rs->frames[i]] = i;
+        if (d.allowed[i] >= 0) {
+            auto* s = dynamic_cast<ml::SurfaceEntity*>(rs->frames[i]);
+            if (!s) throw std::runtime_error("restriction on a non-surface frame");
+            s->set_allowed_frames_to_propagate_to(rs->frames[d.allowed[i]]);
+        }
+        if (d.sensor[i] >= 0) rs->sensor_list.push_back(new ml::sensor::Sensor(d.sensor[i], rs->frames[i]));
+    }
+    rs->sensors.init(rs->sensor_list);
+    rs->scenery.root.init_tree_based_on_mother_child_relations();
+}
+
+// ---------------------------------------------------------------- photons
+struct Photons {
+    int64_t n = 0;
+    std::vector<double> sx, sy, sz, dx, dy, dz, wl;
+};
+
+static Photons load_photons(const char* path) {
+    std::ifstream in(path, std::ios::binary);
+    char magic[8];
+    Photons p;
+    in.read(magic, 8);
+    in.read(reinterpret_cast<char*>(&p.n), 8);
+    if (!in || std::memcmp(magic, "MLGPUPH1", 8) != 0) throw std::runtime_error("bad photon file");
+    const size_t n = static_cast<size_t>(p.n);
+    rd(in, &p.sx, n); rd(in, &p.sy, n); rd(in, &p.sz, n);
+    rd(in, &p.dx, n); rd(in, &p.dy, n); rd(in, &p.dz, n); rd(in, &p.wl, n);
+    if (!in) throw std::runtime_error("truncated photon file");
+    return p;
+}
+
+struct Results {
+    std::vector<int32_t> frame, sensor;
+    std::vector<uint8_t> interaction, n_int, err;
+    std::vector<double> tof, x, y, cx, cy;
+    void resize(size_t n) {
+        frame.resize(n); sensor.resize(n); interaction.resize(n); n_int.resize(n); err.resize(n);
+        tof.resize(n); x.resize(n); y.resize(n); cx.resize(n); cy.resize(n);
+    }
+};
+
+template <typename T>
+static void wr(std::ofstream& o, const std::vector<T>& v) {
+    o.write(reinterpret_cast<const char*>(v.data()), static_cast<std::streamsize>(v.size() * sizeof(T)));
+}
+
+static void one_photon(const RefScene& rs, const Photons& p, int64_t i, uint64_t seed, uint64_t id0,
+                       const ml::PropagationConfig* cfg, Results* r) {
+    try {
+        ml::Photon ph(ml::Vec3(p.sx[i], p.sy[i], p.sz[i]), ml::Vec3(p.dx[i], p.dy[i], p.dz[i]), p.wl[i]);
+        PhiloxGenerator gen(seed, id0 + static_cast<uint64_t>(i));
+        ml::PropagationEnvironment env;
+        env.root_frame = &rs.scenery.root;
+        env.config = cfg;
+        env.prng = &gen;
+        ml::Propagator(&ph, env);
+        const ml::Intersection& fi = ph.final_intersection();
+        const ml::Frame* obj = fi.object();
+        auto it = rs.index.find(obj);
+        r->frame[i] = it != rs.index.end() ? it->second : -1;
+        ml::sensor::FindSensorByFrame finder(obj, &rs.sensors.by_frame);
+        r->sensor[i] = finder.is_absorbed_by_known_sensor ? static_cast<int32_t>(finder.final_sensor->id) : -1;
+        r->interaction[i] = static_cast<uint8_t>(ph.final_interaction());
+        r->n_int[i] = static_cast<uint8_t>(ph.num_interactions());
+        r->err[i] = 0;
+        // VOID_SURFACE_ENTITY is constructed before SurfaceEntity::DEFAULT_*
+        // in SurfaceEntity.cpp, so its function pointers are null and
+        // time_of_flight() would dereference them for photons ending in void.
+        r->tof[i] = obj != &ml::VOID_SURFACE_ENTITY ? ph.time_of_flight() : NAN;
+        r->x[i] = fi.position_in_object_frame().x;
+        r->y[i] = fi.position_in_object_frame().y;
+        const ml::Vec3 inc = ph.final_intersection_incident_direction_wrt_frame();
+        r->cx[i] = -1.0 * inc.x;
+        r->cy[i] = -1.0 * inc.y;
+    } catch (const std::invalid_argument&) {
+        r->err[i] = 1;
+    } catch (const std::out_of_range&) {
+        r->err[i] = 2;
+    }
+}
+
+static int cmd_propagate(int argc, char** argv) {
+    if (argc < 8) return 2;
+    Desc d = load_desc(argv[2]);
+    Photons p = load_photons(argv[3]);
+    const uint64_t seed = std::strtoull(argv[4], nullptr, 10);
+    const unsigned max_int = static_cast<unsigned>(std::strtoul(argv[5], nullptr, 10));
+    const uint64_t id0 = std::strtoull(argv[6], nullptr, 10);
+    const int threads = argc > 8 ? std::atoi(argv[8]) : 1;
+    RefScene rs;
+    build(d, &rs);
+    ml::PropagationConfig cfg;
+    cfg.max_num_interactions_per_photon = max_int;
+    Results r;
+    r.resize(static_cast<size_t>(p.n));
+    std::vector<std::thread> pool;
+    const int64_t chunk = (p.n + threads - 1) / threads;
+    for (int t = 0; t < threads; t++) {
+        pool.emplace_back([&, t]() {
+            for (int64_t i = t * chunk; i < std::min<int64_t>(p.n, (t + 1) * chunk); i++)
+                one_photon(rs, p, i, seed, id0, &cfg, &r);
+        });
+    }
+    for (auto& th : pool) th.join();
+    std::ofstream o(argv[7], std::ios::binary);
+    o.write("MLGPURS1", 8);
+    o.write(reinterpret_cast<const char*>(&p.n), 8);
+    wr(o, r.frame); wr(o, r.sensor); wr(o, r.interaction); wr(o, r.n_int); wr(o, r.err);
+    wr(o, r.tof); wr(o, r.x); wr(o, r.y); wr(o, r.cx); wr(o, r.cy);
+    return o ? 0 : 1;
+}
+
+static void dump_rec(const ml::Frame* f, const RefScene& rs, std::vector<int32_t>* idx, std::vector<int32_t>* nch,
+                     std::vector<double>* t2w, std::vector<double>* rad) {
+    auto it = rs.index.find(f);
+    idx->push_back(it != rs.index.end() ? it->second : -1);
+    nch->push_back(static_cast<int32_t>(f->get_children()->size()));
+    const ml::HomTra3* T = f->frame2world();
+    const ml::Vec3 cx = T->rot_x(), cy = T->rot_y(), cz = T->rot_z(), tr = T->translation();
+    const double row[12] = {cx.x, cy.x, cz.x, tr.x, cx.y, cy.y, cz.y, tr.y, cx.z, cy.z, cz.z, tr.z};
+    for (double v : row) t2w->push_back(v);
+    rad->push_back(f->get_bounding_sphere_radius());
+    for (const ml::Frame* c : *f->get_children()) dump_rec(c, rs, idx, nch, t2w, rad);
+}
+
+static int cmd_tree(int argc, char** argv) {
+    if (argc < 4) return 2;
+    Desc d = load_desc(argv[2]);
+    RefScene rs;
+    build(d, &rs);
+    std::vector<int32_t> idx, nch;
+    std::vector<double> t2w, rad;
+    dump_rec(&rs.scenery.root, rs, &idx, &nch, &t2w, &rad);
+    std::ofstream o(argv[3], std::ios::binary);
+    const int64_t n = static_cast<int64_t>(idx.size());
+    o.write("MLGPUTR1", 8);
+    o.write(reinterpret_cast<const char*>(&n), 8);
+    wr(o, idx); wr(o, nch); wr(o, t2w); wr(o, rad);
+    return o ? 0 : 1;
+}
+
+static int cmd_firsthit(int argc, char** argv) {
+    if (argc < 5) return 2;
+    Desc d = load_desc(argv[2]);
+    Photons p = load_photons(argv[3]);
+    RefScene rs;
+    build(d, &rs);
+    std::vector<int32_t> frame(p.n);
+    std::vector<double> dist(p.n), px(p.n), py(p.n), pz(p.n);
+    for (int64_t i = 0; i < p.n; i++) {
+        ml::Ray ray(ml::Vec3(p.sx[i], p.sy[i], p.sz[i]), ml::Vec3(p.dx[i], p.dy[i], p.dz[i]));
+        ml::Intersection is = ml::rays_first_intersection_with_frame(&ray, &rs.scenery.root);
+        auto it = rs.index.find(is.object());
+        frame[i] = is.does_intersect() && it != rs.index.end() ? it->second : -1;
+        dist[i] = is.distance_to_ray_support();
+        const ml::Vec3 w = is.position_in_root_frame();
+        px[i] = w.x; py[i] = w.y; pz[i] = w.z;
+    }
+    std::ofstream o(argv[4], std::ios::binary);
+    o.write("MLGPUFH1", 8);
+    o.write(reinterpret_cast<const char*>(&p.n), 8);
+    wr(o, frame); wr(o, dist); wr(o, px); wr(o, py); wr(o, pz);
+    return o ? 0 : 1;
+}
+
+static void print_rot(const ml::Rot3& r) {
+    if (r.uses_xyz_angels())
+        std::printf(" 0 %.17g %.17g %.17g 0", r.rot_x(), r.rot_y(), r.rot_z());
+    else
+        std::printf(" 1 %.17g %.17g %.17g %.17g", r.rot_axis().x, r.rot_axis().y, r.rot_axis().z, r.rot_angle());
+}
+
+static int cmd_reflector(int argc, char** argv) {
+    if (argc < 9) return 2;
+    ml::segmented_imaging_reflector::Config c;
+    c.focal_length = std::atof(argv[2]);
+    c.DaviesCotton_over_parabolic_mixing_factor = std::atof(argv[3]);
+    c.max_outer_aperture_radius = std::atof(argv[4]);
+    c.min_inner_aperture_radius = std::atof(argv[5]);
+    c.outer_aperture_shape_hex = std::atoi(argv[6]);
+    c.facet_inner_hex_radius = std::atof(argv[7]);
+    c.gap_between_facets = std::atof(argv[8]);
+    ml::SurfaceEntity refl;
+    refl.set_name_pos_rot("reflector", ml::VEC3_ORIGIN, ml::ROT3_UNITY);
+    ml::segmented_imaging_reflector::Factory fab(c);
+    fab.add_to_SurfaceEntity(&refl);
+    for (const ml::Frame* f : *refl.get_children()) {
+        const ml::Vec3 p = f->position_in_mother();
+        std::printf("%.17g %.17g %.17g", p.x, p.y, p.z);
+        print_rot(f->rotation_in_mother());
+        std::printf(" %.17g\n", f->get_bounding_sphere_radius());
+    }
+    return 0;
+}
+
+static int cmd_lensmaker(int argc, char** argv) {
+    if (argc < 5) return 2;
+    ml::lens_maker::Config c;
+    c.focal_length = std::atof(argv[2]);
+    c.aperture_radius = std::atof(argv[3]);
+    c.refractive_index = std::atof(argv[4]);
+    std::printf("%.17g\n", ml::lens_maker::get_curvature_radius(c));
+    return 0;
+}
+
+static void lf_rec(const ml::Frame* f, int depth) {
+    const ml::Vec3 p = f->position_in_mother();
+    int kind = 0;
+    if (dynamic_cast<const ml::SphereCapWithHexagonalBound*>(f)) kind = 1;
+    else if (dynamic_cast<const ml::Plane*>(f)) kind = 4;
+    else if (dynamic_cast<const ml::Disc*>(f)) kind = 5;
+    else if (dynamic_cast<const ml::HexPlane*>(f)) kind = 6;
+    else if (dynamic_cast<const ml::Annulus*>(f)) kind = 7;
+    else if (dynamic_cast<const ml::PlaneDualSphericalBound*>(f)) kind = 8;
+    else if (dynamic_cast<const ml::Cylinder*>(f)) kind = 9;
+    std::printf("%d %d %zu %.17g %.17g %.17g", depth, kind, f->get_children()->size(), p.x, p.y, p.z);
+    print_rot(f->rotation_in_mother());
+    std::printf(" %.17g\n", f->get_bounding_sphere_radius());
+    for (const ml::Frame* c : *f->get_children()) lf_rec(c, depth + 1);
+}
+
+// light_field_sensor::Factory on the plenoscope example configuration
+static int cmd_lightfield(int argc, char** argv) {
+    if (argc < 9) return 2;
+    plenoscope::light_field_sensor::Config cfg;
+    static ml::function::Func1 refr({{200e-9, std::atof(argv[7])}, {1200e-9, std::atof(argv[7])}});
+    static ml::function::Func1 refl({{200e-9, std::atof(argv[8])}, {1200e-9, std::atof(argv[8])}});
+    cfg.expected_imaging_system_focal_length = std::atof(argv[2]);
+    cfg.expected_imaging_system_max_aperture_radius = std::atof(argv[3]);
+    cfg.max_FoV_diameter = ml::deg2rad(std::atof(argv[4]));
+    cfg.pixel_FoV_hex_flat2flat = ml::deg2rad(std::atof(argv[5]));
+    cfg.num_paxel_on_pixel_diagonal = static_cast<unsigned>(std::atoi(argv[6]));
+    cfg.housing_overhead = argc > 9 ? std::atof(argv[9]) : 1.1;
+    cfg.lens_refraction = &refr;
+    cfg.bin_reflection = &refl;
+    plenoscope::light_field_sensor::Geometry geo(cfg);
+    ml::Scenery scenery;
+    ml::Frame* lfs = scenery.root.add<ml::Frame>();
+    lfs->set_name_pos_rot("light_field_sensor", ml::VEC3_ORIGIN, ml::ROT3_UNITY);
+    plenoscope::light_field_sensor::Factory fab(&geo);
+    fab.add_light_field_sensor_to_frame_in_scenery(lfs, &scenery);
+    std::printf("# lens_R %.17g lens_f %.17g num_lixel %u\n", geo.pixel_lens_curvature_radius(),
+                geo.pixel_lens_focal_length(), geo.num_lixel());
+    lf_rec(lfs, 0);
+    return 0;
+}
+
+// bench <scene> <photons> <threads> <mode: chunked|multi_thread> <seed>
+static int cmd_bench(int argc, char** argv) {
+    if (argc < 7) return 2;
+    Desc d = load_desc(argv[2]);
+    Photons p = load_photons(argv[3]);
+    const int threads = std::max(1, std::atoi(argv[4]));
+    const std::string mode = argv[5];
+    const uint64_t seed = std::strtoull(argv[6], nullptr, 10);
+    RefScene rs;
+    build(d, &rs);
+    ml::PropagationConfig cfg;
+    Results r;
+    r.resize(static_cast<size_t>(p.n));
+    const auto t0 = std::chrono::steady_clock::now();
+    if (mode == "multi_thread") {
+        std::vector<ml::Photon> photons;
+        photons.reserve(static_cast<size_t>(p.n));
+        for (int64_t i = 0; i < p.n; i++)
+            photons.emplace_back(ml::Vec3(p.sx[i], p.sy[i], p.sz[i]), ml::Vec3(p.dx[i], p.dy[i], p.dz[i]), p.wl[i]);
+        ml::random::Mt19937 prng(seed);
+        const auto t1 = std::chrono::steady_clock::now();
+        ml::propagate_photons_in_frame_with_config_multi_thread(&photons, &rs.scenery.root, &cfg, &prng);
+        const auto t2 = std::chrono::steady_clock::now();
+        const double s = std::chrono::duration<double>(t2 - t1).count();
+        std::printf("{\"photons\": %lld, \"seconds\": %.6f, \"threads\": %u, \"mode\": \"multi_thread\"}\n",
+                    static_cast<long long>(p.n), s, std::thread::hardware_concurrency());
+        return 0;
+    }
+    std::vector<std::thread> pool;
+    const int64_t chunk = (p.n + threads - 1) / threads;
+    for (int t = 0; t < threads; t++) {
+        pool.emplace_back([&, t]() {
+            for (int64_t i = t * chunk; i < std::min<int64_t>(p.n, (t + 1) * chunk); i++)
+                one_photon(rs, p, i, seed, 0, &cfg, &r);
+        });
+    }
+    for (auto& th : pool) th.join();
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int64_t detected = 0;
+    for (int64_t i = 0; i < p.n; i++) detected += r.sensor[i] >= 0;
+    std::printf("{\"photons\": %lld, \"seconds\": %.6f, \"threads\": %d, \"mode\": \"chunked\", \"detected\": %lld}\n",
+                static_cast<long long>(p.n), s, threads, static_cast<long long>(detected));
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: merlict_golden propagate|tree|firsthit|reflector|lensmaker|lightfield|bench ...\n");
+        return 2;
+    }
+    try {
+        const std::string c = argv[1];
+        int rc = 2;
+        if (c == "propagate") rc = cmd_propagate(argc, argv);
+        else if (c == "tree") rc = cmd_tree(argc, argv);
+        else if (c == "firsthit") rc = cmd_firsthit(argc, argv);
+        else if (c == "reflector") rc = cmd_reflector(argc, argv);
+        else if (c == "lensmaker") rc = cmd_lensmaker(argc, argv);
+        else if (c == "lightfield") rc = cmd_lightfield(argc, argv);
+        else if (c == "bench") rc = cmd_bench(argc, argv);
+        if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", c.c_str());
+        return rc;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}
