@@ -1,0 +1,149 @@
+"""Benchmark: photons propagated per second on one MI355X.
+
+Workload: the FACT-like telescope (30-facet Davies-Cotton dish, f = 4.889 m,
+1440 hexagonal pixels of 9.5 mm pitch behind PMMA light guides) and a
+synthetic bunch of 1e7 photons per step: supports uniform on a 4.5 m disc,
+directions Gaussian (0.5 deg) about the optical axis, lambda in [280, 700] nm
+with pdf ~ lambda^-2.  A step is one propagation of the whole bunch, inputs
+resident in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import torch
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from merlict_development_kit_amd import GpuScene, PropagationConfig, PropagationResult, SceneDescription, sources  # noqa: E402
+
+METRIC = "photons propagated/sec on 1 MI355X, plain vs surrogate hybrid at ≤1% histogram L1"
+HBM_PEAK = 8.0e12
+IN_BYTES = 7 * 8            # support xyz, direction xyz, wavelength (float64)
+OUT_BYTES = 4 + 4 + 1 + 1 + 1 + 8 + 4 * 4  # frame, sensor, interaction, n, error, tof, x y cx cy
+
+
+def cpu_baseline(desc: SceneDescription, threads: int, target_seconds: float = 12.0) -> dict:
+    """The reference CPU path (oracle/_ref, compiled from the reference's own
+    sources) or, where it is absent, the oracle, on a bounded sample of the
+    same workload."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+
+    with tempfile.TemporaryDirectory() as d:
+        sp = Path(d) / "scene.bin"
+        desc.save(str(sp))
+
+        def run(n):
+            ph = sources.fact_bunch(n, seed=12345).numpy()
+            if O.ref_available():
+                pp = Path(d) / "p.bin"
+                O.write_photons(pp, ph)
+                out = json.loads(O.ref_bench(sp, pp, threads, "chunked").strip().splitlines()[-1])
+                return out["seconds"], "reference"
+            t0 = time.perf_counter()
+            O.OracleScene(desc).propagate(ph, threads=threads)
+            return time.perf_counter() - t0, "port"
+
+        n = 50_000
+        sec, kind = run(n)
+        n2 = int(min(20_000_000, max(n, n * target_seconds / max(sec, 1e-3))))
+        sec, kind = run(n2)
+        return {"value": n2 / sec, "unit": "photons/s", "cores": threads, "kind": kind,
+                "sample": f"{n2} photons of the same FACT-like workload (seed 12345), "
+                          f"{threads} threads, contiguous chunks, one Philox stream per photon, "
+                          f"{sec:.1f} s wall"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--photons", type=int, default=10_000_000)
+    ap.add_argument("--no-light-guides", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the photon propagator runs on a single MI355X here; "
+              "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+        return 2
+    dev = torch.device("cuda:0")
+    desc = SceneDescription.fact(light_guides=not args.no_light_guides)
+    scene = GpuScene(desc)
+    n = args.photons
+    photons = sources.fact_bunch(n, seed=2024, device=dev)
+    out = PropagationResult.empty(n, dev)
+    stream = torch.cuda.current_stream()
+
+    kernel_ms = []
+    for step in range(args.warmup + args.steps):
+        if step == args.warmup:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        scene.propagate(photons, PropagationConfig(5, seed=step), out=out, stream=stream)
+        e1.record(stream)
+        if step >= args.warmup:
+            kernel_ms.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_kernel = sum(a.elapsed_time(b) for a, b in kernel_ms) / len(kernel_ms)
+    detected = int((out.sensor_id >= 0).sum().item())
+    errors = scene.count_errors(out)
+    value = n * args.steps / wall
+    achieved = n * (IN_BYTES + OUT_BYTES) / (ms_kernel * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 traversal, f64 leaf solves and physics",
+        "data": "synthetic",
+        "config": {
+            "workload": "FACT-like telescope with PMMA light guides (plain propagator)"
+            if not args.no_light_guides else "FACT-like telescope without light guides (plain propagator)",
+            "photons_per_step": n,
+            "leaves": scene.stats()["leaves"],
+            "max_num_interactions_per_photon": 5,
+            "detected_fraction": detected / n,
+            "photon_errors": errors,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None,
+            "kernel": "propagate_kernel",
+            "kernel_ms": ms_kernel,
+            "algorithmic_bytes_per_photon": IN_BYTES + OUT_BYTES,
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(desc, args.cpu_threads)
+        line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

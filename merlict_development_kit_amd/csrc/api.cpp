@@ -69,7 +69,7 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
         return MLGPU_ERR_SCENE;
     }
     auto* s = new mlgpu_scene();
-    hipGetDevice(&s->device);
+    (void)hipGetDevice(&s->device);
     int rc = MLGPU_OK;
     if ((rc = upload(s, hs.nodes, &s->dev.nodes)) || (rc = upload(s, hs.geo, &s->dev.geo)) ||
         (rc = upload(s, hs.phys, &s->dev.phys)) || (rc = upload(s, hs.funcs, &s->dev.funcs)) ||
@@ -95,7 +95,7 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
 
 void mlgpu_scene_destroy(mlgpu_scene* s) {
     if (!s) return;
-    for (void* p : s->buffers) hipFree(p);
+    for (void* p : s->buffers) (void)hipFree(p);
     delete s;
 }
 
@@ -146,7 +146,7 @@ int mlgpu_count_errors(const mlgpu_results* r, int64_t n, void* stream, int64_t*
     unsigned long long host = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&host, counter, sizeof(host), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(counter);
+    (void)hipFree(counter);
     if (e != hipSuccess) return hip_fail(e, "count errors");
     *num_errors = static_cast<int64_t>(host);
     return MLGPU_OK;

@@ -91,17 +91,18 @@ __device__ __forceinline__ bool hex_inside(double inner, double x, double y) {
     return u < inner && u > -inner && v < inner && v > -inner && w < inner && w > -inner;
 }
 
-__device__ __forceinline__ bool cap_bound_inside(const LeafGeo& g, double x, double y) {
-    if (g.kind == MLGPU_KIND_SPHERE_CAP_HEX) return hex_inside(g.p[1], x, y);
-    if (g.kind == MLGPU_KIND_SPHERE_CAP_CYL) return x * x + y * y < g.p[1];
-    return fabs(x) < g.p[1] && fabs(y) < g.p[2];
+__device__ __forceinline__ bool cap_bound_inside(int32_t kind, double b1, double b2, double x, double y) {
+    if (kind == MLGPU_KIND_SPHERE_CAP_HEX) return hex_inside(b1, x, y);
+    if (kind == MLGPU_KIND_SPHERE_CAP_CYL) return x * x + y * y < b1;
+    return fabs(x) < b1 && fabs(y) < b2;
 }
 
 // Intersection of a ray (world support s, unit direction d) with one leaf.
 // Returns true and the hit when merlict's calculate_intersection_with of
 // that primitive would emit an intersection.
-__device__ bool intersect_leaf(const LeafGeo& g, double sx, double sy, double sz, double dx, double dy,
+__device__ bool intersect_leaf(const LeafGeo* __restrict__ gp, double sx, double sy, double sz, double dx, double dy,
                                double dz, LeafHit& h) {
+    const LeafGeo& g = *gp;
     // move the support to the point of the ray closest to the leaf origin
     const double wx = g.tx - sx, wy = g.ty - sy, wz = g.tz - sz;
     const double t0 = wx * dx + wy * dy + wz * dz;
@@ -129,8 +130,9 @@ __device__ bool intersect_leaf(const LeafGeo& g, double sx, double sy, double sz
         const double mi = -p2 - sq, pl = -p2 + sq;
         const double pxp = sxl + dxl * pl, pyp = syl + dyl * pl, pzp = szl + dzl * pl;
         const double pxm = sxl + dxl * mi, pym = syl + dyl * mi, pzm = szl + dzl * mi;
-        const bool p_in = cap_bound_inside(g, pxp, pyp);
-        const bool m_in = cap_bound_inside(g, pxm, pym);
+        const double b1 = g.p[1], b2 = g.p[2];
+        const bool p_in = cap_bound_inside(kind, b1, b2, pxp, pyp);
+        const bool m_in = cap_bound_inside(kind, b1, b2, pxm, pym);
         const double rb2 = g.p[7] * g.p[7];
         const bool p_c = (t0 + pl) > 0.0 && pxp * pxp + pyp * pyp + pzp * pzp <= rb2;
         const bool m_c = (t0 + mi) > 0.0 && pxm * pxm + pym * pym + pzm * pzm <= rb2;
@@ -216,7 +218,8 @@ __device__ bool intersect_leaf(const LeafGeo& g, double sx, double sy, double sz
     return true;
 }
 
-__device__ __forceinline__ void leaf_normal(const LeafGeo& g, const LeafHit& h, double& nx, double& ny, double& nz) {
+__device__ __forceinline__ void leaf_normal(const LeafGeo* __restrict__ gp, const LeafHit& h, double& nx, double& ny, double& nz) {
+    const LeafGeo& g = *gp;
     const int32_t kind = g.kind;
     if (kind <= MLGPU_KIND_SPHERE_CAP_RECT) {
         // SphericalCapRayIntersectionEquation.cpp:52-59
@@ -268,7 +271,7 @@ __device__ __forceinline__ void first_hit(const DeviceScene& sc, int32_t scope, 
         if (!hit) { i = nd.skip; continue; }
         if (nd.leaf >= 0) {
             LeafHit h;
-            if (intersect_leaf(sc.geo[nd.leaf], sx, sy, sz, dx, dy, dz, h) && h.t < best.hit.t) {
+            if (intersect_leaf(sc.geo + nd.leaf, sx, sy, sz, dx, dy, dz, h) && h.t < best.hit.t) {
                 best.leaf = nd.leaf;
                 best.hit = h;
                 best_f = static_cast<float>(h.t) * (1.0f + 1e-6f) + sc.eps;
@@ -328,10 +331,10 @@ __global__ __launch_bounds__(256) void propagate_kernel(const DeviceScene sc, co
             out_cy = -dy;
             break;
         }
-        const LeafGeo g = sc.geo[c.leaf];
+        const LeafGeo& g = sc.geo[c.leaf];
         const LeafPhys ph = sc.phys[c.leaf];
         double nx, ny, nz;
-        leaf_normal(g, c.hit, nx, ny, nz);
+        leaf_normal(&g, c.hit, nx, ny, nz);
         // Intersection::ray_runs_from_outside_to_inside (Intersection.cpp:133-139)
         const bool outside_in = (c.hit.dx * nx + c.hit.dy * ny + c.hit.dz * nz) < 0.0;
         const double* R = g.rot;
@@ -475,7 +478,7 @@ __global__ __launch_bounds__(256) void first_hit_kernel(const DeviceScene sc, co
         out.position_z[i] = sz + dz * kVoidDistance;
         return;
     }
-    const LeafGeo g = sc.geo[c.leaf];
+    const LeafGeo& g = sc.geo[c.leaf];
     out.frame[i] = sc.phys[c.leaf].frame;
     out.distance[i] = c.hit.t;
     const double* R = g.rot;

@@ -124,7 +124,10 @@ class GpuScene:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            N.lib().mlgpu_scene_destroy(h)
+            try:
+                N.lib().mlgpu_scene_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     def stats(self) -> dict:

@@ -25,7 +25,10 @@ class SceneDescription:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            N.lib().mlgpu_desc_destroy(h)
+            try:
+                N.lib().mlgpu_desc_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     # ------------------------------------------------------------ builders

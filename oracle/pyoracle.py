@@ -89,7 +89,10 @@ class OracleScene:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().oracle_scene_destroy(self._h)
+            try:
+                lib().oracle_scene_destroy(self._h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     @property

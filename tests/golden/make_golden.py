@@ -1,0 +1,99 @@
+"""Write the golden fixtures of tests/golden/ by running the REFERENCE.
+
+Runs oracle/_ref/merlict_golden (the reference's own sources compiled by
+oracle/Makefile, with the Philox stream plugged into merlict's
+random::Generator) on seeded inputs and stores inputs and outputs as small
+.npz files.  Run in the container that has the reference mounted:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+import pyoracle as O  # noqa: E402
+from merlict_development_kit_amd import SceneDescription, sources  # noqa: E402
+
+PHOTON_KEYS = O.PHOTON_FIELDS
+RESULT_KEYS = ("final_frame", "sensor_id", "final_interaction", "num_interactions", "error", "time_of_flight",
+               "x", "y", "cx", "cy")
+
+
+def save_propagation(name, desc, photons, seed, max_int=5):
+    ref = O.ref_propagate(desc, photons, seed=seed, max_num_interactions=max_int)
+    arrays = {f"in_{k}": np.asarray(photons[k]) for k in PHOTON_KEYS}
+    arrays.update({f"out_{k}": ref[k] for k in RESULT_KEYS})
+    arrays["seed"] = np.array([seed], np.uint64)
+    arrays["max_num_interactions"] = np.array([max_int], np.uint32)
+    np.savez_compressed(HERE / f"{name}.npz", **arrays)
+    print(name, "photons", len(photons["support_x"]), "detected", int((ref["sensor_id"] >= 0).sum()))
+
+
+def probe_rays(n, seed):
+    """Rays from above the FACT dish towards random points of the dish and of
+    the camera plane."""
+    rng = np.random.default_rng(seed)
+    tx = rng.uniform(-2.2, 2.2, n)
+    ty = rng.uniform(-2.2, 2.2, n)
+    tz = np.where(rng.uniform(size=n) < 0.7, 0.1, 4.889)
+    sx = rng.uniform(-1.0, 1.0, n)
+    sy = rng.uniform(-1.0, 1.0, n)
+    sz = np.full(n, 12.0)
+    d = np.stack([tx - sx, ty - sy, tz - sz], axis=1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return {"support_x": sx, "support_y": sy, "support_z": sz, "direction_x": d[:, 0], "direction_y": d[:, 1],
+            "direction_z": d[:, 2], "wavelength": np.full(n, 433e-9)}
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/merlict_golden is not built")
+    lens = SceneDescription.lens()
+    save_propagation("lens_propagation", lens, sources.lens_beam(2000, seed=1).numpy(), seed=7)
+    fact = SceneDescription.fact(light_guides=False)
+    save_propagation("fact_propagation", fact, sources.fact_bunch(2000, seed=3).numpy(), seed=11)
+    fact_lg = SceneDescription.fact(light_guides=True)
+    save_propagation("fact_lg_propagation", fact_lg, sources.fact_bunch(2000, seed=5).numpy(), seed=13)
+    # deeper histories: a higher interaction limit on the light-guide camera
+    save_propagation("fact_lg_max10_propagation", fact_lg, sources.fact_bunch(1000, seed=6).numpy(), seed=17,
+                     max_int=10)
+
+    tree = O.ref_tree(fact)
+    np.savez_compressed(HERE / "fact_tree.npz", **tree)
+    print("fact tree", len(tree["frame"]))
+
+    rays = probe_rays(2000, seed=21)
+    hits = O.ref_first_hit(fact, rays)
+    np.savez_compressed(HERE / "fact_first_hit.npz", **{f"in_{k}": rays[k] for k in PHOTON_KEYS},
+                        out_frame=hits["frame"], out_distance=hits["distance"], out_position=hits["position"])
+    print("probe rays", len(rays["support_x"]), "hits", int((hits["frame"] >= 0).sum()))
+
+    fact_facets = O.ref_reflector(4.889, 0.5, 2.1, 0.25, 0, 0.3, 0.01)
+    pleno_facets = O.ref_reflector(106.5, 0.0, 35.5, 3.5, 0, 0.75, 0.02)
+    hex_facets = O.ref_reflector(4.889, 0.5, 2.1, 0.25, 1, 0.3, 0.01)
+    np.savez_compressed(HERE / "reflector_facets.npz", fact=fact_facets, plenoscope=pleno_facets,
+                        fact_hex_outline=hex_facets)
+    print("facets", len(fact_facets), len(pleno_facets), len(hex_facets))
+
+    lens_cases = np.array([[1.0, 0.25, 1.49], [0.5, 0.1, 1.33], [2.0, 0.3, 1.6]])
+    R = np.array([O.ref_lens_maker(*c) for c in lens_cases])
+    meta, rows = O.ref_lightfield(106.5, 35.5, 6.5, 0.06667, 9, 1.49, 0.8, 1.1)
+    # first rows (lens array head) and summary of the whole light-field sensor
+    np.savez_compressed(HERE / "geometry.npz", lens_cases=lens_cases, lens_R=R,
+                        lightfield_lens_R=np.array([meta["lens_R"]]), lightfield_lens_f=np.array([meta["lens_f"]]),
+                        lightfield_num_lixel=np.array([meta["num_lixel"]]), lightfield_head=rows[:4000],
+                        lightfield_num_rows=np.array([len(rows)]),
+                        lightfield_pos_sum=rows[:, 3:6].sum(axis=0), lightfield_radius_sum=np.array([rows[:, -1].sum()]))
+    print("light field rows", len(rows), meta)
+
+
+if __name__ == "__main__":
+    main()
