@@ -26,6 +26,9 @@ HIP_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
+    # FP64 leaf solves follow the reference's operation order; no contraction
+    # (the FP32 traversal writes its FMAs explicitly)
+    "-ffp-contract=off",
     f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

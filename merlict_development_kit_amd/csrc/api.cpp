@@ -82,6 +82,8 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
     s->dev.root_node = hs.root_node;
     s->dev.default_refraction = hs.default_refraction;
     s->dev.eps = static_cast<float>(hs.eps);
+    s->dev.hex_sin = hs.hex_sin;
+    s->dev.hex_cos = hs.hex_cos;
     s->dev.root_r = hs.nodes[hs.root_node].r;
     s->dev.root_cx = hs.nodes[hs.root_node].cx;
     s->dev.root_cy = hs.nodes[hs.root_node].cy;

@@ -39,16 +39,18 @@ enum LeafKind : int32_t {
     LK_SPHERE = 11,
 };
 
-// Geometry of a leaf (176 B), read by every leaf test.
-//   t      frame2world translation
-//   rot    frame2world rotation, row major; world->object is its transpose
+// Geometry of a leaf (208 B), read by every leaf test.
+//   rot    frame2world rotation, row major (HomTra3 T[i][j], j < 3)
+//   t      frame2world translation (T[i][3])
+//   ti     R^T t, the constant of HomTra3::position_inverse
 //   p      kind parameters, see the kernels
 struct alignas(16) LeafGeo {
     double rot[9];
     double tx, ty, tz;
+    double ti[3];
     double p[8];
     int32_t kind;
-    int32_t pad0, pad1, pad2;
+    int32_t pad0;
 };
 
 // Physics of a leaf, read once per bounce (48 B).
@@ -93,6 +95,7 @@ struct DeviceScene {
     int32_t num_leaves;
     int32_t root_node;           // node of frame 0
     int32_t default_refraction;  // function index of merlict's DEFAULT_REFRACTION
+    double hex_sin, hex_cos;     // HexagonalPrismZ unit-vector components
     float eps;                   // sphere inflation (absolute, metres)
     float root_r;                // radius of the root node sphere
     float root_cx, root_cy, root_cz;

@@ -1,0 +1,492 @@
+// Per-photon propagation shared by the HIP kernels (propagate.hip).
+//
+// Everything here is __host__ __device__ so a host build of the same code
+// can be stepped through on a CPU; the library itself only runs it on the GPU.
+//
+// Precision split:
+//   * traversal: FP32 bounding-sphere tests (the bulk of the arithmetic),
+//     made conservative by inflating every sphere by the float rounding bound,
+//     so the candidate set always contains merlict's;
+//   * leaf solves and physics: FP64 in merlict's own operation order, with
+//     floating-point contraction disabled for this code (the library is built
+//     with -ffp-contract=off), so every decision that merlict makes from a
+//     rounding-level quantity -- e.g. which root of a cap quadratic is the
+//     self-intersection after a reflection -- is made the same way.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/mlgpu.h"
+#include "device_scene.hpp"
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define MLGPU_HD __host__ __device__
+#else
+#define MLGPU_HD
+#endif
+
+namespace mlgpu {
+
+constexpr double kC = 299792458.0;     // merlict/PhysicalConstants.h:7
+constexpr double kVoidDistance = 1e4;  // RayAndFrame.cpp:161-167
+constexpr double kSelfHit2 = 1e-14;    // Vec3.cpp:121-127
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+MLGPU_HD inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                                   uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t& o3) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+        const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = static_cast<uint32_t>(p1); c2 = n2; c3 = static_cast<uint32_t>(p0);
+    }
+    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+}
+
+// Draw k of photon `id`: word k%4 of block k/4 of counter (k/4, 0, id_lo, id_hi)
+// under key (seed_lo, seed_hi); uniform = (word >> 8) * 2^-24.
+struct Rng {
+    uint32_t k0, k1, id_lo, id_hi;
+    uint32_t b0, b1, b2, b3;
+    uint32_t next;
+    MLGPU_HD inline double uniform() {
+        if ((next & 3u) == 0u) philox4x32_10(next >> 2, 0u, id_lo, id_hi, k0, k1, b0, b1, b2, b3);
+        const uint32_t sel = next & 3u;
+        const uint32_t w = sel == 0u ? b0 : (sel == 1u ? b1 : (sel == 2u ? b2 : b3));
+        next++;
+        return static_cast<double>(w >> 8) * (1.0 / 16777216.0);
+    }
+};
+
+// ------------------------------------------------------------------- Func1
+// Func1::evaluate with Limits::assert_contains (merlict/function/Func1.cpp:66-71,
+// Limits.cpp:37-46); false when x is outside [lower, upper).
+MLGPU_HD inline bool func_eval(const DeviceScene& sc, int32_t f, double x, double& y) {
+    const FuncHead h = sc.funcs[f];
+    if (!(x >= h.lower) || !(x < h.upper)) return false;
+    if (h.flags & FUNC_DEFAULT_ABSORPTION) { y = HUGE_VAL; return true; }
+    if (h.flags & FUNC_DEFAULT_REFRACTION) { y = 1.0; return true; }
+    if (h.flags & FUNC_DEFAULT_REFLECTION) { y = 0.0; return true; }
+    int32_t lo = 0, hi = h.n;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (sc.points[h.start + mid].x > x) hi = mid; else lo = mid + 1;
+    }
+    const FuncPoint p0 = sc.points[h.start + lo - 1];
+    y = p0.slope * x + p0.y - p0.slope * p0.x;
+    return true;
+}
+
+// ------------------------------------------------------------- transforms
+struct D3 {
+    double x, y, z;
+};
+
+// Vec3::operator*(v) = v.x*x + v.y*y + v.z*z (merlict/Vec3.cpp:31-33)
+MLGPU_HD inline double dot(const D3& a, const D3& b) { return b.x * a.x + b.y * a.y + b.z * a.z; }
+MLGPU_HD inline D3 normalized(const D3& a) {
+    const double n = sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return D3{a.x / n, a.y / n, a.z / n};
+}
+// HomTra3 position / orientation / *_inverse (merlict/HomTra3.cpp:154-216)
+MLGPU_HD inline D3 orient(const LeafGeo& g, const D3& o) {
+    const double* T = g.rot;
+    return D3{o.x * T[0] + o.y * T[1] + o.z * T[2], o.x * T[3] + o.y * T[4] + o.z * T[5],
+              o.x * T[6] + o.y * T[7] + o.z * T[8]};
+}
+MLGPU_HD inline D3 orient_inv(const LeafGeo& g, const D3& o) {
+    const double* T = g.rot;
+    return D3{o.x * T[0] + o.y * T[3] + o.z * T[6], o.x * T[1] + o.y * T[4] + o.z * T[7],
+              o.x * T[2] + o.y * T[5] + o.z * T[8]};
+}
+MLGPU_HD inline D3 position(const LeafGeo& g, const D3& p) {
+    const double* T = g.rot;
+    return D3{p.x * T[0] + p.y * T[1] + p.z * T[2] + g.tx, p.x * T[3] + p.y * T[4] + p.z * T[5] + g.ty,
+              p.x * T[6] + p.y * T[7] + p.z * T[8] + g.tz};
+}
+MLGPU_HD inline D3 position_inv(const LeafGeo& g, const D3& p) {
+    const double* T = g.rot;
+    return D3{p.x * T[0] + p.y * T[3] + p.z * T[6] - g.ti[0], p.x * T[1] + p.y * T[4] + p.z * T[7] - g.ti[1],
+              p.x * T[2] + p.y * T[5] + p.z * T[8] - g.ti[2]};
+}
+
+// ----------------------------------------------------------------- leaves
+struct LeafHit {
+    double t;   // distance from the ray support
+    D3 p;       // object-frame position
+    D3 d;       // object-frame ray direction
+};
+
+// HexagonalPrismZ::is_inside (merlict/scenery/geometry/HexagonalPrismZ.cpp:14-29),
+// unit vectors U = (0, 1, 0), V = (s, c, 0), W = (-s, c, 0)
+MLGPU_HD inline bool hex_inside(const DeviceScene& sc, double inner, const D3& p) {
+    const double u = p.x * 0.0 + p.y * 1.0 + p.z * 0.0;
+    const double v = p.x * sc.hex_sin + p.y * sc.hex_cos + p.z * 0.0;
+    const double w = p.x * -sc.hex_sin + p.y * sc.hex_cos + p.z * 0.0;
+    return u < inner && u > -inner && v < inner && v > -inner && w < inner && w > -inner;
+}
+
+MLGPU_HD inline bool cap_bound_inside(const DeviceScene& sc, const LeafGeo& g, const D3& p) {
+    if (g.kind == MLGPU_KIND_SPHERE_CAP_HEX) return hex_inside(sc, g.p[1], p);
+    if (g.kind == MLGPU_KIND_SPHERE_CAP_CYL) return p.x * p.x + p.y * p.y < g.p[1];
+    return fabs(p.x) < g.p[1] && fabs(p.y) < g.p[2];
+}
+
+MLGPU_HD inline D3 at(const D3& s, const D3& d, double t) { return D3{s.x + d.x * t, s.y + d.y * t, s.z + d.z * t}; }
+
+// Vec3 != : squared distance > 1e-14 (the drop of self hits)
+MLGPU_HD inline bool differs(const D3& a, const D3& b) {
+    const D3 e{a.x - b.x, a.y - b.y, a.z - b.z};
+    return dot(e, e) > kSelfHit2;
+}
+
+// calculate_intersection_with of one leaf for a world ray (s, d), in the
+// leaf's object frame (RayAndFrame.cpp:109-116,151-158).
+MLGPU_HD inline bool intersect_leaf(const DeviceScene& sc, const LeafGeo* __restrict__ gp, const D3& ws,
+                                    const D3& wd, LeafHit& h) {
+    const LeafGeo& g = *gp;
+    const D3 s = position_inv(g, ws);
+    const D3 d = orient_inv(g, wd);
+    const int32_t kind = g.kind;
+    double t;
+    if (kind <= MLGPU_KIND_SPHERE_CAP_RECT) {
+        // SphericalCapRayIntersectionEquation.cpp:6-40 and
+        // SurfaceWithOuterPrismBound::add_causeal_intersection (…Bound.cpp:7-70)
+        const double R = g.p[0];
+        const double d2 = dot(d, d), ss = dot(s, s), sd = dot(s, d);
+        const double P = 2.0 * (sd - R * d.z) / d2;
+        const double Q = (ss - 2.0 * (R * s.z)) / d2;
+        const double p2 = 0.5 * P;  // QuadraticEquation.cpp:12-30
+        const double inner = p2 * p2 - Q;
+        if (!(inner >= 0.0)) return false;
+        const double sq = sqrt(inner);
+        const double minus = -p2 - sq, plus = -p2 + sq;
+        const D3 pi = at(s, d, plus), mi = at(s, d, minus);
+        const bool p_in = cap_bound_inside(sc, g, pi), m_in = cap_bound_inside(sc, g, mi);
+        const double rb = g.p[7];
+        const bool p_c = plus > 0.0 && dot(pi, pi) <= rb * rb;
+        const bool m_c = minus > 0.0 && dot(mi, mi) <= rb * rb;
+        bool ok = false;
+        t = minus;
+        if (p_in && m_in) {
+            if (p_c && m_c) { t = (minus >= plus) ? plus : minus; ok = true; }
+            else if (p_c) { t = plus; ok = true; }
+            else if (m_c) { t = minus; ok = true; }
+        } else if (p_in && p_c) { t = plus; ok = true; }
+        else if (m_in && m_c) { t = minus; ok = true; }
+        if (!ok) return false;
+        const D3 p = at(s, d, t);
+        if (!differs(s, p)) return false;
+        h.p = p;
+    } else if (kind == MLGPU_KIND_CYLINDER) {
+        // ZaxisCylinderRayIntersectionEquation.cpp:6-57, Cylinder.cpp:121-150
+        const double dx2 = d.x * d.x, dy2 = d.y * d.y;
+        const double sxdx = s.x * d.x, sydy = s.y * d.y;
+        const double sx2 = s.x * s.x, sy2 = s.y * s.y;
+        const double r2 = g.p[0] * g.p[0];
+        const double dnom = dx2 + dy2;
+        const double P = 2.0 * (sxdx + sydy) / dnom;
+        const double Q = (sx2 + sy2 - r2) / dnom;
+        const double p2 = 0.5 * P;
+        const double inner = p2 * p2 - Q;
+        if (!(inner >= 0.0)) return false;
+        const double sq = sqrt(inner);
+        const double plus = -p2 + sq, minus = -p2 - sq;
+        bool causal = false;
+        t = 0.0;
+        if (minus < 0.0 && plus >= 0.0) { causal = true; t = plus; }
+        if (plus >= 0.0 && minus >= 0.0) { causal = true; t = minus; }
+        if (!causal) return false;
+        const D3 p = at(s, d, t);
+        if (!(fabs(p.z) <= g.p[1])) return false;
+        if (!differs(s, p)) return false;
+        h.p = p;
+    } else if (kind == MLGPU_KIND_SPHERE) {
+        // Sphere.cpp:62-105
+        const double sd = dot(s, d), d2 = dot(d, d), ss = dot(s, s);
+        const double r2 = g.p[0] * g.p[0];
+        const double P = 2 * (sd / d2), Q = ss / d2 - r2;
+        const double p2 = 0.5 * P;
+        const double inner = p2 * p2 - Q;
+        if (!(inner >= 0.0)) return false;
+        const double sq = sqrt(inner);
+        const double vP = -p2 + sq, vM = -p2 - sq;
+        if (vP >= 0.0 && vM >= 0.0) t = vM;
+        else if (!(vP < 0.0 && vM < 0.0)) t = vP;
+        else return false;
+        const D3 p = at(s, d, t);
+        if (!differs(s, p)) return false;
+        h.p = p;
+    } else {
+        // XyPlaneRayIntersectionEquation.cpp:6-41 and the planar primitives
+        if (d.z == 0. && (d.x != 0. || d.y != 0.)) return false;
+        t = -s.z / d.z;
+        if (!(t > 0.0)) return false;
+        const D3 p = at(s, d, t);
+        bool in;
+        switch (kind) {
+        case MLGPU_KIND_PLANE: in = fabs(p.x) < g.p[1] && fabs(p.y) < g.p[2]; break;
+        case MLGPU_KIND_DISC: in = p.x * p.x + p.y * p.y < g.p[1]; break;
+        case MLGPU_KIND_HEX_PLANE: in = hex_inside(sc, g.p[1], p); break;
+        case MLGPU_KIND_ANNULUS:
+            in = (p.x * p.x + p.y * p.y < g.p[1]) && !(p.x * p.x + p.y * p.y < g.p[2]);
+            break;
+        case MLGPU_KIND_PLANE_DUAL_SPHERICAL: {
+            // DualSphericalPrismZ::is_inside (DualSphericalPrismZ.cpp:21-27)
+            const double ax = g.p[1] - p.x, ay = 0.0 - p.y;
+            const double bx = -g.p[1] - p.x, by = 0.0 - p.y;
+            in = (ax * ax + ay * ay <= g.p[2]) && (bx * bx + by * by <= g.p[2]);
+            break;
+        }
+        case MLGPU_KIND_TRIANGLE: {
+            // Triangle::is_inside_triangle (Triangle.cpp:92-102)
+            const double Ax = g.p[0], Ay = g.p[1], Bx = g.p[2], By = g.p[3], Cx = g.p[4], Cy = g.p[5];
+            const bool bA = ((p.x - Bx) * (Ay - By) - (Ax - Bx) * (p.y - By)) < 0.0;
+            const bool bB = ((p.x - Cx) * (By - Cy) - (Bx - Cx) * (p.y - Cy)) < 0.0;
+            const bool bC = ((p.x - Ax) * (Cy - Ay) - (Cx - Ax) * (p.y - Ay)) < 0.0;
+            in = (bA == bB) && (bB == bC);
+            break;
+        }
+        default: in = false;
+        }
+        if (!in) return false;
+        if (!differs(s, p)) return false;
+        h.p = p;
+    }
+    h.t = t;
+    h.d = d;
+    return true;
+}
+
+MLGPU_HD inline D3 leaf_normal(const LeafGeo& g, const D3& p) {
+    const int32_t kind = g.kind;
+    if (kind <= MLGPU_KIND_SPHERE_CAP_RECT) {
+        // SphericalCapRayIntersectionEquation.cpp:52-59
+        return normalized(D3{0.0 - p.x, 0.0 - p.y, g.p[0] - p.z});
+    }
+    if (kind == MLGPU_KIND_CYLINDER) return normalized(D3{p.x, p.y, 0.0});  // Cylinder.cpp:157-160
+    if (kind == MLGPU_KIND_SPHERE) return normalized(p);                    // Sphere.cpp:80-81
+    return D3{0.0, 0.0, 1.0};
+}
+
+// --------------------------------------------------------------- traversal
+struct Closest {
+    int32_t leaf;
+    LeafHit hit;
+};
+
+// First intersection below scope node `scope` (rays_first_intersection_with_frame,
+// RayAndFrame.cpp:118-173).
+MLGPU_HD inline void first_hit(const DeviceScene& sc, int32_t scope, const D3& s, const D3& d, Closest& best) {
+    best.leaf = -1;
+    best.hit.t = HUGE_VAL;
+    const float ox = static_cast<float>(s.x), oy = static_cast<float>(s.y), oz = static_cast<float>(s.z);
+    const float fdx = static_cast<float>(d.x), fdy = static_cast<float>(d.y), fdz = static_cast<float>(d.z);
+    float best_f = HUGE_VALF;
+    const int32_t end = sc.nodes[scope].skip;
+    int32_t i = scope;
+    while (i < end) {
+        const Node nd = sc.nodes[i];
+        const float wx = nd.cx - ox, wy = nd.cy - oy, wz = nd.cz - oz;
+        const float alpha = fmaf(wx, fdx, fmaf(wy, fdy, wz * fdz));
+        const float qx = fmaf(-alpha, fdx, wx), qy = fmaf(-alpha, fdy, wy), qz = fmaf(-alpha, fdz, wz);
+        const float dist2 = fmaf(qx, qx, fmaf(qy, qy, qz * qz));
+        const float r2 = nd.r * nd.r;
+        // ray_intersects_frames_bounding_sphere (RayAndFrame.cpp:16-107)
+        bool hit = dist2 <= r2;
+        if (hit && alpha < 0.0f) hit = fmaf(wx, wx, fmaf(wy, wy, wz * wz)) <= r2;
+        // prune spheres entered only behind the best hit
+        if (hit && alpha > best_f) {
+            const float e = alpha - best_f;
+            hit = e * e <= r2 - dist2;
+        }
+        if (!hit) { i = nd.skip; continue; }
+        if (nd.leaf >= 0) {
+            LeafHit h;
+            if (intersect_leaf(sc, sc.geo + nd.leaf, s, d, h) && h.t < best.hit.t) {
+                best.leaf = nd.leaf;
+                best.hit = h;
+                best_f = static_cast<float>(h.t) * (1.0f + 1e-6f) + sc.eps;
+            }
+        }
+        i++;
+    }
+}
+
+// Propagator (merlict/PhotonAndFrame.cpp:6-124) for photon i of the batch.
+MLGPU_HD inline void propagate_photon(const DeviceScene& sc, const int32_t world_node, const mlgpu_photons& in,
+                                      const int64_t i, const uint32_t max_interactions, const uint64_t seed,
+                                      const uint64_t id_offset, const mlgpu_results& out) {
+    D3 s{in.support_x[i], in.support_y[i], in.support_z[i]};
+    D3 d = normalized(D3{in.direction_x[i], in.direction_y[i], in.direction_z[i]});  // Ray.cpp:16-20
+    const double lambda = in.wavelength[i];
+    const uint64_t id = id_offset + static_cast<uint64_t>(i);
+    Rng rng;
+    rng.k0 = static_cast<uint32_t>(seed);
+    rng.k1 = static_cast<uint32_t>(seed >> 32);
+    rng.id_lo = static_cast<uint32_t>(id);
+    rng.id_hi = static_cast<uint32_t>(id >> 32);
+    rng.b0 = rng.b1 = rng.b2 = rng.b3 = 0u;
+    rng.next = 0u;
+
+    uint8_t err = lambda > 0.0 ? MLGPU_PHOTON_OK : MLGPU_PHOTON_BAD_WAVELENGTH;  // Photon.cpp:20-28
+    int32_t root = world_node;
+    uint32_t n_hist = 1;  // production
+    uint8_t last_interaction = MLGPU_PRODUCTION;
+    int32_t last_frame = -1, last_sensor = -1;
+    double tof = 0.0;
+    double out_x = s.x, out_y = s.y, out_cx = 0.0, out_cy = 0.0;
+
+    while (err == MLGPU_PHOTON_OK && n_hist <= max_interactions) {
+        Closest c;
+        first_hit(sc, root, s, d, c);
+        if (c.leaf < 0) {
+            // no hit: absorbed in void at 1e4 m, n = 1 on that leg
+            double nv;
+            if (!func_eval(sc, sc.default_refraction, lambda, nv)) { err = MLGPU_PHOTON_OUT_OF_TABLE; break; }
+            tof += nv * kVoidDistance / kC;
+            n_hist++;
+            last_interaction = MLGPU_ABSORPTION_IN_VOID;
+            last_frame = -1;
+            last_sensor = -1;
+            const D3 e = at(s, d, kVoidDistance);
+            out_x = e.x;
+            out_y = e.y;
+            const D3 inc = normalized(D3{e.x - s.x, e.y - s.y, e.z - s.z});
+            out_cx = -1.0 * inc.x;
+            out_cy = -1.0 * inc.y;
+            break;
+        }
+        const LeafGeo& g = sc.geo[c.leaf];
+        const LeafPhys ph = sc.phys[c.leaf];
+        const D3 n = leaf_normal(g, c.hit.p);
+        // Intersection::ray_runs_from_outside_to_inside (Intersection.cpp:133-139)
+        const bool outside_in = dot(n, c.hit.d) < 0.0;
+        const D3 hit_world = position(g, c.hit.p);
+
+        uint8_t interaction;
+        bool stop = true;
+        D3 nd = d;
+        int32_t new_root = root;
+        // absorbed_in_medium_before_reaching_surface (PhotonAndFrame.cpp:32-38)
+        double ell;
+        if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_ABSORPTION : MLGPU_INNER_ABSORPTION], lambda, ell)) {
+            err = MLGPU_PHOTON_OUT_OF_TABLE;
+            break;
+        }
+        const double survival = exp(-c.hit.t / ell);
+        if (rng.uniform() > survival) {
+            interaction = MLGPU_ABSORPTION_IN_VOID;
+        } else {
+            // interact_with_object (PhotonAndFrame.cpp:40-48)
+            double refl;
+            if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFLECTION : MLGPU_INNER_REFLECTION], lambda, refl)) {
+                err = MLGPU_PHOTON_OUT_OF_TABLE;
+                break;
+            }
+            bool mirror = false;
+            if (refl >= rng.uniform()) {
+                interaction = MLGPU_REFLECTION_ON_SURFACE;
+                mirror = true;
+            } else {
+                const bool transparent = ph.surf[MLGPU_OUTER_REFRACTION] != sc.default_refraction ||
+                                         ph.surf[MLGPU_INNER_REFRACTION] != sc.default_refraction;
+                if (!transparent) {
+                    interaction = MLGPU_ABSORPTION_ON_SURFACE;
+                } else {
+                    // fresnel_refraction_and_reflection (PhotonAndFrame.cpp:75-87), Fresnel.cpp:7-38
+                    double n_from, n_to;
+                    const int32_t f_from = ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION];
+                    const int32_t f_to = ph.surf[outside_in ? MLGPU_INNER_REFRACTION : MLGPU_OUTER_REFRACTION];
+                    if (!func_eval(sc, f_from, lambda, n_from) || !func_eval(sc, f_to, lambda, n_to)) {
+                        err = MLGPU_PHOTON_OUT_OF_TABLE;
+                        break;
+                    }
+                    const D3& inc = c.hit.d;
+                    const D3 nrm = outside_in ? n : D3{n.x * -1.0, n.y * -1.0, n.z * -1.0};
+                    const double cosI = dot(nrm, inc) * (-1.0);
+                    const double eta = n_from / n_to;
+                    const double sinT2 = eta * eta * (1.0 - cosI * cosI);
+                    const double cosT = sqrt(1.0 - sinT2);
+                    double rf;
+                    if (sinT2 > 1.0) {
+                        rf = 1.0;
+                    } else {
+                        const double ro = (n_from * cosI - n_to * cosT) / (n_from * cosI + n_to * cosT);
+                        const double rp = (n_to * cosI - n_from * cosT) / (n_to * cosI + n_from * cosT);
+                        rf = (ro * ro + rp * rp) / 2.0;
+                    }
+                    if (rf > rng.uniform()) {
+                        interaction = MLGPU_FRESNEL_REFLECTION_ON_SURFACE;
+                        mirror = true;
+                    } else {
+                        // pass_the_boundary_layer / propagate_on_after_boundary_layer (:89-118)
+                        interaction = outside_in ? MLGPU_REFRACTION_TO_INSIDE : MLGPU_REFRACTION_TO_OUTSIDE;
+                        const bool to_default = (outside_in ? ph.surf[MLGPU_INNER_REFRACTION]
+                                                            : ph.surf[MLGPU_OUTER_REFRACTION]) == sc.default_refraction;
+                        new_root = (ph.allowed_node >= 0 && !to_default) ? ph.allowed_node : ph.root_node;
+                        const double k = eta * cosI - cosT;
+                        const D3 refr{inc.x * eta + nrm.x * k, inc.y * eta + nrm.y * k, inc.z * eta + nrm.z * k};
+                        nd = normalized(orient(g, refr));
+                        stop = false;
+                    }
+                }
+            }
+            if (mirror) {
+                // Intersection::reflection_direction_in_root_frame with the mirror
+                // matrix I - 2 n n^T (Intersection.cpp:74-79, Vec3.cpp:59-113)
+                const D3& r = c.hit.d;
+                const double x = n.x, y = n.y, z = n.z;
+                const D3 m{(1. - 2. * x * x) * r.x + -2. * x * y * r.y + -2. * x * z * r.z,
+                           -2. * x * y * r.x + (1. - 2. * y * y) * r.y + -2. * y * z * r.z,
+                           -2. * x * z * r.x + -2. * y * z * r.y + (1. - 2. * z * z) * r.z};
+                nd = normalized(orient(g, m));
+                stop = false;
+            }
+        }
+        // push_back_intersection_and_interaction; Photon::time_of_flight (Photon.cpp:44-56)
+        {
+            double n_from;
+            if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION], lambda,
+                           n_from)) {
+                err = MLGPU_PHOTON_OUT_OF_TABLE;
+                break;
+            }
+            tof += n_from * c.hit.t / kC;
+        }
+        n_hist++;
+        last_interaction = interaction;
+        last_frame = ph.frame;
+        last_sensor = ph.sensor;
+        out_x = c.hit.p.x;
+        out_y = c.hit.p.y;
+        {
+            // final_intersection_incident_direction_wrt_frame (RayForPropagation.cpp:114-136)
+            const D3 inc = normalized(D3{hit_world.x - s.x, hit_world.y - s.y, hit_world.z - s.z});
+            const D3 loc = orient_inv(g, inc);
+            out_cx = -1.0 * loc.x;
+            out_cy = -1.0 * loc.y;
+        }
+        if (stop) break;
+        s = hit_world;
+        d = nd;
+        root = new_root;
+    }
+    out.final_frame[i] = last_frame;
+    out.sensor_id[i] = last_sensor;
+    out.final_interaction[i] = last_interaction;
+    out.num_interactions[i] = static_cast<uint8_t>(n_hist);
+    out.error[i] = err;
+    out.time_of_flight[i] = tof;
+    out.x[i] = static_cast<float>(out_x);
+    out.y[i] = static_cast<float>(out_y);
+    out.cx[i] = static_cast<float>(out_cx);
+    out.cy[i] = static_cast<float>(out_cy);
+}
+
+}  // namespace mlgpu

@@ -78,6 +78,159 @@ M34 pose(const double* pos, int mode, const double* rot) {
     return compose(Tr, R);
 }
 
+// frame2world of every description frame exactly as merlict computes it.
+// Frame::init_tree_based_on_mother_child_relations (merlict/Frame.cpp:125-147)
+// first moves the children of any frame with more than 16 children into
+// octant frames centred at their bounding centre (Frame::cluster_children,
+// Frame.cpp:190-255; bound::bounding_sphere_center, Frames.cpp:39-124); the
+// leaves' frame2world products then run through those octants.  The kernels
+// solve each leaf in its object frame with these transforms, so the poses
+// are reproduced bit for bit (only the transforms; the GPU traversal uses its
+// own sphere tree).
+class ReferencePoses {
+ public:
+    ReferencePoses(const mlgpu_scene_desc& d, const std::vector<double>& radius) : d_(d) {
+        const int32_t n = d.n_frames;
+        f_.resize(n);
+        for (int32_t i = 0; i < n; i++) {
+            F& f = f_[i];
+            f.mother = d.mother[i];
+            for (int k = 0; k < 3; k++) f.pos[k] = d.pos[3 * i + k];
+            f.mode = d.rot_mode[i];
+            for (int k = 0; k < 4; k++) f.rot[k] = d.rot[4 * i + k];
+            f.t2m = pose(f.pos, f.mode, f.rot);
+            f.radius = radius[i];
+            if (f.mother >= 0) f_[f.mother].children.push_back(i);
+        }
+        cluster(0);
+        t2w(0);
+    }
+    const M34& frame2world(int32_t i) const { return f_[i].t2w; }
+
+ private:
+    struct F {
+        int32_t mother;
+        double pos[3];
+        int mode;
+        double rot[4];
+        M34 t2m, t2w;
+        double radius;
+        std::vector<int32_t> children;
+    };
+    const mlgpu_scene_desc& d_;
+    std::vector<F> f_;
+
+    static double norm3(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+    void mean(const std::vector<int32_t>& idx, double* out) const {
+        double s[3] = {0., 0., 0.};
+        for (int32_t i : idx)
+            for (int k = 0; k < 3; k++) s[k] = s[k] + f_[i].pos[k];
+        for (int k = 0; k < 3; k++) out[k] = s[k] / static_cast<double>(idx.size());
+    }
+    double radius_about(const std::vector<int32_t>& idx, const double* c) const {
+        double r = 0.0;
+        for (int32_t i : idx) {
+            const double d[3] = {c[0] - f_[i].pos[0], c[1] - f_[i].pos[1], c[2] - f_[i].pos[2]};
+            const double need = norm3(d) + f_[i].radius;
+            if (need > r) r = need;
+        }
+        return r;
+    }
+    void alternative(const std::vector<int32_t>& idx, double* out) const {
+        if (idx.size() == 1) { for (int k = 0; k < 3; k++) out[k] = f_[idx[0]].pos[k]; return; }
+        int32_t a = -1, b = -1;
+        double maximum = 0.0;
+        for (size_t i = 0; i < idx.size(); i++)
+            for (size_t j = i; j < idx.size(); j++) {
+                if (i == j) continue;
+                const F& p = f_[idx[i]];
+                const F& q = f_[idx[j]];
+                const double d[3] = {p.pos[0] - q.pos[0], p.pos[1] - q.pos[1], p.pos[2] - q.pos[2]};
+                const double dist = norm3(d) + p.radius + q.radius;
+                if (dist > maximum) { maximum = dist; a = idx[i]; b = idx[j]; }
+            }
+        const double zero[3] = {0., 0., 0.};
+        const double* ap = a >= 0 ? f_[a].pos : zero;
+        const double* bp = b >= 0 ? f_[b].pos : zero;
+        const double ar = a >= 0 ? f_[a].radius : 0.0;
+        double dir[3] = {bp[0] - ap[0], bp[1] - ap[1], bp[2] - ap[2]};
+        const double nn = norm3(dir);
+        for (int k = 0; k < 3; k++) dir[k] = dir[k] / nn;
+        for (int k = 0; k < 3; k++) out[k] = ap[k] + dir[k] * (0.5 * maximum - ar);
+    }
+    void centre(const std::vector<int32_t>& idx, double* out) const {
+        mean(idx, out);
+        if (idx.size() < 16) {
+            double alt[3];
+            alternative(idx, alt);
+            if (radius_about(idx, out) > radius_about(idx, alt))
+                for (int k = 0; k < 3; k++) out[k] = alt[k];
+        }
+    }
+    bool too_close(const std::vector<int32_t>& idx) const {
+        if (idx.size() < 2) return false;
+        double m[3];
+        mean(idx, m);
+        double u[3] = {0., 0., 0.};
+        for (int32_t i : idx) {
+            const double r[3] = {f_[i].pos[0] - m[0], f_[i].pos[1] - m[1], f_[i].pos[2] - m[2]};
+            u[0] = u[0] + r[0] * r[0];
+            u[1] = u[1] + r[1] * r[1];
+            u[2] = u[2] + r[2] * r[2];
+        }
+        for (int k = 0; k < 3; k++) u[k] = u[k] / static_cast<double>(idx.size());
+        return std::sqrt(norm3(u)) < 1e-6;
+    }
+    void cluster(int32_t fi) {
+        if (f_[fi].children.size() > 16) {
+            const std::vector<int32_t> old = f_[fi].children;
+            std::vector<int32_t> oct[8];
+            for (int32_t c : old) {
+                const double* p = f_[c].pos;
+                oct[4 * (p[0] >= 0.) + 2 * (p[1] >= 0.) + (p[2] >= 0.)].push_back(c);
+            }
+            f_[fi].children.clear();
+            for (int k = 0; k < 8; k++) {
+                if (too_close(oct[k])) {
+                    for (int32_t c : oct[k]) { f_[c].mother = fi; f_[fi].children.push_back(c); }
+                } else if (oct[k].size() == 1) {
+                    f_[oct[k][0]].mother = fi;
+                    f_[fi].children.push_back(oct[k][0]);
+                } else if (oct[k].size() > 1) {
+                    double c[3];
+                    centre(oct[k], c);
+                    F o;
+                    o.mother = fi;
+                    for (int q = 0; q < 3; q++) o.pos[q] = c[q];
+                    o.mode = MLGPU_ROT_XYZ;
+                    for (int q = 0; q < 4; q++) o.rot[q] = 0.0;
+                    o.t2m = pose(o.pos, o.mode, o.rot);
+                    o.radius = 0.0;
+                    const int32_t oi = static_cast<int32_t>(f_.size());
+                    f_.push_back(o);
+                    f_[fi].children.push_back(oi);
+                    for (int32_t ch : oct[k]) {
+                        F& cf = f_[ch];
+                        for (int q = 0; q < 3; q++) cf.pos[q] = cf.pos[q] - c[q];
+                        cf.t2m = pose(cf.pos, cf.mode, cf.rot);
+                        cf.mother = oi;
+                        f_[oi].children.push_back(ch);
+                    }
+                }
+            }
+        }
+        const std::vector<int32_t> kids = f_[fi].children;
+        for (int32_t c : kids) cluster(c);
+    }
+    void t2w(int32_t fi) {
+        F& f = f_[fi];
+        f.t2w = f.mother >= 0 ? compose(f_[f.mother].t2w, f.t2m) : f.t2m;
+        const std::vector<int32_t> kids = f_[fi].children;
+        for (int32_t c : kids) t2w(c);
+    }
+};
+
 // Bounding radius each primitive reports about its own origin and the
 // kernel parameters of the leaf (primitive post_initialize_* methods).
 double leaf_params(int kind, const double* p, double* out) {
@@ -168,10 +321,15 @@ class Builder {
         const int32_t n = d_.n_frames;
         t2w_.resize(n);
         children_.assign(n, {});
+        std::vector<double> radius(n);
         for (int32_t i = 0; i < n; i++) {
-            const M34 t2m = pose(d_.pos + 3 * i, d_.rot_mode[i], d_.rot + 4 * i);
-            t2w_[i] = d_.mother[i] >= 0 ? compose(t2w_[d_.mother[i]], t2m) : t2m;
+            double prm[8];
+            radius[i] = leaf_params(d_.kind[i], d_.params + MLGPU_NUM_PARAMS * i, prm);
             if (d_.mother[i] >= 0) children_[d_.mother[i]].push_back(i);
+        }
+        {
+            const ReferencePoses rp(d_, radius);
+            for (int32_t i = 0; i < n; i++) t2w_[i] = rp.frame2world(i);
         }
         scope_.assign(n, 0);
         scope_[0] = 1;
@@ -303,9 +461,13 @@ class Builder {
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) g.rot[3 * i + j] = m.T[i][j];
         g.tx = m.T[0][3]; g.ty = m.T[1][3]; g.tz = m.T[2][3];
+        // the constant part of HomTra3::position_inverse (merlict/HomTra3.cpp:202-216)
+        g.ti[0] = m.T[0][0] * m.T[0][3] + m.T[1][0] * m.T[1][3] + m.T[2][0] * m.T[2][3];
+        g.ti[1] = m.T[0][1] * m.T[0][3] + m.T[1][1] * m.T[1][3] + m.T[2][1] * m.T[2][3];
+        g.ti[2] = m.T[0][2] * m.T[0][3] + m.T[1][2] * m.T[1][3] + m.T[2][2] * m.T[2][3];
         leaf_params(d_.kind[f], d_.params + MLGPU_NUM_PARAMS * f, g.p);
         g.kind = d_.kind[f];
-        g.pad0 = g.pad1 = g.pad2 = 0;
+        g.pad0 = 0;
         LeafPhys ph;
         for (int s = 0; s < MLGPU_NUM_SURFACE_SLOTS; s++) {
             int32_t fn = d_.surface[MLGPU_NUM_SURFACE_SLOTS * f + s];
@@ -373,6 +535,9 @@ class Builder {
         // absorption evaluates to NaN and never absorbs; flagged explicitly.
         add_function(x, inf, 2, FUNC_DEFAULT_ABSORPTION);
         hs_->default_refraction = def_refr_;
+        // HexagonalPrismZ unit vectors V, W = (+-sin(2 pi/3), cos(2 pi/3)) (HexagonalPrismZ.h:10-16)
+        hs_->hex_sin = std::sin(2. / 3. * M_PI);
+        hs_->hex_cos = std::cos(2. / 3. * M_PI);
     }
 
     // Inflate float spheres so the FP32 test stays conservative: centre

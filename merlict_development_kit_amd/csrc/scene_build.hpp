@@ -24,6 +24,7 @@ struct HostScene {
     int32_t root_node = -1;
     int32_t default_refraction = -1;
     double eps = 0.0;
+    double hex_sin = 0.0, hex_cos = 0.0;
 };
 
 // Flatten a description (throws std::invalid_argument on bad input).
