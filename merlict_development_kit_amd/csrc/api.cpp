@@ -84,10 +84,6 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
     s->dev.eps = static_cast<float>(hs.eps);
     s->dev.hex_sin = hs.hex_sin;
     s->dev.hex_cos = hs.hex_cos;
-    s->dev.root_r = hs.nodes[hs.root_node].r;
-    s->dev.root_cx = hs.nodes[hs.root_node].cx;
-    s->dev.root_cy = hs.nodes[hs.root_node].cy;
-    s->dev.root_cz = hs.nodes[hs.root_node].cz;
     s->scope_node = hs.scope_node;
     s->num_nodes = static_cast<int64_t>(hs.nodes.size());
     s->num_leaves = static_cast<int64_t>(hs.geo.size());

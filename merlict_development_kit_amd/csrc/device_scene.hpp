@@ -1,28 +1,29 @@
 // Device layout of a flattened scene (shared by the host builder and the
 // HIP kernels).
 //
-// Traversal hierarchy: a bounding-sphere tree stored in depth-first order
+// Traversal hierarchy: a bounding-volume tree stored in depth-first order
 // with skip indices, so a subtree is the contiguous range [i, skip[i]) and
 // traversal needs no stack.  Every frame that can be a propagation root (the
 // world and each restriction frame, merlict/SurfaceEntity.cpp:59-65) owns a
 // node whose range holds exactly its leaves; below such a frame the library
-// builds its own binary sphere tree over the leaves instead of merlict's
-// octant clustering (merlict/Frame.cpp:190-255).  Any conservative hierarchy
-// yields the same closest hit, because merlict keeps the first minimum over
-// all candidates (RayAndFrame.cpp:160-173).
+// builds its own binary tree of axis-aligned boxes over the leaves (surface-
+// area heuristic) instead of merlict's bounding spheres and octant clustering
+// (merlict/Frame.cpp:190-255, RayAndFrame.cpp:16-107).  Any conservative
+// hierarchy yields the same closest hit, because merlict keeps the first
+// minimum over all candidates (RayAndFrame.cpp:160-173).
 #pragma once
 
 #include <cstdint>
 
 namespace mlgpu {
 
-// 32 B per node: sphere (centre in world frame, radius inflated by the
-// float rounding bound) and (skip, leaf index or -1).
+// 32 B per node: world-frame box (inflated by the float rounding bound)
+// and (skip, leaf index or -1).
 struct alignas(16) Node {
-    float cx, cy, cz, r;
+    float lo[3];
     int32_t skip;
+    float hi[3];
     int32_t leaf;
-    int32_t pad0, pad1;
 };
 
 enum LeafKind : int32_t {
@@ -96,9 +97,7 @@ struct DeviceScene {
     int32_t root_node;           // node of frame 0
     int32_t default_refraction;  // function index of merlict's DEFAULT_REFRACTION
     double hex_sin, hex_cos;     // HexagonalPrismZ unit-vector components
-    float eps;                   // sphere inflation (absolute, metres)
-    float root_r;                // radius of the root node sphere
-    float root_cx, root_cy, root_cz;
+    float eps;                   // box inflation (absolute, metres)
 };
 
 }  // namespace mlgpu

@@ -4,9 +4,9 @@
 // can be stepped through on a CPU; the library itself only runs it on the GPU.
 //
 // Precision split:
-//   * traversal: FP32 bounding-sphere tests (the bulk of the arithmetic),
-//     made conservative by inflating every sphere by the float rounding bound,
-//     so the candidate set always contains merlict's;
+//   * traversal: FP32 slab tests against axis-aligned boxes (the bulk of the
+//     arithmetic), made conservative by inflating every box by the float
+//     rounding bound, so the candidate set always contains merlict's;
 //   * leaf solves and physics: FP64 in merlict's own operation order, with
 //     floating-point contraction disabled for this code (the library is built
 //     with -ffp-contract=off), so every decision that merlict makes from a
@@ -281,37 +281,50 @@ struct Closest {
 };
 
 // First intersection below scope node `scope` (rays_first_intersection_with_frame,
-// RayAndFrame.cpp:118-173).
+// RayAndFrame.cpp:118-173).  The boxes only cull: every leaf whose box the
+// ray enters before the best hit so far is solved in FP64, so the closest
+// hit is merlict's.
 MLGPU_HD inline void first_hit(const DeviceScene& sc, int32_t scope, const D3& s, const D3& d, Closest& best) {
     best.leaf = -1;
     best.hit.t = HUGE_VAL;
-    const float ox = static_cast<float>(s.x), oy = static_cast<float>(s.y), oz = static_cast<float>(s.z);
+    const Node root = sc.nodes[scope];
+    const int32_t end = root.skip;
+    // direction reciprocals; a zero component becomes tiny so no 0 * inf occurs
     const float fdx = static_cast<float>(d.x), fdy = static_cast<float>(d.y), fdz = static_cast<float>(d.z);
-    float best_f = HUGE_VALF;
-    const int32_t end = sc.nodes[scope].skip;
+    const float idx = 1.0f / (fabsf(fdx) > 1e-30f ? fdx : copysignf(1e-30f, fdx));
+    const float idy = 1.0f / (fabsf(fdy) > 1e-30f ? fdy : copysignf(1e-30f, fdy));
+    const float idz = 1.0f / (fabsf(fdz) > 1e-30f ? fdz : copysignf(1e-30f, fdz));
+    // a support far outside the scope box is advanced (in FP64) to the box so
+    // the float origin keeps the precision the box inflation assumes
+    double t_adv = 0.0;
+    {
+        const double ax = (static_cast<double>(root.lo[0]) - s.x) * idx, bx = (static_cast<double>(root.hi[0]) - s.x) * idx;
+        const double ay = (static_cast<double>(root.lo[1]) - s.y) * idy, by = (static_cast<double>(root.hi[1]) - s.y) * idy;
+        const double az = (static_cast<double>(root.lo[2]) - s.z) * idz, bz = (static_cast<double>(root.hi[2]) - s.z) * idz;
+        const double enter = fmax(fmax(fmin(ax, bx), fmin(ay, by)), fmin(az, bz));
+        if (enter > 0.0) t_adv = enter * (1.0 - 1e-6) - sc.eps;
+        if (t_adv < 0.0) t_adv = 0.0;
+    }
+    const float ox = static_cast<float>(s.x + d.x * t_adv);
+    const float oy = static_cast<float>(s.y + d.y * t_adv);
+    const float oz = static_cast<float>(s.z + d.z * t_adv);
+    const float oix = ox * idx, oiy = oy * idy, oiz = oz * idz;
+    float best_f = HUGE_VALF;  // best distance relative to the float origin, plus margin
     int32_t i = scope;
     while (i < end) {
         const Node nd = sc.nodes[i];
-        const float wx = nd.cx - ox, wy = nd.cy - oy, wz = nd.cz - oz;
-        const float alpha = fmaf(wx, fdx, fmaf(wy, fdy, wz * fdz));
-        const float qx = fmaf(-alpha, fdx, wx), qy = fmaf(-alpha, fdy, wy), qz = fmaf(-alpha, fdz, wz);
-        const float dist2 = fmaf(qx, qx, fmaf(qy, qy, qz * qz));
-        const float r2 = nd.r * nd.r;
-        // ray_intersects_frames_bounding_sphere (RayAndFrame.cpp:16-107)
-        bool hit = dist2 <= r2;
-        if (hit && alpha < 0.0f) hit = fmaf(wx, wx, fmaf(wy, wy, wz * wz)) <= r2;
-        // prune spheres entered only behind the best hit
-        if (hit && alpha > best_f) {
-            const float e = alpha - best_f;
-            hit = e * e <= r2 - dist2;
-        }
-        if (!hit) { i = nd.skip; continue; }
+        const float t1x = fmaf(nd.lo[0], idx, -oix), t2x = fmaf(nd.hi[0], idx, -oix);
+        const float t1y = fmaf(nd.lo[1], idy, -oiy), t2y = fmaf(nd.hi[1], idy, -oiy);
+        const float t1z = fmaf(nd.lo[2], idz, -oiz), t2z = fmaf(nd.hi[2], idz, -oiz);
+        const float tmin = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fmaxf(fminf(t1z, t2z), 0.0f));
+        const float tmax = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fminf(fmaxf(t1z, t2z), best_f));
+        if (!(tmin <= tmax)) { i = nd.skip; continue; }
         if (nd.leaf >= 0) {
             LeafHit h;
             if (intersect_leaf(sc, sc.geo + nd.leaf, s, d, h) && h.t < best.hit.t) {
                 best.leaf = nd.leaf;
                 best.hit = h;
-                best_f = static_cast<float>(h.t) * (1.0f + 1e-6f) + sc.eps;
+                best_f = static_cast<float>(h.t - t_adv) * (1.0f + 1e-6f) + sc.eps;
             }
         }
         i++;

@@ -292,26 +292,70 @@ double leaf_params(int kind, const double* p, double* out) {
     }
 }
 
+// Axis-aligned box in the world frame.
+struct Box {
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL};
+    double hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    void grow(const double* p) {
+        for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], p[k]); hi[k] = std::max(hi[k], p[k]); }
+    }
+    void grow(const Box& b) {
+        for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], b.lo[k]); hi[k] = std::max(hi[k], b.hi[k]); }
+    }
+    bool empty() const { return lo[0] > hi[0]; }
+    double area() const {
+        if (empty()) return 0.0;
+        const double x = hi[0] - lo[0], y = hi[1] - lo[1], z = hi[2] - lo[2];
+        return 2.0 * (x * y + y * z + z * x);
+    }
+    double centre(int k) const { return 0.5 * (lo[k] + hi[k]); }
+};
+
+// Object-frame box holding every point where a leaf can report a hit.
+Box local_box(int kind, const double* p, const double* prm, double radius) {
+    Box b;
+    double e[3] = {0, 0, 0}, z0 = 0.0, z1 = 0.0;
+    switch (kind) {
+    case MLGPU_KIND_SPHERE_CAP_HEX:
+    case MLGPU_KIND_SPHERE_CAP_CYL:
+    case MLGPU_KIND_SPHERE_CAP_RECT: {
+        // hits satisfy |p| <= bounding radius and lie on the sphere
+        // x^2 + y^2 + z^2 = 2 R z, so 0 <= z = |p|^2 / (2 R) <= r^2 / (2 R)
+        const double xy = kind == MLGPU_KIND_SPHERE_CAP_RECT ? std::max(prm[1], prm[2]) : p[1];
+        e[0] = e[1] = std::min(radius, xy);
+        z0 = 0.0;
+        z1 = std::min(radius, radius * radius / (2.0 * p[0]));
+        break;
+    }
+    case MLGPU_KIND_PLANE: e[0] = prm[1]; e[1] = prm[2]; break;
+    case MLGPU_KIND_DISC:
+    case MLGPU_KIND_HEX_PLANE:
+    case MLGPU_KIND_ANNULUS:
+    case MLGPU_KIND_PLANE_DUAL_SPHERICAL:
+        e[0] = e[1] = radius;
+        break;
+    case MLGPU_KIND_TRIANGLE: {
+        double lo[3] = {std::min({p[0], p[2], p[4]}), std::min({p[1], p[3], p[5]}), 0.0};
+        double hi[3] = {std::max({p[0], p[2], p[4]}), std::max({p[1], p[3], p[5]}), 0.0};
+        b.grow(lo);
+        b.grow(hi);
+        return b;
+    }
+    case MLGPU_KIND_CYLINDER: e[0] = e[1] = p[0]; z0 = -0.5 * p[1]; z1 = 0.5 * p[1]; break;
+    case MLGPU_KIND_SPHERE: e[0] = e[1] = p[0]; z0 = -p[0]; z1 = p[0]; break;
+    default: return b;  // a frame without surface is never hit
+    }
+    const double lo[3] = {-e[0], -e[1], z0}, hi[3] = {e[0], e[1], z1};
+    b.grow(lo);
+    b.grow(hi);
+    return b;
+}
+
 struct BNode {
-    double c[3];
-    double r;
+    Box box;
     int32_t leaf_frame = -1;       // description frame of a leaf
     std::vector<int32_t> kids;
 };
-
-// smallest sphere enclosing two spheres
-void enclose(const BNode& a, const BNode& b, BNode* out) {
-    const double dx = b.c[0] - a.c[0], dy = b.c[1] - a.c[1], dz = b.c[2] - a.c[2];
-    const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
-    if (d + b.r <= a.r) { for (int k = 0; k < 3; k++) out->c[k] = a.c[k]; out->r = a.r; return; }
-    if (d + a.r <= b.r) { for (int k = 0; k < 3; k++) out->c[k] = b.c[k]; out->r = b.r; return; }
-    const double R = 0.5 * (d + a.r + b.r);
-    const double w = (R - a.r) / d;
-    out->c[0] = a.c[0] + dx * w;
-    out->c[1] = a.c[1] + dy * w;
-    out->c[2] = a.c[2] + dz * w;
-    out->r = R * (1.0 + 1e-12);
-}
 
 class Builder {
  public:
@@ -368,8 +412,19 @@ class Builder {
     int32_t make_leaf(int32_t f) {
         BNode b;
         double prm[8];
-        b.r = leaf_params(d_.kind[f], d_.params + MLGPU_NUM_PARAMS * f, prm);
-        b.c[0] = t2w_[f].T[0][3]; b.c[1] = t2w_[f].T[1][3]; b.c[2] = t2w_[f].T[2][3];
+        const double* p = d_.params + MLGPU_NUM_PARAMS * f;
+        const double radius = leaf_params(d_.kind[f], p, prm);
+        const Box lb = local_box(d_.kind[f], p, prm, radius);
+        if (!lb.empty()) {
+            const M34& m = t2w_[f];
+            for (int c = 0; c < 8; c++) {
+                const double q[3] = {(c & 1) ? lb.hi[0] : lb.lo[0], (c & 2) ? lb.hi[1] : lb.lo[1],
+                                     (c & 4) ? lb.hi[2] : lb.lo[2]};
+                double w[3];
+                for (int i = 0; i < 3; i++) w[i] = m.T[i][0] * q[0] + m.T[i][1] * q[1] + m.T[i][2] * q[2] + m.T[i][3];
+                b.box.grow(w);
+            }
+        }
         b.leaf_frame = f;
         bnodes_.push_back(b);
         return static_cast<int32_t>(bnodes_.size()) - 1;
@@ -378,8 +433,12 @@ class Builder {
     void collect(int32_t f, std::vector<int32_t>* items) {
         for (int32_t c : children_[f]) {
             if (scope_[c]) items->push_back(build_scope(c));
-            else if (children_[c].empty()) items->push_back(make_leaf(c));
-            else collect(c, items);
+            else if (children_[c].empty()) {
+                // a frame without a surface never reports a hit (Frame.cpp:172-179)
+                if (d_.kind[c] != MLGPU_KIND_FRAME) items->push_back(make_leaf(c));
+            } else {
+                collect(c, items);
+            }
         }
     }
 
@@ -392,9 +451,7 @@ class Builder {
         }
         int32_t root;
         if (items.empty()) {
-            BNode b;
-            b.c[0] = t2w_[f].T[0][3]; b.c[1] = t2w_[f].T[1][3]; b.c[2] = t2w_[f].T[2][3];
-            b.r = 0.0;
+            BNode b;  // empty box: never entered
             bnodes_.push_back(b);
             root = static_cast<int32_t>(bnodes_.size()) - 1;
         } else {
@@ -404,28 +461,71 @@ class Builder {
         return root;
     }
 
-    // binary tree over items[lo, hi): median split along the widest axis
+    // binary tree over items[lo, hi) with the surface-area heuristic, binned
+    // along the three axes of the centroid bounds
     int32_t bvh(std::vector<int32_t>& items, size_t lo, size_t hi) {
         if (hi - lo == 1) return items[lo];
-        double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-        for (size_t i = lo; i < hi; i++)
-            for (int k = 0; k < 3; k++) {
-                mn[k] = std::min(mn[k], bnodes_[items[i]].c[k]);
-                mx[k] = std::max(mx[k], bnodes_[items[i]].c[k]);
+        Box cb, all;
+        for (size_t i = lo; i < hi; i++) {
+            const Box& b = bnodes_[items[i]].box;
+            all.grow(b);
+            if (!b.empty()) {
+                const double c[3] = {b.centre(0), b.centre(1), b.centre(2)};
+                cb.grow(c);
             }
-        int axis = 0;
-        for (int k = 1; k < 3; k++)
-            if (mx[k] - mn[k] > mx[axis] - mn[axis]) axis = k;
-        const size_t mid = lo + (hi - lo) / 2;
-        std::nth_element(items.begin() + lo, items.begin() + mid, items.begin() + hi,
-                         [&](int32_t a, int32_t b) {
-                             const double ca = bnodes_[a].c[axis], cb = bnodes_[b].c[axis];
-                             return ca < cb || (ca == cb && a < b);
-                         });
+        }
+        constexpr int kBins = 32;
+        int best_axis = -1, best_split = 0;
+        double best_cost = HUGE_VAL;
+        for (int axis = 0; axis < 3 && !cb.empty(); axis++) {
+            const double ext = cb.hi[axis] - cb.lo[axis];
+            if (!(ext > 0.0)) continue;
+            Box bins[kBins];
+            int count[kBins] = {0};
+            for (size_t i = lo; i < hi; i++) {
+                const Box& b = bnodes_[items[i]].box;
+                const double c = b.empty() ? cb.lo[axis] : b.centre(axis);
+                int k = static_cast<int>((c - cb.lo[axis]) / ext * kBins);
+                k = std::min(std::max(k, 0), kBins - 1);
+                bins[k].grow(b);
+                count[k]++;
+            }
+            Box left[kBins];
+            int nl[kBins];
+            Box acc;
+            int n = 0;
+            for (int k = 0; k < kBins; k++) { acc.grow(bins[k]); n += count[k]; left[k] = acc; nl[k] = n; }
+            acc = Box();
+            n = 0;
+            for (int k = kBins - 1; k > 0; k--) {
+                acc.grow(bins[k]);
+                n += count[k];
+                if (nl[k - 1] == 0 || n == 0) continue;
+                const double cost = left[k - 1].area() * nl[k - 1] + acc.area() * n;
+                if (cost < best_cost) { best_cost = cost; best_axis = axis; best_split = k; }
+            }
+        }
+        size_t mid;
+        if (best_axis < 0) {
+            mid = lo + (hi - lo) / 2;  // coincident centroids: any split
+        } else {
+            const int axis = best_axis;
+            const double ext = cb.hi[axis] - cb.lo[axis];
+            auto it = std::partition(items.begin() + lo, items.begin() + hi, [&](int32_t q) {
+                const Box& b = bnodes_[q].box;
+                const double c = b.empty() ? cb.lo[axis] : b.centre(axis);
+                int k = static_cast<int>((c - cb.lo[axis]) / ext * kBins);
+                k = std::min(std::max(k, 0), kBins - 1);
+                return k < best_split;
+            });
+            mid = static_cast<size_t>(it - items.begin());
+            if (mid == lo || mid == hi) mid = lo + (hi - lo) / 2;
+        }
         const int32_t left = bvh(items, lo, mid);
         const int32_t right = bvh(items, mid, hi);
         BNode b;
-        enclose(bnodes_[left], bnodes_[right], &b);
+        b.box = bnodes_[left].box;
+        b.box.grow(bnodes_[right].box);
         b.kids = {left, right};
         bnodes_.push_back(b);
         return static_cast<int32_t>(bnodes_.size()) - 1;
@@ -437,15 +537,9 @@ class Builder {
         emitted_[bi] = idx;
         Node nd;
         const BNode& b = bnodes_[bi];
-        nd.cx = static_cast<float>(b.c[0]);
-        nd.cy = static_cast<float>(b.c[1]);
-        nd.cz = static_cast<float>(b.c[2]);
-        nd.r = static_cast<float>(b.r);
         nd.leaf = -1;
-        nd.pad0 = nd.pad1 = 0;
         hs_->nodes.push_back(nd);
-        hs_->node_radius.push_back(b.r);
-        hs_->node_center.push_back({b.c[0], b.c[1], b.c[2]});
+        hs_->node_box.push_back({b.box.lo[0], b.box.lo[1], b.box.lo[2], b.box.hi[0], b.box.hi[1], b.box.hi[2]});
         if (b.leaf_frame >= 0) {
             hs_->nodes[idx].leaf = add_leaf(b.leaf_frame);
         } else {
@@ -540,24 +634,32 @@ class Builder {
         hs_->hex_cos = std::cos(2. / 3. * M_PI);
     }
 
-    // Inflate float spheres so the FP32 test stays conservative: centre
-    // rounding plus the test's own rounding, bounded by a few ulp of the
-    // scene's coordinate magnitude.
+    // Round boxes outwards to float and inflate them so the FP32 slab test
+    // stays conservative: the ray origin is rounded to float and the slab
+    // arithmetic rounds, both bounded by a few ulp of the scene's extent.
     void inflate() {
-        double extent = 0.0;
-        for (size_t i = 0; i < hs_->nodes.size(); i++) {
-            const auto& c = hs_->node_center[i];
-            extent = std::max(extent, std::fabs(c[0]) + hs_->node_radius[i]);
-            extent = std::max(extent, std::fabs(c[1]) + hs_->node_radius[i]);
-            extent = std::max(extent, std::fabs(c[2]) + hs_->node_radius[i]);
-        }
-        const double eps = 2e-6 * std::max(extent, 1.0);
+        double extent = 1.0;
+        for (const auto& b : hs_->node_box)
+            for (int k = 0; k < 6; k++)
+                if (std::isfinite(b[k])) extent = std::max(extent, std::fabs(b[k]));
+        const double eps = 4e-6 * extent;
         hs_->eps = eps;
         for (size_t i = 0; i < hs_->nodes.size(); i++) {
-            const double r = hs_->node_radius[i] * (1.0 + 1e-6) + eps;
-            float rf = static_cast<float>(r);
-            if (static_cast<double>(rf) < r) rf = std::nextafter(rf, INFINITY);
-            hs_->nodes[i].r = rf;
+            const auto& b = hs_->node_box[i];
+            Node& nd = hs_->nodes[i];
+            if (!(b[0] <= b[3])) {
+                // nothing inside (a scope without surfaces): a point box
+                nd.lo[0] = nd.lo[1] = nd.lo[2] = 0.0f;
+                nd.hi[0] = nd.hi[1] = nd.hi[2] = 0.0f;
+                continue;
+            }
+            for (int k = 0; k < 3; k++) {
+                float lo = static_cast<float>(b[k] - eps), hi = static_cast<float>(b[3 + k] + eps);
+                if (static_cast<double>(lo) > b[k] - eps) lo = std::nextafter(lo, -HUGE_VALF);
+                if (static_cast<double>(hi) < b[3 + k] + eps) hi = std::nextafter(hi, HUGE_VALF);
+                nd.lo[k] = lo;
+                nd.hi[k] = hi;
+            }
         }
     }
 };

@@ -13,8 +13,7 @@ namespace mlgpu {
 
 struct HostScene {
     std::vector<Node> nodes;
-    std::vector<double> node_radius;                 // exact (double) radii
-    std::vector<std::array<double, 3>> node_center;  // exact (double) centres
+    std::vector<std::array<double, 6>> node_box;     // exact (double) lo xyz, hi xyz
     std::vector<LeafGeo> geo;
     std::vector<LeafPhys> phys;
     std::vector<int32_t> leaf_frame;
