@@ -72,12 +72,21 @@ def main() -> int:
     ap.add_argument("--no-light-guides", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy a float32 buffer of this size (a plain copy kernel of known "
+                         "traffic, for scaling rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the photon propagator runs on a single MI355X here; "
               "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
         return 2
     dev = torch.device("cuda:0")
+    if args.copy_calibration_mib > 0:
+        src = torch.ones(args.copy_calibration_mib * 2 ** 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     desc = SceneDescription.fact(light_guides=not args.no_light_guides)
     scene = GpuScene(desc)
     n = args.photons
