@@ -199,6 +199,19 @@ void mlgpu_scene_destroy(mlgpu_scene* scene);
 int mlgpu_scene_stats(const mlgpu_scene* scene, int64_t* num_nodes,
                       int64_t* num_leaves, int64_t* device_bytes);
 
+/* Kernel schedule of mlgpu_propagate:
+ *   FUSED      one lane carries a photon through all its bounces;
+ *   WAVEFRONT  per bounce level a trace kernel and a shade kernel over a
+ *              compacted queue of live photons (workspace owned by the scene,
+ *              ~90 B per photon; calls sharing a scene are serialised on it);
+ *   AUTO       WAVEFRONT from 65,536 photons on. */
+enum mlgpu_schedule {
+    MLGPU_SCHEDULE_AUTO = 0,
+    MLGPU_SCHEDULE_FUSED = 1,
+    MLGPU_SCHEDULE_WAVEFRONT = 2
+};
+int mlgpu_scene_set_schedule(mlgpu_scene* scene, int schedule);
+
 /* Propagate n photons, one Philox stream per photon.  Replaces
  * merlict::propagate_photons_in_frame_with_config (merlict/Photons.cpp:9-22)
  * and its multi-thread twin (merlict_multi_thread.cpp:28-55).  world_frame

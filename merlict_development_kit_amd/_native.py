@@ -39,6 +39,10 @@ REFLECTION_ON_SURFACE = 5
 REFRACTION_TO_OUTSIDE = 6
 REFRACTION_TO_INSIDE = 7
 
+SCHEDULE_AUTO = 0
+SCHEDULE_FUSED = 1
+SCHEDULE_WAVEFRONT = 2
+
 PHOTON_OK = 0
 PHOTON_BAD_WAVELENGTH = 1
 PHOTON_OUT_OF_TABLE = 2
@@ -103,6 +107,7 @@ SIGNATURES = {
                                                                    C.POINTER(C.c_void_p)]),
     "mlgpu_scene_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     "mlgpu_scene_destroy": (None, [C.c_void_p]),
+    "mlgpu_scene_set_schedule": (C.c_int, [C.c_void_p, C.c_int]),
     "mlgpu_scene_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlgpu_propagate": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
                                   C.POINTER(Results), C.c_void_p]),

@@ -3,6 +3,7 @@
 // codes plus mlgpu_last_error().
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -17,6 +18,12 @@ struct mlgpu_scene {
     int64_t num_leaves = 0;
     int64_t device_bytes = 0;
     int device = 0;
+    int num_cus = 0;
+    int schedule = MLGPU_SCHEDULE_AUTO;
+    // wavefront workspace, grown on demand
+    std::mutex ws_mutex;
+    void* ws = nullptr;
+    int64_t ws_capacity = 0;
 };
 
 namespace {
@@ -70,6 +77,9 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
     }
     auto* s = new mlgpu_scene();
     (void)hipGetDevice(&s->device);
+    if (hipDeviceGetAttribute(&s->num_cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess ||
+        s->num_cus < 1)
+        s->num_cus = 256;
     int rc = MLGPU_OK;
     if ((rc = upload(s, hs.nodes, &s->dev.nodes)) || (rc = upload(s, hs.geo, &s->dev.geo)) ||
         (rc = upload(s, hs.phys, &s->dev.phys)) || (rc = upload(s, hs.funcs, &s->dev.funcs)) ||
@@ -94,7 +104,17 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
 void mlgpu_scene_destroy(mlgpu_scene* s) {
     if (!s) return;
     for (void* p : s->buffers) (void)hipFree(p);
+    if (s->ws) (void)hipFree(s->ws);
     delete s;
+}
+
+int mlgpu_scene_set_schedule(mlgpu_scene* s, int schedule) {
+    if (!s || schedule < MLGPU_SCHEDULE_AUTO || schedule > MLGPU_SCHEDULE_WAVEFRONT) {
+        set_last_error("bad schedule");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    s->schedule = schedule;
+    return MLGPU_OK;
 }
 
 int mlgpu_scene_stats(const mlgpu_scene* s, int64_t* num_nodes, int64_t* num_leaves, int64_t* device_bytes) {
@@ -124,9 +144,48 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
         set_last_error("max_num_interactions_per_photon must be <= 250");
         return MLGPU_ERR_ARGUMENT;
     }
-    hipError_t e = mlgpu::launch_propagate(s->dev, s->scope_node[world_frame], *photons, n,
-                                           cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset,
-                                           *r, static_cast<hipStream_t>(stream));
+    if (n > INT32_MAX) { set_last_error("at most 2^31 - 1 photons per call"); return MLGPU_ERR_ARGUMENT; }
+    int schedule = s->schedule;
+    if (schedule == MLGPU_SCHEDULE_AUTO) schedule = n >= 65536 ? MLGPU_SCHEDULE_WAVEFRONT : MLGPU_SCHEDULE_FUSED;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    if (schedule == MLGPU_SCHEDULE_FUSED) {
+        e = mlgpu::launch_propagate(s->dev, s->scope_node[world_frame], *photons, n,
+                                    cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset, *r, st);
+    } else {
+        auto* ms = const_cast<mlgpu_scene*>(s);
+        std::lock_guard<std::mutex> lock(ms->ws_mutex);
+        if (ms->ws_capacity < n) {
+            if (ms->ws) { (void)hipFree(ms->ws); ms->ws = nullptr; ms->ws_capacity = 0; }
+            const int64_t cap = n + n / 8;
+            e = hipMalloc(&ms->ws, mlgpu::workspace_bytes(cap));
+            if (e != hipSuccess) return hip_fail(e, "hipMalloc workspace");
+            ms->ws_capacity = cap;
+        }
+        // carve the workspace into 256-byte aligned arrays
+        char* p = static_cast<char*>(ms->ws);
+        const size_t cap = static_cast<size_t>(ms->ws_capacity);
+        auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) & ~static_cast<size_t>(255); return q; };
+        mlgpu::Workspace w;
+        w.sx = reinterpret_cast<double*>(take(8 * cap));
+        w.sy = reinterpret_cast<double*>(take(8 * cap));
+        w.sz = reinterpret_cast<double*>(take(8 * cap));
+        w.dx = reinterpret_cast<double*>(take(8 * cap));
+        w.dy = reinterpret_cast<double*>(take(8 * cap));
+        w.dz = reinterpret_cast<double*>(take(8 * cap));
+        w.tof = reinterpret_cast<double*>(take(8 * cap));
+        w.root = reinterpret_cast<int32_t*>(take(4 * cap));
+        w.n_hist = reinterpret_cast<uint32_t*>(take(4 * cap));
+        w.draws = reinterpret_cast<uint32_t*>(take(4 * cap));
+        w.hit_leaf = reinterpret_cast<int32_t*>(take(4 * cap));
+        w.hit_t = reinterpret_cast<double*>(take(8 * cap));
+        w.queue[0] = reinterpret_cast<int32_t*>(take(4 * cap));
+        w.queue[1] = reinterpret_cast<int32_t*>(take(4 * cap));
+        w.counts = reinterpret_cast<int32_t*>(take(4 * 256));
+        e = mlgpu::launch_propagate_wavefront(s->dev, s->scope_node[world_frame], *photons, n,
+                                              cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset,
+                                              *r, w, 8 * s->num_cus, st);
+    }
     if (e != hipSuccess) return hip_fail(e, "propagate launch");
     return MLGPU_OK;
 }

@@ -53,9 +53,22 @@ MLGPU_HD inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32
 struct Rng {
     uint32_t k0, k1, id_lo, id_hi;
     uint32_t b0, b1, b2, b3;
-    uint32_t next;
+    uint32_t next;   // index of the next draw
+    bool have;       // b0..b3 hold block next / 4
+    MLGPU_HD inline void init(uint64_t seed, uint64_t id, uint32_t draw) {
+        k0 = static_cast<uint32_t>(seed);
+        k1 = static_cast<uint32_t>(seed >> 32);
+        id_lo = static_cast<uint32_t>(id);
+        id_hi = static_cast<uint32_t>(id >> 32);
+        b0 = b1 = b2 = b3 = 0u;
+        next = draw;
+        have = false;
+    }
     MLGPU_HD inline double uniform() {
-        if ((next & 3u) == 0u) philox4x32_10(next >> 2, 0u, id_lo, id_hi, k0, k1, b0, b1, b2, b3);
+        if ((next & 3u) == 0u || !have) {
+            philox4x32_10(next >> 2, 0u, id_lo, id_hi, k0, k1, b0, b1, b2, b3);
+            have = true;
+        }
         const uint32_t sel = next & 3u;
         const uint32_t w = sel == 0u ? b0 : (sel == 1u ? b1 : (sel == 2u ? b2 : b3));
         next++;
@@ -331,175 +344,220 @@ MLGPU_HD inline void first_hit(const DeviceScene& sc, int32_t scope, const D3& s
     }
 }
 
-// Propagator (merlict/PhotonAndFrame.cpp:6-124) for photon i of the batch.
+// ------------------------------------------------------------- propagator
+// State of one photon between bounces (merlict's Photon reduced to what the
+// remaining propagation and the final record need).
+struct PhotonState {
+    D3 s, d;                 // ray support and unit direction, world frame
+    double tof;              // running Photon::time_of_flight
+    int32_t root;            // scope node of env.root_frame
+    uint32_t n_hist;         // history length including production
+    uint8_t err;
+    uint8_t last_interaction;
+    int32_t last_frame, last_sensor;
+    double out_x, out_y, out_cx, out_cy;
+};
+
+MLGPU_HD inline void photon_init(PhotonState& st, const mlgpu_photons& in, const int64_t i, const int32_t world_node) {
+    st.s = D3{in.support_x[i], in.support_y[i], in.support_z[i]};
+    st.d = normalized(D3{in.direction_x[i], in.direction_y[i], in.direction_z[i]});  // Ray.cpp:16-20
+    st.tof = 0.0;
+    st.root = world_node;
+    st.n_hist = 1;  // production
+    st.err = in.wavelength[i] > 0.0 ? MLGPU_PHOTON_OK : MLGPU_PHOTON_BAD_WAVELENGTH;  // Photon.cpp:20-28
+    st.last_interaction = MLGPU_PRODUCTION;
+    st.last_frame = -1;
+    st.last_sensor = -1;
+    st.out_x = st.s.x;
+    st.out_y = st.s.y;
+    st.out_cx = 0.0;
+    st.out_cy = 0.0;
+}
+
+// One pass of Propagator::work_on_first_causal_intersection and what follows
+// (merlict/PhotonAndFrame.cpp:23-124) for the closest hit (leaf, t) of the
+// current ray (leaf < 0: none).  Returns true when the photon propagates on.
+MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const int32_t leaf, const double t,
+                                   const double lambda, Rng& rng, const uint32_t max_interactions) {
+    if (leaf < 0) {
+        // no hit: absorbed in void at 1e4 m, n = 1 on that leg
+        double nv;
+        if (!func_eval(sc, sc.default_refraction, lambda, nv)) { st.err = MLGPU_PHOTON_OUT_OF_TABLE; return false; }
+        st.tof += nv * kVoidDistance / kC;
+        st.n_hist++;
+        st.last_interaction = MLGPU_ABSORPTION_IN_VOID;
+        st.last_frame = -1;
+        st.last_sensor = -1;
+        const D3 e = at(st.s, st.d, kVoidDistance);
+        st.out_x = e.x;
+        st.out_y = e.y;
+        const D3 inc = normalized(D3{e.x - st.s.x, e.y - st.s.y, e.z - st.s.z});
+        st.out_cx = -1.0 * inc.x;
+        st.out_cy = -1.0 * inc.y;
+        return false;
+    }
+    const LeafGeo& g = sc.geo[leaf];
+    const LeafPhys ph = sc.phys[leaf];
+    // the object-frame ray and hit, as the leaf test computed them
+    const D3 ls = position_inv(g, st.s);
+    const D3 ld = orient_inv(g, st.d);
+    const D3 hp = at(ls, ld, t);
+    const D3 n = leaf_normal(g, hp);
+    // Intersection::ray_runs_from_outside_to_inside (Intersection.cpp:133-139)
+    const bool outside_in = dot(n, ld) < 0.0;
+    const D3 hit_world = position(g, hp);
+
+    uint8_t interaction;
+    bool stop = true;
+    D3 nd = st.d;
+    int32_t new_root = st.root;
+    // absorbed_in_medium_before_reaching_surface (PhotonAndFrame.cpp:32-38)
+    double ell;
+    if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_ABSORPTION : MLGPU_INNER_ABSORPTION], lambda, ell)) {
+        st.err = MLGPU_PHOTON_OUT_OF_TABLE;
+        return false;
+    }
+    const double survival = exp(-t / ell);
+    if (rng.uniform() > survival) {
+        interaction = MLGPU_ABSORPTION_IN_VOID;
+    } else {
+        // interact_with_object (PhotonAndFrame.cpp:40-48)
+        double refl;
+        if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFLECTION : MLGPU_INNER_REFLECTION], lambda, refl)) {
+            st.err = MLGPU_PHOTON_OUT_OF_TABLE;
+            return false;
+        }
+        bool mirror = false;
+        if (refl >= rng.uniform()) {
+            interaction = MLGPU_REFLECTION_ON_SURFACE;
+            mirror = true;
+        } else {
+            const bool transparent = ph.surf[MLGPU_OUTER_REFRACTION] != sc.default_refraction ||
+                                     ph.surf[MLGPU_INNER_REFRACTION] != sc.default_refraction;
+            if (!transparent) {
+                interaction = MLGPU_ABSORPTION_ON_SURFACE;
+            } else {
+                // fresnel_refraction_and_reflection (PhotonAndFrame.cpp:75-87), Fresnel.cpp:7-38
+                double n_from, n_to;
+                const int32_t f_from = ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION];
+                const int32_t f_to = ph.surf[outside_in ? MLGPU_INNER_REFRACTION : MLGPU_OUTER_REFRACTION];
+                if (!func_eval(sc, f_from, lambda, n_from) || !func_eval(sc, f_to, lambda, n_to)) {
+                    st.err = MLGPU_PHOTON_OUT_OF_TABLE;
+                    return false;
+                }
+                const D3 nrm = outside_in ? n : D3{n.x * -1.0, n.y * -1.0, n.z * -1.0};
+                const double cosI = dot(nrm, ld) * (-1.0);
+                const double eta = n_from / n_to;
+                const double sinT2 = eta * eta * (1.0 - cosI * cosI);
+                const double cosT = sqrt(1.0 - sinT2);
+                double rf;
+                if (sinT2 > 1.0) {
+                    rf = 1.0;
+                } else {
+                    const double ro = (n_from * cosI - n_to * cosT) / (n_from * cosI + n_to * cosT);
+                    const double rp = (n_to * cosI - n_from * cosT) / (n_to * cosI + n_from * cosT);
+                    rf = (ro * ro + rp * rp) / 2.0;
+                }
+                if (rf > rng.uniform()) {
+                    interaction = MLGPU_FRESNEL_REFLECTION_ON_SURFACE;
+                    mirror = true;
+                } else {
+                    // pass_the_boundary_layer / propagate_on_after_boundary_layer (:89-118)
+                    interaction = outside_in ? MLGPU_REFRACTION_TO_INSIDE : MLGPU_REFRACTION_TO_OUTSIDE;
+                    const bool to_default = (outside_in ? ph.surf[MLGPU_INNER_REFRACTION]
+                                                        : ph.surf[MLGPU_OUTER_REFRACTION]) == sc.default_refraction;
+                    new_root = (ph.allowed_node >= 0 && !to_default) ? ph.allowed_node : ph.root_node;
+                    const double k = eta * cosI - cosT;
+                    const D3 refr{ld.x * eta + nrm.x * k, ld.y * eta + nrm.y * k, ld.z * eta + nrm.z * k};
+                    nd = normalized(orient(g, refr));
+                    stop = false;
+                }
+            }
+        }
+        if (mirror) {
+            // Intersection::reflection_direction_in_root_frame with the mirror
+            // matrix I - 2 n n^T (Intersection.cpp:74-79, Vec3.cpp:59-113)
+            const double x = n.x, y = n.y, z = n.z;
+            const D3 m{(1. - 2. * x * x) * ld.x + -2. * x * y * ld.y + -2. * x * z * ld.z,
+                       -2. * x * y * ld.x + (1. - 2. * y * y) * ld.y + -2. * y * z * ld.z,
+                       -2. * x * z * ld.x + -2. * y * z * ld.y + (1. - 2. * z * z) * ld.z};
+            nd = normalized(orient(g, m));
+            stop = false;
+        }
+    }
+    // push_back_intersection_and_interaction; Photon::time_of_flight (Photon.cpp:44-56)
+    double n_from;
+    if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION], lambda, n_from)) {
+        st.err = MLGPU_PHOTON_OUT_OF_TABLE;
+        return false;
+    }
+    st.tof += n_from * t / kC;
+    st.n_hist++;
+    st.last_interaction = interaction;
+    st.last_frame = ph.frame;
+    st.last_sensor = ph.sensor;
+    st.out_x = hp.x;
+    st.out_y = hp.y;
+    {
+        // final_intersection_incident_direction_wrt_frame (RayForPropagation.cpp:114-136)
+        const D3 inc = normalized(D3{hit_world.x - st.s.x, hit_world.y - st.s.y, hit_world.z - st.s.z});
+        const D3 loc = orient_inv(g, inc);
+        st.out_cx = -1.0 * loc.x;
+        st.out_cy = -1.0 * loc.y;
+    }
+    if (stop) return false;
+    st.s = hit_world;
+    st.d = nd;
+    st.root = new_root;
+    // Propagator::propagate (PhotonAndFrame.cpp:13-21)
+    return st.n_hist <= max_interactions;
+}
+
+MLGPU_HD inline void photon_finish(const PhotonState& st, const mlgpu_results& out, const int64_t i) {
+    if (st.err != MLGPU_PHOTON_OK) {
+        // merlict aborts the batch with an exception here; the record of an
+        // erroneous photon is its error code with a neutral final state
+        out.final_frame[i] = -1;
+        out.sensor_id[i] = -1;
+        out.final_interaction[i] = MLGPU_PRODUCTION;
+        out.num_interactions[i] = 0;
+        out.error[i] = st.err;
+        out.time_of_flight[i] = 0.0;
+        out.x[i] = 0.0f;
+        out.y[i] = 0.0f;
+        out.cx[i] = 0.0f;
+        out.cy[i] = 0.0f;
+        return;
+    }
+    out.final_frame[i] = st.last_frame;
+    out.sensor_id[i] = st.last_sensor;
+    out.final_interaction[i] = st.last_interaction;
+    out.num_interactions[i] = static_cast<uint8_t>(st.n_hist);
+    out.error[i] = st.err;
+    out.time_of_flight[i] = st.tof;
+    out.x[i] = static_cast<float>(st.out_x);
+    out.y[i] = static_cast<float>(st.out_y);
+    out.cx[i] = static_cast<float>(st.out_cx);
+    out.cy[i] = static_cast<float>(st.out_cy);
+}
+
+// Propagator (merlict/PhotonAndFrame.cpp:6-124) for photon i of the batch,
+// all bounces in one lane.
 MLGPU_HD inline void propagate_photon(const DeviceScene& sc, const int32_t world_node, const mlgpu_photons& in,
                                       const int64_t i, const uint32_t max_interactions, const uint64_t seed,
                                       const uint64_t id_offset, const mlgpu_results& out) {
-    D3 s{in.support_x[i], in.support_y[i], in.support_z[i]};
-    D3 d = normalized(D3{in.direction_x[i], in.direction_y[i], in.direction_z[i]});  // Ray.cpp:16-20
+    PhotonState st;
+    photon_init(st, in, i, world_node);
     const double lambda = in.wavelength[i];
-    const uint64_t id = id_offset + static_cast<uint64_t>(i);
     Rng rng;
-    rng.k0 = static_cast<uint32_t>(seed);
-    rng.k1 = static_cast<uint32_t>(seed >> 32);
-    rng.id_lo = static_cast<uint32_t>(id);
-    rng.id_hi = static_cast<uint32_t>(id >> 32);
-    rng.b0 = rng.b1 = rng.b2 = rng.b3 = 0u;
-    rng.next = 0u;
-
-    uint8_t err = lambda > 0.0 ? MLGPU_PHOTON_OK : MLGPU_PHOTON_BAD_WAVELENGTH;  // Photon.cpp:20-28
-    int32_t root = world_node;
-    uint32_t n_hist = 1;  // production
-    uint8_t last_interaction = MLGPU_PRODUCTION;
-    int32_t last_frame = -1, last_sensor = -1;
-    double tof = 0.0;
-    double out_x = s.x, out_y = s.y, out_cx = 0.0, out_cy = 0.0;
-
-    while (err == MLGPU_PHOTON_OK && n_hist <= max_interactions) {
+    rng.init(seed, id_offset + static_cast<uint64_t>(i), 0u);
+    bool go = st.err == MLGPU_PHOTON_OK && st.n_hist <= max_interactions;
+    while (go) {
         Closest c;
-        first_hit(sc, root, s, d, c);
-        if (c.leaf < 0) {
-            // no hit: absorbed in void at 1e4 m, n = 1 on that leg
-            double nv;
-            if (!func_eval(sc, sc.default_refraction, lambda, nv)) { err = MLGPU_PHOTON_OUT_OF_TABLE; break; }
-            tof += nv * kVoidDistance / kC;
-            n_hist++;
-            last_interaction = MLGPU_ABSORPTION_IN_VOID;
-            last_frame = -1;
-            last_sensor = -1;
-            const D3 e = at(s, d, kVoidDistance);
-            out_x = e.x;
-            out_y = e.y;
-            const D3 inc = normalized(D3{e.x - s.x, e.y - s.y, e.z - s.z});
-            out_cx = -1.0 * inc.x;
-            out_cy = -1.0 * inc.y;
-            break;
-        }
-        const LeafGeo& g = sc.geo[c.leaf];
-        const LeafPhys ph = sc.phys[c.leaf];
-        const D3 n = leaf_normal(g, c.hit.p);
-        // Intersection::ray_runs_from_outside_to_inside (Intersection.cpp:133-139)
-        const bool outside_in = dot(n, c.hit.d) < 0.0;
-        const D3 hit_world = position(g, c.hit.p);
-
-        uint8_t interaction;
-        bool stop = true;
-        D3 nd = d;
-        int32_t new_root = root;
-        // absorbed_in_medium_before_reaching_surface (PhotonAndFrame.cpp:32-38)
-        double ell;
-        if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_ABSORPTION : MLGPU_INNER_ABSORPTION], lambda, ell)) {
-            err = MLGPU_PHOTON_OUT_OF_TABLE;
-            break;
-        }
-        const double survival = exp(-c.hit.t / ell);
-        if (rng.uniform() > survival) {
-            interaction = MLGPU_ABSORPTION_IN_VOID;
-        } else {
-            // interact_with_object (PhotonAndFrame.cpp:40-48)
-            double refl;
-            if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFLECTION : MLGPU_INNER_REFLECTION], lambda, refl)) {
-                err = MLGPU_PHOTON_OUT_OF_TABLE;
-                break;
-            }
-            bool mirror = false;
-            if (refl >= rng.uniform()) {
-                interaction = MLGPU_REFLECTION_ON_SURFACE;
-                mirror = true;
-            } else {
-                const bool transparent = ph.surf[MLGPU_OUTER_REFRACTION] != sc.default_refraction ||
-                                         ph.surf[MLGPU_INNER_REFRACTION] != sc.default_refraction;
-                if (!transparent) {
-                    interaction = MLGPU_ABSORPTION_ON_SURFACE;
-                } else {
-                    // fresnel_refraction_and_reflection (PhotonAndFrame.cpp:75-87), Fresnel.cpp:7-38
-                    double n_from, n_to;
-                    const int32_t f_from = ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION];
-                    const int32_t f_to = ph.surf[outside_in ? MLGPU_INNER_REFRACTION : MLGPU_OUTER_REFRACTION];
-                    if (!func_eval(sc, f_from, lambda, n_from) || !func_eval(sc, f_to, lambda, n_to)) {
-                        err = MLGPU_PHOTON_OUT_OF_TABLE;
-                        break;
-                    }
-                    const D3& inc = c.hit.d;
-                    const D3 nrm = outside_in ? n : D3{n.x * -1.0, n.y * -1.0, n.z * -1.0};
-                    const double cosI = dot(nrm, inc) * (-1.0);
-                    const double eta = n_from / n_to;
-                    const double sinT2 = eta * eta * (1.0 - cosI * cosI);
-                    const double cosT = sqrt(1.0 - sinT2);
-                    double rf;
-                    if (sinT2 > 1.0) {
-                        rf = 1.0;
-                    } else {
-                        const double ro = (n_from * cosI - n_to * cosT) / (n_from * cosI + n_to * cosT);
-                        const double rp = (n_to * cosI - n_from * cosT) / (n_to * cosI + n_from * cosT);
-                        rf = (ro * ro + rp * rp) / 2.0;
-                    }
-                    if (rf > rng.uniform()) {
-                        interaction = MLGPU_FRESNEL_REFLECTION_ON_SURFACE;
-                        mirror = true;
-                    } else {
-                        // pass_the_boundary_layer / propagate_on_after_boundary_layer (:89-118)
-                        interaction = outside_in ? MLGPU_REFRACTION_TO_INSIDE : MLGPU_REFRACTION_TO_OUTSIDE;
-                        const bool to_default = (outside_in ? ph.surf[MLGPU_INNER_REFRACTION]
-                                                            : ph.surf[MLGPU_OUTER_REFRACTION]) == sc.default_refraction;
-                        new_root = (ph.allowed_node >= 0 && !to_default) ? ph.allowed_node : ph.root_node;
-                        const double k = eta * cosI - cosT;
-                        const D3 refr{inc.x * eta + nrm.x * k, inc.y * eta + nrm.y * k, inc.z * eta + nrm.z * k};
-                        nd = normalized(orient(g, refr));
-                        stop = false;
-                    }
-                }
-            }
-            if (mirror) {
-                // Intersection::reflection_direction_in_root_frame with the mirror
-                // matrix I - 2 n n^T (Intersection.cpp:74-79, Vec3.cpp:59-113)
-                const D3& r = c.hit.d;
-                const double x = n.x, y = n.y, z = n.z;
-                const D3 m{(1. - 2. * x * x) * r.x + -2. * x * y * r.y + -2. * x * z * r.z,
-                           -2. * x * y * r.x + (1. - 2. * y * y) * r.y + -2. * y * z * r.z,
-                           -2. * x * z * r.x + -2. * y * z * r.y + (1. - 2. * z * z) * r.z};
-                nd = normalized(orient(g, m));
-                stop = false;
-            }
-        }
-        // push_back_intersection_and_interaction; Photon::time_of_flight (Photon.cpp:44-56)
-        {
-            double n_from;
-            if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION], lambda,
-                           n_from)) {
-                err = MLGPU_PHOTON_OUT_OF_TABLE;
-                break;
-            }
-            tof += n_from * c.hit.t / kC;
-        }
-        n_hist++;
-        last_interaction = interaction;
-        last_frame = ph.frame;
-        last_sensor = ph.sensor;
-        out_x = c.hit.p.x;
-        out_y = c.hit.p.y;
-        {
-            // final_intersection_incident_direction_wrt_frame (RayForPropagation.cpp:114-136)
-            const D3 inc = normalized(D3{hit_world.x - s.x, hit_world.y - s.y, hit_world.z - s.z});
-            const D3 loc = orient_inv(g, inc);
-            out_cx = -1.0 * loc.x;
-            out_cy = -1.0 * loc.y;
-        }
-        if (stop) break;
-        s = hit_world;
-        d = nd;
-        root = new_root;
+        first_hit(sc, st.root, st.s, st.d, c);
+        go = photon_bounce(sc, st, c.leaf, c.hit.t, lambda, rng, max_interactions);
     }
-    out.final_frame[i] = last_frame;
-    out.sensor_id[i] = last_sensor;
-    out.final_interaction[i] = last_interaction;
-    out.num_interactions[i] = static_cast<uint8_t>(n_hist);
-    out.error[i] = err;
-    out.time_of_flight[i] = tof;
-    out.x[i] = static_cast<float>(out_x);
-    out.y[i] = static_cast<float>(out_y);
-    out.cx[i] = static_cast<float>(out_cx);
-    out.cy[i] = static_cast<float>(out_cy);
+    photon_finish(st, out, i);
 }
 
 }  // namespace mlgpu

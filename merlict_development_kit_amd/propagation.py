@@ -130,6 +130,11 @@ class GpuScene:
                 pass
             self._h = None
 
+    def set_schedule(self, schedule: str) -> None:
+        """'auto', 'fused' (one lane per photon) or 'wavefront' (trace/shade per bounce)."""
+        code = {"auto": N.SCHEDULE_AUTO, "fused": N.SCHEDULE_FUSED, "wavefront": N.SCHEDULE_WAVEFRONT}[schedule]
+        N.check(N.lib().mlgpu_scene_set_schedule(self._h, code), "mlgpu_scene_set_schedule")
+
     def stats(self) -> dict:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         N.check(N.lib().mlgpu_scene_stats(self._h, C.byref(a), C.byref(b), C.byref(c)), "mlgpu_scene_stats")

@@ -991,6 +991,18 @@ static void propagate_one(const oracle_scene* s, int world, v3 sup, v3 dir, doub
     }
 
     out->error[k] = (uint8_t)err;
+    if (err) {
+        /* merlict aborts the batch with an exception; the record of an
+         * erroneous photon is its error code with a neutral final state */
+        out->num_interactions[k] = 0;
+        out->final_interaction[k] = MLGPU_PRODUCTION;
+        out->time_of_flight[k] = 0.0;
+        if (out->accumulated_distance) out->accumulated_distance[k] = 0.0;
+        out->final_frame[k] = -1;
+        out->sensor_id[k] = -1;
+        out->x[k] = 0.0; out->y[k] = 0.0; out->cx[k] = 0.0; out->cy[k] = 0.0;
+        return;
+    }
     out->num_interactions[k] = (uint8_t)n_hist;
     out->final_interaction[k] = (uint8_t)last_interaction;
     out->time_of_flight[k] = tof;
