@@ -13,17 +13,21 @@
 struct mlgpu_scene {
     mlgpu::DeviceScene dev;
     std::vector<void*> buffers;
-    std::vector<int32_t> scope_node;
+    std::vector<int32_t> scope_of_frame;
     int64_t num_nodes = 0;
     int64_t num_leaves = 0;
     int64_t device_bytes = 0;
     int device = 0;
     int num_cus = 0;
     int schedule = MLGPU_SCHEDULE_AUTO;
-    // wavefront workspace, grown on demand
+    // wavefront workspace, grown on demand.  Calls that use it are ordered
+    // on the device: a call on another stream than the previous user first
+    // waits for the event recorded after that user's kernels.
     std::mutex ws_mutex;
     void* ws = nullptr;
     int64_t ws_capacity = 0;
+    hipEvent_t ws_event = nullptr;
+    hipStream_t ws_stream = nullptr;
 };
 
 namespace {
@@ -81,21 +85,21 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
         s->num_cus < 1)
         s->num_cus = 256;
     int rc = MLGPU_OK;
-    if ((rc = upload(s, hs.nodes, &s->dev.nodes)) || (rc = upload(s, hs.geo, &s->dev.geo)) ||
-        (rc = upload(s, hs.phys, &s->dev.phys)) || (rc = upload(s, hs.funcs, &s->dev.funcs)) ||
+    if ((rc = upload(s, hs.pairs, &s->dev.pairs)) || (rc = upload(s, hs.scopes, &s->dev.scopes)) ||
+        (rc = upload(s, hs.geo, &s->dev.geo)) || (rc = upload(s, hs.phys, &s->dev.phys)) || (rc = upload(s, hs.funcs, &s->dev.funcs)) ||
         (rc = upload(s, hs.points, &s->dev.points))) {
         mlgpu_scene_destroy(s);
         return rc;
     }
-    s->dev.num_nodes = static_cast<int32_t>(hs.nodes.size());
+    s->dev.num_pairs = static_cast<int32_t>(hs.pairs.size());
     s->dev.num_leaves = static_cast<int32_t>(hs.geo.size());
-    s->dev.root_node = hs.root_node;
+    s->dev.root_scope = hs.root_scope;
     s->dev.default_refraction = hs.default_refraction;
     s->dev.eps = static_cast<float>(hs.eps);
     s->dev.hex_sin = hs.hex_sin;
     s->dev.hex_cos = hs.hex_cos;
-    s->scope_node = hs.scope_node;
-    s->num_nodes = static_cast<int64_t>(hs.nodes.size());
+    s->scope_of_frame = hs.scope_of_frame;
+    s->num_nodes = static_cast<int64_t>(hs.pairs.size());
     s->num_leaves = static_cast<int64_t>(hs.geo.size());
     *out = s;
     return MLGPU_OK;
@@ -103,6 +107,10 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
 
 void mlgpu_scene_destroy(mlgpu_scene* s) {
     if (!s) return;
+    if (s->ws_event) {
+        (void)hipEventSynchronize(s->ws_event);
+        (void)hipEventDestroy(s->ws_event);
+    }
     for (void* p : s->buffers) (void)hipFree(p);
     if (s->ws) (void)hipFree(s->ws);
     delete s;
@@ -135,8 +143,8 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
         set_last_error("null result array");
         return MLGPU_ERR_ARGUMENT;
     }
-    if (world_frame < 0 || world_frame >= static_cast<int32_t>(s->scope_node.size()) ||
-        s->scope_node[world_frame] < 0) {
+    if (world_frame < 0 || world_frame >= static_cast<int32_t>(s->scope_of_frame.size()) ||
+        s->scope_of_frame[world_frame] < 0) {
         set_last_error("world_frame must be frame 0 or a restriction frame");
         return MLGPU_ERR_ARGUMENT;
     }
@@ -150,13 +158,25 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
     const hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e;
     if (schedule == MLGPU_SCHEDULE_FUSED) {
-        e = mlgpu::launch_propagate(s->dev, s->scope_node[world_frame], *photons, n,
+        e = mlgpu::launch_propagate(s->dev, s->scope_of_frame[world_frame], *photons, n,
                                     cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset, *r, st);
     } else {
         auto* ms = const_cast<mlgpu_scene*>(s);
         std::lock_guard<std::mutex> lock(ms->ws_mutex);
+        if (!ms->ws_event) {
+            e = hipEventCreateWithFlags(&ms->ws_event, hipEventDisableTiming);
+            if (e != hipSuccess) { ms->ws_event = nullptr; return hip_fail(e, "hipEventCreate"); }
+        } else if (ms->ws_stream != st) {
+            e = hipStreamWaitEvent(st, ms->ws_event, 0);
+            if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+        }
         if (ms->ws_capacity < n) {
-            if (ms->ws) { (void)hipFree(ms->ws); ms->ws = nullptr; ms->ws_capacity = 0; }
+            if (ms->ws) {
+                (void)hipEventSynchronize(ms->ws_event);
+                (void)hipFree(ms->ws);
+                ms->ws = nullptr;
+                ms->ws_capacity = 0;
+            }
             const int64_t cap = n + n / 8;
             e = hipMalloc(&ms->ws, mlgpu::workspace_bytes(cap));
             if (e != hipSuccess) return hip_fail(e, "hipMalloc workspace");
@@ -182,9 +202,11 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
         w.queue[0] = reinterpret_cast<int32_t*>(take(4 * cap));
         w.queue[1] = reinterpret_cast<int32_t*>(take(4 * cap));
         w.counts = reinterpret_cast<int32_t*>(take(4 * 256));
-        e = mlgpu::launch_propagate_wavefront(s->dev, s->scope_node[world_frame], *photons, n,
+        e = mlgpu::launch_propagate_wavefront(s->dev, s->scope_of_frame[world_frame], *photons, n,
                                               cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset,
                                               *r, w, 8 * s->num_cus, st);
+        if (e == hipSuccess) e = hipEventRecord(ms->ws_event, st);
+        ms->ws_stream = st;
     }
     if (e != hipSuccess) return hip_fail(e, "propagate launch");
     return MLGPU_OK;
@@ -218,12 +240,12 @@ int mlgpu_first_hit(const mlgpu_scene* s, int32_t root_frame, const mlgpu_photon
         set_last_error("null array");
         return MLGPU_ERR_ARGUMENT;
     }
-    if (root_frame < 0 || root_frame >= static_cast<int32_t>(s->scope_node.size()) ||
-        s->scope_node[root_frame] < 0) {
+    if (root_frame < 0 || root_frame >= static_cast<int32_t>(s->scope_of_frame.size()) ||
+        s->scope_of_frame[root_frame] < 0) {
         set_last_error("root_frame must be frame 0 or a restriction frame");
         return MLGPU_ERR_ARGUMENT;
     }
-    hipError_t e = mlgpu::launch_first_hit(s->dev, s->scope_node[root_frame], *rays, n, *hits,
+    hipError_t e = mlgpu::launch_first_hit(s->dev, s->scope_of_frame[root_frame], *rays, n, *hits,
                                            static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "first_hit launch");
     return MLGPU_OK;

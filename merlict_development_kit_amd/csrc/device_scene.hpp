@@ -1,30 +1,52 @@
 // Device layout of a flattened scene (shared by the host builder and the
 // HIP kernels).
 //
-// Traversal hierarchy: a bounding-volume tree stored in depth-first order
-// with skip indices, so a subtree is the contiguous range [i, skip[i]) and
-// traversal needs no stack.  Every frame that can be a propagation root (the
-// world and each restriction frame, merlict/SurfaceEntity.cpp:59-65) owns a
-// node whose range holds exactly its leaves; below such a frame the library
-// builds its own binary tree of axis-aligned boxes over the leaves (surface-
-// area heuristic) instead of merlict's bounding spheres and octant clustering
-// (merlict/Frame.cpp:190-255, RayAndFrame.cpp:16-107).  Any conservative
-// hierarchy yields the same closest hit, because merlict keeps the first
-// minimum over all candidates (RayAndFrame.cpp:160-173).
+// Traversal hierarchy: a binary tree of axis-aligned boxes whose nodes
+// store the boxes of both children, so one 64-byte fetch tests two boxes;
+// traversal visits the nearer child first and keeps the farther one on a
+// short per-lane stack.  Every frame that can be a propagation root (the
+// world and each restriction frame, merlict/SurfaceEntity.cpp:59-65) is a
+// scope: a subtree that holds exactly its leaves.  Below a scope the
+// library builds its own tree (surface-area heuristic) instead of merlict's
+// bounding spheres and octant clustering (merlict/Frame.cpp:190-255,
+// RayAndFrame.cpp:16-107).  Any conservative hierarchy yields the same
+// closest hit, because merlict keeps the first minimum over all candidates
+// (RayAndFrame.cpp:160-173).
 #pragma once
 
 #include <cstdint>
+#include <climits>
 
 namespace mlgpu {
 
-// 32 B per node: world-frame box (inflated by the float rounding bound)
-// and (skip, leaf index or -1).
-struct alignas(16) Node {
-    float lo[3];
-    int32_t skip;
-    float hi[3];
-    int32_t leaf;
+// Child codes of a PairNode: >= 0 another PairNode, kNoChild nothing, any
+// other negative value the leaf -1 - code.
+constexpr int32_t kNoChild = INT32_MIN;
+
+// 64 B per node: the world-frame boxes of both children (inflated by the
+// float rounding bound) and their codes.
+struct alignas(16) PairNode {
+    float lo0[3];
+    int32_t c0;
+    float hi0[3];
+    int32_t c1;
+    float lo1[3];
+    int32_t pad0;
+    float hi1[3];
+    int32_t pad1;
 };
+
+// A propagation root: the box of all its leaves and the node to start at.
+struct alignas(16) Scope {
+    float lo[3];
+    int32_t entry;
+    float hi[3];
+    int32_t pad;
+};
+
+// Deepest node chain a scene may have: the traversal stack holds at most
+// one entry per level.
+constexpr int kMaxTreeDepth = 32;
 
 enum LeafKind : int32_t {
     LK_CAP_HEX = 1,
@@ -57,8 +79,8 @@ struct alignas(16) LeafGeo {
 // Physics of a leaf, read once per bounce (48 B).
 struct alignas(16) LeafPhys {
     int32_t surf[6];       // function index per surface slot (defaults resolved)
-    int32_t allowed_node;  // restriction scope node, -1 none
-    int32_t root_node;     // scope node of the tree root
+    int32_t allowed_scope; // scope of the restriction frame, -1 none
+    int32_t root_scope;    // scope of the tree root
     int32_t frame;         // description frame index
     int32_t sensor;        // sensor id or -1
     int32_t pad0, pad1;
@@ -87,14 +109,15 @@ struct alignas(16) FuncPoint {
 };
 
 struct DeviceScene {
-    const Node* nodes;
+    const PairNode* pairs;
+    const Scope* scopes;
     const LeafGeo* geo;
     const LeafPhys* phys;
     const FuncHead* funcs;
     const FuncPoint* points;
-    int32_t num_nodes;
+    int32_t num_pairs;
     int32_t num_leaves;
-    int32_t root_node;           // node of frame 0
+    int32_t root_scope;          // scope of frame 0
     int32_t default_refraction;  // function index of merlict's DEFAULT_REFRACTION
     double hex_sin, hex_cos;     // HexagonalPrismZ unit-vector components
     float eps;                   // box inflation (absolute, metres)

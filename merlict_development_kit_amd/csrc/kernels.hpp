@@ -27,16 +27,16 @@ inline size_t workspace_bytes(int64_t n) {
     return m * (7 * 8 + 4 + 4 + 4 + 4 + 8 + 4 + 4) + 256 * 4 + 16 * 256 + 1024;
 }
 
-hipError_t launch_propagate(const DeviceScene& sc, int32_t world_node, const mlgpu_photons& in, int64_t n,
+hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in, int64_t n,
                             uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                             const mlgpu_results& out, hipStream_t stream);
 
-hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_node, const mlgpu_photons& in,
+hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& out, const Workspace& ws, int grid_blocks,
                                       hipStream_t stream);
 
-hipError_t launch_first_hit(const DeviceScene& sc, int32_t root_node, const mlgpu_photons& rays, int64_t n,
+hipError_t launch_first_hit(const DeviceScene& sc, int32_t root_scope, const mlgpu_photons& rays, int64_t n,
                             const mlgpu_hits& out, hipStream_t stream);
 
 hipError_t launch_count_errors(const uint8_t* err, int64_t n, unsigned long long* counter, hipStream_t stream);

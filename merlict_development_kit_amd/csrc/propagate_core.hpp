@@ -129,11 +129,6 @@ MLGPU_HD inline D3 position_inv(const LeafGeo& g, const D3& p) {
 }
 
 // ----------------------------------------------------------------- leaves
-struct LeafHit {
-    double t;   // distance from the ray support
-    D3 p;       // object-frame position
-    D3 d;       // object-frame ray direction
-};
 
 // HexagonalPrismZ::is_inside (merlict/scenery/geometry/HexagonalPrismZ.cpp:14-29),
 // unit vectors U = (0, 1, 0), V = (s, c, 0), W = (-s, c, 0)
@@ -159,9 +154,11 @@ MLGPU_HD inline bool differs(const D3& a, const D3& b) {
 }
 
 // calculate_intersection_with of one leaf for a world ray (s, d), in the
-// leaf's object frame (RayAndFrame.cpp:109-116,151-158).
+// leaf's object frame (RayAndFrame.cpp:109-116,151-158).  On a hit, t_out is
+// the distance; the object-frame hit point is at(position_inv(s),
+// orient_inv(d), t_out), bit for bit.
 MLGPU_HD inline bool intersect_leaf(const DeviceScene& sc, const LeafGeo* __restrict__ gp, const D3& ws,
-                                    const D3& wd, LeafHit& h) {
+                                    const D3& wd, double& t_out) {
     const LeafGeo& g = *gp;
     const D3 s = position_inv(g, ws);
     const D3 d = orient_inv(g, wd);
@@ -195,7 +192,6 @@ MLGPU_HD inline bool intersect_leaf(const DeviceScene& sc, const LeafGeo* __rest
         if (!ok) return false;
         const D3 p = at(s, d, t);
         if (!differs(s, p)) return false;
-        h.p = p;
     } else if (kind == MLGPU_KIND_CYLINDER) {
         // ZaxisCylinderRayIntersectionEquation.cpp:6-57, Cylinder.cpp:121-150
         const double dx2 = d.x * d.x, dy2 = d.y * d.y;
@@ -218,7 +214,6 @@ MLGPU_HD inline bool intersect_leaf(const DeviceScene& sc, const LeafGeo* __rest
         const D3 p = at(s, d, t);
         if (!(fabs(p.z) <= g.p[1])) return false;
         if (!differs(s, p)) return false;
-        h.p = p;
     } else if (kind == MLGPU_KIND_SPHERE) {
         // Sphere.cpp:62-105
         const double sd = dot(s, d), d2 = dot(d, d), ss = dot(s, s);
@@ -234,7 +229,6 @@ MLGPU_HD inline bool intersect_leaf(const DeviceScene& sc, const LeafGeo* __rest
         else return false;
         const D3 p = at(s, d, t);
         if (!differs(s, p)) return false;
-        h.p = p;
     } else {
         // XyPlaneRayIntersectionEquation.cpp:6-41 and the planar primitives
         if (d.z == 0. && (d.x != 0. || d.y != 0.)) return false;
@@ -269,10 +263,8 @@ MLGPU_HD inline bool intersect_leaf(const DeviceScene& sc, const LeafGeo* __rest
         }
         if (!in) return false;
         if (!differs(s, p)) return false;
-        h.p = p;
     }
-    h.t = t;
-    h.d = d;
+    t_out = t;
     return true;
 }
 
@@ -289,19 +281,43 @@ MLGPU_HD inline D3 leaf_normal(const LeafGeo& g, const D3& p) {
 
 // --------------------------------------------------------------- traversal
 struct Closest {
-    int32_t leaf;
-    LeafHit hit;
+    int32_t leaf;  // -1: no hit
+    double t;
 };
 
-// First intersection below scope node `scope` (rays_first_intersection_with_frame,
+// Traversal stack of one lane: entry k lives at base[k * stride] (an LDS
+// column per lane on the device, a local array on the host), room for
+// kMaxTreeDepth entries.  An entry >= 0 is a node still to visit; a leaf
+// deferred from node p, child k, is kept as -1 - (2 p + k) so that its box
+// can be tested again, against the best hit found meanwhile, before its
+// FP64 solve.
+struct LaneStack {
+    int32_t* base;
+    int stride;
+};
+
+// FP32 slab test of box [lo, hi] for the ray o + t d given as
+// (1/d, o/d): entered at tmin within [0, limit]?
+MLGPU_HD inline bool slab(const float* lo, const float* hi, const float idx, const float idy, const float idz,
+                          const float oix, const float oiy, const float oiz, const float limit, float& tmin) {
+    const float t1x = fmaf(lo[0], idx, -oix), t2x = fmaf(hi[0], idx, -oix);
+    const float t1y = fmaf(lo[1], idy, -oiy), t2y = fmaf(hi[1], idy, -oiy);
+    const float t1z = fmaf(lo[2], idz, -oiz), t2z = fmaf(hi[2], idz, -oiz);
+    tmin = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fmaxf(fminf(t1z, t2z), 0.0f));
+    const float tmax = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fminf(fmaxf(t1z, t2z), limit));
+    return tmin <= tmax;
+}
+
+// First intersection within scope `scope` (rays_first_intersection_with_frame,
 // RayAndFrame.cpp:118-173).  The boxes only cull: every leaf whose box the
 // ray enters before the best hit so far is solved in FP64, so the closest
-// hit is merlict's.
-MLGPU_HD inline void first_hit(const DeviceScene& sc, int32_t scope, const D3& s, const D3& d, Closest& best) {
+// hit is merlict's.  Nearer children are visited first so that the best hit
+// shrinks early and prunes more boxes.
+MLGPU_HD inline void first_hit(const DeviceScene& sc, const int32_t scope, const D3& s, const D3& d, Closest& best,
+                               const LaneStack stk) {
     best.leaf = -1;
-    best.hit.t = HUGE_VAL;
-    const Node root = sc.nodes[scope];
-    const int32_t end = root.skip;
+    best.t = HUGE_VAL;
+    const Scope root = sc.scopes[scope];
     // direction reciprocals; a zero component becomes tiny so no 0 * inf occurs
     const float fdx = static_cast<float>(d.x), fdy = static_cast<float>(d.y), fdz = static_cast<float>(d.z);
     const float idx = 1.0f / (fabsf(fdx) > 1e-30f ? fdx : copysignf(1e-30f, fdx));
@@ -323,24 +339,51 @@ MLGPU_HD inline void first_hit(const DeviceScene& sc, int32_t scope, const D3& s
     const float oz = static_cast<float>(s.z + d.z * t_adv);
     const float oix = ox * idx, oiy = oy * idy, oiz = oz * idz;
     float best_f = HUGE_VALF;  // best distance relative to the float origin, plus margin
-    int32_t i = scope;
-    while (i < end) {
-        const Node nd = sc.nodes[i];
-        const float t1x = fmaf(nd.lo[0], idx, -oix), t2x = fmaf(nd.hi[0], idx, -oix);
-        const float t1y = fmaf(nd.lo[1], idy, -oiy), t2y = fmaf(nd.hi[1], idy, -oiy);
-        const float t1z = fmaf(nd.lo[2], idz, -oiz), t2z = fmaf(nd.hi[2], idz, -oiz);
-        const float tmin = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fmaxf(fminf(t1z, t2z), 0.0f));
-        const float tmax = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fminf(fmaxf(t1z, t2z), best_f));
-        if (!(tmin <= tmax)) { i = nd.skip; continue; }
-        if (nd.leaf >= 0) {
-            LeafHit h;
-            if (intersect_leaf(sc, sc.geo + nd.leaf, s, d, h) && h.t < best.hit.t) {
-                best.leaf = nd.leaf;
-                best.hit = h;
-                best_f = static_cast<float>(h.t - t_adv) * (1.0f + 1e-6f) + sc.eps;
+    int32_t cur = root.entry;
+    int depth = 0;
+    for (;;) {
+        if (cur >= 0) {
+            const PairNode nd = sc.pairs[cur];
+            float n0, n1;
+            const bool h0 = slab(nd.lo0, nd.hi0, idx, idy, idz, oix, oiy, oiz, best_f, n0) && nd.c0 != kNoChild;
+            const bool h1 = slab(nd.lo1, nd.hi1, idx, idy, idz, oix, oiy, oiz, best_f, n1) && nd.c1 != kNoChild;
+            if (h0 && h1) {
+                const bool near0 = n0 <= n1;
+                const int32_t far = near0 ? nd.c1 : nd.c0;
+                stk.base[depth * stk.stride] = far >= 0 ? far : -1 - (2 * cur + (near0 ? 1 : 0));
+                depth++;
+                cur = near0 ? nd.c0 : nd.c1;
+                continue;
+            }
+            if (h0 || h1) {
+                cur = h0 ? nd.c0 : nd.c1;
+                continue;
+            }
+        } else {
+            const int32_t leaf = -1 - cur;
+            double t;
+            if (intersect_leaf(sc, sc.geo + leaf, s, d, t) && t < best.t) {
+                best.leaf = leaf;
+                best.t = t;
+                best_f = static_cast<float>(t - t_adv) * (1.0f + 1e-6f) + sc.eps;
             }
         }
-        i++;
+        // next deferred entry still in front of the best hit
+        cur = kNoChild;
+        while (depth > 0) {
+            depth--;
+            const int32_t e = stk.base[depth * stk.stride];
+            if (e >= 0) { cur = e; break; }
+            const int32_t q = -1 - e;
+            const PairNode& nd = sc.pairs[q >> 1];
+            const bool second = (q & 1) != 0;
+            float tn;
+            if (slab(second ? nd.lo1 : nd.lo0, second ? nd.hi1 : nd.hi0, idx, idy, idz, oix, oiy, oiz, best_f, tn)) {
+                cur = second ? nd.c1 : nd.c0;
+                break;
+            }
+        }
+        if (cur == kNoChild) break;
     }
 }
 
@@ -350,7 +393,7 @@ MLGPU_HD inline void first_hit(const DeviceScene& sc, int32_t scope, const D3& s
 struct PhotonState {
     D3 s, d;                 // ray support and unit direction, world frame
     double tof;              // running Photon::time_of_flight
-    int32_t root;            // scope node of env.root_frame
+    int32_t root;            // scope of env.root_frame
     uint32_t n_hist;         // history length including production
     uint8_t err;
     uint8_t last_interaction;
@@ -358,11 +401,11 @@ struct PhotonState {
     double out_x, out_y, out_cx, out_cy;
 };
 
-MLGPU_HD inline void photon_init(PhotonState& st, const mlgpu_photons& in, const int64_t i, const int32_t world_node) {
+MLGPU_HD inline void photon_init(PhotonState& st, const mlgpu_photons& in, const int64_t i, const int32_t world_scope) {
     st.s = D3{in.support_x[i], in.support_y[i], in.support_z[i]};
     st.d = normalized(D3{in.direction_x[i], in.direction_y[i], in.direction_z[i]});  // Ray.cpp:16-20
     st.tof = 0.0;
-    st.root = world_node;
+    st.root = world_scope;
     st.n_hist = 1;  // production
     st.err = in.wavelength[i] > 0.0 ? MLGPU_PHOTON_OK : MLGPU_PHOTON_BAD_WAVELENGTH;  // Photon.cpp:20-28
     st.last_interaction = MLGPU_PRODUCTION;
@@ -466,7 +509,7 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
                     interaction = outside_in ? MLGPU_REFRACTION_TO_INSIDE : MLGPU_REFRACTION_TO_OUTSIDE;
                     const bool to_default = (outside_in ? ph.surf[MLGPU_INNER_REFRACTION]
                                                         : ph.surf[MLGPU_OUTER_REFRACTION]) == sc.default_refraction;
-                    new_root = (ph.allowed_node >= 0 && !to_default) ? ph.allowed_node : ph.root_node;
+                    new_root = (ph.allowed_scope >= 0 && !to_default) ? ph.allowed_scope : ph.root_scope;
                     const double k = eta * cosI - cosT;
                     const D3 refr{ld.x * eta + nrm.x * k, ld.y * eta + nrm.y * k, ld.z * eta + nrm.z * k};
                     nd = normalized(orient(g, refr));
@@ -543,19 +586,19 @@ MLGPU_HD inline void photon_finish(const PhotonState& st, const mlgpu_results& o
 
 // Propagator (merlict/PhotonAndFrame.cpp:6-124) for photon i of the batch,
 // all bounces in one lane.
-MLGPU_HD inline void propagate_photon(const DeviceScene& sc, const int32_t world_node, const mlgpu_photons& in,
+MLGPU_HD inline void propagate_photon(const DeviceScene& sc, const int32_t world_scope, const mlgpu_photons& in,
                                       const int64_t i, const uint32_t max_interactions, const uint64_t seed,
-                                      const uint64_t id_offset, const mlgpu_results& out) {
+                                      const uint64_t id_offset, const mlgpu_results& out, const LaneStack stk) {
     PhotonState st;
-    photon_init(st, in, i, world_node);
+    photon_init(st, in, i, world_scope);
     const double lambda = in.wavelength[i];
     Rng rng;
     rng.init(seed, id_offset + static_cast<uint64_t>(i), 0u);
     bool go = st.err == MLGPU_PHOTON_OK && st.n_hist <= max_interactions;
     while (go) {
         Closest c;
-        first_hit(sc, st.root, st.s, st.d, c);
-        go = photon_bounce(sc, st, c.leaf, c.hit.t, lambda, rng, max_interactions);
+        first_hit(sc, st.root, st.s, st.d, c, stk);
+        go = photon_bounce(sc, st, c.leaf, c.t, lambda, rng, max_interactions);
     }
     photon_finish(st, out, i);
 }

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <string>
 #include <stdexcept>
 
 #ifndef M_PI
@@ -89,7 +90,7 @@ M34 pose(const double* pos, int mode, const double* rot) {
 // own sphere tree).
 class ReferencePoses {
  public:
-    ReferencePoses(const mlgpu_scene_desc& d, const std::vector<double>& radius) : d_(d) {
+    ReferencePoses(const mlgpu_scene_desc& d, const std::vector<double>& radius) {
         const int32_t n = d.n_frames;
         f_.resize(n);
         for (int32_t i = 0; i < n; i++) {
@@ -117,7 +118,6 @@ class ReferencePoses {
         double radius;
         std::vector<int32_t> children;
     };
-    const mlgpu_scene_desc& d_;
     std::vector<F> f_;
 
     static double norm3(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
@@ -381,20 +381,40 @@ class Builder {
             if (d_.allowed_frame && d_.allowed_frame[i] >= 0) scope_[d_.allowed_frame[i]] = 1;
         build_functions();
         bnodes_.reserve(2 * n);
-        const int32_t root = build_scope(0);
-        hs_->scope_node.assign(n, -1);
-        emit(root);
-        // scope frames -> emitted node index
-        for (int32_t f = 0; f < n; f++)
-            if (scope_bnode_.count(f)) hs_->scope_node[f] = emitted_[scope_bnode_[f]];
-        hs_->root_node = hs_->scope_node[0];
-        // leaf physics needs the scope nodes
+        const int32_t world = build_scope(0);
+        if (internal(world)) emit_pair(world, 1);
+        // scopes in frame order; a scope whose tree is a single leaf (or
+        // nothing) gets a node of its own to start at
+        hs_->scope_of_frame.assign(n, -1);
+        for (int32_t f = 0; f < n; f++) {
+            if (!scope_[f]) continue;
+            const int32_t r = scope_bnode_.at(f);
+            int32_t entry;
+            if (internal(r)) {
+                entry = pair_of_.at(r);
+            } else {
+                entry = new_pair(1);
+                set_child(entry, 0, r, child_code(r, 2));
+                set_child(entry, 1, -1, kNoChild);
+            }
+            hs_->scope_of_frame[f] = static_cast<int32_t>(hs_->scopes.size());
+            Scope sc{};
+            sc.entry = entry;
+            hs_->scopes.push_back(sc);
+            scope_box_.push_back(bnodes_[r].box);
+        }
+        hs_->root_scope = hs_->scope_of_frame[0];
+        if (hs_->tree_depth > kMaxTreeDepth)
+            throw std::invalid_argument("scene tree is " + std::to_string(hs_->tree_depth) +
+                                        " levels deep, more than the traversal stack holds (" +
+                                        std::to_string(kMaxTreeDepth) + ")");
+        // leaf physics needs the scopes
         for (size_t l = 0; l < hs_->phys.size(); l++) {
             LeafPhys& ph = hs_->phys[l];
             const int32_t fr = ph.frame;
             const int32_t a = d_.allowed_frame ? d_.allowed_frame[fr] : -1;
-            ph.allowed_node = a >= 0 ? hs_->scope_node[a] : -1;
-            ph.root_node = hs_->root_node;
+            ph.allowed_scope = a >= 0 ? hs_->scope_of_frame[a] : -1;
+            ph.root_scope = hs_->root_scope;
         }
         inflate();
     }
@@ -407,7 +427,10 @@ class Builder {
     std::vector<char> scope_;
     std::vector<BNode> bnodes_;
     std::unordered_map<int32_t, int32_t> scope_bnode_;
-    std::unordered_map<int32_t, int32_t> emitted_;
+    std::unordered_map<int32_t, int32_t> pair_of_;   // internal BNode -> PairNode
+    std::unordered_map<int32_t, int32_t> leaf_of_;   // frame -> leaf
+    std::vector<std::array<Box, 2>> pair_box_;       // exact child boxes per PairNode
+    std::vector<Box> scope_box_;
 
     int32_t make_leaf(int32_t f) {
         BNode b;
@@ -463,7 +486,7 @@ class Builder {
 
     // binary tree over items[lo, hi) with the surface-area heuristic, binned
     // along the three axes of the centroid bounds
-    int32_t bvh(std::vector<int32_t>& items, size_t lo, size_t hi) {
+    int32_t bvh(std::vector<int32_t>& items, size_t lo, size_t hi, int depth = 0) {
         if (hi - lo == 1) return items[lo];
         Box cb, all;
         for (size_t i = lo; i < hi; i++) {
@@ -477,7 +500,9 @@ class Builder {
         constexpr int kBins = 32;
         int best_axis = -1, best_split = 0;
         double best_cost = HUGE_VAL;
-        for (int axis = 0; axis < 3 && !cb.empty(); axis++) {
+        // deep down the tree, balanced splits bound the depth
+        const bool balanced = depth >= kBalancedFromDepth;
+        for (int axis = 0; axis < 3 && !cb.empty() && !balanced; axis++) {
             const double ext = cb.hi[axis] - cb.lo[axis];
             if (!(ext > 0.0)) continue;
             Box bins[kBins];
@@ -506,7 +531,19 @@ class Builder {
             }
         }
         size_t mid;
-        if (best_axis < 0) {
+        if (balanced && !cb.empty()) {
+            int axis = 0;
+            for (int k = 1; k < 3; k++)
+                if (cb.hi[k] - cb.lo[k] > cb.hi[axis] - cb.lo[axis]) axis = k;
+            mid = lo + (hi - lo) / 2;
+            std::nth_element(items.begin() + lo, items.begin() + mid, items.begin() + hi, [&](int32_t a, int32_t b) {
+                const Box& ba = bnodes_[a].box;
+                const Box& bb = bnodes_[b].box;
+                const double ca = ba.empty() ? cb.lo[axis] : ba.centre(axis);
+                const double cbb = bb.empty() ? cb.lo[axis] : bb.centre(axis);
+                return ca < cbb;
+            });
+        } else if (best_axis < 0) {
             mid = lo + (hi - lo) / 2;  // coincident centroids: any split
         } else {
             const int axis = best_axis;
@@ -521,8 +558,8 @@ class Builder {
             mid = static_cast<size_t>(it - items.begin());
             if (mid == lo || mid == hi) mid = lo + (hi - lo) / 2;
         }
-        const int32_t left = bvh(items, lo, mid);
-        const int32_t right = bvh(items, mid, hi);
+        const int32_t left = bvh(items, lo, mid, depth + 1);
+        const int32_t right = bvh(items, mid, hi, depth + 1);
         BNode b;
         b.box = bnodes_[left].box;
         b.box.grow(bnodes_[right].box);
@@ -531,22 +568,43 @@ class Builder {
         return static_cast<int32_t>(bnodes_.size()) - 1;
     }
 
-    // depth-first emission with skip indices
-    void emit(int32_t bi) {
-        const int32_t idx = static_cast<int32_t>(hs_->nodes.size());
-        emitted_[bi] = idx;
-        Node nd;
+    static constexpr int kBalancedFromDepth = 20;
+
+    bool internal(int32_t bi) const { return !bnodes_[bi].kids.empty(); }
+
+    int32_t new_pair(int depth) {
+        hs_->pairs.push_back(PairNode{});
+        pair_box_.push_back({Box(), Box()});
+        hs_->tree_depth = std::max(hs_->tree_depth, depth);
+        return static_cast<int32_t>(hs_->pairs.size()) - 1;
+    }
+
+    void set_child(int32_t pair, int k, int32_t bi, int32_t code) {
+        PairNode& p = hs_->pairs[pair];
+        (k == 0 ? p.c0 : p.c1) = code;
+        pair_box_[pair][k] = (code == kNoChild || bi < 0) ? Box() : bnodes_[bi].box;
+    }
+
+    // code of BNode bi as a child: a leaf, nothing, or its (emitted) node
+    int32_t child_code(int32_t bi, int depth) {
         const BNode& b = bnodes_[bi];
-        nd.leaf = -1;
-        hs_->nodes.push_back(nd);
-        hs_->node_box.push_back({b.box.lo[0], b.box.lo[1], b.box.lo[2], b.box.hi[0], b.box.hi[1], b.box.hi[2]});
         if (b.leaf_frame >= 0) {
-            hs_->nodes[idx].leaf = add_leaf(b.leaf_frame);
-        } else {
-            const std::vector<int32_t> kids = b.kids;
-            for (int32_t k : kids) emit(k);
+            if (d_.kind[b.leaf_frame] == MLGPU_KIND_FRAME) return kNoChild;  // never reports a hit
+            auto it = leaf_of_.find(b.leaf_frame);
+            const int32_t leaf = it != leaf_of_.end() ? it->second : (leaf_of_[b.leaf_frame] = add_leaf(b.leaf_frame));
+            return -1 - leaf;
         }
-        hs_->nodes[idx].skip = static_cast<int32_t>(hs_->nodes.size());
+        if (b.kids.empty()) return kNoChild;
+        return emit_pair(bi, depth);
+    }
+
+    // depth-first emission: a node's subtree follows it in memory
+    int32_t emit_pair(int32_t bi, int depth) {
+        const int32_t idx = new_pair(depth);
+        pair_of_[bi] = idx;
+        const std::vector<int32_t> kids = bnodes_[bi].kids;
+        for (int k = 0; k < 2; k++) set_child(idx, k, kids[k], child_code(kids[k], depth + 1));
+        return idx;
     }
 
     int32_t add_leaf(int32_t f) {
@@ -572,8 +630,8 @@ class Builder {
             }
             ph.surf[s] = fn;
         }
-        ph.allowed_node = -1;
-        ph.root_node = -1;
+        ph.allowed_scope = -1;
+        ph.root_scope = -1;
         ph.frame = f;
         ph.sensor = d_.sensor_id ? d_.sensor_id[f] : -1;
         ph.pad0 = ph.pad1 = 0;
@@ -639,28 +697,31 @@ class Builder {
     // arithmetic rounds, both bounded by a few ulp of the scene's extent.
     void inflate() {
         double extent = 1.0;
-        for (const auto& b : hs_->node_box)
-            for (int k = 0; k < 6; k++)
-                if (std::isfinite(b[k])) extent = std::max(extent, std::fabs(b[k]));
+        for (const Box& b : scope_box_)
+            if (!b.empty())
+                for (int k = 0; k < 3; k++) extent = std::max({extent, std::fabs(b.lo[k]), std::fabs(b.hi[k])});
         const double eps = 4e-6 * extent;
         hs_->eps = eps;
-        for (size_t i = 0; i < hs_->nodes.size(); i++) {
-            const auto& b = hs_->node_box[i];
-            Node& nd = hs_->nodes[i];
-            if (!(b[0] <= b[3])) {
-                // nothing inside (a scope without surfaces): a point box
-                nd.lo[0] = nd.lo[1] = nd.lo[2] = 0.0f;
-                nd.hi[0] = nd.hi[1] = nd.hi[2] = 0.0f;
-                continue;
+        auto put = [eps](const Box& b, float* lo, float* hi) {
+            if (b.empty()) {
+                // nothing inside: a point box (its code says there is no child)
+                for (int k = 0; k < 3; k++) lo[k] = hi[k] = 0.0f;
+                return;
             }
             for (int k = 0; k < 3; k++) {
-                float lo = static_cast<float>(b[k] - eps), hi = static_cast<float>(b[3 + k] + eps);
-                if (static_cast<double>(lo) > b[k] - eps) lo = std::nextafter(lo, -HUGE_VALF);
-                if (static_cast<double>(hi) < b[3 + k] + eps) hi = std::nextafter(hi, HUGE_VALF);
-                nd.lo[k] = lo;
-                nd.hi[k] = hi;
+                float l = static_cast<float>(b.lo[k] - eps), h = static_cast<float>(b.hi[k] + eps);
+                if (static_cast<double>(l) > b.lo[k] - eps) l = std::nextafter(l, -HUGE_VALF);
+                if (static_cast<double>(h) < b.hi[k] + eps) h = std::nextafter(h, HUGE_VALF);
+                lo[k] = l;
+                hi[k] = h;
             }
+        };
+        for (size_t i = 0; i < hs_->pairs.size(); i++) {
+            PairNode& p = hs_->pairs[i];
+            put(pair_box_[i][0], p.lo0, p.hi0);
+            put(pair_box_[i][1], p.lo1, p.hi1);
         }
+        for (size_t i = 0; i < hs_->scopes.size(); i++) put(scope_box_[i], hs_->scopes[i].lo, hs_->scopes[i].hi);
     }
 };
 

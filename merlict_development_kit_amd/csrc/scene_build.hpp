@@ -12,15 +12,16 @@
 namespace mlgpu {
 
 struct HostScene {
-    std::vector<Node> nodes;
-    std::vector<std::array<double, 6>> node_box;     // exact (double) lo xyz, hi xyz
+    std::vector<PairNode> pairs;
+    std::vector<Scope> scopes;
+    int32_t tree_depth = 0;                          // longest chain of pair nodes
     std::vector<LeafGeo> geo;
     std::vector<LeafPhys> phys;
     std::vector<int32_t> leaf_frame;
     std::vector<FuncHead> funcs;
     std::vector<FuncPoint> points;
-    std::vector<int32_t> scope_node;  // per description frame, -1 if not a scope
-    int32_t root_node = -1;
+    std::vector<int32_t> scope_of_frame;  // per description frame, -1 if not a scope
+    int32_t root_scope = -1;
     int32_t default_refraction = -1;
     double eps = 0.0;
     double hex_sin = 0.0, hex_cos = 0.0;
