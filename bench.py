@@ -73,6 +73,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--schedule", default="auto", choices=["auto", "fused", "wavefront"])
+    ap.add_argument("--trace-budget", type=int, default=None)
+    ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy a float32 buffer of this size (a plain copy kernel of known "
                          "traffic, for scaling rocprof's FETCH_SIZE / WRITE_SIZE)")
@@ -90,7 +92,7 @@ def main() -> int:
         del src, dst
     desc = SceneDescription.fact(light_guides=not args.no_light_guides)
     scene = GpuScene(desc)
-    scene.set_schedule(args.schedule)
+    scene.set_tuning(schedule=args.schedule, trace_budget=args.trace_budget, blocks_per_cu=args.blocks_per_cu)
     n = args.photons
     photons = sources.fact_bunch(n, seed=2024, device=dev)
     out = PropagationResult.empty(n, dev)
@@ -134,7 +136,7 @@ def main() -> int:
             "photons_per_step": n,
             "leaves": scene.stats()["leaves"],
             "max_num_interactions_per_photon": 5,
-            "schedule": args.schedule,
+            "tuning": scene.tuning(),
             "detected_fraction": detected / n,
             "photon_errors": errors,
         },

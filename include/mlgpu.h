@@ -201,16 +201,31 @@ int mlgpu_scene_stats(const mlgpu_scene* scene, int64_t* num_nodes,
 
 /* Kernel schedule of mlgpu_propagate:
  *   FUSED      one lane carries a photon through all its bounces;
- *   WAVEFRONT  per bounce level a trace kernel and a shade kernel over a
+ *   WAVEFRONT  per bounce level a trace pass and a shade kernel over a
  *              compacted queue of live photons (workspace owned by the scene,
- *              ~90 B per photon; calls sharing a scene are serialised on it);
+ *              about 210 B per photon; calls sharing a scene are serialised
+ *              on it);
  *   AUTO       WAVEFRONT from 65,536 photons on. */
 enum mlgpu_schedule {
     MLGPU_SCHEDULE_AUTO = 0,
     MLGPU_SCHEDULE_FUSED = 1,
     MLGPU_SCHEDULE_WAVEFRONT = 2
 };
-int mlgpu_scene_set_schedule(mlgpu_scene* scene, int schedule);
+
+/* Launch parameters of mlgpu_propagate; they never change results.
+ *   schedule       an mlgpu_schedule
+ *   trace_budget   WAVEFRONT: traversal steps a ray gets in the first trace
+ *                  pass of a level; the unfinished rays resume, packed, in a
+ *                  second pass.  <= 0: one pass.
+ *   blocks_per_cu  WAVEFRONT: workgroups per compute unit of the grid-stride
+ *                  kernels, 1 to 64. */
+typedef struct mlgpu_tuning {
+    int32_t schedule;
+    int32_t trace_budget;
+    int32_t blocks_per_cu;
+} mlgpu_tuning;
+int mlgpu_scene_set_tuning(mlgpu_scene* scene, const mlgpu_tuning* tuning);
+int mlgpu_scene_get_tuning(const mlgpu_scene* scene, mlgpu_tuning* tuning);
 
 /* Propagate n photons, one Philox stream per photon.  Replaces
  * merlict::propagate_photons_in_frame_with_config (merlict/Photons.cpp:9-22)

@@ -89,6 +89,14 @@ class Config(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("schedule", C.c_int32),
+        ("trace_budget", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+    ]
+
+
 class Hits(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in ("frame", "distance", "position_x", "position_y", "position_z")]
 
@@ -107,7 +115,8 @@ SIGNATURES = {
                                                                    C.POINTER(C.c_void_p)]),
     "mlgpu_scene_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     "mlgpu_scene_destroy": (None, [C.c_void_p]),
-    "mlgpu_scene_set_schedule": (C.c_int, [C.c_void_p, C.c_int]),
+    "mlgpu_scene_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "mlgpu_scene_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "mlgpu_scene_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlgpu_propagate": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
                                   C.POINTER(Results), C.c_void_p]),

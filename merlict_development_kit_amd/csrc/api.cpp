@@ -19,7 +19,7 @@ struct mlgpu_scene {
     int64_t device_bytes = 0;
     int device = 0;
     int num_cus = 0;
-    int schedule = MLGPU_SCHEDULE_AUTO;
+    mlgpu_tuning tuning{MLGPU_SCHEDULE_AUTO, 24, 32};
     // wavefront workspace, grown on demand.  Calls that use it are ordered
     // on the device: a call on another stream than the previous user first
     // waits for the event recorded after that user's kernels.
@@ -86,7 +86,8 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
         s->num_cus = 256;
     int rc = MLGPU_OK;
     if ((rc = upload(s, hs.pairs, &s->dev.pairs)) || (rc = upload(s, hs.scopes, &s->dev.scopes)) ||
-        (rc = upload(s, hs.geo, &s->dev.geo)) || (rc = upload(s, hs.phys, &s->dev.phys)) || (rc = upload(s, hs.funcs, &s->dev.funcs)) ||
+        (rc = upload(s, hs.geo, &s->dev.geo)) || (rc = upload(s, hs.boxes, &s->dev.boxes)) ||
+        (rc = upload(s, hs.phys, &s->dev.phys)) || (rc = upload(s, hs.funcs, &s->dev.funcs)) ||
         (rc = upload(s, hs.points, &s->dev.points))) {
         mlgpu_scene_destroy(s);
         return rc;
@@ -116,12 +117,23 @@ void mlgpu_scene_destroy(mlgpu_scene* s) {
     delete s;
 }
 
-int mlgpu_scene_set_schedule(mlgpu_scene* s, int schedule) {
-    if (!s || schedule < MLGPU_SCHEDULE_AUTO || schedule > MLGPU_SCHEDULE_WAVEFRONT) {
+int mlgpu_scene_set_tuning(mlgpu_scene* s, const mlgpu_tuning* t) {
+    if (!s || !t) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (t->schedule < MLGPU_SCHEDULE_AUTO || t->schedule > MLGPU_SCHEDULE_WAVEFRONT) {
         set_last_error("bad schedule");
         return MLGPU_ERR_ARGUMENT;
     }
-    s->schedule = schedule;
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) {
+        set_last_error("blocks_per_cu must be in [1, 64]");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    s->tuning = *t;
+    return MLGPU_OK;
+}
+
+int mlgpu_scene_get_tuning(const mlgpu_scene* s, mlgpu_tuning* t) {
+    if (!s || !t) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    *t = s->tuning;
     return MLGPU_OK;
 }
 
@@ -153,7 +165,7 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
         return MLGPU_ERR_ARGUMENT;
     }
     if (n > INT32_MAX) { set_last_error("at most 2^31 - 1 photons per call"); return MLGPU_ERR_ARGUMENT; }
-    int schedule = s->schedule;
+    int schedule = s->tuning.schedule;
     if (schedule == MLGPU_SCHEDULE_AUTO) schedule = n >= 65536 ? MLGPU_SCHEDULE_WAVEFRONT : MLGPU_SCHEDULE_FUSED;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e;
@@ -187,6 +199,7 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
         const size_t cap = static_cast<size_t>(ms->ws_capacity);
         auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) & ~static_cast<size_t>(255); return q; };
         mlgpu::Workspace w;
+        w.cap = ms->ws_capacity;
         w.sx = reinterpret_cast<double*>(take(8 * cap));
         w.sy = reinterpret_cast<double*>(take(8 * cap));
         w.sz = reinterpret_cast<double*>(take(8 * cap));
@@ -199,12 +212,17 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
         w.draws = reinterpret_cast<uint32_t*>(take(4 * cap));
         w.hit_leaf = reinterpret_cast<int32_t*>(take(4 * cap));
         w.hit_t = reinterpret_cast<double*>(take(8 * cap));
+        w.tr_cur = reinterpret_cast<int32_t*>(take(4 * cap));
+        w.tr_depth = reinterpret_cast<int32_t*>(take(4 * cap));
+        w.tr_stack = reinterpret_cast<int32_t*>(take(4 * cap * mlgpu::kMaxTreeDepth));
+        w.deferred = reinterpret_cast<int32_t*>(take(4 * cap));
         w.queue[0] = reinterpret_cast<int32_t*>(take(4 * cap));
         w.queue[1] = reinterpret_cast<int32_t*>(take(4 * cap));
-        w.counts = reinterpret_cast<int32_t*>(take(4 * 256));
+        w.counts = reinterpret_cast<int32_t*>(take(4 * mlgpu::kNumCounts));
         e = mlgpu::launch_propagate_wavefront(s->dev, s->scope_of_frame[world_frame], *photons, n,
                                               cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset,
-                                              *r, w, 8 * s->num_cus, st);
+                                              *r, w, s->tuning.blocks_per_cu * s->num_cus,
+                                              s->tuning.trace_budget, st);
         if (e == hipSuccess) e = hipEventRecord(ms->ws_event, st);
         ms->ws_stream = st;
     }

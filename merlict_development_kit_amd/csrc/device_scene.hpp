@@ -46,7 +46,7 @@ struct alignas(16) Scope {
 
 // Deepest node chain a scene may have: the traversal stack holds at most
 // one entry per level.
-constexpr int kMaxTreeDepth = 32;
+constexpr int kMaxTreeDepth = 24;
 
 enum LeafKind : int32_t {
     LK_CAP_HEX = 1,
@@ -74,6 +74,17 @@ struct alignas(16) LeafGeo {
     double p[8];
     int32_t kind;
     int32_t pad0;
+};
+
+// Float oriented box of a leaf (64 B): a point x of the world lies in it when
+// |m (x - c)|_k <= h_k for k = x, y, z.  It holds every point where the leaf
+// can report a hit, inflated like the tree's boxes, so a ray that misses it
+// (or meets it only behind the best hit so far) needs no FP64 solve.
+struct alignas(16) LeafBox {
+    float m[9];  // world -> object rotation, row major
+    float c[3];  // world centre
+    float h[3];  // half extents
+    float pad;
 };
 
 // Physics of a leaf, read once per bounce (48 B).
@@ -112,6 +123,7 @@ struct DeviceScene {
     const PairNode* pairs;
     const Scope* scopes;
     const LeafGeo* geo;
+    const LeafBox* boxes;
     const LeafPhys* phys;
     const FuncHead* funcs;
     const FuncPoint* points;

@@ -8,23 +8,32 @@
 
 namespace mlgpu {
 
-// Device workspace of the wavefront schedule, all arrays of length n (the
-// photon index) except the per-slot hit record and the two queues.
+// Device workspace of the wavefront schedule.  Photon state is indexed by
+// photon (length cap), trace records by queue slot (length cap).
 struct Workspace {
+    int64_t cap;
     double *sx, *sy, *sz, *dx, *dy, *dz, *tof;
     int32_t* root;
     uint32_t* n_hist;
     uint32_t* draws;
-    int32_t* hit_leaf;
-    double* hit_t;
+    int32_t* hit_leaf;   // best leaf so far per slot
+    double* hit_t;       // its distance
+    int32_t* tr_cur;     // traversal position of a deferred slot
+    int32_t* tr_depth;   // its stack height
+    int32_t* tr_stack;   // its stack, entry k at [k * cap + slot]
+    int32_t* deferred;   // slots whose traversal goes on in the second pass
     int32_t* queue[2];
-    int32_t* counts;  // live photons per bounce level
+    int32_t* counts;     // [level]: live photons; [kDeferredCounts + level]: deferred slots
 };
 
-// bytes of workspace for n photons
-inline size_t workspace_bytes(int64_t n) {
-    const size_t m = static_cast<size_t>(n);
-    return m * (7 * 8 + 4 + 4 + 4 + 4 + 8 + 4 + 4) + 256 * 4 + 16 * 256 + 1024;
+constexpr int kDeferredCounts = 256;
+constexpr int kNumCounts = 2 * kDeferredCounts;
+
+// bytes of workspace for cap photons (each array 256-byte aligned)
+inline size_t workspace_bytes(int64_t cap) {
+    const size_t m = static_cast<size_t>(cap);
+    const size_t per = 7 * 8 + 4 + 4 + 4 + 4 + 8 + 4 + 4 + 4 * kMaxTreeDepth + 4 + 4 + 4;
+    return m * per + 4 * kNumCounts + 32 * 256;
 }
 
 hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in, int64_t n,
@@ -34,7 +43,7 @@ hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const ml
 hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& out, const Workspace& ws, int grid_blocks,
-                                      hipStream_t stream);
+                                      int32_t trace_budget, hipStream_t stream);
 
 hipError_t launch_first_hit(const DeviceScene& sc, int32_t root_scope, const mlgpu_photons& rays, int64_t n,
                             const mlgpu_hits& out, hipStream_t stream);

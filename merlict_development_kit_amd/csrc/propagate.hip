@@ -54,14 +54,16 @@ __global__ __launch_bounds__(kBlock) void propagate_kernel(const DeviceScene sc,
 
 constexpr int kTile = 8 * kBlock;
 
+template <int Cap>
 struct TileQueue {
-    int32_t buf[kTile];
+    int32_t buf[Cap];
     int32_t n;
     int32_t flush_n;
     int32_t flush_base;
 };
 
-__device__ __forceinline__ void tile_push(TileQueue& q, const bool want, const int32_t value) {
+template <int Cap>
+__device__ __forceinline__ void tile_push(TileQueue<Cap>& q, const bool want, const int32_t value) {
     const uint64_t mask = __ballot(want);
     if (mask == 0) return;
     const int lane = static_cast<int>(threadIdx.x & 63u);
@@ -73,7 +75,8 @@ __device__ __forceinline__ void tile_push(TileQueue& q, const bool want, const i
 }
 
 // Block-uniform: every thread of the block calls it once per tile.
-__device__ __forceinline__ void tile_flush(TileQueue& q, int32_t* queue, int32_t* count) {
+template <int Cap>
+__device__ __forceinline__ void tile_flush(TileQueue<Cap>& q, int32_t* queue, int32_t* count) {
     __syncthreads();  // all pushes of this tile are in q
     if (threadIdx.x == 0) {
         const int32_t m = q.n;
@@ -88,14 +91,13 @@ __device__ __forceinline__ void tile_flush(TileQueue& q, int32_t* queue, int32_t
 }
 
 __global__ void wf_reset_kernel(int32_t* counts, int n) {
-    const int i = static_cast<int>(threadIdx.x);
-    if (i < n) counts[i] = 0;
+    for (int i = static_cast<int>(threadIdx.x); i < n; i += static_cast<int>(blockDim.x)) counts[i] = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void wf_init_kernel(const int32_t world_scope, const mlgpu_photons in,
                                                          const int64_t n, const uint32_t max_interactions,
                                                          const Workspace ws, const mlgpu_results out) {
-    __shared__ TileQueue q;
+    __shared__ TileQueue<kTile> q;
     if (threadIdx.x == 0) q.n = 0;
     __syncthreads();
     for (int64_t tile = static_cast<int64_t>(blockIdx.x) * kTile; tile < n;
@@ -124,7 +126,13 @@ __global__ __launch_bounds__(kBlock) void wf_init_kernel(const int32_t world_sco
     }
 }
 
-__global__ __launch_bounds__(kBlock) void wf_trace_kernel(const DeviceScene sc, const Workspace ws, const int level) {
+// First pass of a level's traces: every query gets `budget` steps, so no
+// wave waits long on a few costly rays; the unfinished ones keep their
+// traversal state (tr_depth >= 0 marks them) and go on in wf_resume_kernel,
+// packed into full waves by wf_defer_kernel.  No barriers here: a block
+// barrier would make every wave wait for the slowest of its block.
+__global__ __launch_bounds__(kBlock, 5) void wf_trace_kernel(const DeviceScene sc, const Workspace ws, const int level,
+                                                             const int32_t budget) {
     MLGPU_LANE_STACK(stk);
     const int32_t count = ws.counts[level];
     const int32_t* queue = ws.queue[level & 1];
@@ -133,10 +141,64 @@ __global__ __launch_bounds__(kBlock) void wf_trace_kernel(const DeviceScene sc, 
         const int32_t i = queue[slot];
         const D3 s{ws.sx[i], ws.sy[i], ws.sz[i]};
         const D3 d{ws.dx[i], ws.dy[i], ws.dz[i]};
-        Closest c;
-        first_hit(sc, ws.root[i], s, d, c, stk);
-        ws.hit_leaf[slot] = c.leaf;
-        ws.hit_t[slot] = c.t;
+        const int32_t scope = ws.root[i];
+        RayF r;
+        ray_setup(sc, scope, s, d, r);
+        TraceState ts;
+        trace_begin(sc, scope, ts);
+        const bool done = trace_run<true>(sc, r, s, d, ts, stk, budget);
+        ws.hit_leaf[slot] = ts.best.leaf;
+        ws.hit_t[slot] = ts.best.t;
+        if (done) {
+            ws.tr_depth[slot] = -1;
+        } else {
+            ws.tr_cur[slot] = ts.cur;
+            ws.tr_depth[slot] = ts.depth;
+            for (int e = 0; e < ts.depth; e++)
+                ws.tr_stack[static_cast<int64_t>(e) * ws.cap + slot] = stk.base[e * stk.stride];
+        }
+    }
+}
+
+// The list of a level's deferred slots.
+__global__ __launch_bounds__(kBlock) void wf_defer_kernel(const Workspace ws, const int level) {
+    __shared__ TileQueue<kTile> q;
+    if (threadIdx.x == 0) q.n = 0;
+    __syncthreads();
+    const int32_t count = ws.counts[level];
+    for (int64_t tile = static_cast<int64_t>(blockIdx.x) * kTile; tile < count;
+         tile += static_cast<int64_t>(gridDim.x) * kTile) {
+        for (int k = 0; k < kTile; k += kBlock) {
+            const int32_t slot = static_cast<int32_t>(tile + k + threadIdx.x);
+            tile_push(q, slot < count && ws.tr_depth[slot] >= 0, slot);
+        }
+        tile_flush(q, ws.deferred, ws.counts + kDeferredCounts + level);
+    }
+}
+
+// Second pass: the deferred traversals of a level, run to the end.
+__global__ __launch_bounds__(kBlock) void wf_resume_kernel(const DeviceScene sc, const Workspace ws, const int level) {
+    MLGPU_LANE_STACK(stk);
+    const int32_t count = ws.counts[kDeferredCounts + level];
+    const int32_t* queue = ws.queue[level & 1];
+    for (int32_t j = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); j < count;
+         j += static_cast<int32_t>(gridDim.x * blockDim.x)) {
+        const int32_t slot = ws.deferred[j];
+        const int32_t i = queue[slot];
+        const D3 s{ws.sx[i], ws.sy[i], ws.sz[i]};
+        const D3 d{ws.dx[i], ws.dy[i], ws.dz[i]};
+        RayF r;
+        ray_setup(sc, ws.root[i], s, d, r);
+        TraceState ts;
+        ts.cur = ws.tr_cur[slot];
+        ts.depth = ws.tr_depth[slot];
+        ts.best.leaf = ws.hit_leaf[slot];
+        ts.best.t = ws.hit_t[slot];
+        for (int e = 0; e < ts.depth; e++)
+            stk.base[e * stk.stride] = ws.tr_stack[static_cast<int64_t>(e) * ws.cap + slot];
+        trace_run<false>(sc, r, s, d, ts, stk, 0);
+        ws.hit_leaf[slot] = ts.best.leaf;
+        ws.hit_t[slot] = ts.best.t;
     }
 }
 
@@ -144,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
                                                           const Workspace ws, const int level,
                                                           const uint32_t max_interactions, const uint64_t seed,
                                                           const uint64_t id_offset, const mlgpu_results out) {
-    __shared__ TileQueue q;
+    __shared__ TileQueue<kTile> q;
     if (threadIdx.x == 0) q.n = 0;
     __syncthreads();
     const int32_t count = ws.counts[level];
@@ -237,10 +299,11 @@ hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const ml
 hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& out, const Workspace& ws, int grid_blocks,
-                                      hipStream_t stream) {
+                                      int32_t trace_budget, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const int levels = static_cast<int>(max_interactions) + 1;
-    hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(256), 0, stream, ws.counts, levels + 1);
+    if (levels + 1 > kDeferredCounts) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(256), 0, stream, ws.counts, kNumCounts);
     const int64_t tiles = (n + kTile - 1) / kTile;
     const int tile_grid = static_cast<int>(tiles < grid_blocks ? tiles : grid_blocks);
     const int64_t blocks = (n + kBlock - 1) / kBlock;
@@ -248,7 +311,12 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
     hipLaunchKernelGGL(wf_init_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, world_scope, in, n,
                        max_interactions, ws, out);
     for (int level = 0; level < static_cast<int>(max_interactions); level++) {
-        hipLaunchKernelGGL(wf_trace_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, ws, level);
+        const int32_t budget = trace_budget > 0 ? trace_budget : INT32_MAX;
+        hipLaunchKernelGGL(wf_trace_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, ws, level, budget);
+        if (trace_budget > 0) {
+            hipLaunchKernelGGL(wf_defer_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, ws, level);
+            hipLaunchKernelGGL(wf_resume_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, ws, level);
+        }
         hipLaunchKernelGGL(wf_shade_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, sc, in, ws, level,
                            max_interactions, seed, id_offset, out);
     }

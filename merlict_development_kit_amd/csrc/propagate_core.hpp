@@ -287,10 +287,12 @@ struct Closest {
 
 // Traversal stack of one lane: entry k lives at base[k * stride] (an LDS
 // column per lane on the device, a local array on the host), room for
-// kMaxTreeDepth entries.  An entry >= 0 is a node still to visit; a leaf
-// deferred from node p, child k, is kept as -1 - (2 p + k) so that its box
-// can be tested again, against the best hit found meanwhile, before its
+// kMaxTreeDepth entries.  An entry is a node index (>= 0) or a leaf deferred
+// from node p, child k, kept as -1 - (kDeferredLeaf | (2 p + k)) so that its
+// box can be tested again, against the best hit found meanwhile, before the
 // FP64 solve.
+constexpr int32_t kDeferredLeaf = 0x40000000;
+
 struct LaneStack {
     int32_t* base;
     int stride;
@@ -308,49 +310,110 @@ MLGPU_HD inline bool slab(const float* lo, const float* hi, const float idx, con
     return tmin <= tmax;
 }
 
-// First intersection within scope `scope` (rays_first_intersection_with_frame,
-// RayAndFrame.cpp:118-173).  The boxes only cull: every leaf whose box the
-// ray enters before the best hit so far is solved in FP64, so the closest
-// hit is merlict's.  Nearer children are visited first so that the best hit
-// shrinks early and prunes more boxes.
-MLGPU_HD inline void first_hit(const DeviceScene& sc, const int32_t scope, const D3& s, const D3& d, Closest& best,
-                               const LaneStack stk) {
-    best.leaf = -1;
-    best.t = HUGE_VAL;
+// FP32 test of the oriented box of a leaf for the ray o + t d: does the ray
+// meet it within [0, limit]?
+MLGPU_HD inline bool leaf_box_hit(const LeafBox& b, const float ox, const float oy, const float oz, const float dx,
+                                  const float dy, const float dz, const float limit) {
+    const float px = ox - b.c[0], py = oy - b.c[1], pz = oz - b.c[2];
+    float tmin = 0.0f, tmax = limit;
+    for (int k = 0; k < 3; k++) {
+        const float* r = b.m + 3 * k;
+        const float lo = fmaf(r[0], px, fmaf(r[1], py, r[2] * pz));
+        const float ld = fmaf(r[0], dx, fmaf(r[1], dy, r[2] * dz));
+        const float inv = 1.0f / (fabsf(ld) > 1e-30f ? ld : copysignf(1e-30f, ld));
+        const float t1 = (-b.h[k] - lo) * inv, t2 = (b.h[k] - lo) * inv;
+        tmin = fmaxf(tmin, fminf(t1, t2));
+        tmax = fminf(tmax, fmaxf(t1, t2));
+    }
+    return tmin <= tmax;
+}
+
+// The float form of a ray for the box tests: origin o (the support, moved
+// along the ray to the scope box when it starts far outside), direction,
+// reciprocals and o / d.  t_adv is how far the origin was moved.
+struct RayF {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float idx, idy, idz;
+    float oix, oiy, oiz;
+    double t_adv;
+};
+
+MLGPU_HD inline void ray_setup(const DeviceScene& sc, const int32_t scope, const D3& s, const D3& d, RayF& r) {
     const Scope root = sc.scopes[scope];
     // direction reciprocals; a zero component becomes tiny so no 0 * inf occurs
-    const float fdx = static_cast<float>(d.x), fdy = static_cast<float>(d.y), fdz = static_cast<float>(d.z);
-    const float idx = 1.0f / (fabsf(fdx) > 1e-30f ? fdx : copysignf(1e-30f, fdx));
-    const float idy = 1.0f / (fabsf(fdy) > 1e-30f ? fdy : copysignf(1e-30f, fdy));
-    const float idz = 1.0f / (fabsf(fdz) > 1e-30f ? fdz : copysignf(1e-30f, fdz));
+    r.dx = static_cast<float>(d.x);
+    r.dy = static_cast<float>(d.y);
+    r.dz = static_cast<float>(d.z);
+    r.idx = 1.0f / (fabsf(r.dx) > 1e-30f ? r.dx : copysignf(1e-30f, r.dx));
+    r.idy = 1.0f / (fabsf(r.dy) > 1e-30f ? r.dy : copysignf(1e-30f, r.dy));
+    r.idz = 1.0f / (fabsf(r.dz) > 1e-30f ? r.dz : copysignf(1e-30f, r.dz));
     // a support far outside the scope box is advanced (in FP64) to the box so
     // the float origin keeps the precision the box inflation assumes
     double t_adv = 0.0;
     {
-        const double ax = (static_cast<double>(root.lo[0]) - s.x) * idx, bx = (static_cast<double>(root.hi[0]) - s.x) * idx;
-        const double ay = (static_cast<double>(root.lo[1]) - s.y) * idy, by = (static_cast<double>(root.hi[1]) - s.y) * idy;
-        const double az = (static_cast<double>(root.lo[2]) - s.z) * idz, bz = (static_cast<double>(root.hi[2]) - s.z) * idz;
+        const double ax = (static_cast<double>(root.lo[0]) - s.x) * r.idx, bx = (static_cast<double>(root.hi[0]) - s.x) * r.idx;
+        const double ay = (static_cast<double>(root.lo[1]) - s.y) * r.idy, by = (static_cast<double>(root.hi[1]) - s.y) * r.idy;
+        const double az = (static_cast<double>(root.lo[2]) - s.z) * r.idz, bz = (static_cast<double>(root.hi[2]) - s.z) * r.idz;
         const double enter = fmax(fmax(fmin(ax, bx), fmin(ay, by)), fmin(az, bz));
         if (enter > 0.0) t_adv = enter * (1.0 - 1e-6) - sc.eps;
         if (t_adv < 0.0) t_adv = 0.0;
     }
-    const float ox = static_cast<float>(s.x + d.x * t_adv);
-    const float oy = static_cast<float>(s.y + d.y * t_adv);
-    const float oz = static_cast<float>(s.z + d.z * t_adv);
-    const float oix = ox * idx, oiy = oy * idy, oiz = oz * idz;
-    float best_f = HUGE_VALF;  // best distance relative to the float origin, plus margin
-    int32_t cur = root.entry;
-    int depth = 0;
+    r.t_adv = t_adv;
+    r.ox = static_cast<float>(s.x + d.x * t_adv);
+    r.oy = static_cast<float>(s.y + d.y * t_adv);
+    r.oz = static_cast<float>(s.z + d.z * t_adv);
+    r.oix = r.ox * r.idx;
+    r.oiy = r.oy * r.idy;
+    r.oiz = r.oz * r.idz;
+}
+
+// Where a traversal stands: the next node or leaf code to visit (kNoChild:
+// finished), the number of stack entries and the best hit so far.
+struct TraceState {
+    int32_t cur;
+    int32_t depth;
+    Closest best;
+};
+
+MLGPU_HD inline void trace_begin(const DeviceScene& sc, const int32_t scope, TraceState& ts) {
+    ts.cur = sc.scopes[scope].entry;
+    ts.depth = 0;
+    ts.best.leaf = -1;
+    ts.best.t = HUGE_VAL;
+}
+
+// Runs the traversal on, for at most `budget` steps (node visits and leaf
+// tests) when Budgeted; returns true when it has finished.  A
+// traversal stopped early resumes from ts and the stack, and ends with the
+// same closest hit.  The boxes only cull: every leaf whose box the ray
+// enters before the best hit so far is solved in FP64, so the closest hit is
+// merlict's (rays_first_intersection_with_frame, RayAndFrame.cpp:118-173).
+// Nearer children are visited first so that the best hit shrinks early and
+// prunes more boxes.
+template <bool Budgeted>
+MLGPU_HD inline bool trace_run(const DeviceScene& sc, const RayF& r, const D3& s, const D3& d, TraceState& ts,
+                               const LaneStack stk, int32_t budget) {
+    // best distance relative to the float origin, plus margin
+    float best_f = ts.best.leaf >= 0 ? static_cast<float>(ts.best.t - r.t_adv) * (1.0f + 1e-6f) + sc.eps : HUGE_VALF;
+    int32_t cur = ts.cur;
+    int depth = ts.depth;
+    if (cur == kNoChild) return true;
     for (;;) {
+        if (Budgeted && budget-- <= 0) {
+            ts.cur = cur;
+            ts.depth = depth;
+            return false;
+        }
         if (cur >= 0) {
             const PairNode nd = sc.pairs[cur];
             float n0, n1;
-            const bool h0 = slab(nd.lo0, nd.hi0, idx, idy, idz, oix, oiy, oiz, best_f, n0) && nd.c0 != kNoChild;
-            const bool h1 = slab(nd.lo1, nd.hi1, idx, idy, idz, oix, oiy, oiz, best_f, n1) && nd.c1 != kNoChild;
+            const bool h0 = slab(nd.lo0, nd.hi0, r.idx, r.idy, r.idz, r.oix, r.oiy, r.oiz, best_f, n0) && nd.c0 != kNoChild;
+            const bool h1 = slab(nd.lo1, nd.hi1, r.idx, r.idy, r.idz, r.oix, r.oiy, r.oiz, best_f, n1) && nd.c1 != kNoChild;
             if (h0 && h1) {
                 const bool near0 = n0 <= n1;
                 const int32_t far = near0 ? nd.c1 : nd.c0;
-                stk.base[depth * stk.stride] = far >= 0 ? far : -1 - (2 * cur + (near0 ? 1 : 0));
+                stk.base[depth * stk.stride] = far >= 0 ? far : -1 - (kDeferredLeaf | (2 * cur + (near0 ? 1 : 0)));
                 depth++;
                 cur = near0 ? nd.c0 : nd.c1;
                 continue;
@@ -362,29 +425,49 @@ MLGPU_HD inline void first_hit(const DeviceScene& sc, const int32_t scope, const
         } else {
             const int32_t leaf = -1 - cur;
             double t;
-            if (intersect_leaf(sc, sc.geo + leaf, s, d, t) && t < best.t) {
-                best.leaf = leaf;
-                best.t = t;
-                best_f = static_cast<float>(t - t_adv) * (1.0f + 1e-6f) + sc.eps;
+            if (leaf_box_hit(sc.boxes[leaf], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, best_f) &&
+                intersect_leaf(sc, sc.geo + leaf, s, d, t) && t < ts.best.t) {
+                ts.best.leaf = leaf;
+                ts.best.t = t;
+                best_f = static_cast<float>(t - r.t_adv) * (1.0f + 1e-6f) + sc.eps;
             }
         }
-        // next deferred entry still in front of the best hit
+        // next stack entry still in front of the best hit; a deferred leaf's
+        // box is tested again first
         cur = kNoChild;
         while (depth > 0) {
             depth--;
             const int32_t e = stk.base[depth * stk.stride];
-            if (e >= 0) { cur = e; break; }
-            const int32_t q = -1 - e;
-            const PairNode& nd = sc.pairs[q >> 1];
-            const bool second = (q & 1) != 0;
+            if (e >= 0) {
+                cur = e;
+                break;
+            }
+            const int32_t p = (-1 - e) & (kDeferredLeaf - 1);
+            const PairNode& nd = sc.pairs[p >> 1];
+            const bool second = (p & 1) != 0;
             float tn;
-            if (slab(second ? nd.lo1 : nd.lo0, second ? nd.hi1 : nd.hi0, idx, idy, idz, oix, oiy, oiz, best_f, tn)) {
+            if (slab(second ? nd.lo1 : nd.lo0, second ? nd.hi1 : nd.hi0, r.idx, r.idy, r.idz, r.oix, r.oiy, r.oiz,
+                     best_f, tn)) {
                 cur = second ? nd.c1 : nd.c0;
                 break;
             }
         }
         if (cur == kNoChild) break;
     }
+    ts.cur = kNoChild;
+    ts.depth = 0;
+    return true;
+}
+
+// First intersection within scope `scope`.
+MLGPU_HD inline void first_hit(const DeviceScene& sc, const int32_t scope, const D3& s, const D3& d, Closest& best,
+                               const LaneStack stk) {
+    RayF r;
+    ray_setup(sc, scope, s, d, r);
+    TraceState ts;
+    trace_begin(sc, scope, ts);
+    trace_run<false>(sc, r, s, d, ts, stk, 0);
+    best = ts.best;
 }
 
 // ------------------------------------------------------------- propagator

@@ -404,6 +404,8 @@ class Builder {
             scope_box_.push_back(bnodes_[r].box);
         }
         hs_->root_scope = hs_->scope_of_frame[0];
+        if (hs_->pairs.size() >= (size_t{1} << 29) || hs_->geo.size() >= (size_t{1} << 30))
+            throw std::invalid_argument("scene has too many surfaces for the node codes");
         if (hs_->tree_depth > kMaxTreeDepth)
             throw std::invalid_argument("scene tree is " + std::to_string(hs_->tree_depth) +
                                         " levels deep, more than the traversal stack holds (" +
@@ -431,6 +433,7 @@ class Builder {
     std::unordered_map<int32_t, int32_t> leaf_of_;   // frame -> leaf
     std::vector<std::array<Box, 2>> pair_box_;       // exact child boxes per PairNode
     std::vector<Box> scope_box_;
+    std::vector<std::array<double, 15>> leaf_obb_;   // rotation, centre, half extents
 
     int32_t make_leaf(int32_t f) {
         BNode b;
@@ -568,7 +571,7 @@ class Builder {
         return static_cast<int32_t>(bnodes_.size()) - 1;
     }
 
-    static constexpr int kBalancedFromDepth = 20;
+    static constexpr int kBalancedFromDepth = 16;
 
     bool internal(int32_t bi) const { return !bnodes_[bi].kids.empty(); }
 
@@ -637,6 +640,26 @@ class Builder {
         ph.pad0 = ph.pad1 = 0;
         hs_->geo.push_back(g);
         hs_->phys.push_back(ph);
+        {
+            // object-frame box of the leaf, as centre and half extents in the world
+            double prm[8];
+            const double* p = d_.params + MLGPU_NUM_PARAMS * f;
+            const double radius = leaf_params(d_.kind[f], p, prm);
+            const Box lb = local_box(d_.kind[f], p, prm, radius);
+            std::array<double, 15> ob{};
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) ob[3 * i + j] = m.T[j][i];
+            if (!lb.empty()) {
+                const double cl[3] = {lb.centre(0), lb.centre(1), lb.centre(2)};
+                for (int i = 0; i < 3; i++) {
+                    ob[9 + i] = m.T[i][0] * cl[0] + m.T[i][1] * cl[1] + m.T[i][2] * cl[2] + m.T[i][3];
+                    ob[12 + i] = 0.5 * (lb.hi[i] - lb.lo[i]);
+                }
+            } else {
+                for (int i = 0; i < 3; i++) ob[12 + i] = -1.0;  // never entered
+            }
+            leaf_obb_.push_back(ob);
+        }
         hs_->leaf_frame.push_back(f);
         return static_cast<int32_t>(hs_->geo.size()) - 1;
     }
@@ -722,6 +745,25 @@ class Builder {
             put(pair_box_[i][1], p.lo1, p.hi1);
         }
         for (size_t i = 0; i < hs_->scopes.size(); i++) put(scope_box_[i], hs_->scopes[i].lo, hs_->scopes[i].hi);
+        // oriented leaf boxes: half extents grow by the same bound, rounded up
+        hs_->boxes.resize(leaf_obb_.size());
+        for (size_t i = 0; i < leaf_obb_.size(); i++) {
+            const auto& ob = leaf_obb_[i];
+            LeafBox& lb = hs_->boxes[i];
+            for (int k = 0; k < 9; k++) lb.m[k] = static_cast<float>(ob[k]);
+            for (int k = 0; k < 3; k++) {
+                lb.c[k] = static_cast<float>(ob[9 + k]);
+                if (ob[12 + k] < 0.0) {
+                    lb.h[k] = -1.0f;
+                    continue;
+                }
+                const double h = ob[12 + k] + eps;
+                float hf = static_cast<float>(h);
+                if (static_cast<double>(hf) < h) hf = std::nextafter(hf, HUGE_VALF);
+                lb.h[k] = hf;
+            }
+            lb.pad = 0.0f;
+        }
     }
 };
 

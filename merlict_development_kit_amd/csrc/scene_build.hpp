@@ -16,6 +16,7 @@ struct HostScene {
     std::vector<Scope> scopes;
     int32_t tree_depth = 0;                          // longest chain of pair nodes
     std::vector<LeafGeo> geo;
+    std::vector<LeafBox> boxes;
     std::vector<LeafPhys> phys;
     std::vector<int32_t> leaf_frame;
     std::vector<FuncHead> funcs;

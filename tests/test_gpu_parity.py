@@ -12,6 +12,7 @@ import torch
 
 import pyoracle as O
 from conftest import split_golden
+from merlict_development_kit_amd._native import MlgpuError
 from merlict_development_kit_amd import (GpuScene, PhotonBatch, PropagationConfig, SceneDescription,
                                          propagate_photons_in_frame_with_config, sources)
 
@@ -82,15 +83,30 @@ def test_gpu_matches_oracle_many_photons(cuda, scene, schedule, oracle_threads):
 
 @pytest.mark.parametrize("max_int", [1, 5, 12])
 def test_gpu_schedules_agree_bitwise(cuda, max_int):
+    """Launch parameters never change results: fused vs wavefront, and
+    wavefront traces deferred after 1, 7 or 24 steps (their state saved and
+    resumed) or never."""
     gs = GpuScene(SCENES["fact_lg"]())
     ph = sources.fact_bunch(300_000, seed=8, device=cuda)
     ph.wavelength[:1000] = 150e-9  # out of the tables: error records
-    out = {}
-    for schedule in ("fused", "wavefront"):
-        gs.set_schedule(schedule)
-        out[schedule] = gs.propagate(ph, PropagationConfig(max_int, seed=5)).numpy()
-    for k in out["fused"]:
-        np.testing.assert_array_equal(out["fused"][k], out["wavefront"][k], err_msg=k)
+    gs.set_tuning(schedule="fused")
+    ref = gs.propagate(ph, PropagationConfig(max_int, seed=5)).numpy()
+    for budget, blocks in ((24, 8), (1, 8), (7, 3), (0, 8)):
+        gs.set_tuning(schedule="wavefront", trace_budget=budget, blocks_per_cu=blocks)
+        res = gs.propagate(ph, PropagationConfig(max_int, seed=5)).numpy()
+        for k in ref:
+            np.testing.assert_array_equal(ref[k], res[k], err_msg=f"{k} (budget {budget}, {blocks} blocks/CU)")
+
+
+def test_gpu_tuning_is_validated(cuda):
+    gs = GpuScene(SCENES["lens"]())
+    assert gs.tuning() == {"schedule": "auto", "trace_budget": 24, "blocks_per_cu": 32}
+    with pytest.raises(MlgpuError):
+        gs.set_tuning(blocks_per_cu=0)
+    with pytest.raises(MlgpuError):
+        gs.set_tuning(blocks_per_cu=65)
+    gs.set_tuning(trace_budget=-1)
+    assert gs.tuning()["trace_budget"] == -1
 
 
 def test_gpu_first_hit_matches_reference(cuda, golden):
