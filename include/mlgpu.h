@@ -148,6 +148,17 @@ typedef struct mlgpu_photons {
     const double* wavelength;
 } mlgpu_photons;
 
+/* The same arrays, written by mlgpu_generate_photons. */
+typedef struct mlgpu_photon_buffers {
+    double* support_x;
+    double* support_y;
+    double* support_z;
+    double* direction_x;
+    double* direction_y;
+    double* direction_z;
+    double* wavelength;
+} mlgpu_photon_buffers;
+
 /* merlict::Interaction (RayForPropagation.h:14-28) */
 enum mlgpu_interaction {
     MLGPU_PRODUCTION = 0,
@@ -236,6 +247,46 @@ int mlgpu_propagate(const mlgpu_scene* scene, int32_t world_frame,
                     const mlgpu_photons* photons, int64_t n,
                     const mlgpu_config* config, mlgpu_results* results,
                     void* stream);
+
+/* ------------------------------------------------------------ photon sources
+ * Synthetic photons drawn on the GPU from Philox4x32-10 stream 1 of
+ * (seed, photon id) -- propagation uses stream 0 -- with only +, -, *, /
+ * and sqrt in IEEE double, so a CPU restatement yields the same photons bit
+ * for bit.  Replaces the reference's sources: photon_source::
+ * parallel_towards_z_from_xy_disc and point_like_towards_z_opening_angle_
+ * num_photons (merlict/Photons.cpp:85-125), the calibration photons
+ * (merlict_portal_plenoscope/calibration/Calibrator.cpp:37-102) and the pose
+ * of transform_photons (merlict_json/json.cpp:512-519).
+ *
+ * Per photon: a support on the xy plane (the origin, or uniform on a disc by
+ * the reference's rejection rule, Generator.cpp:22-34), a direction about +z,
+ * a wavelength; the photon then starts start_distance back along its
+ * direction, and the pose (rot row major, pos) maps support and direction:
+ * s' = rot s + pos, d' = rot d. */
+enum mlgpu_support {
+    MLGPU_SUPPORT_POINT = 0,
+    MLGPU_SUPPORT_DISC = 1
+};
+enum mlgpu_direction {
+    MLGPU_DIRECTION_AXIS = 0,      /* +z */
+    MLGPU_DIRECTION_TILT = 1,      /* tilt ~ U[0, angle], azimuth ~ U[0, 2 pi) */
+    MLGPU_DIRECTION_GAUSSIAN = 2,  /* tan of the x and y tilts ~ N(0, angle) */
+    MLGPU_DIRECTION_CAP = 3        /* uniform on the cap zenith <= angle */
+};
+typedef struct mlgpu_source {
+    int32_t support;        /* mlgpu_support */
+    int32_t direction;      /* mlgpu_direction */
+    double radius;          /* disc radius (m) */
+    double angle;           /* rad; TILT and CAP: [0, pi/2] */
+    double lambda_min;      /* m; lambda_min == lambda_max: monochromatic, */
+    double lambda_max;      /* else pdf ~ lambda^-2 on [lambda_min, lambda_max] */
+    double start_distance;  /* m */
+    double rot[9];
+    double pos[3];
+} mlgpu_source;
+int mlgpu_generate_photons(const mlgpu_source* source, int64_t n, uint64_t seed,
+                           uint64_t photon_id_offset, const mlgpu_photon_buffers* out,
+                           void* stream);
 
 /* Number of photons with a non-zero error flag in the last n results
  * (synchronises `stream`). */

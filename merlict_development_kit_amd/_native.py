@@ -75,6 +75,25 @@ class Photons(C.Structure):
         "support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z", "wavelength")]
 
 
+class PhotonBuffers(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in (
+        "support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z", "wavelength")]
+
+
+class Source(C.Structure):
+    _fields_ = [
+        ("support", C.c_int32),
+        ("direction", C.c_int32),
+        ("radius", C.c_double),
+        ("angle", C.c_double),
+        ("lambda_min", C.c_double),
+        ("lambda_max", C.c_double),
+        ("start_distance", C.c_double),
+        ("rot", C.c_double * 9),
+        ("pos", C.c_double * 3),
+    ]
+
+
 class Results(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in (
         "final_frame", "sensor_id", "final_interaction", "num_interactions", "error", "time_of_flight",
@@ -120,6 +139,8 @@ SIGNATURES = {
     "mlgpu_scene_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlgpu_propagate": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
                                   C.POINTER(Results), C.c_void_p]),
+    "mlgpu_generate_photons": (C.c_int, [C.POINTER(Source), C.c_int64, C.c_uint64, C.c_uint64,
+                                         C.POINTER(PhotonBuffers), C.c_void_p]),
     "mlgpu_count_errors": (C.c_int, [C.POINTER(Results), C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
     "mlgpu_first_hit": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Hits),
                                   C.c_void_p]),

@@ -3,6 +3,7 @@
 // codes plus mlgpu_last_error().
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -227,6 +228,41 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
         ms->ws_stream = st;
     }
     if (e != hipSuccess) return hip_fail(e, "propagate launch");
+    return MLGPU_OK;
+}
+
+int mlgpu_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed, uint64_t photon_id_offset,
+                           const mlgpu_photon_buffers* out, void* stream) {
+    if (!src || !out || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (n == 0) return MLGPU_OK;
+    if (!out->support_x || !out->support_y || !out->support_z || !out->direction_x || !out->direction_y ||
+        !out->direction_z || !out->wavelength) {
+        set_last_error("null photon array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (src->support != MLGPU_SUPPORT_POINT && src->support != MLGPU_SUPPORT_DISC) {
+        set_last_error("bad support kind");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (src->direction < MLGPU_DIRECTION_AXIS || src->direction > MLGPU_DIRECTION_CAP) {
+        set_last_error("bad direction kind");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const bool bounded = src->direction == MLGPU_DIRECTION_TILT || src->direction == MLGPU_DIRECTION_CAP;
+    if (!(src->angle >= 0.0) || (bounded && !(src->angle <= 1.57079632679489661923)) || !std::isfinite(src->angle)) {
+        set_last_error("angle must be finite and >= 0, and <= pi/2 for TILT and CAP");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (!(src->radius >= 0.0) || !std::isfinite(src->radius) || !std::isfinite(src->start_distance)) {
+        set_last_error("radius must be finite and >= 0, start_distance finite");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (!(src->lambda_min > 0.0) || !(src->lambda_max >= src->lambda_min) || !std::isfinite(src->lambda_max)) {
+        set_last_error("need 0 < lambda_min <= lambda_max < inf");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const hipError_t e = mlgpu::launch_generate(*src, n, seed, photon_id_offset, *out, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "generate launch");
     return MLGPU_OK;
 }
 

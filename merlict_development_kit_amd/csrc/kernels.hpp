@@ -48,6 +48,9 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
 hipError_t launch_first_hit(const DeviceScene& sc, int32_t root_scope, const mlgpu_photons& rays, int64_t n,
                             const mlgpu_hits& out, hipStream_t stream);
 
+hipError_t launch_generate(const mlgpu_source& src, int64_t n, uint64_t seed, uint64_t id_offset,
+                           const mlgpu_photon_buffers& out, hipStream_t stream);
+
 hipError_t launch_count_errors(const uint8_t* err, int64_t n, unsigned long long* counter, hipStream_t stream);
 
 }  // namespace mlgpu

@@ -1,0 +1,37 @@
+// HIP kernel of the photon sources (sources_core.hpp): one lane per photon.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "sources_core.hpp"
+
+namespace mlgpu {
+namespace {
+
+__global__ __launch_bounds__(256) void generate_kernel(const mlgpu_source src, const int64_t n, const uint64_t seed,
+                                                       const uint64_t id_offset, const mlgpu_photon_buffers out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    D3 s, d;
+    double lambda;
+    source_photon(src, seed, id_offset + static_cast<uint64_t>(i), s, d, lambda);
+    out.support_x[i] = s.x;
+    out.support_y[i] = s.y;
+    out.support_z[i] = s.z;
+    out.direction_x[i] = d.x;
+    out.direction_y[i] = d.y;
+    out.direction_z[i] = d.z;
+    out.wavelength[i] = lambda;
+}
+
+}  // namespace
+
+hipError_t launch_generate(const mlgpu_source& src, int64_t n, uint64_t seed, uint64_t id_offset,
+                           const mlgpu_photon_buffers& out, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int64_t grid = (n + 255) / 256;
+    hipLaunchKernelGGL(generate_kernel, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, stream, src, n, seed,
+                       id_offset, out);
+    return hipGetLastError();
+}
+
+}  // namespace mlgpu

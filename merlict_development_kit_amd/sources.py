@@ -1,80 +1,140 @@
-"""Synthetic photon sources of the measured configurations.
+"""Photon sources drawn on the GPU (mlgpu_generate_photons).
 
-Generated with torch on any device from a seed; the same seed on the same
-device gives the same photons.  Geometry conventions follow merlict: z is the
-optical axis, photons from the sky travel towards -z.
+Every photon comes from Philox4x32-10 stream 1 of (seed, photon id), so a
+bunch is reproducible from its seed and id range alone, on any GPU and by
+the CPU restatement.  The presets are the synthetic inputs of the measured
+configurations and the reference's own sources:
+
+* :func:`lens_beam` -- supports on a disc above a lens, tilt U[0, max],
+  azimuth U[0, 2 pi), one wavelength;
+* :func:`fact_bunch` -- supports on a 4.5 m disc above the dish, x and y
+  tilts N(0, 0.5 deg), lambda in [280, 700] nm with pdf ~ lambda^-2;
+* :func:`calibration` -- plenoscope calibration photons
+  (merlict_portal_plenoscope/calibration/Calibrator.cpp:37-102);
+* :func:`point_like` and :func:`parallel_disc` --
+  photon_source::point_like_towards_z_opening_angle_num_photons and
+  parallel_towards_z_from_xy_disc (merlict/Photons.cpp:85-125).
+
+A pose (rotation, position) applied to supports and directions restates
+transform_photons (merlict_json/json.cpp:512-519).
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
+from dataclasses import dataclass, field, replace
+from typing import Sequence, Tuple
 
+import numpy as np
 import torch
 
+from . import _native as N
 from .propagation import PhotonBatch
 
+SUPPORTS = {"point": 0, "disc": 1}
+DIRECTIONS = {"axis": 0, "tilt": 1, "gaussian": 2, "cap": 3}
 
-def _gen(seed: int, device) -> torch.Generator:
-    g = torch.Generator(device=device)
-    g.manual_seed(int(seed))
-    return g
-
-
-def _disc(n, radius, g, device):
-    r = radius * torch.sqrt(torch.rand(n, generator=g, device=device, dtype=torch.float64))
-    phi = 2.0 * math.pi * torch.rand(n, generator=g, device=device, dtype=torch.float64)
-    return r * torch.cos(phi), r * torch.sin(phi)
+IDENTITY = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+# rotation by pi about x: directions about +z become directions about -z
+FLIP = (1.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0.0, 0.0, -1.0)
 
 
-def _lambda_inverse_square(n, lo, hi, g, device):
-    # pdf ~ lambda^-2 on [lo, hi]: 1/lambda uniform
-    u = torch.rand(n, generator=g, device=device, dtype=torch.float64)
-    return 1.0 / (1.0 / lo - u * (1.0 / lo - 1.0 / hi))
+def rotation_xyz(rx: float, ry: float, rz: float) -> Tuple[float, ...]:
+    """Row-major rotation of HomTra3 from xyz angles, the T that
+    set_rotation_component_based_on_xyz_angles fills (merlict/HomTra3.cpp:93-112)."""
+    cx, sx, cy, sy, cz, sz = math.cos(rx), math.sin(rx), math.cos(ry), math.sin(ry), math.cos(rz), math.sin(rz)
+    return (cy * cz, cx * sz + sx * sy * cz, sx * sz - cx * sy * cz,
+            -cy * sz, cx * cz - sx * sy * sz, sx * cz + cx * sy * sz,
+            sy, -sx * cy, cx * cy)
 
 
-def lens_beam(n: int, seed: int = 0, device="cpu", disc_radius=0.25, height=1.0, max_tilt_deg=2.0,
-              wavelength=433e-9) -> PhotonBatch:
-    """Parallel-beam photons on a disc above the lens, each tilted by
-    theta ~ U[0, max_tilt], azimuth ~ U[0, 2 pi)."""
-    g = _gen(seed, device)
-    x, y = _disc(n, disc_radius, g, device)
-    theta = math.radians(max_tilt_deg) * torch.rand(n, generator=g, device=device, dtype=torch.float64)
-    phi = 2.0 * math.pi * torch.rand(n, generator=g, device=device, dtype=torch.float64)
-    z = torch.full_like(x, height)
-    return PhotonBatch(x, y, z, torch.sin(theta) * torch.cos(phi), torch.sin(theta) * torch.sin(phi),
-                       -torch.cos(theta), torch.full_like(x, wavelength))
+def compose(a: Sequence[float], b: Sequence[float]) -> Tuple[float, ...]:
+    """Row-major product a b of two 3x3 rotations."""
+    A = np.asarray(a, dtype=np.float64).reshape(3, 3)
+    B = np.asarray(b, dtype=np.float64).reshape(3, 3)
+    return tuple((A @ B).reshape(9).tolist())
 
 
-def fact_bunch(n: int, seed: int = 0, device="cpu", disc_radius=2.25, height=10.0, sigma_deg=0.5,
-               lambda_min=280e-9, lambda_max=700e-9) -> PhotonBatch:
-    """Supports uniform on a 4.5 m disc at z = height, directions Gaussian
-    (sigma per axis) about -z, lambda in [280, 700] nm with pdf ~ lambda^-2."""
-    g = _gen(seed, device)
-    x, y = _disc(n, disc_radius, g, device)
-    s = math.radians(sigma_deg)
-    tx = s * torch.randn(n, generator=g, device=device, dtype=torch.float64)
-    ty = s * torch.randn(n, generator=g, device=device, dtype=torch.float64)
-    dz = -torch.ones_like(tx)
-    norm = torch.sqrt(tx * tx + ty * ty + 1.0)
-    z = torch.full_like(x, height)
-    wl = _lambda_inverse_square(n, lambda_min, lambda_max, g, device)
-    return PhotonBatch(x, y, z, tx / norm, ty / norm, dz / norm, wl)
+@dataclass(frozen=True)
+class Source:
+    """mlgpu_source: support on the xy plane, direction about +z, wavelength,
+    start distance back along the direction, then the pose."""
+    support: str = "point"
+    direction: str = "axis"
+    radius: float = 0.0
+    angle: float = 0.0
+    lambda_min: float = 433e-9
+    lambda_max: float = 433e-9
+    start_distance: float = 0.0
+    rot: Tuple[float, ...] = IDENTITY
+    pos: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+
+    def to_c(self, struct_type=N.Source):
+        """The fields as a ctypes struct laid out as mlgpu_source."""
+        s = struct_type()
+        s.support = SUPPORTS[self.support]
+        s.direction = DIRECTIONS[self.direction]
+        s.radius = self.radius
+        s.angle = self.angle
+        s.lambda_min = self.lambda_min
+        s.lambda_max = self.lambda_max
+        s.start_distance = self.start_distance
+        for k in range(9):
+            s.rot[k] = self.rot[k]
+        for k in range(3):
+            s.pos[k] = self.pos[k]
+        return s
+
+    def posed(self, rot: Sequence[float] = IDENTITY, pos: Sequence[float] = (0.0, 0.0, 0.0)) -> "Source":
+        """The source moved by a further pose (applied after its own)."""
+        R = np.asarray(rot, dtype=np.float64).reshape(3, 3)
+        new_pos = R @ np.asarray(self.pos, dtype=np.float64) + np.asarray(pos, dtype=np.float64)
+        return replace(self, rot=compose(rot, self.rot), pos=tuple(new_pos.tolist()))
+
+    def generate(self, n: int, seed: int = 0, photon_id_offset: int = 0, device="cuda",
+                 stream=None) -> PhotonBatch:
+        """Photons photon_id_offset .. + n - 1, drawn on the GPU."""
+        device = torch.device(device)
+        if device.type != "cuda":
+            raise ValueError("photon sources draw on the GPU; copy the batch to the host afterwards")
+        arrays = [torch.empty(n, dtype=torch.float64, device=device) for _ in range(7)]
+        if n > 0:
+            buf = N.PhotonBuffers(*[a.data_ptr() for a in arrays])
+            st = stream if stream is not None else torch.cuda.current_stream(device)
+            N.check(N.lib().mlgpu_generate_photons(C.byref(self.to_c()), n, seed, photon_id_offset, C.byref(buf),
+                                                   C.c_void_p(st.cuda_stream)), "mlgpu_generate_photons")
+        return PhotonBatch(*arrays)
 
 
-def calibration_photons(n: int, seed: int = 0, device="cpu", aperture_radius=35.5 * 1.05,
-                        max_incident_angle=math.radians(3.25) * 1.2, distance=1e4, wavelength=433e-9):
-    """Plenoscope calibration photons (merlict_portal_plenoscope/calibration/
-    Calibrator.cpp:37-102): support on the aperture disc, direction within a
-    cone, started `distance` upstream.  Returns (photons, aperture x, y,
-    incident cx, cy)."""
-    g = _gen(seed, device)
-    x, y = _disc(n, aperture_radius, g, device)
-    # ZenithDistancePicker (merlict/random/SpherePointPicker.cpp:8-20)
-    z_min = (math.cos(0.0) + 1.0) / 2.0
-    z_range = (math.cos(max_incident_angle) + 1.0) / 2.0 - z_min
-    u = torch.rand(n, generator=g, device=device, dtype=torch.float64)
-    zd = torch.acos(2.0 * (z_range * u + z_min) - 1.0)
-    az = 2.0 * math.pi * torch.rand(n, generator=g, device=device, dtype=torch.float64)
-    cx, cy, cz = torch.sin(zd) * torch.cos(az), torch.sin(zd) * torch.sin(az), torch.cos(zd)
-    sx, sy, sz = x + cx * distance, y + cy * distance, cz * distance
-    ph = PhotonBatch(sx, sy, sz, -cx, -cy, -cz, torch.full_like(x, wavelength))
-    return ph, x, y, cx, cy
+def lens_beam(disc_radius: float = 0.25, height: float = 1.0, max_tilt_deg: float = 2.0,
+              wavelength: float = 433e-9) -> Source:
+    """Supports on a disc at z = height, towards -z tilted by U[0, max_tilt]."""
+    return Source("disc", "tilt", disc_radius, math.radians(max_tilt_deg), wavelength, wavelength, 0.0,
+                  FLIP, (0.0, 0.0, height))
+
+
+def fact_bunch(disc_radius: float = 2.25, height: float = 10.0, sigma_deg: float = 0.5,
+               lambda_min: float = 280e-9, lambda_max: float = 700e-9) -> Source:
+    """Supports on a disc at z = height, towards -z with x and y tilts
+    N(0, sigma), lambda with pdf ~ lambda^-2."""
+    return Source("disc", "gaussian", disc_radius, math.radians(sigma_deg), lambda_min, lambda_max, 0.0,
+                  FLIP, (0.0, 0.0, height))
+
+
+def calibration(aperture_radius: float = 35.5 * 1.05, max_incident_angle: float = math.radians(3.25) * 1.2,
+                distance: float = 1e4, wavelength: float = 433e-9) -> Source:
+    """Calibration photons: aperture point uniform on a disc, incident
+    direction c uniform on the cap within max_incident_angle of +z, started
+    `distance` upstream (at aperture point + c distance), travelling -c."""
+    return Source("disc", "cap", aperture_radius, max_incident_angle, wavelength, wavelength, distance,
+                  FLIP, (0.0, 0.0, 0.0))
+
+
+def point_like(opening_angle: float, wavelength: float = 433e-9) -> Source:
+    """photon_source::point_like_towards_z_opening_angle_num_photons."""
+    return Source("point", "cap", 0.0, opening_angle, wavelength, wavelength)
+
+
+def parallel_disc(disc_radius: float, wavelength: float = 433e-9) -> Source:
+    """photon_source::parallel_towards_z_from_xy_disc."""
+    return Source("disc", "axis", disc_radius, 0.0, wavelength, wavelength)

@@ -842,6 +842,118 @@ static double gen_uniform(philox_gen* g) {
     return (double)(w >> 8) * (1.0 / 16777216.0);
 }
 
+/* ----------------------------------------------------------- photon sources
+ * The photon sources of the library restated: Philox4x32-10 blocks
+ * (block, 1, id_lo, id_hi); block 0 direction, block 1 wavelength, block
+ * 2 + k disc try k; 53-bit uniforms; log and sin/cos by series so that only
+ * correctly rounded IEEE operations occur. */
+static void src_block(uint64_t seed, uint64_t id, uint32_t block, uint32_t w[4]) {
+    const uint32_t ctr[4] = {block, 1u, (uint32_t)id, (uint32_t)(id >> 32)};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    oracle_philox4x32_10(ctr, key, w);
+}
+
+static double src_u53(uint32_t a, uint32_t b) {
+    /* (a >> 5) 2^-27 + (b >> 6) 2^-53 */
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
+}
+
+double oracle_log_series(double x) {
+    /* x = m 2^e, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh((m-1)/(m+1)) */
+    int e;
+    double m = frexp(x, &e);  /* m in [0.5, 1) */
+    if (m < 0.70710678118654752440) { m = m * 2.0; e = e - 1; }
+    const double z = (m - 1.0) / (m + 1.0);
+    const double z2 = z * z;
+    double acc = 1.0 / 25.0;
+    for (int k = 11; k >= 0; k--) acc = acc * z2 + 1.0 / (double)(2 * k + 1);
+    return (double)e * 6.93147180559945286227e-01 + ((double)e * 2.31904681384629955842e-17 + 2.0 * z * acc);
+}
+
+/* sin, cos of t in [0, pi/2]: nested Taylor factors down to t^25 */
+static void src_sincos(double t, double* s, double* c) {
+    const double t2 = t * t;
+    double fs = 1.0, fc = 1.0;
+    for (int k = 12; k >= 1; k--) {
+        fs = 1.0 - t2 / ((double)(2 * k) * (double)(2 * k + 1)) * fs;
+        fc = 1.0 - t2 / ((double)(2 * k - 1) * (double)(2 * k)) * fc;
+    }
+    *s = t * fs;
+    *c = fc;
+}
+
+void oracle_sincos_turn(double u, double* s, double* c) {
+    const double u4 = u * 4.0;
+    const int q = (int)u4;
+    double a, b;
+    src_sincos((u4 - (double)q) * 1.57079632679489661923, &a, &b);
+    switch (q) {
+    case 0: *s = a; *c = b; break;
+    case 1: *s = b; *c = -a; break;
+    case 2: *s = -a; *c = -b; break;
+    default: *s = -b; *c = a; break;
+    }
+}
+
+void oracle_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed,
+                             uint64_t photon_id_offset, double* const out[7]) {
+    for (int64_t i = 0; i < n; i++) {
+        const uint64_t id = photon_id_offset + (uint64_t)i;
+        uint32_t w[4];
+        double dx = 0.0, dy = 0.0, dz = 1.0, lambda, px = 0.0, py = 0.0;
+        src_block(seed, id, 0u, w);
+        const double u0 = src_u53(w[0], w[1]), u1 = src_u53(w[2], w[3]);
+        if (src->direction == MLGPU_DIRECTION_TILT) {
+            double st, ct, sp, cp;
+            src_sincos(src->angle * u0, &st, &ct);
+            oracle_sincos_turn(u1, &sp, &cp);
+            dx = st * cp; dy = st * sp; dz = ct;
+        } else if (src->direction == MLGPU_DIRECTION_GAUSSIAN) {
+            const double rad = sqrt(-2.0 * oracle_log_series(1.0 - u0));
+            double sp, cp;
+            oracle_sincos_turn(u1, &sp, &cp);
+            const double tx = src->angle * (rad * cp), ty = src->angle * (rad * sp);
+            const double len = sqrt(tx * tx + ty * ty + 1.0);
+            dx = tx / len; dy = ty / len; dz = 1.0 / len;
+        } else if (src->direction == MLGPU_DIRECTION_CAP) {
+            double sa, ca, sp, cp;
+            src_sincos(src->angle, &sa, &ca);
+            const double cz = 1.0 - u0 * (1.0 - ca);
+            const double sz = sqrt(1.0 - cz * cz);
+            oracle_sincos_turn(u1, &sp, &cp);
+            dx = sz * cp; dy = sz * sp; dz = cz;
+        }
+        src_block(seed, id, 1u, w);
+        if (src->lambda_min == src->lambda_max) {
+            lambda = src->lambda_min;
+        } else {
+            const double inv_lo = 1.0 / src->lambda_min, inv_hi = 1.0 / src->lambda_max;
+            lambda = 1.0 / (inv_lo - src_u53(w[0], w[1]) * (inv_lo - inv_hi));
+        }
+        if (src->support == MLGPU_SUPPORT_DISC) {
+            /* Generator::get_point_on_xy_disc_within_radius (Generator.cpp:22-34),
+             * at most 64 tries */
+            const double r = src->radius;
+            for (uint32_t k = 0; k < 64; k++) {
+                src_block(seed, id, 2u + k, w);
+                const double x = r * (src_u53(w[0], w[1]) * 2.0 - 1.0);
+                const double y = r * (src_u53(w[2], w[3]) * 2.0 - 1.0);
+                if (x * x + y * y <= r * r) { px = x; py = y; break; }
+            }
+        }
+        const double g = src->start_distance;
+        const double sx = px - dx * g, sy = py - dy * g, sz = 0.0 - dz * g;
+        const double* R = src->rot;
+        out[0][i] = sx * R[0] + sy * R[1] + sz * R[2] + src->pos[0];
+        out[1][i] = sx * R[3] + sy * R[4] + sz * R[5] + src->pos[1];
+        out[2][i] = sx * R[6] + sy * R[7] + sz * R[8] + src->pos[2];
+        out[3][i] = dx * R[0] + dy * R[1] + dz * R[2];
+        out[4][i] = dx * R[3] + dy * R[4] + dz * R[5];
+        out[5][i] = dx * R[6] + dy * R[7] + dz * R[8];
+        out[6][i] = lambda;
+    }
+}
+
 /* -------------------------------------------------------------- propagator */
 #define C_VACUUM 299792458.0  /* PhysicalConstants.h:7 */
 

@@ -88,6 +88,14 @@ double oracle_uniform(uint64_t seed, uint32_t stream, uint64_t photon_id,
 int oracle_func_evaluate(const oracle_scene* s, int32_t f, double x,
                          double* y);
 
+/* Photon sources (mlgpu_source of include/mlgpu.h): photons
+ * photon_id_offset .. + n - 1 into host arrays out[0..6] (support xyz,
+ * direction xyz, wavelength).  The helpers are exposed for their tests. */
+void oracle_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed,
+                             uint64_t photon_id_offset, double* const out[7]);
+double oracle_log_series(double x);
+void oracle_sincos_turn(double u, double* s, double* c);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,15 @@ class _Photons(C.Structure):
         "support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z", "wavelength")]
 
 
+class Source(C.Structure):
+    """mlgpu_source of include/mlgpu.h."""
+    _fields_ = [
+        ("support", C.c_int32), ("direction", C.c_int32), ("radius", C.c_double), ("angle", C.c_double),
+        ("lambda_min", C.c_double), ("lambda_max", C.c_double), ("start_distance", C.c_double),
+        ("rot", C.c_double * 9), ("pos", C.c_double * 3),
+    ]
+
+
 PHOTON_FIELDS = ("support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z", "wavelength")
 
 _lib = None
@@ -66,6 +75,11 @@ def lib() -> C.CDLL:
         L.oracle_uniform.restype = C.c_double
         L.oracle_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32]
         L.oracle_func_evaluate.argtypes = [C.c_void_p, C.c_int32, C.c_double, _P_F64]
+        L.oracle_generate_photons.argtypes = [C.POINTER(Source), C.c_int64, C.c_uint64, C.c_uint64,
+                                              C.POINTER(_P_F64)]
+        L.oracle_log_series.restype = C.c_double
+        L.oracle_log_series.argtypes = [C.c_double]
+        L.oracle_sincos_turn.argtypes = [C.c_double, _P_F64, _P_F64]
         _lib = L
     return _lib
 
@@ -157,6 +171,24 @@ class OracleScene:
         y = C.c_double()
         rc = lib().oracle_func_evaluate(self._h, f, x, C.byref(y))
         return (None if rc else y.value)
+
+
+def generate_photons(src: Source, n: int, seed: int = 0, photon_id_offset: int = 0) -> dict:
+    """Photons of a source, restated on the CPU (oracle_generate_photons)."""
+    out = {k: np.empty(n, dtype=np.float64) for k in PHOTON_FIELDS}
+    arr = (_P_F64 * 7)(*[_ptr(out[k], _P_F64) for k in PHOTON_FIELDS])
+    lib().oracle_generate_photons(C.byref(src), n, seed, photon_id_offset, arr)
+    return out
+
+
+def log_series(x: float) -> float:
+    return lib().oracle_log_series(x)
+
+
+def sincos_turn(u: float) -> tuple:
+    s, c = C.c_double(), C.c_double()
+    lib().oracle_sincos_turn(u, C.byref(s), C.byref(c))
+    return s.value, c.value
 
 
 def philox4x32_10(counter, key):

@@ -18,9 +18,11 @@ HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent.parent
 sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "oracle"))
+sys.path.insert(0, str(HERE.parent))
 
+import photon_sets as sources  # noqa: E402
 import pyoracle as O  # noqa: E402
-from merlict_development_kit_amd import SceneDescription, sources  # noqa: E402
+from merlict_development_kit_amd import SceneDescription  # noqa: E402
 
 PHOTON_KEYS = O.PHOTON_FIELDS
 RESULT_KEYS = ("final_frame", "sensor_id", "final_interaction", "num_interactions", "error", "time_of_flight",

@@ -13,8 +13,9 @@ import torch
 import pyoracle as O
 from conftest import split_golden
 from merlict_development_kit_amd._native import MlgpuError
+import photon_sets as sources
 from merlict_development_kit_amd import (GpuScene, PhotonBatch, PropagationConfig, SceneDescription,
-                                         propagate_photons_in_frame_with_config, sources)
+                                         propagate_photons_in_frame_with_config)
 
 pytestmark = pytest.mark.gpu
 

@@ -10,11 +10,13 @@
 import math
 
 import numpy as np
+from scipy import stats as sp_stats
 import pytest
 
 import pyoracle as O
 from conftest import split_golden
-from merlict_development_kit_amd import SceneDescription, sources
+import photon_sets as sources
+from merlict_development_kit_amd import SceneDescription
 
 RESULT_INT = ("final_frame", "sensor_id", "final_interaction", "num_interactions", "error")
 RESULT_FLOAT = ("time_of_flight", "x", "y", "cx", "cy")
@@ -239,3 +241,99 @@ def test_out_of_table_wavelength_flags_error(oracle_threads):
     r = o.propagate(ph, threads=oracle_threads)
     assert np.all(r["error"] == 1)
     del hit
+
+
+# ------------------------------------------------------------ photon sources
+def _src(**kw):
+    from merlict_development_kit_amd import sources as S
+    return S.Source(**kw).to_c(O.Source)
+
+
+def test_source_log_and_sincos_series_are_accurate():
+    xs = np.concatenate([np.linspace(1e-12, 1.0, 2001), [2.0 ** -53, 0.5, 0.7071067811865476, 1.0 - 2.0 ** -53]])
+    got = np.array([O.log_series(x) for x in xs])
+    np.testing.assert_allclose(got, np.log(xs), rtol=4e-16, atol=1e-17)
+    us = np.linspace(0.0, 1.0, 4001, endpoint=False)
+    sc = np.array([O.sincos_turn(u) for u in us])
+    np.testing.assert_allclose(sc[:, 0], np.sin(2 * np.pi * us), atol=2e-16)
+    np.testing.assert_allclose(sc[:, 1], np.cos(2 * np.pi * us), atol=2e-16)
+
+
+def test_source_disc_support_is_uniform_and_inside():
+    n = 200_000
+    ph = O.generate_photons(_src(support="disc", radius=2.0), n, seed=3)
+    r2 = ph["support_x"] ** 2 + ph["support_y"] ** 2
+    assert r2.max() <= 4.0
+    # r^2 / R^2 is U[0, 1] for a uniform disc; the angle is uniform
+    assert sp_stats.kstest(r2 / 4.0, "uniform").pvalue > 0.01
+    ang = (np.arctan2(ph["support_y"], ph["support_x"]) + np.pi) / (2 * np.pi)
+    assert sp_stats.kstest(ang, "uniform").pvalue > 0.01
+    assert np.all(ph["direction_z"] == 1.0) and np.all(ph["support_z"] == 0.0)
+
+
+def test_source_directions_follow_their_laws():
+    n = 200_000
+    sigma = np.radians(0.5)
+    g = O.generate_photons(_src(direction="gaussian", angle=sigma), n, seed=4)
+    tx, ty = g["direction_x"] / g["direction_z"], g["direction_y"] / g["direction_z"]
+    assert sp_stats.kstest(tx / sigma, "norm").pvalue > 0.01
+    assert sp_stats.kstest(ty / sigma, "norm").pvalue > 0.01
+    assert abs(np.corrcoef(tx, ty)[0, 1]) < 0.01
+    norm = np.sqrt(g["direction_x"] ** 2 + g["direction_y"] ** 2 + g["direction_z"] ** 2)
+    np.testing.assert_allclose(norm, 1.0, rtol=1e-15)
+    # tilt: theta ~ U[0, a], azimuth ~ U[0, 2 pi)
+    a = np.radians(2.0)
+    t = O.generate_photons(_src(direction="tilt", angle=a), n, seed=5)
+    theta = np.arccos(np.clip(t["direction_z"], -1, 1))
+    assert sp_stats.kstest(theta / a, "uniform").pvalue > 0.01
+    # cap: cos(zenith) ~ U[cos a, 1]
+    a = np.radians(30.0)
+    c = O.generate_photons(_src(direction="cap", angle=a), n, seed=6)
+    assert sp_stats.kstest((1.0 - c["direction_z"]) / (1.0 - np.cos(a)), "uniform").pvalue > 0.01
+    az = (np.arctan2(c["direction_y"], c["direction_x"]) + np.pi) / (2 * np.pi)
+    assert sp_stats.kstest(az, "uniform").pvalue > 0.01
+
+
+def test_source_wavelength_inverse_square_law():
+    lo, hi = 280e-9, 700e-9
+    ph = O.generate_photons(_src(lambda_min=lo, lambda_max=hi), 200_000, seed=7)
+    lam = ph["wavelength"]
+    assert lam.min() >= lo and lam.max() <= hi
+    cdf = lambda x: (1.0 / lo - 1.0 / x) / (1.0 / lo - 1.0 / hi)  # noqa: E731
+    assert sp_stats.kstest(lam, cdf).pvalue > 0.01
+    mono = O.generate_photons(_src(lambda_min=433e-9, lambda_max=433e-9), 10, seed=7)
+    assert np.all(mono["wavelength"] == 433e-9)
+
+
+def test_source_pose_and_start_distance():
+    from merlict_development_kit_amd import sources as S
+    # calibration: photons start `distance` up the incident direction c and travel -c
+    src = S.calibration(aperture_radius=1.0, max_incident_angle=np.radians(3.0), distance=1e4)
+    ph = O.generate_photons(src.to_c(O.Source), 1000, seed=9)
+    c = -np.stack([ph["direction_x"], ph["direction_y"], ph["direction_z"]], 1)
+    s = np.stack([ph["support_x"], ph["support_y"], ph["support_z"]], 1)
+    ap = s - c * 1e4
+    np.testing.assert_allclose(ap[:, 2], 0.0, atol=1e-9)
+    assert np.all(np.hypot(ap[:, 0], ap[:, 1]) <= 1.0 + 1e-9)
+    assert np.all(c[:, 2] >= np.cos(np.radians(3.0)) - 1e-15)
+    # a further pose moves supports and turns directions (transform_photons)
+    rot = S.rotation_xyz(0.1, -0.2, 0.3)
+    moved = S.fact_bunch().posed(rot, (1.0, 2.0, 3.0))
+    a = O.generate_photons(S.fact_bunch().to_c(O.Source), 500, seed=10)
+    b = O.generate_photons(moved.to_c(O.Source), 500, seed=10)
+    R = np.asarray(rot).reshape(3, 3)
+    sa = np.stack([a["support_x"], a["support_y"], a["support_z"]], 1)
+    sb = np.stack([b["support_x"], b["support_y"], b["support_z"]], 1)
+    np.testing.assert_allclose(sb, sa @ R.T + [1.0, 2.0, 3.0], atol=1e-12)
+    np.testing.assert_allclose(np.asarray(rot).reshape(3, 3) @ np.asarray(rot).reshape(3, 3).T, np.eye(3), atol=1e-15)
+
+
+def test_source_draws_are_keyed_by_photon_id():
+    src = _src(support="disc", radius=1.0, direction="gaussian", angle=0.01, lambda_min=300e-9,
+               lambda_max=600e-9)
+    whole = O.generate_photons(src, 100, seed=11)
+    tail = O.generate_photons(src, 40, seed=11, photon_id_offset=60)
+    for k in whole:
+        np.testing.assert_array_equal(whole[k][60:], tail[k])
+    other = O.generate_photons(src, 100, seed=12)
+    assert not np.array_equal(whole["support_x"], other["support_x"])
