@@ -46,7 +46,7 @@ struct alignas(16) Scope {
 
 // Deepest node chain a scene may have: the traversal stack holds at most
 // one entry per level.
-constexpr int kMaxTreeDepth = 24;
+constexpr int kMaxTreeDepth = 28;
 
 enum LeafKind : int32_t {
     LK_CAP_HEX = 1,
@@ -67,13 +67,14 @@ enum LeafKind : int32_t {
 //   t      frame2world translation (T[i][3])
 //   ti     R^T t, the constant of HomTra3::position_inverse
 //   p      kind parameters, see the kernels
+//   rank   position in merlict's candidate order, which breaks exact ties
 struct alignas(16) LeafGeo {
     double rot[9];
     double tx, ty, tz;
     double ti[3];
     double p[8];
     int32_t kind;
-    int32_t pad0;
+    int32_t rank;
 };
 
 // Float oriented box of a leaf (64 B): a point x of the world lies in it when

@@ -384,11 +384,14 @@ MLGPU_HD inline void trace_begin(const DeviceScene& sc, const int32_t scope, Tra
 }
 
 // Runs the traversal on, for at most `budget` steps (node visits and leaf
-// tests) when Budgeted; returns true when it has finished.  A
-// traversal stopped early resumes from ts and the stack, and ends with the
-// same closest hit.  The boxes only cull: every leaf whose box the ray
-// enters before the best hit so far is solved in FP64, so the closest hit is
-// merlict's (rays_first_intersection_with_frame, RayAndFrame.cpp:118-173).
+// tests) when Budgeted; returns true when it has finished.  A traversal
+// stopped early resumes from ts and the stack, and ends with the same
+// closest hit.  The boxes only cull: every leaf whose box the ray enters
+// before the best hit so far is solved in FP64, so the closest hit is
+// merlict's (rays_first_intersection_with_frame, RayAndFrame.cpp:118-173);
+// leaves at exactly the same distance (coplanar surfaces) are ordered by
+// their rank in merlict's candidate order, so the winner does not depend on
+// the traversal order.
 // Nearer children are visited first so that the best hit shrinks early and
 // prunes more boxes.
 template <bool Budgeted>
@@ -425,8 +428,10 @@ MLGPU_HD inline bool trace_run(const DeviceScene& sc, const RayF& r, const D3& s
         } else {
             const int32_t leaf = -1 - cur;
             double t;
+            // the closest hit; an exact tie goes to the earlier candidate
             if (leaf_box_hit(sc.boxes[leaf], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, best_f) &&
-                intersect_leaf(sc, sc.geo + leaf, s, d, t) && t < ts.best.t) {
+                intersect_leaf(sc, sc.geo + leaf, s, d, t) &&
+                (t < ts.best.t || (t == ts.best.t && sc.geo[leaf].rank < sc.geo[ts.best.leaf].rank))) {
                 ts.best.leaf = leaf;
                 ts.best.t = t;
                 best_f = static_cast<float>(t - r.t_adv) * (1.0f + 1e-6f) + sc.eps;

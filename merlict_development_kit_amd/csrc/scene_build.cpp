@@ -108,6 +108,27 @@ class ReferencePoses {
     }
     const M34& frame2world(int32_t i) const { return f_[i].t2w; }
 
+    // Rank of every childless description frame in merlict's candidate order:
+    // the depth-first walk over the clustered children lists
+    // (RayAndFrame.cpp:137-149), whose first minimum wins exact ties of
+    // distance (RayAndFrame.cpp:168-172).
+    std::vector<int32_t> candidate_rank(int32_t n_desc) const {
+        std::vector<int32_t> rank(n_desc, -1);
+        int32_t next = 0;
+        std::vector<int32_t> todo{0};
+        while (!todo.empty()) {
+            const int32_t fi = todo.back();
+            todo.pop_back();
+            const std::vector<int32_t>& kids = f_[fi].children;
+            if (kids.empty()) {
+                if (fi < n_desc) rank[fi] = next++;
+                continue;
+            }
+            for (auto it = kids.rbegin(); it != kids.rend(); ++it) todo.push_back(*it);
+        }
+        return rank;
+    }
+
  private:
     struct F {
         int32_t mother;
@@ -374,6 +395,7 @@ class Builder {
         {
             const ReferencePoses rp(d_, radius);
             for (int32_t i = 0; i < n; i++) t2w_[i] = rp.frame2world(i);
+            rank_ = rp.candidate_rank(n);
         }
         scope_.assign(n, 0);
         scope_[0] = 1;
@@ -425,6 +447,7 @@ class Builder {
     const mlgpu_scene_desc& d_;
     HostScene* hs_;
     std::vector<M34> t2w_;
+    std::vector<int32_t> rank_;
     std::vector<std::vector<int32_t>> children_;
     std::vector<char> scope_;
     std::vector<BNode> bnodes_;
@@ -622,7 +645,7 @@ class Builder {
         g.ti[2] = m.T[0][2] * m.T[0][3] + m.T[1][2] * m.T[1][3] + m.T[2][2] * m.T[2][3];
         leaf_params(d_.kind[f], d_.params + MLGPU_NUM_PARAMS * f, g.p);
         g.kind = d_.kind[f];
-        g.pad0 = 0;
+        g.rank = rank_[f];
         LeafPhys ph;
         for (int s = 0; s < MLGPU_NUM_SURFACE_SLOTS; s++) {
             int32_t fn = d_.surface[MLGPU_NUM_SURFACE_SLOTS * f + s];

@@ -11,6 +11,8 @@ configurations and the reference's own sources:
   tilts N(0, 0.5 deg), lambda in [280, 700] nm with pdf ~ lambda^-2;
 * :func:`calibration` -- plenoscope calibration photons
   (merlict_portal_plenoscope/calibration/Calibrator.cpp:37-102);
+* :func:`plenoscope_bunch` -- a photon bunch for the plenoscope with a
+  per-bunch pointing offset;
 * :func:`point_like` and :func:`parallel_disc` --
   photon_source::point_like_towards_z_opening_angle_num_photons and
   parallel_towards_z_from_xy_disc (merlict/Photons.cpp:85-125).
@@ -128,6 +130,19 @@ def calibration(aperture_radius: float = 35.5 * 1.05, max_incident_angle: float 
     `distance` upstream (at aperture point + c distance), travelling -c."""
     return Source("disc", "cap", aperture_radius, max_incident_angle, wavelength, wavelength, distance,
                   FLIP, (0.0, 0.0, 0.0))
+
+
+def plenoscope_bunch(offset_x: float = 0.0, offset_y: float = 0.0, aperture_radius: float = 35.5 * 1.05,
+                     spread_deg: float = 0.05, distance: float = 1e3, lambda_min: float = 280e-9,
+                     lambda_max: float = 700e-9) -> Source:
+    """One bunch of Cherenkov-like photons for the plenoscope: supports on the
+    aperture disc, x and y tilts N(0, spread) about the bunch's arrival
+    direction, which is the optical axis turned by the pointing offsets
+    (rad, about x then y), started `distance` upstream, lambda with pdf ~
+    lambda^-2."""
+    src = Source("disc", "gaussian", aperture_radius, math.radians(spread_deg), lambda_min, lambda_max, distance,
+                 FLIP, (0.0, 0.0, 0.0))
+    return src.posed(rotation_xyz(offset_x, offset_y, 0.0))
 
 
 def point_like(opening_angle: float, wavelength: float = 433e-9) -> Source:

@@ -5,7 +5,7 @@ oracle/Makefile, with the Philox stream plugged into merlict's
 random::Generator) on seeded inputs and stores inputs and outputs as small
 .npz files.  Run in the container that has the reference mounted:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [plenoscope]
 """
 from __future__ import annotations
 
@@ -55,9 +55,24 @@ def probe_rays(n, seed):
             "direction_z": d[:, 2], "wavelength": np.full(n, 433e-9)}
 
 
+def plenoscope_fixtures():
+    """Plenoscope propagation: calibration photons and one bunch with a
+    pointing offset, drawn by the library's Philox sources (restated by the
+    oracle), up to 10 interactions as plenoscope_propagation.cpp:95."""
+    from merlict_development_kit_amd import sources as S
+    pleno = SceneDescription.plenoscope()
+    cal = O.generate_photons(S.calibration().to_c(O.Source), 2000, seed=1)
+    save_propagation("plenoscope_calibration_propagation", pleno, cal, seed=3, max_int=10)
+    bunch = O.generate_photons(S.plenoscope_bunch(0.02, -0.035).to_c(O.Source), 2000, seed=2)
+    save_propagation("plenoscope_bunch_propagation", pleno, bunch, seed=5, max_int=10)
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/merlict_golden is not built")
+    if len(sys.argv) > 1 and sys.argv[1] == "plenoscope":
+        plenoscope_fixtures()
+        return
     lens = SceneDescription.lens()
     save_propagation("lens_propagation", lens, sources.lens_beam(2000, seed=1).numpy(), seed=7)
     fact = SceneDescription.fact(light_guides=False)
@@ -95,6 +110,7 @@ def main():
                         lightfield_num_rows=np.array([len(rows)]),
                         lightfield_pos_sum=rows[:, 3:6].sum(axis=0), lightfield_radius_sum=np.array([rows[:, -1].sum()]))
     print("light field rows", len(rows), meta)
+    plenoscope_fixtures()
 
 
 if __name__ == "__main__":

@@ -36,6 +36,7 @@ SCENES = {
     "lens": lambda: SceneDescription.lens(),
     "fact": lambda: SceneDescription.fact(light_guides=False),
     "fact_lg": lambda: SceneDescription.fact(light_guides=True),
+    "plenoscope": lambda: SceneDescription.plenoscope(),
 }
 
 
@@ -57,7 +58,9 @@ def compare(gpu, ref, n, label):
 
 @pytest.mark.parametrize("name,scene", [("lens_propagation", "lens"), ("fact_propagation", "fact"),
                                         ("fact_lg_propagation", "fact_lg"),
-                                        ("fact_lg_max10_propagation", "fact_lg")])
+                                        ("fact_lg_max10_propagation", "fact_lg"),
+                                        ("plenoscope_calibration_propagation", "plenoscope"),
+                                        ("plenoscope_bunch_propagation", "plenoscope")])
 def test_gpu_matches_reference_golden(cuda, golden, name, scene):
     g = golden(name)
     photons, ref = split_golden(g)
@@ -80,6 +83,23 @@ def test_gpu_matches_oracle_many_photons(cuda, scene, schedule, oracle_threads):
     res = gs.propagate(ph.to(cuda), PropagationConfig(5, seed=99)).numpy()
     ref = O.OracleScene(desc).propagate(ph.numpy(), seed=99, threads=oracle_threads)
     compare(res, ref, n, f"{scene} ({schedule}) vs oracle")
+
+
+@pytest.mark.parametrize("schedule", ["fused", "wavefront"])
+def test_gpu_plenoscope_matches_oracle(cuda, schedule, oracle_threads):
+    """Plenoscope (635k leaves, 515,023 lixels): calibration photons from
+    1e4 m and a bunch with a pointing offset, drawn by the GPU sources."""
+    from merlict_development_kit_amd import sources as S
+    desc = SCENES["plenoscope"]()
+    gs = GpuScene(desc)
+    gs.set_tuning(schedule=schedule)
+    oracle = O.OracleScene(desc)
+    for label, src, seed in (("calibration", S.calibration(), 31), ("bunch", S.plenoscope_bunch(-0.03, 0.01), 32)):
+        n = 100_000
+        ph = src.generate(n, seed=seed, device=cuda)
+        res = gs.propagate(ph, PropagationConfig(10, seed=7)).numpy()
+        ref = oracle.propagate(ph.numpy(), seed=7, max_num_interactions=10, threads=oracle_threads)
+        compare(res, ref, n, f"plenoscope {label} ({schedule}) vs oracle")
 
 
 @pytest.mark.parametrize("max_int", [1, 5, 12])

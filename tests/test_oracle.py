@@ -28,11 +28,14 @@ def scene_for(name):
         "fact_propagation": lambda: SceneDescription.fact(light_guides=False),
         "fact_lg_propagation": lambda: SceneDescription.fact(light_guides=True),
         "fact_lg_max10_propagation": lambda: SceneDescription.fact(light_guides=True),
+        "plenoscope_calibration_propagation": lambda: SceneDescription.plenoscope(),
+        "plenoscope_bunch_propagation": lambda: SceneDescription.plenoscope(),
     }[name]()
 
 
 @pytest.mark.parametrize("name", ["lens_propagation", "fact_propagation", "fact_lg_propagation",
-                                  "fact_lg_max10_propagation"])
+                                  "fact_lg_max10_propagation", "plenoscope_calibration_propagation",
+                                  "plenoscope_bunch_propagation"])
 def test_oracle_matches_reference_golden(golden, name, oracle_threads):
     g = golden(name)
     photons, ref = split_golden(g)
@@ -326,6 +329,19 @@ def test_source_pose_and_start_distance():
     sb = np.stack([b["support_x"], b["support_y"], b["support_z"]], 1)
     np.testing.assert_allclose(sb, sa @ R.T + [1.0, 2.0, 3.0], atol=1e-12)
     np.testing.assert_allclose(np.asarray(rot).reshape(3, 3) @ np.asarray(rot).reshape(3, 3).T, np.eye(3), atol=1e-15)
+
+
+def test_plenoscope_golden_inputs_are_the_library_sources(golden):
+    """The plenoscope fixtures were drawn by the Philox sources: the oracle's
+    restatement reproduces their inputs bit for bit."""
+    from merlict_development_kit_amd import sources as S
+    for name, src, seed in (("plenoscope_calibration_propagation", S.calibration(), 1),
+                            ("plenoscope_bunch_propagation", S.plenoscope_bunch(0.02, -0.035), 2)):
+        g = golden(name)
+        photons, _ = split_golden(g)
+        again = O.generate_photons(src.to_c(O.Source), len(photons["support_x"]), seed=seed)
+        for k in again:
+            np.testing.assert_array_equal(photons[k], again[k], err_msg=f"{name} {k}")
 
 
 def test_source_draws_are_keyed_by_photon_id():
