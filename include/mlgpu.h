@@ -288,6 +288,52 @@ int mlgpu_generate_photons(const mlgpu_source* source, int64_t n, uint64_t seed,
                            uint64_t photon_id_offset, const mlgpu_photon_buffers* out,
                            void* stream);
 
+/* ------------------------------------------------------- sensor arrivals
+ * Arrival-time histograms per sensor.  Sensor::assign_photon counts a photon
+ * on the sensor of its final frame whatever its final interaction
+ * (merlict/sensor/Sensor.cpp:14-30); here counts[s * num_bins + b] += 1 for
+ * every photon without error flag whose sensor_id s is in [0, num_sensors)
+ * and whose arrival time t = time_of_flight + jitter falls in bin
+ * b = floor((t - t_start) / bin_width) < num_bins.  jitter_sigma > 0 adds
+ * N(0, jitter_sigma) drawn from Philox stream 2 of
+ * (seed, photon_id_offset + i).  Counts accumulate; clear them first. */
+typedef struct mlgpu_histogram {
+    uint32_t* counts;          /* device, num_sensors * num_bins */
+    int32_t num_sensors;
+    int32_t num_bins;
+    double t_start;            /* s */
+    double bin_width;          /* s */
+    double jitter_sigma;       /* s */
+    uint64_t seed;
+    uint64_t photon_id_offset;
+} mlgpu_histogram;
+int mlgpu_histogram_arrivals(const mlgpu_results* results, int64_t n,
+                             const mlgpu_histogram* hist, void* stream);
+
+/* ------------------------------------------------------------ photon bunches
+ * Many bunches of synthetic photons in one call, all on the device: bunch b
+ * draws photons 0 .. num_photons - 1 of its own source under its own seed
+ * (its source pose carries the bunch's pointing, as transform_photons does,
+ * merlict_json/json.cpp:512-519), the bunches are propagated together
+ * (photon i of their concatenation uses Philox stream 0 of (config seed,
+ * config photon_id_offset + i)), and, when hist is not NULL, their arrivals
+ * are binned into it (photon i as hist photon_id_offset + i).  results, when
+ * not NULL, receives the per-photon results of the concatenation (device
+ * arrays of the total length).  Replaces the per-event loop of
+ * plenoscope_propagation.cpp:160-220 (read, transform, propagate, record).
+ * The photons live in the scene's device memory; calls sharing a scene are
+ * serialised on it. */
+typedef struct mlgpu_bunch {
+    mlgpu_source source;
+    uint64_t seed;
+    int64_t num_photons;
+} mlgpu_bunch;
+int mlgpu_propagate_bunches(const mlgpu_scene* scene, int32_t world_frame,
+                            const mlgpu_bunch* bunches, int32_t num_bunches,
+                            const mlgpu_config* config,
+                            const mlgpu_histogram* hist, mlgpu_results* results,
+                            void* stream);
+
 /* Number of photons with a non-zero error flag in the last n results
  * (synchronises `stream`). */
 int mlgpu_count_errors(const mlgpu_results* results, int64_t n, void* stream,

@@ -10,6 +10,8 @@ from .scenery import SceneDescription
 from .propagation import (GpuScene, PhotonBatch, PropagationConfig, PropagationResult,
                           propagate_photons_in_frame_with_config)
 from . import sources
+from .arrivals import ArrivalBinning, histogram_arrivals
+from .bunches import Bunch, propagate_bunches
 
 __all__ = [
     "SceneDescription",
@@ -19,4 +21,8 @@ __all__ = [
     "PropagationResult",
     "propagate_photons_in_frame_with_config",
     "sources",
+    "ArrivalBinning",
+    "histogram_arrivals",
+    "Bunch",
+    "propagate_bunches",
 ]

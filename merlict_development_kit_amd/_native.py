@@ -94,6 +94,27 @@ class Source(C.Structure):
     ]
 
 
+class Bunch(C.Structure):
+    _fields_ = [
+        ("source", Source),
+        ("seed", C.c_uint64),
+        ("num_photons", C.c_int64),
+    ]
+
+
+class Histogram(C.Structure):
+    _fields_ = [
+        ("counts", C.c_void_p),
+        ("num_sensors", C.c_int32),
+        ("num_bins", C.c_int32),
+        ("t_start", C.c_double),
+        ("bin_width", C.c_double),
+        ("jitter_sigma", C.c_double),
+        ("seed", C.c_uint64),
+        ("photon_id_offset", C.c_uint64),
+    ]
+
+
 class Results(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in (
         "final_frame", "sensor_id", "final_interaction", "num_interactions", "error", "time_of_flight",
@@ -141,6 +162,9 @@ SIGNATURES = {
                                   C.POINTER(Results), C.c_void_p]),
     "mlgpu_generate_photons": (C.c_int, [C.POINTER(Source), C.c_int64, C.c_uint64, C.c_uint64,
                                          C.POINTER(PhotonBuffers), C.c_void_p]),
+    "mlgpu_propagate_bunches": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Bunch), C.c_int32, C.POINTER(Config),
+                                          C.POINTER(Histogram), C.POINTER(Results), C.c_void_p]),
+    "mlgpu_histogram_arrivals": (C.c_int, [C.POINTER(Results), C.c_int64, C.POINTER(Histogram), C.c_void_p]),
     "mlgpu_count_errors": (C.c_int, [C.POINTER(Results), C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
     "mlgpu_first_hit": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Hits),
                                   C.c_void_p]),

@@ -3,6 +3,7 @@
 // codes plus mlgpu_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -29,6 +30,15 @@ struct mlgpu_scene {
     int64_t ws_capacity = 0;
     hipEvent_t ws_event = nullptr;
     hipStream_t ws_stream = nullptr;
+    // bunch arena (photons, results, bunch table), guarded like ws; the
+    // table goes up from a pinned staging buffer that is rewritten only
+    // after the previous upload from it has finished
+    void* arena = nullptr;
+    int64_t arena_photons = 0;
+    int32_t arena_bunches = 0;
+    mlgpu::BunchDev* staging = nullptr;
+    int32_t staging_bunches = 0;
+    hipEvent_t staging_event = nullptr;
 };
 
 namespace {
@@ -59,6 +69,243 @@ int upload(mlgpu_scene* s, const std::vector<T>& v, const T** out) {
 bool photons_ok(const mlgpu_photons* p) {
     return p && p->support_x && p->support_y && p->support_z && p->direction_x && p->direction_y &&
            p->direction_z && p->wavelength;
+}
+
+
+int check_results(const mlgpu_results* r) {
+    if (!r->final_frame || !r->sensor_id || !r->final_interaction || !r->num_interactions || !r->error ||
+        !r->time_of_flight || !r->x || !r->y || !r->cx || !r->cy) {
+        set_last_error("null result array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    return MLGPU_OK;
+}
+
+int check_world(const mlgpu_scene* s, int32_t world_frame) {
+    if (world_frame < 0 || world_frame >= static_cast<int32_t>(s->scope_of_frame.size()) ||
+        s->scope_of_frame[world_frame] < 0) {
+        set_last_error("world_frame must be frame 0 or a restriction frame");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    return MLGPU_OK;
+}
+
+int check_config(const mlgpu_config* cfg) {
+    if (cfg->max_num_interactions_per_photon > 250) {
+        set_last_error("max_num_interactions_per_photon must be <= 250");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    return MLGPU_OK;
+}
+
+int check_source(const mlgpu_source* src) {
+    if (src->support != MLGPU_SUPPORT_POINT && src->support != MLGPU_SUPPORT_DISC) {
+        set_last_error("bad support kind");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (src->direction < MLGPU_DIRECTION_AXIS || src->direction > MLGPU_DIRECTION_CAP) {
+        set_last_error("bad direction kind");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const bool bounded = src->direction == MLGPU_DIRECTION_TILT || src->direction == MLGPU_DIRECTION_CAP;
+    if (!(src->angle >= 0.0) || (bounded && !(src->angle <= 1.57079632679489661923)) || !std::isfinite(src->angle)) {
+        set_last_error("angle must be finite and >= 0, and <= pi/2 for TILT and CAP");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (!(src->radius >= 0.0) || !std::isfinite(src->radius) || !std::isfinite(src->start_distance)) {
+        set_last_error("radius must be finite and >= 0, start_distance finite");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (!(src->lambda_min > 0.0) || !(src->lambda_max >= src->lambda_min) || !std::isfinite(src->lambda_max)) {
+        set_last_error("need 0 < lambda_min <= lambda_max < inf");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    for (int k = 0; k < 9; k++)
+        if (!std::isfinite(src->rot[k])) { set_last_error("pose must be finite"); return MLGPU_ERR_ARGUMENT; }
+    for (int k = 0; k < 3; k++)
+        if (!std::isfinite(src->pos[k])) { set_last_error("pose must be finite"); return MLGPU_ERR_ARGUMENT; }
+    return MLGPU_OK;
+}
+
+int check_histogram(const mlgpu_histogram* h) {
+    if (!h->counts) { set_last_error("null histogram counts"); return MLGPU_ERR_ARGUMENT; }
+    if (h->num_sensors < 1 || h->num_bins < 1 ||
+        static_cast<int64_t>(h->num_sensors) * h->num_bins > (int64_t{1} << 40)) {
+        set_last_error("need num_sensors >= 1 and num_bins >= 1");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (!(h->bin_width > 0.0) || !std::isfinite(h->bin_width) || !std::isfinite(h->t_start) ||
+        !(h->jitter_sigma >= 0.0) || !std::isfinite(h->jitter_sigma)) {
+        set_last_error("need finite t_start, bin_width > 0 and jitter_sigma >= 0");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    return MLGPU_OK;
+}
+
+int schedule_for(const mlgpu_scene* s, int64_t n) {
+    const int schedule = s->tuning.schedule;
+    if (schedule != MLGPU_SCHEDULE_AUTO) return schedule;
+    return n >= 65536 ? MLGPU_SCHEDULE_WAVEFRONT : MLGPU_SCHEDULE_FUSED;
+}
+
+// Calls that use the scene's shared device memory (wavefront workspace,
+// bunch arena) hold ws_mutex on the host; on the device, a call on another
+// stream than the previous user first waits for that user's last kernel.
+int begin_shared_use(mlgpu_scene* ms, hipStream_t st) {
+    hipError_t e;
+    if (!ms->ws_event) {
+        e = hipEventCreateWithFlags(&ms->ws_event, hipEventDisableTiming);
+        if (e != hipSuccess) { ms->ws_event = nullptr; return hip_fail(e, "hipEventCreate"); }
+        ms->ws_stream = st;
+    } else if (ms->ws_stream != st) {
+        e = hipStreamWaitEvent(st, ms->ws_event, 0);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+    }
+    return MLGPU_OK;
+}
+
+void end_shared_use(mlgpu_scene* ms, hipStream_t st) {
+    (void)hipEventRecord(ms->ws_event, st);
+    ms->ws_stream = st;
+}
+
+// (Re)allocate a shared buffer once no earlier call can still use it.
+int grow(mlgpu_scene* ms, void** buf, size_t bytes) {
+    if (*buf) {
+        (void)hipEventSynchronize(ms->ws_event);
+        (void)hipFree(*buf);
+        *buf = nullptr;
+    }
+    const hipError_t e = hipMalloc(buf, bytes);
+    if (e != hipSuccess) { *buf = nullptr; return hip_fail(e, "hipMalloc"); }
+    return MLGPU_OK;
+}
+
+// Hands out 256-byte aligned pieces of a buffer.
+struct Carver {
+    char* p;
+    template <typename T>
+    T* take(size_t count) {
+        T* q = reinterpret_cast<T*>(p);
+        p += (count * sizeof(T) + 255) & ~static_cast<size_t>(255);
+        return q;
+    }
+};
+
+int launch_fused(const mlgpu_scene* s, int32_t scope, const mlgpu_photons& ph, int64_t n, const mlgpu_config& cfg,
+                 const mlgpu_results& r, hipStream_t st) {
+    const hipError_t e = mlgpu::launch_propagate(s->dev, scope, ph, n, cfg.max_num_interactions_per_photon, cfg.seed,
+                                                 cfg.photon_id_offset, r, st);
+    if (e != hipSuccess) return hip_fail(e, "propagate launch");
+    return MLGPU_OK;
+}
+
+// Wavefront propagation; the caller holds ws_mutex and began shared use.
+int launch_wavefront(mlgpu_scene* ms, int32_t scope, const mlgpu_photons& ph, int64_t n, const mlgpu_config& cfg,
+                     const mlgpu_results& r, hipStream_t st) {
+    if (ms->ws_capacity < n) {
+        const int64_t cap = n + n / 8;
+        const int rc = grow(ms, &ms->ws, mlgpu::workspace_bytes(cap));
+        ms->ws_capacity = rc ? 0 : cap;
+        if (rc) return rc;
+    }
+    const size_t cap = static_cast<size_t>(ms->ws_capacity);
+    Carver c{static_cast<char*>(ms->ws)};
+    mlgpu::Workspace w;
+    w.cap = ms->ws_capacity;
+    w.sx = c.take<double>(cap);
+    w.sy = c.take<double>(cap);
+    w.sz = c.take<double>(cap);
+    w.dx = c.take<double>(cap);
+    w.dy = c.take<double>(cap);
+    w.dz = c.take<double>(cap);
+    w.tof = c.take<double>(cap);
+    w.root = c.take<int32_t>(cap);
+    w.n_hist = c.take<uint32_t>(cap);
+    w.draws = c.take<uint32_t>(cap);
+    w.hit_leaf = c.take<int32_t>(cap);
+    w.hit_t = c.take<double>(cap);
+    w.tr_cur = c.take<int32_t>(cap);
+    w.tr_depth = c.take<int32_t>(cap);
+    w.tr_stack = c.take<int32_t>(cap * mlgpu::kMaxTreeDepth);
+    w.deferred = c.take<int32_t>(cap);
+    w.queue[0] = c.take<int32_t>(cap);
+    w.queue[1] = c.take<int32_t>(cap);
+    w.counts = c.take<int32_t>(mlgpu::kNumCounts);
+    const hipError_t e = mlgpu::launch_propagate_wavefront(
+        ms->dev, scope, ph, n, cfg.max_num_interactions_per_photon, cfg.seed, cfg.photon_id_offset, r, w,
+        ms->tuning.blocks_per_cu * ms->num_cus, ms->tuning.trace_budget, st);
+    if (e != hipSuccess) return hip_fail(e, "propagate launch");
+    return MLGPU_OK;
+}
+
+size_t arena_bytes(int64_t photons, int32_t bunches) {
+    const size_t m = static_cast<size_t>(photons);
+    // 7 photon arrays, the result arrays, the bunch table, alignment slack
+    return m * (7 * 8 + 4 + 4 + 1 + 1 + 1 + 8 + 4 * 4) + sizeof(mlgpu::BunchDev) * static_cast<size_t>(bunches) +
+           20 * 256;
+}
+
+// Draw, propagate and bin bunches; the caller holds ws_mutex and began shared use.
+int run_bunches(mlgpu_scene* ms, int32_t scope, const mlgpu_bunch* bunches, int32_t nb, int64_t total,
+                const mlgpu_config& cfg, const mlgpu_histogram* hist, const mlgpu_results* results, hipStream_t st) {
+    int rc;
+    hipError_t e;
+    if (ms->arena_photons < total || ms->arena_bunches < nb) {
+        const int64_t cap = std::max(total + total / 8, ms->arena_photons);
+        const int32_t cb = std::max(nb, ms->arena_bunches);
+        rc = grow(ms, &ms->arena, arena_bytes(cap, cb));
+        ms->arena_photons = rc ? 0 : cap;
+        ms->arena_bunches = rc ? 0 : cb;
+        if (rc) return rc;
+    }
+    // bunch table through the pinned staging buffer
+    if (!ms->staging_event) {
+        e = hipEventCreateWithFlags(&ms->staging_event, hipEventDisableTiming);
+        if (e != hipSuccess) { ms->staging_event = nullptr; return hip_fail(e, "hipEventCreate"); }
+    } else {
+        (void)hipEventSynchronize(ms->staging_event);
+    }
+    if (ms->staging_bunches < nb) {
+        if (ms->staging) (void)hipHostFree(ms->staging);
+        ms->staging = nullptr;
+        ms->staging_bunches = 0;
+        e = hipHostMalloc(reinterpret_cast<void**>(&ms->staging), sizeof(mlgpu::BunchDev) * static_cast<size_t>(nb));
+        if (e != hipSuccess) { ms->staging = nullptr; return hip_fail(e, "hipHostMalloc"); }
+        ms->staging_bunches = nb;
+    }
+    int64_t start = 0;
+    for (int32_t b = 0; b < nb; b++) {
+        ms->staging[b].source = bunches[b].source;
+        ms->staging[b].seed = bunches[b].seed;
+        ms->staging[b].start = start;
+        start += bunches[b].num_photons;
+    }
+    const size_t cap = static_cast<size_t>(ms->arena_photons);
+    Carver c{static_cast<char*>(ms->arena)};
+    auto* table = c.take<mlgpu::BunchDev>(static_cast<size_t>(ms->arena_bunches));
+    mlgpu_photon_buffers gen{c.take<double>(cap), c.take<double>(cap), c.take<double>(cap), c.take<double>(cap),
+                             c.take<double>(cap), c.take<double>(cap), c.take<double>(cap)};
+    mlgpu_results own{c.take<int32_t>(cap), c.take<int32_t>(cap), c.take<uint8_t>(cap), c.take<uint8_t>(cap),
+                      c.take<uint8_t>(cap), c.take<double>(cap), c.take<float>(cap), c.take<float>(cap),
+                      c.take<float>(cap), c.take<float>(cap)};
+    e = hipMemcpyAsync(table, ms->staging, sizeof(mlgpu::BunchDev) * static_cast<size_t>(nb), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(ms->staging_event, st);
+    if (e == hipSuccess) e = mlgpu::launch_generate_bunches(table, nb, total, gen, st);
+    if (e != hipSuccess) return hip_fail(e, "bunch upload or generation");
+    const mlgpu_photons ph{gen.support_x, gen.support_y, gen.support_z, gen.direction_x, gen.direction_y,
+                           gen.direction_z, gen.wavelength};
+    const mlgpu_results& r = results ? *results : own;
+    if (schedule_for(ms, total) == MLGPU_SCHEDULE_FUSED)
+        rc = launch_fused(ms, scope, ph, total, cfg, r, st);
+    else
+        rc = launch_wavefront(ms, scope, ph, total, cfg, r, st);
+    if (rc) return rc;
+    if (hist) {
+        e = mlgpu::launch_histogram(r, total, *hist, ms->num_cus, st);
+        if (e != hipSuccess) return hip_fail(e, "histogram launch");
+    }
+    return MLGPU_OK;
 }
 
 }  // namespace
@@ -113,6 +360,12 @@ void mlgpu_scene_destroy(mlgpu_scene* s) {
         (void)hipEventSynchronize(s->ws_event);
         (void)hipEventDestroy(s->ws_event);
     }
+    if (s->staging_event) {
+        (void)hipEventSynchronize(s->staging_event);
+        (void)hipEventDestroy(s->staging_event);
+    }
+    if (s->staging) (void)hipHostFree(s->staging);
+    if (s->arena) (void)hipFree(s->arena);
     for (void* p : s->buffers) (void)hipFree(p);
     if (s->ws) (void)hipFree(s->ws);
     delete s;
@@ -151,84 +404,43 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
     if (!s || !cfg || !r || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
     if (n == 0) return MLGPU_OK;
     if (!photons_ok(photons)) { set_last_error("null photon array"); return MLGPU_ERR_ARGUMENT; }
-    if (!r->final_frame || !r->sensor_id || !r->final_interaction || !r->num_interactions || !r->error ||
-        !r->time_of_flight || !r->x || !r->y || !r->cx || !r->cy) {
-        set_last_error("null result array");
-        return MLGPU_ERR_ARGUMENT;
-    }
-    if (world_frame < 0 || world_frame >= static_cast<int32_t>(s->scope_of_frame.size()) ||
-        s->scope_of_frame[world_frame] < 0) {
-        set_last_error("world_frame must be frame 0 or a restriction frame");
-        return MLGPU_ERR_ARGUMENT;
-    }
-    if (cfg->max_num_interactions_per_photon > 250) {
-        set_last_error("max_num_interactions_per_photon must be <= 250");
-        return MLGPU_ERR_ARGUMENT;
-    }
+    int rc;
+    if ((rc = check_results(r)) || (rc = check_world(s, world_frame)) || (rc = check_config(cfg))) return rc;
     if (n > INT32_MAX) { set_last_error("at most 2^31 - 1 photons per call"); return MLGPU_ERR_ARGUMENT; }
-    int schedule = s->tuning.schedule;
-    if (schedule == MLGPU_SCHEDULE_AUTO) schedule = n >= 65536 ? MLGPU_SCHEDULE_WAVEFRONT : MLGPU_SCHEDULE_FUSED;
+    auto* ms = const_cast<mlgpu_scene*>(s);
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    hipError_t e;
-    if (schedule == MLGPU_SCHEDULE_FUSED) {
-        e = mlgpu::launch_propagate(s->dev, s->scope_of_frame[world_frame], *photons, n,
-                                    cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset, *r, st);
-    } else {
-        auto* ms = const_cast<mlgpu_scene*>(s);
-        std::lock_guard<std::mutex> lock(ms->ws_mutex);
-        if (!ms->ws_event) {
-            e = hipEventCreateWithFlags(&ms->ws_event, hipEventDisableTiming);
-            if (e != hipSuccess) { ms->ws_event = nullptr; return hip_fail(e, "hipEventCreate"); }
-        } else if (ms->ws_stream != st) {
-            e = hipStreamWaitEvent(st, ms->ws_event, 0);
-            if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
-        }
-        if (ms->ws_capacity < n) {
-            if (ms->ws) {
-                (void)hipEventSynchronize(ms->ws_event);
-                (void)hipFree(ms->ws);
-                ms->ws = nullptr;
-                ms->ws_capacity = 0;
-            }
-            const int64_t cap = n + n / 8;
-            e = hipMalloc(&ms->ws, mlgpu::workspace_bytes(cap));
-            if (e != hipSuccess) return hip_fail(e, "hipMalloc workspace");
-            ms->ws_capacity = cap;
-        }
-        // carve the workspace into 256-byte aligned arrays
-        char* p = static_cast<char*>(ms->ws);
-        const size_t cap = static_cast<size_t>(ms->ws_capacity);
-        auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) & ~static_cast<size_t>(255); return q; };
-        mlgpu::Workspace w;
-        w.cap = ms->ws_capacity;
-        w.sx = reinterpret_cast<double*>(take(8 * cap));
-        w.sy = reinterpret_cast<double*>(take(8 * cap));
-        w.sz = reinterpret_cast<double*>(take(8 * cap));
-        w.dx = reinterpret_cast<double*>(take(8 * cap));
-        w.dy = reinterpret_cast<double*>(take(8 * cap));
-        w.dz = reinterpret_cast<double*>(take(8 * cap));
-        w.tof = reinterpret_cast<double*>(take(8 * cap));
-        w.root = reinterpret_cast<int32_t*>(take(4 * cap));
-        w.n_hist = reinterpret_cast<uint32_t*>(take(4 * cap));
-        w.draws = reinterpret_cast<uint32_t*>(take(4 * cap));
-        w.hit_leaf = reinterpret_cast<int32_t*>(take(4 * cap));
-        w.hit_t = reinterpret_cast<double*>(take(8 * cap));
-        w.tr_cur = reinterpret_cast<int32_t*>(take(4 * cap));
-        w.tr_depth = reinterpret_cast<int32_t*>(take(4 * cap));
-        w.tr_stack = reinterpret_cast<int32_t*>(take(4 * cap * mlgpu::kMaxTreeDepth));
-        w.deferred = reinterpret_cast<int32_t*>(take(4 * cap));
-        w.queue[0] = reinterpret_cast<int32_t*>(take(4 * cap));
-        w.queue[1] = reinterpret_cast<int32_t*>(take(4 * cap));
-        w.counts = reinterpret_cast<int32_t*>(take(4 * mlgpu::kNumCounts));
-        e = mlgpu::launch_propagate_wavefront(s->dev, s->scope_of_frame[world_frame], *photons, n,
-                                              cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset,
-                                              *r, w, s->tuning.blocks_per_cu * s->num_cus,
-                                              s->tuning.trace_budget, st);
-        if (e == hipSuccess) e = hipEventRecord(ms->ws_event, st);
-        ms->ws_stream = st;
+    if (schedule_for(s, n) == MLGPU_SCHEDULE_FUSED)
+        return launch_fused(s, s->scope_of_frame[world_frame], *photons, n, *cfg, *r, st);
+    std::lock_guard<std::mutex> lock(ms->ws_mutex);
+    if ((rc = begin_shared_use(ms, st))) return rc;
+    rc = launch_wavefront(ms, s->scope_of_frame[world_frame], *photons, n, *cfg, *r, st);
+    end_shared_use(ms, st);
+    return rc;
+}
+
+int mlgpu_propagate_bunches(const mlgpu_scene* s, int32_t world_frame, const mlgpu_bunch* bunches,
+                            int32_t num_bunches, const mlgpu_config* cfg, const mlgpu_histogram* hist,
+                            mlgpu_results* results, void* stream) {
+    if (!s || !cfg || !bunches || num_bunches < 1) { set_last_error("null argument or no bunch"); return MLGPU_ERR_ARGUMENT; }
+    int rc;
+    if ((rc = check_world(s, world_frame)) || (rc = check_config(cfg))) return rc;
+    if (results && (rc = check_results(results))) return rc;
+    if (hist && (rc = check_histogram(hist))) return rc;
+    int64_t total = 0;
+    for (int32_t b = 0; b < num_bunches; b++) {
+        if (bunches[b].num_photons < 0) { set_last_error("negative bunch size"); return MLGPU_ERR_ARGUMENT; }
+        if ((rc = check_source(&bunches[b].source))) return rc;
+        total += bunches[b].num_photons;
+        if (total > INT32_MAX) { set_last_error("at most 2^31 - 1 photons per call"); return MLGPU_ERR_ARGUMENT; }
     }
-    if (e != hipSuccess) return hip_fail(e, "propagate launch");
-    return MLGPU_OK;
+    if (total == 0) return MLGPU_OK;
+    auto* ms = const_cast<mlgpu_scene*>(s);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(ms->ws_mutex);
+    if ((rc = begin_shared_use(ms, st))) return rc;
+    rc = run_bunches(ms, s->scope_of_frame[world_frame], bunches, num_bunches, total, *cfg, hist, results, st);
+    end_shared_use(ms, st);
+    return rc;
 }
 
 int mlgpu_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed, uint64_t photon_id_offset,
@@ -240,29 +452,27 @@ int mlgpu_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed, ui
         set_last_error("null photon array");
         return MLGPU_ERR_ARGUMENT;
     }
-    if (src->support != MLGPU_SUPPORT_POINT && src->support != MLGPU_SUPPORT_DISC) {
-        set_last_error("bad support kind");
-        return MLGPU_ERR_ARGUMENT;
-    }
-    if (src->direction < MLGPU_DIRECTION_AXIS || src->direction > MLGPU_DIRECTION_CAP) {
-        set_last_error("bad direction kind");
-        return MLGPU_ERR_ARGUMENT;
-    }
-    const bool bounded = src->direction == MLGPU_DIRECTION_TILT || src->direction == MLGPU_DIRECTION_CAP;
-    if (!(src->angle >= 0.0) || (bounded && !(src->angle <= 1.57079632679489661923)) || !std::isfinite(src->angle)) {
-        set_last_error("angle must be finite and >= 0, and <= pi/2 for TILT and CAP");
-        return MLGPU_ERR_ARGUMENT;
-    }
-    if (!(src->radius >= 0.0) || !std::isfinite(src->radius) || !std::isfinite(src->start_distance)) {
-        set_last_error("radius must be finite and >= 0, start_distance finite");
-        return MLGPU_ERR_ARGUMENT;
-    }
-    if (!(src->lambda_min > 0.0) || !(src->lambda_max >= src->lambda_min) || !std::isfinite(src->lambda_max)) {
-        set_last_error("need 0 < lambda_min <= lambda_max < inf");
-        return MLGPU_ERR_ARGUMENT;
-    }
+    const int rc = check_source(src);
+    if (rc) return rc;
     const hipError_t e = mlgpu::launch_generate(*src, n, seed, photon_id_offset, *out, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "generate launch");
+    return MLGPU_OK;
+}
+
+int mlgpu_histogram_arrivals(const mlgpu_results* r, int64_t n, const mlgpu_histogram* h, void* stream) {
+    if (!r || !h || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (n == 0) return MLGPU_OK;
+    if (!r->sensor_id || !r->error || !r->time_of_flight) {
+        set_last_error("null array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const int rc = check_histogram(h);
+    if (rc) return rc;
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    const hipError_t e = mlgpu::launch_histogram(*r, n, *h, cus, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "histogram launch");
     return MLGPU_OK;
 }
 

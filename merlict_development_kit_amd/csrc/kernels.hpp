@@ -48,8 +48,22 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
 hipError_t launch_first_hit(const DeviceScene& sc, int32_t root_scope, const mlgpu_photons& rays, int64_t n,
                             const mlgpu_hits& out, hipStream_t stream);
 
+// A bunch on the device: its source, seed and first index in the
+// concatenation of all bunches of a call.
+struct BunchDev {
+    mlgpu_source source;
+    uint64_t seed;
+    int64_t start;
+};
+
+hipError_t launch_generate_bunches(const BunchDev* table, int32_t num_bunches, int64_t total,
+                                   const mlgpu_photon_buffers& out, hipStream_t stream);
+
 hipError_t launch_generate(const mlgpu_source& src, int64_t n, uint64_t seed, uint64_t id_offset,
                            const mlgpu_photon_buffers& out, hipStream_t stream);
+
+hipError_t launch_histogram(const mlgpu_results& r, int64_t n, const mlgpu_histogram& h, int num_cus,
+                            hipStream_t stream);
 
 hipError_t launch_count_errors(const uint8_t* err, int64_t n, unsigned long long* counter, hipStream_t stream);
 

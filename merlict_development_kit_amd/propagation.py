@@ -130,6 +130,11 @@ class GpuScene:
                 pass
             self._h = None
 
+    @property
+    def handle(self) -> C.c_void_p:
+        """The mlgpu_scene handle."""
+        return self._h
+
     _SCHEDULES = {"auto": N.SCHEDULE_AUTO, "fused": N.SCHEDULE_FUSED, "wavefront": N.SCHEDULE_WAVEFRONT}
 
     def tuning(self) -> dict:

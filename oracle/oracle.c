@@ -954,6 +954,33 @@ void oracle_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed,
     }
 }
 
+/* Arrival-time jitter: Box-Muller normal from Philox block (0, 2, id_lo, id_hi) */
+double oracle_arrival_jitter(uint64_t seed, uint64_t photon_id) {
+    const uint32_t ctr[4] = {0u, 2u, (uint32_t)photon_id, (uint32_t)(photon_id >> 32)};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t w[4];
+    double s, c;
+    oracle_philox4x32_10(ctr, key, w);
+    const double rad = sqrt(-2.0 * oracle_log_series(1.0 - src_u53(w[0], w[1])));
+    oracle_sincos_turn(src_u53(w[2], w[3]), &s, &c);
+    return rad * c;
+}
+
+void oracle_histogram_arrivals(const int32_t* sensor_id, const uint8_t* error,
+                               const double* time_of_flight, int64_t n,
+                               const mlgpu_histogram* h, uint32_t* counts) {
+    for (int64_t i = 0; i < n; i++) {
+        const int32_t sensor = sensor_id[i];
+        if (error[i] != 0 || sensor < 0 || sensor >= h->num_sensors) continue;
+        double t = time_of_flight[i];
+        if (h->jitter_sigma > 0.0)
+            t = t + h->jitter_sigma * oracle_arrival_jitter(h->seed, h->photon_id_offset + (uint64_t)i);
+        const double bin = (t - h->t_start) / h->bin_width;
+        if (bin >= 0.0 && bin < (double)h->num_bins)
+            counts[(int64_t)sensor * h->num_bins + (int64_t)bin] += 1u;
+    }
+}
+
 /* -------------------------------------------------------------- propagator */
 #define C_VACUUM 299792458.0  /* PhysicalConstants.h:7 */
 

@@ -96,6 +96,13 @@ void oracle_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed,
 double oracle_log_series(double x);
 void oracle_sincos_turn(double u, double* s, double* c);
 
+/* Arrival-time histograms (mlgpu_histogram of include/mlgpu.h) of host
+ * result arrays: counts (host, num_sensors * num_bins) accumulate. */
+void oracle_histogram_arrivals(const int32_t* sensor_id, const uint8_t* error,
+                               const double* time_of_flight, int64_t n,
+                               const mlgpu_histogram* hist, uint32_t* counts);
+double oracle_arrival_jitter(uint64_t seed, uint64_t photon_id);
+
 #ifdef __cplusplus
 }
 #endif

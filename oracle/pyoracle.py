@@ -49,6 +49,15 @@ class Source(C.Structure):
     ]
 
 
+class Histogram(C.Structure):
+    """mlgpu_histogram of include/mlgpu.h."""
+    _fields_ = [
+        ("counts", C.c_void_p), ("num_sensors", C.c_int32), ("num_bins", C.c_int32), ("t_start", C.c_double),
+        ("bin_width", C.c_double), ("jitter_sigma", C.c_double), ("seed", C.c_uint64),
+        ("photon_id_offset", C.c_uint64),
+    ]
+
+
 PHOTON_FIELDS = ("support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z", "wavelength")
 
 _lib = None
@@ -77,6 +86,10 @@ def lib() -> C.CDLL:
         L.oracle_func_evaluate.argtypes = [C.c_void_p, C.c_int32, C.c_double, _P_F64]
         L.oracle_generate_photons.argtypes = [C.POINTER(Source), C.c_int64, C.c_uint64, C.c_uint64,
                                               C.POINTER(_P_F64)]
+        L.oracle_histogram_arrivals.argtypes = [_P_I32, _P_U8, _P_F64, C.c_int64, C.POINTER(Histogram),
+                                                C.POINTER(C.c_uint32)]
+        L.oracle_arrival_jitter.restype = C.c_double
+        L.oracle_arrival_jitter.argtypes = [C.c_uint64, C.c_uint64]
         L.oracle_log_series.restype = C.c_double
         L.oracle_log_series.argtypes = [C.c_double]
         L.oracle_sincos_turn.argtypes = [C.c_double, _P_F64, _P_F64]
@@ -179,6 +192,23 @@ def generate_photons(src: Source, n: int, seed: int = 0, photon_id_offset: int =
     arr = (_P_F64 * 7)(*[_ptr(out[k], _P_F64) for k in PHOTON_FIELDS])
     lib().oracle_generate_photons(C.byref(src), n, seed, photon_id_offset, arr)
     return out
+
+
+def histogram_arrivals(results: dict, num_sensors: int, num_bins: int, t_start: float, bin_width: float,
+                       jitter_sigma: float = 0.0, seed: int = 0, photon_id_offset: int = 0) -> np.ndarray:
+    """Arrival-time histogram (num_sensors, num_bins) of oracle results."""
+    sensor = np.ascontiguousarray(results["sensor_id"], dtype=np.int32)
+    err = np.ascontiguousarray(results["error"], dtype=np.uint8)
+    tof = np.ascontiguousarray(results["time_of_flight"], dtype=np.float64)
+    counts = np.zeros(num_sensors * num_bins, dtype=np.uint32)
+    h = Histogram(None, num_sensors, num_bins, t_start, bin_width, jitter_sigma, seed, photon_id_offset)
+    lib().oracle_histogram_arrivals(_ptr(sensor, _P_I32), _ptr(err, _P_U8), _ptr(tof, _P_F64), len(sensor),
+                                    C.byref(h), counts.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return counts.reshape(num_sensors, num_bins)
+
+
+def arrival_jitter(seed: int, photon_id: int) -> float:
+    return lib().oracle_arrival_jitter(seed, photon_id)
 
 
 def log_series(x: float) -> float:

@@ -51,3 +51,61 @@ def test_gpu_source_rejects_bad_parameters(cuda):
     assert len(S.fact_bunch().generate(0, device=cuda)) == 0
     with pytest.raises(ValueError):
         S.fact_bunch().generate(10, device="cpu")
+
+
+def test_gpu_arrival_histograms_match_oracle(cuda):
+    """Arrival histograms of a propagated FACT-like bunch: the GPU counts equal
+    the oracle's binning of the same results, with and without jitter, for
+    the LDS (1440 pixels x 8 bins) and the global-atomic (x 64 bins) paths."""
+    from merlict_development_kit_amd import (ArrivalBinning, GpuScene, PropagationConfig, SceneDescription,
+                                             histogram_arrivals)
+    gs = GpuScene(SceneDescription.fact(light_guides=True))
+    n = 500_000
+    res = gs.propagate(S.fact_bunch().generate(n, seed=3, device=cuda), PropagationConfig(5, seed=4))
+    host = res.numpy()
+    for bins, jitter in ((8, 0.0), (8, 2e-9), (64, 0.0), (64, 5e-9)):
+        b = ArrivalBinning(1440, bins, 40e-9, 32e-9 / bins, jitter, seed=9, photon_id_offset=100)
+        got = histogram_arrivals(res, b).cpu().numpy().view(np.uint32)
+        want = O.histogram_arrivals(host, 1440, bins, 40e-9, 32e-9 / bins, jitter, seed=9, photon_id_offset=100)
+        assert want.sum() > 0.25 * n
+        np.testing.assert_array_equal(got, want, err_msg=f"{bins} bins, jitter {jitter}")
+    # counts accumulate
+    b = ArrivalBinning(1440, 8, 48e-9, 2e-9)
+    twice = histogram_arrivals(res, b, histogram_arrivals(res, b))
+    np.testing.assert_array_equal(twice.cpu().numpy().view(np.uint32), 2 * O.histogram_arrivals(host, 1440, 8, 48e-9, 2e-9))
+
+
+def test_gpu_bunches_equal_separate_draws_and_oracle(cuda, oracle_threads):
+    """mlgpu_propagate_bunches: the same photons as drawing each bunch on its
+    own, the same results as propagating their concatenation (and the
+    oracle's), the same histogram as binning those results."""
+    from merlict_development_kit_amd import (ArrivalBinning, Bunch, GpuScene, PhotonBatch, PropagationConfig,
+                                             SceneDescription, propagate_bunches)
+    desc = SceneDescription.fact(light_guides=True)
+    gs = GpuScene(desc)
+    bunches = [Bunch(S.fact_bunch().posed(S.rotation_xyz(0.01 * k, -0.02 * k, 0.0)), seed=40 + k, num_photons=m)
+               for k, m in enumerate((30_000, 0, 1, 77_777, 12_345))]
+    cfg = PropagationConfig(5, seed=8, photon_id_offset=1000)
+    binning = ArrivalBinning(1440, 16, 48e-9, 1e-9, jitter_sigma=1e-9, seed=2, photon_id_offset=5)
+    for schedule in ("fused", "wavefront"):
+        gs.set_tuning(schedule=schedule)
+        counts, res = propagate_bunches(gs, bunches, cfg, binning, want_results=True, device=cuda)
+        parts = [b.source.generate(b.num_photons, seed=b.seed, device=cuda) for b in bunches]
+        cat = PhotonBatch(*[torch.cat([getattr(p, f) for p in parts]) for f in
+                            ("support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z",
+                             "wavelength")])
+        again = gs.propagate(cat, cfg).numpy()
+        got = res.numpy()
+        for k in got:
+            np.testing.assert_array_equal(got[k], again[k], err_msg=f"{schedule} {k}")
+        want = O.histogram_arrivals(got, 1440, 16, 48e-9, 1e-9, 1e-9, seed=2, photon_id_offset=5)
+        np.testing.assert_array_equal(counts.cpu().numpy().view(np.uint32), want)
+    host = {k: np.concatenate([O.generate_photons(b.source.to_c(O.Source), b.num_photons, seed=b.seed)[k]
+                               for b in bunches]) for k in O.PHOTON_FIELDS}
+    ref = O.OracleScene(desc).propagate(host, seed=8, photon_id_offset=1000, threads=oracle_threads)
+    for k in ("final_frame", "sensor_id", "final_interaction", "num_interactions"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    # histogram only, no per-photon results
+    counts2, none = propagate_bunches(gs, bunches, cfg, binning, device=cuda)
+    assert none is None
+    np.testing.assert_array_equal(counts2.cpu().numpy(), counts.cpu().numpy())

@@ -353,3 +353,29 @@ def test_source_draws_are_keyed_by_photon_id():
         np.testing.assert_array_equal(whole[k][60:], tail[k])
     other = O.generate_photons(src, 100, seed=12)
     assert not np.array_equal(whole["support_x"], other["support_x"])
+
+
+# ------------------------------------------------------- arrival histograms
+def test_arrival_histogram_counts_sensor_hits_in_range():
+    res = {"sensor_id": np.array([0, 1, -1, 2, 1, 0, 5], np.int32),
+           "error": np.array([0, 0, 0, 0, 1, 0, 0], np.uint8),
+           "time_of_flight": np.array([1.5e-9, 9.9e-9, 3e-9, 0.5e-9, 2e-9, 12e-9, 1e-9])}
+    h = O.histogram_arrivals(res, num_sensors=3, num_bins=10, t_start=0.0, bin_width=1e-9)
+    want = np.zeros((3, 10), np.uint32)
+    want[0, 1] = 1   # 1.5 ns
+    want[1, 9] = 1   # 9.9 ns; the photon with an error flag is skipped
+    want[2, 0] = 1   # 0.5 ns; 12 ns is past the last bin, sensor 5 beyond num_sensors
+    np.testing.assert_array_equal(h, want)
+
+
+def test_arrival_jitter_is_standard_normal():
+    z = np.array([O.arrival_jitter(3, i) for i in range(100_000)])
+    assert sp_stats.kstest(z, "norm").pvalue > 0.01
+    assert O.arrival_jitter(3, 7) == O.arrival_jitter(3, 7) and O.arrival_jitter(3, 7) != O.arrival_jitter(4, 7)
+    res = {"sensor_id": np.zeros(20_000, np.int32), "error": np.zeros(20_000, np.uint8),
+           "time_of_flight": np.full(20_000, 50e-9)}
+    h = O.histogram_arrivals(res, 1, 100, 0.0, 1e-9, jitter_sigma=5e-9, seed=3)[0]
+    centres = np.arange(100) + 0.5
+    mean = (h * centres).sum() / h.sum()
+    sd = np.sqrt((h * (centres - mean) ** 2).sum() / h.sum())
+    assert abs(mean - 50.0) < 0.2 and abs(sd - 5.0) < 0.2
