@@ -104,6 +104,8 @@ typedef struct mlgpu_desc mlgpu_desc;
 void mlgpu_desc_destroy(mlgpu_desc* d);
 /* View of an owned description; valid until the description is destroyed. */
 int mlgpu_desc_view(const mlgpu_desc* d, mlgpu_scene_desc* out);
+/* an owned copy of a description given as arrays (validated) */
+int mlgpu_desc_create(const mlgpu_scene_desc* view, mlgpu_desc** out);
 /* Write / read the binary description file ("MLGPUSC1"). */
 int mlgpu_desc_save(const mlgpu_desc* d, const char* path);
 int mlgpu_desc_load(const char* path, mlgpu_desc** out);
@@ -190,6 +192,8 @@ typedef struct mlgpu_results {
     float* y;
     float* cx;                  /* -incident direction, object frame         */
     float* cy;
+    double* accumulated_distance; /* RayForPropagation::accumulated_distance
+                                     (RayForPropagation.cpp:91-98); may be NULL */
 } mlgpu_results;
 
 typedef struct mlgpu_config {

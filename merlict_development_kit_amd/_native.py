@@ -118,7 +118,7 @@ class Histogram(C.Structure):
 class Results(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in (
         "final_frame", "sensor_id", "final_interaction", "num_interactions", "error", "time_of_flight",
-        "x", "y", "cx", "cy")]
+        "x", "y", "cx", "cy", "accumulated_distance")]
 
 
 class Config(C.Structure):
@@ -147,6 +147,7 @@ SIGNATURES = {
     "mlgpu_abi_version": (C.c_int, []),
     "mlgpu_desc_destroy": (None, [C.c_void_p]),
     "mlgpu_desc_view": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "mlgpu_desc_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     "mlgpu_desc_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mlgpu_desc_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "mlgpu_build_lens_scene": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),

@@ -220,6 +220,7 @@ int launch_wavefront(mlgpu_scene* ms, int32_t scope, const mlgpu_photons& ph, in
     w.dy = c.take<double>(cap);
     w.dz = c.take<double>(cap);
     w.tof = c.take<double>(cap);
+    w.dist = c.take<double>(cap);
     w.root = c.take<int32_t>(cap);
     w.n_hist = c.take<uint32_t>(cap);
     w.draws = c.take<uint32_t>(cap);
@@ -288,7 +289,7 @@ int run_bunches(mlgpu_scene* ms, int32_t scope, const mlgpu_bunch* bunches, int3
                              c.take<double>(cap), c.take<double>(cap), c.take<double>(cap)};
     mlgpu_results own{c.take<int32_t>(cap), c.take<int32_t>(cap), c.take<uint8_t>(cap), c.take<uint8_t>(cap),
                       c.take<uint8_t>(cap), c.take<double>(cap), c.take<float>(cap), c.take<float>(cap),
-                      c.take<float>(cap), c.take<float>(cap)};
+                      c.take<float>(cap), c.take<float>(cap), nullptr};
     e = hipMemcpyAsync(table, ms->staging, sizeof(mlgpu::BunchDev) * static_cast<size_t>(nb), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipEventRecord(ms->staging_event, st);
     if (e == hipSuccess) e = mlgpu::launch_generate_bunches(table, nb, total, gen, st);

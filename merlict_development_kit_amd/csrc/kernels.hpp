@@ -12,7 +12,7 @@ namespace mlgpu {
 // photon (length cap), trace records by queue slot (length cap).
 struct Workspace {
     int64_t cap;
-    double *sx, *sy, *sz, *dx, *dy, *dz, *tof;
+    double *sx, *sy, *sz, *dx, *dy, *dz, *tof, *dist;
     int32_t* root;
     uint32_t* n_hist;
     uint32_t* draws;
@@ -32,8 +32,8 @@ constexpr int kNumCounts = 2 * kDeferredCounts;
 // bytes of workspace for cap photons (each array 256-byte aligned)
 inline size_t workspace_bytes(int64_t cap) {
     const size_t m = static_cast<size_t>(cap);
-    const size_t per = 7 * 8 + 4 + 4 + 4 + 4 + 8 + 4 + 4 + 4 * kMaxTreeDepth + 4 + 4 + 4;
-    return m * per + 4 * kNumCounts + 32 * 256;
+    const size_t per = 8 * 8 + 4 + 4 + 4 + 4 + 8 + 4 + 4 + 4 * kMaxTreeDepth + 4 + 4 + 4;
+    return m * per + 4 * kNumCounts + 40 * 256;
 }
 
 hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in, int64_t n,

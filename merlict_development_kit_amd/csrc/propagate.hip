@@ -113,6 +113,7 @@ __global__ __launch_bounds__(kBlock) void wf_init_kernel(const int32_t world_sco
                     ws.sx[i] = st.s.x; ws.sy[i] = st.s.y; ws.sz[i] = st.s.z;
                     ws.dx[i] = st.d.x; ws.dy[i] = st.d.y; ws.dz[i] = st.d.z;
                     ws.tof[i] = 0.0;
+                    ws.dist[i] = 0.0;
                     ws.root[i] = st.root;
                     ws.n_hist[i] = st.n_hist;
                     ws.draws[i] = 0u;
@@ -224,6 +225,7 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
                 st.s = D3{ws.sx[i], ws.sy[i], ws.sz[i]};
                 st.d = D3{ws.dx[i], ws.dy[i], ws.dz[i]};
                 st.tof = ws.tof[i];
+                st.dist = ws.dist[i];
                 st.root = ws.root[i];
                 st.n_hist = ws.n_hist[i];
                 st.err = MLGPU_PHOTON_OK;
@@ -235,6 +237,7 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
                     ws.sx[i] = st.s.x; ws.sy[i] = st.s.y; ws.sz[i] = st.s.z;
                     ws.dx[i] = st.d.x; ws.dy[i] = st.d.y; ws.dz[i] = st.d.z;
                     ws.tof[i] = st.tof;
+                    ws.dist[i] = st.dist;
                     ws.root[i] = st.root;
                     ws.n_hist[i] = st.n_hist;
                     ws.draws[i] = rng.next;

@@ -481,6 +481,7 @@ MLGPU_HD inline void first_hit(const DeviceScene& sc, const int32_t scope, const
 struct PhotonState {
     D3 s, d;                 // ray support and unit direction, world frame
     double tof;              // running Photon::time_of_flight
+    double dist;             // running accumulated distance
     int32_t root;            // scope of env.root_frame
     uint32_t n_hist;         // history length including production
     uint8_t err;
@@ -493,6 +494,7 @@ MLGPU_HD inline void photon_init(PhotonState& st, const mlgpu_photons& in, const
     st.s = D3{in.support_x[i], in.support_y[i], in.support_z[i]};
     st.d = normalized(D3{in.direction_x[i], in.direction_y[i], in.direction_z[i]});  // Ray.cpp:16-20
     st.tof = 0.0;
+    st.dist = 0.0;
     st.root = world_scope;
     st.n_hist = 1;  // production
     st.err = in.wavelength[i] > 0.0 ? MLGPU_PHOTON_OK : MLGPU_PHOTON_BAD_WAVELENGTH;  // Photon.cpp:20-28
@@ -515,6 +517,7 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
         double nv;
         if (!func_eval(sc, sc.default_refraction, lambda, nv)) { st.err = MLGPU_PHOTON_OUT_OF_TABLE; return false; }
         st.tof += nv * kVoidDistance / kC;
+        st.dist += kVoidDistance;
         st.n_hist++;
         st.last_interaction = MLGPU_ABSORPTION_IN_VOID;
         st.last_frame = -1;
@@ -623,6 +626,7 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
         return false;
     }
     st.tof += n_from * t / kC;
+    st.dist += t;
     st.n_hist++;
     st.last_interaction = interaction;
     st.last_frame = ph.frame;
@@ -658,6 +662,7 @@ MLGPU_HD inline void photon_finish(const PhotonState& st, const mlgpu_results& o
         out.y[i] = 0.0f;
         out.cx[i] = 0.0f;
         out.cy[i] = 0.0f;
+        if (out.accumulated_distance) out.accumulated_distance[i] = 0.0;
         return;
     }
     out.final_frame[i] = st.last_frame;
@@ -670,6 +675,7 @@ MLGPU_HD inline void photon_finish(const PhotonState& st, const mlgpu_results& o
     out.y[i] = static_cast<float>(st.out_y);
     out.cx[i] = static_cast<float>(st.out_cx);
     out.cy[i] = static_cast<float>(st.out_cy);
+    if (out.accumulated_distance) out.accumulated_distance[i] = st.dist;
 }
 
 // Propagator (merlict/PhotonAndFrame.cpp:6-124) for photon i of the batch,

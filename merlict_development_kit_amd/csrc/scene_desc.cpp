@@ -113,6 +113,39 @@ int mlgpu_desc_view(const mlgpu_desc* d, mlgpu_scene_desc* out) {
     return MLGPU_OK;
 }
 
+int mlgpu_desc_create(const mlgpu_scene_desc* v, mlgpu_desc** out) {
+    if (!v || !out) { mlgpu::set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    *out = nullptr;
+    try {
+        mlgpu::validate_desc(*v);
+        auto* d = new mlgpu_desc();
+        const size_t n = static_cast<size_t>(v->n_frames);
+        d->mother.assign(v->mother, v->mother + n);
+        d->pos.assign(v->pos, v->pos + 3 * n);
+        d->rot_mode.assign(v->rot_mode, v->rot_mode + n);
+        d->rot.assign(v->rot, v->rot + 4 * n);
+        d->kind.assign(v->kind, v->kind + n);
+        d->params.assign(v->params, v->params + MLGPU_NUM_PARAMS * n);
+        d->surface.assign(v->surface, v->surface + MLGPU_NUM_SURFACE_SLOTS * n);
+        if (v->allowed_frame) d->allowed_frame.assign(v->allowed_frame, v->allowed_frame + n);
+        else d->allowed_frame.assign(n, MLGPU_NO_FRAME);
+        if (v->sensor_id) d->sensor_id.assign(v->sensor_id, v->sensor_id + n);
+        else d->sensor_id.assign(n, MLGPU_NO_SENSOR);
+        const size_t nf = static_cast<size_t>(v->n_functions);
+        if (nf > 0) {
+            d->func_start.assign(v->func_start, v->func_start + nf + 1);
+            const size_t np = static_cast<size_t>(v->func_start[nf]);
+            d->func_x.assign(v->func_x, v->func_x + np);
+            d->func_y.assign(v->func_y, v->func_y + np);
+        }
+        *out = d;
+        return MLGPU_OK;
+    } catch (const std::exception& e) {
+        mlgpu::set_last_error(e.what());
+        return MLGPU_ERR_SCENE;
+    }
+}
+
 int mlgpu_desc_save(const mlgpu_desc* d, const char* path) {
     if (!d || !path) { mlgpu::set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
     std::ofstream o(path, std::ios::binary);

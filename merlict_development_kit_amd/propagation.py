@@ -73,6 +73,7 @@ class PropagationResult:
     y: torch.Tensor
     cx: torch.Tensor                 # float32, -incident direction, object frame
     cy: torch.Tensor
+    accumulated_distance: torch.Tensor  # float64, metres (RayForPropagation.cpp:91-98)
 
     @classmethod
     def empty(cls, n: int, device) -> "PropagationResult":
@@ -83,7 +84,7 @@ class PropagationResult:
                    final_interaction=torch.empty(n, **u8), num_interactions=torch.empty(n, **u8),
                    error=torch.empty(n, **u8), time_of_flight=torch.empty(n, dtype=torch.float64, device=device),
                    x=torch.empty(n, **f32), y=torch.empty(n, **f32), cx=torch.empty(n, **f32),
-                   cy=torch.empty(n, **f32))
+                   cy=torch.empty(n, **f32), accumulated_distance=torch.empty(n, dtype=torch.float64, device=device))
 
     def c_struct(self) -> N.Results:
         r = N.Results()

@@ -62,6 +62,38 @@ class SceneDescription:
         return cls(h)
 
     @classmethod
+    def from_arrays(cls, mother, pos, rot_mode, rot, kind, params, surface=None, allowed_frame=None,
+                    sensor_id=None, functions=()):
+        """A description from per-frame arrays (frame 0 the root, mothers
+        before children): pos (n, 3), rot (n, 4) as xyz angles (rot_mode 0)
+        or axis + angle (1), params (n, 6), surface (n, 6) function indices
+        (-1: merlict's defaults), and functions as (x, y) tables."""
+        n = len(mother)
+        mother = np.ascontiguousarray(mother, np.int32)
+        pos = np.ascontiguousarray(np.reshape(pos, (n, 3)), np.float64)
+        rot_mode = np.ascontiguousarray(rot_mode, np.int32)
+        rot = np.ascontiguousarray(np.reshape(rot, (n, 4)), np.float64)
+        kind = np.ascontiguousarray(kind, np.int32)
+        params = np.ascontiguousarray(np.reshape(params, (n, N.MLGPU_NUM_PARAMS)), np.float64)
+        surface = np.full((n, N.MLGPU_NUM_SURFACE_SLOTS), -1, np.int32) if surface is None else \
+            np.ascontiguousarray(np.reshape(surface, (n, N.MLGPU_NUM_SURFACE_SLOTS)), np.int32)
+        allowed = np.full(n, -1, np.int32) if allowed_frame is None else np.ascontiguousarray(allowed_frame, np.int32)
+        sensor = np.full(n, -1, np.int32) if sensor_id is None else np.ascontiguousarray(sensor_id, np.int32)
+        starts = np.cumsum([0] + [len(x) for x, _ in functions]).astype(np.int32)
+        fx = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64) for x, _ in functions])
+                                  if functions else np.zeros(1), np.float64)
+        fy = np.ascontiguousarray(np.concatenate([np.asarray(y, np.float64) for _, y in functions])
+                                  if functions else np.zeros(1), np.float64)
+        p = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
+        v = N.SceneDesc(n, p(mother, C.c_int32), p(pos, C.c_double), p(rot_mode, C.c_int32), p(rot, C.c_double),
+                        p(kind, C.c_int32), p(params, C.c_double), p(surface, C.c_int32), p(allowed, C.c_int32),
+                        p(sensor, C.c_int32), len(functions), p(starts, C.c_int32), p(fx, C.c_double),
+                        p(fy, C.c_double))
+        h = C.c_void_p()
+        N.check(N.lib().mlgpu_desc_create(C.byref(v), C.byref(h)), "mlgpu_desc_create")
+        return cls(h)
+
+    @classmethod
     def load(cls, path: str):
         h = C.c_void_p()
         N.check(N.lib().mlgpu_desc_load(str(path).encode(), C.byref(h)), "mlgpu_desc_load")

@@ -253,12 +253,16 @@ def read_photons(path) -> dict:
 
 def read_results(path) -> dict:
     with open(path, "rb") as f:
-        assert f.read(8) == b"MLGPURS1"
+        magic = f.read(8)
+        assert magic in (b"MLGPURS1", b"MLGPURS2")
         n = int(np.frombuffer(f.read(8), np.int64)[0])
         out = {}
-        for k, t in (("final_frame", np.int32), ("sensor_id", np.int32), ("final_interaction", np.uint8),
-                     ("num_interactions", np.uint8), ("error", np.uint8), ("time_of_flight", np.float64),
-                     ("x", np.float64), ("y", np.float64), ("cx", np.float64), ("cy", np.float64)):
+        keys = [("final_frame", np.int32), ("sensor_id", np.int32), ("final_interaction", np.uint8),
+                ("num_interactions", np.uint8), ("error", np.uint8), ("time_of_flight", np.float64),
+                ("x", np.float64), ("y", np.float64), ("cx", np.float64), ("cy", np.float64)]
+        if magic == b"MLGPURS2":
+            keys.append(("accumulated_distance", np.float64))
+        for k, t in keys:
             out[k] = np.frombuffer(f.read(np.dtype(t).itemsize * n), t).copy()
         return out
 

@@ -228,10 +228,10 @@ static Photons load_photons(const char* path) {
 struct Results {
     std::vector<int32_t> frame, sensor;
     std::vector<uint8_t> interaction, n_int, err;
-    std::vector<double> tof, x, y, cx, cy;
+    std::vector<double> tof, x, y, cx, cy, acc;
     void resize(size_t n) {
         frame.resize(n); sensor.resize(n); interaction.resize(n); n_int.resize(n); err.resize(n);
-        tof.resize(n); x.resize(n); y.resize(n); cx.resize(n); cy.resize(n);
+        tof.resize(n); x.resize(n); y.resize(n); cx.resize(n); cy.resize(n); acc.resize(n);
     }
 };
 
@@ -268,6 +268,7 @@ static void one_photon(const RefScene& rs, const Photons& p, int64_t i, uint64_t
         const ml::Vec3 inc = ph.final_intersection_incident_direction_wrt_frame();
         r->cx[i] = -1.0 * inc.x;
         r->cy[i] = -1.0 * inc.y;
+        r->acc[i] = ph.accumulated_distance();
     } catch (const std::invalid_argument&) {
         r->err[i] = 1;
     } catch (const std::out_of_range&) {
@@ -299,10 +300,10 @@ static int cmd_propagate(int argc, char** argv) {
     }
     for (auto& th : pool) th.join();
     std::ofstream o(argv[7], std::ios::binary);
-    o.write("MLGPURS1", 8);
+    o.write("MLGPURS2", 8);
     o.write(reinterpret_cast<const char*>(&p.n), 8);
     wr(o, r.frame); wr(o, r.sensor); wr(o, r.interaction); wr(o, r.n_int); wr(o, r.err);
-    wr(o, r.tof); wr(o, r.x); wr(o, r.y); wr(o, r.cx); wr(o, r.cy);
+    wr(o, r.tof); wr(o, r.x); wr(o, r.y); wr(o, r.cx); wr(o, r.cy); wr(o, r.acc);
     return o ? 0 : 1;
 }
 

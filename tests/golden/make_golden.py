@@ -5,7 +5,7 @@ oracle/Makefile, with the Philox stream plugged into merlict's
 random::Generator) on seeded inputs and stores inputs and outputs as small
 .npz files.  Run in the container that has the reference mounted:
 
-    make -C oracle ref && python tests/golden/make_golden.py [plenoscope]
+    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo]
 """
 from __future__ import annotations
 
@@ -26,7 +26,7 @@ from merlict_development_kit_amd import SceneDescription  # noqa: E402
 
 PHOTON_KEYS = O.PHOTON_FIELDS
 RESULT_KEYS = ("final_frame", "sensor_id", "final_interaction", "num_interactions", "error", "time_of_flight",
-               "x", "y", "cx", "cy")
+               "x", "y", "cx", "cy", "accumulated_distance")
 
 
 def save_propagation(name, desc, photons, seed, max_int=5):
@@ -67,11 +67,28 @@ def plenoscope_fixtures():
     save_propagation("plenoscope_bunch_propagation", pleno, bunch, seed=5, max_int=10)
 
 
+def zoo_fixtures():
+    """Every primitive kind (tests/scene_zoo.py): probe-ray first hits and a
+    propagation of the same rays as photons."""
+    import scene_zoo
+    zoo = scene_zoo.zoo()
+    rays = scene_zoo.probe_rays(4000, seed=1)
+    hits = O.ref_first_hit(zoo, rays)
+    np.savez_compressed(HERE / "zoo_first_hit.npz", **{f"in_{k}": rays[k] for k in PHOTON_KEYS},
+                        out_frame=hits["frame"], out_distance=hits["distance"], out_position=hits["position"])
+    print("zoo probe rays", len(rays["support_x"]), "hits", int((hits["frame"] >= 0).sum()))
+    photons = scene_zoo.probe_rays(3000, seed=2)
+    save_propagation("zoo_propagation", zoo, photons, seed=23, max_int=8)
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/merlict_golden is not built")
     if len(sys.argv) > 1 and sys.argv[1] == "plenoscope":
         plenoscope_fixtures()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "zoo":
+        zoo_fixtures()
         return
     lens = SceneDescription.lens()
     save_propagation("lens_propagation", lens, sources.lens_beam(2000, seed=1).numpy(), seed=7)
@@ -111,6 +128,7 @@ def main():
                         lightfield_pos_sum=rows[:, 3:6].sum(axis=0), lightfield_radius_sum=np.array([rows[:, -1].sum()]))
     print("light field rows", len(rows), meta)
     plenoscope_fixtures()
+    zoo_fixtures()
 
 
 if __name__ == "__main__":

@@ -14,6 +14,7 @@ import pyoracle as O
 from conftest import split_golden
 from merlict_development_kit_amd._native import MlgpuError
 import photon_sets as sources
+import scene_zoo
 from merlict_development_kit_amd import (GpuScene, PhotonBatch, PropagationConfig, SceneDescription,
                                          propagate_photons_in_frame_with_config)
 
@@ -37,6 +38,7 @@ SCENES = {
     "fact": lambda: SceneDescription.fact(light_guides=False),
     "fact_lg": lambda: SceneDescription.fact(light_guides=True),
     "plenoscope": lambda: SceneDescription.plenoscope(),
+    "zoo": lambda: scene_zoo.zoo(),
 }
 
 
@@ -53,6 +55,9 @@ def compare(gpu, ref, n, label):
     np.testing.assert_allclose(gpu["time_of_flight"][ok], ref["time_of_flight"][ok], rtol=TOF_RTOL)
     for k in ("x", "y", "cx", "cy"):
         np.testing.assert_allclose(gpu[k][ok], ref[k][ok], atol=POS_ATOL, rtol=0, err_msg=k)
+    if "accumulated_distance" in gpu and "accumulated_distance" in ref:
+        np.testing.assert_allclose(gpu["accumulated_distance"][ok], ref["accumulated_distance"][ok], rtol=TOF_RTOL,
+                                   err_msg="accumulated_distance")
     return rate
 
 
@@ -60,7 +65,8 @@ def compare(gpu, ref, n, label):
                                         ("fact_lg_propagation", "fact_lg"),
                                         ("fact_lg_max10_propagation", "fact_lg"),
                                         ("plenoscope_calibration_propagation", "plenoscope"),
-                                        ("plenoscope_bunch_propagation", "plenoscope")])
+                                        ("plenoscope_bunch_propagation", "plenoscope"),
+                                        ("zoo_propagation", "zoo")])
 def test_gpu_matches_reference_golden(cuda, golden, name, scene):
     g = golden(name)
     photons, ref = split_golden(g)
@@ -140,6 +146,21 @@ def test_gpu_first_hit_matches_reference(cuda, golden):
     np.testing.assert_allclose(h["distance"][hit], g["out_distance"][hit], rtol=1e-6)
     pos = np.stack([h["position_x"], h["position_y"], h["position_z"]], 1)
     np.testing.assert_allclose(pos[hit], g["out_position"][hit], atol=1e-6)
+
+
+@pytest.mark.parametrize("name,scene", [("fact_first_hit", "fact"), ("zoo_first_hit", "zoo")])
+def test_gpu_first_hits_match_reference_exactly(cuda, golden, name, scene):
+    """Probe-ray first hits, every primitive kind included: the frame, the
+    distance and the world position equal the reference's."""
+    g = golden(name)
+    rays = {k[3:]: g[k] for k in g if k.startswith("in_")}
+    gs = GpuScene(SCENES[scene]())
+    h = {k: v.cpu().numpy() for k, v in gs.first_hit(PhotonBatch.from_numpy(rays, cuda)).items()}
+    np.testing.assert_array_equal(h["frame"], g["out_frame"])
+    hit = g["out_frame"] >= 0
+    np.testing.assert_array_equal(h["distance"][hit], g["out_distance"][hit])
+    pos = np.stack([h["position_x"], h["position_y"], h["position_z"]], 1)
+    np.testing.assert_array_equal(pos[hit], g["out_position"][hit])
 
 
 def test_gpu_is_deterministic(cuda):

@@ -16,6 +16,7 @@ import pytest
 import pyoracle as O
 from conftest import split_golden
 import photon_sets as sources
+import scene_zoo
 from merlict_development_kit_amd import SceneDescription
 
 RESULT_INT = ("final_frame", "sensor_id", "final_interaction", "num_interactions", "error")
@@ -30,12 +31,13 @@ def scene_for(name):
         "fact_lg_max10_propagation": lambda: SceneDescription.fact(light_guides=True),
         "plenoscope_calibration_propagation": lambda: SceneDescription.plenoscope(),
         "plenoscope_bunch_propagation": lambda: SceneDescription.plenoscope(),
+        "zoo_propagation": lambda: scene_zoo.zoo(),
     }[name]()
 
 
 @pytest.mark.parametrize("name", ["lens_propagation", "fact_propagation", "fact_lg_propagation",
                                   "fact_lg_max10_propagation", "plenoscope_calibration_propagation",
-                                  "plenoscope_bunch_propagation"])
+                                  "plenoscope_bunch_propagation", "zoo_propagation"])
 def test_oracle_matches_reference_golden(golden, name, oracle_threads):
     g = golden(name)
     photons, ref = split_golden(g)
@@ -49,6 +51,23 @@ def test_oracle_matches_reference_golden(golden, name, oracle_threads):
         np.testing.assert_array_equal(res[k][hit], ref[k][hit], err_msg=k)
     # void photons: n = 1 over the 1e4 m void leg (RayAndFrame.cpp:161-167)
     assert np.all(res["time_of_flight"][~hit] >= 1e4 / 299792458.0)
+    # RayForPropagation::accumulated_distance, void legs included
+    ok = ref["error"] == 0
+    np.testing.assert_array_equal(res["accumulated_distance"][ok], ref["accumulated_distance"][ok])
+
+
+def test_oracle_zoo_first_hits_match_reference(golden):
+    """Every primitive kind, both rotation modes, octant clustering and
+    coplanar ties: the oracle's first hits are the reference's."""
+    g = golden("zoo_first_hit")
+    rays = {k[3:]: g[k] for k in g if k.startswith("in_")}
+    h = O.OracleScene(scene_zoo.zoo()).first_hit(rays)
+    np.testing.assert_array_equal(h["frame"], g["out_frame"])
+    hit = g["out_frame"] >= 0
+    np.testing.assert_array_equal(h["distance"][hit], g["out_distance"][hit])
+    np.testing.assert_array_equal(h["position"][hit], g["out_position"][hit])
+    kinds = scene_zoo.zoo().arrays()["kind"][g["out_frame"][hit]]
+    assert set(np.unique(kinds)) == set(range(1, 12))  # every surface kind is hit
 
 
 def test_oracle_tree_matches_reference(golden):
