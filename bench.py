@@ -27,8 +27,12 @@ from merlict_development_kit_amd import GpuScene, PropagationConfig, Propagation
 
 METRIC = "photons propagated/sec on 1 MI355X, plain vs surrogate hybrid at ≤1% histogram L1"
 HBM_PEAK = 8.0e12
+# HBM bytes per photon of the default workload (FACT-like with light guides, 1e7
+# photons, default tuning), rocprofv3 FETCH_SIZE + WRITE_SIZE of one propagation
+# pass scaled by a 1 GiB copy: profiles/propagation_pass_fact_lg_1e7_counters.md
+TRAFFIC_BYTES_PER_PHOTON = 1494.0
 IN_BYTES = 7 * 8            # support xyz, direction xyz, wavelength (float64)
-OUT_BYTES = 4 + 4 + 1 + 1 + 1 + 8 + 4 * 4  # frame, sensor, interaction, n, error, tof, x y cx cy
+OUT_BYTES = 4 + 4 + 1 + 1 + 1 + 8 + 4 * 4 + 8  # frame, sensor, interaction, n, error, tof, x y cx cy, distance
 
 
 def cpu_baseline(desc: SceneDescription, photons, threads: int, target_seconds: float = 12.0) -> dict:
@@ -117,6 +121,8 @@ def main() -> int:
     errors = scene.count_errors(out)
     value = n * args.steps / wall
     achieved = n * (IN_BYTES + OUT_BYTES) / (ms_kernel * 1e-3)
+    default_workload = (n == 10_000_000 and not args.no_light_guides and args.trace_budget is None
+                        and args.blocks_per_cu is None and args.schedule == "auto")
     line = {
         "metric": METRIC,
         "value": value,
@@ -146,8 +152,9 @@ def main() -> int:
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
-            "kernel": "propagate_kernel",
+            "traffic": (TRAFFIC_BYTES_PER_PHOTON * n if default_workload else None),
+            "kernel": "propagation pass: the kernels of one mlgpu_propagate call on one stream "
+                      "(wavefront: wf_init, then per level wf_trace, wf_defer, wf_resume, wf_shade)",
             "kernel_ms": ms_kernel,
             "algorithmic_bytes_per_photon": IN_BYTES + OUT_BYTES,
         },
