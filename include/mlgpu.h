@@ -338,6 +338,39 @@ int mlgpu_propagate_bunches(const mlgpu_scene* scene, int32_t world_frame,
                             const mlgpu_histogram* hist, mlgpu_results* results,
                             void* stream);
 
+/* ------------------------------------------------------------- surrogate
+ * A learned surrogate of the propagator: an MLP 5 -> 256 -> 256 -> 256 ->
+ * 256 -> 57 from a photon's (support x, y, direction cx, cy, wavelength),
+ * normalised, to the survival logit and an 8-component Gaussian mixture
+ * (weight logit, mean x y t, log sigma x y t) over the sensor-plane
+ * position and arrival time of a photon that reaches a sensor.  bf16
+ * weights and activations on the matrix cores with fp32 accumulation,
+ * fp32 master parameters and Adam.  The flat fp32 parameter vector is
+ * W0 b0 W1 b1 ... W4 b4, each W row-major (out x in).  Features are n x 5
+ * fp32, targets n x 4 fp32 (detected 0/1, x, y, t), device pointers.  No
+ * counterpart in the reference: merlict only has the plain propagator,
+ * which trains and judges this model. */
+typedef struct mlgpu_surrogate mlgpu_surrogate;
+int64_t mlgpu_surrogate_num_params(void);
+int mlgpu_surrogate_create(uint64_t seed, mlgpu_surrogate** out);
+void mlgpu_surrogate_destroy(mlgpu_surrogate* s);
+int mlgpu_surrogate_get_params(const mlgpu_surrogate* s, float* dst, void* stream);
+/* also resets the optimiser state */
+int mlgpu_surrogate_set_params(mlgpu_surrogate* s, const float* src, void* stream);
+/* out: n x 57 fp32 */
+int mlgpu_surrogate_forward(mlgpu_surrogate* s, const float* features, int64_t n,
+                            float* out, void* stream);
+/* mean loss over the batch (*loss, device) and its gradient (grads,
+ * device, num_params; either may be NULL) */
+int mlgpu_surrogate_gradients(mlgpu_surrogate* s, const float* features,
+                              const float* targets, int64_t n, float* grads,
+                              float* loss, void* stream);
+/* one Adam step (beta 0.9 / 0.999) on the batch; *loss (device, may be
+ * NULL) receives the summed loss before the step */
+int mlgpu_surrogate_train_step(mlgpu_surrogate* s, const float* features,
+                               const float* targets, int64_t n,
+                               float learning_rate, float* loss, void* stream);
+
 /* Number of photons with a non-zero error flag in the last n results
  * (synchronises `stream`). */
 int mlgpu_count_errors(const mlgpu_results* results, int64_t n, void* stream,

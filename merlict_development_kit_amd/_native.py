@@ -165,6 +165,16 @@ SIGNATURES = {
                                          C.POINTER(PhotonBuffers), C.c_void_p]),
     "mlgpu_propagate_bunches": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Bunch), C.c_int32, C.POINTER(Config),
                                           C.POINTER(Histogram), C.POINTER(Results), C.c_void_p]),
+    "mlgpu_surrogate_num_params": (C.c_int64, []),
+    "mlgpu_surrogate_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "mlgpu_surrogate_destroy": (None, [C.c_void_p]),
+    "mlgpu_surrogate_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlgpu_surrogate_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlgpu_surrogate_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "mlgpu_surrogate_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]),
+    "mlgpu_surrogate_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p,
+                                             C.c_void_p]),
     "mlgpu_histogram_arrivals": (C.c_int, [C.POINTER(Results), C.c_int64, C.POINTER(Histogram), C.c_void_p]),
     "mlgpu_count_errors": (C.c_int, [C.POINTER(Results), C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
     "mlgpu_first_hit": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Hits),

@@ -36,7 +36,8 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["scene_desc.cpp", "builders.cpp", "scene_build.cpp", "api.cpp", "propagate.hip", "sources.hip", "arrivals.hip"]
+SOURCES = ["scene_desc.cpp", "builders.cpp", "scene_build.cpp", "api.cpp", "propagate.hip", "sources.hip", "arrivals.hip", "surrogate.hip",
+           "surrogate_api.cpp"]
 
 
 def _hipcc() -> str:

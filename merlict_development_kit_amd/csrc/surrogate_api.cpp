@@ -1,0 +1,303 @@
+// C ABI of the learned surrogate (include/mlgpu.h): parameters, forward,
+// gradients and Adam training steps on the device.  The network's matrix
+// products run in surrogate.hip on the matrix cores.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <mutex>
+#include <string>
+
+#include "scene.hpp"
+#include "surrogate.hpp"
+
+using namespace mlgpu::sur;
+
+struct mlgpu_surrogate {
+    int device = 0;
+    float* params = nullptr;  // fp32 master parameters, flat
+    float* grads = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    PaddedWeights pw{};
+    void* weights_mem = nullptr;
+    int64_t step = 0;
+    std::mutex mutex;  // guards the batch workspace and the step counter
+    // batch workspace, rows = padded batch
+    void* ws = nullptr;
+    int64_t ws_rows = 0;
+    uint16_t* x0 = nullptr;            // rows x kInPad
+    uint16_t* h[kLayers - 1] = {};     // rows x kHidden, post-ReLU activations
+    float* out = nullptr;              // rows x kOutPad
+    uint16_t* gout = nullptr;          // rows x kOutPad
+    uint16_t* gh[2] = {};              // rows x kHidden, gradients wrt pre-activations
+    float* partial = nullptr;          // split-K slices of a weight gradient
+    float* loss = nullptr;             // scalar
+};
+
+namespace {
+
+using mlgpu::set_last_error;
+
+constexpr int kKChunk = 512;  // rows of the batch per split of a weight-gradient product
+
+int hip_fail(hipError_t e, const char* what) {
+    set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+    return MLGPU_ERR_HIP;
+}
+
+int64_t padded_rows(int64_t n) { return (n + kBatchAlign - 1) / kBatchAlign * kBatchAlign; }
+
+size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+int ensure_workspace(mlgpu_surrogate* s, int64_t rows) {
+    if (s->ws_rows >= rows) return MLGPU_OK;
+    if (s->ws) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(s->ws);
+        s->ws = nullptr;
+        s->ws_rows = 0;
+    }
+    const size_t r = static_cast<size_t>(rows);
+    const int splits = static_cast<int>((rows + kKChunk - 1) / kKChunk);
+    size_t bytes = align256(r * kInPad * 2) + (kLayers - 1) * align256(r * kHidden * 2) +
+                   align256(r * kOutPad * 4) + align256(r * kOutPad * 2) + 2 * align256(r * kHidden * 2) +
+                   align256(static_cast<size_t>(splits) * kHidden * kHidden * 4) + 256;
+    hipError_t e = hipMalloc(&s->ws, bytes);
+    if (e != hipSuccess) { s->ws = nullptr; return hip_fail(e, "hipMalloc surrogate workspace"); }
+    char* p = static_cast<char*>(s->ws);
+    auto take = [&](size_t b) { char* q = p; p += align256(b); return q; };
+    s->x0 = reinterpret_cast<uint16_t*>(take(r * kInPad * 2));
+    for (int l = 0; l < kLayers - 1; l++) s->h[l] = reinterpret_cast<uint16_t*>(take(r * kHidden * 2));
+    s->out = reinterpret_cast<float*>(take(r * kOutPad * 4));
+    s->gout = reinterpret_cast<uint16_t*>(take(r * kOutPad * 2));
+    for (int k = 0; k < 2; k++) s->gh[k] = reinterpret_cast<uint16_t*>(take(r * kHidden * 2));
+    s->partial = reinterpret_cast<float*>(take(static_cast<size_t>(splits) * kHidden * kHidden * 4));
+    s->loss = reinterpret_cast<float*>(take(256));
+    s->ws_rows = rows;
+    return MLGPU_OK;
+}
+
+// activations h[0..3] and outputs `out` of the padded batch in x0
+hipError_t forward(mlgpu_surrogate* s, int64_t rows, hipStream_t st) {
+    const int M = static_cast<int>(rows);
+    hipError_t e = hipSuccess;
+    const uint16_t* in = s->x0;
+    for (int l = 0; l < kLayers; l++) {
+        GemmOut o{};
+        o.bias = s->pw.b[l];
+        if (l < kLayers - 1) {
+            o.c16 = s->h[l];
+            o.ldc = kHidden;
+            e = launch_gemm(true, true, EPI_BF16_RELU, in, kPadIn[l], s->pw.w[l], kPadIn[l], M, kPadOut[l], kPadIn[l],
+                            1, o, st);
+            in = s->h[l];
+        } else {
+            o.c32 = s->out;
+            o.ldc = kOutPad;
+            e = launch_gemm(true, true, EPI_F32, in, kPadIn[l], s->pw.w[l], kPadIn[l], M, kPadOut[l], kPadIn[l], 1, o,
+                            st);
+        }
+        if (e != hipSuccess) return e;
+    }
+    return e;
+}
+
+// gradient of layer l's weights: dW_l = G^T X over the batch, G (rows x
+// kPadOut[l], ld ldg) the gradient wrt the layer's outputs, X its input
+hipError_t weight_gradient(mlgpu_surrogate* s, int l, const uint16_t* g, int ldg, const uint16_t* x, int rows,
+                           hipStream_t st) {
+    const int splits = rows / kKChunk > 0 && rows % kKChunk == 0 ? rows / kKChunk : 1;
+    GemmOut o{};
+    o.c32 = s->partial;
+    o.ldc = kPadIn[l];
+    // C[kPadOut x kPadIn] = sum_b G(b, n) X(b, k): A = G^T (n contiguous), B = X (k contiguous per row b)
+    hipError_t e = launch_gemm(false, false, EPI_F32, g, ldg, x, kPadIn[l], kPadOut[l], kPadIn[l], rows, splits, o, st);
+    if (e != hipSuccess) return e;
+    // fold the splits into slice 0, then copy the logical block
+    e = launch_sum_slices(s->partial, static_cast<int64_t>(kPadOut[l]) * kPadIn[l], splits, s->partial, st);
+    if (e != hipSuccess) return e;
+    // logical rows / columns into the flat gradient vector
+    const size_t row_bytes = static_cast<size_t>(kLayerIn[l]) * 4;
+    e = hipMemcpy2DAsync(s->grads + weight_offset(l), row_bytes, s->partial, static_cast<size_t>(kPadIn[l]) * 4,
+                         row_bytes, static_cast<size_t>(kLayerOut[l]), hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+    return launch_column_sums(g, rows, kLayerOut[l], ldg, s->grads + bias_offset(l), st);
+}
+
+// loss and gradients of the batch (features already packed in x0)
+hipError_t backward(mlgpu_surrogate* s, const float* targets, int64_t n, int64_t rows, hipStream_t st) {
+    const int M = static_cast<int>(rows);
+    hipError_t e = hipMemsetAsync(s->loss, 0, sizeof(float), st);
+    if (e == hipSuccess) e = forward(s, rows, st);
+    if (e == hipSuccess) e = launch_mixture_loss(s->out, targets, n, rows, s->gout, s->loss, st);
+    if (e != hipSuccess) return e;
+    const uint16_t* g = s->gout;
+    int ldg = kOutPad;
+    for (int l = kLayers - 1; l >= 0; l--) {
+        const uint16_t* x = l > 0 ? s->h[l - 1] : s->x0;
+        e = weight_gradient(s, l, g, ldg, x, M, st);
+        if (e != hipSuccess) return e;
+        if (l == 0) break;
+        // gradient wrt layer l's input, through the ReLU of layer l - 1:
+        // C[rows x kHidden] = sum_n G(b, n) W_l(n, k), W_l row-major kPadOut x kPadIn
+        uint16_t* gh = s->gh[l & 1];
+        GemmOut o{};
+        o.c16 = gh;
+        o.mask = s->h[l - 1];
+        o.ldc = kHidden;
+        e = launch_gemm(true, false, EPI_BF16_MASK, g, ldg, s->pw.w[l], kPadIn[l], M, kPadIn[l], kPadOut[l], 1, o, st);
+        if (e != hipSuccess) return e;
+        g = gh;
+        ldg = kHidden;
+    }
+    return hipSuccess;
+}
+
+int check_batch(const mlgpu_surrogate* s, const float* features, int64_t n) {
+    if (!s || !features || n < 1) { set_last_error("null argument or empty batch"); return MLGPU_ERR_ARGUMENT; }
+    if (n > (int64_t{1} << 24)) { set_last_error("at most 2^24 photons per surrogate batch"); return MLGPU_ERR_ARGUMENT; }
+    return MLGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t mlgpu_surrogate_num_params(void) { return kNumParams; }
+
+int mlgpu_surrogate_create(uint64_t seed, mlgpu_surrogate** out) {
+    if (!out) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    *out = nullptr;
+    auto* s = new mlgpu_surrogate();
+    (void)hipGetDevice(&s->device);
+    const size_t pbytes = static_cast<size_t>(kNumParams) * 4;
+    hipError_t e = hipMalloc(&s->params, 4 * align256(pbytes));
+    if (e != hipSuccess) { delete s; return hip_fail(e, "hipMalloc surrogate parameters"); }
+    char* p = reinterpret_cast<char*>(s->params);
+    s->grads = reinterpret_cast<float*>(p + align256(pbytes));
+    s->adam_m = reinterpret_cast<float*>(p + 2 * align256(pbytes));
+    s->adam_v = reinterpret_cast<float*>(p + 3 * align256(pbytes));
+    size_t wbytes = 0;
+    for (int l = 0; l < kLayers; l++)
+        wbytes += align256(static_cast<size_t>(kPadOut[l]) * kPadIn[l] * 2) + align256(static_cast<size_t>(kPadOut[l]) * 4);
+    e = hipMalloc(&s->weights_mem, wbytes);
+    if (e != hipSuccess) { (void)hipFree(s->params); delete s; return hip_fail(e, "hipMalloc surrogate weights"); }
+    char* q = static_cast<char*>(s->weights_mem);
+    for (int l = 0; l < kLayers; l++) {
+        s->pw.w[l] = reinterpret_cast<uint16_t*>(q);
+        q += align256(static_cast<size_t>(kPadOut[l]) * kPadIn[l] * 2);
+        s->pw.b[l] = reinterpret_cast<float*>(q);
+        q += align256(static_cast<size_t>(kPadOut[l]) * 4);
+    }
+    e = launch_init_params(s->params, seed, nullptr);
+    if (e == hipSuccess) e = hipMemsetAsync(s->adam_m, 0, pbytes, nullptr);
+    if (e == hipSuccess) e = hipMemsetAsync(s->adam_v, 0, pbytes, nullptr);
+    if (e == hipSuccess) e = launch_pad_weights(s->params, s->pw, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) { mlgpu_surrogate_destroy(s); return hip_fail(e, "surrogate init"); }
+    *out = s;
+    return MLGPU_OK;
+}
+
+void mlgpu_surrogate_destroy(mlgpu_surrogate* s) {
+    if (!s) return;
+    (void)hipDeviceSynchronize();
+    if (s->ws) (void)hipFree(s->ws);
+    if (s->weights_mem) (void)hipFree(s->weights_mem);
+    if (s->params) (void)hipFree(s->params);
+    delete s;
+}
+
+int mlgpu_surrogate_get_params(const mlgpu_surrogate* s, float* dst, void* stream) {
+    if (!s || !dst) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    const hipError_t e = hipMemcpyAsync(dst, s->params, static_cast<size_t>(kNumParams) * 4, hipMemcpyDefault,
+                                        static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "copy parameters");
+    return MLGPU_OK;
+}
+
+int mlgpu_surrogate_set_params(mlgpu_surrogate* s, const float* src, void* stream) {
+    if (!s || !src) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(s->mutex);
+    const size_t pbytes = static_cast<size_t>(kNumParams) * 4;
+    hipError_t e = hipMemcpyAsync(s->params, src, pbytes, hipMemcpyDefault, st);
+    if (e == hipSuccess) e = hipMemsetAsync(s->adam_m, 0, pbytes, st);
+    if (e == hipSuccess) e = hipMemsetAsync(s->adam_v, 0, pbytes, st);
+    if (e == hipSuccess) e = launch_pad_weights(s->params, s->pw, st);
+    if (e != hipSuccess) return hip_fail(e, "set parameters");
+    s->step = 0;
+    return MLGPU_OK;
+}
+
+int mlgpu_surrogate_forward(mlgpu_surrogate* s, const float* features, int64_t n, float* out, void* stream) {
+    int rc = check_batch(s, features, n);
+    if (rc) return rc;
+    if (!out) { set_last_error("null output"); return MLGPU_ERR_ARGUMENT; }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(s->mutex);
+    const int64_t rows = padded_rows(n);
+    if ((rc = ensure_workspace(s, rows))) return rc;
+    hipError_t e = launch_pack_features(features, n, rows, s->x0, st);
+    if (e == hipSuccess) e = forward(s, rows, st);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(out, kOut * 4, s->out, kOutPad * 4, kOut * 4, static_cast<size_t>(n),
+                             hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "surrogate forward");
+    return MLGPU_OK;
+}
+
+int mlgpu_surrogate_gradients(mlgpu_surrogate* s, const float* features, const float* targets, int64_t n,
+                              float* grads, float* loss, void* stream) {
+    int rc = check_batch(s, features, n);
+    if (rc) return rc;
+    if (!targets) { set_last_error("null targets"); return MLGPU_ERR_ARGUMENT; }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(s->mutex);
+    const int64_t rows = padded_rows(n);
+    if ((rc = ensure_workspace(s, rows))) return rc;
+    hipError_t e = launch_pack_features(features, n, rows, s->x0, st);
+    if (e == hipSuccess) e = backward(s, targets, n, rows, st);
+    if (e == hipSuccess && grads)
+        e = hipMemcpyAsync(grads, s->grads, static_cast<size_t>(kNumParams) * 4, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, s->loss, 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "surrogate gradients");
+    return MLGPU_OK;
+}
+
+int mlgpu_surrogate_train_step(mlgpu_surrogate* s, const float* features, const float* targets, int64_t n,
+                               float learning_rate, float* loss, void* stream) {
+    int rc = check_batch(s, features, n);
+    if (rc) return rc;
+    if (!targets || !(learning_rate > 0.0f) || !std::isfinite(learning_rate)) {
+        set_last_error("null targets or bad learning rate");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(s->mutex);
+    const int64_t rows = padded_rows(n);
+    if ((rc = ensure_workspace(s, rows))) return rc;
+    hipError_t e = launch_pack_features(features, n, rows, s->x0, st);
+    if (e == hipSuccess) e = backward(s, targets, n, rows, st);
+    if (e == hipSuccess) {
+        s->step++;
+        AdamArgs a{};
+        a.params = s->params;
+        a.grads = s->grads;
+        a.m = s->adam_m;
+        a.v = s->adam_v;
+        a.lr = learning_rate;
+        a.beta1 = 0.9f;
+        a.beta2 = 0.999f;
+        a.eps = 1e-8f;
+        a.c1 = static_cast<float>(1.0 / (1.0 - std::pow(0.9, static_cast<double>(s->step))));
+        a.c2 = static_cast<float>(1.0 / (1.0 - std::pow(0.999, static_cast<double>(s->step))));
+        e = launch_adam(a, st);
+    }
+    if (e == hipSuccess) e = launch_pad_weights(s->params, s->pw, st);
+    if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, s->loss, 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "surrogate train step");
+    return MLGPU_OK;
+}
+
+}  // extern "C"
