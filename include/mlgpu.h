@@ -106,6 +106,10 @@ void mlgpu_desc_destroy(mlgpu_desc* d);
 int mlgpu_desc_view(const mlgpu_desc* d, mlgpu_scene_desc* out);
 /* an owned copy of a description given as arrays (validated) */
 int mlgpu_desc_create(const mlgpu_scene_desc* view, mlgpu_desc** out);
+/* Frame::frame2world() of every frame after merlict's tree initialisation
+ * (merlict/Frame.cpp:26-31,125-147): out (host) holds 12 doubles per frame,
+ * the row-major 3 x 4 transform */
+int mlgpu_desc_frame_poses(const mlgpu_scene_desc* desc, double* out);
 /* Write / read the binary description file ("MLGPUSC1"). */
 int mlgpu_desc_save(const mlgpu_desc* d, const char* path);
 int mlgpu_desc_load(const char* path, mlgpu_desc** out);
@@ -370,6 +374,65 @@ int mlgpu_surrogate_gradients(mlgpu_surrogate* s, const float* features,
 int mlgpu_surrogate_train_step(mlgpu_surrogate* s, const float* features,
                                const float* targets, int64_t n,
                                float learning_rate, float* loss, void* stream);
+
+/* The surrogate's domain and units.  Features of a photon: x / r, y / r of
+ * its support (r = support_radius), its direction's x and y over
+ * sin(max_angle), and 2 (1/lambda - 1/lambda_max) / (1/lambda_min -
+ * 1/lambda_max) - 1.  In the domain: support within r of the axis,
+ * direction within max_angle of -z, lambda in [lambda_min, lambda_max].
+ * Targets: sensor-plane position (the final intersection in the world, x
+ * and y over sensor_radius) and (time_of_flight - time_offset) / time_scale. */
+typedef struct mlgpu_surrogate_domain {
+    double support_radius;
+    double max_angle;
+    double lambda_min;
+    double lambda_max;
+    double sensor_radius;
+    double time_offset;
+    double time_scale;
+} mlgpu_surrogate_domain;
+
+/* Features (n x 5) of photons and, when results is not NULL, targets
+ * (n x 4: detected, x, y, t) of their propagation; frame_poses (device) is
+ * mlgpu_desc_frame_poses of the scene.  All device pointers. */
+int mlgpu_surrogate_prepare(const mlgpu_surrogate_domain* domain,
+                            const mlgpu_photons* photons,
+                            const mlgpu_results* results,
+                            const double* frame_poses, int64_t n,
+                            float* features, float* targets, void* stream);
+
+/* Sensor cells on a hexagonal lattice in the sensor plane: the cell whose
+ * centre origin + i a1 + j a2 is nearest to a point, cell id
+ * cell[(i + radius) * (2 radius + 1) + (j + radius)] (device; -1: none). */
+typedef struct mlgpu_hex_cells {
+    double a1[2];
+    double a2[2];
+    double origin[2];
+    int32_t radius;
+    const int32_t* cell;
+} mlgpu_hex_cells;
+
+/* The surrogate + fallback hybrid: every photon in the domain whose
+ * mixture entropy bound (entropy of the weights plus their mean component
+ * entropy, nats, normalised units) is at most max_entropy is drawn from the
+ * surrogate (survival, component, position, time from Philox stream 3 of
+ * (seed, config photon_id_offset + i)) and binned into hist at the cell
+ * its position falls in; every other photon is propagated by the plain
+ * propagator and binned from its result.  *num_fallback (host, may be
+ * NULL) receives the number of propagated photons.  Synchronises the
+ * stream once per chunk of 2^22 photons. */
+typedef struct mlgpu_hybrid_config {
+    double max_entropy;
+    uint64_t seed;
+} mlgpu_hybrid_config;
+int mlgpu_propagate_hybrid(const mlgpu_scene* scene, mlgpu_surrogate* surrogate,
+                           const mlgpu_surrogate_domain* domain,
+                           const mlgpu_hex_cells* cells, int32_t world_frame,
+                           const mlgpu_photons* photons, int64_t n,
+                           const mlgpu_config* config,
+                           const mlgpu_hybrid_config* hybrid,
+                           const mlgpu_histogram* hist, int64_t* num_fallback,
+                           void* stream);
 
 /* Number of photons with a non-zero error flag in the last n results
  * (synchronises `stream`). */

@@ -102,6 +102,25 @@ class Bunch(C.Structure):
     ]
 
 
+class SurrogateDomain(C.Structure):
+    _fields_ = [(name, C.c_double) for name in (
+        "support_radius", "max_angle", "lambda_min", "lambda_max", "sensor_radius", "time_offset", "time_scale")]
+
+
+class HexCells(C.Structure):
+    _fields_ = [
+        ("a1", C.c_double * 2),
+        ("a2", C.c_double * 2),
+        ("origin", C.c_double * 2),
+        ("radius", C.c_int32),
+        ("cell", C.c_void_p),
+    ]
+
+
+class HybridConfig(C.Structure):
+    _fields_ = [("max_entropy", C.c_double), ("seed", C.c_uint64)]
+
+
 class Histogram(C.Structure):
     _fields_ = [
         ("counts", C.c_void_p),
@@ -148,6 +167,7 @@ SIGNATURES = {
     "mlgpu_desc_destroy": (None, [C.c_void_p]),
     "mlgpu_desc_view": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
     "mlgpu_desc_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    "mlgpu_desc_frame_poses": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_double)]),
     "mlgpu_desc_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mlgpu_desc_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "mlgpu_build_lens_scene": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
@@ -175,6 +195,12 @@ SIGNATURES = {
                                             C.c_void_p]),
     "mlgpu_surrogate_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p,
                                              C.c_void_p]),
+    "mlgpu_surrogate_prepare": (C.c_int, [C.POINTER(SurrogateDomain), C.POINTER(Photons), C.POINTER(Results),
+                                          C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlgpu_propagate_hybrid": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SurrogateDomain), C.POINTER(HexCells),
+                                         C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
+                                         C.POINTER(HybridConfig), C.POINTER(Histogram), C.POINTER(C.c_int64),
+                                         C.c_void_p]),
     "mlgpu_histogram_arrivals": (C.c_int, [C.POINTER(Results), C.c_int64, C.POINTER(Histogram), C.c_void_p]),
     "mlgpu_count_errors": (C.c_int, [C.POINTER(Results), C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
     "mlgpu_first_hit": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Hits),

@@ -392,6 +392,17 @@ int mlgpu_scene_get_tuning(const mlgpu_scene* s, mlgpu_tuning* t) {
     return MLGPU_OK;
 }
 
+int mlgpu_desc_frame_poses(const mlgpu_scene_desc* d, double* out) {
+    if (!d || !out) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    try {
+        mlgpu::frame_poses(*d, out);
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return MLGPU_ERR_SCENE;
+    }
+    return MLGPU_OK;
+}
+
 int mlgpu_scene_stats(const mlgpu_scene* s, int64_t* num_nodes, int64_t* num_leaves, int64_t* device_bytes) {
     if (!s) { set_last_error("null scene"); return MLGPU_ERR_ARGUMENT; }
     if (num_nodes) *num_nodes = s->num_nodes;

@@ -798,4 +798,19 @@ void build_host_scene(const mlgpu_scene_desc& d, HostScene* hs) {
     b.run();
 }
 
+void frame_poses(const mlgpu_scene_desc& d, double* out) {
+    validate_desc(d);
+    std::vector<double> radius(d.n_frames);
+    for (int32_t i = 0; i < d.n_frames; i++) {
+        double prm[8];
+        radius[i] = leaf_params(d.kind[i], d.params + MLGPU_NUM_PARAMS * i, prm);
+    }
+    const ReferencePoses rp(d, radius);
+    for (int32_t i = 0; i < d.n_frames; i++) {
+        const M34& m = rp.frame2world(i);
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 4; c++) out[12 * i + 4 * r + c] = m.T[r][c];
+    }
+}
+
 }  // namespace mlgpu

@@ -31,4 +31,8 @@ struct HostScene {
 // Flatten a description (throws std::invalid_argument on bad input).
 void build_host_scene(const mlgpu_scene_desc& d, HostScene* hs);
 
+// frame2world of every frame (12 doubles each, row-major 3 x 4), as merlict
+// computes them after clustering (throws std::invalid_argument on bad input).
+void frame_poses(const mlgpu_scene_desc& d, double* out);
+
 }  // namespace mlgpu

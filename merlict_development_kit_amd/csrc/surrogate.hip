@@ -6,6 +6,7 @@
 
 #include <cmath>
 
+#include "arrivals_core.hpp"
 #include "sources_core.hpp"
 #include "surrogate.hpp"
 
@@ -207,16 +208,27 @@ __global__ void adam_kernel(const AdamArgs a) {
     a.params[i] -= a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
 }
 
-// padded element e of all layers: layer by layer, weights then biases
+// padded element e of all layers: layer by layer, weights then biases.
+// The fragment copy holds, for output tile ot (32 rows), K step ks (16
+// columns) and lane l, the 8 weights W[32 ot + l % 32][16 ks + 8 (l / 32) + j]:
+// element ((ot * ksteps + ks) * 64 + l) * 8 + j; layer 0 uses K = 16.
+__device__ __forceinline__ int fragment_ksteps(int l) { return l == 0 ? 1 : kPadIn[l] / 16; }
+
 __global__ void pad_weights_kernel(const float* __restrict__ params, const PaddedWeights pw) {
     int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     for (int l = 0; l < kLayers; l++) {
         const int64_t nw = static_cast<int64_t>(kPadOut[l]) * kPadIn[l];
         if (e < nw) {
             const int r = static_cast<int>(e / kPadIn[l]), c = static_cast<int>(e % kPadIn[l]);
-            pw.w[l][e] = (r < kLayerOut[l] && c < kLayerIn[l])
-                             ? f2bf(params[weight_offset(l) + static_cast<int64_t>(r) * kLayerIn[l] + c])
-                             : static_cast<uint16_t>(0);
+            const uint16_t v = (r < kLayerOut[l] && c < kLayerIn[l])
+                                   ? f2bf(params[weight_offset(l) + static_cast<int64_t>(r) * kLayerIn[l] + c])
+                                   : static_cast<uint16_t>(0);
+            pw.w[l][e] = v;
+            const int ks_n = fragment_ksteps(l);
+            if (c < 16 * ks_n) {
+                const int ot = r / 32, ks = c / 16, lane = (r % 32) + 32 * ((c % 16) / 8), j = c % 8;
+                pw.wf[l][((static_cast<int64_t>(ot) * ks_n + ks) * 64 + lane) * 8 + j] = v;
+            }
             return;
         }
         e -= nw;
@@ -247,6 +259,302 @@ __global__ void init_params_kernel(float* __restrict__ params, const uint64_t se
     sincos_turn(u53(w2, w3), sn, cs);
     const double std = sqrt(2.0 / kLayerIn[l]) * (l == kLayers - 1 ? 0.1 : 1.0);
     params[i] = static_cast<float>(std * r * cs);
+}
+
+// the five normalised features of photon i (mlgpu_surrogate_domain)
+__device__ __forceinline__ void photon_features(const mlgpu_surrogate_domain& dom, const mlgpu_photons& ph,
+                                                const int64_t i, float* f) {
+    const double dx = ph.direction_x[i], dy = ph.direction_y[i], dz = ph.direction_z[i];
+    const double dn = sqrt(dx * dx + dy * dy + dz * dz);
+    const double sa = sin(dom.max_angle);
+    const double il = 1.0 / ph.wavelength[i], ilo = 1.0 / dom.lambda_max, ihi = 1.0 / dom.lambda_min;
+    f[0] = static_cast<float>(ph.support_x[i] / dom.support_radius);
+    f[1] = static_cast<float>(ph.support_y[i] / dom.support_radius);
+    f[2] = static_cast<float>(dx / dn / sa);
+    f[3] = static_cast<float>(dy / dn / sa);
+    // a direction away from -z is outside the domain
+    if (!(dz < 0.0)) f[2] = 2.0f;
+    f[4] = static_cast<float>(2.0 * (il - ilo) / (ihi - ilo) - 1.0);
+}
+
+__global__ void prepare_kernel(const mlgpu_surrogate_domain dom, const mlgpu_photons ph, const mlgpu_results r,
+                               const bool with_targets, const double* __restrict__ poses, const int64_t n,
+                               float* __restrict__ features, float* __restrict__ targets) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    photon_features(dom, ph, i, features + i * kIn);
+    if (!with_targets) return;
+    float* t = targets + i * 4;
+    const int32_t fr = r.final_frame[i];
+    const bool det = r.sensor_id[i] >= 0 && r.error[i] == 0 && fr >= 0;
+    t[0] = det ? 1.0f : 0.0f;
+    if (det) {
+        const double* T = poses + 12 * static_cast<int64_t>(fr);
+        const double x = r.x[i], y = r.y[i];
+        t[1] = static_cast<float>((T[0] * x + T[1] * y + T[3]) / dom.sensor_radius);
+        t[2] = static_cast<float>((T[4] * x + T[5] * y + T[7]) / dom.sensor_radius);
+        t[3] = static_cast<float>((r.time_of_flight[i] - dom.time_offset) / dom.time_scale);
+    } else {
+        t[1] = t[2] = t[3] = 0.0f;
+    }
+}
+
+// nearest lattice cell of a sensor-plane point (cube rounding), or -1
+__device__ int32_t hex_cell(const mlgpu_hex_cells& c, const double x, const double y) {
+    const double px = x - c.origin[0], py = y - c.origin[1];
+    const double det = c.a1[0] * c.a2[1] - c.a1[1] * c.a2[0];
+    const double fi = (px * c.a2[1] - py * c.a2[0]) / det;
+    const double fj = (c.a1[0] * py - c.a1[1] * px) / det;
+    const double fk = -fi - fj;
+    double ri = rint(fi), rj = rint(fj), rk = rint(fk);
+    const double di = fabs(ri - fi), dj = fabs(rj - fj), dk = fabs(rk - fk);
+    if (di > dj && di > dk) ri = -rj - rk;
+    else if (dj > dk) rj = -ri - rk;
+    const int ii = static_cast<int>(ri), jj = static_cast<int>(rj);
+    if (ii < -c.radius || ii > c.radius || jj < -c.radius || jj > c.radius) return -1;
+    return c.cell[(ii + c.radius) * (2 * c.radius + 1) + (jj + c.radius)];
+}
+
+// The surrogate's draw for photon i of the call (features f, outputs o):
+// false when it is binned, true when it must fall back to propagation.
+__device__ bool sample_one(const SampleArgs& a, const float* f, const float* o, const int64_t i) {
+    const bool in_domain = f[0] * f[0] + f[1] * f[1] <= 1.0f && f[2] * f[2] + f[3] * f[3] <= 1.0f &&
+                           f[4] >= -1.0f && f[4] <= 1.0f;
+    // entropy bound of the mixture: H(weights) + sum_k pi_k H(component k)
+    float wmax = -INFINITY;
+    for (int k = 0; k < kComp; k++) wmax = fmaxf(wmax, o[1 + 7 * k]);
+    float wsum = 0.0f;
+    for (int k = 0; k < kComp; k++) wsum += expf(o[1 + 7 * k] - wmax);
+    const float lse = wmax + logf(wsum);
+    float h = 0.0f;
+    for (int k = 0; k < kComp; k++) {
+        const float lp = o[1 + 7 * k] - lse, p = expf(lp);
+        float hk = 3.0f * 1.4189385332046727f;  // 0.5 log(2 pi e) per dimension
+        for (int j = 0; j < 3; j++) hk += fminf(fmaxf(o[5 + 7 * k + j], kLogSigmaMin), kLogSigmaMax);
+        h += p * (hk - lp);
+    }
+    if (!in_domain || !(h <= static_cast<float>(a.max_entropy))) return true;
+    const uint64_t id = a.id_offset + static_cast<uint64_t>(i);
+    const uint32_t id_lo = static_cast<uint32_t>(id), id_hi = static_cast<uint32_t>(id >> 32);
+    const uint32_t k_lo = static_cast<uint32_t>(a.seed), k_hi = static_cast<uint32_t>(a.seed >> 32);
+    uint32_t w[4], v[4];
+    philox4x32_10(0u, 3u, id_lo, id_hi, k_lo, k_hi, w[0], w[1], w[2], w[3]);
+    const double survive = 1.0 / (1.0 + exp(-static_cast<double>(o[0])));
+    if (!(u53(w[0], w[1]) < survive)) return false;
+    // component by the cumulative weights
+    const double uc = u53(w[2], w[3]);
+    int k = kComp - 1;
+    double acc = 0.0;
+    for (int q = 0; q < kComp; q++) {
+        acc += exp(static_cast<double>(o[1 + 7 * q] - lse));
+        if (uc < acc) { k = q; break; }
+    }
+    philox4x32_10(1u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
+    const double r1 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
+    double s1, c1;
+    sincos_turn(u53(v[2], v[3]), s1, c1);
+    philox4x32_10(2u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
+    const double r2 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
+    double s2, c2;
+    sincos_turn(u53(v[2], v[3]), s2, c2);
+    const double z[3] = {r1 * c1, r1 * s1, r2 * c2};
+    double y[3];
+    for (int j = 0; j < 3; j++) {
+        const float ls = fminf(fmaxf(o[5 + 7 * k + j], kLogSigmaMin), kLogSigmaMax);
+        y[j] = static_cast<double>(o[2 + 7 * k + j]) + exp(static_cast<double>(ls)) * z[j];
+    }
+    const double x = y[0] * a.dom.sensor_radius, yy = y[1] * a.dom.sensor_radius;
+    const double t = y[2] * a.dom.time_scale + a.dom.time_offset;
+    const int32_t cell = hex_cell(a.cells, x, yy);
+    const int64_t slot = arrival_cell(a.hist, cell, 0, t, a.hist.photon_id_offset + static_cast<uint64_t>(i));
+    if (slot >= 0) atomicAdd(&a.hist.counts[slot], 1u);
+    return false;
+}
+
+// wave-aggregated append of the fallback photons (every lane of the wave calls it)
+__device__ __forceinline__ void append_fallback(const SampleArgs& a, const bool fallback, const int64_t i) {
+    const uint64_t mask = __ballot(fallback);
+    if (mask == 0) return;
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(a.fallback_count, __popcll(mask));
+    base = __shfl(base, leader);
+    if (fallback) a.fallback[base + __popcll(mask & ((1ull << lane) - 1ull))] = static_cast<int32_t>(i);
+}
+
+__global__ __launch_bounds__(256) void sample_kernel(const SampleArgs a, const int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool fallback = i < n && sample_one(a, a.features + i * kIn, a.out + i * kOutPad, i);
+    append_fallback(a, fallback, i);
+}
+
+// ------------------------------------------------------- fused inference
+// Features, the whole network and the draws for tiles of kTileP photons in
+// one kernel.  Activations stay in LDS as bf16, photon-major (kTileP rows
+// of kHStride).  The matrix instructions take the weights as the A operand
+// (rows = output features) and the activations as B (columns = photons),
+// so a lane's accumulators hold four consecutive outputs of one photon,
+// stored to LDS as one 8-byte write.  The weights come from a copy laid out
+// in fragment order (pad_weights_kernel): one 1 KB coalesced load feeds a
+// wave's instruction.  Wave w computes outputs [64 w, 64 w + 64) of all
+// photons in the hidden layers and output tile w % 2 of photon tiles
+// 2 (w / 2), + 1 in the last layer.  Optionally writes the outputs
+// (FusedArgs::out) and skips the draws (FusedArgs::sample false).
+constexpr int kTileP = 128;
+constexpr int kHStride = kHidden + 8;
+constexpr int kFStride = 16 + 8;
+constexpr int kOStride = kOutPad + 4;
+
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    const uint16_t a = __builtin_bit_cast(uint16_t, static_cast<__bf16>(lo));
+    const uint16_t b = __builtin_bit_cast(uint16_t, static_cast<__bf16>(hi));
+    return static_cast<uint32_t>(a) | (static_cast<uint32_t>(b) << 16);
+}
+
+__global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const int64_t n) {
+    __shared__ __attribute__((aligned(16))) uint16_t H[kTileP * kHStride];
+    __shared__ __attribute__((aligned(16))) uint16_t F16[kTileP * kFStride];
+    __shared__ float F[kTileP * kIn];
+    float* O = reinterpret_cast<float*>(H);
+    const int t = static_cast<int>(threadIdx.x), lane = t & 63, w = t >> 6;
+    const int lr = lane & 31, lk = (lane >> 5) * 8, lq = (lane >> 5) * 4;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kTileP; base < n;
+         base += static_cast<int64_t>(gridDim.x) * kTileP) {
+        if (t < kTileP) {
+            const int64_t i = base + t;
+            float* f = F + t * kIn;
+            if (i < n) {
+                if (a.features) {
+#pragma unroll
+                    for (int j = 0; j < kIn; j++) f[j] = a.features[i * kIn + j];
+                } else {
+                    photon_features(a.s.dom, a.ph, i, f);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < kIn; j++) f[j] = 0.0f;
+            }
+            uint32_t* row = reinterpret_cast<uint32_t*>(F16 + t * kFStride);
+            row[0] = pack_bf16(f[0], f[1]);
+            row[1] = pack_bf16(f[2], f[3]);
+            row[2] = pack_bf16(f[4], 0.0f);
+#pragma unroll
+            for (int j = 3; j < 8; j++) row[j] = 0u;
+        }
+        __syncthreads();
+        // layer 0 (K = 16, five of them non-zero) and the hidden layers
+        for (int l = 0; l < kLayers - 1; l++) {
+            const bf16x8* Wf = reinterpret_cast<const bf16x8*>(a.wf[l]);
+            const int ksteps = l == 0 ? 1 : kHidden / 16;
+            const uint16_t* src = l == 0 ? F16 : H;
+            const int stride = l == 0 ? kFStride : kHStride;
+            f32x16 acc[2][4];
+#pragma unroll
+            for (int ot = 0; ot < 2; ot++)
+#pragma unroll
+                for (int pt = 0; pt < 4; pt++)
+#pragma unroll
+                    for (int v = 0; v < 16; v++) acc[ot][pt][v] = 0.0f;
+            for (int ks = 0; ks < ksteps; ks++) {
+                bf16x8 wv[2];
+#pragma unroll
+                for (int ot = 0; ot < 2; ot++) wv[ot] = Wf[((2 * w + ot) * ksteps + ks) * 64 + lane];
+#pragma unroll
+                for (int pt = 0; pt < 4; pt++) {
+                    const bf16x8 hv = *reinterpret_cast<const bf16x8*>(src + (pt * 32 + lr) * stride + ks * 16 + lk);
+#pragma unroll
+                    for (int ot = 0; ot < 2; ot++)
+                        acc[ot][pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv[ot], hv, acc[ot][pt], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ot = 0; ot < 2; ot++) {
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const int col = w * 64 + ot * 32 + g * 8 + lq;
+                    const float4 bias = *reinterpret_cast<const float4*>(a.b[l] + col);
+#pragma unroll
+                    for (int pt = 0; pt < 4; pt++) {
+                        const int row = pt * 32 + lr;
+                        const float z0 = fmaxf(acc[ot][pt][4 * g + 0] + bias.x, 0.0f);
+                        const float z1 = fmaxf(acc[ot][pt][4 * g + 1] + bias.y, 0.0f);
+                        const float z2 = fmaxf(acc[ot][pt][4 * g + 2] + bias.z, 0.0f);
+                        const float z3 = fmaxf(acc[ot][pt][4 * g + 3] + bias.w, 0.0f);
+                        uint2 packed;
+                        packed.x = pack_bf16(z0, z1);
+                        packed.y = pack_bf16(z2, z3);
+                        *reinterpret_cast<uint2*>(H + row * kHStride + col) = packed;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        {
+            const bf16x8* Wf = reinterpret_cast<const bf16x8*>(a.wf[kLayers - 1]);
+            const int ot = w & 1, pt0 = (w >> 1) * 2;
+            f32x16 acc[2];
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) acc[p][v] = 0.0f;
+#pragma unroll 2
+            for (int ks = 0; ks < kHidden / 16; ks++) {
+                const bf16x8 wv = Wf[(ot * (kHidden / 16) + ks) * 64 + lane];
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    const bf16x8 hv =
+                        *reinterpret_cast<const bf16x8*>(H + ((pt0 + p) * 32 + lr) * kHStride + ks * 16 + lk);
+                    acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, hv, acc[p], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int col = ot * 32 + g * 8 + lq;
+                const float4 bias = *reinterpret_cast<const float4*>(a.b[kLayers - 1] + col);
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    const int row = (pt0 + p) * 32 + lr;
+                    float4 z;
+                    z.x = acc[p][4 * g + 0] + bias.x;
+                    z.y = acc[p][4 * g + 1] + bias.y;
+                    z.z = acc[p][4 * g + 2] + bias.z;
+                    z.w = acc[p][4 * g + 3] + bias.w;
+                    *reinterpret_cast<float4*>(O + row * kOStride + col) = z;
+                }
+            }
+            __syncthreads();
+        }
+        if (a.out) {
+            for (int e = t; e < kTileP * kOut; e += 256) {
+                const int row = e / kOut, col = e % kOut;
+                if (base + row < n) a.out[(base + row) * kOut + col] = O[row * kOStride + col];
+            }
+        }
+        if (a.sample && w < 2) {
+            const int64_t i = base + t;
+            const bool fallback = i < n && sample_one(a.s, F + t * kIn, O + t * kOStride, i);
+            append_fallback(a.s, fallback, i);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void gather_photons_kernel(const mlgpu_photons src, const int32_t* __restrict__ idx,
+                                      const int32_t* __restrict__ count, const int64_t n,
+                                      const mlgpu_photon_buffers dst) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n || j >= *count) return;
+    const int64_t i = idx[j];
+    dst.support_x[j] = src.support_x[i];
+    dst.support_y[j] = src.support_y[i];
+    dst.support_z[j] = src.support_z[i];
+    dst.direction_x[j] = src.direction_x[i];
+    dst.direction_y[j] = src.direction_y[i];
+    dst.direction_z[j] = src.direction_z[i];
+    dst.wavelength[j] = src.wavelength[i];
 }
 
 int64_t padded_elements() {
@@ -318,6 +626,38 @@ hipError_t launch_adam(const AdamArgs& a, hipStream_t stream) {
 hipError_t launch_init_params(float* params, uint64_t seed, hipStream_t stream) {
     hipLaunchKernelGGL(init_params_kernel, dim3(static_cast<uint32_t>((kNumParams + 255) / 256)), dim3(256), 0, stream,
                        params, seed);
+    return hipGetLastError();
+}
+
+hipError_t launch_prepare(const mlgpu_surrogate_domain& dom, const mlgpu_photons& ph, const mlgpu_results* r,
+                          const double* poses, int64_t n, float* features, float* targets, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const mlgpu_results none{};
+    hipLaunchKernelGGL(prepare_kernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, stream, dom, ph,
+                       r ? *r : none, r != nullptr, poses, n, features, targets);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample(const SampleArgs& a, int64_t n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sample_kernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, stream, a, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused(const FusedArgs& a, int64_t n, int num_cus, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int64_t tiles = (n + kTileP - 1) / kTileP;
+    const int64_t cap = static_cast<int64_t>(num_cus) * 2;
+    hipLaunchKernelGGL(fused_kernel, dim3(static_cast<uint32_t>(tiles < cap ? tiles : cap)), dim3(256), 0, stream, a,
+                       n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_photons(const mlgpu_photons& src, const int32_t* idx, const int32_t* count, int64_t n,
+                                 const mlgpu_photon_buffers& dst, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_photons_kernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, stream, src,
+                       idx, count, n, dst);
     return hipGetLastError();
 }
 

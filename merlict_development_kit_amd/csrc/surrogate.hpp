@@ -96,11 +96,50 @@ hipError_t launch_adam(const AdamArgs& a, hipStream_t stream);
 struct PaddedWeights {
     uint16_t* w[kLayers];
     float* b[kLayers];
+    uint16_t* wf[kLayers];   // fragment-ordered copy for the fused inference kernel
 };
 hipError_t launch_pad_weights(const float* params, const PaddedWeights& pw, hipStream_t stream);
 
 // He-normal initial parameters from Philox (seed).
 hipError_t launch_init_params(float* params, uint64_t seed, hipStream_t stream);
+
+// Features and (when r is not null) targets of photons (mlgpu_surrogate_prepare).
+hipError_t launch_prepare(const mlgpu_surrogate_domain& dom, const mlgpu_photons& ph, const mlgpu_results* r,
+                          const double* poses, int64_t n, float* features, float* targets, hipStream_t stream);
+
+// Surrogate draws of n photons from their outputs (rows of kOutPad fp32):
+// photons in the domain with a low entropy bound are binned into hist;
+// the others are appended to fallback (count in *fallback_count).
+struct SampleArgs {
+    const float* out;
+    const float* features;   // n x kIn, for the domain test
+    mlgpu_surrogate_domain dom;
+    mlgpu_hex_cells cells;
+    mlgpu_histogram hist;
+    double max_entropy;
+    uint64_t seed;
+    uint64_t id_offset;      // Philox id of photon 0
+    int32_t* fallback;
+    int32_t* fallback_count;
+};
+hipError_t launch_sample(const SampleArgs& a, int64_t n, hipStream_t stream);
+
+// Features, network and draws in one kernel (the inference path): photons
+// (or ready features) in, outputs (n x kOut, optional) and draws out.
+struct FusedArgs {
+    mlgpu_photons ph;
+    const float* features;     // n x kIn instead of photons, or null
+    SampleArgs s;              // its out and features are unused
+    bool sample;
+    float* out;                // n x kOut, or null
+    const uint16_t* wf[kLayers];  // fragment-ordered bf16 weights
+    const float* b[kLayers];
+};
+hipError_t launch_fused(const FusedArgs& a, int64_t n, int num_cus, hipStream_t stream);
+
+// dst[j] = src[idx[j]] for the 7 photon arrays, j < *count (count <= n)
+hipError_t launch_gather_photons(const mlgpu_photons& src, const int32_t* idx, const int32_t* count, int64_t n,
+                                 const mlgpu_photon_buffers& dst, hipStream_t stream);
 
 }  // namespace sur
 }  // namespace mlgpu

@@ -3,6 +3,7 @@
 // products run in surrogate.hip on the matrix cores.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -32,6 +33,14 @@ struct mlgpu_surrogate {
     uint16_t* gh[2] = {};              // rows x kHidden, gradients wrt pre-activations
     float* partial = nullptr;          // split-K slices of a weight gradient
     float* loss = nullptr;             // scalar
+    // hybrid buffers, one chunk of photons
+    void* hy = nullptr;
+    float* hy_features = nullptr;      // chunk x kIn
+    int32_t* hy_fallback = nullptr;    // chunk
+    int32_t* hy_count = nullptr;       // 1
+    mlgpu_photon_buffers hy_photons{}; // chunk, the gathered fallback photons
+    mlgpu_results hy_results{};        // chunk, their plain results
+    int32_t* hy_count_host = nullptr;  // pinned
 };
 
 namespace {
@@ -153,6 +162,68 @@ hipError_t backward(mlgpu_surrogate* s, const float* targets, int64_t n, int64_t
     return hipSuccess;
 }
 
+constexpr int64_t kHybridChunk = int64_t{1} << 22;
+
+int ensure_hybrid(mlgpu_surrogate* s) {
+    if (s->hy) return MLGPU_OK;
+    const size_t c = static_cast<size_t>(kHybridChunk);
+    const size_t bytes = align256(c * kIn * 4) + align256(c * 4) + 256 + 7 * align256(c * 8) + 2 * align256(c * 4) +
+                         3 * align256(c) + align256(c * 8) + 4 * align256(c * 4);
+    hipError_t e = hipMalloc(&s->hy, bytes);
+    if (e != hipSuccess) { s->hy = nullptr; return hip_fail(e, "hipMalloc hybrid buffers"); }
+    e = hipHostMalloc(reinterpret_cast<void**>(&s->hy_count_host), sizeof(int32_t));
+    if (e != hipSuccess) { (void)hipFree(s->hy); s->hy = nullptr; return hip_fail(e, "hipHostMalloc"); }
+    char* p = static_cast<char*>(s->hy);
+    auto take = [&](size_t b) { char* q = p; p += align256(b); return q; };
+    s->hy_features = reinterpret_cast<float*>(take(c * kIn * 4));
+    s->hy_fallback = reinterpret_cast<int32_t*>(take(c * 4));
+    s->hy_count = reinterpret_cast<int32_t*>(take(256));
+    double** ph[7] = {&s->hy_photons.support_x, &s->hy_photons.support_y, &s->hy_photons.support_z,
+                      &s->hy_photons.direction_x, &s->hy_photons.direction_y, &s->hy_photons.direction_z,
+                      &s->hy_photons.wavelength};
+    for (double** q : ph) *q = reinterpret_cast<double*>(take(c * 8));
+    s->hy_results.final_frame = reinterpret_cast<int32_t*>(take(c * 4));
+    s->hy_results.sensor_id = reinterpret_cast<int32_t*>(take(c * 4));
+    s->hy_results.final_interaction = reinterpret_cast<uint8_t*>(take(c));
+    s->hy_results.num_interactions = reinterpret_cast<uint8_t*>(take(c));
+    s->hy_results.error = reinterpret_cast<uint8_t*>(take(c));
+    s->hy_results.time_of_flight = reinterpret_cast<double*>(take(c * 8));
+    s->hy_results.x = reinterpret_cast<float*>(take(c * 4));
+    s->hy_results.y = reinterpret_cast<float*>(take(c * 4));
+    s->hy_results.cx = reinterpret_cast<float*>(take(c * 4));
+    s->hy_results.cy = reinterpret_cast<float*>(take(c * 4));
+    s->hy_results.accumulated_distance = nullptr;
+    return MLGPU_OK;
+}
+
+bool domain_ok(const mlgpu_surrogate_domain* d) {
+    return d && d->support_radius > 0.0 && d->max_angle > 0.0 && d->max_angle < 1.5707963267948966 &&
+           d->lambda_min > 0.0 && d->lambda_max > d->lambda_min && d->sensor_radius > 0.0 && d->time_scale > 0.0 &&
+           std::isfinite(d->support_radius) && std::isfinite(d->lambda_max) && std::isfinite(d->sensor_radius) &&
+           std::isfinite(d->time_offset) && std::isfinite(d->time_scale);
+}
+
+int num_cus_of(int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
+    return cus;
+}
+
+FusedArgs fused_args(const mlgpu_surrogate* s) {
+    FusedArgs a{};
+    for (int l = 0; l < kLayers; l++) {
+        a.wf[l] = s->pw.wf[l];
+        a.b[l] = s->pw.b[l];
+    }
+    return a;
+}
+
+// bf16 elements of layer l's fragment-ordered weight copy
+size_t fragment_elements(int l) {
+    const int ksteps = l == 0 ? 1 : kPadIn[l] / 16;
+    return static_cast<size_t>(kPadOut[l] / 32) * ksteps * 64 * 8;
+}
+
 int check_batch(const mlgpu_surrogate* s, const float* features, int64_t n) {
     if (!s || !features || n < 1) { set_last_error("null argument or empty batch"); return MLGPU_ERR_ARGUMENT; }
     if (n > (int64_t{1} << 24)) { set_last_error("at most 2^24 photons per surrogate batch"); return MLGPU_ERR_ARGUMENT; }
@@ -179,7 +250,8 @@ int mlgpu_surrogate_create(uint64_t seed, mlgpu_surrogate** out) {
     s->adam_v = reinterpret_cast<float*>(p + 3 * align256(pbytes));
     size_t wbytes = 0;
     for (int l = 0; l < kLayers; l++)
-        wbytes += align256(static_cast<size_t>(kPadOut[l]) * kPadIn[l] * 2) + align256(static_cast<size_t>(kPadOut[l]) * 4);
+        wbytes += align256(static_cast<size_t>(kPadOut[l]) * kPadIn[l] * 2) + align256(static_cast<size_t>(kPadOut[l]) * 4) +
+                  align256(fragment_elements(l) * 2);
     e = hipMalloc(&s->weights_mem, wbytes);
     if (e != hipSuccess) { (void)hipFree(s->params); delete s; return hip_fail(e, "hipMalloc surrogate weights"); }
     char* q = static_cast<char*>(s->weights_mem);
@@ -188,6 +260,8 @@ int mlgpu_surrogate_create(uint64_t seed, mlgpu_surrogate** out) {
         q += align256(static_cast<size_t>(kPadOut[l]) * kPadIn[l] * 2);
         s->pw.b[l] = reinterpret_cast<float*>(q);
         q += align256(static_cast<size_t>(kPadOut[l]) * 4);
+        s->pw.wf[l] = reinterpret_cast<uint16_t*>(q);
+        q += align256(fragment_elements(l) * 2);
     }
     e = launch_init_params(s->params, seed, nullptr);
     if (e == hipSuccess) e = hipMemsetAsync(s->adam_m, 0, pbytes, nullptr);
@@ -203,6 +277,8 @@ void mlgpu_surrogate_destroy(mlgpu_surrogate* s) {
     if (!s) return;
     (void)hipDeviceSynchronize();
     if (s->ws) (void)hipFree(s->ws);
+    if (s->hy) (void)hipFree(s->hy);
+    if (s->hy_count_host) (void)hipHostFree(s->hy_count_host);
     if (s->weights_mem) (void)hipFree(s->weights_mem);
     if (s->params) (void)hipFree(s->params);
     delete s;
@@ -235,14 +311,11 @@ int mlgpu_surrogate_forward(mlgpu_surrogate* s, const float* features, int64_t n
     if (rc) return rc;
     if (!out) { set_last_error("null output"); return MLGPU_ERR_ARGUMENT; }
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    std::lock_guard<std::mutex> lock(s->mutex);
-    const int64_t rows = padded_rows(n);
-    if ((rc = ensure_workspace(s, rows))) return rc;
-    hipError_t e = launch_pack_features(features, n, rows, s->x0, st);
-    if (e == hipSuccess) e = forward(s, rows, st);
-    if (e == hipSuccess)
-        e = hipMemcpy2DAsync(out, kOut * 4, s->out, kOutPad * 4, kOut * 4, static_cast<size_t>(n),
-                             hipMemcpyDeviceToDevice, st);
+    FusedArgs a = fused_args(s);
+    a.features = features;
+    a.out = out;
+    a.sample = false;
+    const hipError_t e = launch_fused(a, n, num_cus_of(s->device), st);
     if (e != hipSuccess) return hip_fail(e, "surrogate forward");
     return MLGPU_OK;
 }
@@ -297,6 +370,76 @@ int mlgpu_surrogate_train_step(mlgpu_surrogate* s, const float* features, const 
     if (e == hipSuccess) e = launch_pad_weights(s->params, s->pw, st);
     if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, s->loss, 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "surrogate train step");
+    return MLGPU_OK;
+}
+
+int mlgpu_surrogate_prepare(const mlgpu_surrogate_domain* dom, const mlgpu_photons* ph, const mlgpu_results* r,
+                            const double* poses, int64_t n, float* features, float* targets, void* stream) {
+    if (!domain_ok(dom) || !ph || !features || n < 0 || (r && (!targets || !poses))) {
+        set_last_error("bad domain or null argument");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const hipError_t e = launch_prepare(*dom, *ph, r, poses, n, features, targets, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "surrogate prepare");
+    return MLGPU_OK;
+}
+
+int mlgpu_propagate_hybrid(const mlgpu_scene* scene, mlgpu_surrogate* s, const mlgpu_surrogate_domain* dom,
+                           const mlgpu_hex_cells* cells, int32_t world_frame, const mlgpu_photons* ph, int64_t n,
+                           const mlgpu_config* cfg, const mlgpu_hybrid_config* hc, const mlgpu_histogram* hist,
+                           int64_t* num_fallback, void* stream) {
+    if (!scene || !s || !domain_ok(dom) || !cells || !cells->cell || cells->radius < 0 || !ph || !cfg || !hc ||
+        !hist || !hist->counts || n < 0) {
+        set_last_error("bad domain or null argument");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (num_fallback) *num_fallback = 0;
+    if (n == 0) return MLGPU_OK;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(s->mutex);
+    int rc;
+    if ((rc = ensure_hybrid(s))) return rc;
+    const int cus = num_cus_of(s->device);
+    int64_t fallbacks = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += kHybridChunk) {
+        const int64_t m = std::min(kHybridChunk, n - c0);
+        const mlgpu_photons chunk{ph->support_x + c0, ph->support_y + c0, ph->support_z + c0, ph->direction_x + c0,
+                                  ph->direction_y + c0, ph->direction_z + c0, ph->wavelength + c0};
+        hipError_t e = hipMemsetAsync(s->hy_count, 0, sizeof(int32_t), st);
+        if (e == hipSuccess) {
+            FusedArgs a = fused_args(s);
+            a.ph = chunk;
+            a.sample = true;
+            a.s.dom = *dom;
+            a.s.cells = *cells;
+            a.s.hist = *hist;
+            a.s.hist.photon_id_offset = hist->photon_id_offset + static_cast<uint64_t>(c0);
+            a.s.max_entropy = hc->max_entropy;
+            a.s.seed = hc->seed;
+            a.s.id_offset = cfg->photon_id_offset + static_cast<uint64_t>(c0);
+            a.s.fallback = s->hy_fallback;
+            a.s.fallback_count = s->hy_count;
+            e = launch_fused(a, m, cus, st);
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(s->hy_count_host, s->hy_count, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "hybrid surrogate pass");
+        const int64_t k = *s->hy_count_host;
+        if (k == 0) continue;
+        fallbacks += k;
+        e = launch_gather_photons(chunk, s->hy_fallback, s->hy_count, m, s->hy_photons, st);
+        if (e != hipSuccess) return hip_fail(e, "hybrid gather");
+        const mlgpu_photons fb{s->hy_photons.support_x, s->hy_photons.support_y, s->hy_photons.support_z,
+                               s->hy_photons.direction_x, s->hy_photons.direction_y, s->hy_photons.direction_z,
+                               s->hy_photons.wavelength};
+        mlgpu_config fcfg = *cfg;
+        fcfg.photon_id_offset = cfg->photon_id_offset + static_cast<uint64_t>(c0);
+        if ((rc = mlgpu_propagate(scene, world_frame, &fb, k, &fcfg, &s->hy_results, stream))) return rc;
+        mlgpu_histogram fh = *hist;
+        fh.photon_id_offset = hist->photon_id_offset + static_cast<uint64_t>(c0);
+        if ((rc = mlgpu_histogram_arrivals(&s->hy_results, k, &fh, stream))) return rc;
+    }
+    if (num_fallback) *num_fallback = fallbacks;
     return MLGPU_OK;
 }
 

@@ -8,8 +8,6 @@ the surface functions and the sensor ids.
 from __future__ import annotations
 
 import ctypes as C
-import math
-from dataclasses import dataclass
 
 import numpy as np
 
@@ -138,6 +136,14 @@ class SceneDescription:
             "func_y": arr(v.func_y, npts, np.float64),
         }
 
+    def frame_poses(self) -> np.ndarray:
+        """frame2world of every frame (n, 3, 4), as merlict's initialised tree
+        has it."""
+        out = np.zeros((self.num_frames, 3, 4), dtype=np.float64)
+        N.check(N.lib().mlgpu_desc_frame_poses(C.byref(self._view), out.ctypes.data_as(C.POINTER(C.c_double))),
+                "mlgpu_desc_frame_poses")
+        return out
+
     def sensor_frames(self) -> np.ndarray:
         """Frame index of every sensor, indexed by sensor id."""
         s = self.arrays()["sensor_id"]
@@ -146,22 +152,3 @@ class SceneDescription:
         out[s[idx]] = idx
         return out
 
-
-@dataclass
-class LensBenchGeometry:
-    """Derived numbers of the lens test bench (for tests)."""
-    focal_length: float
-    aperture_radius: float
-    refractive_index: float
-
-    def curvature_radius(self) -> float:
-        # merlict/scenery/geometry/lens_maker.cpp:15-46, restated for checks
-        f, r, n = self.focal_length, self.aperture_radius, self.refractive_index
-        R, dev, it = f, 0.0, 0
-        while True:
-            it += 1
-            R = R - dev * 0.1
-            t = 2.0 * R - 2.0 * math.sqrt(R * R - r * r)
-            dev = 1.0 / ((n - 1.0) * (2.0 / R - ((n - 1.0) * t) / (n * R * R))) - f
-            if abs(dev) <= 1e-6 or it > 1000:
-                return R

@@ -92,3 +92,164 @@ class Surrogate:
         N.check(N.lib().mlgpu_surrogate_train_step(self._h, _ptr(features), _ptr(targets), len(features),
                                                    float(learning_rate), _ptr(loss) if loss is not None else None,
                                                    _stream(stream)), "mlgpu_surrogate_train_step")
+
+
+# ------------------------------------------------------------ domain, cells
+import math  # noqa: E402
+from dataclasses import dataclass, replace  # noqa: E402
+
+import numpy as np  # noqa: E402
+
+from .arrivals import ArrivalBinning  # noqa: E402
+from .propagation import GpuScene, PhotonBatch, PropagationConfig, PropagationResult  # noqa: E402
+from .scenery import SceneDescription  # noqa: E402
+from . import sources as S  # noqa: E402
+
+
+@dataclass(frozen=True)
+class SurrogateDomain:
+    """mlgpu_surrogate_domain: where the surrogate is trained and trusted,
+    and the units of its targets."""
+    support_radius: float
+    max_angle: float
+    lambda_min: float
+    lambda_max: float
+    sensor_radius: float
+    time_offset: float = 0.0
+    time_scale: float = 1e-9
+
+    def c_struct(self) -> N.SurrogateDomain:
+        return N.SurrogateDomain(self.support_radius, self.max_angle, self.lambda_min, self.lambda_max,
+                                 self.sensor_radius, self.time_offset, self.time_scale)
+
+    def source(self, height: float) -> S.Source:
+        """Photons spanning the domain: supports uniform on the disc at z =
+        height, directions uniform on the cap within max_angle of -z, lambda
+        with pdf ~ lambda^-2."""
+        return S.Source("disc", "cap", self.support_radius, self.max_angle, self.lambda_min, self.lambda_max, 0.0,
+                        S.FLIP, (0.0, 0.0, height))
+
+
+def fact_domain() -> SurrogateDomain:
+    """The FACT-like telescope's domain: supports within 1.05 x the 2.1 m
+    mirror radius (fact_propagation.cpp:295), directions within 1.2 x the
+    camera's 2.2 deg field radius, lambda in [280, 700] nm; sensor plane
+    normalised by 0.2 m.  The time offset is set by :func:`train`."""
+    return SurrogateDomain(1.05 * 2.1, math.radians(1.2 * 2.2), 280e-9, 700e-9, 0.2)
+
+
+class HexCells:
+    """mlgpu_hex_cells of a scene's sensors: their world centres (frame
+    poses) must sit on a hexagonal lattice; the lattice and the lookup table
+    of axial coordinates are derived from them."""
+
+    def __init__(self, desc: SceneDescription, device="cuda"):
+        frames = desc.sensor_frames()
+        poses = desc.frame_poses()
+        centres = poses[frames][:, :2, 3]
+        d = centres[:, None, :] - centres[None, :, :]
+        dist = np.hypot(d[..., 0], d[..., 1])
+        np.fill_diagonal(dist, np.inf)
+        pitch = float(np.median(dist.min(axis=1)))
+        # first lattice vector: a nearest-neighbour offset of the most central cell
+        c0 = int(np.argmin(np.hypot(centres[:, 0], centres[:, 1])))
+        nb = np.nonzero(np.abs(dist[c0] - pitch) < 1e-3 * pitch)[0]
+        vecs = centres[nb] - centres[c0]
+        ang = np.mod(np.arctan2(vecs[:, 1], vecs[:, 0]), 2 * np.pi)
+        a1 = vecs[int(np.argmin(ang))]
+        rot = np.array([[0.5, -math.sqrt(3) / 2], [math.sqrt(3) / 2, 0.5]])
+        a2 = rot @ a1
+        origin = centres[c0]
+        A = np.stack([a1, a2], axis=1)
+        ij = np.linalg.solve(A, (centres - origin).T).T
+        rij = np.rint(ij)
+        if np.abs(ij - rij).max() > 1e-3:
+            raise ValueError("sensor centres are not on a hexagonal lattice")
+        radius = int(np.abs(rij).max())
+        table = np.full((2 * radius + 1) ** 2, -1, dtype=np.int32)
+        idx = ((rij[:, 0] + radius) * (2 * radius + 1) + (rij[:, 1] + radius)).astype(np.int64)
+        table[idx] = np.arange(len(frames), dtype=np.int32)
+        self.a1, self.a2, self.origin, self.radius = a1, a2, origin, radius
+        self.table = torch.tensor(table, device=device)
+        self.pitch = pitch
+
+    def c_struct(self) -> N.HexCells:
+        c = N.HexCells()
+        for k in range(2):
+            c.a1[k], c.a2[k], c.origin[k] = float(self.a1[k]), float(self.a2[k]), float(self.origin[k])
+        c.radius = self.radius
+        c.cell = self.table.data_ptr()
+        return c
+
+
+def prepare(domain: SurrogateDomain, photons: PhotonBatch, result: PropagationResult = None,
+            poses: torch.Tensor = None, stream=None):
+    """(features, targets) of photons and their propagation results."""
+    n = len(photons)
+    dev = photons.device
+    feats = torch.empty((n, 5), dtype=torch.float32, device=dev)
+    targs = torch.empty((n, 4), dtype=torch.float32, device=dev) if result is not None else None
+    ps = photons.c_struct()
+    rs = result.c_struct() if result is not None else None
+    N.check(N.lib().mlgpu_surrogate_prepare(C.byref(domain.c_struct()), C.byref(ps),
+                                            C.byref(rs) if rs is not None else None,
+                                            _ptr(poses) if poses is not None else None, n, _ptr(feats),
+                                            _ptr(targs) if targs is not None else None, _stream(stream)),
+            "mlgpu_surrogate_prepare")
+    return feats, targs
+
+
+def train(scene: GpuScene, desc: SceneDescription, surrogate: Surrogate, domain: SurrogateDomain, n_photons: int,
+          height: float = 10.0, batch: int = 16384, learning_rate: float = 1e-3, final_learning_rate: float = 3e-5,
+          seed: int = 0, chunk: int = 1 << 21, max_interactions: int = 5, device="cuda", log_every: int = 0):
+    """Train on n_photons drawn from the domain and propagated by the plain
+    propagator (Philox seed `seed`, ids 0 .. n_photons - 1), Adam with a
+    cosine learning-rate decay.  Returns the domain (its time offset and
+    scale set from the data when they were unset) and the loss history
+    (mean loss per batch, every log_every steps)."""
+    poses = torch.tensor(desc.frame_poses().reshape(-1), dtype=torch.float64, device=device)
+    src = domain.source(height)
+    cfg = PropagationConfig(max_interactions, seed=seed)
+    if domain.time_offset == 0.0:
+        ph = src.generate(1 << 18, seed=seed ^ 0x5EED, device=device)
+        res = scene.propagate(ph, cfg)
+        det = res.sensor_id >= 0
+        tof = res.time_of_flight[det]
+        domain = replace(domain, time_offset=float(tof.mean()), time_scale=float(max(tof.std().item(), 1e-12)))
+    steps = max(1, n_photons // batch)
+    step = 0
+    history = []
+    loss = torch.zeros(1, dtype=torch.float32, device=device)
+    for c0 in range(0, n_photons, chunk):
+        m = min(chunk, n_photons - c0)
+        ph = src.generate(m, seed=seed, photon_id_offset=c0, device=device)
+        res = scene.propagate(ph, PropagationConfig(max_interactions, seed=seed, photon_id_offset=c0))
+        f, t = prepare(domain, ph, res, poses)
+        for b0 in range(0, m - batch + 1, batch):
+            lr = final_learning_rate + 0.5 * (learning_rate - final_learning_rate) * (1 + math.cos(math.pi * step / steps))
+            surrogate.train_step(f[b0:b0 + batch], t[b0:b0 + batch], lr, loss)
+            if log_every and step % log_every == 0:
+                history.append((step, loss.item() / batch))
+            step += 1
+    return domain, history
+
+
+def propagate_hybrid(scene: GpuScene, surrogate: Surrogate, domain: SurrogateDomain, cells: HexCells,
+                     photons: PhotonBatch, binning: ArrivalBinning, config: PropagationConfig = PropagationConfig(),
+                     max_entropy: float = 0.0, seed: int = 0, counts: torch.Tensor = None, world_frame: int = 0,
+                     stream=None):
+    """Histogram of the photons' arrivals, each drawn from the surrogate or,
+    out of its domain or above the entropy bound, propagated.  Returns
+    (counts, number of propagated photons)."""
+    if counts is None:
+        counts = binning.empty(photons.device)
+    ps = photons.c_struct()
+    cfg = config.c_struct()
+    hc = N.HybridConfig(max_entropy, seed)
+    h = binning.c_struct(counts)
+    nf = C.c_int64()
+    N.check(N.lib().mlgpu_propagate_hybrid(scene.handle, surrogate.handle, C.byref(domain.c_struct()),
+                                           C.byref(cells.c_struct()), world_frame, C.byref(ps), len(photons),
+                                           C.byref(cfg), C.byref(hc), C.byref(h), C.byref(nf), _stream(stream)),
+            "mlgpu_propagate_hybrid")
+    return counts, nf.value
