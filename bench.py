@@ -18,6 +18,7 @@ import tempfile
 import time
 from pathlib import Path
 
+import numpy as np
 import torch
 
 ROOT = Path(__file__).resolve().parent
@@ -67,6 +68,69 @@ def cpu_baseline(desc: SceneDescription, photons, threads: int, target_seconds: 
                           f"{sec:.1f} s wall"}
 
 
+def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, warmup: int, train_photons: int,
+               max_entropy: float) -> dict:
+    """The surrogate + fallback hybrid on the same bunch: train the surrogate
+    on photons the plain propagator produced (its domain: supports within
+    1.05 x the mirror radius, directions within 2.64 deg, lambda 280-700 nm),
+    then time the hybrid and compare its per-pixel and per-pixel-and-time
+    histograms with the plain propagator's."""
+    from merlict_development_kit_amd import ArrivalBinning, histogram_arrivals
+    from merlict_development_kit_amd import surrogate as SG
+    dev = photons.device
+    n = len(photons)
+    cells = SG.HexCells(desc, dev)
+    sur = SG.Surrogate(seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dom, _ = SG.train(scene, desc, sur, SG.fact_domain(), train_photons, seed=3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    sigma = dom.time_scale
+    binning = ArrivalBinning(1440, 16, dom.time_offset - 8 * sigma, sigma)
+    plain = histogram_arrivals(scene.propagate(photons, PropagationConfig(5, seed=0)), binning)
+    plain = plain.cpu().numpy().astype(np.float64)
+    again = histogram_arrivals(scene.propagate(photons, PropagationConfig(5, seed=1)), binning)
+    again = again.cpu().numpy().astype(np.float64)
+    counts = binning.empty(dev)
+    fallback = 0
+    for step in range(warmup + steps):
+        if step == warmup:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            counts.zero_()
+            fallback = 0
+        _, f = SG.propagate_hybrid(scene, sur, dom, cells, photons, binning, PropagationConfig(5, seed=step),
+                                   max_entropy=max_entropy, seed=step, counts=counts)
+        fallback += f
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    hy = counts.cpu().numpy().astype(np.float64) / steps
+
+    def l1(a, b):
+        return float(np.abs(a - b).sum() / b.sum())
+
+    value = n * steps / wall
+    cell_l1 = l1(hy.sum(axis=1), plain.sum(axis=1))
+    return {
+        "value": value,
+        "unit": "photons/s",
+        "ms_per_step": wall / steps * 1e3,
+        "max_entropy": max_entropy,
+        "fallback_fraction": fallback / (n * steps),
+        "pixel_l1": cell_l1,
+        "pixel_time_l1": l1(hy, plain),
+        "noise_floor_pixel_l1": l1(again.sum(axis=1), plain.sum(axis=1)),
+        "noise_floor_pixel_time_l1": l1(again, plain),
+        "meets_1pct_l1": cell_l1 <= 0.01,
+        "trained_photons": train_photons,
+        "train_seconds": train_s,
+        "network": "MLP 5-256-256-256-256-57, bf16 on MFMA 32x32x16, fp32 accumulation",
+        "mfma_tflops": value * 2 * 212_480 / 1e12,
+        "mfma_frac_of_bf16_dense_peak": value * 2 * 212_480 / 2.5e15,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +143,9 @@ def main() -> int:
     ap.add_argument("--schedule", default="auto", choices=["auto", "fused", "wavefront"])
     ap.add_argument("--trace-budget", type=int, default=None)
     ap.add_argument("--blocks-per-cu", type=int, default=None)
+    ap.add_argument("--no-hybrid", action="store_true")
+    ap.add_argument("--hybrid-train-photons", type=int, default=100_000_000)
+    ap.add_argument("--hybrid-max-entropy", type=float, default=float("inf"))
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy a float32 buffer of this size (a plain copy kernel of known "
                          "traffic, for scaling rocprof's FETCH_SIZE / WRITE_SIZE)")
@@ -159,6 +226,10 @@ def main() -> int:
             "algorithmic_bytes_per_photon": IN_BYTES + OUT_BYTES,
         },
     }
+    if not args.no_hybrid:
+        line["hybrid"] = hybrid_leg(desc, scene, photons, args.steps, args.warmup, args.hybrid_train_photons,
+                                    args.hybrid_max_entropy)
+        line["hybrid"]["speedup_vs_plain"] = line["hybrid"]["value"] / value
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(desc, photons, args.cpu_threads)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]

@@ -380,14 +380,17 @@ int mlgpu_surrogate_train_step(mlgpu_surrogate* s, const float* features,
  * sin(max_angle), and 2 (1/lambda - 1/lambda_max) / (1/lambda_min -
  * 1/lambda_max) - 1.  In the domain: support within r of the axis,
  * direction within max_angle of -z, lambda in [lambda_min, lambda_max].
- * Targets: sensor-plane position (the final intersection in the world, x
- * and y over sensor_radius) and (time_of_flight - time_offset) / time_scale. */
+ * Targets: the sensor-plane position (the world centre of the sensor frame
+ * the photon ends on) relative to the nominal image point image_scale
+ * (dx / dz, dy / dz) of the photon's direction, over position_scale, and
+ * (time_of_flight - time_offset) / time_scale. */
 typedef struct mlgpu_surrogate_domain {
     double support_radius;
     double max_angle;
     double lambda_min;
     double lambda_max;
-    double sensor_radius;
+    double image_scale;
+    double position_scale;
     double time_offset;
     double time_scale;
 } mlgpu_surrogate_domain;
@@ -417,8 +420,9 @@ typedef struct mlgpu_hex_cells {
  * entropy, nats, normalised units) is at most max_entropy is drawn from the
  * surrogate (survival, component, position, time from Philox stream 3 of
  * (seed, config photon_id_offset + i)) and binned into hist at the cell
- * its position falls in; every other photon is propagated by the plain
- * propagator and binned from its result.  *num_fallback (host, may be
+ * its position falls in (a detected photon's position is drawn again, up to
+ * four times, until it falls on a cell); every other photon is propagated
+ * by the plain propagator and binned from its result.  *num_fallback (host, may be
  * NULL) receives the number of propagated photons.  Synchronises the
  * stream once per chunk of 2^22 photons. */
 typedef struct mlgpu_hybrid_config {

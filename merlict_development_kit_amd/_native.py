@@ -104,7 +104,8 @@ class Bunch(C.Structure):
 
 class SurrogateDomain(C.Structure):
     _fields_ = [(name, C.c_double) for name in (
-        "support_radius", "max_angle", "lambda_min", "lambda_max", "sensor_radius", "time_offset", "time_scale")]
+        "support_radius", "max_angle", "lambda_min", "lambda_max", "image_scale", "position_scale", "time_offset",
+        "time_scale")]
 
 
 class HexCells(C.Structure):

@@ -105,18 +105,46 @@ __global__ void sum_slices_kernel(const float* __restrict__ src, const int64_t c
     dst[i] = s;
 }
 
-// 64 columns per block; threads (c, r) sum rows r, r + 4, ... of column c.
-__global__ __launch_bounds__(256) void column_sums_kernel(const uint16_t* __restrict__ g, const int M, const int N,
-                                                          const int ld, float* __restrict__ db) {
-    __shared__ float part[4][64];
-    const int c = static_cast<int>(threadIdx.x) & 63, r = static_cast<int>(threadIdx.x) >> 6;
-    const int n = static_cast<int>(blockIdx.x) * 64 + c;
-    float s = 0.0f;
-    if (n < N)
-        for (int m = r; m < M; m += 4) s += bf2f(g[static_cast<size_t>(m) * ld + n]);
-    part[r][c] = s;
+// Column sums of a bf16 M x ld matrix, first stage: block z sums rows
+// [256 z, 256 z + 256) of every column c < ld into part[z * ld + c].
+// Threads read 8 consecutive columns (16 bytes) of a row; ld is a multiple
+// of 8 dividing 2048.
+constexpr int kColRows = 256;
+__global__ __launch_bounds__(256) void column_partials_kernel(const uint16_t* __restrict__ g, const int M,
+                                                              const int ld, float* __restrict__ part) {
+    __shared__ float red[2048];
+    const int groups = ld / 8, t = static_cast<int>(threadIdx.x);
+    const int cg = t % groups, r0 = t / groups, step = 256 / groups;
+    const int m0 = static_cast<int>(blockIdx.x) * kColRows;
+    const int m1 = min(M, m0 + kColRows);
+    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int m = m0 + r0; m < m1; m += step) {
+        const uint4 v = *reinterpret_cast<const uint4*>(g + static_cast<size_t>(m) * ld + cg * 8);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            acc[2 * j] += __uint_as_float(w[j] << 16);
+            acc[2 * j + 1] += __uint_as_float(w[j] & 0xFFFF0000u);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) red[r0 * ld + cg * 8 + j] = acc[j];
     __syncthreads();
-    if (r == 0 && n < N) db[n] = (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
+    if (t < ld) {
+        float s = 0.0f;
+        for (int r = 0; r < step; r++) s += red[r * ld + t];
+        part[static_cast<size_t>(blockIdx.x) * ld + t] = s;
+    }
+}
+
+// second stage: db[n] = sum over z < blocks of part[z * ld + n], n < N
+__global__ void column_finish_kernel(const float* __restrict__ part, const int blocks, const int ld, const int N,
+                                     float* __restrict__ db) {
+    const int n = static_cast<int>(blockIdx.x) * blockDim.x + static_cast<int>(threadIdx.x);
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int z = 0; z < blocks; z++) s += part[static_cast<size_t>(z) * ld + n];
+    db[n] = s;
 }
 
 __global__ void pack_features_kernel(const float* __restrict__ f, const int64_t B, const int64_t B_pad,
@@ -277,6 +305,17 @@ __device__ __forceinline__ void photon_features(const mlgpu_surrogate_domain& do
     f[4] = static_cast<float>(2.0 * (il - ilo) / (ihi - ilo) - 1.0);
 }
 
+// nominal image point image_scale (dx / dz, dy / dz) of a photon from its
+// features (the direction components are exact to float precision there)
+__device__ __forceinline__ void image_point(const mlgpu_surrogate_domain& dom, const float* f, double& px,
+                                            double& py) {
+    const double sa = sin(dom.max_angle);
+    const double dx = static_cast<double>(f[2]) * sa, dy = static_cast<double>(f[3]) * sa;
+    const double dz = -sqrt(fmax(1.0 - dx * dx - dy * dy, 1e-12));
+    px = dom.image_scale * (dx / dz);
+    py = dom.image_scale * (dy / dz);
+}
+
 __global__ void prepare_kernel(const mlgpu_surrogate_domain dom, const mlgpu_photons ph, const mlgpu_results r,
                                const bool with_targets, const double* __restrict__ poses, const int64_t n,
                                float* __restrict__ features, float* __restrict__ targets) {
@@ -289,10 +328,12 @@ __global__ void prepare_kernel(const mlgpu_surrogate_domain dom, const mlgpu_pho
     const bool det = r.sensor_id[i] >= 0 && r.error[i] == 0 && fr >= 0;
     t[0] = det ? 1.0f : 0.0f;
     if (det) {
+        // the sensor's centre: the surrogate predicts which sensor is hit
         const double* T = poses + 12 * static_cast<int64_t>(fr);
-        const double x = r.x[i], y = r.y[i];
-        t[1] = static_cast<float>((T[0] * x + T[1] * y + T[3]) / dom.sensor_radius);
-        t[2] = static_cast<float>((T[4] * x + T[5] * y + T[7]) / dom.sensor_radius);
+        double px, py;
+        image_point(dom, features + i * kIn, px, py);
+        t[1] = static_cast<float>((T[3] - px) / dom.position_scale);
+        t[2] = static_cast<float>((T[7] - py) / dom.position_scale);
         t[3] = static_cast<float>((r.time_of_flight[i] - dom.time_offset) / dom.time_scale);
     } else {
         t[1] = t[2] = t[3] = 0.0f;
@@ -314,6 +355,8 @@ __device__ int32_t hex_cell(const mlgpu_hex_cells& c, const double x, const doub
     if (ii < -c.radius || ii > c.radius || jj < -c.radius || jj > c.radius) return -1;
     return c.cell[(ii + c.radius) * (2 * c.radius + 1) + (jj + c.radius)];
 }
+
+constexpr int kDrawTries = 4;
 
 // The surrogate's draw for photon i of the call (features f, outputs o):
 // false when it is binned, true when it must fall back to propagation.
@@ -341,34 +384,50 @@ __device__ bool sample_one(const SampleArgs& a, const float* f, const float* o, 
     philox4x32_10(0u, 3u, id_lo, id_hi, k_lo, k_hi, w[0], w[1], w[2], w[3]);
     const double survive = 1.0 / (1.0 + exp(-static_cast<double>(o[0])));
     if (!(u53(w[0], w[1]) < survive)) return false;
-    // component by the cumulative weights
-    const double uc = u53(w[2], w[3]);
-    int k = kComp - 1;
-    double acc = 0.0;
-    for (int q = 0; q < kComp; q++) {
-        acc += exp(static_cast<double>(o[1 + 7 * q] - lse));
-        if (uc < acc) { k = q; break; }
+    // a detected photon lands on a sensor: position and time are drawn from
+    // the mixture until the position falls on a cell, at most kDrawTries
+    // times; a photon whose draws all miss the cells falls back
+    for (int attempt = 0; attempt < kDrawTries; attempt++) {
+        // component by the cumulative weights
+        double uc;
+        if (attempt == 0) {
+            uc = u53(w[2], w[3]);
+        } else {
+            philox4x32_10(9u + static_cast<uint32_t>(attempt), 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
+            uc = u53(v[0], v[1]);
+        }
+        int k = kComp - 1;
+        double acc = 0.0;
+        for (int q = 0; q < kComp; q++) {
+            acc += exp(static_cast<double>(o[1 + 7 * q] - lse));
+            if (uc < acc) { k = q; break; }
+        }
+        const uint32_t b0 = 1u + 2u * static_cast<uint32_t>(attempt);
+        philox4x32_10(b0, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
+        const double r1 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
+        double s1, c1;
+        sincos_turn(u53(v[2], v[3]), s1, c1);
+        philox4x32_10(b0 + 1u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
+        const double r2 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
+        double s2, c2;
+        sincos_turn(u53(v[2], v[3]), s2, c2);
+        const double z[3] = {r1 * c1, r1 * s1, r2 * c2};
+        double y[3];
+        for (int j = 0; j < 3; j++) {
+            const float ls = fminf(fmaxf(o[5 + 7 * k + j], kLogSigmaMin), kLogSigmaMax);
+            y[j] = static_cast<double>(o[2 + 7 * k + j]) + exp(static_cast<double>(ls)) * z[j];
+        }
+        double px, py;
+        image_point(a.dom, f, px, py);
+        const double x = px + y[0] * a.dom.position_scale, yy = py + y[1] * a.dom.position_scale;
+        const int32_t cell = hex_cell(a.cells, x, yy);
+        if (cell < 0) continue;
+        const double t = y[2] * a.dom.time_scale + a.dom.time_offset;
+        const int64_t slot = arrival_cell(a.hist, cell, 0, t, a.hist.photon_id_offset + static_cast<uint64_t>(i));
+        if (slot >= 0) atomicAdd(&a.hist.counts[slot], 1u);
+        return false;
     }
-    philox4x32_10(1u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
-    const double r1 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
-    double s1, c1;
-    sincos_turn(u53(v[2], v[3]), s1, c1);
-    philox4x32_10(2u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
-    const double r2 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
-    double s2, c2;
-    sincos_turn(u53(v[2], v[3]), s2, c2);
-    const double z[3] = {r1 * c1, r1 * s1, r2 * c2};
-    double y[3];
-    for (int j = 0; j < 3; j++) {
-        const float ls = fminf(fmaxf(o[5 + 7 * k + j], kLogSigmaMin), kLogSigmaMax);
-        y[j] = static_cast<double>(o[2 + 7 * k + j]) + exp(static_cast<double>(ls)) * z[j];
-    }
-    const double x = y[0] * a.dom.sensor_radius, yy = y[1] * a.dom.sensor_radius;
-    const double t = y[2] * a.dom.time_scale + a.dom.time_offset;
-    const int32_t cell = hex_cell(a.cells, x, yy);
-    const int64_t slot = arrival_cell(a.hist, cell, 0, t, a.hist.photon_id_offset + static_cast<uint64_t>(i));
-    if (slot >= 0) atomicAdd(&a.hist.counts[slot], 1u);
-    return false;
+    return true;
 }
 
 // wave-aggregated append of the fallback photons (every lane of the wave calls it)
@@ -598,9 +657,13 @@ hipError_t launch_sum_slices(const float* src, int64_t count, int splits, float*
     return hipGetLastError();
 }
 
-hipError_t launch_column_sums(const uint16_t* g, int M, int N, int ld, float* db, hipStream_t stream) {
-    hipLaunchKernelGGL(column_sums_kernel, dim3(static_cast<uint32_t>((N + 63) / 64)), dim3(256), 0, stream, g, M, N,
-                       ld, db);
+hipError_t launch_column_sums(const uint16_t* g, int M, int N, int ld, float* part, float* db, hipStream_t stream) {
+    if (ld % 8 != 0 || 2048 % ld != 0 || N > ld) return hipErrorInvalidValue;
+    const int blocks = (M + kColRows - 1) / kColRows;
+    hipLaunchKernelGGL(column_partials_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream, g, M, ld,
+                       part);
+    hipLaunchKernelGGL(column_finish_kernel, dim3(static_cast<uint32_t>((N + 255) / 256)), dim3(256), 0, stream, part,
+                       blocks, ld, N, db);
     return hipGetLastError();
 }
 

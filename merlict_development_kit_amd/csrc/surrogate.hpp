@@ -68,8 +68,9 @@ hipError_t launch_gemm(bool a_kcontig, bool b_kcontig, int epilogue, const uint1
 // dst[i] = sum over z < splits of src[z * count + i]
 hipError_t launch_sum_slices(const float* src, int64_t count, int splits, float* dst, hipStream_t stream);
 
-// bias gradient: db[n] = sum_m g(m, n) of a bf16 M x ld matrix, n < N
-hipError_t launch_column_sums(const uint16_t* g, int M, int N, int ld, float* db, hipStream_t stream);
+// bias gradient: db[n] = sum_m g(m, n) of a bf16 M x ld matrix, n < N;
+// part holds ceil(M / 256) * ld floats of per-block partial sums
+hipError_t launch_column_sums(const uint16_t* g, int M, int N, int ld, float* part, float* db, hipStream_t stream);
 
 // Features (B x kInPad bf16, zero padded) from fp32 B x kIn.
 hipError_t launch_pack_features(const float* f, int64_t B, int64_t B_pad, uint16_t* x, hipStream_t stream);

@@ -130,7 +130,8 @@ hipError_t weight_gradient(mlgpu_surrogate* s, int l, const uint16_t* g, int ldg
     e = hipMemcpy2DAsync(s->grads + weight_offset(l), row_bytes, s->partial, static_cast<size_t>(kPadIn[l]) * 4,
                          row_bytes, static_cast<size_t>(kLayerOut[l]), hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return e;
-    return launch_column_sums(g, rows, kLayerOut[l], ldg, s->grads + bias_offset(l), st);
+    // the slices are free again once the copy above has run (same stream)
+    return launch_column_sums(g, rows, kLayerOut[l], ldg, s->partial, s->grads + bias_offset(l), st);
 }
 
 // loss and gradients of the batch (features already packed in x0)
@@ -198,8 +199,9 @@ int ensure_hybrid(mlgpu_surrogate* s) {
 
 bool domain_ok(const mlgpu_surrogate_domain* d) {
     return d && d->support_radius > 0.0 && d->max_angle > 0.0 && d->max_angle < 1.5707963267948966 &&
-           d->lambda_min > 0.0 && d->lambda_max > d->lambda_min && d->sensor_radius > 0.0 && d->time_scale > 0.0 &&
-           std::isfinite(d->support_radius) && std::isfinite(d->lambda_max) && std::isfinite(d->sensor_radius) &&
+           d->lambda_min > 0.0 && d->lambda_max > d->lambda_min && d->position_scale > 0.0 && d->time_scale > 0.0 &&
+           std::isfinite(d->image_scale) && std::isfinite(d->support_radius) && std::isfinite(d->lambda_max) &&
+           std::isfinite(d->position_scale) &&
            std::isfinite(d->time_offset) && std::isfinite(d->time_scale);
 }
 

@@ -114,13 +114,14 @@ class SurrogateDomain:
     max_angle: float
     lambda_min: float
     lambda_max: float
-    sensor_radius: float
+    image_scale: float = 0.0
+    position_scale: float = 0.01
     time_offset: float = 0.0
     time_scale: float = 1e-9
 
     def c_struct(self) -> N.SurrogateDomain:
         return N.SurrogateDomain(self.support_radius, self.max_angle, self.lambda_min, self.lambda_max,
-                                 self.sensor_radius, self.time_offset, self.time_scale)
+                                 self.image_scale, self.position_scale, self.time_offset, self.time_scale)
 
     def source(self, height: float) -> S.Source:
         """Photons spanning the domain: supports uniform on the disc at z =
@@ -133,9 +134,10 @@ class SurrogateDomain:
 def fact_domain() -> SurrogateDomain:
     """The FACT-like telescope's domain: supports within 1.05 x the 2.1 m
     mirror radius (fact_propagation.cpp:295), directions within 1.2 x the
-    camera's 2.2 deg field radius, lambda in [280, 700] nm; sensor plane
-    normalised by 0.2 m.  The time offset is set by :func:`train`."""
-    return SurrogateDomain(1.05 * 2.1, math.radians(1.2 * 2.2), 280e-9, 700e-9, 0.2)
+    camera's 2.2 deg field radius, lambda in [280, 700] nm; positions
+    relative to the nominal image point in units of 1 cm.  The image scale
+    and the time units are set from data by :func:`train`."""
+    return SurrogateDomain(1.05 * 2.1, math.radians(1.2 * 2.2), 280e-9, 700e-9)
 
 
 class HexCells:
@@ -210,12 +212,22 @@ def train(scene: GpuScene, desc: SceneDescription, surrogate: Surrogate, domain:
     poses = torch.tensor(desc.frame_poses().reshape(-1), dtype=torch.float64, device=device)
     src = domain.source(height)
     cfg = PropagationConfig(max_interactions, seed=seed)
-    if domain.time_offset == 0.0:
+    if domain.time_offset == 0.0 or domain.image_scale == 0.0:
+        # time units and the image scale from a first sample: the least-squares
+        # slope of the sensor-plane position against (dx / dz, dy / dz)
         ph = src.generate(1 << 18, seed=seed ^ 0x5EED, device=device)
         res = scene.propagate(ph, cfg)
         det = res.sensor_id >= 0
         tof = res.time_of_flight[det]
-        domain = replace(domain, time_offset=float(tof.mean()), time_scale=float(max(tof.std().item(), 1e-12)))
+        f, t = prepare(replace(domain, image_scale=0.0, position_scale=1.0), ph, res, poses)
+        sa = math.sin(domain.max_angle)
+        d = f[det][:, 2:4].double() * sa
+        dz = -torch.sqrt(torch.clamp(1.0 - (d * d).sum(dim=1), min=1e-12))
+        u = (d / dz[:, None]).reshape(-1)
+        p = t[det][:, 1:3].double().reshape(-1)
+        scale = float((u * p).sum() / (u * u).sum())
+        domain = replace(domain, image_scale=scale, time_offset=float(tof.mean()),
+                         time_scale=float(max(tof.std().item(), 1e-12)))
     steps = max(1, n_photons // batch)
     step = 0
     history = []
