@@ -123,8 +123,13 @@ int mlgpu_build_lens_scene(double focal_length, double aperture_radius,
                            mlgpu_desc** out);
 /* FACT-like telescope: segmented_imaging_reflector (30 facets, f = 4.889 m),
  * 1440 hexagonal pixels of 9.5 mm pitch at the focal plane and a shield disc;
- * with_light_guides != 0 adds a hexagonal PMMA light guide above each pixel. */
-int mlgpu_build_fact_scene(int with_light_guides, mlgpu_desc** out);
+ * camera 1 adds a hexagonal PMMA light guide (15 mm) in front of each pixel;
+ * camera 0 has bare pixels; camera 2 replaces the camera by one absorbing
+ * sensor disc (sensor 0, radius 0.33 m) at the guides' entrance plane, 15 mm
+ * in front of the pixels: the same dish and shield, so a photon reaches the
+ * entrance plane there exactly as it does in the camera-1 scene under the
+ * same draws (the surrogate's focal-plane targets). */
+int mlgpu_build_fact_scene(int camera, mlgpu_desc** out);
 /* Plenoscope: segmented reflector (f = 106.5 m, 71 m aperture) and the
  * light-field sensor of light_field_sensor/Factory.cpp (8,443 lenslets,
  * 515,023 lixels for num_paxel_on_pixel_diagonal = 9). */
@@ -380,10 +385,10 @@ int mlgpu_surrogate_train_step(mlgpu_surrogate* s, const float* features,
  * sin(max_angle), and 2 (1/lambda - 1/lambda_max) / (1/lambda_min -
  * 1/lambda_max) - 1.  In the domain: support within r of the axis,
  * direction within max_angle of -z, lambda in [lambda_min, lambda_max].
- * Targets: the sensor-plane position (the world centre of the sensor frame
- * the photon ends on) relative to the nominal image point image_scale
- * (dx / dz, dy / dz) of the photon's direction, over position_scale, and
- * (time_of_flight - time_offset) / time_scale. */
+ * Targets: the sensor-plane position (mlgpu_surrogate_prepare) relative
+ * to the nominal image point image_scale (dx / dz, dy / dz) of the
+ * photon's direction, over position_scale, and (time_of_flight -
+ * time_offset) / time_scale. */
 typedef struct mlgpu_surrogate_domain {
     double support_radius;
     double max_angle;
@@ -397,11 +402,19 @@ typedef struct mlgpu_surrogate_domain {
 
 /* Features (n x 5) of photons and, when results is not NULL, targets
  * (n x 4: detected, x, y, t) of their propagation; frame_poses (device) is
- * mlgpu_desc_frame_poses of the scene.  All device pointers. */
+ * mlgpu_desc_frame_poses of the scene.  The position target of a detected
+ * photon is where it crosses the focal plane: the world position of its
+ * result in focal_results, the same photons propagated under the same
+ * config through a scene whose sensor is that plane (focal_poses: that
+ * scene's frame poses), when that result is a detection; else, or when
+ * focal_results is NULL, the world centre of the sensor frame it ends on.
+ * All device pointers. */
 int mlgpu_surrogate_prepare(const mlgpu_surrogate_domain* domain,
                             const mlgpu_photons* photons,
                             const mlgpu_results* results,
-                            const double* frame_poses, int64_t n,
+                            const double* frame_poses,
+                            const mlgpu_results* focal_results,
+                            const double* focal_poses, int64_t n,
                             float* features, float* targets, void* stream);
 
 /* Sensor cells on a hexagonal lattice in the sensor plane: the cell whose
@@ -415,14 +428,25 @@ typedef struct mlgpu_hex_cells {
     const int32_t* cell;
 } mlgpu_hex_cells;
 
+/* The sensor cells of the surrogate's draws (mlgpu_propagate_hybrid: a
+ * drawn position off the cells is a photon that misses the sensors).  The
+ * model's probability of a detection on cell c is sigmoid(s) times the
+ * mixture's mass on c.  While a region is set, the training loss of a
+ * photon that was not detected is -log(1 - sigmoid(s) Z), Z the mixture's
+ * mass on all cells, from a 32-point Gauss-Laguerre x angular rule per
+ * component (rotated per photon and step); without a region it is
+ * -log(1 - sigmoid(s)).  Domain and cells are copied; cells->cell (device)
+ * must stay valid while set.  NULL cells clears the region. */
+int mlgpu_surrogate_set_region(mlgpu_surrogate* s, const mlgpu_surrogate_domain* domain,
+                               const mlgpu_hex_cells* cells);
+
 /* The surrogate + fallback hybrid: every photon in the domain whose
  * mixture entropy bound (entropy of the weights plus their mean component
  * entropy, nats, normalised units) is at most max_entropy is drawn from the
  * surrogate (survival, component, position, time from Philox stream 3 of
  * (seed, config photon_id_offset + i)) and binned into hist at the cell
- * its position falls in (a detected photon's position is drawn again, up to
- * four times, until it falls on a cell); every other photon is propagated
- * by the plain propagator and binned from its result.  *num_fallback (host, may be
+ * its position falls in, if any; every other photon is propagated by the
+ * plain propagator and binned from its result.  *num_fallback (host, may be
  * NULL) receives the number of propagated photons.  Synchronises the
  * stream once per chunk of 2^22 photons. */
 typedef struct mlgpu_hybrid_config {

@@ -196,8 +196,10 @@ SIGNATURES = {
                                             C.c_void_p]),
     "mlgpu_surrogate_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p,
                                              C.c_void_p]),
+    "mlgpu_surrogate_set_region": (C.c_int, [C.c_void_p, C.POINTER(SurrogateDomain), C.POINTER(HexCells)]),
     "mlgpu_surrogate_prepare": (C.c_int, [C.POINTER(SurrogateDomain), C.POINTER(Photons), C.POINTER(Results),
-                                          C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+                                          C.c_void_p, C.POINTER(Results), C.c_void_p, C.c_int64, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]),
     "mlgpu_propagate_hybrid": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SurrogateDomain), C.POINTER(HexCells),
                                          C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
                                          C.POINTER(HybridConfig), C.POINTER(Histogram), C.POINTER(C.c_int64),

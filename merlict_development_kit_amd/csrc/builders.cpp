@@ -317,8 +317,9 @@ int mlgpu_build_lens_scene(double focal_length, double aperture_radius, double r
     }
 }
 
-int mlgpu_build_fact_scene(int with_light_guides, mlgpu_desc** out) {
-    if (!out) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+int mlgpu_build_fact_scene(int camera, mlgpu_desc** out) {
+    if (!out || camera < 0 || camera > 2) { set_last_error("null argument or bad camera"); return MLGPU_ERR_ARGUMENT; }
+    const bool with_light_guides = camera == 1;
     try {
         auto* d = new mlgpu_desc();
         const int32_t root = d->add_frame(-1, v3(0, 0, 0), Rot::xyz(0, 0, 0), MLGPU_KIND_FRAME);
@@ -359,6 +360,13 @@ int mlgpu_build_fact_scene(int with_light_guides, mlgpu_desc** out) {
         });
         const double pixel_r = 0.00495;
         const double guide_h = 0.015;
+        if (camera == 2) {
+            d->add_frame(cam, v3(0, 0, -guide_h), Rot::xyz(0, 0, 0), MLGPU_KIND_DISC, {0.33}, Surface(),
+                         MLGPU_NO_FRAME, 0);
+            validate_desc(d->view());
+            *out = d;
+            return MLGPU_OK;
+        }
         const double entry_r = pitch / std::sqrt(3.0) * 0.995;
         int32_t n_index = -1, n_abs = -1;
         if (with_light_guides) {

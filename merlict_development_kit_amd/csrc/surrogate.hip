@@ -159,9 +159,80 @@ __global__ void pack_features_kernel(const float* __restrict__ f, const int64_t 
 // log-sigma range of the mixture components (normalised units)
 constexpr float kLogSigmaMin = -9.0f, kLogSigmaMax = 6.0f;
 
+__device__ int32_t hex_cell(const mlgpu_hex_cells& c, double x, double y);
+__device__ __forceinline__ void image_point(const mlgpu_surrogate_domain& dom, const float* f, double& px,
+                                            double& py);
+
+// Gauss-Laguerre nodes and weights of order 4 (Abramowitz & Stegun,
+// table 25.9): with u = r^2 / 2, E[g(eps)] of a 2-d standard normal eps is
+// sum_j w_j mean over kRuleAngles equally spaced angles of g(sqrt(2 u_j) e_theta).
+// The rule is exact for 1, eps and eps^2, so a component entirely on the
+// cells gets Z = 1 and no gradient from the truncation.
+constexpr int kRuleRadii = 4, kRuleAngles = 8;
+__constant__ float kRuleR[kRuleRadii] = {0.803178300f, 1.868561530f, 3.012182031f, 4.334759717f};  // sqrt(2 u_j)
+__constant__ float kRuleW[kRuleRadii] = {0.603154104341634f, 0.357418692437800f, 0.0388879085150054f,
+                                         0.000539294705561327f};
+
+// Mass Z of the mixture's position marginal on the cells and its gradient
+// wrt the outputs o (weight logits, means and log sigmas of x and y) into
+// dz (zeroed here); pi are the mixture weights.
+__device__ float region_mass(const LossRegion& reg, const float* f, const float* o, const float* pi,
+                             const uint32_t rot, float* dz) {
+    double px, py;
+    image_point(reg.dom, f, px, py);
+    const float phi = static_cast<float>(rot >> 8) * (6.2831853071795865f / kRuleAngles / 16777216.0f);
+    float c0, s0;
+    __sincosf(phi, &s0, &c0);
+    constexpr float kHalfSqrt2 = 0.70710678118654752f;
+    float zk[kComp], sm[kComp][2], sl[kComp][2];
+    float Z = 0.0f;
+    for (int k = 0; k < kComp; k++) {
+        const float mx = o[2 + 7 * k], my = o[3 + 7 * k];
+        const float sx = expf(fminf(fmaxf(o[5 + 7 * k], kLogSigmaMin), kLogSigmaMax));
+        const float sy = expf(fminf(fmaxf(o[6 + 7 * k], kLogSigmaMin), kLogSigmaMax));
+        float z = 0.0f, a0 = 0.0f, a1 = 0.0f, b0 = 0.0f, b1 = 0.0f;
+        for (int j = 0; j < kRuleRadii; j++) {
+            const float r = kRuleR[j], w = kRuleW[j] * (1.0f / kRuleAngles);
+            float c = c0, sn = s0;
+            for (int q = 0; q < kRuleAngles; q++) {
+                const float ex = r * c, ey = r * sn;
+                const double x = px + static_cast<double>(mx + sx * ex) * reg.dom.position_scale;
+                const double y = py + static_cast<double>(my + sy * ey) * reg.dom.position_scale;
+                if (hex_cell(reg.cells, x, y) >= 0) {
+                    z += w;
+                    a0 += w * ex;
+                    a1 += w * ey;
+                    b0 += w * (ex * ex - 1.0f);
+                    b1 += w * (ey * ey - 1.0f);
+                }
+                const float cn = (c - sn) * kHalfSqrt2;  // rotate by 2 pi / 8
+                sn = (c + sn) * kHalfSqrt2;
+                c = cn;
+            }
+        }
+        zk[k] = z;
+        sm[k][0] = a0 / sx;
+        sm[k][1] = a1 / sy;
+        sl[k][0] = b0;
+        sl[k][1] = b1;
+        Z += pi[k] * z;
+    }
+    for (int j = 0; j < kOutPad; j++) dz[j] = 0.0f;
+    for (int k = 0; k < kComp; k++) {
+        dz[1 + 7 * k] = pi[k] * (zk[k] - Z);
+        for (int d = 0; d < 2; d++) {
+            dz[2 + 7 * k + d] = pi[k] * sm[k][d];
+            const float raw = o[5 + 7 * k + d];
+            if (raw >= kLogSigmaMin && raw <= kLogSigmaMax) dz[5 + 7 * k + d] = pi[k] * sl[k][d];
+        }
+    }
+    return Z;
+}
+
 __global__ __launch_bounds__(256) void mixture_loss_kernel(const float* __restrict__ out,
                                                            const float* __restrict__ targets, const int64_t B,
-                                                           const int64_t B_pad, uint16_t* __restrict__ grad,
+                                                           const int64_t B_pad, const LossRegion reg,
+                                                           uint16_t* __restrict__ grad,
                                                            float* __restrict__ loss_sum) {
     __shared__ float part[256];
     const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -211,6 +282,26 @@ __global__ __launch_bounds__(256) void mixture_loss_kernel(const float* __restri
                         g[5 + 7 * k + j] = (raw >= kLogSigmaMin && raw <= kLogSigmaMax) ? r * (1.0f - z * z) : 0.0f;
                     }
                 }
+            } else if (reg.on) {
+                // lost photon: -log(1 - sigmoid(s) Z), Z the mixture's mass on the cells
+                float wmax = -INFINITY;
+                for (int k = 0; k < kComp; k++) wmax = fmaxf(wmax, o[1 + 7 * k]);
+                float wsum = 0.0f;
+                for (int k = 0; k < kComp; k++) wsum += expf(o[1 + 7 * k] - wmax);
+                float pis[kComp];
+                for (int k = 0; k < kComp; k++) pis[k] = expf(o[1 + 7 * k] - wmax) / wsum;
+                uint32_t h = static_cast<uint32_t>(b) * 0x9E3779B1u ^ reg.rotation_key;
+                h ^= h >> 15;
+                h *= 0x2C1B3C6Du;
+                h ^= h >> 12;
+                float dz[kOutPad];
+                const float Z = fminf(region_mass(reg, reg.features + b * kIn, o, pis, h, dz), 1.0f);
+                const float sg = 1.0f / (1.0f + expf(-s)), om = 1.0f / (1.0f + expf(s));
+                const float val = fmaxf(om + sg * (1.0f - Z), 1e-30f);
+                loss = -logf(val);
+                g[0] = sg * om * Z / val;
+                const float gz = -sg / val;
+                for (int j = 1; j < kOutPad; j++) g[j] = gz * dz[j];
             }
         }
         const float scale = 1.0f / static_cast<float>(B);
@@ -316,24 +407,39 @@ __device__ __forceinline__ void image_point(const mlgpu_surrogate_domain& dom, c
     py = dom.image_scale * (dy / dz);
 }
 
+// Targets of photon i: detected (r), the sensor-plane position and the
+// arrival time.  The position is where the photon crosses the focal plane
+// (the world position of its result in the focal-plane scene, fr) when
+// that result is a detection, else the centre of the sensor it ends on.
 __global__ void prepare_kernel(const mlgpu_surrogate_domain dom, const mlgpu_photons ph, const mlgpu_results r,
-                               const bool with_targets, const double* __restrict__ poses, const int64_t n,
+                               const bool with_targets, const double* __restrict__ poses, const mlgpu_results fr,
+                               const bool with_focal, const double* __restrict__ focal_poses, const int64_t n,
                                float* __restrict__ features, float* __restrict__ targets) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     photon_features(dom, ph, i, features + i * kIn);
     if (!with_targets) return;
     float* t = targets + i * 4;
-    const int32_t fr = r.final_frame[i];
-    const bool det = r.sensor_id[i] >= 0 && r.error[i] == 0 && fr >= 0;
+    const int32_t frame = r.final_frame[i];
+    const bool det = r.sensor_id[i] >= 0 && r.error[i] == 0 && frame >= 0;
     t[0] = det ? 1.0f : 0.0f;
     if (det) {
-        // the sensor's centre: the surrogate predicts which sensor is hit
-        const double* T = poses + 12 * static_cast<int64_t>(fr);
+        double wx, wy;
+        const int32_t ff = with_focal ? fr.final_frame[i] : -1;
+        if (with_focal && fr.sensor_id[i] >= 0 && fr.error[i] == 0 && ff >= 0) {
+            const double* T = focal_poses + 12 * static_cast<int64_t>(ff);
+            const double x = fr.x[i], y = fr.y[i];
+            wx = T[0] * x + T[1] * y + T[3];
+            wy = T[4] * x + T[5] * y + T[7];
+        } else {
+            const double* T = poses + 12 * static_cast<int64_t>(frame);
+            wx = T[3];
+            wy = T[7];
+        }
         double px, py;
         image_point(dom, features + i * kIn, px, py);
-        t[1] = static_cast<float>((T[3] - px) / dom.position_scale);
-        t[2] = static_cast<float>((T[7] - py) / dom.position_scale);
+        t[1] = static_cast<float>((wx - px) / dom.position_scale);
+        t[2] = static_cast<float>((wy - py) / dom.position_scale);
         t[3] = static_cast<float>((r.time_of_flight[i] - dom.time_offset) / dom.time_scale);
     } else {
         t[1] = t[2] = t[3] = 0.0f;
@@ -356,10 +462,9 @@ __device__ int32_t hex_cell(const mlgpu_hex_cells& c, const double x, const doub
     return c.cell[(ii + c.radius) * (2 * c.radius + 1) + (jj + c.radius)];
 }
 
-constexpr int kDrawTries = 4;
-
 // The surrogate's draw for photon i of the call (features f, outputs o):
-// false when it is binned, true when it must fall back to propagation.
+// false when it is drawn (and binned if it lands on a cell), true when it
+// must fall back to propagation.
 __device__ bool sample_one(const SampleArgs& a, const float* f, const float* o, const int64_t i) {
     const bool in_domain = f[0] * f[0] + f[1] * f[1] <= 1.0f && f[2] * f[2] + f[3] * f[3] <= 1.0f &&
                            f[4] >= -1.0f && f[4] <= 1.0f;
@@ -384,50 +489,38 @@ __device__ bool sample_one(const SampleArgs& a, const float* f, const float* o, 
     philox4x32_10(0u, 3u, id_lo, id_hi, k_lo, k_hi, w[0], w[1], w[2], w[3]);
     const double survive = 1.0 / (1.0 + exp(-static_cast<double>(o[0])));
     if (!(u53(w[0], w[1]) < survive)) return false;
-    // a detected photon lands on a sensor: position and time are drawn from
-    // the mixture until the position falls on a cell, at most kDrawTries
-    // times; a photon whose draws all miss the cells falls back
-    for (int attempt = 0; attempt < kDrawTries; attempt++) {
-        // component by the cumulative weights
-        double uc;
-        if (attempt == 0) {
-            uc = u53(w[2], w[3]);
-        } else {
-            philox4x32_10(9u + static_cast<uint32_t>(attempt), 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
-            uc = u53(v[0], v[1]);
-        }
-        int k = kComp - 1;
-        double acc = 0.0;
-        for (int q = 0; q < kComp; q++) {
-            acc += exp(static_cast<double>(o[1 + 7 * q] - lse));
-            if (uc < acc) { k = q; break; }
-        }
-        const uint32_t b0 = 1u + 2u * static_cast<uint32_t>(attempt);
-        philox4x32_10(b0, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
-        const double r1 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
-        double s1, c1;
-        sincos_turn(u53(v[2], v[3]), s1, c1);
-        philox4x32_10(b0 + 1u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
-        const double r2 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
-        double s2, c2;
-        sincos_turn(u53(v[2], v[3]), s2, c2);
-        const double z[3] = {r1 * c1, r1 * s1, r2 * c2};
-        double y[3];
-        for (int j = 0; j < 3; j++) {
-            const float ls = fminf(fmaxf(o[5 + 7 * k + j], kLogSigmaMin), kLogSigmaMax);
-            y[j] = static_cast<double>(o[2 + 7 * k + j]) + exp(static_cast<double>(ls)) * z[j];
-        }
-        double px, py;
-        image_point(a.dom, f, px, py);
-        const double x = px + y[0] * a.dom.position_scale, yy = py + y[1] * a.dom.position_scale;
-        const int32_t cell = hex_cell(a.cells, x, yy);
-        if (cell < 0) continue;
-        const double t = y[2] * a.dom.time_scale + a.dom.time_offset;
-        const int64_t slot = arrival_cell(a.hist, cell, 0, t, a.hist.photon_id_offset + static_cast<uint64_t>(i));
-        if (slot >= 0) atomicAdd(&a.hist.counts[slot], 1u);
-        return false;
+    // component by the cumulative weights
+    const double uc = u53(w[2], w[3]);
+    int k = kComp - 1;
+    double acc = 0.0;
+    for (int q = 0; q < kComp; q++) {
+        acc += exp(static_cast<double>(o[1 + 7 * q] - lse));
+        if (uc < acc) { k = q; break; }
     }
-    return true;
+    philox4x32_10(1u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
+    const double r1 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
+    double s1, c1;
+    sincos_turn(u53(v[2], v[3]), s1, c1);
+    philox4x32_10(2u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
+    const double r2 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
+    double s2, c2;
+    sincos_turn(u53(v[2], v[3]), s2, c2);
+    const double z[3] = {r1 * c1, r1 * s1, r2 * c2};
+    double y[3];
+    for (int j = 0; j < 3; j++) {
+        const float ls = fminf(fmaxf(o[5 + 7 * k + j], kLogSigmaMin), kLogSigmaMax);
+        y[j] = static_cast<double>(o[2 + 7 * k + j]) + exp(static_cast<double>(ls)) * z[j];
+    }
+    double px, py;
+    image_point(a.dom, f, px, py);
+    const double x = px + y[0] * a.dom.position_scale, yy = py + y[1] * a.dom.position_scale;
+    // a position off the cells is a photon that misses the sensors
+    const int32_t cell = hex_cell(a.cells, x, yy);
+    if (cell < 0) return false;
+    const double t = y[2] * a.dom.time_scale + a.dom.time_offset;
+    const int64_t slot = arrival_cell(a.hist, cell, 0, t, a.hist.photon_id_offset + static_cast<uint64_t>(i));
+    if (slot >= 0) atomicAdd(&a.hist.counts[slot], 1u);
+    return false;
 }
 
 // wave-aggregated append of the fallback photons (every lane of the wave calls it)
@@ -674,10 +767,10 @@ hipError_t launch_pack_features(const float* f, int64_t B, int64_t B_pad, uint16
     return hipGetLastError();
 }
 
-hipError_t launch_mixture_loss(const float* out, const float* targets, int64_t B, int64_t B_pad, uint16_t* grad,
-                               float* loss_sum, hipStream_t stream) {
+hipError_t launch_mixture_loss(const float* out, const float* targets, int64_t B, int64_t B_pad,
+                               const LossRegion& region, uint16_t* grad, float* loss_sum, hipStream_t stream) {
     hipLaunchKernelGGL(mixture_loss_kernel, dim3(static_cast<uint32_t>((B_pad + 255) / 256)), dim3(256), 0, stream,
-                       out, targets, B, B_pad, grad, loss_sum);
+                       out, targets, B, B_pad, region, grad, loss_sum);
     return hipGetLastError();
 }
 
@@ -693,11 +786,13 @@ hipError_t launch_init_params(float* params, uint64_t seed, hipStream_t stream) 
 }
 
 hipError_t launch_prepare(const mlgpu_surrogate_domain& dom, const mlgpu_photons& ph, const mlgpu_results* r,
-                          const double* poses, int64_t n, float* features, float* targets, hipStream_t stream) {
+                          const double* poses, const mlgpu_results* focal, const double* focal_poses, int64_t n,
+                          float* features, float* targets, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const mlgpu_results none{};
     hipLaunchKernelGGL(prepare_kernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, stream, dom, ph,
-                       r ? *r : none, r != nullptr, poses, n, features, targets);
+                       r ? *r : none, r != nullptr, poses, focal ? *focal : none, focal != nullptr, focal_poses, n,
+                       features, targets);
     return hipGetLastError();
 }
 

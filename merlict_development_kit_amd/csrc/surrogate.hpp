@@ -75,11 +75,20 @@ hipError_t launch_column_sums(const uint16_t* g, int M, int N, int ld, float* pa
 // Features (B x kInPad bf16, zero padded) from fp32 B x kIn.
 hipError_t launch_pack_features(const float* f, int64_t B, int64_t B_pad, uint16_t* x, hipStream_t stream);
 
+// Truncation of the mixture to sensor cells in the loss (mlgpu_surrogate_set_region).
+struct LossRegion {
+    bool on;
+    mlgpu_surrogate_domain dom;
+    mlgpu_hex_cells cells;
+    const float* features;  // B x kIn fp32, for the photons' image points
+    uint32_t rotation_key;  // varies the quadrature's rotation between steps
+};
+
 // Loss of the outputs (B x kOutPad fp32) for targets (B x 4: detected,
 // x, y, t), its gradient as bf16 (B x kOutPad, scaled by 1 / B) and the
 // summed loss (added to *loss_sum).
-hipError_t launch_mixture_loss(const float* out, const float* targets, int64_t B, int64_t B_pad, uint16_t* grad,
-                               float* loss_sum, hipStream_t stream);
+hipError_t launch_mixture_loss(const float* out, const float* targets, int64_t B, int64_t B_pad,
+                               const LossRegion& region, uint16_t* grad, float* loss_sum, hipStream_t stream);
 
 // Adam on the flat fp32 parameters; writes the padded bf16 weight copies.
 struct AdamArgs {
@@ -106,11 +115,13 @@ hipError_t launch_init_params(float* params, uint64_t seed, hipStream_t stream);
 
 // Features and (when r is not null) targets of photons (mlgpu_surrogate_prepare).
 hipError_t launch_prepare(const mlgpu_surrogate_domain& dom, const mlgpu_photons& ph, const mlgpu_results* r,
-                          const double* poses, int64_t n, float* features, float* targets, hipStream_t stream);
+                          const double* poses, const mlgpu_results* focal, const double* focal_poses, int64_t n,
+                          float* features, float* targets, hipStream_t stream);
 
 // Surrogate draws of n photons from their outputs (rows of kOutPad fp32):
-// photons in the domain with a low entropy bound are binned into hist;
-// the others are appended to fallback (count in *fallback_count).
+// photons in the domain with a low entropy bound are drawn and binned into
+// hist when they land on a cell; the others are appended to fallback
+// (count in *fallback_count).
 struct SampleArgs {
     const float* out;
     const float* features;   // n x kIn, for the domain test

@@ -22,7 +22,11 @@ struct mlgpu_surrogate {
     PaddedWeights pw{};
     void* weights_mem = nullptr;
     int64_t step = 0;
-    std::mutex mutex;  // guards the batch workspace and the step counter
+    std::mutex mutex;  // guards the batch workspace, the step counter and the region
+    // truncation region of the training loss (mlgpu_surrogate_set_region)
+    bool region_on = false;
+    mlgpu_surrogate_domain region_dom{};
+    mlgpu_hex_cells region_cells{};
     // batch workspace, rows = padded batch
     void* ws = nullptr;
     int64_t ws_rows = 0;
@@ -135,11 +139,18 @@ hipError_t weight_gradient(mlgpu_surrogate* s, int l, const uint16_t* g, int ldg
 }
 
 // loss and gradients of the batch (features already packed in x0)
-hipError_t backward(mlgpu_surrogate* s, const float* targets, int64_t n, int64_t rows, hipStream_t st) {
+hipError_t backward(mlgpu_surrogate* s, const float* features, const float* targets, int64_t n, int64_t rows,
+                    hipStream_t st) {
     const int M = static_cast<int>(rows);
+    LossRegion reg{};
+    reg.on = s->region_on;
+    reg.dom = s->region_dom;
+    reg.cells = s->region_cells;
+    reg.features = features;
+    reg.rotation_key = static_cast<uint32_t>(static_cast<uint64_t>(s->step) * 0x9E3779B97F4A7C15ull >> 32);
     hipError_t e = hipMemsetAsync(s->loss, 0, sizeof(float), st);
     if (e == hipSuccess) e = forward(s, rows, st);
-    if (e == hipSuccess) e = launch_mixture_loss(s->out, targets, n, rows, s->gout, s->loss, st);
+    if (e == hipSuccess) e = launch_mixture_loss(s->out, targets, n, rows, reg, s->gout, s->loss, st);
     if (e != hipSuccess) return e;
     const uint16_t* g = s->gout;
     int ldg = kOutPad;
@@ -332,7 +343,7 @@ int mlgpu_surrogate_gradients(mlgpu_surrogate* s, const float* features, const f
     const int64_t rows = padded_rows(n);
     if ((rc = ensure_workspace(s, rows))) return rc;
     hipError_t e = launch_pack_features(features, n, rows, s->x0, st);
-    if (e == hipSuccess) e = backward(s, targets, n, rows, st);
+    if (e == hipSuccess) e = backward(s, features, targets, n, rows, st);
     if (e == hipSuccess && grads)
         e = hipMemcpyAsync(grads, s->grads, static_cast<size_t>(kNumParams) * 4, hipMemcpyDeviceToDevice, st);
     if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, s->loss, 4, hipMemcpyDeviceToDevice, st);
@@ -353,7 +364,7 @@ int mlgpu_surrogate_train_step(mlgpu_surrogate* s, const float* features, const 
     const int64_t rows = padded_rows(n);
     if ((rc = ensure_workspace(s, rows))) return rc;
     hipError_t e = launch_pack_features(features, n, rows, s->x0, st);
-    if (e == hipSuccess) e = backward(s, targets, n, rows, st);
+    if (e == hipSuccess) e = backward(s, features, targets, n, rows, st);
     if (e == hipSuccess) {
         s->step++;
         AdamArgs a{};
@@ -375,13 +386,32 @@ int mlgpu_surrogate_train_step(mlgpu_surrogate* s, const float* features, const 
     return MLGPU_OK;
 }
 
+int mlgpu_surrogate_set_region(mlgpu_surrogate* s, const mlgpu_surrogate_domain* dom, const mlgpu_hex_cells* cells) {
+    if (!s) { set_last_error("null surrogate"); return MLGPU_ERR_ARGUMENT; }
+    std::lock_guard<std::mutex> lock(s->mutex);
+    if (!cells) {
+        s->region_on = false;
+        return MLGPU_OK;
+    }
+    if (!domain_ok(dom) || !cells->cell || cells->radius < 0) {
+        set_last_error("bad domain or cells");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    s->region_dom = *dom;
+    s->region_cells = *cells;
+    s->region_on = true;
+    return MLGPU_OK;
+}
+
 int mlgpu_surrogate_prepare(const mlgpu_surrogate_domain* dom, const mlgpu_photons* ph, const mlgpu_results* r,
-                            const double* poses, int64_t n, float* features, float* targets, void* stream) {
-    if (!domain_ok(dom) || !ph || !features || n < 0 || (r && (!targets || !poses))) {
+                            const double* poses, const mlgpu_results* focal, const double* focal_poses, int64_t n,
+                            float* features, float* targets, void* stream) {
+    if (!domain_ok(dom) || !ph || !features || n < 0 || (r && (!targets || !poses)) || (focal && (!r || !focal_poses))) {
         set_last_error("bad domain or null argument");
         return MLGPU_ERR_ARGUMENT;
     }
-    const hipError_t e = launch_prepare(*dom, *ph, r, poses, n, features, targets, static_cast<hipStream_t>(stream));
+    const hipError_t e = launch_prepare(*dom, *ph, r, poses, focal, focal_poses, n, features, targets,
+                                        static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "surrogate prepare");
     return MLGPU_OK;
 }

@@ -39,10 +39,14 @@ class SceneDescription:
         return cls(h)
 
     @classmethod
-    def fact(cls, light_guides: bool = True):
-        """FACT-like telescope: 30-facet Davies-Cotton dish, 1440-pixel camera."""
+    def fact(cls, light_guides: bool = True, focal_plane: bool = False):
+        """FACT-like telescope: 30-facet Davies-Cotton dish, 1440-pixel camera
+        (with PMMA light guides in front of the pixels, or bare pixels).
+        focal_plane: the same dish with one sensor disc at the light guides'
+        entrance plane instead of the camera (mlgpu_build_fact_scene)."""
         h = C.c_void_p()
-        N.check(N.lib().mlgpu_build_fact_scene(1 if light_guides else 0, C.byref(h)), "mlgpu_build_fact_scene")
+        camera = 2 if focal_plane else (1 if light_guides else 0)
+        N.check(N.lib().mlgpu_build_fact_scene(camera, C.byref(h)), "mlgpu_build_fact_scene")
         return cls(h)
 
     @classmethod

@@ -93,6 +93,18 @@ class Surrogate:
                                                    float(learning_rate), _ptr(loss) if loss is not None else None,
                                                    _stream(stream)), "mlgpu_surrogate_train_step")
 
+    def set_region(self, domain: "SurrogateDomain" = None, cells: "HexCells" = None) -> None:
+        """Train with the mixture truncated to `cells`, the cells its draws
+        are restricted to in the hybrid (mlgpu_surrogate_set_region); no
+        arguments clear the region.  The cells' table is kept alive here."""
+        if cells is None:
+            N.check(N.lib().mlgpu_surrogate_set_region(self._h, None, None), "mlgpu_surrogate_set_region")
+            self._region = None
+            return
+        N.check(N.lib().mlgpu_surrogate_set_region(self._h, C.byref(domain.c_struct()), C.byref(cells.c_struct())),
+                "mlgpu_surrogate_set_region")
+        self._region = cells
+
 
 # ------------------------------------------------------------ domain, cells
 import math  # noqa: E402
@@ -185,17 +197,24 @@ class HexCells:
 
 
 def prepare(domain: SurrogateDomain, photons: PhotonBatch, result: PropagationResult = None,
-            poses: torch.Tensor = None, stream=None):
-    """(features, targets) of photons and their propagation results."""
+            poses: torch.Tensor = None, focal: PropagationResult = None, focal_poses: torch.Tensor = None,
+            stream=None):
+    """(features, targets) of photons and their propagation results; with
+    `focal` (the same photons and config through the focal-plane scene,
+    frame poses `focal_poses`) the position targets are the focal-plane
+    crossings (mlgpu_surrogate_prepare)."""
     n = len(photons)
     dev = photons.device
     feats = torch.empty((n, 5), dtype=torch.float32, device=dev)
     targs = torch.empty((n, 4), dtype=torch.float32, device=dev) if result is not None else None
     ps = photons.c_struct()
     rs = result.c_struct() if result is not None else None
+    fs = focal.c_struct() if focal is not None else None
     N.check(N.lib().mlgpu_surrogate_prepare(C.byref(domain.c_struct()), C.byref(ps),
                                             C.byref(rs) if rs is not None else None,
-                                            _ptr(poses) if poses is not None else None, n, _ptr(feats),
+                                            _ptr(poses) if poses is not None else None,
+                                            C.byref(fs) if fs is not None else None,
+                                            _ptr(focal_poses) if focal_poses is not None else None, n, _ptr(feats),
                                             _ptr(targs) if targs is not None else None, _stream(stream)),
             "mlgpu_surrogate_prepare")
     return feats, targs
@@ -203,23 +222,37 @@ def prepare(domain: SurrogateDomain, photons: PhotonBatch, result: PropagationRe
 
 def train(scene: GpuScene, desc: SceneDescription, surrogate: Surrogate, domain: SurrogateDomain, n_photons: int,
           height: float = 10.0, batch: int = 16384, learning_rate: float = 1e-3, final_learning_rate: float = 3e-5,
-          seed: int = 0, chunk: int = 1 << 21, max_interactions: int = 5, device="cuda", log_every: int = 0):
+          seed: int = 0, chunk: int = 1 << 21, max_interactions: int = 5, device="cuda", log_every: int = 0,
+          cells: "HexCells" = None, focal_desc: SceneDescription = None):
     """Train on n_photons drawn from the domain and propagated by the plain
     propagator (Philox seed `seed`, ids 0 .. n_photons - 1), Adam with a
-    cosine learning-rate decay.  Returns the domain (its time offset and
+    cosine learning-rate decay.  With `cells`, lost photons are explained by
+    the mixture's mass off the cells as well (Surrogate.set_region), as the
+    hybrid's draws treat them.  With `focal_desc` (the scene with a sensor at
+    the focal plane, e.g. SceneDescription.fact(focal_plane=True)) the same
+    photons are propagated through it too and the position targets are
+    their focal-plane crossings.  Returns the domain (its time offset and
     scale set from the data when they were unset) and the loss history
     (mean loss per batch, every log_every steps)."""
     poses = torch.tensor(desc.frame_poses().reshape(-1), dtype=torch.float64, device=device)
     src = domain.source(height)
     cfg = PropagationConfig(max_interactions, seed=seed)
+    focal_scene = GpuScene(focal_desc) if focal_desc is not None else None
+    focal_poses = (torch.tensor(focal_desc.frame_poses().reshape(-1), dtype=torch.float64, device=device)
+                   if focal_desc is not None else None)
+
+    def targets(dom, ph, c):
+        res = scene.propagate(ph, c)
+        foc = focal_scene.propagate(ph, c) if focal_scene is not None else None
+        return res, prepare(dom, ph, res, poses, foc, focal_poses)
+
     if domain.time_offset == 0.0 or domain.image_scale == 0.0:
         # time units and the image scale from a first sample: the least-squares
         # slope of the sensor-plane position against (dx / dz, dy / dz)
         ph = src.generate(1 << 18, seed=seed ^ 0x5EED, device=device)
-        res = scene.propagate(ph, cfg)
+        res, (f, t) = targets(replace(domain, image_scale=0.0, position_scale=1.0), ph, cfg)
         det = res.sensor_id >= 0
         tof = res.time_of_flight[det]
-        f, t = prepare(replace(domain, image_scale=0.0, position_scale=1.0), ph, res, poses)
         sa = math.sin(domain.max_angle)
         d = f[det][:, 2:4].double() * sa
         dz = -torch.sqrt(torch.clamp(1.0 - (d * d).sum(dim=1), min=1e-12))
@@ -228,6 +261,8 @@ def train(scene: GpuScene, desc: SceneDescription, surrogate: Surrogate, domain:
         scale = float((u * p).sum() / (u * u).sum())
         domain = replace(domain, image_scale=scale, time_offset=float(tof.mean()),
                          time_scale=float(max(tof.std().item(), 1e-12)))
+    if cells is not None:
+        surrogate.set_region(domain, cells)
     steps = max(1, n_photons // batch)
     step = 0
     history = []
@@ -235,8 +270,7 @@ def train(scene: GpuScene, desc: SceneDescription, surrogate: Surrogate, domain:
     for c0 in range(0, n_photons, chunk):
         m = min(chunk, n_photons - c0)
         ph = src.generate(m, seed=seed, photon_id_offset=c0, device=device)
-        res = scene.propagate(ph, PropagationConfig(max_interactions, seed=seed, photon_id_offset=c0))
-        f, t = prepare(domain, ph, res, poses)
+        _, (f, t) = targets(domain, ph, PropagationConfig(max_interactions, seed=seed, photon_id_offset=c0))
         for b0 in range(0, m - batch + 1, batch):
             lr = final_learning_rate + 0.5 * (learning_rate - final_learning_rate) * (1 + math.cos(math.pi * step / steps))
             surrogate.train_step(f[b0:b0 + batch], t[b0:b0 + batch], lr, loss)
