@@ -380,20 +380,34 @@ __global__ void init_params_kernel(float* __restrict__ params, const uint64_t se
     params[i] = static_cast<float>(std * r * cs);
 }
 
-// the five normalised features of photon i (mlgpu_surrogate_domain)
-__device__ __forceinline__ void photon_features(const mlgpu_surrogate_domain& dom, const mlgpu_photons& ph,
-                                                const int64_t i, float* f) {
-    const double dx = ph.direction_x[i], dy = ph.direction_y[i], dz = ph.direction_z[i];
-    const double dn = sqrt(dx * dx + dy * dy + dz * dz);
-    const double sa = sin(dom.max_angle);
-    const double il = 1.0 / ph.wavelength[i], ilo = 1.0 / dom.lambda_max, ihi = 1.0 / dom.lambda_min;
-    f[0] = static_cast<float>(ph.support_x[i] / dom.support_radius);
-    f[1] = static_cast<float>(ph.support_y[i] / dom.support_radius);
-    f[2] = static_cast<float>(dx / dn / sa);
-    f[3] = static_cast<float>(dy / dn / sa);
+// constants of the feature map of a domain
+struct FeatureConsts {
+    float inv_r, inv_sa, ilo, lam_scale;
+};
+
+__device__ __forceinline__ FeatureConsts feature_consts(const mlgpu_surrogate_domain& dom) {
+    FeatureConsts c;
+    c.inv_r = static_cast<float>(1.0 / dom.support_radius);
+    c.inv_sa = static_cast<float>(1.0 / sin(dom.max_angle));
+    c.ilo = static_cast<float>(1.0 / dom.lambda_max);
+    c.lam_scale = static_cast<float>(2.0 / (1.0 / dom.lambda_min - 1.0 / dom.lambda_max));
+    return c;
+}
+
+// the five normalised features of photon i (mlgpu_surrogate_domain), in
+// float: the network sees them in bf16
+__device__ __forceinline__ void photon_features(const FeatureConsts& c, const mlgpu_photons& ph, const int64_t i,
+                                                float* f) {
+    const float dx = static_cast<float>(ph.direction_x[i]), dy = static_cast<float>(ph.direction_y[i]);
+    const float dz = static_cast<float>(ph.direction_z[i]);
+    const float inv_n = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    f[0] = static_cast<float>(ph.support_x[i]) * c.inv_r;
+    f[1] = static_cast<float>(ph.support_y[i]) * c.inv_r;
+    f[2] = dx * inv_n * c.inv_sa;
+    f[3] = dy * inv_n * c.inv_sa;
     // a direction away from -z is outside the domain
-    if (!(dz < 0.0)) f[2] = 2.0f;
-    f[4] = static_cast<float>(2.0 * (il - ilo) / (ihi - ilo) - 1.0);
+    if (!(dz < 0.0f)) f[2] = 2.0f;
+    f[4] = (1.0f / static_cast<float>(ph.wavelength[i]) - c.ilo) * c.lam_scale - 1.0f;
 }
 
 // nominal image point image_scale (dx / dz, dy / dz) of a photon from its
@@ -417,7 +431,7 @@ __global__ void prepare_kernel(const mlgpu_surrogate_domain dom, const mlgpu_pho
                                float* __restrict__ features, float* __restrict__ targets) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    photon_features(dom, ph, i, features + i * kIn);
+    photon_features(feature_consts(dom), ph, i, features + i * kIn);
     if (!with_targets) return;
     float* t = targets + i * 4;
     const int32_t frame = r.final_frame[i];
@@ -462,62 +476,120 @@ __device__ int32_t hex_cell(const mlgpu_hex_cells& c, const double x, const doub
     return c.cell[(ii + c.radius) * (2 * c.radius + 1) + (jj + c.radius)];
 }
 
+// float form of the draws' constants: the lattice's inverse basis, the
+// image-point and position scales
+struct DrawConsts {
+    float m00, m01, m10, m11;  // inverse of [a1 a2]
+    float ox, oy;              // lattice origin
+    float sin_angle, image_scale, position_scale;
+};
+
+__device__ __forceinline__ DrawConsts draw_consts(const SampleArgs& a) {
+    DrawConsts k;
+    const double det = a.cells.a1[0] * a.cells.a2[1] - a.cells.a1[1] * a.cells.a2[0];
+    k.m00 = static_cast<float>(a.cells.a2[1] / det);
+    k.m01 = static_cast<float>(-a.cells.a2[0] / det);
+    k.m10 = static_cast<float>(-a.cells.a1[1] / det);
+    k.m11 = static_cast<float>(a.cells.a1[0] / det);
+    k.ox = static_cast<float>(a.cells.origin[0]);
+    k.oy = static_cast<float>(a.cells.origin[1]);
+    k.sin_angle = static_cast<float>(sin(a.dom.max_angle));
+    k.image_scale = static_cast<float>(a.dom.image_scale);
+    k.position_scale = static_cast<float>(a.dom.position_scale);
+    return k;
+}
+
+// hex_cell in float (sensor planes of metres, lattice pitches of millimetres)
+__device__ __forceinline__ int32_t hex_cell_f(const mlgpu_hex_cells& c, const DrawConsts& k, const float x,
+                                             const float y) {
+    const float px = x - k.ox, py = y - k.oy;
+    const float fi = k.m00 * px + k.m01 * py;
+    const float fj = k.m10 * px + k.m11 * py;
+    const float fk = -fi - fj;
+    float ri = rintf(fi), rj = rintf(fj), rk = rintf(fk);
+    const float di = fabsf(ri - fi), dj = fabsf(rj - fj), dk = fabsf(rk - fk);
+    if (di > dj && di > dk) ri = -rj - rk;
+    else if (dj > dk) rj = -ri - rk;
+    const int ii = static_cast<int>(ri), jj = static_cast<int>(rj);
+    if (ii < -c.radius || ii > c.radius || jj < -c.radius || jj > c.radius) return -1;
+    return c.cell[(ii + c.radius) * (2 * c.radius + 1) + (jj + c.radius)];
+}
+
+// uniform in (0, 1] from the top 24 bits of a word
+__device__ __forceinline__ float u24_open(const uint32_t w) {
+    return static_cast<float>((w >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+__device__ __forceinline__ float u24(const uint32_t w) { return static_cast<float>(w >> 8) * (1.0f / 16777216.0f); }
+
 // The surrogate's draw for photon i of the call (features f, outputs o):
 // false when it is drawn (and binned if it lands on a cell), true when it
-// must fall back to propagation.
-__device__ bool sample_one(const SampleArgs& a, const float* f, const float* o, const int64_t i) {
+// must fall back to propagation.  Float arithmetic with 24-bit uniforms:
+// Philox stream 3 block 0 gives the survival and component uniforms and
+// the x, y normals (Box-Muller), block 1 the time normal.
+__device__ bool sample_one(const SampleArgs& a, const DrawConsts& k, const float* f, const float* o,
+                           const int64_t i) {
     const bool in_domain = f[0] * f[0] + f[1] * f[1] <= 1.0f && f[2] * f[2] + f[3] * f[3] <= 1.0f &&
                            f[4] >= -1.0f && f[4] <= 1.0f;
-    // entropy bound of the mixture: H(weights) + sum_k pi_k H(component k)
     float wmax = -INFINITY;
-    for (int k = 0; k < kComp; k++) wmax = fmaxf(wmax, o[1 + 7 * k]);
-    float wsum = 0.0f;
-    for (int k = 0; k < kComp; k++) wsum += expf(o[1 + 7 * k] - wmax);
-    const float lse = wmax + logf(wsum);
-    float h = 0.0f;
-    for (int k = 0; k < kComp; k++) {
-        const float lp = o[1 + 7 * k] - lse, p = expf(lp);
-        float hk = 3.0f * 1.4189385332046727f;  // 0.5 log(2 pi e) per dimension
-        for (int j = 0; j < 3; j++) hk += fminf(fmaxf(o[5 + 7 * k + j], kLogSigmaMin), kLogSigmaMax);
-        h += p * (hk - lp);
+    for (int q = 0; q < kComp; q++) wmax = fmaxf(wmax, o[1 + 7 * q]);
+    float ew[kComp], wsum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kComp; q++) {
+        ew[q] = __expf(o[1 + 7 * q] - wmax);
+        wsum += ew[q];
     }
-    if (!in_domain || !(h <= static_cast<float>(a.max_entropy))) return true;
+    if (a.max_entropy < static_cast<double>(INFINITY)) {
+        // entropy bound of the mixture: H(weights) + sum_k pi_k H(component k)
+        const float lse = wmax + __logf(wsum);
+        float h = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kComp; q++) {
+            const float lp = o[1 + 7 * q] - lse, p = ew[q] / wsum;
+            float hk = 3.0f * 1.4189385332046727f;  // 0.5 log(2 pi e) per dimension
+            for (int j = 0; j < 3; j++) hk += fminf(fmaxf(o[5 + 7 * q + j], kLogSigmaMin), kLogSigmaMax);
+            h += p * (hk - lp);
+        }
+        if (!(h <= static_cast<float>(a.max_entropy))) return true;
+    }
+    if (!in_domain) return true;
     const uint64_t id = a.id_offset + static_cast<uint64_t>(i);
     const uint32_t id_lo = static_cast<uint32_t>(id), id_hi = static_cast<uint32_t>(id >> 32);
     const uint32_t k_lo = static_cast<uint32_t>(a.seed), k_hi = static_cast<uint32_t>(a.seed >> 32);
-    uint32_t w[4], v[4];
+    uint32_t w[4];
     philox4x32_10(0u, 3u, id_lo, id_hi, k_lo, k_hi, w[0], w[1], w[2], w[3]);
-    const double survive = 1.0 / (1.0 + exp(-static_cast<double>(o[0])));
-    if (!(u53(w[0], w[1]) < survive)) return false;
+    // u < 1 / (1 + exp(-s))
+    if (!(u24(w[0]) * (1.0f + __expf(-o[0])) < 1.0f)) return false;
     // component by the cumulative weights
-    const double uc = u53(w[2], w[3]);
-    int k = kComp - 1;
-    double acc = 0.0;
+    const float target = u24(w[1]) * wsum;
+    int comp = kComp - 1;
+    float acc = 0.0f;
+#pragma unroll
     for (int q = 0; q < kComp; q++) {
-        acc += exp(static_cast<double>(o[1 + 7 * q] - lse));
-        if (uc < acc) { k = q; break; }
+        acc += ew[q];
+        if (comp == kComp - 1 && target < acc) comp = q;
     }
-    philox4x32_10(1u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
-    const double r1 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
-    double s1, c1;
-    sincos_turn(u53(v[2], v[3]), s1, c1);
-    philox4x32_10(2u, 3u, id_lo, id_hi, k_lo, k_hi, v[0], v[1], v[2], v[3]);
-    const double r2 = sqrt(-2.0 * log_series(1.0 - u53(v[0], v[1])));
-    double s2, c2;
-    sincos_turn(u53(v[2], v[3]), s2, c2);
-    const double z[3] = {r1 * c1, r1 * s1, r2 * c2};
-    double y[3];
-    for (int j = 0; j < 3; j++) {
-        const float ls = fminf(fmaxf(o[5 + 7 * k + j], kLogSigmaMin), kLogSigmaMax);
-        y[j] = static_cast<double>(o[2 + 7 * k + j]) + exp(static_cast<double>(ls)) * z[j];
-    }
-    double px, py;
-    image_point(a.dom, f, px, py);
-    const double x = px + y[0] * a.dom.position_scale, yy = py + y[1] * a.dom.position_scale;
+    const float r1 = sqrtf(-2.0f * __logf(u24_open(w[2])));
+    float s1, c1;
+    __sincosf(6.2831853071795865f * u24(w[3]), &s1, &c1);
+    const float* oc = o + 7 * comp;
+    const float ls0 = fminf(fmaxf(oc[5], kLogSigmaMin), kLogSigmaMax);
+    const float ls1 = fminf(fmaxf(oc[6], kLogSigmaMin), kLogSigmaMax);
+    const float y0 = oc[2] + __expf(ls0) * (r1 * c1);
+    const float y1 = oc[3] + __expf(ls1) * (r1 * s1);
+    // nominal image point from the direction features
+    const float dx = f[2] * k.sin_angle, dy = f[3] * k.sin_angle;
+    const float inv_dz = -1.0f / sqrtf(fmaxf(1.0f - dx * dx - dy * dy, 1e-12f));
+    const float x = k.image_scale * dx * inv_dz + y0 * k.position_scale;
+    const float y = k.image_scale * dy * inv_dz + y1 * k.position_scale;
     // a position off the cells is a photon that misses the sensors
-    const int32_t cell = hex_cell(a.cells, x, yy);
+    const int32_t cell = hex_cell_f(a.cells, k, x, y);
     if (cell < 0) return false;
-    const double t = y[2] * a.dom.time_scale + a.dom.time_offset;
+    philox4x32_10(1u, 3u, id_lo, id_hi, k_lo, k_hi, w[0], w[1], w[2], w[3]);
+    const float r2 = sqrtf(-2.0f * __logf(u24_open(w[0])));
+    const float z2 = r2 * __cosf(6.2831853071795865f * u24(w[1]));
+    const float ls2 = fminf(fmaxf(oc[7], kLogSigmaMin), kLogSigmaMax);
+    const float yt = oc[4] + __expf(ls2) * z2;
+    const double t = static_cast<double>(yt) * a.dom.time_scale + a.dom.time_offset;
     const int64_t slot = arrival_cell(a.hist, cell, 0, t, a.hist.photon_id_offset + static_cast<uint64_t>(i));
     if (slot >= 0) atomicAdd(&a.hist.counts[slot], 1u);
     return false;
@@ -537,7 +609,8 @@ __device__ __forceinline__ void append_fallback(const SampleArgs& a, const bool 
 
 __global__ __launch_bounds__(256) void sample_kernel(const SampleArgs a, const int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool fallback = i < n && sample_one(a, a.features + i * kIn, a.out + i * kOutPad, i);
+    const DrawConsts k = draw_consts(a);
+    const bool fallback = i < n && sample_one(a, k, a.features + i * kIn, a.out + i * kOutPad, i);
     append_fallback(a, fallback, i);
 }
 
@@ -558,10 +631,22 @@ constexpr int kHStride = kHidden + 8;
 constexpr int kFStride = 16 + 8;
 constexpr int kOStride = kOutPad + 4;
 
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+// two floats to packed bf16 (round to nearest even) in one conversion
 __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
-    const uint16_t a = __builtin_bit_cast(uint16_t, static_cast<__bf16>(lo));
-    const uint16_t b = __builtin_bit_cast(uint16_t, static_cast<__bf16>(hi));
-    return static_cast<uint32_t>(a) | (static_cast<uint32_t>(b) << 16);
+    const float2v f = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2v));
+}
+
+// ReLU of two packed bf16 values: as 16-bit integers every negative bf16
+// (sign bit set) is below 0 and the non-negative ones keep their order, so
+// a packed integer max with 0 is the ReLU (-0 becomes +0).
+typedef short short2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+    const short2v x = __builtin_bit_cast(short2v, v);
+    const short2v zero = {0, 0};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(x, zero));
 }
 
 __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const int64_t n) {
@@ -571,6 +656,8 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
     float* O = reinterpret_cast<float*>(H);
     const int t = static_cast<int>(threadIdx.x), lane = t & 63, w = t >> 6;
     const int lr = lane & 31, lk = (lane >> 5) * 8, lq = (lane >> 5) * 4;
+    const DrawConsts dk = draw_consts(a.s);
+    const FeatureConsts fc = feature_consts(a.s.dom);
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * kTileP; base < n;
          base += static_cast<int64_t>(gridDim.x) * kTileP) {
         if (t < kTileP) {
@@ -581,7 +668,7 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
 #pragma unroll
                     for (int j = 0; j < kIn; j++) f[j] = a.features[i * kIn + j];
                 } else {
-                    photon_features(a.s.dom, a.ph, i, f);
+                    photon_features(fc, a.ph, i, f);
                 }
             } else {
 #pragma unroll
@@ -595,29 +682,76 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
             for (int j = 3; j < 8; j++) row[j] = 0u;
         }
         __syncthreads();
-        // layer 0 (K = 16, five of them non-zero) and the hidden layers
+        // layer 0 (K = 16, five of them non-zero) and the hidden layers.
+        // The accumulators start at the bias; the weights of k-step ks + 1
+        // are in flight while the instructions of k-step ks run, and the
+        // four activation fragments of a k-step are read before its eight
+        // matrix instructions.
+#pragma unroll 1
         for (int l = 0; l < kLayers - 1; l++) {
-            const bf16x8* Wf = reinterpret_cast<const bf16x8*>(a.wf[l]);
             const int ksteps = l == 0 ? 1 : kHidden / 16;
+            const bf16x8* W0 = reinterpret_cast<const bf16x8*>(a.wf[l]) + (2 * w) * ksteps * 64 + lane;
+            const bf16x8* W1 = W0 + ksteps * 64;
             const uint16_t* src = l == 0 ? F16 : H;
             const int stride = l == 0 ? kFStride : kHStride;
+            bf16x8 wc0 = W0[0], wc1 = W1[0];
+            static_assert(kHidden / 16 % 2 == 0, "the k-loop takes k-steps in pairs");
+            // bias in accumulator layout: the C operand of the first k-step
+            f32x16 bt[2];
+#pragma unroll
+            for (int ot = 0; ot < 2; ot++) {
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const float4 bias = *reinterpret_cast<const float4*>(a.b[l] + w * 64 + ot * 32 + g * 8 + lq);
+                    bt[ot][4 * g + 0] = bias.x;
+                    bt[ot][4 * g + 1] = bias.y;
+                    bt[ot][4 * g + 2] = bias.z;
+                    bt[ot][4 * g + 3] = bias.w;
+                }
+            }
             f32x16 acc[2][4];
+            // one k-step: the eight instructions of k-step ks on the weights (w0, w1)
+            auto kstep = [&](const int ks, const bf16x8& w0, const bf16x8& w1, const bool first) {
 #pragma unroll
-            for (int ot = 0; ot < 2; ot++)
+                for (int half = 0; half < 2; half++) {
+                    bf16x8 hv[2];
 #pragma unroll
-                for (int pt = 0; pt < 4; pt++)
+                    for (int q = 0; q < 2; q++)
+                        hv[q] = *reinterpret_cast<const bf16x8*>(src + ((2 * half + q) * 32 + lr) * stride + ks * 16 + lk);
 #pragma unroll
-                    for (int v = 0; v < 16; v++) acc[ot][pt][v] = 0.0f;
-            for (int ks = 0; ks < ksteps; ks++) {
-                bf16x8 wv[2];
-#pragma unroll
-                for (int ot = 0; ot < 2; ot++) wv[ot] = Wf[((2 * w + ot) * ksteps + ks) * 64 + lane];
-#pragma unroll
-                for (int pt = 0; pt < 4; pt++) {
-                    const bf16x8 hv = *reinterpret_cast<const bf16x8*>(src + (pt * 32 + lr) * stride + ks * 16 + lk);
-#pragma unroll
-                    for (int ot = 0; ot < 2; ot++)
-                        acc[ot][pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv[ot], hv, acc[ot][pt], 0, 0, 0);
+                    for (int q = 0; q < 2; q++) {
+                        const int pt = 2 * half + q;
+                        acc[0][pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, hv[q], first ? bt[0] : acc[0][pt], 0, 0, 0);
+                        acc[1][pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, hv[q], first ? bt[1] : acc[1][pt], 0, 0, 0);
+                    }
+                }
+            };
+            if (ksteps == 1) {
+                kstep(0, wc0, wc1, true);
+            } else {
+                {
+                    // k-steps 0 and 1
+                    const bf16x8 wb0 = W0[64], wb1 = W1[64];
+                    __builtin_amdgcn_sched_barrier(0);
+                    kstep(0, wc0, wc1, true);
+                    wc0 = W0[2 * 64];
+                    wc1 = W1[2 * 64];
+                    __builtin_amdgcn_sched_barrier(0);
+                    kstep(1, wb0, wb1, false);
+                }
+                // two register sets: the loads of one k-step are issued before
+                // the instructions of the previous one (the scheduling
+                // barriers keep the compiler from sinking them to their use)
+#pragma unroll 1
+                for (int ks = 2; ks < ksteps; ks += 2) {
+                    const bf16x8 wb0 = W0[(ks + 1) * 64], wb1 = W1[(ks + 1) * 64];
+                    __builtin_amdgcn_sched_barrier(0);
+                    kstep(ks, wc0, wc1, false);
+                    const int kn = ks + 2 < ksteps ? ks + 2 : ks + 1;
+                    wc0 = W0[kn * 64];
+                    wc1 = W1[kn * 64];
+                    __builtin_amdgcn_sched_barrier(0);
+                    kstep(ks + 1, wb0, wb1, false);
                 }
             }
             __syncthreads();
@@ -626,17 +760,12 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
 #pragma unroll
                 for (int g = 0; g < 4; g++) {
                     const int col = w * 64 + ot * 32 + g * 8 + lq;
-                    const float4 bias = *reinterpret_cast<const float4*>(a.b[l] + col);
 #pragma unroll
                     for (int pt = 0; pt < 4; pt++) {
                         const int row = pt * 32 + lr;
-                        const float z0 = fmaxf(acc[ot][pt][4 * g + 0] + bias.x, 0.0f);
-                        const float z1 = fmaxf(acc[ot][pt][4 * g + 1] + bias.y, 0.0f);
-                        const float z2 = fmaxf(acc[ot][pt][4 * g + 2] + bias.z, 0.0f);
-                        const float z3 = fmaxf(acc[ot][pt][4 * g + 3] + bias.w, 0.0f);
                         uint2 packed;
-                        packed.x = pack_bf16(z0, z1);
-                        packed.y = pack_bf16(z2, z3);
+                        packed.x = relu_bf16x2(pack_bf16(acc[ot][pt][4 * g + 0], acc[ot][pt][4 * g + 1]));
+                        packed.y = relu_bf16x2(pack_bf16(acc[ot][pt][4 * g + 2], acc[ot][pt][4 * g + 3]));
                         *reinterpret_cast<uint2*>(H + row * kHStride + col) = packed;
                     }
                 }
@@ -644,36 +773,57 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
             __syncthreads();
         }
         {
-            const bf16x8* Wf = reinterpret_cast<const bf16x8*>(a.wf[kLayers - 1]);
             const int ot = w & 1, pt0 = (w >> 1) * 2;
+            const bf16x8* W = reinterpret_cast<const bf16x8*>(a.wf[kLayers - 1]) + ot * (kHidden / 16) * 64 + lane;
+            f32x16 bt;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const float4 bias = *reinterpret_cast<const float4*>(a.b[kLayers - 1] + ot * 32 + g * 8 + lq);
+                bt[4 * g + 0] = bias.x;
+                bt[4 * g + 1] = bias.y;
+                bt[4 * g + 2] = bias.z;
+                bt[4 * g + 3] = bias.w;
+            }
             f32x16 acc[2];
+            auto kstep = [&](const int ks, const bf16x8& wv, const bool first) {
+                bf16x8 hv[2];
 #pragma unroll
-            for (int p = 0; p < 2; p++)
+                for (int p = 0; p < 2; p++)
+                    hv[p] = *reinterpret_cast<const bf16x8*>(H + ((pt0 + p) * 32 + lr) * kHStride + ks * 16 + lk);
 #pragma unroll
-                for (int v = 0; v < 16; v++) acc[p][v] = 0.0f;
-#pragma unroll 2
-            for (int ks = 0; ks < kHidden / 16; ks++) {
-                const bf16x8 wv = Wf[(ot * (kHidden / 16) + ks) * 64 + lane];
-#pragma unroll
-                for (int p = 0; p < 2; p++) {
-                    const bf16x8 hv =
-                        *reinterpret_cast<const bf16x8*>(H + ((pt0 + p) * 32 + lr) * kHStride + ks * 16 + lk);
-                    acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, hv, acc[p], 0, 0, 0);
-                }
+                for (int p = 0; p < 2; p++)
+                    acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, hv[p], first ? bt : acc[p], 0, 0, 0);
+            };
+            bf16x8 wc = W[0];
+            {
+                const bf16x8 wb = W[64];
+                __builtin_amdgcn_sched_barrier(0);
+                kstep(0, wc, true);
+                wc = W[2 * 64];
+                __builtin_amdgcn_sched_barrier(0);
+                kstep(1, wb, false);
+            }
+#pragma unroll 1
+            for (int ks = 2; ks < kHidden / 16; ks += 2) {
+                const bf16x8 wb = W[(ks + 1) * 64];
+                __builtin_amdgcn_sched_barrier(0);
+                kstep(ks, wc, false);
+                wc = W[(ks + 2 < kHidden / 16 ? ks + 2 : ks + 1) * 64];
+                __builtin_amdgcn_sched_barrier(0);
+                kstep(ks + 1, wb, false);
             }
             __syncthreads();
 #pragma unroll
             for (int g = 0; g < 4; g++) {
                 const int col = ot * 32 + g * 8 + lq;
-                const float4 bias = *reinterpret_cast<const float4*>(a.b[kLayers - 1] + col);
 #pragma unroll
                 for (int p = 0; p < 2; p++) {
                     const int row = (pt0 + p) * 32 + lr;
                     float4 z;
-                    z.x = acc[p][4 * g + 0] + bias.x;
-                    z.y = acc[p][4 * g + 1] + bias.y;
-                    z.z = acc[p][4 * g + 2] + bias.z;
-                    z.w = acc[p][4 * g + 3] + bias.w;
+                    z.x = acc[p][4 * g + 0];
+                    z.y = acc[p][4 * g + 1];
+                    z.z = acc[p][4 * g + 2];
+                    z.w = acc[p][4 * g + 3];
                     *reinterpret_cast<float4*>(O + row * kOStride + col) = z;
                 }
             }
@@ -687,7 +837,7 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
         }
         if (a.sample && w < 2) {
             const int64_t i = base + t;
-            const bool fallback = i < n && sample_one(a.s, F + t * kIn, O + t * kOStride, i);
+            const bool fallback = i < n && sample_one(a.s, dk, F + t * kIn, O + t * kOStride, i);
             append_fallback(a.s, fallback, i);
         }
         __syncthreads();
