@@ -72,7 +72,7 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
                max_entropy: float, accuracy_passes: int = 20) -> dict:
     """The surrogate + fallback hybrid on the same bunch: train the surrogate
     on photons the plain propagator produced (its domain: supports within
-    1.05 x the mirror radius, directions within 2.64 deg, lambda 280-700 nm;
+    1.1 x the mirror radius, directions within 2.64 deg, lambda 280-700 nm;
     position targets from the focal-plane twin scene; lost photons
     explained by the mixture's mass off the pixels), time the hybrid, and
     compare its per-pixel and per-pixel-and-time histograms with the plain

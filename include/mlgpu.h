@@ -448,7 +448,7 @@ int mlgpu_surrogate_set_region(mlgpu_surrogate* s, const mlgpu_surrogate_domain*
  * its position falls in, if any; every other photon is propagated by the
  * plain propagator and binned from its result.  *num_fallback (host, may be
  * NULL) receives the number of propagated photons.  Synchronises the
- * stream once per chunk of 2^22 photons. */
+ * stream once per chunk of 2^24 photons. */
 typedef struct mlgpu_hybrid_config {
     double max_entropy;
     uint64_t seed;

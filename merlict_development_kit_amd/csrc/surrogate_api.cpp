@@ -174,7 +174,7 @@ hipError_t backward(mlgpu_surrogate* s, const float* features, const float* targ
     return hipSuccess;
 }
 
-constexpr int64_t kHybridChunk = int64_t{1} << 22;
+constexpr int64_t kHybridChunk = int64_t{1} << 24;
 
 int ensure_hybrid(mlgpu_surrogate* s) {
     if (s->hy) return MLGPU_OK;

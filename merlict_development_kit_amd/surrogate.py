@@ -144,12 +144,12 @@ class SurrogateDomain:
 
 
 def fact_domain() -> SurrogateDomain:
-    """The FACT-like telescope's domain: supports within 1.05 x the 2.1 m
+    """The FACT-like telescope's domain: supports within 1.1 x the 2.1 m
     mirror radius (fact_propagation.cpp:295), directions within 1.2 x the
     camera's 2.2 deg field radius, lambda in [280, 700] nm; positions
     relative to the nominal image point in units of 1 cm.  The image scale
     and the time units are set from data by :func:`train`."""
-    return SurrogateDomain(1.05 * 2.1, math.radians(1.2 * 2.2), 280e-9, 700e-9)
+    return SurrogateDomain(1.1 * 2.1, math.radians(1.2 * 2.2), 280e-9, 700e-9)
 
 
 class HexCells:
