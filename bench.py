@@ -158,6 +158,8 @@ def main() -> int:
     ap.add_argument("--schedule", default="auto", choices=["auto", "fused", "wavefront"])
     ap.add_argument("--trace-budget", type=int, default=None)
     ap.add_argument("--blocks-per-cu", type=int, default=None)
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the library's autotuner on the bunch before timing (mlgpu_scene_autotune)")
     ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--hybrid-train-photons", type=int, default=1_000_000_000)
     ap.add_argument("--hybrid-max-entropy", type=float, default=float("inf"))
@@ -183,6 +185,10 @@ def main() -> int:
     photons = sources.fact_bunch().generate(n, seed=2024, device=dev)
     out = PropagationResult.empty(n, dev)
     stream = torch.cuda.current_stream()
+    autotuned = None
+    if args.autotune:
+        tuning, ms = scene.autotune(photons, PropagationConfig(5, seed=0), repeats=2, stream=stream)
+        autotuned = {"tuning": tuning, "ms_per_call": ms}
 
     kernel_ms = []
     for step in range(args.warmup + args.steps):
@@ -204,7 +210,7 @@ def main() -> int:
     value = n * args.steps / wall
     achieved = n * (IN_BYTES + OUT_BYTES) / (ms_kernel * 1e-3)
     default_workload = (n == 10_000_000 and not args.no_light_guides and args.trace_budget is None
-                        and args.blocks_per_cu is None and args.schedule == "auto")
+                        and args.blocks_per_cu is None and args.schedule == "auto" and not args.autotune)
     line = {
         "metric": METRIC,
         "value": value,
@@ -225,6 +231,7 @@ def main() -> int:
             "leaves": scene.stats()["leaves"],
             "max_num_interactions_per_photon": 5,
             "tuning": scene.tuning(),
+            "autotuned": autotuned,
             "detected_fraction": detected / n,
             "photon_errors": errors,
         },

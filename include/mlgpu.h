@@ -251,6 +251,19 @@ typedef struct mlgpu_tuning {
 int mlgpu_scene_set_tuning(mlgpu_scene* scene, const mlgpu_tuning* tuning);
 int mlgpu_scene_get_tuning(const mlgpu_scene* scene, mlgpu_tuning* tuning);
 
+/* In-library autotuner.  Times mlgpu_propagate of the given photons (device
+ * arrays) under each candidate tuning: FUSED, and WAVEFRONT with trace
+ * budget in {0, 12, 16, 24, 32, 48} x blocks per CU in {8, 16, 24, 32, 48};
+ * one warm-up call and `repeats` (>= 1) timed calls each, between HIP events
+ * on `stream`.  Keeps the fastest on the scene and returns it in *best and
+ * its mean time per call in *best_ms (either may be NULL).  The results of
+ * the trial calls go to a temporary device buffer; results never depend on
+ * the tuning.  Synchronises `stream`.  No counterpart in the reference. */
+int mlgpu_scene_autotune(mlgpu_scene* scene, int32_t world_frame,
+                         const mlgpu_photons* photons, int64_t n,
+                         const mlgpu_config* config, int32_t repeats,
+                         mlgpu_tuning* best, double* best_ms, void* stream);
+
 /* Propagate n photons, one Philox stream per photon.  Replaces
  * merlict::propagate_photons_in_frame_with_config (merlict/Photons.cpp:9-22)
  * and its multi-thread twin (merlict_multi_thread.cpp:28-55).  world_frame

@@ -179,6 +179,8 @@ SIGNATURES = {
     "mlgpu_scene_destroy": (None, [C.c_void_p]),
     "mlgpu_scene_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "mlgpu_scene_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "mlgpu_scene_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
+                                       C.c_int32, C.POINTER(Tuning), C.POINTER(C.c_double), C.c_void_p]),
     "mlgpu_scene_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlgpu_propagate": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
                                   C.POINTER(Results), C.c_void_p]),

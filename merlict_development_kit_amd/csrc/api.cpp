@@ -392,6 +392,69 @@ int mlgpu_scene_get_tuning(const mlgpu_scene* s, mlgpu_tuning* t) {
     return MLGPU_OK;
 }
 
+int mlgpu_scene_autotune(mlgpu_scene* s, int32_t world_frame, const mlgpu_photons* photons, int64_t n,
+                         const mlgpu_config* cfg, int32_t repeats, mlgpu_tuning* best, double* best_ms,
+                         void* stream) {
+    if (!s || !cfg || n <= 0 || repeats < 1) { set_last_error("null argument, no photons or repeats < 1"); return MLGPU_ERR_ARGUMENT; }
+    if (!photons_ok(photons)) { set_last_error("null photon array"); return MLGPU_ERR_ARGUMENT; }
+    int rc;
+    if ((rc = check_world(s, world_frame)) || (rc = check_config(cfg))) return rc;
+    if (n > INT32_MAX) { set_last_error("at most 2^31 - 1 photons per call"); return MLGPU_ERR_ARGUMENT; }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    // temporary results
+    const size_t cap = static_cast<size_t>(n);
+    const size_t bytes = cap * (4 + 4 + 1 + 1 + 1 + 8 + 4 * 4) + 12 * 256;
+    void* buf = nullptr;
+    hipError_t e = hipMalloc(&buf, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc autotune results");
+    Carver c{static_cast<char*>(buf)};
+    mlgpu_results r{c.take<int32_t>(cap), c.take<int32_t>(cap), c.take<uint8_t>(cap), c.take<uint8_t>(cap),
+                    c.take<uint8_t>(cap), c.take<double>(cap), c.take<float>(cap), c.take<float>(cap),
+                    c.take<float>(cap), c.take<float>(cap), nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    std::vector<mlgpu_tuning> cands;
+    cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_FUSED, s->tuning.trace_budget, s->tuning.blocks_per_cu});
+    for (int32_t budget : {0, 12, 16, 24, 32, 48})
+        for (int32_t bpc : {8, 16, 24, 32, 48}) cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_WAVEFRONT, budget, bpc});
+    const mlgpu_tuning saved = s->tuning;
+    mlgpu_tuning chosen = saved;
+    double chosen_ms = -1.0;
+    for (const mlgpu_tuning& t : cands) {
+        if (e != hipSuccess) break;
+        s->tuning = t;
+        rc = mlgpu_propagate(s, world_frame, photons, n, cfg, &r, stream);  // warm-up
+        if (rc) break;
+        e = hipEventRecord(ev[0], st);
+        for (int32_t k = 0; k < repeats && rc == MLGPU_OK && e == hipSuccess; k++)
+            rc = mlgpu_propagate(s, world_frame, photons, n, cfg, &r, stream);
+        if (rc) break;
+        if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (e != hipSuccess) break;
+        const double per_call = static_cast<double>(ms) / repeats;
+        if (chosen_ms < 0.0 || per_call < chosen_ms) {
+            chosen_ms = per_call;
+            chosen = t;
+        }
+    }
+    (void)hipStreamSynchronize(st);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    (void)hipFree(buf);
+    if (rc || e != hipSuccess) {
+        s->tuning = saved;
+        return rc ? rc : hip_fail(e, "autotune timing");
+    }
+    s->tuning = chosen;
+    if (best) *best = chosen;
+    if (best_ms) *best_ms = chosen_ms;
+    return MLGPU_OK;
+}
+
 int mlgpu_desc_frame_poses(const mlgpu_scene_desc* d, double* out) {
     if (!d || !out) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
     try {

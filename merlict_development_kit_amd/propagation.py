@@ -158,6 +158,18 @@ class GpuScene:
     def set_schedule(self, schedule: str) -> None:
         self.set_tuning(schedule=schedule)
 
+    def autotune(self, photons: PhotonBatch, config: PropagationConfig = PropagationConfig(), repeats: int = 3,
+                 world_frame: int = 0, stream: Optional[torch.cuda.Stream] = None):
+        """Time the candidate tunings on these photons (mlgpu_scene_autotune),
+        keep the fastest; returns (tuning, mean ms per call)."""
+        if photons.device.type != "cuda":
+            raise ValueError("photons must be on the GPU")
+        ph, cfg = photons.c_struct(), config.c_struct()
+        t, ms = N.Tuning(), C.c_double()
+        N.check(N.lib().mlgpu_scene_autotune(self._h, world_frame, C.byref(ph), len(photons), C.byref(cfg), repeats,
+                                             C.byref(t), C.byref(ms), _stream_ptr(stream)), "mlgpu_scene_autotune")
+        return self.tuning(), ms.value
+
     def stats(self) -> dict:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         N.check(N.lib().mlgpu_scene_stats(self._h, C.byref(a), C.byref(b), C.byref(c)), "mlgpu_scene_stats")

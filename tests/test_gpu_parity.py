@@ -125,6 +125,29 @@ def test_gpu_schedules_agree_bitwise(cuda, max_int):
             np.testing.assert_array_equal(ref[k], res[k], err_msg=f"{k} (budget {budget}, {blocks} blocks/CU)")
 
 
+def test_gpu_autotune_picks_a_candidate_and_keeps_results(cuda):
+    """The in-library autotuner times the candidate tunings, leaves the
+    fastest on the scene, and propagation results stay bit for bit the
+    same."""
+    gs = GpuScene(SCENES["fact_lg"]())
+    ph = sources.fact_bunch(400_000, seed=9, device=cuda)
+    ref = gs.propagate(ph, PropagationConfig(5, seed=3)).numpy()
+    tuning, ms = gs.autotune(ph, PropagationConfig(5, seed=3), repeats=1)
+    print(f"autotune: {tuning}, {ms:.3f} ms per call")
+    assert ms > 0
+    assert gs.tuning() == tuning
+    if tuning["schedule"] == "wavefront":
+        assert tuning["trace_budget"] in (0, 12, 16, 24, 32, 48)
+        assert tuning["blocks_per_cu"] in (8, 16, 24, 32, 48)
+    else:
+        assert tuning["schedule"] == "fused"
+    res = gs.propagate(ph, PropagationConfig(5, seed=3)).numpy()
+    for k in ref:
+        np.testing.assert_array_equal(ref[k], res[k], err_msg=k)
+    with pytest.raises(MlgpuError):
+        gs.autotune(ph, PropagationConfig(5, seed=3), repeats=0)
+
+
 def test_gpu_tuning_is_validated(cuda):
     gs = GpuScene(SCENES["lens"]())
     assert gs.tuning() == {"schedule": "auto", "trace_budget": 24, "blocks_per_cu": 32}
