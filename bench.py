@@ -146,6 +146,36 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
     }
 
 
+def corsika_leg(dev, n: int = 10_000_000, reps: int = 5) -> dict:
+    """CORSIKA photon-bunch ingestion (mlgpu_corsika_photons): n synthetic
+    bunches (weights uniform on [0, 2]) to photons, whole call timed
+    (capacity bound, output allocation, count + scan, emit)."""
+    from merlict_development_kit_amd.corsika import photons_from_bunches
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    b = torch.rand((n, 8), generator=g, device=dev, dtype=torch.float32)
+    b[:, 0:2] = (b[:, 0:2] - 0.5) * 4e4
+    b[:, 2:4] = (b[:, 2:4] - 0.5) * 0.4
+    b[:, 4] = b[:, 4] * 100.0
+    b[:, 5] = 1e6
+    b[:, 6] = b[:, 6] * 2.0
+    b[:, 7] = -(250.0 + 450.0 * b[:, 7])
+    ph, _ = photons_from_bunches(b, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        ph, _ = photons_from_bunches(b, seed=r)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    m = len(ph)
+    algorithmic = n * 32 + m * (7 * 8 + 4)
+    return {"value": n / dt, "unit": "bunches/s", "photons_per_s": m / dt, "bunches": n, "photons": m,
+            "ms_per_call": dt * 1e3,
+            "roofline": {"bound": "hbm", "achieved": algorithmic / dt / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": algorithmic / dt / HBM_PEAK,
+                         "algorithmic_bytes": "32 B per bunch in, 60 B per photon out"}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +191,7 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="run the library's autotuner on the bunch before timing (mlgpu_scene_autotune)")
     ap.add_argument("--no-hybrid", action="store_true")
+    ap.add_argument("--no-corsika", action="store_true")
     ap.add_argument("--hybrid-train-photons", type=int, default=1_000_000_000)
     ap.add_argument("--hybrid-max-entropy", type=float, default=float("inf"))
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -252,6 +283,8 @@ def main() -> int:
         line["hybrid"] = hybrid_leg(desc, scene, photons, args.steps, args.warmup, args.hybrid_train_photons,
                                     args.hybrid_max_entropy)
         line["hybrid"]["speedup_vs_plain"] = line["hybrid"]["value"] / value
+    if not args.no_corsika:
+        line["corsika_ingestion"] = corsika_leg(dev)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(desc, photons, args.cpu_threads)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]

@@ -475,6 +475,26 @@ int mlgpu_propagate_hybrid(const mlgpu_scene* scene, mlgpu_surrogate* surrogate,
                            const mlgpu_histogram* hist, int64_t* num_fallback,
                            void* stream);
 
+/* CORSIKA photon bunches to photons.  Replaces merlict::EventIoPhotonFactory
+ * (merlict_corsika/PhotonFactory.cpp:8-148) as the reference's apps apply it
+ * to every bunch of an event (plenoscope_propagation.cpp:196-203).
+ * bunches: n x 8 float (device, 16-byte aligned): x, y [cm], cx, cy (the
+ * direction cosines of the motion), arrival time on the ground [ns],
+ * emission height [cm], weight, wavelength [nm] (either sign).  A bunch of
+ * weight w yields floor(w) photons plus one more when the uniform (w0 >> 8)
+ * 2^-24 of Philox stream 5 of (seed, id_offset + bunch) is <= fmod(w, 1);
+ * each starts 1e3 m + c t above its ground point (x, y in metres) against
+ * its direction (cx, cy, -sqrt(1 - cx^2 - cy^2)), wavelength |lambda| nm.
+ * The photons go to out in bunch order, their bunch index to truth_id (may
+ * be NULL); *num_photons (host) receives their number.  A weight outside
+ * [0, 16] (the reference's BadPhotonWeight) or a capacity below the number
+ * of photons is an MLGPU_ERR_ARGUMENT (then nothing is written; the number
+ * is still reported for a capacity error).  Synchronises `stream`. */
+int mlgpu_corsika_photons(const float* bunches, int64_t n, uint64_t seed,
+                          uint64_t id_offset, const mlgpu_photon_buffers* out,
+                          int32_t* truth_id, int64_t capacity,
+                          int64_t* num_photons, void* stream);
+
 /* Number of photons with a non-zero error flag in the last n results
  * (synchronises `stream`). */
 int mlgpu_count_errors(const mlgpu_results* results, int64_t n, void* stream,

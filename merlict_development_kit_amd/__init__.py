@@ -12,6 +12,7 @@ from .propagation import (GpuScene, PhotonBatch, PropagationConfig, PropagationR
 from . import sources
 from .arrivals import ArrivalBinning, histogram_arrivals
 from .bunches import Bunch, propagate_bunches
+from . import corsika
 
 __all__ = [
     "SceneDescription",
@@ -25,4 +26,5 @@ __all__ = [
     "histogram_arrivals",
     "Bunch",
     "propagate_bunches",
+    "corsika",
 ]

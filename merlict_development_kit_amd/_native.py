@@ -206,6 +206,8 @@ SIGNATURES = {
                                          C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
                                          C.POINTER(HybridConfig), C.POINTER(Histogram), C.POINTER(C.c_int64),
                                          C.c_void_p]),
+    "mlgpu_corsika_photons": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(PhotonBuffers),
+                                        C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]),
     "mlgpu_histogram_arrivals": (C.c_int, [C.POINTER(Results), C.c_int64, C.POINTER(Histogram), C.c_void_p]),
     "mlgpu_count_errors": (C.c_int, [C.POINTER(Results), C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]),
     "mlgpu_first_hit": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Hits),

@@ -36,7 +36,7 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["scene_desc.cpp", "builders.cpp", "scene_build.cpp", "api.cpp", "propagate.hip", "sources.hip", "arrivals.hip", "surrogate.hip",
+SOURCES = ["scene_desc.cpp", "builders.cpp", "scene_build.cpp", "api.cpp", "propagate.hip", "sources.hip", "arrivals.hip", "corsika.hip", "surrogate.hip",
            "surrogate_api.cpp"]
 
 

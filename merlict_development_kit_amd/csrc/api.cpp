@@ -534,6 +534,68 @@ int mlgpu_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed, ui
     return MLGPU_OK;
 }
 
+int mlgpu_corsika_photons(const float* bunches, int64_t n, uint64_t seed, uint64_t id_offset,
+                          const mlgpu_photon_buffers* out, int32_t* truth_id, int64_t capacity, int64_t* num_photons,
+                          void* stream) {
+    if (!out || !num_photons || n < 0 || capacity < 0) {
+        set_last_error("null argument or negative size");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    *num_photons = 0;
+    if (n == 0) return MLGPU_OK;
+    if (!bunches) {
+        set_last_error("null bunches");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if ((reinterpret_cast<uintptr_t>(bunches) & 15u) != 0) {
+        set_last_error("bunches must be 16-byte aligned");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (capacity > 0 && (!out->support_x || !out->support_y || !out->support_z || !out->direction_x ||
+                         !out->direction_y || !out->direction_z || !out->wavelength)) {
+        set_last_error("null photon array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t ws_bytes = mlgpu::corsika_workspace_bytes(n);
+    void* ws = nullptr;
+    hipError_t e = hipMalloc(&ws, ws_bytes + 256);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc CORSIKA workspace");
+    int64_t* host = nullptr;  // {bad, total}
+    e = hipHostMalloc(reinterpret_cast<void**>(&host), 2 * sizeof(int64_t));
+    if (e != hipSuccess) {
+        (void)hipFree(ws);
+        return hip_fail(e, "hipHostMalloc");
+    }
+    char* flags = static_cast<char*>(ws) + ws_bytes;  // int32 bad at +0, int64 total at +8
+    int32_t* bad = reinterpret_cast<int32_t*>(flags);
+    int64_t* total = reinterpret_cast<int64_t*>(flags + 8);
+    int rc = MLGPU_OK;
+    e = hipMemsetAsync(flags, 0, 16, st);
+    if (e == hipSuccess) e = mlgpu::launch_corsika_count(bunches, n, seed, id_offset, ws, bad, total, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, flags, 16, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "CORSIKA count");
+    } else if (static_cast<int32_t>(host[0]) != 0) {
+        set_last_error("CORSIKA bunch weight outside [0, 16]");  // PhotonFactory.cpp:138-148
+        rc = MLGPU_ERR_ARGUMENT;
+    } else {
+        *num_photons = host[1];
+        if (host[1] > capacity) {
+            set_last_error("capacity below the number of photons of the bunches (*num_photons)");
+            rc = MLGPU_ERR_ARGUMENT;
+        } else {
+            e = mlgpu::launch_corsika_emit(bunches, n, seed, id_offset, ws, capacity, *out, truth_id, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) rc = hip_fail(e, "CORSIKA emit");
+        }
+    }
+    (void)hipHostFree(host);
+    (void)hipFree(ws);
+    return rc;
+}
+
 int mlgpu_histogram_arrivals(const mlgpu_results* r, int64_t n, const mlgpu_histogram* h, void* stream) {
     if (!r || !h || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
     if (n == 0) return MLGPU_OK;

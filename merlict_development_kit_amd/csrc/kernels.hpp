@@ -67,4 +67,13 @@ hipError_t launch_histogram(const mlgpu_results& r, int64_t n, const mlgpu_histo
 
 hipError_t launch_count_errors(const uint8_t* err, int64_t n, unsigned long long* counter, hipStream_t stream);
 
+// CORSIKA bunches (corsika.hip): workspace of n bunches; count + scan
+// (bad weights flagged in *bad, the photon total in *total), then emit.
+size_t corsika_workspace_bytes(int64_t n);
+hipError_t launch_corsika_count(const float* bunches, int64_t n, uint64_t seed, uint64_t id_offset, void* ws,
+                                int32_t* bad, int64_t* total, hipStream_t stream);
+hipError_t launch_corsika_emit(const float* bunches, int64_t n, uint64_t seed, uint64_t id_offset, const void* ws,
+                               int64_t capacity, const mlgpu_photon_buffers& out, int32_t* truth,
+                               hipStream_t stream);
+
 }  // namespace mlgpu

@@ -1216,3 +1216,45 @@ int oracle_propagate(const oracle_scene* s, int32_t world, const mlgpu_photons* 
     free(th);
     return 0;
 }
+
+/* ------------------------------------------------------ CORSIKA bunches
+ * merlict_corsika/PhotonFactory.cpp: weight check :138-148 and collapse
+ * :8-24 (floor(w) photons, one more when a uniform <= fmod(w, 1)),
+ * production point :30-70,126-128 (the ground point x, y [cm -> m] moved
+ * against the direction by c t + 1e3 m), direction :72-124 (cx, cy,
+ * -sqrt(1 - cx^2 - cy^2)), wavelength :134-136; Ray and Photon normalise
+ * their directions (Ray.cpp:16-20, Vec3.cpp:20-26). */
+int64_t oracle_corsika_photons(const float* bunches, int64_t n, uint64_t seed, uint64_t id_offset,
+                               double* const out[7], int32_t* truth, int64_t capacity) {
+    int64_t k = 0;
+    for (int64_t b = 0; b < n; b++) {
+        const float* c = bunches + 8 * b;
+        const double weight = c[6];
+        if (weight < 0.0 || weight > 16.0) return -1;
+        uint32_t num = (uint32_t)floor(weight);
+        const double rest = fmod(weight, 1.0);
+        const uint64_t id = id_offset + (uint64_t)b;
+        const uint32_t ctr[4] = {0u, 5u, (uint32_t)id, (uint32_t)(id >> 32)};
+        const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+        uint32_t w[4];
+        oracle_philox4x32_10(ctr, key, w);
+        if ((double)(w[0] >> 8) * (1.0 / 16777216.0) <= rest) num += 1;
+        const double cx = c[2], cy = c[3];
+        const double d[3] = {cx, cy, -sqrt(1.0 - cx * cx - cy * cy)};
+        double up[3] = {d[0] * -1.0, d[1] * -1.0, d[2] * -1.0};
+        const double un = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+        up[0] = up[0] / un; up[1] = up[1] / un; up[2] = up[2] / un;
+        const double gx = c[0] * 1e-2, gy = c[1] * 1e-2;
+        const double s = c[4] * 1e-9 * 299792458.0 + 1e3;
+        const double sx = gx + up[0] * s, sy = gy + up[1] * s, sz = 0.0 + up[2] * s;
+        const double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double v[7] = {sx, sy, sz, d[0] / dn, d[1] / dn, d[2] / dn, fabs(c[7] * 1e-9)};
+        for (uint32_t j = 0; j < num; j++) {
+            if (k >= capacity) return -2;
+            for (int q = 0; q < 7; q++) out[q][k] = v[q];
+            if (truth) truth[k] = (int32_t)b;
+            k++;
+        }
+    }
+    return k;
+}

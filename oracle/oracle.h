@@ -103,6 +103,17 @@ void oracle_histogram_arrivals(const int32_t* sensor_id, const uint8_t* error,
                                const mlgpu_histogram* hist, uint32_t* counts);
 double oracle_arrival_jitter(uint64_t seed, uint64_t photon_id);
 
+/* CORSIKA photon bunches to photons (merlict::EventIoPhotonFactory,
+ * merlict_corsika/PhotonFactory.cpp:8-148; the weight draw is Philox stream
+ * 5 of (seed, id_offset + bunch), uniform (w0 >> 8) 2^-24).  bunches: n x 8
+ * float (x, y [cm], cx, cy, time [ns], emission height [cm], weight,
+ * wavelength [nm]).  The photons of every bunch, in bunch order, go to
+ * out[0..6] (support xyz, direction xyz, wavelength) and their bunch index
+ * to truth (both up to capacity entries).  Returns the number of photons,
+ * -1 for a weight outside [0, 16], -2 when capacity is too small. */
+int64_t oracle_corsika_photons(const float* bunches, int64_t n, uint64_t seed, uint64_t id_offset,
+                               double* const out[7], int32_t* truth, int64_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

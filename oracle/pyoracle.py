@@ -90,6 +90,9 @@ def lib() -> C.CDLL:
                                                 C.POINTER(C.c_uint32)]
         L.oracle_arrival_jitter.restype = C.c_double
         L.oracle_arrival_jitter.argtypes = [C.c_uint64, C.c_uint64]
+        L.oracle_corsika_photons.restype = C.c_int64
+        L.oracle_corsika_photons.argtypes = [C.POINTER(C.c_float), C.c_int64, C.c_uint64, C.c_uint64,
+                                             C.POINTER(_P_F64), _P_I32, C.c_int64]
         L.oracle_log_series.restype = C.c_double
         L.oracle_log_series.argtypes = [C.c_double]
         L.oracle_sincos_turn.argtypes = [C.c_double, _P_F64, _P_F64]
@@ -205,6 +208,55 @@ def histogram_arrivals(results: dict, num_sensors: int, num_bins: int, t_start: 
     lib().oracle_histogram_arrivals(_ptr(sensor, _P_I32), _ptr(err, _P_U8), _ptr(tof, _P_F64), len(sensor),
                                     C.byref(h), counts.ctypes.data_as(C.POINTER(C.c_uint32)))
     return counts.reshape(num_sensors, num_bins)
+
+
+def corsika_photons(bunches: np.ndarray, seed: int = 0, id_offset: int = 0):
+    """Photons of CORSIKA bunches (n x 8 float32), restated on the CPU
+    (oracle_corsika_photons): (photons dict, truth ids).  Raises ValueError
+    for a weight outside [0, 16]."""
+    b = np.ascontiguousarray(bunches, dtype=np.float32)
+    n = len(b)
+    cap = int(np.floor(np.clip(b[:, 6].astype(np.float64), 0, 16)).sum()) + n if n else 0
+    out = {k: np.empty(cap, dtype=np.float64) for k in PHOTON_FIELDS}
+    truth = np.empty(cap, dtype=np.int32)
+    arr = (_P_F64 * 7)(*[_ptr(out[k], _P_F64) for k in PHOTON_FIELDS])
+    m = lib().oracle_corsika_photons(b.ctypes.data_as(C.POINTER(C.c_float)), n, seed, id_offset, arr,
+                                     _ptr(truth, _P_I32), cap)
+    if m == -1:
+        raise ValueError("CORSIKA bunch weight outside [0, 16]")
+    if m < 0:
+        raise RuntimeError("capacity")
+    return {k: v[:m] for k, v in out.items()}, truth[:m]
+
+
+def ref_corsika_extract(path) -> np.ndarray:
+    """Every photon bunch of a CORSIKA eventio file, read by the reference."""
+    with tempfile.TemporaryDirectory() as d:
+        o = Path(d) / "b.bin"
+        _run_ref(["corsika-extract", path, o])
+        raw = o.read_bytes()
+    n = int(np.frombuffer(raw[:8], np.int64)[0])
+    return np.frombuffer(raw[8:], np.float32).reshape(n, 8).copy()
+
+
+def ref_corsika(bunches: np.ndarray, seed: int = 0, id_offset: int = 0) -> dict:
+    """The reference's EventIoPhotonFactory on each bunch (Philox stream 5 of
+    (seed, id_offset + bunch)): status (1 = bad weight), count, photons, truth."""
+    b = np.ascontiguousarray(bunches, dtype=np.float32)
+    with tempfile.TemporaryDirectory() as d:
+        i, o = Path(d) / "b.bin", Path(d) / "o.bin"
+        i.write_bytes(np.array([len(b)], np.int64).tobytes() + b.tobytes())
+        _run_ref(["corsika", i, seed, id_offset, o])
+        raw = o.read_bytes()
+    n = int(np.frombuffer(raw[:8], np.int64)[0])
+    pos = 8
+    status = np.frombuffer(raw[pos:pos + n], np.uint8).copy(); pos += n
+    count = np.frombuffer(raw[pos:pos + 4 * n], np.uint32).copy(); pos += 4 * n
+    m = int(np.frombuffer(raw[pos:pos + 8], np.int64)[0]); pos += 8
+    ph = np.frombuffer(raw[pos:pos + 56 * m], np.float64).reshape(m, 7); pos += 56 * m
+    truth = np.frombuffer(raw[pos:pos + 4 * m], np.int32).copy()
+    return {"status": status, "count": count, "photons": {k: ph[:, j].copy() for j, k in enumerate(PHOTON_FIELDS)},
+            "truth": truth}
 
 
 def arrival_jitter(seed: int, photon_id: int) -> float:

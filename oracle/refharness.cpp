@@ -31,6 +31,8 @@
 #include "merlict/scenery/segmented_imaging_reflector/segmented_imaging_reflector.h"
 #include "merlict_multi_thread/merlict_multi_thread.h"
 #include "merlict_portal_plenoscope/light_field_sensor/LightFieldSensor.h"
+#include "merlict_corsika/PhotonFactory.h"
+#include "merlict_corsika/eventio.h"
 
 namespace ml = merlict;
 
@@ -55,10 +57,10 @@ class PhiloxGenerator : public ml::random::Generator {
     uint32_t next_ = 0;
 
  public:
-    PhiloxGenerator(uint64_t seed, uint64_t photon_id) {
+    PhiloxGenerator(uint64_t seed, uint64_t photon_id, uint32_t stream = 0) {
         key_[0] = static_cast<uint32_t>(seed);
         key_[1] = static_cast<uint32_t>(seed >> 32);
-        ctr_[0] = 0; ctr_[1] = 0;
+        ctr_[0] = 0; ctr_[1] = stream;
         ctr_[2] = static_cast<uint32_t>(photon_id);
         ctr_[3] = static_cast<uint32_t>(photon_id >> 32);
         seed_ = seed;
@@ -390,6 +392,69 @@ static int cmd_reflector(int argc, char** argv) {
     return 0;
 }
 
+// corsika-extract <eventio file> <out>: every photon bunch of every event
+// of a CORSIKA eventio file, as n x 8 float32 after an int64 count.
+static int cmd_corsika_extract(int argc, char** argv) {
+    if (argc != 4) return 2;
+    eventio::Run run(argv[2]);
+    std::vector<float> all;
+    while (run.has_still_events_left()) {
+        eventio::Event ev = run.next_event();
+        for (const auto& b : ev.photons) all.insert(all.end(), b.begin(), b.end());
+    }
+    std::ofstream o(argv[3], std::ios::binary);
+    const int64_t n = static_cast<int64_t>(all.size() / 8);
+    o.write(reinterpret_cast<const char*>(&n), 8);
+    o.write(reinterpret_cast<const char*>(all.data()), static_cast<std::streamsize>(all.size() * 4));
+    return 0;
+}
+
+// corsika <bunches> <seed> <id0> <out>: the reference's EventIoPhotonFactory
+// on each bunch (int64 count + n x 8 float32), one Philox stream 5 per bunch
+// id0 + b; writes int64 n, then per bunch the uint8 status (0 ok, 1 bad
+// weight) and the uint32 number of photons, then int64 m and m photons of
+// 7 float64 (support, direction, wavelength) + int32 truth id.
+static int cmd_corsika(int argc, char** argv) {
+    if (argc != 6) return 2;
+    std::ifstream in(argv[2], std::ios::binary);
+    int64_t n = 0;
+    in.read(reinterpret_cast<char*>(&n), 8);
+    std::vector<float> b(static_cast<size_t>(n) * 8);
+    in.read(reinterpret_cast<char*>(b.data()), static_cast<std::streamsize>(b.size() * 4));
+    const uint64_t seed = std::strtoull(argv[3], nullptr, 10);
+    const uint64_t id0 = std::strtoull(argv[4], nullptr, 10);
+    std::vector<uint8_t> status(static_cast<size_t>(n), 0);
+    std::vector<uint32_t> count(static_cast<size_t>(n), 0);
+    std::vector<double> ph;
+    std::vector<int32_t> truth;
+    for (int64_t i = 0; i < n; i++) {
+        std::array<float, 8> c;
+        for (int k = 0; k < 8; k++) c[k] = b[static_cast<size_t>(i) * 8 + k];
+        PhiloxGenerator prng(seed, id0 + static_cast<uint64_t>(i), 5u);
+        try {
+            ml::EventIoPhotonFactory f(c, static_cast<unsigned>(i), &prng);
+            while (f.has_still_photons_to_be_made()) {
+                const ml::Photon p = f.make_photon();
+                const ml::Vec3 s = p.support(), d = p.direction();
+                ph.insert(ph.end(), {s.x, s.y, s.z, d.x, d.y, d.z, p.wavelength});
+                truth.push_back(static_cast<int32_t>(p.simulation_truth_id));
+                count[static_cast<size_t>(i)]++;
+            }
+        } catch (const ml::EventIoPhotonFactory::BadPhotonWeight&) {
+            status[static_cast<size_t>(i)] = 1;
+        }
+    }
+    std::ofstream o(argv[5], std::ios::binary);
+    o.write(reinterpret_cast<const char*>(&n), 8);
+    o.write(reinterpret_cast<const char*>(status.data()), static_cast<std::streamsize>(n));
+    o.write(reinterpret_cast<const char*>(count.data()), static_cast<std::streamsize>(n * 4));
+    const int64_t m = static_cast<int64_t>(truth.size());
+    o.write(reinterpret_cast<const char*>(&m), 8);
+    o.write(reinterpret_cast<const char*>(ph.data()), static_cast<std::streamsize>(ph.size() * 8));
+    o.write(reinterpret_cast<const char*>(truth.data()), static_cast<std::streamsize>(m * 4));
+    return 0;
+}
+
 static int cmd_lensmaker(int argc, char** argv) {
     if (argc < 5) return 2;
     ml::lens_maker::Config c;
@@ -489,7 +554,7 @@ static int cmd_bench(int argc, char** argv) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: merlict_golden propagate|tree|firsthit|reflector|lensmaker|lightfield|bench ...\n");
+        std::fprintf(stderr, "usage: merlict_golden propagate|tree|firsthit|reflector|lensmaker|lightfield|bench|corsika-extract|corsika ...\n");
         return 2;
     }
     try {
@@ -502,6 +567,8 @@ int main(int argc, char** argv) {
         else if (c == "lensmaker") rc = cmd_lensmaker(argc, argv);
         else if (c == "lightfield") rc = cmd_lightfield(argc, argv);
         else if (c == "bench") rc = cmd_bench(argc, argv);
+        else if (c == "corsika-extract") rc = cmd_corsika_extract(argc, argv);
+        else if (c == "corsika") rc = cmd_corsika(argc, argv);
         if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", c.c_str());
         return rc;
     } catch (const std::exception& e) {

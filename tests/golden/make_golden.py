@@ -5,7 +5,7 @@ oracle/Makefile, with the Philox stream plugged into merlict's
 random::Generator) on seeded inputs and stores inputs and outputs as small
 .npz files.  Run in the container that has the reference mounted:
 
-    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo]
+    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika]
 """
 from __future__ import annotations
 
@@ -81,9 +81,46 @@ def zoo_fixtures():
     save_propagation("zoo_propagation", zoo, photons, seed=23, max_int=8)
 
 
+def corsika_fixtures():
+    """CORSIKA photon bunches through the reference's EventIoPhotonFactory:
+    the first 2000 bunches of the reference's test file telescope.dat
+    (weights 0.23-1, negative wavelengths as CORSIKA writes them) and 300
+    synthetic bunches with weights over [0, 16] (0, 1, 15.5 and 16 among
+    them) and both wavelength signs; plus three bunches with bad weights."""
+    tel = O.ref_corsika_extract("/root/reference/merlict_corsika/tests/resources/telescope.dat")[:2000]
+    rng = np.random.default_rng(11)
+    m = 300
+    syn = np.zeros((m, 8), np.float32)
+    syn[:, 0] = rng.uniform(-3e4, 3e4, m)
+    syn[:, 1] = rng.uniform(-3e4, 3e4, m)
+    ang = rng.uniform(0, 2 * np.pi, m)
+    sin_t = rng.uniform(0, 0.6, m)
+    syn[:, 2] = sin_t * np.cos(ang)
+    syn[:, 3] = sin_t * np.sin(ang)
+    syn[:, 4] = rng.uniform(-50, 200, m)
+    syn[:, 5] = rng.uniform(5e5, 2e6, m)
+    syn[:, 6] = rng.uniform(0, 16, m)
+    syn[:4, 6] = [0.0, 1.0, 15.5, 16.0]
+    syn[:, 7] = rng.uniform(250, 700, m) * np.where(rng.uniform(size=m) < 0.5, -1, 1)
+    bunches = np.concatenate([tel, syn])
+    ref = O.ref_corsika(bunches, seed=21, id_offset=5)
+    assert not ref["status"].any()
+    bad = bunches[:3].copy()
+    bad[:, 6] = [-0.1, 16.1, 0.5]
+    ref_bad = O.ref_corsika(bad, seed=21)
+    arrays = {"bunches": bunches, "count": ref["count"], "truth": ref["truth"], "seed": np.array([21], np.uint64),
+              "id_offset": np.array([5], np.uint64), "bad_bunches": bad, "bad_status": ref_bad["status"]}
+    arrays.update({f"out_{k}": v for k, v in ref["photons"].items()})
+    np.savez_compressed(HERE / "corsika_photons.npz", **arrays)
+    print("corsika_photons", len(bunches), "bunches", len(ref["truth"]), "photons")
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/merlict_golden is not built")
+    if len(sys.argv) > 1 and sys.argv[1] == "corsika":
+        corsika_fixtures()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "plenoscope":
         plenoscope_fixtures()
         return

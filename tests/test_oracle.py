@@ -398,3 +398,67 @@ def test_arrival_jitter_is_standard_normal():
     mean = (h * centres).sum() / h.sum()
     sd = np.sqrt((h * (centres - mean) ** 2).sum() / h.sum())
     assert abs(mean - 50.0) < 0.2 and abs(sd - 5.0) < 0.2
+
+
+# ------------------------------------------------------------ CORSIKA bunches
+def test_oracle_corsika_photons_match_reference(golden):
+    """The oracle's EventIoPhotonFactory restatement reproduces the
+    reference's photons bit for bit (telescope.dat bunches and synthetic
+    weights over [0, 16]), with the same weight collapse under the same
+    Philox stream 5 draws."""
+    g = golden("corsika_photons")
+    ph, truth = O.corsika_photons(g["bunches"], seed=int(g["seed"][0]), id_offset=int(g["id_offset"][0]))
+    np.testing.assert_array_equal(truth, g["truth"])
+    np.testing.assert_array_equal(np.bincount(truth, minlength=len(g["bunches"])), g["count"])
+    for k in O.PHOTON_FIELDS:
+        np.testing.assert_array_equal(ph[k], g[f"out_{k}"], err_msg=k)
+    assert g["count"][len(g["bunches"]) - 300:][:4].tolist()[1:4] in ([1, 15, 16], [1, 16, 16])
+    # the reference rejects exactly the bunches with weights outside [0, 16]
+    assert g["bad_status"].tolist() == [1, 1, 0]
+    with pytest.raises(ValueError):
+        O.corsika_photons(g["bad_bunches"])
+
+
+def test_oracle_corsika_closed_forms():
+    """EventIoPhotonFactoryTest.cpp: a vertical bunch starts 1e3 m + c t
+    above its ground point (x, y in cm), wavelength |lambda| nm, direction
+    (cx, cy, -sqrt(1 - cx^2 - cy^2)); a photon moved back by that distance
+    along its direction lands on the ground point."""
+    b = np.array([[1.2, 3.4, 0.0, 0.0, 1e-9, 1e5, 1.0, -433.0],
+                  [150.0, -80.0, 0.3, -0.2, 12.5, 1e5, 1.0, 500.0]], np.float32)
+    ph, truth = O.corsika_photons(b)
+    assert truth.tolist() == [0, 1]
+    assert ph["support_x"][0] == pytest.approx(0.012, abs=1e-9)
+    assert ph["support_y"][0] == pytest.approx(0.034, abs=1e-9)
+    assert ph["support_z"][0] == pytest.approx(1e3 + 1e-18 * 299792458.0, abs=1e-9)
+    assert ph["wavelength"][0] == pytest.approx(433e-9, abs=1e-15)
+    cz = -np.sqrt(1 - 0.3 ** 2 - 0.2 ** 2)
+    assert (ph["direction_x"][1], ph["direction_y"][1], ph["direction_z"][1]) == pytest.approx((0.3, -0.2, cz), abs=1e-7)
+    dist = float(np.float32(12.5)) * 1e-9 * 299792458.0 + 1e3
+    for k, g0 in (("x", 1.5), ("y", -0.8)):
+        back = ph[f"support_{k}"][1] + ph[f"direction_{k}"][1] * dist
+        assert back == pytest.approx(g0, abs=1e-9)
+    assert ph["support_z"][1] + ph["direction_z"][1] * dist == pytest.approx(0.0, abs=1e-9)
+
+
+def test_oracle_corsika_weight_collapse_rate():
+    """EventIoPhotonFactoryTest.cpp: a bunch of weight w in [0, 1) yields a
+    photon with probability w (mean pass ratio 0.5 over w uniform on [0, 1)),
+    and a bunch of weight 15.5 yields 15 or 16 photons."""
+    n = 100_000
+    b = np.zeros((n, 8), np.float32)
+    b[:, 5] = 1e5
+    b[:, 7] = 433.0
+    b[:, 6] = np.arange(n, dtype=np.float64) / n
+    _, truth = O.corsika_photons(b, seed=3)
+    assert len(truth) / n == pytest.approx(0.5, abs=1e-2)
+    # per weight decile the pass ratio follows the weight
+    passed = np.bincount(truth, minlength=n).astype(bool)
+    for d in range(10):
+        sel = slice(d * n // 10, (d + 1) * n // 10)
+        assert passed[sel].mean() == pytest.approx((d + 0.5) / 10, abs=0.02)
+    b[:, 6] = 15.5
+    _, truth = O.corsika_photons(b[:1000], seed=4)
+    c = np.bincount(truth, minlength=1000)
+    assert set(np.unique(c)) <= {15, 16}
+    assert c.mean() == pytest.approx(15.5, abs=0.05)
