@@ -32,6 +32,13 @@ HBM_PEAK = 8.0e12
 # photons, default tuning), rocprofv3 FETCH_SIZE + WRITE_SIZE of one propagation
 # pass scaled by a 1 GiB copy: profiles/propagation_pass_fact_lg_1e7_counters.md
 TRAFFIC_BYTES_PER_PHOTON = 1494.0
+# VALU work of the same pass (profiles/propagation_pass_fact_lg_1e7_valu.md):
+# lane-FLOPs issued per photon and VALU wave instructions per photon
+VALU_FP32_FLOPS_PER_PHOTON = 7093.0
+VALU_FP64_FLOPS_PER_PHOTON = 3331.0
+VALU_INSTS_PER_PHOTON = 294.7
+FP32_PEAK, FP64_PEAK = 157.3e12, 78.6e12
+SIMDS, CLOCK_HZ = 256 * 4, 2.4e9
 IN_BYTES = 7 * 8            # support xyz, direction xyz, wavelength (float64)
 OUT_BYTES = 4 + 4 + 1 + 1 + 1 + 8 + 4 * 4 + 8  # frame, sensor, interaction, n, error, tof, x y cx cy, distance
 
@@ -279,6 +286,17 @@ def main() -> int:
             "algorithmic_bytes_per_photon": IN_BYTES + OUT_BYTES,
         },
     }
+    if default_workload:
+        sec = ms_kernel * 1e-3
+        line["valu"] = {
+            "fp32_tflops": VALU_FP32_FLOPS_PER_PHOTON * n / sec / 1e12,
+            "fp32_frac": VALU_FP32_FLOPS_PER_PHOTON * n / sec / FP32_PEAK,
+            "fp64_tflops": VALU_FP64_FLOPS_PER_PHOTON * n / sec / 1e12,
+            "fp64_frac": VALU_FP64_FLOPS_PER_PHOTON * n / sec / FP64_PEAK,
+            "issue_frac": VALU_INSTS_PER_PHOTON * n * 4 / (SIMDS * CLOCK_HZ * sec),
+            "source": "rocprofv3 SQ_INSTS_VALU_FLOPS_FP32/FP64 and SQ_INSTS_VALU per photon of this pass "
+                      "(profiles/propagation_pass_fact_lg_1e7_valu.md) over the measured pass time",
+        }
     if not args.no_hybrid:
         line["hybrid"] = hybrid_leg(desc, scene, photons, args.steps, args.warmup, args.hybrid_train_photons,
                                     args.hybrid_max_entropy)
