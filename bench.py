@@ -133,7 +133,8 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
 
     value = n * steps / wall
     cell_l1 = l1(hy.sum(axis=1), plain.sum(axis=1))
-    return {
+    trained = (sur, dom, cells)
+    return trained, {
         "value": value,
         "unit": "photons/s",
         "ms_per_step": wall / steps * 1e3,
@@ -150,6 +151,90 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
         "network": "MLP 5-256-256-256-256-57, bf16 on MFMA 32x32x16, fp32 accumulation",
         "mfma_tflops": value * 2 * 212_480 / 1e12,
         "mfma_frac_of_bf16_dense_peak": value * 2 * 212_480 / 2.5e15,
+    }
+
+
+def hybrid_stream_leg(scene: GpuScene, trained, dev, events: int = 4, bunches_per_event: int = 1024,
+                      nsb_per_event: int = 100_000_000, chunk: int = 1 << 24) -> dict:
+    """The full-scale hybrid stream on the FACT-like telescope: per event
+    1,024 shower-like bunches (Cherenkov rings of radius U[0.5, 1.3] deg,
+    width 0.1 deg, sizes log-uniform in [1e3, 1e6], lambda ~ lambda^-2 on
+    280-700 nm) and 1e8 night-sky photons (uniform within 2 deg of the axis,
+    lambda uniform on 300-600 nm); four events (about 1e9 photons).  The
+    same photons go through the plain propagator and through the surrogate
+    + fallback hybrid (both timed whole, photon generation included);
+    per-pixel and per-pixel-and-time histograms are compared, with the noise
+    floor from a second plain pass under other propagation seeds."""
+    from merlict_development_kit_amd import ArrivalBinning, histogram_arrivals
+    from merlict_development_kit_amd import surrogate as SG
+    from merlict_development_kit_amd.bunches import Bunch, generate_bunches
+    sur, dom, cells = trained
+    rng = np.random.default_rng(2025)
+    jobs = []  # ("bunches", [Bunch]) or ("nsb", source, n, seed, offset)
+    for ev in range(events):
+        radii = rng.uniform(0.5, 1.3, bunches_per_event)
+        sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e6), bunches_per_event)).astype(np.int64)
+        group, tot = [], 0
+        for k in range(bunches_per_event):
+            b = Bunch(sources.shower_ring(float(radii[k])), seed=10_000 * ev + k, num_photons=int(sizes[k]))
+            if tot + b.num_photons > chunk and group:
+                jobs.append(("bunches", group))
+                group, tot = [], 0
+            group.append(b)
+            tot += b.num_photons
+        if group:
+            jobs.append(("bunches", group))
+        for off in range(0, nsb_per_event, chunk):
+            jobs.append(("nsb", sources.night_sky(), min(chunk, nsb_per_event - off), 500 + ev, off))
+    total = sum(sum(b.num_photons for b in j[1]) if j[0] == "bunches" else j[2] for j in jobs)
+    sigma = dom.time_scale
+    binning = ArrivalBinning(1440, 16, dom.time_offset - 8 * sigma, sigma)
+
+    def photons_of(job):
+        if job[0] == "bunches":
+            return generate_bunches(job[1], device=dev)
+        return job[1].generate(job[2], seed=job[3], photon_id_offset=job[4], device=dev)
+
+    def run(method, seed_base):
+        counts = binning.empty(dev)
+        fallback, offset = 0, 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for j, job in enumerate(jobs):
+            ph = photons_of(job)
+            cfg = PropagationConfig(5, seed=seed_base + j, photon_id_offset=offset)
+            if method == "plain":
+                histogram_arrivals(scene.propagate(ph, cfg), binning, counts=counts)
+            else:
+                _, f = SG.propagate_hybrid(scene, sur, dom, cells, ph, binning, cfg, max_entropy=float("inf"),
+                                           seed=seed_base + j, counts=counts)
+                fallback += f
+            offset += len(ph)
+            del ph
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, counts.cpu().numpy().astype(np.float64), fallback
+
+    t_plain, plain, _ = run("plain", 0)
+    t_hy, hy, fallback = run("hybrid", 0)
+    _, again, _ = run("plain", 100_000)
+
+    def l1(a, b):
+        return float(np.abs(a - b).sum() / b.sum())
+
+    return {
+        "workload": f"{events} events x ({bunches_per_event} Cherenkov-ring bunches + {nsb_per_event:.0e} night-sky "
+                    "photons), FACT-like telescope with light guides",
+        "photons": int(total),
+        "plain_photons_per_s": total / t_plain,
+        "hybrid_photons_per_s": total / t_hy,
+        "speedup": t_plain / t_hy,
+        "fallback_fraction": fallback / total,
+        "pixel_l1": l1(hy.sum(axis=1), plain.sum(axis=1)),
+        "pixel_time_l1": l1(hy, plain),
+        "noise_floor_pixel_l1": l1(again.sum(axis=1), plain.sum(axis=1)),
+        "noise_floor_pixel_time_l1": l1(again, plain),
+        "detected_plain": float(plain.sum()),
+        "detected_hybrid": float(hy.sum()),
     }
 
 
@@ -199,6 +284,7 @@ def main() -> int:
                     help="run the library's autotuner on the bunch before timing (mlgpu_scene_autotune)")
     ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--no-corsika", action="store_true")
+    ap.add_argument("--no-stream", action="store_true", help="skip the full-scale hybrid stream (about 1e9 photons)")
     ap.add_argument("--hybrid-train-photons", type=int, default=1_000_000_000)
     ap.add_argument("--hybrid-max-entropy", type=float, default=float("inf"))
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -298,9 +384,11 @@ def main() -> int:
                       "(profiles/propagation_pass_fact_lg_1e7_valu.md) over the measured pass time",
         }
     if not args.no_hybrid:
-        line["hybrid"] = hybrid_leg(desc, scene, photons, args.steps, args.warmup, args.hybrid_train_photons,
-                                    args.hybrid_max_entropy)
+        trained, line["hybrid"] = hybrid_leg(desc, scene, photons, args.steps, args.warmup,
+                                             args.hybrid_train_photons, args.hybrid_max_entropy)
         line["hybrid"]["speedup_vs_plain"] = line["hybrid"]["value"] / value
+        if not args.no_stream:
+            line["hybrid_stream"] = hybrid_stream_leg(scene, trained, dev)
     if not args.no_corsika:
         line["corsika_ingestion"] = corsika_leg(dev)
     if not args.no_cpu_baseline:

@@ -297,18 +297,27 @@ enum mlgpu_direction {
     MLGPU_DIRECTION_AXIS = 0,      /* +z */
     MLGPU_DIRECTION_TILT = 1,      /* tilt ~ U[0, angle], azimuth ~ U[0, 2 pi) */
     MLGPU_DIRECTION_GAUSSIAN = 2,  /* tan of the x and y tilts ~ N(0, angle) */
-    MLGPU_DIRECTION_CAP = 3        /* uniform on the cap zenith <= angle */
+    MLGPU_DIRECTION_CAP = 3,       /* uniform on the cap zenith <= angle */
+    MLGPU_DIRECTION_RING = 4       /* zenith ~ |N(angle, angle_width)|, azimuth ~ U[0, 2 pi):
+                                      a Cherenkov ring of radius angle */
+};
+enum mlgpu_spectrum {
+    MLGPU_SPECTRUM_INVERSE_SQUARE = 0,  /* pdf ~ lambda^-2 on [lambda_min, lambda_max] */
+    MLGPU_SPECTRUM_UNIFORM = 1          /* uniform on [lambda_min, lambda_max] */
 };
 typedef struct mlgpu_source {
     int32_t support;        /* mlgpu_support */
     int32_t direction;      /* mlgpu_direction */
     double radius;          /* disc radius (m) */
-    double angle;           /* rad; TILT and CAP: [0, pi/2] */
+    double angle;           /* rad; TILT, CAP and RING: [0, pi/2] */
     double lambda_min;      /* m; lambda_min == lambda_max: monochromatic, */
-    double lambda_max;      /* else pdf ~ lambda^-2 on [lambda_min, lambda_max] */
+    double lambda_max;      /* else drawn by `spectrum` */
     double start_distance;  /* m */
     double rot[9];
     double pos[3];
+    double angle_width;     /* rad; RING: the zenith's standard deviation */
+    int32_t spectrum;       /* mlgpu_spectrum */
+    int32_t pad;
 } mlgpu_source;
 int mlgpu_generate_photons(const mlgpu_source* source, int64_t n, uint64_t seed,
                            uint64_t photon_id_offset, const mlgpu_photon_buffers* out,
@@ -354,6 +363,14 @@ typedef struct mlgpu_bunch {
     uint64_t seed;
     int64_t num_photons;
 } mlgpu_bunch;
+/* The photons of bunches, concatenated in bunch order into out (capacity
+ * photons): photon j of bunch b is photon j of its source under its seed,
+ * as mlgpu_generate_photons draws it with photon_id_offset 0.  The bunch
+ * table is host memory.  Synchronises `stream`.  No counterpart in the
+ * reference (its apps build each event's photons on the host). */
+int mlgpu_generate_bunches(const mlgpu_bunch* bunches, int32_t num_bunches,
+                           const mlgpu_photon_buffers* out, int64_t capacity,
+                           void* stream);
 int mlgpu_propagate_bunches(const mlgpu_scene* scene, int32_t world_frame,
                             const mlgpu_bunch* bunches, int32_t num_bunches,
                             const mlgpu_config* config,

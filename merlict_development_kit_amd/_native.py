@@ -91,6 +91,9 @@ class Source(C.Structure):
         ("start_distance", C.c_double),
         ("rot", C.c_double * 9),
         ("pos", C.c_double * 3),
+        ("angle_width", C.c_double),
+        ("spectrum", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 
@@ -205,6 +208,8 @@ SIGNATURES = {
     "mlgpu_propagate_hybrid": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SurrogateDomain), C.POINTER(HexCells),
                                          C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
                                          C.POINTER(HybridConfig), C.POINTER(Histogram), C.POINTER(C.c_int64),
+                                         C.c_void_p]),
+    "mlgpu_generate_bunches": (C.c_int, [C.POINTER(Bunch), C.c_int32, C.POINTER(PhotonBuffers), C.c_int64,
                                          C.c_void_p]),
     "mlgpu_corsika_photons": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(PhotonBuffers),
                                         C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]),

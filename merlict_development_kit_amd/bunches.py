@@ -48,3 +48,22 @@ def propagate_bunches(scene: GpuScene, bunches: Sequence[Bunch], config: Propaga
                                             C.byref(rs) if rs is not None else None,
                                             _stream_ptr(stream)), "mlgpu_propagate_bunches")
     return counts, res
+
+
+def generate_bunches(bunches: Sequence[Bunch], device="cuda", stream=None):
+    """The photons of the bunches, concatenated in bunch order
+    (mlgpu_generate_bunches): photon j of a bunch is photon j of its source
+    under its seed."""
+    from .propagation import PhotonBatch
+    device = torch.device(device)
+    arr = (N.Bunch * len(bunches))()
+    for k, b in enumerate(bunches):
+        arr[k].source = b.source.to_c()
+        arr[k].seed = b.seed
+        arr[k].num_photons = b.num_photons
+    total = sum(b.num_photons for b in bunches)
+    arrays = [torch.empty(total, dtype=torch.float64, device=device) for _ in range(7)]
+    buf = N.PhotonBuffers(*[a.data_ptr() if total else None for a in arrays])
+    N.check(N.lib().mlgpu_generate_bunches(arr, len(bunches), C.byref(buf), total, _stream_ptr(stream)),
+            "mlgpu_generate_bunches")
+    return PhotonBatch(*arrays)

@@ -103,13 +103,23 @@ int check_source(const mlgpu_source* src) {
         set_last_error("bad support kind");
         return MLGPU_ERR_ARGUMENT;
     }
-    if (src->direction < MLGPU_DIRECTION_AXIS || src->direction > MLGPU_DIRECTION_CAP) {
+    if (src->direction < MLGPU_DIRECTION_AXIS || src->direction > MLGPU_DIRECTION_RING) {
         set_last_error("bad direction kind");
         return MLGPU_ERR_ARGUMENT;
     }
-    const bool bounded = src->direction == MLGPU_DIRECTION_TILT || src->direction == MLGPU_DIRECTION_CAP;
+    if (src->spectrum != MLGPU_SPECTRUM_INVERSE_SQUARE && src->spectrum != MLGPU_SPECTRUM_UNIFORM) {
+        set_last_error("bad spectrum kind");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const bool bounded = src->direction == MLGPU_DIRECTION_TILT || src->direction == MLGPU_DIRECTION_CAP ||
+                         src->direction == MLGPU_DIRECTION_RING;
     if (!(src->angle >= 0.0) || (bounded && !(src->angle <= 1.57079632679489661923)) || !std::isfinite(src->angle)) {
-        set_last_error("angle must be finite and >= 0, and <= pi/2 for TILT and CAP");
+        set_last_error("angle must be finite and >= 0, and <= pi/2 for TILT, CAP and RING");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (src->direction == MLGPU_DIRECTION_RING &&
+        (!(src->angle_width >= 0.0) || !(src->angle + 8.0 * src->angle_width <= 1.57079632679489661923))) {
+        set_last_error("RING: need angle_width >= 0 and angle + 8 angle_width <= pi/2");
         return MLGPU_ERR_ARGUMENT;
     }
     if (!(src->radius >= 0.0) || !std::isfinite(src->radius) || !std::isfinite(src->start_distance)) {
@@ -516,6 +526,38 @@ int mlgpu_propagate_bunches(const mlgpu_scene* s, int32_t world_frame, const mlg
     rc = run_bunches(ms, s->scope_of_frame[world_frame], bunches, num_bunches, total, *cfg, hist, results, st);
     end_shared_use(ms, st);
     return rc;
+}
+
+int mlgpu_generate_bunches(const mlgpu_bunch* bunches, int32_t nb, const mlgpu_photon_buffers* out, int64_t capacity,
+                           void* stream) {
+    if (!bunches || !out || nb < 0 || capacity < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    int64_t total = 0;
+    std::vector<mlgpu::BunchDev> table(static_cast<size_t>(nb));
+    for (int32_t b = 0; b < nb; b++) {
+        if (bunches[b].num_photons < 0) { set_last_error("negative bunch size"); return MLGPU_ERR_ARGUMENT; }
+        const int rc = check_source(&bunches[b].source);
+        if (rc) return rc;
+        table[static_cast<size_t>(b)] = mlgpu::BunchDev{bunches[b].source, bunches[b].seed, total};
+        total += bunches[b].num_photons;
+    }
+    if (total > capacity) { set_last_error("capacity below the total of the bunches"); return MLGPU_ERR_ARGUMENT; }
+    if (total == 0) return MLGPU_OK;
+    if (!out->support_x || !out->support_y || !out->support_z || !out->direction_x || !out->direction_y ||
+        !out->direction_z || !out->wavelength) {
+        set_last_error("null photon array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    mlgpu::BunchDev* dev = nullptr;
+    const size_t bytes = sizeof(mlgpu::BunchDev) * table.size();
+    hipError_t e = hipMalloc(&dev, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc bunch table");
+    e = hipMemcpyAsync(dev, table.data(), bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = mlgpu::launch_generate_bunches(dev, nb, total, *out, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return hip_fail(e, "bunch generation");
+    return MLGPU_OK;
 }
 
 int mlgpu_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed, uint64_t photon_id_offset,

@@ -128,12 +128,32 @@ MLGPU_HD inline void source_photon(const mlgpu_source& src, const uint64_t seed,
         d = D3{sz * cp, sz * sp, cz};
         break;
     }
+    case MLGPU_DIRECTION_RING: {
+        // zenith angle + angle_width z, z standard normal (Box-Muller), folded
+        // at 0 (the validation bounds it below pi/2); azimuth from the
+        // wavelength block's second pair
+        const double r = sqrt(-2.0 * log_series(1.0 - ua));
+        double sb, cb;
+        sincos_turn(ub, sb, cb);
+        double zen = src.angle + src.angle_width * (r * cb);
+        if (zen < 0.0) zen = -zen;
+        if (zen > 1.57079632679489661923) zen = 1.57079632679489661923;
+        double st, ct;
+        sincos_series(zen, st, ct);
+        source_block(seed, id, 1u, w);
+        double sp, cp;
+        sincos_turn(u53(w[2], w[3]), sp, cp);
+        d = D3{st * cp, st * sp, ct};
+        break;
+    }
     default: d = D3{0.0, 0.0, 1.0};
     }
     // wavelength: pdf ~ lambda^-2 means 1 / lambda uniform
     source_block(seed, id, 1u, w);
     if (src.lambda_min == src.lambda_max) {
         lambda = src.lambda_min;
+    } else if (src.spectrum == MLGPU_SPECTRUM_UNIFORM) {
+        lambda = src.lambda_min + u53(w[0], w[1]) * (src.lambda_max - src.lambda_min);
     } else {
         const double a = 1.0 / src.lambda_min, b = 1.0 / src.lambda_max;
         lambda = 1.0 / (a - u53(w[0], w[1]) * (a - b));

@@ -34,7 +34,8 @@ from . import _native as N
 from .propagation import PhotonBatch
 
 SUPPORTS = {"point": 0, "disc": 1}
-DIRECTIONS = {"axis": 0, "tilt": 1, "gaussian": 2, "cap": 3}
+DIRECTIONS = {"axis": 0, "tilt": 1, "gaussian": 2, "cap": 3, "ring": 4}
+SPECTRA = {"inverse_square": 0, "uniform": 1}
 
 IDENTITY = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
 # rotation by pi about x: directions about +z become directions about -z
@@ -70,6 +71,8 @@ class Source:
     start_distance: float = 0.0
     rot: Tuple[float, ...] = IDENTITY
     pos: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    angle_width: float = 0.0
+    spectrum: str = "inverse_square"
 
     def to_c(self, struct_type=N.Source):
         """The fields as a ctypes struct laid out as mlgpu_source."""
@@ -85,6 +88,8 @@ class Source:
             s.rot[k] = self.rot[k]
         for k in range(3):
             s.pos[k] = self.pos[k]
+        s.angle_width = self.angle_width
+        s.spectrum = SPECTRA[self.spectrum]
         return s
 
     def posed(self, rot: Sequence[float] = IDENTITY, pos: Sequence[float] = (0.0, 0.0, 0.0)) -> "Source":
@@ -153,3 +158,21 @@ def point_like(opening_angle: float, wavelength: float = 433e-9) -> Source:
 def parallel_disc(disc_radius: float, wavelength: float = 433e-9) -> Source:
     """photon_source::parallel_towards_z_from_xy_disc."""
     return Source("disc", "axis", disc_radius, 0.0, wavelength, wavelength)
+
+
+def shower_ring(ring_radius_deg: float, width_deg: float = 0.1, disc_radius: float = 2.25, height: float = 10.0,
+                lambda_min: float = 280e-9, lambda_max: float = 700e-9) -> Source:
+    """A shower-like bunch for a telescope: supports on a disc at z =
+    height, directions on a Cherenkov ring about -z (zenith ~ N(radius,
+    width), azimuth uniform), lambda with pdf ~ lambda^-2."""
+    return Source("disc", "ring", disc_radius, math.radians(ring_radius_deg), lambda_min, lambda_max, 0.0,
+                  FLIP, (0.0, 0.0, height), angle_width=math.radians(width_deg))
+
+
+def night_sky(field_radius_deg: float = 2.0, disc_radius: float = 2.25, height: float = 10.0,
+              lambda_min: float = 300e-9, lambda_max: float = 600e-9) -> Source:
+    """Night-sky background for a telescope: supports on a disc at z =
+    height, directions uniform on the cap within field_radius of -z, lambda
+    uniform."""
+    return Source("disc", "cap", disc_radius, math.radians(field_radius_deg), lambda_min, lambda_max, 0.0,
+                  FLIP, (0.0, 0.0, height), spectrum="uniform")

@@ -922,10 +922,23 @@ void oracle_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed,
             const double sz = sqrt(1.0 - cz * cz);
             oracle_sincos_turn(u1, &sp, &cp);
             dx = sz * cp; dy = sz * sp; dz = cz;
+        } else if (src->direction == MLGPU_DIRECTION_RING) {
+            const double rad = sqrt(-2.0 * oracle_log_series(1.0 - u0));
+            double sb, cb, st, ct, sp, cp;
+            oracle_sincos_turn(u1, &sb, &cb);
+            double zen = src->angle + src->angle_width * (rad * cb);
+            if (zen < 0.0) zen = -zen;
+            if (zen > 1.57079632679489661923) zen = 1.57079632679489661923;
+            src_sincos(zen, &st, &ct);
+            src_block(seed, id, 1u, w);
+            oracle_sincos_turn(src_u53(w[2], w[3]), &sp, &cp);
+            dx = st * cp; dy = st * sp; dz = ct;
         }
         src_block(seed, id, 1u, w);
         if (src->lambda_min == src->lambda_max) {
             lambda = src->lambda_min;
+        } else if (src->spectrum == MLGPU_SPECTRUM_UNIFORM) {
+            lambda = src->lambda_min + src_u53(w[0], w[1]) * (src->lambda_max - src->lambda_min);
         } else {
             const double inv_lo = 1.0 / src->lambda_min, inv_hi = 1.0 / src->lambda_max;
             lambda = 1.0 / (inv_lo - src_u53(w[0], w[1]) * (inv_lo - inv_hi));

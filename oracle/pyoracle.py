@@ -45,7 +45,8 @@ class Source(C.Structure):
     _fields_ = [
         ("support", C.c_int32), ("direction", C.c_int32), ("radius", C.c_double), ("angle", C.c_double),
         ("lambda_min", C.c_double), ("lambda_max", C.c_double), ("start_distance", C.c_double),
-        ("rot", C.c_double * 9), ("pos", C.c_double * 3),
+        ("rot", C.c_double * 9), ("pos", C.c_double * 3), ("angle_width", C.c_double), ("spectrum", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 

@@ -22,6 +22,8 @@ SOURCES = {
     "point_like": S.point_like(math.radians(10.0)),
     "parallel_disc": S.parallel_disc(0.3),
     "posed_bunch": S.fact_bunch().posed(S.rotation_xyz(0.02, -0.03, 1.0), (0.5, -0.25, 0.0)),
+    "shower_ring": S.shower_ring(0.9),
+    "night_sky": S.night_sky(),
 }
 
 
@@ -81,6 +83,7 @@ def test_gpu_bunches_equal_separate_draws_and_oracle(cuda, oracle_threads):
     oracle's), the same histogram as binning those results."""
     from merlict_development_kit_amd import (ArrivalBinning, Bunch, GpuScene, PhotonBatch, PropagationConfig,
                                              SceneDescription, propagate_bunches)
+    from merlict_development_kit_amd.bunches import generate_bunches
     desc = SceneDescription.fact(light_guides=True)
     gs = GpuScene(desc)
     bunches = [Bunch(S.fact_bunch().posed(S.rotation_xyz(0.01 * k, -0.02 * k, 0.0)), seed=40 + k, num_photons=m)
@@ -95,6 +98,9 @@ def test_gpu_bunches_equal_separate_draws_and_oracle(cuda, oracle_threads):
                             ("support_x", "support_y", "support_z", "direction_x", "direction_y", "direction_z",
                              "wavelength")])
         again = gs.propagate(cat, cfg).numpy()
+        gen = generate_bunches(bunches, device=cuda).numpy()
+        for f in O.PHOTON_FIELDS:
+            np.testing.assert_array_equal(gen[f], getattr(cat, f).cpu().numpy(), err_msg=f"generate_bunches {f}")
         got = res.numpy()
         for k in got:
             np.testing.assert_array_equal(got[k], again[k], err_msg=f"{schedule} {k}")

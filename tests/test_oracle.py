@@ -316,6 +316,25 @@ def test_source_directions_follow_their_laws():
     assert sp_stats.kstest(az, "uniform").pvalue > 0.01
 
 
+def test_source_ring_and_uniform_spectrum_follow_their_laws():
+    """Ring: the zenith is N(radius, width) (no folding at these sizes), the
+    azimuth uniform and independent of it; uniform spectrum: lambda U[lo, hi]."""
+    n = 200_000
+    r, w = np.radians(1.0), np.radians(0.1)
+    g = O.generate_photons(_src(direction="ring", angle=r, angle_width=w), n, seed=8)
+    zen = np.arccos(np.clip(g["direction_z"], -1, 1))
+    assert sp_stats.kstest((zen - r) / w, "norm").pvalue > 0.01
+    az = (np.arctan2(g["direction_y"], g["direction_x"]) + np.pi) / (2 * np.pi)
+    assert sp_stats.kstest(az, "uniform").pvalue > 0.01
+    assert abs(np.corrcoef(az, zen)[0, 1]) < 0.01
+    norm = np.sqrt(g["direction_x"] ** 2 + g["direction_y"] ** 2 + g["direction_z"] ** 2)
+    np.testing.assert_allclose(norm, 1.0, rtol=1e-15)
+    lo, hi = 300e-9, 600e-9
+    u = O.generate_photons(_src(lambda_min=lo, lambda_max=hi, spectrum="uniform"), n, seed=9)
+    assert u["wavelength"].min() >= lo and u["wavelength"].max() <= hi
+    assert sp_stats.kstest((u["wavelength"] - lo) / (hi - lo), "uniform").pvalue > 0.01
+
+
 def test_source_wavelength_inverse_square_law():
     lo, hi = 280e-9, 700e-9
     ph = O.generate_photons(_src(lambda_min=lo, lambda_max=hi), 200_000, seed=7)
