@@ -238,6 +238,63 @@ def hybrid_stream_leg(scene: GpuScene, trained, dev, events: int = 4, bunches_pe
     }
 
 
+def plenoscope_bunches_leg(dev, batches: int = 4, bunches_per_batch: int = 256) -> dict:
+    """Batched bunches on the plenoscope (1,860 facets, 8,443 lenslets,
+    515,023 lixels): per batch 256 bunches with sizes log-uniform in
+    [1e3, 1e6], each pointed up to 3 deg off axis, lambda ~ lambda^-2 on
+    280-700 nm, at most 10 interactions, binned per lixel into 64 time bins
+    of 1 ns with a N(0, 5 ns) arrival jitter.  The same batches run through
+    mlgpu_propagate_bunches (eager launches) and through one bunch plan (a
+    hipGraph replayed per batch); the histograms must agree exactly."""
+    from merlict_development_kit_amd import ArrivalBinning, Bunch, propagate_bunches
+    from merlict_development_kit_amd.bunches import BunchPlan
+    scene = GpuScene(SceneDescription.plenoscope())
+    rng = np.random.default_rng(7)
+    plan_batches = []
+    for b in range(batches):
+        sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e6), bunches_per_batch)).astype(np.int64)
+        off = np.radians(3.0) * np.sqrt(rng.uniform(0, 1, bunches_per_batch))
+        phi = rng.uniform(0, 2 * np.pi, bunches_per_batch)
+        plan_batches.append([Bunch(sources.plenoscope_bunch(float(off[k] * np.cos(phi[k])),
+                                                            float(off[k] * np.sin(phi[k]))),
+                                   seed=1000 * b + k, num_photons=int(sizes[k])) for k in range(bunches_per_batch)])
+    totals = [sum(x.num_photons for x in bb) for bb in plan_batches]
+    # time window from a probe batch
+    probe = scene.propagate(sources.plenoscope_bunch().generate(100_000, seed=1, device=dev), PropagationConfig(10))
+    det = probe.sensor_id >= 0
+    t0 = float(probe.time_of_flight[det].mean())
+    binning = ArrivalBinning(515_023, 64, t0 - 32e-9, 1e-9, jitter_sigma=5e-9, seed=3)
+    eager = binning.empty(dev)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for b, bb in enumerate(plan_batches):
+        propagate_bunches(scene, bb, PropagationConfig(10, seed=b, photon_id_offset=0), binning, counts=eager,
+                          device=dev)
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t
+    plan = BunchPlan(scene, bunches_per_batch, max(totals), PropagationConfig(10, seed=0), binning, device=dev)
+    plan.run(plan_batches[0], PropagationConfig(10, seed=0))  # warm-up replay
+    torch.cuda.synchronize()
+    plan.counts.zero_()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for b, bb in enumerate(plan_batches):
+        plan.run(bb, PropagationConfig(10, seed=b, photon_id_offset=0))
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t
+    total = sum(totals)
+    return {
+        "workload": f"plenoscope, {batches} batches x {bunches_per_batch} bunches (log-uniform 1e3-1e6 photons, "
+                    "pointing <= 3 deg, 5 ns jitter), <= 10 interactions, 515,023 lixels x 64 time bins",
+        "photons": int(total),
+        "eager_photons_per_s": total / t_eager,
+        "graph_photons_per_s": total / t_graph,
+        "histograms_equal": bool(torch.equal(eager, plan.counts)),
+        "detected": int(eager.sum().item()),
+        "tuning": scene.tuning(),
+    }
+
+
 def corsika_leg(dev, n: int = 10_000_000, reps: int = 5) -> dict:
     """CORSIKA photon-bunch ingestion (mlgpu_corsika_photons): n synthetic
     bunches (weights uniform on [0, 2]) to photons, whole call timed
@@ -284,6 +341,7 @@ def main() -> int:
                     help="run the library's autotuner on the bunch before timing (mlgpu_scene_autotune)")
     ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--no-corsika", action="store_true")
+    ap.add_argument("--no-plenoscope", action="store_true")
     ap.add_argument("--no-stream", action="store_true", help="skip the full-scale hybrid stream (about 1e9 photons)")
     ap.add_argument("--hybrid-train-photons", type=int, default=1_000_000_000)
     ap.add_argument("--hybrid-max-entropy", type=float, default=float("inf"))
@@ -389,6 +447,8 @@ def main() -> int:
         line["hybrid"]["speedup_vs_plain"] = line["hybrid"]["value"] / value
         if not args.no_stream:
             line["hybrid_stream"] = hybrid_stream_leg(scene, trained, dev)
+    if not args.no_plenoscope:
+        line["plenoscope_bunches"] = plenoscope_bunches_leg(dev)
     if not args.no_corsika:
         line["corsika_ingestion"] = corsika_leg(dev)
     if not args.no_cpu_baseline:

@@ -377,6 +377,32 @@ int mlgpu_propagate_bunches(const mlgpu_scene* scene, int32_t world_frame,
                             const mlgpu_histogram* hist, mlgpu_results* results,
                             void* stream);
 
+/* Bunch plans: the launch sequence of mlgpu_propagate_bunches (bunch table
+ * up, photon draw, WAVEFRONT propagation with the scene's tuning at
+ * creation, histogram into hist when it is not NULL) captured once in a
+ * hipGraph for batches of up to max_bunches bunches and max_photons photons,
+ * and replayed per batch: the kernels read the batch's photon count, seed
+ * and photon id offset from a device parameter block that each run uploads
+ * with the bunch table from pinned memory, so batches of any size up to the
+ * capacities replay the same graph.  A run gives the results (and histogram
+ * counts, which accumulate) that mlgpu_propagate_bunches gives for the same
+ * batch and config; its max_num_interactions_per_photon must be the plan's.
+ * Runs on one plan are serialised (a run waits for the previous one before
+ * rewriting the staging buffer).  hist->counts and the scene must outlive
+ * the plan.  mlgpu_bunch_plan_results gives the device arrays holding the
+ * last run's per-photon results (valid once the run has finished).  No
+ * counterpart in the reference. */
+typedef struct mlgpu_bunch_plan mlgpu_bunch_plan;
+int mlgpu_bunch_plan_create(const mlgpu_scene* scene, int32_t world_frame,
+                            int32_t max_bunches, int64_t max_photons,
+                            const mlgpu_config* config,
+                            const mlgpu_histogram* hist, mlgpu_bunch_plan** out);
+int mlgpu_bunch_plan_run(mlgpu_bunch_plan* plan, const mlgpu_bunch* bunches,
+                         int32_t num_bunches, const mlgpu_config* config,
+                         void* stream);
+int mlgpu_bunch_plan_results(const mlgpu_bunch_plan* plan, mlgpu_results* out);
+void mlgpu_bunch_plan_destroy(mlgpu_bunch_plan* plan);
+
 /* ------------------------------------------------------------- surrogate
  * A learned surrogate of the propagator: an MLP 5 -> 256 -> 256 -> 256 ->
  * 256 -> 57 from a photon's (support x, y, direction cx, cy, wavelength),

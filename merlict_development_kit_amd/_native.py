@@ -209,6 +209,11 @@ SIGNATURES = {
                                          C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
                                          C.POINTER(HybridConfig), C.POINTER(Histogram), C.POINTER(C.c_int64),
                                          C.c_void_p]),
+    "mlgpu_bunch_plan_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int64, C.POINTER(Config),
+                                          C.POINTER(Histogram), C.POINTER(C.c_void_p)]),
+    "mlgpu_bunch_plan_run": (C.c_int, [C.c_void_p, C.POINTER(Bunch), C.c_int32, C.POINTER(Config), C.c_void_p]),
+    "mlgpu_bunch_plan_results": (C.c_int, [C.c_void_p, C.POINTER(Results)]),
+    "mlgpu_bunch_plan_destroy": (None, [C.c_void_p]),
     "mlgpu_generate_bunches": (C.c_int, [C.POINTER(Bunch), C.c_int32, C.POINTER(PhotonBuffers), C.c_int64,
                                          C.c_void_p]),
     "mlgpu_corsika_photons": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(PhotonBuffers),

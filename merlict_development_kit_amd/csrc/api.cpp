@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -210,19 +211,12 @@ int launch_fused(const mlgpu_scene* s, int32_t scope, const mlgpu_photons& ph, i
     return MLGPU_OK;
 }
 
-// Wavefront propagation; the caller holds ws_mutex and began shared use.
-int launch_wavefront(mlgpu_scene* ms, int32_t scope, const mlgpu_photons& ph, int64_t n, const mlgpu_config& cfg,
-                     const mlgpu_results& r, hipStream_t st) {
-    if (ms->ws_capacity < n) {
-        const int64_t cap = n + n / 8;
-        const int rc = grow(ms, &ms->ws, mlgpu::workspace_bytes(cap));
-        ms->ws_capacity = rc ? 0 : cap;
-        if (rc) return rc;
-    }
-    const size_t cap = static_cast<size_t>(ms->ws_capacity);
-    Carver c{static_cast<char*>(ms->ws)};
+// The wavefront workspace of cap photons laid out in mem (workspace_bytes(cap)).
+mlgpu::Workspace carve_workspace(void* mem, int64_t capacity) {
+    const size_t cap = static_cast<size_t>(capacity);
+    Carver c{static_cast<char*>(mem)};
     mlgpu::Workspace w;
-    w.cap = ms->ws_capacity;
+    w.cap = capacity;
     w.sx = c.take<double>(cap);
     w.sy = c.take<double>(cap);
     w.sz = c.take<double>(cap);
@@ -243,6 +237,19 @@ int launch_wavefront(mlgpu_scene* ms, int32_t scope, const mlgpu_photons& ph, in
     w.queue[0] = c.take<int32_t>(cap);
     w.queue[1] = c.take<int32_t>(cap);
     w.counts = c.take<int32_t>(mlgpu::kNumCounts);
+    return w;
+}
+
+// Wavefront propagation; the caller holds ws_mutex and began shared use.
+int launch_wavefront(mlgpu_scene* ms, int32_t scope, const mlgpu_photons& ph, int64_t n, const mlgpu_config& cfg,
+                     const mlgpu_results& r, hipStream_t st) {
+    if (ms->ws_capacity < n) {
+        const int64_t cap = n + n / 8;
+        const int rc = grow(ms, &ms->ws, mlgpu::workspace_bytes(cap));
+        ms->ws_capacity = rc ? 0 : cap;
+        if (rc) return rc;
+    }
+    const mlgpu::Workspace w = carve_workspace(ms->ws, ms->ws_capacity);
     const hipError_t e = mlgpu::launch_propagate_wavefront(
         ms->dev, scope, ph, n, cfg.max_num_interactions_per_photon, cfg.seed, cfg.photon_id_offset, r, w,
         ms->tuning.blocks_per_cu * ms->num_cus, ms->tuning.trace_budget, st);
@@ -695,3 +702,157 @@ int mlgpu_first_hit(const mlgpu_scene* s, int32_t root_frame, const mlgpu_photon
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------- bunch plans
+// A captured launch sequence for repeated batches of bunches: bunch table
+// and call parameters up from pinned staging, photon draw, wavefront
+// propagation, histogram.  The kernels read the batch's photon count, seed
+// and id offset from the parameter block, so one graph serves every batch
+// up to the plan's capacity.
+struct mlgpu_bunch_plan {
+    const mlgpu_scene* scene = nullptr;
+    int32_t max_bunches = 0;
+    int64_t max_photons = 0;
+    uint32_t max_interactions = 0;
+    void* mem = nullptr;       // device: parameters + table, photons, results, workspace
+    char* staging = nullptr;   // pinned: parameters + table
+    size_t head_bytes = 0;
+    mlgpu_results results{};
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool launched = false;
+    std::mutex mutex;
+};
+
+namespace {
+
+size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+void plan_free(mlgpu_bunch_plan* p) {
+    if (p->done) {
+        if (p->launched) (void)hipEventSynchronize(p->done);
+        (void)hipEventDestroy(p->done);
+    }
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->capture) (void)hipStreamDestroy(p->capture);
+    if (p->staging) (void)hipHostFree(p->staging);
+    if (p->mem) (void)hipFree(p->mem);
+    delete p;
+}
+
+}  // namespace
+
+int mlgpu_bunch_plan_create(const mlgpu_scene* s, int32_t world_frame, int32_t max_bunches, int64_t max_photons,
+                            const mlgpu_config* cfg, const mlgpu_histogram* hist, mlgpu_bunch_plan** out) {
+    if (!s || !cfg || !out || max_bunches < 1 || max_photons < 1 || max_photons > INT32_MAX) {
+        set_last_error("null argument, or capacities outside [1, 2^31 - 1]");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    int rc;
+    if ((rc = check_world(s, world_frame)) || (rc = check_config(cfg))) return rc;
+    if (hist && (rc = check_histogram(hist))) return rc;
+    auto* p = new mlgpu_bunch_plan();
+    p->scene = s;
+    p->max_bunches = max_bunches;
+    p->max_photons = max_photons;
+    p->max_interactions = cfg->max_num_interactions_per_photon;
+    const size_t cap = static_cast<size_t>(max_photons);
+    p->head_bytes = round256(sizeof(mlgpu::CallParams)) + round256(sizeof(mlgpu::BunchDev) * max_bunches);
+    const size_t photon_bytes = 7 * round256(cap * 8);
+    const size_t result_bytes = 2 * round256(cap * 4) + 3 * round256(cap) + round256(cap * 8) + 4 * round256(cap * 4);
+    const size_t bytes = p->head_bytes + photon_bytes + result_bytes + mlgpu::workspace_bytes(max_photons);
+    hipError_t e = hipMalloc(&p->mem, bytes);
+    if (e != hipSuccess) { p->mem = nullptr; plan_free(p); return hip_fail(e, "hipMalloc bunch plan"); }
+    e = hipHostMalloc(reinterpret_cast<void**>(&p->staging), p->head_bytes);
+    if (e != hipSuccess) { p->staging = nullptr; plan_free(p); return hip_fail(e, "hipHostMalloc bunch plan"); }
+    std::memset(p->staging, 0, p->head_bytes);
+    Carver c{static_cast<char*>(p->mem)};
+    auto* params = reinterpret_cast<mlgpu::CallParams*>(c.p);
+    auto* table = reinterpret_cast<mlgpu::BunchDev*>(c.p + round256(sizeof(mlgpu::CallParams)));
+    c.p += p->head_bytes;
+    mlgpu_photon_buffers gen{c.take<double>(cap), c.take<double>(cap), c.take<double>(cap), c.take<double>(cap),
+                             c.take<double>(cap), c.take<double>(cap), c.take<double>(cap)};
+    p->results = mlgpu_results{c.take<int32_t>(cap), c.take<int32_t>(cap), c.take<uint8_t>(cap), c.take<uint8_t>(cap),
+                               c.take<uint8_t>(cap), c.take<double>(cap), c.take<float>(cap), c.take<float>(cap),
+                               c.take<float>(cap), c.take<float>(cap), nullptr};
+    const mlgpu::Workspace ws = carve_workspace(c.p, max_photons);
+    const mlgpu_photons ph{gen.support_x, gen.support_y, gen.support_z, gen.direction_x, gen.direction_y,
+                           gen.direction_z, gen.wavelength};
+    e = hipEventCreateWithFlags(&p->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking);
+    if (e != hipSuccess) { plan_free(p); return hip_fail(e, "bunch plan stream"); }
+    e = hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) { plan_free(p); return hip_fail(e, "hipStreamBeginCapture"); }
+    hipError_t le = hipMemcpyAsync(params, p->staging, p->head_bytes, hipMemcpyHostToDevice, p->capture);
+    if (le == hipSuccess) le = mlgpu::launch_generate_bunches(table, max_bunches, max_photons, gen, p->capture, params);
+    if (le == hipSuccess)
+        le = mlgpu::launch_propagate_wavefront(s->dev, s->scope_of_frame[world_frame], ph, max_photons,
+                                               p->max_interactions, cfg->seed, cfg->photon_id_offset, p->results, ws,
+                                               s->tuning.blocks_per_cu * s->num_cus, s->tuning.trace_budget,
+                                               p->capture, params);
+    if (le == hipSuccess && hist) le = mlgpu::launch_histogram(p->results, max_photons, *hist, s->num_cus, p->capture,
+                                                                params);
+    e = hipStreamEndCapture(p->capture, &p->graph);
+    if (le != hipSuccess || e != hipSuccess) {
+        if (e == hipSuccess) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+        plan_free(p);
+        return hip_fail(le != hipSuccess ? le : e, "bunch plan capture");
+    }
+    e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) { p->exec = nullptr; plan_free(p); return hip_fail(e, "hipGraphInstantiate"); }
+    *out = p;
+    return MLGPU_OK;
+}
+
+int mlgpu_bunch_plan_run(mlgpu_bunch_plan* p, const mlgpu_bunch* bunches, int32_t nb, const mlgpu_config* cfg,
+                         void* stream) {
+    if (!p || !cfg || nb < 0 || (nb > 0 && !bunches)) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (nb > p->max_bunches) { set_last_error("more bunches than the plan holds"); return MLGPU_ERR_ARGUMENT; }
+    if (cfg->max_num_interactions_per_photon != p->max_interactions) {
+        set_last_error("max_num_interactions_per_photon differs from the plan's");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    int64_t total = 0;
+    for (int32_t b = 0; b < nb; b++) {
+        if (bunches[b].num_photons < 0) { set_last_error("negative bunch size"); return MLGPU_ERR_ARGUMENT; }
+        const int rc = check_source(&bunches[b].source);
+        if (rc) return rc;
+        total += bunches[b].num_photons;
+    }
+    if (total > p->max_photons) { set_last_error("more photons than the plan holds"); return MLGPU_ERR_ARGUMENT; }
+    if (total == 0) return MLGPU_OK;
+    std::lock_guard<std::mutex> lock(p->mutex);
+    // the previous replay has read the staging buffer once it has finished
+    if (p->launched) (void)hipEventSynchronize(p->done);
+    auto* cp = reinterpret_cast<mlgpu::CallParams*>(p->staging);
+    cp->n = total;
+    cp->seed = cfg->seed;
+    cp->id_offset = cfg->photon_id_offset;
+    cp->num_bunches = nb;
+    auto* table = reinterpret_cast<mlgpu::BunchDev*>(p->staging + round256(sizeof(mlgpu::CallParams)));
+    int64_t start = 0;
+    for (int32_t b = 0; b < nb; b++) {
+        table[b] = mlgpu::BunchDev{bunches[b].source, bunches[b].seed, start};
+        start += bunches[b].num_photons;
+    }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = hipGraphLaunch(p->exec, st);
+    if (e == hipSuccess) e = hipEventRecord(p->done, st);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    p->launched = true;
+    return MLGPU_OK;
+}
+
+int mlgpu_bunch_plan_results(const mlgpu_bunch_plan* p, mlgpu_results* out) {
+    if (!p || !out) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    *out = p->results;
+    return MLGPU_OK;
+}
+
+void mlgpu_bunch_plan_destroy(mlgpu_bunch_plan* p) {
+    if (p) plan_free(p);
+}

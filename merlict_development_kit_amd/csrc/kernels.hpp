@@ -26,6 +26,18 @@ struct Workspace {
     int32_t* counts;     // [level]: live photons; [kDeferredCounts + level]: deferred slots
 };
 
+// Parameters a captured launch sequence (mlgpu_bunch_plan) reads from
+// device memory at each replay: the photon count, the propagation seed and
+// photon id offset, the number of bunches.  Kernels given a null pointer
+// use their arguments.
+struct CallParams {
+    int64_t n;
+    uint64_t seed;
+    uint64_t id_offset;
+    int32_t num_bunches;
+    int32_t pad;
+};
+
 constexpr int kDeferredCounts = 256;
 constexpr int kNumCounts = 2 * kDeferredCounts;
 
@@ -40,10 +52,12 @@ hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const ml
                             uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                             const mlgpu_results& out, hipStream_t stream);
 
+// n (and seed, id_offset) size the grids; with cp, the kernels take the
+// photon count, seed and id offset from *cp (n is then the capacity)
 hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& out, const Workspace& ws, int grid_blocks,
-                                      int32_t trace_budget, hipStream_t stream);
+                                      int32_t trace_budget, hipStream_t stream, const CallParams* cp = nullptr);
 
 hipError_t launch_first_hit(const DeviceScene& sc, int32_t root_scope, const mlgpu_photons& rays, int64_t n,
                             const mlgpu_hits& out, hipStream_t stream);
@@ -57,13 +71,14 @@ struct BunchDev {
 };
 
 hipError_t launch_generate_bunches(const BunchDev* table, int32_t num_bunches, int64_t total,
-                                   const mlgpu_photon_buffers& out, hipStream_t stream);
+                                   const mlgpu_photon_buffers& out, hipStream_t stream,
+                                   const CallParams* cp = nullptr);
 
 hipError_t launch_generate(const mlgpu_source& src, int64_t n, uint64_t seed, uint64_t id_offset,
                            const mlgpu_photon_buffers& out, hipStream_t stream);
 
 hipError_t launch_histogram(const mlgpu_results& r, int64_t n, const mlgpu_histogram& h, int num_cus,
-                            hipStream_t stream);
+                            hipStream_t stream, const CallParams* cp = nullptr);
 
 hipError_t launch_count_errors(const uint8_t* err, int64_t n, unsigned long long* counter, hipStream_t stream);
 

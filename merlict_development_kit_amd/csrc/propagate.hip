@@ -95,8 +95,10 @@ __global__ void wf_reset_kernel(int32_t* counts, int n) {
 }
 
 __global__ __launch_bounds__(kBlock) void wf_init_kernel(const int32_t world_scope, const mlgpu_photons in,
-                                                         const int64_t n, const uint32_t max_interactions,
-                                                         const Workspace ws, const mlgpu_results out) {
+                                                         const int64_t n_arg, const uint32_t max_interactions,
+                                                         const Workspace ws, const mlgpu_results out,
+                                                         const CallParams* __restrict__ cp) {
+    const int64_t n = cp ? cp->n : n_arg;
     __shared__ TileQueue<kTile> q;
     if (threadIdx.x == 0) q.n = 0;
     __syncthreads();
@@ -205,8 +207,11 @@ __global__ __launch_bounds__(kBlock) void wf_resume_kernel(const DeviceScene sc,
 
 __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, const mlgpu_photons in,
                                                           const Workspace ws, const int level,
-                                                          const uint32_t max_interactions, const uint64_t seed,
-                                                          const uint64_t id_offset, const mlgpu_results out) {
+                                                          const uint32_t max_interactions, const uint64_t seed_arg,
+                                                          const uint64_t id_offset_arg, const mlgpu_results out,
+                                                          const CallParams* __restrict__ cp) {
+    const uint64_t seed = cp ? cp->seed : seed_arg;
+    const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
     __shared__ TileQueue<kTile> q;
     if (threadIdx.x == 0) q.n = 0;
     __syncthreads();
@@ -302,7 +307,7 @@ hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const ml
 hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& out, const Workspace& ws, int grid_blocks,
-                                      int32_t trace_budget, hipStream_t stream) {
+                                      int32_t trace_budget, hipStream_t stream, const CallParams* cp) {
     if (n <= 0) return hipSuccess;
     const int levels = static_cast<int>(max_interactions) + 1;
     if (levels + 1 > kDeferredCounts) return hipErrorInvalidValue;
@@ -312,7 +317,7 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
     const int64_t blocks = (n + kBlock - 1) / kBlock;
     const int grid = static_cast<int>(blocks < grid_blocks ? blocks : grid_blocks);
     hipLaunchKernelGGL(wf_init_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, world_scope, in, n,
-                       max_interactions, ws, out);
+                       max_interactions, ws, out, cp);
     for (int level = 0; level < static_cast<int>(max_interactions); level++) {
         const int32_t budget = trace_budget > 0 ? trace_budget : INT32_MAX;
         hipLaunchKernelGGL(wf_trace_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, ws, level, budget);
@@ -321,7 +326,7 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
             hipLaunchKernelGGL(wf_resume_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, ws, level);
         }
         hipLaunchKernelGGL(wf_shade_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, sc, in, ws, level,
-                           max_interactions, seed, id_offset, out);
+                           max_interactions, seed, id_offset, out, cp);
     }
     return hipGetLastError();
 }

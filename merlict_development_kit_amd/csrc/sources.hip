@@ -26,10 +26,14 @@ __global__ __launch_bounds__(256) void generate_kernel(const mlgpu_source src, c
 // Photon i of the concatenation of a call's bunches: its bunch is the last
 // one starting at or before i.
 __global__ __launch_bounds__(256) void generate_bunches_kernel(const BunchDev* __restrict__ table,
-                                                               const int32_t num_bunches, const int64_t total,
-                                                               const mlgpu_photon_buffers out) {
+                                                               const int32_t num_bunches_arg,
+                                                               const int64_t total_arg,
+                                                               const mlgpu_photon_buffers out,
+                                                               const CallParams* __restrict__ cp) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+    const int64_t total = cp ? cp->n : total_arg;
+    const int32_t num_bunches = cp ? cp->num_bunches : num_bunches_arg;
+    if (i >= total || num_bunches <= 0) return;
     int32_t lo = 0, hi = num_bunches - 1;
     while (lo < hi) {
         const int32_t mid = (lo + hi + 1) >> 1;
@@ -51,11 +55,11 @@ __global__ __launch_bounds__(256) void generate_bunches_kernel(const BunchDev* _
 }  // namespace
 
 hipError_t launch_generate_bunches(const BunchDev* table, int32_t num_bunches, int64_t total,
-                                   const mlgpu_photon_buffers& out, hipStream_t stream) {
+                                   const mlgpu_photon_buffers& out, hipStream_t stream, const CallParams* cp) {
     if (total <= 0) return hipSuccess;
     const int64_t grid = (total + 255) / 256;
     hipLaunchKernelGGL(generate_bunches_kernel, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, stream, table,
-                       num_bunches, total, out);
+                       num_bunches, total, out, cp);
     return hipGetLastError();
 }
 

@@ -115,3 +115,46 @@ def test_gpu_bunches_equal_separate_draws_and_oracle(cuda, oracle_threads):
     counts2, none = propagate_bunches(gs, bunches, cfg, binning, device=cuda)
     assert none is None
     np.testing.assert_array_equal(counts2.cpu().numpy(), counts.cpu().numpy())
+
+
+def test_gpu_bunch_plan_replays_match_eager(cuda):
+    """A bunch plan (one hipGraph, replayed per batch with the batch's size,
+    seed and id offset read from device memory) gives, replay after replay,
+    the histograms and per-photon results of mlgpu_propagate_bunches on the
+    same batch; other work runs eagerly between the replays."""
+    from merlict_development_kit_amd import (ArrivalBinning, Bunch, GpuScene, PropagationConfig, SceneDescription,
+                                             propagate_bunches)
+    from merlict_development_kit_amd._native import MlgpuError
+    from merlict_development_kit_amd.bunches import BunchPlan
+    gs = GpuScene(SceneDescription.fact(light_guides=True))
+    binning = ArrivalBinning(1440, 16, 48e-9, 1e-9, jitter_sigma=1e-9, seed=2, photon_id_offset=5)
+    plan = BunchPlan(gs, max_bunches=8, max_photons=300_000, config=PropagationConfig(5, seed=1), binning=binning,
+                     device=cuda)
+    batches = [
+        [Bunch(S.fact_bunch().posed(S.rotation_xyz(0.01, 0.0, 0.0)), seed=3, num_photons=120_000),
+         Bunch(S.shower_ring(0.8), seed=4, num_photons=50_001)],
+        [Bunch(S.night_sky(), seed=5, num_photons=7)],
+        [Bunch(S.fact_bunch(), seed=6 + k, num_photons=37_000 + 1000 * k) for k in range(7)],
+    ]
+    for rep in range(2):
+        for j, batch in enumerate(batches):
+            cfg = PropagationConfig(5, seed=100 + j, photon_id_offset=1000 * j)
+            want, res = propagate_bunches(gs, batch, cfg, binning, want_results=True, device=cuda)
+            torch.cuda.synchronize()
+            plan.counts.zero_()
+            n = plan.run(batch, cfg)
+            torch.cuda.synchronize()
+            assert torch.equal(plan.counts, want), (rep, j)
+            got = plan.results(n).numpy()
+            ref = res.numpy()
+            for k in got:
+                if k != "accumulated_distance":
+                    np.testing.assert_array_equal(got[k], ref[k], err_msg=f"replay {rep}, batch {j}, {k}")
+            # eager work between replays
+            gs.propagate(S.fact_bunch().generate(20_000, seed=j, device=cuda), PropagationConfig(5, seed=j))
+    with pytest.raises(MlgpuError):
+        plan.run(batches[2] + batches[0], PropagationConfig(5))
+    with pytest.raises(MlgpuError):
+        plan.run(batches[1], PropagationConfig(4))
+    with pytest.raises(MlgpuError):
+        plan.run([Bunch(S.fact_bunch(), seed=1, num_photons=300_001)], PropagationConfig(5))
