@@ -295,6 +295,53 @@ def plenoscope_bunches_leg(dev, batches: int = 4, bunches_per_batch: int = 256) 
     }
 
 
+def signal_leg(result, photons, dev, reps: int = 5) -> dict:
+    """The sensor signal chain on the detected photons of one FACT-like
+    propagation pass (1440 pixels): pipelines (sorted by arrival time),
+    photo-electric conversion (a quantum efficiency table, 1e7 Hz dark rate
+    over 100 ns, 5 % crosstalk) and pulse extraction (0.5 ns slices,
+    0.4 ns spread), each timed on the GPU; the oracle (C, one thread) times
+    the same chain on the host."""
+    from merlict_development_kit_amd.signal_chain import extract_pulses, photoelectric_convert, sensor_pipelines
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    qx, qy = np.array([200e-9, 300e-9, 400e-9, 500e-9, 700e-9, 1200e-9]), np.array([0.0, 0.2, 0.35, 0.3, 0.1, 0.0])
+    args = (1e7, 0.05, 1e-7)
+    times = {"pipelines": 0.0, "convert": 0.0, "extract": 0.0}
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pipes = sensor_pipelines(result.sensor_id, result.error, result.time_of_flight, photons.wavelength, 1440)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pulses = photoelectric_convert(pipes, 1440, qx, qy, *args, seed=r)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        extract_pulses(pulses, 1440, 0.5e-9, 0.4e-9, seed=r)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if r > 0:
+            times["pipelines"] += (t1 - t0) / reps
+            times["convert"] += (t2 - t1) / reps
+            times["extract"] += (t3 - t2) / reps
+    n = len(result.sensor_id)
+    m = len(pipes.arrival_time)
+    sample = min(n, 2_000_000)
+    res = {k: getattr(result, k)[:sample].cpu().numpy() for k in ("sensor_id", "error", "time_of_flight")}
+    lam = photons.wavelength[:sample].cpu().numpy()
+    c0 = time.perf_counter()
+    off, t, l, tr = O.sensor_pipelines(res["sensor_id"], res["error"], res["time_of_flight"], lam, None, 1440)
+    po, pt, _ = O.photoelectric(off, t, l, tr, 1440, qx, qy, *args, 0)
+    O.extract_pulses(po, pt, 1440, 0.5e-9, 0.4e-9, 0)
+    cpu = time.perf_counter() - c0
+    gpu = sum(times.values())
+    return {"photons": n, "detected": m, "pulses": len(pulses.arrival_time),
+            "ms": {k: v * 1e3 for k, v in times.items()},
+            "value": n / gpu, "unit": "photons/s (whole chain on one pass's results)",
+            "cpu_baseline": {"value": sample / cpu, "unit": "photons/s", "cores": 1, "kind": "port",
+                             "sample": f"the first {sample} photon results, oracle restatement"}}
+
+
 def corsika_leg(dev, n: int = 10_000_000, reps: int = 5) -> dict:
     """CORSIKA photon-bunch ingestion (mlgpu_corsika_photons): n synthetic
     bunches (weights uniform on [0, 2]) to photons, whole call timed
@@ -342,6 +389,7 @@ def main() -> int:
     ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--no-corsika", action="store_true")
     ap.add_argument("--no-plenoscope", action="store_true")
+    ap.add_argument("--no-signal", action="store_true")
     ap.add_argument("--no-stream", action="store_true", help="skip the full-scale hybrid stream (about 1e9 photons)")
     ap.add_argument("--hybrid-train-photons", type=int, default=1_000_000_000)
     ap.add_argument("--hybrid-max-entropy", type=float, default=float("inf"))
@@ -447,6 +495,8 @@ def main() -> int:
         line["hybrid"]["speedup_vs_plain"] = line["hybrid"]["value"] / value
         if not args.no_stream:
             line["hybrid_stream"] = hybrid_stream_leg(scene, trained, dev)
+    if not args.no_signal:
+        line["signal_chain"] = signal_leg(out, photons, dev)
     if not args.no_plenoscope:
         line["plenoscope_bunches"] = plenoscope_bunches_leg(dev)
     if not args.no_corsika:

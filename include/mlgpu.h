@@ -538,6 +538,70 @@ int mlgpu_corsika_photons(const float* bunches, int64_t n, uint64_t seed,
                           int32_t* truth_id, int64_t capacity,
                           int64_t* num_photons, void* stream);
 
+/* ------------------------------------------------------- signal chain
+ * Per-sensor photon pipelines (merlict_signal_processing/PipelinePhoton.cpp:
+ * 25-66): the detected photons (sensor_id in [0, num_sensors), no error
+ * flag) of n results, grouped by sensor and sorted by arrival time (ties
+ * by photon index): pipeline s is entries offsets[s] .. offsets[s + 1] - 1
+ * of arrival_time, wavelength and truth_id (the photon's truth id, or its
+ * index when truth_id is NULL).  All device arrays; offsets has
+ * num_sensors + 1 entries.  *num_photons (host) receives the number of
+ * detected photons; a capacity below it is an MLGPU_ERR_ARGUMENT.
+ * Synchronises `stream`. */
+typedef struct mlgpu_pipelines {
+    int64_t* offsets;
+    double* arrival_time;
+    double* wavelength;
+    int32_t* truth_id;
+    int64_t capacity;
+} mlgpu_pipelines;
+int mlgpu_sensor_pipelines(const mlgpu_results* results, const double* wavelength,
+                           const int32_t* truth_id, int64_t n, int32_t num_sensors,
+                           const mlgpu_pipelines* out, int64_t* num_photons,
+                           void* stream);
+
+/* Photo-electric conversion of every pipeline (PhotoElectricConverter.cpp:
+ * 10-95): a photon becomes a pulse when the quantum efficiency at its
+ * wavelength (a Func1 table, host arrays qe_x, qe_y) is >= a uniform; each
+ * pulse is followed by crosstalk pulses (truth -202) while crosstalk >= a
+ * uniform; then accidental pulses (truth -201) at the times of a Poisson
+ * process of dark_rate over [0, exposure_time).  Photon j of the pipelines
+ * draws its uniforms from Philox stream 6 of (seed, j) and sensor s its
+ * accidental pulses from stream 8 of (seed, s), each in the reference's
+ * order.  Pulses of sensor s (its photons' pulses in pipeline order, then
+ * its accidental pulses) are entries offsets[s] .. offsets[s + 1] - 1 of
+ * out; *num_pulses (host)
+ * receives their number.  A wavelength outside the table (the reference's
+ * std::out_of_range), crosstalk >= 1 or a capacity below the number of
+ * pulses is an MLGPU_ERR_ARGUMENT.  Synchronises `stream`. */
+typedef struct mlgpu_pulses {
+    int64_t* offsets;
+    double* arrival_time;
+    int32_t* truth_id;
+    int64_t capacity;
+} mlgpu_pulses;
+typedef struct mlgpu_converter {
+    const double* qe_x;
+    const double* qe_y;
+    int32_t qe_n;
+    double dark_rate;
+    double crosstalk;
+    double exposure_time;
+    uint64_t seed;
+} mlgpu_converter;
+int mlgpu_photoelectric_convert(const mlgpu_pipelines* pipelines, int32_t num_sensors,
+                                const mlgpu_converter* converter, const mlgpu_pulses* out,
+                                int64_t* num_pulses, void* stream);
+
+/* Single-photon extraction (pulse_extraction.cpp:9-30): the time slice
+ * round((t + N(0, arrival_time_std)) / time_slice_duration) of every pulse,
+ * -1 when outside [0, 100); pulse k draws from Philox stream 7 of
+ * (seed, k).  slice (device) has one entry per pulse.  Reads the pulse
+ * count (offsets[num_sensors]) back, so it synchronises `stream`. */
+int mlgpu_extract_pulses(const mlgpu_pulses* pulses, int32_t num_sensors,
+                         double time_slice_duration, double arrival_time_std,
+                         uint64_t seed, int16_t* slice, void* stream);
+
 /* Number of photons with a non-zero error flag in the last n results
  * (synchronises `stream`). */
 int mlgpu_count_errors(const mlgpu_results* results, int64_t n, void* stream,

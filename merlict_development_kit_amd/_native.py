@@ -144,6 +144,21 @@ class Results(C.Structure):
         "x", "y", "cx", "cy", "accumulated_distance")]
 
 
+class Pipelines(C.Structure):
+    _fields_ = [("offsets", C.c_void_p), ("arrival_time", C.c_void_p), ("wavelength", C.c_void_p),
+                ("truth_id", C.c_void_p), ("capacity", C.c_int64)]
+
+
+class Pulses(C.Structure):
+    _fields_ = [("offsets", C.c_void_p), ("arrival_time", C.c_void_p), ("truth_id", C.c_void_p),
+                ("capacity", C.c_int64)]
+
+
+class Converter(C.Structure):
+    _fields_ = [("qe_x", C.c_void_p), ("qe_y", C.c_void_p), ("qe_n", C.c_int32), ("dark_rate", C.c_double),
+                ("crosstalk", C.c_double), ("exposure_time", C.c_double), ("seed", C.c_uint64)]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("max_num_interactions_per_photon", C.c_uint32),
@@ -216,6 +231,12 @@ SIGNATURES = {
     "mlgpu_bunch_plan_destroy": (None, [C.c_void_p]),
     "mlgpu_generate_bunches": (C.c_int, [C.POINTER(Bunch), C.c_int32, C.POINTER(PhotonBuffers), C.c_int64,
                                          C.c_void_p]),
+    "mlgpu_sensor_pipelines": (C.c_int, [C.POINTER(Results), C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                         C.POINTER(Pipelines), C.POINTER(C.c_int64), C.c_void_p]),
+    "mlgpu_photoelectric_convert": (C.c_int, [C.POINTER(Pipelines), C.c_int32, C.POINTER(Converter),
+                                              C.POINTER(Pulses), C.POINTER(C.c_int64), C.c_void_p]),
+    "mlgpu_extract_pulses": (C.c_int, [C.POINTER(Pulses), C.c_int32, C.c_double, C.c_double, C.c_uint64, C.c_void_p,
+                                       C.c_void_p]),
     "mlgpu_corsika_photons": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(PhotonBuffers),
                                         C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]),
     "mlgpu_histogram_arrivals": (C.c_int, [C.POINTER(Results), C.c_int64, C.POINTER(Histogram), C.c_void_p]),

@@ -1,0 +1,423 @@
+// The sensor signal chain on the GPU: per-sensor photon pipelines sorted by
+// arrival time (merlict_signal_processing/PipelinePhoton.cpp:25-66), the
+// photo-electric converter (PhotoElectricConverter.cpp:10-95: quantum
+// efficiency, crosstalk chain, dark-rate accidental pulses) and the
+// single-photon pulse extraction (pulse_extraction.cpp:9-30).
+//
+// The reference draws every random number of an event from one generator
+// in sensor order, which serialises the chain.  Here each photon of the
+// sorted pipelines (its conversion and crosstalk chain), each sensor (its
+// accidental pulses) and each pulse (its extraction jitter) owns a Philox
+// stream, consumed one 24-bit uniform at a time in the reference's order, so
+// every photon, sensor and pulse is a lane of its own.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "kernels.hpp"
+#include "propagate_core.hpp"
+#include "signal.hpp"
+
+namespace mlgpu {
+namespace {
+
+constexpr uint32_t kConverterStream = 6;
+constexpr uint32_t kExtractionStream = 7;
+constexpr uint32_t kAccidentalStream = 8;
+constexpr int32_t kCrosstalk = -202;   // simulation_truth.h:17
+constexpr int32_t kAccidental = -201;  // simulation_truth.h:16
+constexpr int32_t kNumTimeSlices = 100;  // pulse_extraction.h:13
+constexpr int32_t kMaxChain = 1 << 20;
+
+// sequential 24-bit uniforms of one Philox stream (as the reference's
+// generator is driven through random::Generator::uniform)
+struct StreamRng {
+    uint32_t k0, k1, id_lo, id_hi, stream, next;
+    uint32_t b[4];
+    __device__ void init(uint64_t seed, uint64_t id, uint32_t s) {
+        k0 = static_cast<uint32_t>(seed);
+        k1 = static_cast<uint32_t>(seed >> 32);
+        id_lo = static_cast<uint32_t>(id);
+        id_hi = static_cast<uint32_t>(id >> 32);
+        stream = s;
+        next = 0;
+    }
+    __device__ double uniform() {
+        if ((next & 3u) == 0u) philox4x32_10(next >> 2, stream, id_lo, id_hi, k0, k1, b[0], b[1], b[2], b[3]);
+        const uint32_t w = b[next & 3u];
+        next++;
+        return static_cast<double>(w >> 8) * (1.0 / 16777216.0);
+    }
+    // Generator::expovariate (merlict/random/Generator.cpp:36-39)
+    __device__ double expovariate(double rate) { return -log(uniform()) / rate; }
+    // normal(mean, std) as the reference's Philox generator draws it:
+    // Box-Muller on 1 - u1 and u2
+    __device__ double normal(double mean, double std_dev) {
+        const double u1 = 1.0 - uniform(), u2 = uniform();
+        return mean + std_dev * sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+    }
+};
+
+// Func1::evaluate (merlict/function/Func1.cpp:66-71) on a table sorted by
+// x with the slopes computed before sorting; false outside [x0, xN)
+__device__ bool table_eval(const FuncPoint* p, int32_t n, double x, double& y) {
+    if (!(x >= p[0].x) || !(x < p[n - 1].x)) return false;
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (p[mid].x > x) hi = mid; else lo = mid + 1;
+    }
+    const FuncPoint p0 = p[lo - 1];
+    y = p0.slope * x + p0.y - p0.slope * p0.x;
+    return true;
+}
+
+// order-preserving map of a double to an unsigned key
+__device__ __forceinline__ uint64_t time_key(double t) {
+    const uint64_t b = __double_as_longlong(t);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__global__ void select_kernel(const mlgpu_results r, const int64_t n, const int32_t num_sensors,
+                              uint32_t* __restrict__ sensor_key, uint64_t* __restrict__ tkey,
+                              int32_t* __restrict__ index, int32_t* __restrict__ count) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool det = false;
+    if (i < n) {
+        const int32_t s = r.sensor_id[i];
+        det = s >= 0 && s < num_sensors && r.error[i] == 0;
+    }
+    const uint64_t mask = __ballot(det);
+    if (mask == 0) return;
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, leader);
+    if (det) {
+        const int k = base + __popcll(mask & ((1ull << lane) - 1ull));
+        sensor_key[k] = static_cast<uint32_t>(r.sensor_id[i]);
+        tkey[k] = time_key(r.time_of_flight[i]);
+        index[k] = static_cast<int32_t>(i);
+    }
+}
+
+// sensor of each photon in time order
+__global__ void sensor_of_kernel(const int32_t* __restrict__ order, const mlgpu_results r, const int64_t m,
+                                 uint32_t* __restrict__ sk) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < m) sk[k] = static_cast<uint32_t>(r.sensor_id[order[k]]);
+}
+
+// offsets[s] = first position of sensor s in the sorted keys (s <= num_sensors)
+__global__ void offsets_kernel(const uint32_t* __restrict__ sorted_sensor, const int64_t m, const int32_t num_sensors,
+                               int64_t* __restrict__ offsets) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s > num_sensors) return;
+    int64_t lo = 0, hi = m;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (static_cast<int64_t>(sorted_sensor[mid]) < s) lo = mid + 1; else hi = mid;
+    }
+    offsets[s] = lo;
+}
+
+__global__ void gather_kernel(const int32_t* __restrict__ index, const int64_t m, const mlgpu_results r,
+                              const double* __restrict__ wavelength, const int32_t* __restrict__ truth,
+                              const mlgpu_pipelines out) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int32_t i = index[k];
+    out.arrival_time[k] = r.time_of_flight[i];
+    out.wavelength[k] = wavelength[i];
+    out.truth_id[k] = truth ? truth[i] : i;
+}
+
+// The converter's random numbers: photon j of the sorted pipelines draws
+// its quantum-efficiency uniform and its crosstalk chain from Philox stream
+// 6 of (seed, j); sensor s draws its accidental pulses from stream 8 of
+// (seed, s).  Every photon and every sensor is then independent.
+
+// pulses of photon j (0 or 1 + its crosstalk chain); *bad for a wavelength
+// outside the table (Func1 throws std::out_of_range, Limits.cpp:37-46)
+__device__ int32_t photon_pulses(const mlgpu_pipelines& p, const int64_t j, const FuncPoint* qe, const int32_t qe_n,
+                                 const double crosstalk, const uint64_t seed, bool& bad) {
+    StreamRng rng;
+    rng.init(seed, static_cast<uint64_t>(j), kConverterStream);
+    double q;
+    bad = !table_eval(qe, qe_n, p.wavelength[j], q);
+    if (bad || !(q >= rng.uniform())) return 0;
+    // Converter::add_pulse: a crosstalk pulse follows while p >= u (the
+    // chain is cut at kMaxChain, which p < 1 - 1e-6 makes unreachable in
+    // practice)
+    int32_t k = 1;
+    for (int32_t c = 0; c < kMaxChain && crosstalk >= rng.uniform(); c++) k++;
+    return k;
+}
+
+__global__ void photon_count_kernel(const mlgpu_pipelines p, const int64_t m, const FuncPoint* __restrict__ qe,
+                                    const int32_t qe_n, const double crosstalk, const uint64_t seed,
+                                    int64_t* __restrict__ count, int32_t* __restrict__ bad) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    bool b = false;
+    count[j] = photon_pulses(p, j, qe, qe_n, crosstalk, seed, b);
+    if (b) atomicOr(bad, 1);
+}
+
+// Converter::add_accidental_pulse: the sensor's accidental pulses, each
+// with its crosstalk chain; with Emit they are written from `at` on
+template <bool Emit>
+__device__ int64_t accidentals(const uint64_t seed, const int32_t s, const double dark_rate, const double crosstalk,
+                               const double exposure_time, const mlgpu_pulses& out, const int64_t at) {
+    StreamRng rng;
+    rng.init(seed, static_cast<uint64_t>(s), kAccidentalStream);
+    int64_t k = 0;
+    double sum = rng.expovariate(dark_rate);
+    while (sum < exposure_time) {
+        const double next = rng.expovariate(dark_rate);
+        if (Emit) {
+            out.arrival_time[at + k] = sum;
+            out.truth_id[at + k] = kAccidental;
+        }
+        k++;
+        for (int32_t c = 0; c < kMaxChain && crosstalk >= rng.uniform(); c++) {
+            if (Emit) {
+                out.arrival_time[at + k] = sum;
+                out.truth_id[at + k] = kCrosstalk;
+            }
+            k++;
+        }
+        sum += next;
+    }
+    return k;
+}
+
+// per sensor: its photons' pulses (from the scan of the photon counts) plus
+// its accidental pulses
+__global__ void sensor_count_kernel(const mlgpu_pipelines p, const int32_t num_sensors,
+                                    const int64_t* __restrict__ photon_scan, const uint64_t seed,
+                                    const double dark_rate, const double crosstalk, const double exposure_time,
+                                    int64_t* __restrict__ sensor_count) {
+    const int32_t s = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= num_sensors) return;
+    const int64_t own = photon_scan[p.offsets[s + 1]] - photon_scan[p.offsets[s]];
+    sensor_count[s] = own + accidentals<false>(seed, s, dark_rate, crosstalk, exposure_time, mlgpu_pulses{}, 0);
+}
+
+// sensor of pipeline entry j (binary search of the offsets)
+__device__ __forceinline__ int32_t sensor_of(const int64_t* offsets, const int32_t num_sensors, const int64_t j) {
+    int32_t lo = 0, hi = num_sensors - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (offsets[mid] <= j) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void photon_emit_kernel(const mlgpu_pipelines p, const int64_t m, const int32_t num_sensors,
+                                   const FuncPoint* __restrict__ qe, const int32_t qe_n, const double crosstalk,
+                                   const uint64_t seed, const int64_t* __restrict__ photon_scan,
+                                   const mlgpu_pulses out) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    bool b = false;
+    const int32_t k = photon_pulses(p, j, qe, qe_n, crosstalk, seed, b);
+    if (k == 0) return;
+    const int32_t s = sensor_of(p.offsets, num_sensors, j);
+    const int64_t at = out.offsets[s] + photon_scan[j] - photon_scan[p.offsets[s]];
+    out.arrival_time[at] = p.arrival_time[j];
+    out.truth_id[at] = p.truth_id[j];
+    for (int32_t c = 1; c < k; c++) {
+        out.arrival_time[at + c] = p.arrival_time[j];
+        out.truth_id[at + c] = kCrosstalk;
+    }
+}
+
+__global__ void accidental_emit_kernel(const mlgpu_pipelines p, const int32_t num_sensors,
+                                       const int64_t* __restrict__ photon_scan, const uint64_t seed,
+                                       const double dark_rate, const double crosstalk, const double exposure_time,
+                                       const mlgpu_pulses out) {
+    const int32_t s = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= num_sensors) return;
+    const int64_t at = out.offsets[s] + photon_scan[p.offsets[s + 1]] - photon_scan[p.offsets[s]];
+    (void)accidentals<true>(seed, s, dark_rate, crosstalk, exposure_time, out, at);
+}
+
+// extract_pulses (pulse_extraction.cpp:9-30) for pulse k, with the normal
+// from Philox stream 7 of (seed, k): slice = round((t + N(0, std)) /
+// duration), -1 outside [0, 100)
+__global__ void extract_kernel(const mlgpu_pulses p, const int64_t count, const double duration,
+                               const double std_dev, const uint64_t seed, int16_t* __restrict__ slice) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    StreamRng rng;
+    rng.init(seed, static_cast<uint64_t>(k), kExtractionStream);
+    const double t = p.arrival_time[k] + rng.normal(0.0, std_dev);
+    const int32_t sl = static_cast<int32_t>(round(t / duration));
+    slice[k] = (sl >= 0 && sl < kNumTimeSlices) ? static_cast<int16_t>(sl) : static_cast<int16_t>(-1);
+}
+
+size_t align_up(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+}  // namespace
+
+hipError_t build_pipelines(const mlgpu_results& r, const double* wavelength, const int32_t* truth, int64_t n,
+                           int32_t num_sensors, const mlgpu_pipelines& out, int64_t* num_photons,
+                           hipStream_t stream, int* too_small) {
+    *num_photons = 0;
+    *too_small = 0;
+    const size_t cap = static_cast<size_t>(n);
+    // keys and values, doubled for the sort's alternate buffers
+    size_t temp_bytes = 0, temp2 = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, temp_bytes, static_cast<uint64_t*>(nullptr),
+                                             static_cast<uint64_t*>(nullptr), static_cast<int32_t*>(nullptr),
+                                             static_cast<int32_t*>(nullptr), cap, 0, 64, stream);
+    if (e != hipSuccess) return e;
+    e = rocprim::radix_sort_pairs(nullptr, temp2, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+                                  static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr), cap, 0, 32, stream);
+    if (e != hipSuccess) return e;
+    temp_bytes = temp_bytes > temp2 ? temp_bytes : temp2;
+    const size_t bytes = 2 * align_up(cap * 4) + 2 * align_up(cap * 8) + 2 * align_up(cap * 4) + align_up(temp_bytes) +
+                         256;
+    char* mem = nullptr;
+    e = hipMalloc(&mem, bytes);
+    if (e != hipSuccess) return e;
+    char* q = mem;
+    auto take = [&](size_t b) { char* x = q; q += align_up(b); return x; };
+    auto* skey = reinterpret_cast<uint32_t*>(take(cap * 4));
+    auto* skey2 = reinterpret_cast<uint32_t*>(take(cap * 4));
+    auto* tkey = reinterpret_cast<uint64_t*>(take(cap * 8));
+    auto* tkey2 = reinterpret_cast<uint64_t*>(take(cap * 8));
+    auto* idx = reinterpret_cast<int32_t*>(take(cap * 4));
+    auto* idx2 = reinterpret_cast<int32_t*>(take(cap * 4));
+    void* temp = take(temp_bytes);
+    auto* cnt = reinterpret_cast<int32_t*>(take(4));
+    int32_t m32 = 0;
+    e = hipMemsetAsync(cnt, 0, 4, stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(select_kernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, stream, r, n,
+                           num_sensors, skey, tkey, idx, cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&m32, cnt, 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    const size_t m = static_cast<size_t>(m32);
+    *num_photons = m32;
+    if (e == hipSuccess && static_cast<int64_t>(m) > out.capacity) *too_small = 1;
+    if (e == hipSuccess && !*too_small && m > 0) {
+        // by time, then stably by sensor: (sensor, time, photon index)
+        size_t tb = temp_bytes;
+        e = rocprim::radix_sort_pairs(temp, tb, tkey, tkey2, idx, idx2, m, 0, 64, stream);
+        // the sensor keys follow the time order through a gather by index:
+        // sort (sensor of idx2[k], idx2[k]) stably by sensor
+        if (e == hipSuccess) {
+            // sensor key of each time-sorted photon
+            hipLaunchKernelGGL(sensor_of_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream,
+                               idx2, r, static_cast<int64_t>(m), skey);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) {
+            tb = temp_bytes;
+            e = rocprim::radix_sort_pairs(temp, tb, skey, skey2, idx2, idx, m, 0, 32, stream);
+        }
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(offsets_kernel, dim3(static_cast<uint32_t>((num_sensors + 1 + 255) / 256)), dim3(256),
+                               0, stream, skey2, static_cast<int64_t>(m), num_sensors, out.offsets);
+            hipLaunchKernelGGL(gather_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream,
+                               idx, static_cast<int64_t>(m), r, wavelength, truth, out);
+            e = hipGetLastError();
+        }
+    } else if (e == hipSuccess && !*too_small) {
+        hipLaunchKernelGGL(offsets_kernel, dim3(static_cast<uint32_t>((num_sensors + 1 + 255) / 256)), dim3(256), 0,
+                           stream, skey2, static_cast<int64_t>(0), num_sensors, out.offsets);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(mem);
+    return e;
+}
+
+hipError_t convert_pulses(const mlgpu_pipelines& p, int32_t num_sensors, int64_t num_photons, const FuncPoint* qe,
+                          int32_t qe_n, double dark_rate, double crosstalk, double exposure_time, uint64_t seed,
+                          const mlgpu_pulses& out, int64_t* num_pulses, int* bad_wavelength, int* too_small,
+                          hipStream_t stream) {
+    *num_pulses = 0;
+    *bad_wavelength = 0;
+    *too_small = 0;
+    const size_t ns = static_cast<size_t>(num_sensors), m = static_cast<size_t>(num_photons);
+    size_t scan_a = 0, scan_b = 0;
+    hipError_t e = rocprim::exclusive_scan(nullptr, scan_a, static_cast<int64_t*>(nullptr),
+                                           static_cast<int64_t*>(nullptr), int64_t{0}, m + 1,
+                                           rocprim::plus<int64_t>(), stream);
+    if (e == hipSuccess)
+        e = rocprim::exclusive_scan(nullptr, scan_b, static_cast<int64_t*>(nullptr), static_cast<int64_t*>(nullptr),
+                                    int64_t{0}, ns + 1, rocprim::plus<int64_t>(), stream);
+    if (e != hipSuccess) return e;
+    const size_t scan_bytes = scan_a > scan_b ? scan_a : scan_b;
+    const size_t bytes = 2 * align_up((m + 1) * 8) + align_up((ns + 1) * 8) + align_up(scan_bytes) + 256;
+    char* mem = nullptr;
+    e = hipMalloc(&mem, bytes);
+    if (e != hipSuccess) return e;
+    char* q = mem;
+    auto take = [&](size_t b) { char* x = q; q += align_up(b); return x; };
+    auto* pcount = reinterpret_cast<int64_t*>(take((m + 1) * 8));
+    auto* pscan = reinterpret_cast<int64_t*>(take((m + 1) * 8));
+    auto* scount = reinterpret_cast<int64_t*>(take((ns + 1) * 8));
+    void* temp = take(scan_bytes);
+    auto* bad = reinterpret_cast<int32_t*>(take(4));
+    int32_t host_bad = 0;
+    int64_t total = 0;
+    e = hipMemsetAsync(pcount, 0, (m + 1) * 8, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(scount, 0, (ns + 1) * 8, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(bad, 0, 4, stream);
+    if (e == hipSuccess && m > 0) {
+        hipLaunchKernelGGL(photon_count_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream, p,
+                           num_photons, qe, qe_n, crosstalk, seed, pcount, bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = rocprim::exclusive_scan(temp, scan_a, pcount, pscan, int64_t{0}, m + 1, rocprim::plus<int64_t>(), stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(sensor_count_kernel, dim3(static_cast<uint32_t>((ns + 63) / 64)), dim3(64), 0, stream, p,
+                           num_sensors, pscan, seed, dark_rate, crosstalk, exposure_time, scount);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = rocprim::exclusive_scan(temp, scan_b, scount, out.offsets, int64_t{0}, ns + 1, rocprim::plus<int64_t>(),
+                                    stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, out.offsets + ns, 8, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&host_bad, bad, 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) {
+        *num_pulses = total;
+        *bad_wavelength = host_bad;
+        if (total > out.capacity) *too_small = 1;
+    }
+    if (e == hipSuccess && !*too_small && !host_bad && total > 0) {
+        if (m > 0)
+            hipLaunchKernelGGL(photon_emit_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream,
+                               p, num_photons, num_sensors, qe, qe_n, crosstalk, seed, pscan, out);
+        hipLaunchKernelGGL(accidental_emit_kernel, dim3(static_cast<uint32_t>((ns + 63) / 64)), dim3(64), 0, stream,
+                           p, num_sensors, pscan, seed, dark_rate, crosstalk, exposure_time, out);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    }
+    (void)hipFree(mem);
+    return e;
+}
+
+hipError_t extract_pulses(const mlgpu_pulses& p, int64_t count, double duration, double std_dev, uint64_t seed,
+                          int16_t* slice, hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(extract_kernel, dim3(static_cast<uint32_t>((count + 255) / 256)), dim3(256), 0, stream, p,
+                       count, duration, std_dev, seed, slice);
+    return hipGetLastError();
+}
+
+}  // namespace mlgpu

@@ -1,0 +1,102 @@
+"""The sensor signal chain on the GPU (merlict_signal_processing):
+per-sensor photon pipelines sorted by arrival time, the photo-electric
+converter and the single-photon pulse extraction (include/mlgpu.h)."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+
+def _st(stream, dev):
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    return C.c_void_p(s.cuda_stream)
+
+
+@dataclass
+class Pipelines:
+    """Pipeline s: entries offsets[s] .. offsets[s + 1] - 1."""
+    offsets: torch.Tensor        # int64, num_sensors + 1
+    arrival_time: torch.Tensor   # float64
+    wavelength: torch.Tensor     # float64
+    truth_id: torch.Tensor       # int32
+
+    def c_struct(self) -> N.Pipelines:
+        return N.Pipelines(self.offsets.data_ptr(), self.arrival_time.data_ptr(), self.wavelength.data_ptr(),
+                           self.truth_id.data_ptr(), len(self.arrival_time))
+
+
+@dataclass
+class Pulses:
+    offsets: torch.Tensor        # int64, num_sensors + 1
+    arrival_time: torch.Tensor   # float64
+    truth_id: torch.Tensor       # int32
+
+    def c_struct(self) -> N.Pulses:
+        return N.Pulses(self.offsets.data_ptr(), self.arrival_time.data_ptr(), self.truth_id.data_ptr(),
+                        len(self.arrival_time))
+
+
+def sensor_pipelines(sensor_id: torch.Tensor, error: torch.Tensor, time_of_flight: torch.Tensor,
+                     wavelength: torch.Tensor, num_sensors: int, truth_id: Optional[torch.Tensor] = None,
+                     stream=None) -> Pipelines:
+    """Per-sensor pipelines of the detected photons (mlgpu_sensor_pipelines);
+    the arrays are per photon, on the GPU (a PropagationResult's sensor_id,
+    error and time_of_flight with the photons' wavelengths)."""
+    dev = sensor_id.device
+    n = len(sensor_id)
+    r = N.Results()
+    r.sensor_id, r.error, r.time_of_flight = sensor_id.data_ptr(), error.data_ptr(), time_of_flight.data_ptr()
+    out = Pipelines(torch.zeros(num_sensors + 1, dtype=torch.int64, device=dev),
+                    torch.empty(n, dtype=torch.float64, device=dev), torch.empty(n, dtype=torch.float64, device=dev),
+                    torch.empty(n, dtype=torch.int32, device=dev))
+    m = C.c_int64()
+    N.check(N.lib().mlgpu_sensor_pipelines(C.byref(r), C.c_void_p(wavelength.data_ptr()),
+                                           C.c_void_p(truth_id.data_ptr() if truth_id is not None else None), n,
+                                           num_sensors, C.byref(out.c_struct()), C.byref(m), _st(stream, dev)),
+            "mlgpu_sensor_pipelines")
+    k = m.value
+    return Pipelines(out.offsets, out.arrival_time[:k], out.wavelength[:k], out.truth_id[:k])
+
+
+def photoelectric_convert(pipes: Pipelines, num_sensors: int, qe_x, qe_y, dark_rate: float = 0.0,
+                          crosstalk: float = 0.0, exposure_time: float = 0.0, seed: int = 0,
+                          stream=None) -> Pulses:
+    """Pulses of every pipeline (mlgpu_photoelectric_convert)."""
+    dev = pipes.offsets.device
+    qx = np.ascontiguousarray(qe_x, np.float64)
+    qy = np.ascontiguousarray(qe_y, np.float64)
+    conv = N.Converter(qx.ctypes.data, qy.ctypes.data, len(qx), dark_rate, crosstalk, exposure_time, seed)
+    m = len(pipes.arrival_time)
+    cap = int(m / max(1e-9, 1.0 - crosstalk)) * 2 + int(dark_rate * exposure_time * num_sensors * 2 / max(
+        1e-9, 1.0 - crosstalk)) + 64 * (num_sensors + 1)
+    for _ in range(2):
+        out = Pulses(torch.zeros(num_sensors + 1, dtype=torch.int64, device=dev),
+                     torch.empty(cap, dtype=torch.float64, device=dev), torch.empty(cap, dtype=torch.int32, device=dev))
+        k = C.c_int64()
+        rc = N.lib().mlgpu_photoelectric_convert(C.byref(pipes.c_struct()), num_sensors, C.byref(conv),
+                                                 C.byref(out.c_struct()), C.byref(k), _st(stream, dev))
+        if rc == N.MLGPU_OK:
+            return Pulses(out.offsets, out.arrival_time[:k.value], out.truth_id[:k.value])
+        if k.value > cap:  # the count is known: once more with room for it
+            cap = k.value
+            continue
+        N.check(rc, "mlgpu_photoelectric_convert")
+    N.check(rc, "mlgpu_photoelectric_convert")
+
+
+def extract_pulses(pulses: Pulses, num_sensors: int, time_slice_duration: float, arrival_time_std: float,
+                   seed: int = 0, stream=None) -> torch.Tensor:
+    """Time slice of every pulse, -1 when outside the 100 slices
+    (mlgpu_extract_pulses)."""
+    dev = pulses.offsets.device
+    out = torch.empty(len(pulses.arrival_time), dtype=torch.int16, device=dev)
+    N.check(N.lib().mlgpu_extract_pulses(C.byref(pulses.c_struct()), num_sensors, time_slice_duration,
+                                         arrival_time_std, seed, C.c_void_p(out.data_ptr()), _st(stream, dev)),
+            "mlgpu_extract_pulses")
+    return out

@@ -1271,3 +1271,128 @@ int64_t oracle_corsika_photons(const float* bunches, int64_t n, uint64_t seed, u
     }
     return k;
 }
+
+/* ------------------------------------------------------- signal chain */
+typedef struct { int32_t sensor; double t; int64_t idx; } sig_key;
+
+static int sig_cmp(const void* a, const void* b) {
+    const sig_key* x = (const sig_key*)a;
+    const sig_key* y = (const sig_key*)b;
+    if (x->sensor != y->sensor) return x->sensor < y->sensor ? -1 : 1;
+    if (x->t != y->t) return x->t < y->t ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+int64_t oracle_sensor_pipelines(const int32_t* sensor_id, const uint8_t* error, const double* time_of_flight,
+                                const double* wavelength, const int32_t* truth_id, int64_t n, int32_t num_sensors,
+                                int64_t* offsets, double* arrival_time, double* out_wavelength,
+                                int32_t* out_truth) {
+    sig_key* k = (sig_key*)malloc(sizeof(sig_key) * (size_t)(n > 0 ? n : 1));
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (sensor_id[i] >= 0 && sensor_id[i] < num_sensors && error[i] == 0) {
+            k[m].sensor = sensor_id[i]; k[m].t = time_of_flight[i]; k[m].idx = i; m++;
+        }
+    qsort(k, (size_t)m, sizeof(sig_key), sig_cmp);
+    int64_t j = 0;
+    for (int32_t s = 0; s <= num_sensors; s++) {
+        while (j < m && k[j].sensor < s) j++;
+        offsets[s] = j;
+    }
+    for (int64_t q = 0; q < m; q++) {
+        arrival_time[q] = k[q].t;
+        out_wavelength[q] = wavelength[k[q].idx];
+        out_truth[q] = truth_id ? truth_id[k[q].idx] : (int32_t)k[q].idx;
+    }
+    free(k);
+    return m;
+}
+
+/* sequential 24-bit uniforms of Philox stream `stream` of (seed, id) */
+typedef struct { uint32_t key[2], ctr[4], buf[4], next; } sig_gen;
+static void sig_init(sig_gen* g, uint64_t seed, uint64_t id, uint32_t stream) {
+    g->key[0] = (uint32_t)seed; g->key[1] = (uint32_t)(seed >> 32);
+    g->ctr[0] = 0u; g->ctr[1] = stream; g->ctr[2] = (uint32_t)id; g->ctr[3] = (uint32_t)(id >> 32);
+    g->next = 0u;
+}
+static double sig_uniform(sig_gen* g) {
+    if ((g->next & 3u) == 0u) { g->ctr[0] = g->next >> 2; oracle_philox4x32_10(g->ctr, g->key, g->buf); }
+    const uint32_t w = g->buf[g->next & 3u];
+    g->next++;
+    return (double)(w >> 8) * (1.0 / 16777216.0);
+}
+
+/* Func1 of (x, y) with the slopes of the unsorted table (Func1.cpp:22-71) */
+typedef struct { double x, y, slope; } sig_pt;
+static int sig_pt_cmp(const void* a, const void* b) {
+    const double x = ((const sig_pt*)a)->x, y = ((const sig_pt*)b)->x;
+    return x < y ? -1 : (x > y);
+}
+static int sig_eval(const sig_pt* p, int32_t n, double x, double* y) {
+    if (!(x >= p[0].x) || !(x < p[n - 1].x)) return 0;
+    int32_t lo = 0, hi = n;
+    while (lo < hi) { const int32_t mid = (lo + hi) / 2; if (p[mid].x > x) hi = mid; else lo = mid + 1; }
+    *y = p[lo - 1].slope * x + p[lo - 1].y - p[lo - 1].slope * p[lo - 1].x;
+    return 1;
+}
+
+int64_t oracle_photoelectric(const int64_t* offsets, const double* arrival_time, const double* wavelength,
+                             const int32_t* truth_id, int32_t num_sensors, const double* qe_x, const double* qe_y,
+                             int32_t qe_n, double dark_rate, double crosstalk, double exposure_time, uint64_t seed,
+                             int64_t* out_offsets, double* out_time, int32_t* out_truth, int64_t capacity) {
+    sig_pt* p = (sig_pt*)malloc(sizeof(sig_pt) * (size_t)qe_n);
+    for (int32_t i = 0; i < qe_n; i++) {
+        p[i].x = qe_x[i]; p[i].y = qe_y[i];
+        p[i].slope = (i != qe_n - 1) ? (qe_y[i] - qe_y[i + 1]) / (qe_x[i] - qe_x[i + 1]) : 0.0;
+    }
+    qsort(p, (size_t)qe_n, sizeof(sig_pt), sig_pt_cmp);
+    int64_t k = 0;
+    int64_t rc = 0;
+    /* add_pulse (PhotoElectricConverter.cpp:53-66): the pulse, then a
+       crosstalk pulse while crosstalk >= u, all drawn from g */
+#define SIG_PUSH(T, TR) do { if (k >= capacity) { rc = -2; goto done; } out_time[k] = (T); out_truth[k] = (TR); k++; \
+        while (crosstalk >= sig_uniform(&g)) { if (k >= capacity) { rc = -2; goto done; } \
+            out_time[k] = (T); out_truth[k] = -202; k++; } } while (0)
+    for (int32_t s = 0; s < num_sensors; s++) {
+        out_offsets[s] = k;
+        /* get_pulse_pipeline_for_photon_pipeline (:31-51), photon j of the
+           sorted pipelines drawing from stream 6 of (seed, j) */
+        for (int64_t j = offsets[s]; j < offsets[s + 1]; j++) {
+            sig_gen g;
+            sig_init(&g, seed, (uint64_t)j, 6u);
+            double q;
+            if (!sig_eval(p, qe_n, wavelength[j], &q)) { rc = -1; goto done; }
+            if (q >= sig_uniform(&g)) SIG_PUSH(arrival_time[j], truth_id[j]);
+        }
+        /* add_accidental_pulse (:68-89), sensor s drawing from stream 8 of
+           (seed, s) */
+        sig_gen g;
+        sig_init(&g, seed, (uint64_t)s, 8u);
+        double sum = -log(sig_uniform(&g)) / dark_rate;
+        while (sum < exposure_time) {
+            const double next = -log(sig_uniform(&g)) / dark_rate;
+            SIG_PUSH(sum, -201);
+            sum += next;
+        }
+    }
+    out_offsets[num_sensors] = k;
+    rc = k;
+done:
+#undef SIG_PUSH
+    free(p);
+    return rc;
+}
+
+void oracle_extract_pulses(const int64_t* offsets, const double* arrival_time, int32_t num_sensors,
+                           double duration, double std_dev, uint64_t seed, int16_t* slice) {
+    /* extract_pulses (pulse_extraction.cpp:9-30), pulse j drawing its normal
+       from stream 7 of (seed, j) */
+    for (int64_t j = 0; j < offsets[num_sensors]; j++) {
+        sig_gen g;
+        sig_init(&g, seed, (uint64_t)j, 7u);
+        const double u1 = 1.0 - sig_uniform(&g), u2 = sig_uniform(&g);
+        const double t = arrival_time[j] + (0.0 + std_dev * sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2));
+        const int32_t sl = (int32_t)round(t / duration);
+        slice[j] = (sl >= 0 && sl < 100) ? (int16_t)sl : (int16_t)-1;
+    }
+}

@@ -114,6 +114,24 @@ double oracle_arrival_jitter(uint64_t seed, uint64_t photon_id);
 int64_t oracle_corsika_photons(const float* bunches, int64_t n, uint64_t seed, uint64_t id_offset,
                                double* const out[7], int32_t* truth, int64_t capacity);
 
+/* Sensor signal chain (merlict_signal_processing): per-sensor pipelines of
+ * the detected photons sorted by (arrival time, photon index)
+ * (PipelinePhoton.cpp:25-66), returning their number; the photo-electric
+ * converter (PhotoElectricConverter.cpp:10-95) with Philox stream 6 of
+ * (seed, sensor), returning the number of pulses (-1: a wavelength outside
+ * the table, -2: capacity); the pulse extraction (pulse_extraction.cpp:
+ * 9-30) with Philox stream 7 of (seed, sensor), slice -1 when dropped. */
+int64_t oracle_sensor_pipelines(const int32_t* sensor_id, const uint8_t* error, const double* time_of_flight,
+                                const double* wavelength, const int32_t* truth_id, int64_t n, int32_t num_sensors,
+                                int64_t* offsets, double* arrival_time, double* out_wavelength,
+                                int32_t* out_truth);
+int64_t oracle_photoelectric(const int64_t* offsets, const double* arrival_time, const double* wavelength,
+                             const int32_t* truth_id, int32_t num_sensors, const double* qe_x, const double* qe_y,
+                             int32_t qe_n, double dark_rate, double crosstalk, double exposure_time, uint64_t seed,
+                             int64_t* out_offsets, double* out_time, int32_t* out_truth, int64_t capacity);
+void oracle_extract_pulses(const int64_t* offsets, const double* arrival_time, int32_t num_sensors,
+                           double duration, double std_dev, uint64_t seed, int16_t* slice);
+
 #ifdef __cplusplus
 }
 #endif

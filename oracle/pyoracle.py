@@ -403,3 +403,108 @@ def ref_lightfield(f, aperture, fov_deg, pixel_fov_deg, n_paxel, lens_n, bin_ref
 
 def ref_bench(scene_path, photons_path, threads, mode, seed=0) -> str:
     return _run_ref(["bench", scene_path, photons_path, threads, mode, seed])
+
+
+# ------------------------------------------------------------ signal chain
+def _sig_lib():
+    L = lib()
+    if not getattr(L, "_sig_ready", False):
+        L.oracle_sensor_pipelines.restype = C.c_int64
+        L.oracle_sensor_pipelines.argtypes = [_P_I32, _P_U8, _P_F64, _P_F64, _P_I32, C.c_int64, C.c_int32,
+                                              C.POINTER(C.c_int64), _P_F64, _P_F64, _P_I32]
+        L.oracle_photoelectric.restype = C.c_int64
+        L.oracle_photoelectric.argtypes = [C.POINTER(C.c_int64), _P_F64, _P_F64, _P_I32, C.c_int32, _P_F64, _P_F64,
+                                           C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                           C.POINTER(C.c_int64), _P_F64, _P_I32, C.c_int64]
+        L.oracle_extract_pulses.argtypes = [C.POINTER(C.c_int64), _P_F64, C.c_int32, C.c_double, C.c_double,
+                                            C.c_uint64, C.POINTER(C.c_int16)]
+        L._sig_ready = True
+    return L
+
+
+def sensor_pipelines(sensor_id, error, time_of_flight, wavelength, truth_id, num_sensors):
+    """(offsets, arrival_time, wavelength, truth_id) of the per-sensor
+    pipelines (oracle_sensor_pipelines)."""
+    s = np.ascontiguousarray(sensor_id, np.int32)
+    e = np.ascontiguousarray(error, np.uint8)
+    t = np.ascontiguousarray(time_of_flight, np.float64)
+    lam = np.ascontiguousarray(wavelength, np.float64)
+    tr = np.ascontiguousarray(truth_id, np.int32) if truth_id is not None else None
+    n = len(s)
+    off = np.zeros(num_sensors + 1, np.int64)
+    ot, ol, otr = np.empty(n), np.empty(n), np.empty(n, np.int32)
+    m = _sig_lib().oracle_sensor_pipelines(_ptr(s, _P_I32), _ptr(e, _P_U8), _ptr(t, _P_F64), _ptr(lam, _P_F64),
+                                           _ptr(tr, _P_I32) if tr is not None else None, n, num_sensors,
+                                           off.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(ot, _P_F64),
+                                           _ptr(ol, _P_F64), _ptr(otr, _P_I32))
+    return off, ot[:m], ol[:m], otr[:m]
+
+
+def photoelectric(offsets, arrival_time, wavelength, truth_id, num_sensors, qe_x, qe_y, dark_rate, crosstalk,
+                  exposure_time, seed, capacity=None):
+    """(offsets, time, truth) of the converter's pulses (oracle_photoelectric)."""
+    off = np.ascontiguousarray(offsets, np.int64)
+    t = np.ascontiguousarray(arrival_time, np.float64)
+    lam = np.ascontiguousarray(wavelength, np.float64)
+    tr = np.ascontiguousarray(truth_id, np.int32)
+    qx, qy = np.ascontiguousarray(qe_x, np.float64), np.ascontiguousarray(qe_y, np.float64)
+    cap = capacity or (4 * len(t) + 64 * num_sensors + int(dark_rate * exposure_time * num_sensors * 4) + 64)
+    oo = np.zeros(num_sensors + 1, np.int64)
+    ot, otr = np.empty(cap), np.empty(cap, np.int32)
+    k = _sig_lib().oracle_photoelectric(off.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(t, _P_F64),
+                                        _ptr(lam, _P_F64), _ptr(tr, _P_I32), num_sensors, _ptr(qx, _P_F64),
+                                        _ptr(qy, _P_F64), len(qx), dark_rate, crosstalk, exposure_time, seed,
+                                        oo.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(ot, _P_F64),
+                                        _ptr(otr, _P_I32), cap)
+    if k == -1:
+        raise IndexError("wavelength outside the quantum efficiency table")
+    if k < 0:
+        raise RuntimeError("capacity")
+    return oo, ot[:k], otr[:k]
+
+
+def extract_pulses(offsets, arrival_time, num_sensors, duration, std_dev, seed):
+    """Time slice per pulse (-1: dropped) (oracle_extract_pulses)."""
+    off = np.ascontiguousarray(offsets, np.int64)
+    t = np.ascontiguousarray(arrival_time, np.float64)
+    out = np.empty(len(t), np.int16)
+    _sig_lib().oracle_extract_pulses(off.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(t, _P_F64), num_sensors,
+                                     duration, std_dev, seed, out.ctypes.data_as(C.POINTER(C.c_int16)))
+    return out
+
+
+def ref_signal(sensor_id, time_of_flight, wavelength, truth_id, num_sensors, qe_x, qe_y, dark_rate, crosstalk,
+               exposure_time, seed, duration, std_dev) -> dict:
+    """The reference's signal chain per sensor (harness `signal`)."""
+    n = len(sensor_id)
+    parts = [np.array([n], np.int64).tobytes(), np.array([num_sensors], np.int32).tobytes(),
+             np.ascontiguousarray(sensor_id, np.int32).tobytes(), np.ascontiguousarray(time_of_flight, np.float64).tobytes(),
+             np.ascontiguousarray(wavelength, np.float64).tobytes(), np.ascontiguousarray(truth_id, np.int32).tobytes(),
+             np.array([len(qe_x)], np.int32).tobytes(), np.ascontiguousarray(qe_x, np.float64).tobytes(),
+             np.ascontiguousarray(qe_y, np.float64).tobytes(),
+             np.array([dark_rate, crosstalk, exposure_time], np.float64).tobytes(), np.array([seed], np.uint64).tobytes(),
+             np.array([duration, std_dev], np.float64).tobytes()]
+    with tempfile.TemporaryDirectory() as d:
+        i, o = Path(d) / "in.bin", Path(d) / "out.bin"
+        i.write_bytes(b"".join(parts))
+        _run_ref(["signal", i, o])
+        raw = o.read_bytes()
+    pos = 0
+
+    def take(dtype, count):
+        nonlocal pos
+        a = np.frombuffer(raw[pos:pos + np.dtype(dtype).itemsize * count], dtype).copy()
+        pos += np.dtype(dtype).itemsize * count
+        return a
+
+    out = {}
+    out["pipe_offsets"] = take(np.int64, num_sensors + 1)
+    m = int(out["pipe_offsets"][-1])
+    out["pipe_time"], out["pipe_wavelength"], out["pipe_truth"] = take(np.float64, m), take(np.float64, m), take(np.int32, m)
+    out["pulse_offsets"] = take(np.int64, num_sensors + 1)
+    k = int(out["pulse_offsets"][-1])
+    out["pulse_time"], out["pulse_truth"] = take(np.float64, k), take(np.int32, k)
+    out["extracted_offsets"] = take(np.int64, num_sensors + 1)
+    x = int(out["extracted_offsets"][-1])
+    out["extracted_slice"], out["extracted_truth"] = take(np.int16, x), take(np.int32, x)
+    return out

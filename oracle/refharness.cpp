@@ -33,6 +33,8 @@
 #include "merlict_portal_plenoscope/light_field_sensor/LightFieldSensor.h"
 #include "merlict_corsika/PhotonFactory.h"
 #include "merlict_corsika/eventio.h"
+#include "merlict_signal_processing/signal_processing.h"
+namespace sp = signal_processing;
 
 namespace ml = merlict;
 
@@ -455,6 +457,90 @@ static int cmd_corsika(int argc, char** argv) {
     return 0;
 }
 
+// signal <in> <out>: the reference's signal chain per sensor.  in: int64 n,
+// int32 num_sensors, n int32 sensor ids, n float64 arrival times, n float64
+// wavelengths, n int32 truth ids, int32 m, m float64 qe x, m float64 qe y,
+// float64 dark rate, crosstalk, exposure time, uint64 seed, float64 slice
+// duration, arrival time std.  Pipelines are sorted by the reference
+// (sort_photon_pipelines_arrival_time); sensor s converts with a Philox
+// stream 6 and extracts with a Philox stream 7 of (seed, s).  out: per
+// stage the int64 offsets (num_sensors + 1) and the entries: pipelines
+// (time, wavelength, truth), pulses (time, truth), extracted (slice as
+// int16, truth).
+static int cmd_signal(int argc, char** argv) {
+    if (argc != 4) return 2;
+    std::ifstream in(argv[2], std::ios::binary);
+    int64_t n = 0;
+    int32_t ns = 0, m = 0;
+    in.read(reinterpret_cast<char*>(&n), 8);
+    in.read(reinterpret_cast<char*>(&ns), 4);
+    std::vector<int32_t> sensor(n), truth(n);
+    std::vector<double> t(n), lam(n);
+    in.read(reinterpret_cast<char*>(sensor.data()), n * 4);
+    in.read(reinterpret_cast<char*>(t.data()), n * 8);
+    in.read(reinterpret_cast<char*>(lam.data()), n * 8);
+    in.read(reinterpret_cast<char*>(truth.data()), n * 4);
+    in.read(reinterpret_cast<char*>(&m), 4);
+    std::vector<double> qx(m), qy(m);
+    in.read(reinterpret_cast<char*>(qx.data()), m * 8);
+    in.read(reinterpret_cast<char*>(qy.data()), m * 8);
+    double dark = 0, xt = 0, expo = 0, dur = 0, sd = 0;
+    uint64_t seed = 0;
+    in.read(reinterpret_cast<char*>(&dark), 8);
+    in.read(reinterpret_cast<char*>(&xt), 8);
+    in.read(reinterpret_cast<char*>(&expo), 8);
+    in.read(reinterpret_cast<char*>(&seed), 8);
+    in.read(reinterpret_cast<char*>(&dur), 8);
+    in.read(reinterpret_cast<char*>(&sd), 8);
+    std::vector<std::vector<double>> table;
+    for (int32_t i = 0; i < m; i++) table.push_back({qx[i], qy[i]});
+    const ml::function::Func1 qe(table);
+    sp::PhotoElectricConverter::Config cfg;
+    cfg.quantum_efficiency_vs_wavelength = &qe;
+    cfg.dark_rate = dark;
+    cfg.probability_for_second_puls = xt;
+    sp::PhotoElectricConverter::Converter conv(&cfg);
+    std::vector<std::vector<sp::PipelinePhoton>> pipes(static_cast<size_t>(ns));
+    for (int64_t i = 0; i < n; i++)
+        if (sensor[i] >= 0 && sensor[i] < ns) pipes[sensor[i]].emplace_back(t[i], lam[i], truth[i]);
+    std::vector<int64_t> po{0}, eo{0}, xo{0};
+    std::vector<double> pt, pl, et;
+    std::vector<int32_t> ptr, etr, xtr;
+    std::vector<int16_t> xs;
+    // The reference's converter and extraction, called on one photon (seed,
+    // j, stream 6; exposure 0 adds no accidental), one sensor's empty
+    // pipeline (seed, s, stream 8: its accidental pulses) and one pulse
+    // (seed, k, stream 7) at a time.
+    int64_t j = 0, k = 0;
+    for (int32_t s = 0; s < ns; s++) {
+        sp::sort_photon_pipelines_arrival_time(&pipes[s]);
+        for (const auto& ph : pipes[s]) { pt.push_back(ph.arrival_time); pl.push_back(ph.wavelength); ptr.push_back(ph.simulation_truth_id); }
+        po.push_back(static_cast<int64_t>(pt.size()));
+        std::vector<sp::ElectricPulse> pulses;
+        for (const auto& ph : pipes[s]) {
+            PhiloxGenerator g6(seed, static_cast<uint64_t>(j++), 6u);
+            const std::vector<sp::PipelinePhoton> one{ph};
+            for (const auto& pu : conv.get_pulse_pipeline_for_photon_pipeline(one, 0.0, &g6)) pulses.push_back(pu);
+        }
+        PhiloxGenerator g8(seed, static_cast<uint64_t>(s), 8u);
+        for (const auto& pu : conv.get_pulse_pipeline_for_photon_pipeline({}, expo, &g8)) pulses.push_back(pu);
+        for (const auto& pu : pulses) { et.push_back(pu.arrival_time); etr.push_back(pu.simulation_truth_id); }
+        eo.push_back(static_cast<int64_t>(et.size()));
+        for (const auto& pu : pulses) {
+            PhiloxGenerator g7(seed, static_cast<uint64_t>(k++), 7u);
+            const std::vector<sp::ElectricPulse> one{pu};
+            for (const auto& x : sp::extract_pulses(one, dur, sd, &g7)) { xs.push_back(static_cast<int16_t>(x.arrival_time_slice)); xtr.push_back(x.simulation_truth_id); }
+        }
+        xo.push_back(static_cast<int64_t>(xs.size()));
+    }
+    std::ofstream o(argv[3], std::ios::binary);
+    auto w = [&](const void* p, size_t b) { o.write(static_cast<const char*>(p), static_cast<std::streamsize>(b)); };
+    w(po.data(), po.size() * 8); w(pt.data(), pt.size() * 8); w(pl.data(), pl.size() * 8); w(ptr.data(), ptr.size() * 4);
+    w(eo.data(), eo.size() * 8); w(et.data(), et.size() * 8); w(etr.data(), etr.size() * 4);
+    w(xo.data(), xo.size() * 8); w(xs.data(), xs.size() * 2); w(xtr.data(), xtr.size() * 4);
+    return 0;
+}
+
 static int cmd_lensmaker(int argc, char** argv) {
     if (argc < 5) return 2;
     ml::lens_maker::Config c;
@@ -554,7 +640,7 @@ static int cmd_bench(int argc, char** argv) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: merlict_golden propagate|tree|firsthit|reflector|lensmaker|lightfield|bench|corsika-extract|corsika ...\n");
+        std::fprintf(stderr, "usage: merlict_golden propagate|tree|firsthit|reflector|lensmaker|lightfield|bench|corsika-extract|corsika|signal ...\n");
         return 2;
     }
     try {
@@ -569,6 +655,7 @@ int main(int argc, char** argv) {
         else if (c == "bench") rc = cmd_bench(argc, argv);
         else if (c == "corsika-extract") rc = cmd_corsika_extract(argc, argv);
         else if (c == "corsika") rc = cmd_corsika(argc, argv);
+        else if (c == "signal") rc = cmd_signal(argc, argv);
         if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", c.c_str());
         return rc;
     } catch (const std::exception& e) {

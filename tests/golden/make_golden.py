@@ -5,7 +5,7 @@ oracle/Makefile, with the Philox stream plugged into merlict's
 random::Generator) on seeded inputs and stores inputs and outputs as small
 .npz files.  Run in the container that has the reference mounted:
 
-    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika]
+    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika | signal]
 """
 from __future__ import annotations
 
@@ -115,9 +115,47 @@ def corsika_fixtures():
     print("corsika_photons", len(bunches), "bunches", len(ref["truth"]), "photons")
 
 
+def signal_inputs(n=5000, num_sensors=40, seed=13):
+    """Synthetic detected photons for the signal chain: sensor ids with some
+    out of range, error flags, distinct arrival times over 0-60 ns,
+    wavelengths 280-700 nm, truth ids; a quantum efficiency table over
+    200-1200 nm; dark rate, crosstalk, exposure, slices."""
+    rng = np.random.default_rng(seed)
+    sensor = rng.integers(-3, num_sensors + 3, n).astype(np.int32)
+    error = (rng.uniform(size=n) < 0.02).astype(np.uint8)
+    t = rng.uniform(0.0, 60e-9, n)
+    lam = rng.uniform(280e-9, 700e-9, n)
+    truth = rng.integers(0, 1_000_000, n).astype(np.int32)
+    qe_x = np.array([1200e-9, 200e-9, 300e-9, 400e-9, 500e-9, 600e-9, 800e-9, 1000e-9])  # unsorted, as merlict allows
+    qe_y = np.array([0.02, 0.0, 0.25, 0.4, 0.35, 0.2, 0.08, 0.05])
+    return dict(sensor_id=sensor, error=error, time_of_flight=t, wavelength=lam, truth_id=truth,
+                num_sensors=np.array([num_sensors], np.int32), qe_x=qe_x, qe_y=qe_y,
+                dark_rate=np.array([2e7]), crosstalk=np.array([0.1]), exposure_time=np.array([100e-9]),
+                seed=np.array([9], np.uint64), duration=np.array([0.5e-9]), std_dev=np.array([0.4e-9]))
+
+
+def signal_fixtures():
+    """The reference's signal chain (PipelinePhoton sorting, the
+    photo-electric converter, pulse extraction) on the synthetic inputs."""
+    a = signal_inputs()
+    det = (a["error"] == 0)
+    ref = O.ref_signal(np.where(det, a["sensor_id"], -1), a["time_of_flight"], a["wavelength"], a["truth_id"],
+                       int(a["num_sensors"][0]), a["qe_x"], a["qe_y"], float(a["dark_rate"][0]),
+                       float(a["crosstalk"][0]), float(a["exposure_time"][0]), int(a["seed"][0]),
+                       float(a["duration"][0]), float(a["std_dev"][0]))
+    arrays = {f"in_{k}": v for k, v in a.items()}
+    arrays.update({f"out_{k}": v for k, v in ref.items()})
+    np.savez_compressed(HERE / "signal_chain.npz", **arrays)
+    print("signal_chain", int(ref["pipe_offsets"][-1]), "photons", int(ref["pulse_offsets"][-1]), "pulses",
+          int(ref["extracted_offsets"][-1]), "extracted")
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/merlict_golden is not built")
+    if len(sys.argv) > 1 and sys.argv[1] == "signal":
+        signal_fixtures()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "corsika":
         corsika_fixtures()
         return

@@ -481,3 +481,59 @@ def test_oracle_corsika_weight_collapse_rate():
     c = np.bincount(truth, minlength=1000)
     assert set(np.unique(c)) <= {15, 16}
     assert c.mean() == pytest.approx(15.5, abs=0.05)
+
+
+# ------------------------------------------------------------- signal chain
+def _signal_golden(golden):
+    g = golden("signal_chain")
+    a = {k[3:]: g[k] for k in g if k.startswith("in_")}
+    ref = {k[4:]: g[k] for k in g if k.startswith("out_")}
+    return a, ref
+
+
+def test_oracle_signal_chain_matches_reference(golden):
+    """Pipelines, converter pulses and extracted slices of the oracle equal
+    the reference's (sort_photon_pipelines_arrival_time,
+    PhotoElectricConverter::Converter, extract_pulses) under the same
+    per-sensor Philox streams."""
+    a, ref = _signal_golden(golden)
+    ns = int(a["num_sensors"][0])
+    off, t, lam, tr = O.sensor_pipelines(a["sensor_id"], a["error"], a["time_of_flight"], a["wavelength"],
+                                         a["truth_id"], ns)
+    np.testing.assert_array_equal(off, ref["pipe_offsets"])
+    np.testing.assert_array_equal(t, ref["pipe_time"])
+    np.testing.assert_array_equal(lam, ref["pipe_wavelength"])
+    np.testing.assert_array_equal(tr, ref["pipe_truth"])
+    po, pt, ptr = O.photoelectric(off, t, lam, tr, ns, a["qe_x"], a["qe_y"], float(a["dark_rate"][0]),
+                                  float(a["crosstalk"][0]), float(a["exposure_time"][0]), int(a["seed"][0]))
+    np.testing.assert_array_equal(po, ref["pulse_offsets"])
+    np.testing.assert_array_equal(pt, ref["pulse_time"])
+    np.testing.assert_array_equal(ptr, ref["pulse_truth"])
+    assert (ptr == -201).sum() > 30 and (ptr == -202).sum() > 30  # accidentals and crosstalk occur
+    sl = O.extract_pulses(po, pt, ns, float(a["duration"][0]), float(a["std_dev"][0]), int(a["seed"][0]))
+    kept = sl >= 0
+    np.testing.assert_array_equal(sl[kept], ref["extracted_slice"])
+    np.testing.assert_array_equal(ptr[kept], ref["extracted_truth"])
+    counts = np.add.reduceat(kept.astype(np.int64), po[:-1]) if len(kept) else np.zeros(ns, np.int64)
+    counts = np.where(np.diff(po) > 0, counts, 0)
+    np.testing.assert_array_equal(np.concatenate([[0], np.cumsum(counts)]), ref["extracted_offsets"])
+
+
+def test_oracle_converter_rates():
+    """PhotoElectricConverterTest.cpp: with quantum efficiency q the pulse
+    rate is q; with crosstalk p a pulse has 1 / (1 - p) pulses on average;
+    accidental pulses follow the dark rate."""
+    ns, n = 10, 200_000
+    s = np.repeat(np.arange(ns, dtype=np.int32), n // ns)
+    t = np.tile(np.linspace(0, 1e-6, n // ns), ns)
+    lam = np.full(n, 433e-9)
+    off, tt, ll, tr = O.sensor_pipelines(s, np.zeros(n, np.uint8), t, lam, np.arange(n, dtype=np.int32), ns)
+    qx, qy = np.array([200e-9, 1200e-9]), np.array([0.5, 0.5])
+    po, pt, ptr = O.photoelectric(off, tt, ll, tr, ns, qx, qy, 0.0, 0.0, 1e-6, 3)
+    assert len(pt) / n == pytest.approx(0.5, abs=0.005)
+    po, pt, ptr = O.photoelectric(off, tt, ll, tr, ns, qx, np.array([1.0, 1.0]), 0.0, 0.2, 1e-6, 4)
+    assert len(pt) / n == pytest.approx(1.0 / 0.8, abs=0.01)
+    po, pt, ptr = O.photoelectric(off[:1] * 0 + np.zeros(ns + 1, np.int64), tt[:0], ll[:0], tr[:0], ns, qx, qy,
+                                  1e8, 0.0, 1e-5, 5)
+    assert len(pt) / ns == pytest.approx(1e3, rel=0.05)
+    assert (pt >= 0).all() and (pt < 1e-5).all()
