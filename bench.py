@@ -14,6 +14,7 @@ import json
 import os
 import subprocess
 import sys
+import math
 import tempfile
 import time
 from pathlib import Path
@@ -295,35 +296,59 @@ def plenoscope_bunches_leg(dev, batches: int = 4, bunches_per_batch: int = 256) 
     }
 
 
+def signal_card() -> dict:
+    """The plenoscope propagation example card (night-sky flux, exposure,
+    converter, photon stream), with the quantum efficiency padded by zero at
+    200 and 1200 nm: the card's table starts at 266 nm while the flux
+    reaches down to 240 nm."""
+    card = json.loads((ROOT / "tests" / "golden" / "plenoscope_propagation_input_card.json").read_text())
+    nsb, pec = card["night_sky_background_ligth"], card["photo_electric_converter"]
+    flux = np.array(nsb["flux_vs_wavelength"], np.float64)
+    qe = np.array(pec["quantum_efficiency_vs_wavelength"], np.float64)
+    qe = np.concatenate([[[200e-9, 0.0]], qe, [[1200e-9, 0.0]]])
+    return {"flux_x": flux[:, 0].copy(), "flux_y": flux[:, 1].copy(), "exposure": float(nsb["exposure_time"]),
+            "qe_x": qe[:, 0].copy(), "qe_y": qe[:, 1].copy(), "dark_rate": float(pec["dark_rate"]),
+            "crosstalk": float(pec["probability_for_second_puls"]),
+            "slice": float(card["photon_stream"]["time_slice_duration"]),
+            "spread": float(card["photon_stream"]["single_photon_arrival_time_resolution"])}
+
+
 def signal_leg(result, photons, dev, reps: int = 5) -> dict:
     """The sensor signal chain on the detected photons of one FACT-like
-    propagation pass (1440 pixels): pipelines (sorted by arrival time),
-    photo-electric conversion (a quantum efficiency table, 1e7 Hz dark rate
-    over 100 ns, 5 % crosstalk) and pulse extraction (0.5 ns slices,
-    0.4 ns spread), each timed on the GPU; the oracle (C, one thread) times
-    the same chain on the host."""
-    from merlict_development_kit_amd.signal_chain import extract_pulses, photoelectric_convert, sensor_pipelines
+    propagation pass (1440 pixels) with the plenoscope example card's
+    settings: pipelines (sorted by arrival time), night-sky background (the
+    card's flux through a 2.1 m aperture over a 2.2 deg FoV radius, every
+    pixel efficiency 1), photo-electric conversion (the card's quantum
+    efficiency, dark rate and crosstalk over the exposure) and pulse
+    extraction (the card's slices and time resolution), each timed on the
+    GPU; the oracle (C, one thread) times the same chain on the host."""
+    from merlict_development_kit_amd.signal_chain import (extract_pulses, inject_nsb, nsb_light_rate,
+                                                          photoelectric_convert, sensor_pipelines)
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
-    qx, qy = np.array([200e-9, 300e-9, 400e-9, 500e-9, 700e-9, 1200e-9]), np.array([0.0, 0.2, 0.35, 0.3, 0.1, 0.0])
-    args = (1e7, 0.05, 1e-7)
-    times = {"pipelines": 0.0, "convert": 0.0, "extract": 0.0}
+    c = signal_card()
+    rate = nsb_light_rate(c["flux_x"], c["flux_y"], math.radians(2.2), 2.1)
+    eff = torch.ones(1440, dtype=torch.float32, device=dev)
+    conv = (c["qe_x"], c["qe_y"], c["dark_rate"], c["crosstalk"], c["exposure"])
+    times = {"pipelines": 0.0, "nsb": 0.0, "convert": 0.0, "extract": 0.0}
     for r in range(reps + 1):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         pipes = sensor_pipelines(result.sensor_id, result.error, result.time_of_flight, photons.wavelength, 1440)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        pulses = photoelectric_convert(pipes, 1440, qx, qy, *args, seed=r)
+        lit, _ = inject_nsb(pipes, 1440, eff, c["flux_x"], c["flux_y"], rate, c["exposure"], seed=r)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        extract_pulses(pulses, 1440, 0.5e-9, 0.4e-9, seed=r)
+        pulses = photoelectric_convert(lit, 1440, *conv, seed=r)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
+        extract_pulses(pulses, 1440, c["slice"], c["spread"], seed=r)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
         if r > 0:
-            times["pipelines"] += (t1 - t0) / reps
-            times["convert"] += (t2 - t1) / reps
-            times["extract"] += (t3 - t2) / reps
+            for k, dt in zip(times, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
+                times[k] += dt / reps
     n = len(result.sensor_id)
     m = len(pipes.arrival_time)
     sample = min(n, 2_000_000)
@@ -331,11 +356,14 @@ def signal_leg(result, photons, dev, reps: int = 5) -> dict:
     lam = photons.wavelength[:sample].cpu().numpy()
     c0 = time.perf_counter()
     off, t, l, tr = O.sensor_pipelines(res["sensor_id"], res["error"], res["time_of_flight"], lam, None, 1440)
-    po, pt, _ = O.photoelectric(off, t, l, tr, 1440, qx, qy, *args, 0)
-    O.extract_pulses(po, pt, 1440, 0.5e-9, 0.4e-9, 0)
+    off, t, l, tr, _ = O.inject_nsb(off, t, l, tr, np.ones(1440, np.float32), c["flux_x"], c["flux_y"], rate,
+                                    c["exposure"], 0)
+    po, pt, _ = O.photoelectric(off, t, l, tr, 1440, *conv, 0)
+    O.extract_pulses(po, pt, 1440, c["slice"], c["spread"], 0)
     cpu = time.perf_counter() - c0
     gpu = sum(times.values())
-    return {"photons": n, "detected": m, "pulses": len(pulses.arrival_time),
+    return {"photons": n, "detected": m, "with_background": len(lit.arrival_time),
+            "pulses": len(pulses.arrival_time), "nsb_rate_hz": rate,
             "ms": {k: v * 1e3 for k, v in times.items()},
             "value": n / gpu, "unit": "photons/s (whole chain on one pass's results)",
             "cpu_baseline": {"value": sample / cpu, "unit": "photons/s", "cores": 1, "kind": "port",

@@ -602,6 +602,42 @@ int mlgpu_extract_pulses(const mlgpu_pulses* pulses, int32_t num_sensors,
                          double time_slice_duration, double arrival_time_std,
                          uint64_t seed, int16_t* slice, void* stream);
 
+/* Night-sky-background injection (merlict_portal_plenoscope/
+ * night_sky_background/Injector.cpp:11-107) into the pipelines `in`: the
+ * exposure [start, start + exposure_time) is centred on the mode of the
+ * photons' arrival times (a histogram of 2 + sqrt(m) edges from the
+ * earliest to the latest photon, Injector.cpp:23-66); sensor s receives
+ * the photons of a Poisson process of rate * efficiencies[s] / num_sensors
+ * over the exposure, drawn from Philox stream 9 of (seed, s) (arrival times
+ * first, then one wavelength each from the flux's inverse cumulative
+ * distribution, SamplesFromDistribution.cpp:8-25 with 4096-step
+ * integral and inverse, function/tools.cpp:37-76), truth id -100; each
+ * pipeline is merged by arrival time (a photon of `in` before a background
+ * photon of the same time) and every arrival time becomes t - start.
+ * efficiencies: device float per sensor; flux_x/flux_y: host table
+ * (Func1).  `out` has its own arrays (not those of `in`); *num_photons and
+ * *exposure_start_time (host) receive the merged count and start.  A
+ * capacity below the count is an MLGPU_ERR_ARGUMENT with the count set.
+ * Synchronises `stream`. */
+typedef struct mlgpu_nsb {
+    const double* flux_x;
+    const double* flux_y;
+    int32_t flux_n;
+    double rate;
+    double exposure_time;
+    uint64_t seed;
+} mlgpu_nsb;
+int mlgpu_inject_nsb(const mlgpu_pipelines* in, int32_t num_sensors, const float* efficiencies,
+                     const mlgpu_nsb* nsb, const mlgpu_pipelines* out, int64_t* num_photons,
+                     double* exposure_start_time, void* stream);
+
+/* Rate [1/s] of the night-sky light over the whole sensor, as
+ * night_sky_background::Light computes it (Light.cpp:9-28): the integral of
+ * flux vs wavelength (4096 steps) x pi (1.05 aperture_radius)^2 x the solid
+ * angle of the cone of 1.2 max_fov_radius.  Host only. */
+int mlgpu_nsb_light_rate(const double* flux_x, const double* flux_y, int32_t flux_n,
+                         double max_fov_radius, double max_aperture_radius, double* rate);
+
 /* Number of photons with a non-zero error flag in the last n results
  * (synchronises `stream`). */
 int mlgpu_count_errors(const mlgpu_results* results, int64_t n, void* stream,

@@ -159,6 +159,11 @@ class Converter(C.Structure):
                 ("crosstalk", C.c_double), ("exposure_time", C.c_double), ("seed", C.c_uint64)]
 
 
+class Nsb(C.Structure):
+    _fields_ = [("flux_x", C.c_void_p), ("flux_y", C.c_void_p), ("flux_n", C.c_int32), ("rate", C.c_double),
+                ("exposure_time", C.c_double), ("seed", C.c_uint64)]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("max_num_interactions_per_photon", C.c_uint32),
@@ -237,6 +242,10 @@ SIGNATURES = {
                                               C.POINTER(Pulses), C.POINTER(C.c_int64), C.c_void_p]),
     "mlgpu_extract_pulses": (C.c_int, [C.POINTER(Pulses), C.c_int32, C.c_double, C.c_double, C.c_uint64, C.c_void_p,
                                        C.c_void_p]),
+    "mlgpu_inject_nsb": (C.c_int, [C.POINTER(Pipelines), C.c_int32, C.c_void_p, C.POINTER(Nsb), C.POINTER(Pipelines),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_void_p]),
+    "mlgpu_nsb_light_rate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_double,
+                                       C.POINTER(C.c_double)]),
     "mlgpu_corsika_photons": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(PhotonBuffers),
                                         C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]),
     "mlgpu_histogram_arrivals": (C.c_int, [C.POINTER(Results), C.c_int64, C.POINTER(Histogram), C.c_void_p]),

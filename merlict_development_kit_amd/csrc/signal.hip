@@ -4,6 +4,10 @@
 // efficiency, crosstalk chain, dark-rate accidental pulses) and the
 // single-photon pulse extraction (pulse_extraction.cpp:9-30).
 //
+// The night-sky background is the step between the pipelines and the
+// converter (Injector.cpp:11-107); sensor s draws its background photons
+// from Philox stream 9 of (seed, s).
+//
 // The reference draws every random number of an event from one generator
 // in sensor order, which serialises the chain.  Here each photon of the
 // sorted pipelines (its conversion and crosstalk chain), each sensor (its
@@ -15,6 +19,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -29,6 +34,9 @@ namespace {
 constexpr uint32_t kConverterStream = 6;
 constexpr uint32_t kExtractionStream = 7;
 constexpr uint32_t kAccidentalStream = 8;
+constexpr uint32_t kNsbStream = 9;
+constexpr int32_t kNightSky = -100;    // simulation_truth.h:15
+constexpr double kMaxMeanPerSensor = 1e7;
 constexpr int32_t kCrosstalk = -202;   // simulation_truth.h:17
 constexpr int32_t kAccidental = -201;  // simulation_truth.h:16
 constexpr int32_t kNumTimeSlices = 100;  // pulse_extraction.h:13
@@ -263,6 +271,249 @@ __global__ void extract_kernel(const mlgpu_pulses p, const int64_t count, const 
     slice[k] = (sl >= 0 && sl < kNumTimeSlices) ? static_cast<int16_t>(sl) : static_cast<int16_t>(-1);
 }
 
+// ------------------------------------------------ night-sky background
+
+__device__ __forceinline__ double key_time(uint64_t k) {
+    const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double(static_cast<long long>(b));
+}
+
+// earliest and latest arrival time (as order-preserving keys)
+__global__ __launch_bounds__(256) void time_range_kernel(const double* __restrict__ t, const int64_t m,
+                                                         unsigned long long* __restrict__ range) {
+    __shared__ unsigned long long lo_s[256], hi_s[256];
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < m;
+         j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const unsigned long long k = time_key(t[j]);
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+    lo_s[threadIdx.x] = lo;
+    hi_s[threadIdx.x] = hi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) {
+            const unsigned long long a = lo_s[threadIdx.x + off], b = hi_s[threadIdx.x + off];
+            if (a < lo_s[threadIdx.x]) lo_s[threadIdx.x] = a;
+            if (b > hi_s[threadIdx.x]) hi_s[threadIdx.x] = b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMin(&range[0], lo_s[0]);
+        atomicMax(&range[1], hi_s[0]);
+    }
+}
+
+// Injector.cpp:26-45: min starts at the largest double, max at the
+// smallest positive one
+struct ModeEdges {
+    double lo, step;
+    uint32_t count;  // number of bin edges
+    __device__ double edge(uint32_t i) const { return lo + step * static_cast<double>(i); }
+};
+
+__device__ __forceinline__ ModeEdges mode_edges(const unsigned long long* range, const uint32_t count) {
+    double lo = key_time(range[0]), hi = key_time(range[1]);
+    if (!(lo < 1.7976931348623157e308)) lo = 1.7976931348623157e308;
+    if (!(hi > 2.2250738585072014e-308)) hi = 2.2250738585072014e-308;
+    ModeEdges e;
+    e.lo = lo;
+    e.count = count;
+    e.step = (hi - lo) / static_cast<double>(count - 1u);  // numeric::linspace (numeric.cpp:35-44)
+    return e;
+}
+
+// Histogram1::fill_in (Histogram1.cpp:29-47): the bin below the first edge
+// greater than t; -1 for under- and overflow
+__device__ __forceinline__ int64_t mode_bin(const ModeEdges& e, const double t) {
+    const int64_t last = static_cast<int64_t>(e.count) - 1;
+    int64_t k = 0;
+    if (e.step > 0.0) {
+        const double g = floor((t - e.lo) / e.step);
+        k = g < 0.0 ? 0 : (g > static_cast<double>(last) ? last : static_cast<int64_t>(g));
+    }
+    while (k < last && e.edge(static_cast<uint32_t>(k + 1)) <= t) k++;
+    while (k >= 0 && e.edge(static_cast<uint32_t>(k)) > t) k--;
+    return (k < 0 || k >= last) ? -1 : k;
+}
+
+// Up to kModeLdsBins bins each block counts in LDS and adds its counts
+// once (the photons crowd into a few bins around the peak, where global
+// atomics would serialise); beyond, global atomics.
+constexpr uint32_t kModeLdsBins = 12288;
+
+__global__ __launch_bounds__(256) void mode_histogram_kernel(const double* __restrict__ t, const int64_t m,
+                                                             const unsigned long long* __restrict__ range,
+                                                             const uint32_t count, uint32_t* __restrict__ bins) {
+    __shared__ uint32_t local[kModeLdsBins];
+    const ModeEdges e = mode_edges(range, count);
+    const uint32_t nb = count - 1u;
+    const bool lds = nb <= kModeLdsBins;
+    if (lds)
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) local[i] = 0u;
+    __syncthreads();
+    for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < m;
+         j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t b = mode_bin(e, t[j]);
+        if (b < 0) continue;
+        if (lds) atomicAdd(&local[b], 1u); else atomicAdd(&bins[b], 1u);
+    }
+    __syncthreads();
+    if (lds)
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+            if (local[i]) atomicAdd(&bins[i], local[i]);
+}
+
+// Histogram1::mode / arg_max (Histogram1.cpp:49-65): the first bin of the
+// largest count; start = mode - exposure / 2 (Injector.cpp:68-71)
+__global__ __launch_bounds__(1024) void mode_start_kernel(const uint32_t* __restrict__ bins,
+                                                          const unsigned long long* __restrict__ range,
+                                                          const uint32_t count, const int64_t m,
+                                                          const double exposure_time, double* __restrict__ start) {
+    __shared__ uint32_t best_c[1024];
+    __shared__ uint32_t best_i[1024];
+    uint32_t c = 0u, idx = 0xffffffffu;
+    const uint32_t nb = m > 0 ? count - 1u : 0u;
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+        if (bins[i] > c) { c = bins[i]; idx = i; }
+    best_c[threadIdx.x] = c;
+    best_i[threadIdx.x] = idx;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) {
+            const uint32_t c2 = best_c[threadIdx.x + off], i2 = best_i[threadIdx.x + off];
+            if (c2 > best_c[threadIdx.x] || (c2 == best_c[threadIdx.x] && i2 < best_i[threadIdx.x])) {
+                best_c[threadIdx.x] = c2;
+                best_i[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double mode = 0.0;
+        if (m > 0) {
+            const ModeEdges e = mode_edges(range, count);
+            const uint32_t i = best_c[0] > 0u ? best_i[0] : 0u;
+            mode = 0.5 * (e.edge(i + 1u) + e.edge(i));
+        }
+        *start = mode - 0.5 * exposure_time;
+    }
+}
+
+// the Poisson process of sensor s (Injector.cpp:73-96); Emit writes the
+// arrival times (start + running sum) and the wavelengths from `at` on
+template <bool Emit>
+__device__ int64_t background(const uint64_t seed, const int32_t s, const double rate, const double exposure_time,
+                              const double start, const FuncPoint* cdf, const int32_t cdf_n, const double cdf_total,
+                              double* time, double* lambda, const int64_t at, bool& bad) {
+    StreamRng rng;
+    rng.init(seed, static_cast<uint64_t>(s), kNsbStream);
+    int64_t k = 0;
+    double sum = rng.expovariate(rate);
+    while (sum < exposure_time) {
+        const double next = rng.expovariate(rate);
+        if (Emit) time[at + k] = start + sum;
+        k++;
+        sum += next;
+    }
+    if (Emit) {
+        // SamplesFromDistribution::draw: the inverse cdf at u * total
+        for (int64_t q = 0; q < k; q++) {
+            double y;
+            if (!table_eval(cdf, cdf_n, rng.uniform() * cdf_total, y)) { bad = true; y = 0.0; }
+            lambda[at + q] = y;
+        }
+    }
+    return k;
+}
+
+__device__ __forceinline__ bool sensor_rate(const float* eff, const int32_t s, const int32_t num_sensors,
+                                            const double rate, const double exposure_time, double& r) {
+    r = rate * static_cast<double>(eff[s]) / static_cast<double>(num_sensors);
+    return r >= 0.0 && r * exposure_time <= kMaxMeanPerSensor;
+}
+
+__global__ void nsb_count_kernel(const int32_t num_sensors, const float* __restrict__ eff, const double rate,
+                                 const double exposure_time, const uint64_t seed, int64_t* __restrict__ count,
+                                 int32_t* __restrict__ bad) {
+    const int32_t s = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= num_sensors) return;
+    double r;
+    if (!sensor_rate(eff, s, num_sensors, rate, exposure_time, r)) {
+        count[s] = 0;
+        atomicOr(bad, 1);
+        return;
+    }
+    bool b = false;
+    count[s] = background<false>(seed, s, r, exposure_time, 0.0, nullptr, 0, 0.0, nullptr, nullptr, 0, b);
+}
+
+__global__ void nsb_emit_kernel(const int32_t num_sensors, const float* __restrict__ eff, const double rate,
+                                const double exposure_time, const uint64_t seed, const double* __restrict__ start,
+                                const FuncPoint* __restrict__ cdf, const int32_t cdf_n, const double cdf_total,
+                                const int64_t* __restrict__ offsets, double* __restrict__ time,
+                                double* __restrict__ lambda, int32_t* __restrict__ bad) {
+    const int32_t s = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= num_sensors) return;
+    double r;
+    if (!sensor_rate(eff, s, num_sensors, rate, exposure_time, r)) return;
+    bool b = false;
+    (void)background<true>(seed, s, r, exposure_time, *start, cdf, cdf_n, cdf_total, time, lambda, offsets[s], b);
+    if (b) atomicOr(bad, 1);
+}
+
+// number of entries of t[lo, hi) below x (Strict) or not above x
+template <bool Strict>
+__device__ __forceinline__ int64_t rank_in(const double* t, int64_t lo, int64_t hi, const double x) {
+    const int64_t base = lo;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (Strict ? (t[mid] < x) : (t[mid] <= x)) lo = mid + 1; else hi = mid;
+    }
+    return lo - base;
+}
+
+// merged position of pipeline photon j: its rank in its pipeline + the
+// background photons of its sensor that arrive strictly earlier
+__global__ void merge_photons_kernel(const mlgpu_pipelines in, const int64_t m, const int32_t num_sensors,
+                                     const int64_t* __restrict__ nsb_offsets, const double* __restrict__ nsb_time,
+                                     const double* __restrict__ start, const mlgpu_pipelines out) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const int32_t s = sensor_of(in.offsets, num_sensors, j);
+    const double t = in.arrival_time[j];
+    const int64_t pos = in.offsets[s] + nsb_offsets[s] + (j - in.offsets[s]) +
+                        rank_in<true>(nsb_time, nsb_offsets[s], nsb_offsets[s + 1], t);
+    out.arrival_time[pos] = t - *start;
+    out.wavelength[pos] = in.wavelength[j];
+    out.truth_id[pos] = in.truth_id[j];
+}
+
+// merged position of background photon q: its rank among its sensor's
+// background photons + the sensor's pipeline photons not later than it
+__global__ void merge_background_kernel(const mlgpu_pipelines in, const int32_t num_sensors, const int64_t total,
+                                        const int64_t* __restrict__ nsb_offsets, const double* __restrict__ nsb_time,
+                                        const double* __restrict__ nsb_lambda, const double* __restrict__ start,
+                                        const mlgpu_pipelines out) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    const int32_t s = sensor_of(nsb_offsets, num_sensors, q);
+    const double t = nsb_time[q];
+    const int64_t pos = in.offsets[s] + nsb_offsets[s] + (q - nsb_offsets[s]) +
+                        rank_in<false>(in.arrival_time, in.offsets[s], in.offsets[s + 1], t);
+    out.arrival_time[pos] = t - *start;
+    out.wavelength[pos] = nsb_lambda[q];
+    out.truth_id[pos] = kNightSky;
+}
+
+__global__ void add_offsets_kernel(const int64_t* __restrict__ a, const int64_t* __restrict__ b, const int32_t n,
+                                   int64_t* __restrict__ out) {
+    const int32_t s = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s <= n) out[s] = a[s] + b[s];
+}
+
 size_t align_up(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
 }  // namespace
@@ -418,6 +669,107 @@ hipError_t extract_pulses(const mlgpu_pulses& p, int64_t count, double duration,
     hipLaunchKernelGGL(extract_kernel, dim3(static_cast<uint32_t>((count + 255) / 256)), dim3(256), 0, stream, p,
                        count, duration, std_dev, seed, slice);
     return hipGetLastError();
+}
+
+hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m, const float* efficiencies,
+                      const FuncPoint* cdf, int32_t cdf_n, double cdf_total, double rate, double exposure_time,
+                      uint64_t seed, const mlgpu_pipelines& out, int64_t* num_photons, double* start, int* bad,
+                      int* too_small, hipStream_t stream) {
+    *num_photons = 0;
+    *start = 0.0;
+    *bad = 0;
+    *too_small = 0;
+    const size_t ns = static_cast<size_t>(num_sensors);
+    // Injector.cpp:52-55: 2 + sqrt(m) bin edges
+    const uint32_t count = m > 0 ? static_cast<uint32_t>(2u + std::sqrt(static_cast<double>(
+                                                                   static_cast<uint32_t>(m))))
+                                 : 2u;
+    size_t scan_bytes = 0;
+    hipError_t e = rocprim::exclusive_scan(nullptr, scan_bytes, static_cast<int64_t*>(nullptr),
+                                           static_cast<int64_t*>(nullptr), int64_t{0}, ns + 1,
+                                           rocprim::plus<int64_t>(), stream);
+    if (e != hipSuccess) return e;
+    const size_t bytes = align_up(16) + align_up(static_cast<size_t>(count) * 4) + 2 * align_up((ns + 1) * 8) +
+                         align_up(scan_bytes) + align_up(8) + align_up(4) + 256;
+    char* mem = nullptr;
+    e = hipMalloc(&mem, bytes);
+    if (e != hipSuccess) return e;
+    char* q = mem;
+    auto take = [&](size_t b) { char* x = q; q += align_up(b); return x; };
+    auto* range = reinterpret_cast<unsigned long long*>(take(16));
+    auto* bins = reinterpret_cast<uint32_t*>(take(static_cast<size_t>(count) * 4));
+    auto* ncount = reinterpret_cast<int64_t*>(take((ns + 1) * 8));
+    auto* noff = reinterpret_cast<int64_t*>(take((ns + 1) * 8));
+    void* temp = take(scan_bytes);
+    auto* dstart = reinterpret_cast<double*>(take(8));
+    auto* dbad = reinterpret_cast<int32_t*>(take(4));
+    char* nsb_mem = nullptr;
+    int64_t total = 0;
+    int32_t host_bad = 0;
+    const unsigned long long init[2] = {~0ull, 0ull};
+    e = hipMemcpyAsync(range, init, 16, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(bins, 0, static_cast<size_t>(count) * 4, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ncount, 0, (ns + 1) * 8, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(dbad, 0, 4, stream);
+    if (e == hipSuccess && m > 0) {
+        const uint32_t grid = static_cast<uint32_t>(std::min<int64_t>((m + 255) / 256, 2048));
+        hipLaunchKernelGGL(time_range_kernel, dim3(grid), dim3(256), 0, stream, in.arrival_time, m, range);
+        // LDS counts: a few photons per thread before each block's flush
+        const uint32_t hgrid = static_cast<uint32_t>(std::min<int64_t>((m + 4095) / 4096, 1024));
+        hipLaunchKernelGGL(mode_histogram_kernel, dim3(hgrid), dim3(256), 0, stream, in.arrival_time, m, range, count,
+                           bins);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mode_start_kernel, dim3(1), dim3(1024), 0, stream, bins, range, count, m, exposure_time,
+                           dstart);
+        hipLaunchKernelGGL(nsb_count_kernel, dim3(static_cast<uint32_t>((ns + 255) / 256)), dim3(256), 0, stream,
+                           num_sensors, efficiencies, rate, exposure_time, seed, ncount, dbad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = rocprim::exclusive_scan(temp, scan_bytes, ncount, noff, int64_t{0}, ns + 1, rocprim::plus<int64_t>(),
+                                    stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, noff + ns, 8, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(start, dstart, 8, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&host_bad, dbad, 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) {
+        *num_photons = m + total;
+        if (host_bad) *bad = 1;
+        if (m + total > out.capacity) *too_small = 1;
+    }
+    if (e == hipSuccess && !*bad && !*too_small) {
+        const size_t nt = static_cast<size_t>(total > 0 ? total : 1);
+        e = hipMalloc(&nsb_mem, 2 * align_up(nt * 8));
+        double* nsb_time = reinterpret_cast<double*>(nsb_mem);
+        double* nsb_lambda = reinterpret_cast<double*>(nsb_mem + align_up(nt * 8));
+        if (e == hipSuccess && total > 0) {
+            hipLaunchKernelGGL(nsb_emit_kernel, dim3(static_cast<uint32_t>((ns + 255) / 256)), dim3(256), 0, stream,
+                               num_sensors, efficiencies, rate, exposure_time, seed, dstart, cdf, cdf_n, cdf_total,
+                               noff, nsb_time, nsb_lambda, dbad);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipMemcpyAsync(&host_bad, dbad, 4, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e == hipSuccess && host_bad) *bad = 1;
+        }
+        if (e == hipSuccess && !*bad) {
+            hipLaunchKernelGGL(add_offsets_kernel, dim3(static_cast<uint32_t>((ns + 1 + 255) / 256)), dim3(256), 0,
+                               stream, in.offsets, noff, num_sensors, out.offsets);
+            if (m > 0)
+                hipLaunchKernelGGL(merge_photons_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0,
+                                   stream, in, m, num_sensors, noff, nsb_time, dstart, out);
+            if (total > 0)
+                hipLaunchKernelGGL(merge_background_kernel, dim3(static_cast<uint32_t>((total + 255) / 256)),
+                                   dim3(256), 0, stream, in, num_sensors, total, noff, nsb_time, nsb_lambda, dstart,
+                                   out);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        }
+    }
+    if (nsb_mem) (void)hipFree(nsb_mem);
+    (void)hipFree(mem);
+    return e;
 }
 
 }  // namespace mlgpu

@@ -31,4 +31,15 @@ hipError_t convert_pulses(const mlgpu_pipelines& p, int32_t num_sensors, int64_t
 hipError_t extract_pulses(const mlgpu_pulses& p, int64_t count, double duration, double std_dev, uint64_t seed,
                           int16_t* slice, hipStream_t stream);
 
+// Night-sky-background injection into the m photons of `in` (ns sensors):
+// cdf (device) is the inverse cumulative table of the flux, cdf_total the
+// flux integral.  Synchronises; *num_photons receives m + the background
+// photons, *start the exposure start; *too_small when out.capacity is below
+// it, *bad when a wavelength draw falls outside the table or a sensor's
+// rate is negative or above 1e7 / exposure_time (nothing is written then).
+hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m, const float* efficiencies,
+                      const FuncPoint* cdf, int32_t cdf_n, double cdf_total, double rate, double exposure_time,
+                      uint64_t seed, const mlgpu_pipelines& out, int64_t* num_photons, double* start, int* bad,
+                      int* too_small, hipStream_t stream);
+
 }  // namespace mlgpu

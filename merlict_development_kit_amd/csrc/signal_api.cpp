@@ -24,6 +24,107 @@ int hip_fail(hipError_t e, const char* what) {
     return MLGPU_ERR_HIP;
 }
 
+// A merlict Func1 on the host (merlict/function/Func1.cpp:22-71): slopes
+// from the table as given, then sorted by x; no duplicate x.
+struct HostFunc {
+    std::vector<mlgpu::FuncPoint> p;
+    double lower = 0.0, upper = 0.0;
+    bool make(const std::vector<double>& x, const std::vector<double>& y) {
+        const size_t n = x.size();
+        if (n == 0) return false;
+        p.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            p[i].x = x[i];
+            p[i].y = y[i];
+            p[i].slope = (i != n - 1) ? (y[i] - y[i + 1]) / (x[i] - x[i + 1]) : 0.0;
+            p[i].pad = 0.0;
+        }
+        std::sort(p.begin(), p.end(), [](const mlgpu::FuncPoint& a, const mlgpu::FuncPoint& b) { return a.x < b.x; });
+        for (size_t i = 0; i + 1 < n; i++)
+            if (p[i].x == p[i + 1].x) return false;
+        lower = p.front().x;
+        upper = p.back().x;
+        return true;
+    }
+    // Func1::evaluate: false outside [lower, upper)
+    bool eval(double x, double& y) const {
+        if (x < lower || x >= upper) return false;
+        const auto it = std::upper_bound(p.begin(), p.end(), x,
+                                         [](double v, const mlgpu::FuncPoint& q) { return q.x > v; });
+        if (it == p.begin()) return false;
+        const mlgpu::FuncPoint& q = *(it - 1);
+        y = q.slope * x + q.y - q.slope * q.x;
+        return true;
+    }
+    double max() const {
+        double m = p.front().y;
+        for (const auto& q : p) if (q.y > m) m = q.y;
+        return m;
+    }
+};
+
+constexpr uint64_t kSteps = 4096;  // function::DEFAULT_STEPS (function/tools.h:12)
+
+// function::integral (function/tools.cpp:37-56): the running sum of
+// f(x) * step on the grid lower + i * step, the last point at the upper
+// limit with f just below it
+bool integral(const HostFunc& f, HostFunc& out) {
+    const double step = (f.upper - f.lower) / static_cast<double>(kSteps);
+    std::vector<double> x{f.lower}, y{0.0};
+    for (uint64_t i = 1; i < kSteps; i++) {
+        const double xi = f.lower + static_cast<double>(i) * step;
+        double v;
+        if (!f.eval(xi, v)) return false;
+        x.push_back(xi);
+        y.push_back(v * step + y.back());
+    }
+    double v;
+    if (!f.eval(std::nextafter(f.upper, f.lower), v)) return false;
+    x.push_back(f.upper);
+    y.push_back(v * step + y.back());
+    return out.make(x, y);
+}
+
+// function::inverse (function/tools.cpp:58-76): (f(x), x) on the same grid
+bool inverse(const HostFunc& f, HostFunc& out) {
+    const double step = (f.upper - f.lower) / static_cast<double>(kSteps);
+    std::vector<double> x, y;
+    double v;
+    if (!f.eval(f.lower, v)) return false;
+    x.push_back(v);
+    y.push_back(f.lower);
+    for (uint64_t i = 1; i < kSteps; i++) {
+        const double xi = f.lower + static_cast<double>(i) * step;
+        if (!f.eval(xi, v)) return false;
+        x.push_back(v);
+        y.push_back(xi);
+    }
+    if (!f.eval(std::nextafter(f.upper, f.lower), v)) return false;
+    x.push_back(v);
+    y.push_back(f.upper);
+    return out.make(x, y);
+}
+
+// SamplesFromDistribution (random/SamplesFromDistribution.cpp:8-17): the
+// inverse cumulative table and the total integral of the flux
+int flux_tables(const double* fx, const double* fy, int32_t n, HostFunc& cdf, double& total) {
+    if (!fx || !fy || n < 2) {
+        set_last_error("flux table needs at least two points");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    HostFunc f, F;
+    if (!f.make(std::vector<double>(fx, fx + n), std::vector<double>(fy, fy + n))) {
+        set_last_error("flux table has a duplicate wavelength");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (!integral(f, F) || !inverse(F, cdf)) {
+        set_last_error("flux table: its cumulative distribution is not invertible (a flux <= 0 somewhere)");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    total = F.max();
+    return MLGPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -125,6 +226,77 @@ int mlgpu_extract_pulses(const mlgpu_pulses* p, int32_t num_sensors, double dura
     }
     if (e == hipSuccess) e = mlgpu::extract_pulses(*p, count, duration, std_dev, seed, slice, st);
     if (e != hipSuccess) return hip_fail(e, "pulse extraction");
+    return MLGPU_OK;
+}
+
+int mlgpu_nsb_light_rate(const double* flux_x, const double* flux_y, int32_t flux_n, double max_fov_radius,
+                         double max_aperture_radius, double* rate) {
+    if (!rate) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    HostFunc cdf;
+    double total = 0.0;
+    const int rc = flux_tables(flux_x, flux_y, flux_n, cdf, total);
+    if (rc != MLGPU_OK) return rc;
+    // Light.cpp:15-27 with NightSkyBackground.h:11-12 and
+    // get_solid_angle_for_opening_angle (merlict/tools.cpp:25-48)
+    const double fov_radius = 1.20 * max_fov_radius;
+    const double solid_angle = 2.0 * M_PI * (1.0 - std::cos(fov_radius));
+    const double aperture_radius = 1.05 * max_aperture_radius;
+    const double area = M_PI * aperture_radius * aperture_radius;
+    *rate = total * area * solid_angle;
+    return MLGPU_OK;
+}
+
+int mlgpu_inject_nsb(const mlgpu_pipelines* in, int32_t num_sensors, const float* efficiencies, const mlgpu_nsb* nsb,
+                     const mlgpu_pipelines* out, int64_t* num_photons, double* exposure_start_time, void* stream) {
+    if (!in || !nsb || !out || !num_photons || !exposure_start_time || num_sensors < 0 || !in->offsets ||
+        !out->offsets) {
+        set_last_error("null argument or negative size");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    *num_photons = 0;
+    *exposure_start_time = 0.0;
+    if (!(nsb->rate >= 0.0) || !(nsb->exposure_time >= 0.0)) {
+        set_last_error("need rate >= 0 and exposure_time >= 0");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (num_sensors == 0) return MLGPU_OK;  // Injector.cpp:20-21
+    if (!efficiencies) { set_last_error("null efficiencies"); return MLGPU_ERR_ARGUMENT; }
+    if (out->offsets == in->offsets || (out->arrival_time && out->arrival_time == in->arrival_time)) {
+        set_last_error("out must not share arrays with in");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    HostFunc cdf;
+    double total = 0.0;
+    const int rc = flux_tables(nsb->flux_x, nsb->flux_y, nsb->flux_n, cdf, total);
+    if (rc != MLGPU_OK) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    int64_t m = 0;
+    hipError_t e = hipMemcpyAsync(&m, in->offsets + num_sensors, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "night-sky background");
+    if (m > INT32_MAX) { set_last_error("at most 2^31 - 1 photons"); return MLGPU_ERR_ARGUMENT; }
+    if (m > 0 && (!in->arrival_time || !in->wavelength || !in->truth_id)) {
+        set_last_error("null pipeline array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    mlgpu::FuncPoint* dcdf = nullptr;
+    e = hipMalloc(&dcdf, sizeof(mlgpu::FuncPoint) * cdf.p.size());
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc night-sky table");
+    e = hipMemcpyAsync(dcdf, cdf.p.data(), sizeof(mlgpu::FuncPoint) * cdf.p.size(), hipMemcpyHostToDevice, st);
+    int bad = 0, too_small = 0;
+    if (e == hipSuccess)
+        e = mlgpu::inject_nsb(*in, num_sensors, m, efficiencies, dcdf, static_cast<int32_t>(cdf.p.size()), total,
+                              nsb->rate, nsb->exposure_time, nsb->seed, *out, num_photons, exposure_start_time, &bad,
+                              &too_small, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(dcdf);
+    if (e != hipSuccess) return hip_fail(e, "night-sky background");
+    if (bad) {
+        set_last_error("night-sky background: a negative sensor rate, more than 1e7 expected photons in one sensor, "
+                       "or a wavelength draw outside the table");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (too_small) { set_last_error("capacity below the number of photons with the background"); return MLGPU_ERR_ARGUMENT; }
     return MLGPU_OK;
 }
 

@@ -1,9 +1,11 @@
 """The sensor signal chain on the GPU (merlict_signal_processing):
 per-sensor photon pipelines sorted by arrival time, the photo-electric
-converter and the single-photon pulse extraction (include/mlgpu.h)."""
+converter, the night-sky-background injection and the single-photon pulse
+extraction (include/mlgpu.h)."""
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
 from typing import Optional
 
@@ -100,3 +102,50 @@ def extract_pulses(pulses: Pulses, num_sensors: int, time_slice_duration: float,
                                          arrival_time_std, seed, C.c_void_p(out.data_ptr()), _st(stream, dev)),
             "mlgpu_extract_pulses")
     return out
+
+
+def nsb_light_rate(flux_x, flux_y, max_fov_radius: float, max_aperture_radius: float) -> float:
+    """Rate [1/s] of the night-sky light over the whole light-field sensor
+    (night_sky_background::Light, Light.cpp:9-28; mlgpu_nsb_light_rate):
+    max_fov_radius is half the sensor's max FoV diameter [rad]."""
+    fx = np.ascontiguousarray(flux_x, np.float64)
+    fy = np.ascontiguousarray(flux_y, np.float64)
+    r = C.c_double()
+    N.check(N.lib().mlgpu_nsb_light_rate(fx.ctypes.data, fy.ctypes.data, len(fx), max_fov_radius,
+                                         max_aperture_radius, C.byref(r)), "mlgpu_nsb_light_rate")
+    return r.value
+
+
+def inject_nsb(pipes: Pipelines, num_sensors: int, efficiencies: torch.Tensor, flux_x, flux_y, rate: float,
+               exposure_time: float, seed: int = 0, stream=None):
+    """(pipelines with the night-sky background, exposure start time)
+    (inject_nsb_into_photon_pipeline, Injector.cpp:11-107;
+    mlgpu_inject_nsb).  efficiencies: float32 per sensor on the GPU."""
+    dev = pipes.offsets.device
+    if efficiencies.dtype != torch.float32 or efficiencies.device != dev or len(efficiencies) != num_sensors:
+        raise ValueError("efficiencies must be num_sensors float32 values on the pipelines' device")
+    eff = efficiencies.contiguous()
+    fx = np.ascontiguousarray(flux_x, np.float64)
+    fy = np.ascontiguousarray(flux_y, np.float64)
+    nsb = N.Nsb(fx.ctypes.data, fy.ctypes.data, len(fx), rate, exposure_time, seed)
+    m = len(pipes.arrival_time)
+    mean = rate * exposure_time * float(eff.double().sum().item()) / max(num_sensors, 1)
+    # a first guess (a larger count comes back and is allocated exactly)
+    cap = m + int(min(mean + 10.0 * math.sqrt(mean + 1.0), float(1 << 26))) + 16 * (num_sensors + 1)
+    for _ in range(2):
+        out = Pipelines(torch.zeros(num_sensors + 1, dtype=torch.int64, device=dev),
+                        torch.empty(cap, dtype=torch.float64, device=dev),
+                        torch.empty(cap, dtype=torch.float64, device=dev),
+                        torch.empty(cap, dtype=torch.int32, device=dev))
+        k, start = C.c_int64(), C.c_double()
+        rc = N.lib().mlgpu_inject_nsb(C.byref(pipes.c_struct()), num_sensors, C.c_void_p(eff.data_ptr()),
+                                      C.byref(nsb), C.byref(out.c_struct()), C.byref(k), C.byref(start),
+                                      _st(stream, dev))
+        if rc == N.MLGPU_OK:
+            n = k.value
+            return Pipelines(out.offsets, out.arrival_time[:n], out.wavelength[:n], out.truth_id[:n]), start.value
+        if k.value > cap:  # the count is known: once more with room for it
+            cap = k.value
+            continue
+        N.check(rc, "mlgpu_inject_nsb")
+    N.check(rc, "mlgpu_inject_nsb")

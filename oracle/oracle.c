@@ -1396,3 +1396,174 @@ void oracle_extract_pulses(const int64_t* offsets, const double* arrival_time, i
         slice[j] = (sl >= 0 && sl < 100) ? (int16_t)sl : (int16_t)-1;
     }
 }
+
+/* ------------------------------------------------ night-sky background */
+
+/* Func1 from (x, y) pairs in the given order: slopes first, then sorted;
+   0 on a duplicate x (Func1.cpp:22-50) */
+static int nsb_func(const double* x, const double* y, int32_t n, sig_pt* p) {
+    for (int32_t i = 0; i < n; i++) {
+        p[i].x = x[i]; p[i].y = y[i];
+        p[i].slope = (i != n - 1) ? (y[i] - y[i + 1]) / (x[i] - x[i + 1]) : 0.0;
+    }
+    qsort(p, (size_t)n, sizeof(sig_pt), sig_pt_cmp);
+    for (int32_t i = 0; i + 1 < n; i++) if (p[i].x == p[i + 1].x) return 0;
+    return 1;
+}
+
+#define NSB_STEPS 4096
+/* SamplesFromDistribution (SamplesFromDistribution.cpp:8-17) over
+   function::integral and function::inverse (tools.cpp:37-76); cdf gets
+   NSB_STEPS + 1 points; returns the total integral, or -1 on failure */
+static double nsb_cdf(const double* fx, const double* fy, int32_t n, sig_pt* cdf) {
+    sig_pt* f = (sig_pt*)malloc(sizeof(sig_pt) * (size_t)n);
+    sig_pt* F = (sig_pt*)malloc(sizeof(sig_pt) * (NSB_STEPS + 1));
+    double* X = (double*)malloc(sizeof(double) * (NSB_STEPS + 1));
+    double* Y = (double*)malloc(sizeof(double) * (NSB_STEPS + 1));
+    double total = -1.0, v;
+    if (!nsb_func(fx, fy, n, f)) goto out;
+    {
+        const double lo = f[0].x, hi = f[n - 1].x, step = (hi - lo) / (double)NSB_STEPS;
+        X[0] = lo; Y[0] = 0.0;
+        for (int i = 1; i < NSB_STEPS; i++) {
+            X[i] = lo + (double)i * step;
+            if (!sig_eval(f, n, X[i], &v)) goto out;
+            Y[i] = v * step + Y[i - 1];
+        }
+        if (!sig_eval(f, n, nextafter(hi, lo), &v)) goto out;
+        X[NSB_STEPS] = hi; Y[NSB_STEPS] = v * step + Y[NSB_STEPS - 1];
+        if (!nsb_func(X, Y, NSB_STEPS + 1, F)) goto out;
+        double mx = F[0].y;
+        for (int i = 0; i <= NSB_STEPS; i++) if (F[i].y > mx) mx = F[i].y;
+        /* the inverse on the grid of F */
+        const double Flo = F[0].x, Fhi = F[NSB_STEPS].x, Fstep = (Fhi - Flo) / (double)NSB_STEPS;
+        if (!sig_eval(F, NSB_STEPS + 1, Flo, &v)) goto out;
+        X[0] = v; Y[0] = Flo;
+        for (int i = 1; i < NSB_STEPS; i++) {
+            const double xi = Flo + (double)i * Fstep;
+            if (!sig_eval(F, NSB_STEPS + 1, xi, &v)) goto out;
+            X[i] = v; Y[i] = xi;
+        }
+        if (!sig_eval(F, NSB_STEPS + 1, nextafter(Fhi, Flo), &v)) goto out;
+        X[NSB_STEPS] = v; Y[NSB_STEPS] = Fhi;
+        if (!nsb_func(X, Y, NSB_STEPS + 1, cdf)) goto out;
+        total = mx;
+    }
+out:
+    free(f); free(F); free(X); free(Y);
+    return total;
+}
+
+double oracle_nsb_light_rate(const double* fx, const double* fy, int32_t n, double max_fov_radius,
+                             double max_aperture_radius) {
+    sig_pt* cdf = (sig_pt*)malloc(sizeof(sig_pt) * (NSB_STEPS + 1));
+    const double total = nsb_cdf(fx, fy, n, cdf);
+    free(cdf);
+    if (total < 0.0) return -1.0;
+    /* Light.cpp:15-27: fov 1.2 x, aperture 1.05 x (NightSkyBackground.h) */
+    const double fov = 1.20 * max_fov_radius, ap = 1.05 * max_aperture_radius;
+    const double solid = 2.0 * M_PI * (1.0 - cos(fov));
+    const double area = M_PI * ap * ap;
+    return total * area * solid;
+}
+
+typedef struct { double t; int64_t order; } nsb_key;
+static int nsb_key_cmp(const void* a, const void* b) {
+    const nsb_key* x = (const nsb_key*)a;
+    const nsb_key* y = (const nsb_key*)b;
+    if (x->t != y->t) return x->t < y->t ? -1 : 1;
+    return x->order < y->order ? -1 : (x->order > y->order);
+}
+
+/* inject_nsb_into_photon_pipeline (Injector.cpp:11-107) with sensor s
+   drawing from Philox stream 9 of (seed, s); returns the merged count, -1
+   for a bad table or draw, -2 when capacity is too small */
+int64_t oracle_inject_nsb(const int64_t* offsets, const double* arrival_time, const double* wavelength,
+                          const int32_t* truth_id, int32_t num_sensors, const float* eff, const double* flux_x,
+                          const double* flux_y, int32_t flux_n, double rate, double exposure_time, uint64_t seed,
+                          int64_t* out_offsets, double* out_time, double* out_wavelength, int32_t* out_truth,
+                          int64_t capacity, double* start_out) {
+    *start_out = 0.0;
+    if (num_sensors == 0) return 0;
+    sig_pt* cdf = (sig_pt*)malloc(sizeof(sig_pt) * (NSB_STEPS + 1));
+    const double total = nsb_cdf(flux_x, flux_y, flux_n, cdf);
+    if (total < 0.0) { free(cdf); return -1; }
+    const int64_t m = offsets[num_sensors];
+    /* the mode of the arrival times (Injector.cpp:23-66) */
+    double lo = 1.7976931348623157e308, hi = 2.2250738585072014e-308;
+    for (int32_t s = 0; s < num_sensors; s++)
+        if (offsets[s + 1] > offsets[s]) {
+            if (arrival_time[offsets[s]] < lo) lo = arrival_time[offsets[s]];
+            if (arrival_time[offsets[s + 1] - 1] > hi) hi = arrival_time[offsets[s + 1] - 1];
+        }
+    double mode = 0.0;
+    if (m > 0) {
+        const unsigned count = (unsigned)(2u + sqrt((double)(unsigned)m));
+        double* edge = (double*)malloc(sizeof(double) * count);
+        unsigned* bins = (unsigned*)calloc(count, sizeof(unsigned));
+        const double step = (hi - lo) / (double)(count - 1u);
+        for (unsigned i = 0; i < count; i++) edge[i] = lo + step * (double)i;
+        for (int64_t j = 0; j < m; j++) {
+            /* first edge > t */
+            unsigned a = 0, b = count;
+            while (a < b) { const unsigned mid = (a + b) / 2; if (edge[mid] > arrival_time[j]) b = mid; else a = mid + 1; }
+            if (a == 0 || a == count) continue;
+            bins[a - 1]++;
+        }
+        unsigned best = 0, best_c = 0;
+        for (unsigned i = 0; i + 1 < count; i++) if (bins[i] > best_c) { best_c = bins[i]; best = i; }
+        mode = 0.5 * (edge[best + 1] + edge[best]);
+        free(edge); free(bins);
+    }
+    const double start = mode - 0.5 * exposure_time;
+    *start_out = start;
+    int64_t k = 0, rc = 0;
+    nsb_key* keys = NULL;
+    double* bt = NULL;
+    double* bl = NULL;
+    size_t cap_keys = 0, cap_b = 0;
+    for (int32_t s = 0; s < num_sensors; s++) {
+        const double r = rate * (double)eff[s] / (double)num_sensors;
+        if (!(r >= 0.0) || r * exposure_time > 1e7) { rc = -1; goto done; }
+        sig_gen g;
+        sig_init(&g, seed, (uint64_t)s, 9u);
+        int64_t nb = 0;
+        double sum = -log(sig_uniform(&g)) / r;
+        while (sum < exposure_time) {
+            const double next = -log(sig_uniform(&g)) / r;
+            if ((size_t)nb >= cap_b) {
+                cap_b = cap_b ? 2 * cap_b : 64;
+                bt = (double*)realloc(bt, sizeof(double) * cap_b);
+                bl = (double*)realloc(bl, sizeof(double) * cap_b);
+            }
+            bt[nb++] = start + sum;
+            sum += next;
+        }
+        for (int64_t q = 0; q < nb; q++)
+            if (!sig_eval(cdf, NSB_STEPS + 1, sig_uniform(&g) * total, &bl[q])) { rc = -1; goto done; }
+        const int64_t na = offsets[s + 1] - offsets[s];
+        if ((size_t)(na + nb) > cap_keys) {
+            cap_keys = (size_t)(na + nb) * 2;
+            keys = (nsb_key*)realloc(keys, sizeof(nsb_key) * cap_keys);
+        }
+        for (int64_t a = 0; a < na; a++) { keys[a].t = arrival_time[offsets[s] + a]; keys[a].order = a; }
+        for (int64_t q = 0; q < nb; q++) { keys[na + q].t = bt[q]; keys[na + q].order = na + q; }
+        /* stable by time: a pipeline photon before a background photon of
+           the same time */
+        qsort(keys, (size_t)(na + nb), sizeof(nsb_key), nsb_key_cmp);
+        out_offsets[s] = k;
+        for (int64_t i = 0; i < na + nb; i++) {
+            if (k >= capacity) { rc = -2; goto done; }
+            const int64_t o = keys[i].order;
+            out_time[k] = keys[i].t - start;
+            if (o < na) { out_wavelength[k] = wavelength[offsets[s] + o]; out_truth[k] = truth_id[offsets[s] + o]; }
+            else { out_wavelength[k] = bl[o - na]; out_truth[k] = -100; }
+            k++;
+        }
+    }
+    out_offsets[num_sensors] = k;
+    rc = k;
+done:
+    free(cdf); free(keys); free(bt); free(bl);
+    return rc;
+}

@@ -132,6 +132,18 @@ int64_t oracle_photoelectric(const int64_t* offsets, const double* arrival_time,
 void oracle_extract_pulses(const int64_t* offsets, const double* arrival_time, int32_t num_sensors,
                            double duration, double std_dev, uint64_t seed, int16_t* slice);
 
+/* Night-sky background (Injector.cpp:11-107): sensor s draws from Philox
+   stream 9 of (seed, s); the merged pipelines (arrival times minus the
+   exposure start, stored in *start); -1 bad table/rate/draw, -2 capacity. */
+int64_t oracle_inject_nsb(const int64_t* offsets, const double* arrival_time, const double* wavelength,
+                          const int32_t* truth_id, int32_t num_sensors, const float* eff, const double* flux_x,
+                          const double* flux_y, int32_t flux_n, double rate, double exposure_time, uint64_t seed,
+                          int64_t* out_offsets, double* out_time, double* out_wavelength, int32_t* out_truth,
+                          int64_t capacity, double* start);
+/* night_sky_background::Light::rate (Light.cpp:9-28); -1 for a bad table */
+double oracle_nsb_light_rate(const double* flux_x, const double* flux_y, int32_t flux_n, double max_fov_radius,
+                             double max_aperture_radius);
+
 #ifdef __cplusplus
 }
 #endif

@@ -418,6 +418,13 @@ def _sig_lib():
                                            C.POINTER(C.c_int64), _P_F64, _P_I32, C.c_int64]
         L.oracle_extract_pulses.argtypes = [C.POINTER(C.c_int64), _P_F64, C.c_int32, C.c_double, C.c_double,
                                             C.c_uint64, C.POINTER(C.c_int16)]
+        L.oracle_inject_nsb.restype = C.c_int64
+        L.oracle_inject_nsb.argtypes = [C.POINTER(C.c_int64), _P_F64, _P_F64, _P_I32, C.c_int32, C.POINTER(C.c_float),
+                                        _P_F64, _P_F64, C.c_int32, C.c_double, C.c_double, C.c_uint64,
+                                        C.POINTER(C.c_int64), _P_F64, _P_F64, _P_I32, C.c_int64,
+                                        C.POINTER(C.c_double)]
+        L.oracle_nsb_light_rate.restype = C.c_double
+        L.oracle_nsb_light_rate.argtypes = [_P_F64, _P_F64, C.c_int32, C.c_double, C.c_double]
         L._sig_ready = True
     return L
 
@@ -471,6 +478,75 @@ def extract_pulses(offsets, arrival_time, num_sensors, duration, std_dev, seed):
     _sig_lib().oracle_extract_pulses(off.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(t, _P_F64), num_sensors,
                                      duration, std_dev, seed, out.ctypes.data_as(C.POINTER(C.c_int16)))
     return out
+
+
+def nsb_light_rate(flux_x, flux_y, max_fov_radius, max_aperture_radius) -> float:
+    """night_sky_background::Light::rate (oracle_nsb_light_rate)."""
+    fx, fy = np.ascontiguousarray(flux_x, np.float64), np.ascontiguousarray(flux_y, np.float64)
+    return float(_sig_lib().oracle_nsb_light_rate(_ptr(fx, _P_F64), _ptr(fy, _P_F64), len(fx), max_fov_radius,
+                                                  max_aperture_radius))
+
+
+def inject_nsb(offsets, arrival_time, wavelength, truth_id, efficiencies, flux_x, flux_y, rate, exposure_time, seed):
+    """(offsets, time, wavelength, truth, start) of the pipelines with the
+    night-sky background (oracle_inject_nsb)."""
+    off = np.ascontiguousarray(offsets, np.int64)
+    ns = len(off) - 1
+    t = np.ascontiguousarray(arrival_time, np.float64)
+    lam = np.ascontiguousarray(wavelength, np.float64)
+    tr = np.ascontiguousarray(truth_id, np.int32)
+    eff = np.ascontiguousarray(efficiencies, np.float32)
+    fx, fy = np.ascontiguousarray(flux_x, np.float64), np.ascontiguousarray(flux_y, np.float64)
+    mean = float(rate) * float(exposure_time) * float(eff.sum(dtype=np.float64)) / max(ns, 1)
+    cap = len(t) + int(mean + 10 * np.sqrt(mean + 1) + 64 * (ns + 1))
+    oo = np.zeros(ns + 1, np.int64)
+    ot, ol, otr = np.empty(cap), np.empty(cap), np.empty(cap, np.int32)
+    start = C.c_double()
+    k = _sig_lib().oracle_inject_nsb(off.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(t, _P_F64), _ptr(lam, _P_F64),
+                                     _ptr(tr, _P_I32), ns, eff.ctypes.data_as(C.POINTER(C.c_float)),
+                                     _ptr(fx, _P_F64), _ptr(fy, _P_F64), len(fx), rate, exposure_time, seed,
+                                     oo.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(ot, _P_F64), _ptr(ol, _P_F64),
+                                     _ptr(otr, _P_I32), cap, C.byref(start))
+    if k == -1:
+        raise ValueError("bad flux table, sensor rate or wavelength draw")
+    if k < 0:
+        raise RuntimeError("capacity")
+    return oo, ot[:k], ol[:k], otr[:k], start.value
+
+
+def ref_nsb(offsets, arrival_time, wavelength, truth_id, efficiencies, flux_x, flux_y, exposure_time, seed,
+            geometry) -> dict:
+    """The reference's inject_nsb_into_photon_pipeline with sensor s drawing
+    from Philox stream 9 of (seed, s) (harness `nsb`); geometry = (focal
+    length, aperture radius, FoV diameter [deg], pixel FoV [deg], paxels on
+    the pixel diagonal) of the light-field sensor whose lixels are the
+    sensors."""
+    off = np.ascontiguousarray(offsets, np.int64)
+    ns = len(off) - 1
+    parts = [np.array([ns], np.int32).tobytes(), off.tobytes(),
+             np.ascontiguousarray(arrival_time, np.float64).tobytes(),
+             np.ascontiguousarray(wavelength, np.float64).tobytes(),
+             np.ascontiguousarray(truth_id, np.int32).tobytes(),
+             np.ascontiguousarray(efficiencies, np.float32).tobytes(), np.array([len(flux_x)], np.int32).tobytes(),
+             np.ascontiguousarray(flux_x, np.float64).tobytes(), np.ascontiguousarray(flux_y, np.float64).tobytes(),
+             np.array([exposure_time], np.float64).tobytes(), np.array([seed], np.uint64).tobytes()]
+    with tempfile.TemporaryDirectory() as d:
+        i, o = Path(d) / "in.bin", Path(d) / "out.bin"
+        i.write_bytes(b"".join(parts))
+        _run_ref(["nsb", i, o] + [repr(float(v)) if k < 4 else str(int(v)) for k, v in enumerate(geometry)])
+        raw = o.read_bytes()
+    head = np.frombuffer(raw[:32], np.float64)
+    pos = 32
+    oo = np.frombuffer(raw[pos:pos + 8 * (ns + 1)], np.int64).copy()
+    pos += 8 * (ns + 1)
+    k = int(oo[-1])
+    t = np.frombuffer(raw[pos:pos + 8 * k], np.float64).copy()
+    pos += 8 * k
+    lam = np.frombuffer(raw[pos:pos + 8 * k], np.float64).copy()
+    pos += 8 * k
+    tr = np.frombuffer(raw[pos:pos + 4 * k], np.int32).copy()
+    return {"rate": float(head[0]), "start": float(head[1]), "max_fov_radius": float(head[2]),
+            "max_aperture_radius": float(head[3]), "offsets": oo, "time": t, "wavelength": lam, "truth": tr}
 
 
 def ref_signal(sensor_id, time_of_flight, wavelength, truth_id, num_sensors, qe_x, qe_y, dark_rate, crosstalk,

@@ -34,6 +34,8 @@
 #include "merlict_corsika/PhotonFactory.h"
 #include "merlict_corsika/eventio.h"
 #include "merlict_signal_processing/signal_processing.h"
+#include "merlict_portal_plenoscope/night_sky_background/Injector.h"
+#include "merlict_portal_plenoscope/night_sky_background/Light.h"
 namespace sp = signal_processing;
 
 namespace ml = merlict;
@@ -541,6 +543,114 @@ static int cmd_signal(int argc, char** argv) {
     return 0;
 }
 
+// The Philox stream of sensor s (stream 9 of (seed, s)) for each sensor
+// in turn, as inject_nsb_into_photon_pipeline consumes them: the
+// exponential gaps of sensor s until their running sum reaches the
+// exposure, then one uniform per background photon for its wavelength.
+class NsbStreams : public ml::random::Generator {
+    uint64_t seed_nsb_;
+    std::vector<double> rate_;
+    double exposure_;
+    size_t s_ = 0;
+    bool gaps_ = true;
+    double sum_ = 0.0;
+    int64_t draws_ = 0, left_ = 0;
+    PhiloxGenerator g_;
+
+    void begin() {
+        g_ = PhiloxGenerator(seed_nsb_, static_cast<uint64_t>(s_), 9u);
+        gaps_ = true;
+        sum_ = 0.0;
+        draws_ = 0;
+        left_ = 0;
+    }
+
+ public:
+    NsbStreams(uint64_t seed, std::vector<double> rate, double exposure)
+        : seed_nsb_(seed), rate_(std::move(rate)), exposure_(exposure), g_(seed, 0, 9u) { begin(); }
+    double uniform() override {
+        while (!gaps_ && left_ == 0) { s_++; begin(); }
+        const double u = g_.uniform();
+        if (gaps_) {
+            const double d = -log(u) / rate_.at(s_);
+            sum_ = draws_ == 0 ? d : sum_ + d;
+            draws_++;
+            if (!(sum_ < exposure_)) { gaps_ = false; left_ = draws_ - 1; }
+        } else {
+            left_--;
+        }
+        return u;
+    }
+    uint64_t create_seed() override { return 0; }
+    double normal(const double, const double) override { throw std::logic_error("no normal draws here"); }
+};
+
+// nsb <in> <out> <focal> <aperture_radius> <fov_deg> <pixel_fov_deg> <paxel>
+static int cmd_nsb(int argc, char** argv) {
+    if (argc != 9) return 2;
+    plenoscope::light_field_sensor::Config cfg;
+    static ml::function::Func1 refr({{200e-9, 1.49}, {1200e-9, 1.49}});
+    static ml::function::Func1 refl({{200e-9, 0.0}, {1200e-9, 0.0}});
+    cfg.expected_imaging_system_focal_length = std::atof(argv[4]);
+    cfg.expected_imaging_system_max_aperture_radius = std::atof(argv[5]);
+    cfg.max_FoV_diameter = ml::deg2rad(std::atof(argv[6]));
+    cfg.pixel_FoV_hex_flat2flat = ml::deg2rad(std::atof(argv[7]));
+    cfg.num_paxel_on_pixel_diagonal = static_cast<unsigned>(std::atoi(argv[8]));
+    cfg.lens_refraction = &refr;
+    cfg.bin_reflection = &refl;
+    plenoscope::light_field_sensor::Geometry geo(cfg);
+    std::ifstream in(argv[2], std::ios::binary);
+    int32_t ns = 0, fn = 0;
+    in.read(reinterpret_cast<char*>(&ns), 4);
+    if (static_cast<unsigned>(ns) != geo.num_lixel()) {
+        std::fprintf(stderr, "need %u sensors\n", geo.num_lixel());
+        return 1;
+    }
+    std::vector<int64_t> off(ns + 1);
+    in.read(reinterpret_cast<char*>(off.data()), (ns + 1) * 8);
+    const int64_t m = off[ns];
+    std::vector<double> t(m), lam(m);
+    std::vector<int32_t> truth(m);
+    std::vector<float> eff(ns);
+    in.read(reinterpret_cast<char*>(t.data()), m * 8);
+    in.read(reinterpret_cast<char*>(lam.data()), m * 8);
+    in.read(reinterpret_cast<char*>(truth.data()), m * 4);
+    in.read(reinterpret_cast<char*>(eff.data()), ns * 4);
+    in.read(reinterpret_cast<char*>(&fn), 4);
+    std::vector<double> fx(fn), fy(fn);
+    in.read(reinterpret_cast<char*>(fx.data()), fn * 8);
+    in.read(reinterpret_cast<char*>(fy.data()), fn * 8);
+    double expo = 0;
+    uint64_t seed = 0;
+    in.read(reinterpret_cast<char*>(&expo), 8);
+    in.read(reinterpret_cast<char*>(&seed), 8);
+    std::vector<std::vector<double>> table;
+    for (int32_t i = 0; i < fn; i++) table.push_back({fx[i], fy[i]});
+    const ml::function::Func1 flux(table);
+    const plenoscope::night_sky_background::Light light(&geo, &flux);
+    std::vector<std::vector<sp::PipelinePhoton>> pipes(static_cast<size_t>(ns));
+    for (int32_t s = 0; s < ns; s++)
+        for (int64_t j = off[s]; j < off[s + 1]; j++) pipes[s].emplace_back(t[j], lam[j], truth[j]);
+    std::vector<double> rate(static_cast<size_t>(ns));
+    for (int32_t s = 0; s < ns; s++) rate[s] = light.rate * eff.at(s) / light.sensor_geometry->num_lixel();
+    NsbStreams gen(seed, rate, expo);
+    double start = 0.0;
+    plenoscope::night_sky_background::inject_nsb_into_photon_pipeline(&pipes, expo, &eff, &light, &start, &gen);
+    std::vector<int64_t> po{0};
+    std::vector<double> pt, pl;
+    std::vector<int32_t> ptr;
+    for (const auto& pipe : pipes) {
+        for (const auto& ph : pipe) { pt.push_back(ph.arrival_time); pl.push_back(ph.wavelength); ptr.push_back(ph.simulation_truth_id); }
+        po.push_back(static_cast<int64_t>(pt.size()));
+    }
+    std::ofstream o(argv[3], std::ios::binary);
+    auto w = [&](const void* p, size_t b) { o.write(static_cast<const char*>(p), static_cast<std::streamsize>(b)); };
+    const double head[4] = {light.rate, start, geo.max_FoV_radius(), geo.expected_imaging_system_max_aperture_radius()};
+    w(head, sizeof(head));
+    w(po.data(), po.size() * 8); w(pt.data(), pt.size() * 8); w(pl.data(), pl.size() * 8); w(ptr.data(), ptr.size() * 4);
+    return 0;
+}
+
 static int cmd_lensmaker(int argc, char** argv) {
     if (argc < 5) return 2;
     ml::lens_maker::Config c;
@@ -656,6 +766,7 @@ int main(int argc, char** argv) {
         else if (c == "corsika-extract") rc = cmd_corsika_extract(argc, argv);
         else if (c == "corsika") rc = cmd_corsika(argc, argv);
         else if (c == "signal") rc = cmd_signal(argc, argv);
+        else if (c == "nsb") rc = cmd_nsb(argc, argv);
         if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", c.c_str());
         return rc;
     } catch (const std::exception& e) {

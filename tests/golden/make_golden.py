@@ -5,10 +5,11 @@ oracle/Makefile, with the Philox stream plugged into merlict's
 random::Generator) on seeded inputs and stores inputs and outputs as small
 .npz files.  Run in the container that has the reference mounted:
 
-    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika | signal]
+    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika | signal | nsb]
 """
 from __future__ import annotations
 
+import json
 import sys
 from pathlib import Path
 
@@ -150,9 +151,51 @@ def signal_fixtures():
           int(ref["extracted_offsets"][-1]), "extracted")
 
 
+NSB_GEOMETRY = (10.0, 3.0, 1.0, 0.2, 3)  # focal length, aperture radius, FoV [deg], pixel FoV [deg], paxels
+
+
+def nsb_fixtures():
+    """The reference's night-sky-background injection on synthetic pipelines
+    of the lixels of a small light-field sensor, with the flux of the
+    plenoscope propagation example card (apps/examples/PlenoscopePropagation/
+    propagation_input_card.json, kept as tests/golden/
+    plenoscope_propagation_input_card.json)."""
+    card = json.loads((HERE / "plenoscope_propagation_input_card.json").read_text())
+    flux = np.array(card["night_sky_background_ligth"]["flux_vs_wavelength"], np.float64)
+    exposure = float(card["night_sky_background_ligth"]["exposure_time"])
+    meta, _ = O.ref_lightfield(*NSB_GEOMETRY, 1.49, 0.0, 1.1)
+    ns = int(meta["num_lixel"])
+    rng = np.random.default_rng(23)
+    counts = rng.poisson(6.0, ns)
+    counts[:3] = 0
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    m = int(off[-1])
+    t = np.concatenate([np.sort(rng.normal(30e-9, 3e-9, c)) for c in counts]) if m else np.zeros(0)
+    lam = rng.uniform(280e-9, 700e-9, m)
+    truth = rng.integers(0, 100000, m).astype(np.int32)
+    eff = rng.uniform(0.5, 1.0, ns).astype(np.float32)
+    eff[5] = 0.0
+    arrays = {"geometry": np.array(NSB_GEOMETRY, np.float64), "flux_x": flux[:, 0].copy(),
+              "flux_y": flux[:, 1].copy(), "exposure_time": np.array([exposure]),
+              "seed": np.array([17], np.uint64), "in_offsets": off, "in_time": t, "in_wavelength": lam,
+              "in_truth": truth, "efficiencies": eff}
+    ref = O.ref_nsb(off, t, lam, truth, eff, flux[:, 0], flux[:, 1], exposure, 17, NSB_GEOMETRY)
+    arrays.update({f"out_{k}": np.asarray(v) for k, v in ref.items()})
+    # no pipeline photon: the exposure starts at -exposure / 2
+    z = O.ref_nsb(np.zeros(ns + 1, np.int64), np.zeros(0), np.zeros(0), np.zeros(0, np.int32), eff, flux[:, 0],
+                  flux[:, 1], exposure, 18, NSB_GEOMETRY)
+    arrays.update({f"empty_{k}": np.asarray(v) for k, v in z.items()})
+    np.savez_compressed(HERE / "nsb_injection.npz", **arrays)
+    print("nsb_injection", ns, "lixels", m, "photons ->", int(ref["offsets"][-1]), "rate", ref["rate"],
+          "start", ref["start"], "| empty ->", int(z["offsets"][-1]), "start", z["start"])
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/merlict_golden is not built")
+    if len(sys.argv) > 1 and sys.argv[1] == "nsb":
+        nsb_fixtures()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "signal":
         signal_fixtures()
         return

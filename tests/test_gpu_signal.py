@@ -1,7 +1,8 @@
 """The sensor signal chain on the GPU (mlgpu_sensor_pipelines,
-mlgpu_photoelectric_convert, mlgpu_extract_pulses) against the reference's
-(golden vectors of merlict_signal_processing under the same per-sensor
-Philox streams) and the oracle."""
+mlgpu_inject_nsb, mlgpu_photoelectric_convert, mlgpu_extract_pulses) against
+the reference's (golden vectors of merlict_signal_processing and the
+night-sky-background injector under the same Philox streams) and the
+oracle."""
 import numpy as np
 import pytest
 import torch
@@ -9,7 +10,8 @@ import torch
 import pyoracle as O
 from merlict_development_kit_amd._native import MlgpuError
 from merlict_development_kit_amd import GpuScene, PropagationConfig, SceneDescription, sources
-from merlict_development_kit_amd.signal_chain import extract_pulses, photoelectric_convert, sensor_pipelines
+from merlict_development_kit_amd.signal_chain import (Pipelines, extract_pulses, inject_nsb, photoelectric_convert,
+                                                      sensor_pipelines)
 
 pytestmark = pytest.mark.gpu
 
@@ -86,3 +88,86 @@ def test_gpu_signal_chain_errors_and_empty_input(cuda):
         photoelectric_convert(pipes, 2, qx, qy)
     with pytest.raises(MlgpuError):  # crosstalk >= 1 (Converter::Converter)
         photoelectric_convert(pipes, 2, np.array([100e-9, 1200e-9]), qy, crosstalk=1.0)
+
+
+def _pipes(cuda, off, t, lam, tr):
+    T = lambda x, dt: torch.tensor(np.ascontiguousarray(x), dtype=dt, device=cuda)
+    return Pipelines(T(off, torch.int64), T(t, torch.float64), T(lam, torch.float64), T(tr, torch.int32))
+
+
+def _same_times(t, ref, truth):
+    """Pipeline photons keep their time exactly; a background photon's time
+    is a running sum of -log(u) / rate, and the device log differs from
+    glibc's by at most 1 ulp: 1e-14 relative."""
+    bg = truth == -100
+    np.testing.assert_array_equal(t[~bg], ref[~bg])
+    np.testing.assert_allclose(t[bg], ref[bg], rtol=1e-14, atol=1e-22)
+
+
+def test_gpu_nsb_matches_reference(cuda, golden):
+    """mlgpu_inject_nsb on the golden pipelines of a 91-lixel sensor (and on
+    empty pipelines) equals the reference's injector bit for bit: exposure
+    start, offsets, arrival times, wavelengths, truth ids."""
+    g = golden("nsb_injection")
+    ns = len(g["in_offsets"]) - 1
+    eff = torch.tensor(g["efficiencies"], device=cuda)
+    T = float(g["exposure_time"][0])
+    cases = {"out": (g["in_offsets"], g["in_time"], g["in_wavelength"], g["in_truth"], 17),
+             "empty": (np.zeros(ns + 1, np.int64), np.zeros(0), np.zeros(0), np.zeros(0, np.int32), 18)}
+    for pre, (off, t, lam, tr, seed) in cases.items():
+        out, start = inject_nsb(_pipes(cuda, off, t, lam, tr), ns, eff, g["flux_x"], g["flux_y"],
+                                float(g[pre + "_rate"]), T, seed)
+        assert start == float(g[pre + "_start"]), pre
+        np.testing.assert_array_equal(out.offsets.cpu().numpy(), g[pre + "_offsets"])
+        _same_times(out.arrival_time.cpu().numpy(), g[pre + "_time"], g[pre + "_truth"])
+        np.testing.assert_array_equal(out.wavelength.cpu().numpy(), g[pre + "_wavelength"])
+        np.testing.assert_array_equal(out.truth_id.cpu().numpy(), g[pre + "_truth"])
+
+
+def test_gpu_nsb_of_propagated_pipelines_matches_oracle(cuda, golden):
+    """Background injected into the pipelines of 1e6 propagated FACT-like
+    photons (1440 pixels, ~2,300 photons each, ~400 background photons per
+    pixel): GPU and oracle agree bit for bit, and the converter runs on the
+    result."""
+    g = golden("nsb_injection")
+    gs = GpuScene(SceneDescription.fact(light_guides=True))
+    ph = sources.fact_bunch().generate(1_000_000, seed=5, device=cuda)
+    res = gs.propagate(ph, PropagationConfig(5, seed=6))
+    pipes = sensor_pipelines(res.sensor_id, res.error, res.time_of_flight, ph.wavelength, 1440)
+    rng = np.random.default_rng(3)
+    eff_np = rng.uniform(0.6, 1.0, 1440).astype(np.float32)
+    rate = 1440 * 1e10  # 500 photons per pixel at efficiency 1 over 50 ns
+    out, start = inject_nsb(pipes, 1440, torch.tensor(eff_np, device=cuda), g["flux_x"], g["flux_y"], rate, 50e-9,
+                            seed=21)
+    oo, ot, ol, otr, ostart = O.inject_nsb(pipes.offsets.cpu().numpy(), pipes.arrival_time.cpu().numpy(),
+                                           pipes.wavelength.cpu().numpy(), pipes.truth_id.cpu().numpy(), eff_np,
+                                           g["flux_x"], g["flux_y"], rate, 50e-9, 21)
+    assert start == ostart
+    np.testing.assert_array_equal(out.offsets.cpu().numpy(), oo)
+    _same_times(out.arrival_time.cpu().numpy(), ot, otr)
+    np.testing.assert_array_equal(out.wavelength.cpu().numpy(), ol)
+    np.testing.assert_array_equal(out.truth_id.cpu().numpy(), otr)
+    assert (otr == -100).sum() > 400_000
+    qx, qy = np.array([200e-9, 400e-9, 1200e-9]), np.array([0.3, 0.35, 0.1])
+    pulses = photoelectric_convert(out, 1440, qx, qy, 1e6, 0.1, 50e-9, seed=8)
+    assert len(pulses.arrival_time) > 0
+
+
+def test_gpu_nsb_errors_and_edges(cuda, golden):
+    g = golden("nsb_injection")
+    pipes = _pipes(cuda, np.array([0, 1, 1]), np.array([3e-9]), np.array([433e-9]), np.array([7]))
+    eff = torch.tensor([1.0, -0.5], device=cuda)
+    with pytest.raises(MlgpuError):  # a negative rate never ends the reference's Poisson loop
+        inject_nsb(pipes, 2, eff, g["flux_x"], g["flux_y"], 1e9, 50e-9)
+    with pytest.raises(MlgpuError):  # more than 1e7 expected photons in one sensor
+        inject_nsb(pipes, 2, torch.ones(2, device=cuda), g["flux_x"], g["flux_y"], 1e18, 50e-9)
+    with pytest.raises(MlgpuError):  # a flux that is zero somewhere has no inverse cdf
+        inject_nsb(pipes, 2, torch.ones(2, device=cuda), [300e-9, 400e-9, 500e-9], [1.0, 0.0, 0.0], 1e9, 50e-9)
+    # no rate: the pipelines come back shifted by the start (one photon: the mode is its time)
+    out, start = inject_nsb(pipes, 2, torch.ones(2, device=cuda), g["flux_x"], g["flux_y"], 0.0, 50e-9)
+    assert start == 3e-9 - 25e-9
+    assert out.arrival_time.cpu().tolist() == [3e-9 - start] and out.truth_id.cpu().tolist() == [7]
+    e = Pipelines(torch.zeros(1, dtype=torch.int64, device=cuda),
+                  *(torch.zeros(0, dtype=d, device=cuda) for d in (torch.float64, torch.float64, torch.int32)))
+    out, start = inject_nsb(e, 0, torch.zeros(0, device=cuda), g["flux_x"], g["flux_y"], 1e9, 50e-9)
+    assert start == 0.0 and len(out.arrival_time) == 0

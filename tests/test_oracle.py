@@ -495,7 +495,7 @@ def test_oracle_signal_chain_matches_reference(golden):
     """Pipelines, converter pulses and extracted slices of the oracle equal
     the reference's (sort_photon_pipelines_arrival_time,
     PhotoElectricConverter::Converter, extract_pulses) under the same
-    per-sensor Philox streams."""
+    Philox streams (one per photon, per sensor's accidentals, per pulse)."""
     a, ref = _signal_golden(golden)
     ns = int(a["num_sensors"][0])
     off, t, lam, tr = O.sensor_pipelines(a["sensor_id"], a["error"], a["time_of_flight"], a["wavelength"],
@@ -537,3 +537,102 @@ def test_oracle_converter_rates():
                                   1e8, 0.0, 1e-5, 5)
     assert len(pt) / ns == pytest.approx(1e3, rel=0.05)
     assert (pt >= 0).all() and (pt < 1e-5).all()
+
+
+# ----------------------------------------------------- night-sky background
+def _nsb_inputs(g, prefix):
+    if prefix == "out":
+        return g["in_offsets"], g["in_time"], g["in_wavelength"], g["in_truth"], 17
+    ns = len(g["in_offsets"]) - 1
+    return np.zeros(ns + 1, np.int64), np.zeros(0), np.zeros(0), np.zeros(0, np.int32), 18
+
+
+def test_oracle_nsb_matches_reference(golden):
+    """inject_nsb_into_photon_pipeline (Injector.cpp:11-107) of the reference,
+    driven with sensor s on Philox stream 9 of (seed, s), on pipelines of a
+    91-lixel light-field sensor and on empty pipelines: the oracle's start
+    time and merged pipelines are identical, and so is its Light rate."""
+    g = golden("nsb_injection")
+    rate = O.nsb_light_rate(g["flux_x"], g["flux_y"], float(g["out_max_fov_radius"]),
+                            float(g["out_max_aperture_radius"]))
+    assert rate == float(g["out_rate"])
+    for pre in ("out", "empty"):
+        off, t, lam, tr, seed = _nsb_inputs(g, pre)
+        oo, ot, ol, otr, start = O.inject_nsb(off, t, lam, tr, g["efficiencies"], g["flux_x"], g["flux_y"],
+                                              float(g[pre + "_rate"]), float(g["exposure_time"][0]), seed)
+        assert start == float(g[pre + "_start"])
+        np.testing.assert_array_equal(oo, g[pre + "_offsets"])
+        np.testing.assert_array_equal(ot, g[pre + "_time"])
+        np.testing.assert_array_equal(ol, g[pre + "_wavelength"])
+        np.testing.assert_array_equal(otr, g[pre + "_truth"])
+    assert float(g["empty_start"]) == -0.5 * float(g["exposure_time"][0])  # the mode of no photon is 0
+
+
+def test_c_abi_nsb_light_rate_matches_reference(golden):
+    """mlgpu_nsb_light_rate (host only) equals night_sky_background::Light's
+    rate for the golden sensor, and follows rate ~ aperture^2 (1 - cos(1.2 fov))."""
+    from merlict_development_kit_amd.signal_chain import nsb_light_rate
+    g = golden("nsb_injection")
+    fov, ap = float(g["out_max_fov_radius"]), float(g["out_max_aperture_radius"])
+    assert nsb_light_rate(g["flux_x"], g["flux_y"], fov, ap) == float(g["out_rate"])
+    r2 = nsb_light_rate(g["flux_x"], g["flux_y"], fov, 2 * ap)
+    assert r2 / float(g["out_rate"]) == pytest.approx(4.0, rel=1e-12)
+    r3 = nsb_light_rate(g["flux_x"], g["flux_y"], 2 * fov, ap)
+    assert r3 / float(g["out_rate"]) == pytest.approx((1 - math.cos(2.4 * fov)) / (1 - math.cos(1.2 * fov)),
+                                                      rel=1e-12)
+    # the integral of a flat flux of 1 over [300, 700] nm is 400 nm (to the
+    # 4096-step Riemann sum)
+    flat = nsb_light_rate([300e-9, 700e-9], [1.0, 1.0], 0.1, 1.0)
+    expect = 400e-9 * math.pi * 1.05 ** 2 * 2 * math.pi * (1 - math.cos(0.12))
+    assert flat == pytest.approx(expect, rel=1e-12)
+
+
+def test_oracle_nsb_laws():
+    """The injected background: Poisson counts per sensor with mean rate x
+    eff / num_sensors x exposure, times uniform over the exposure, wavelengths
+    distributed as the flux (KS against the flux's cumulative integral),
+    every pipeline sorted, pipeline photons kept, and the exposure centred on
+    the mode of the arrival times (found here with numpy's searchsorted over
+    the same edges)."""
+    rng = np.random.default_rng(5)
+    ns, T = 64, 50e-9
+    counts = rng.poisson(40, ns)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    m = int(off[-1])
+    t = np.concatenate([np.sort(np.concatenate([rng.normal(20e-9, 0.5e-9, c - c // 4),
+                                                rng.uniform(0, 60e-9, c // 4)])) for c in counts])
+    lam = rng.uniform(300e-9, 600e-9, m)
+    tr = np.arange(m, dtype=np.int32)
+    eff = rng.uniform(0.5, 1.0, ns).astype(np.float32)
+    fx = np.array([250e-9, 400e-9, 550e-9, 700e-9])
+    fy = np.array([1.0, 3.0, 2.0, 0.5])
+    rate = 2e9 * ns  # 100 photons per sensor at eff 1
+    oo, ot, ol, otr, start = O.inject_nsb(off, t, lam, tr, eff, fx, fy, rate, T, 9)
+    # mode: 2 + sqrt(m) edges from min to max, upper-bound binning
+    count = int(2 + math.sqrt(m))
+    edges = t.min() + (t.max() - t.min()) / (count - 1) * np.arange(count)
+    b = np.searchsorted(edges, t, side="right") - 1
+    h = np.bincount(b[(b >= 0) & (b < count - 1)], minlength=count - 1)
+    k = int(np.argmax(h))
+    assert start == pytest.approx(0.5 * (edges[k] + edges[k + 1]) - 0.5 * T, abs=1e-21)
+    assert abs(start + 0.5 * T - 20e-9) < 1e-9
+    nsb = otr == -100
+    per = np.add.reduceat(nsb.astype(np.int64), oo[:-1])
+    per = np.where(np.diff(oo) > 0, per, 0)
+    mean = rate * eff.astype(np.float64) / ns * T
+    assert abs(per.sum() - mean.sum()) < 5 * math.sqrt(mean.sum())
+    chi2 = float((((per - mean) ** 2) / mean).sum())
+    assert sp_stats.chi2.sf(chi2, ns) > 1e-4
+    tn = ot[nsb]
+    assert (tn >= -1e-22).all() and (tn < T + 1e-22).all()
+    assert sp_stats.kstest(tn / T, "uniform").pvalue > 1e-4
+    grid = np.linspace(fx[0], fx[-1], 20001)
+    cdf = np.cumsum(np.interp(grid, fx, fy))
+    cdf /= cdf[-1]
+    assert sp_stats.kstest(ol[nsb], lambda x: np.interp(x, grid, cdf)).pvalue > 1e-4
+    for s in range(ns):
+        seg = ot[oo[s]:oo[s + 1]]
+        assert (np.diff(seg) >= 0).all()
+    keep = ~nsb
+    np.testing.assert_array_equal(np.sort(otr[keep]), tr)
+    np.testing.assert_array_equal(ot[keep][np.argsort(otr[keep])], t - start)
