@@ -239,6 +239,64 @@ def hybrid_stream_leg(scene: GpuScene, trained, dev, events: int = 4, bunches_pe
     }
 
 
+_PLENOSCOPE = {}
+
+
+def plenoscope_scene():
+    """The plenoscope of merlict's calibration example (built once)."""
+    if "scene" not in _PLENOSCOPE:
+        _PLENOSCOPE["desc"] = SceneDescription.plenoscope()
+        _PLENOSCOPE["scene"] = GpuScene(_PLENOSCOPE["desc"])
+    return _PLENOSCOPE["desc"], _PLENOSCOPE["scene"]
+
+
+def calibration_leg(dev, cpu_threads: int, num_blocks: int = 8, photons_per_block: int = 1_000_000,
+                    cpu: bool = True) -> dict:
+    """Light-field calibration of the plenoscope as the reference's
+    Calibrator runs it by default (8 blocks of 1e6 photons, 5 interactions):
+    per block the calibration photons are drawn, propagated, turned into
+    records and added to the per-lixel statistics; the whole run is timed,
+    and the records + statistics part alone.  The CPU baseline is the
+    reference's propagation of the same calibration photons (the
+    statistics' share of the reference's time is negligible)."""
+    import math as _m
+    from merlict_development_kit_amd import calibration as CAL
+    desc, scene = plenoscope_scene()
+    lixels = 515_023
+    src = CAL.calibration_source(35.5, _m.radians(3.25))
+    CAL.run_calibration(scene, lixels, 35.5, _m.radians(3.25), CAL.CalibrationConfig(1, 1 << 18), seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = CAL.run_calibration(scene, lixels, 35.5, _m.radians(3.25), CAL.CalibrationConfig(num_blocks, photons_per_block),
+                             seed=2)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    # the statistics part alone, on one propagated block
+    ph = src.generate(photons_per_block, seed=3, device=dev)
+    res = scene.propagate(ph, PropagationConfig(5, seed=3))
+    filler = CAL.LixelStatisticsFiller(lixels, num_blocks, photons_per_block)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b in range(num_blocks):
+        filler.fill_in_block(CAL.calibration_photons(src, 3, 0, res, lixels))
+    filler.get_lixel_statistics()
+    torch.cuda.synchronize()
+    stats_s = time.perf_counter() - t0
+    filler.close()
+    n = num_blocks * photons_per_block
+    out = {"workload": f"plenoscope light-field calibration, {num_blocks} blocks x {photons_per_block} photons, "
+                       "5 interactions, 515,023 lixels",
+           "photons": n, "value": n / total, "unit": "calibration photons/s (draw + propagate + statistics)",
+           "statistics_ms_per_block": stats_s / num_blocks * 1e3,
+           "lixels_seen": int((st["efficiency"] > 0).sum()),
+           "mean_efficiency": float(st["efficiency"].astype(np.float64).mean())}
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(desc, ph, cpu_threads, target_seconds=10.0)
+        out["cpu_baseline"]["sample"] = out["cpu_baseline"]["sample"].replace(
+            "of the same bunch (seed 2024)", "of a calibration block (seed 3)")
+    return out
+
+
 def plenoscope_bunches_leg(dev, batches: int = 4, bunches_per_batch: int = 256) -> dict:
     """Batched bunches on the plenoscope (1,860 facets, 8,443 lenslets,
     515,023 lixels): per batch 256 bunches with sizes log-uniform in
@@ -249,7 +307,7 @@ def plenoscope_bunches_leg(dev, batches: int = 4, bunches_per_batch: int = 256) 
     hipGraph replayed per batch); the histograms must agree exactly."""
     from merlict_development_kit_amd import ArrivalBinning, Bunch, propagate_bunches
     from merlict_development_kit_amd.bunches import BunchPlan
-    scene = GpuScene(SceneDescription.plenoscope())
+    _, scene = plenoscope_scene()
     rng = np.random.default_rng(7)
     plan_batches = []
     for b in range(batches):
@@ -418,6 +476,7 @@ def main() -> int:
     ap.add_argument("--no-corsika", action="store_true")
     ap.add_argument("--no-plenoscope", action="store_true")
     ap.add_argument("--no-signal", action="store_true")
+    ap.add_argument("--no-calibration", action="store_true")
     ap.add_argument("--no-stream", action="store_true", help="skip the full-scale hybrid stream (about 1e9 photons)")
     ap.add_argument("--hybrid-train-photons", type=int, default=1_000_000_000)
     ap.add_argument("--hybrid-max-entropy", type=float, default=float("inf"))
@@ -527,6 +586,8 @@ def main() -> int:
         line["signal_chain"] = signal_leg(out, photons, dev)
     if not args.no_plenoscope:
         line["plenoscope_bunches"] = plenoscope_bunches_leg(dev)
+    if not args.no_calibration:
+        line["plenoscope_calibration"] = calibration_leg(dev, args.cpu_threads, cpu=not args.no_cpu_baseline)
     if not args.no_corsika:
         line["corsika_ingestion"] = corsika_leg(dev)
     if not args.no_cpu_baseline:

@@ -638,6 +638,65 @@ int mlgpu_inject_nsb(const mlgpu_pipelines* in, int32_t num_sensors, const float
 int mlgpu_nsb_light_rate(const double* flux_x, const double* flux_y, int32_t flux_n,
                          double max_fov_radius, double max_aperture_radius, double* rate);
 
+/* --------------------------------------------- light-field calibration
+ * One calibration photon's result, laid out as plenoscope::
+ * CalibrationPhotonResult (merlict_portal_plenoscope/calibration/
+ * CalibrationPhotonResult.h:9-19, 28 bytes): reached_sensor 0/1, the lixel,
+ * the aperture point and the incident direction's x, y, and the time of
+ * flight minus the travel to the aperture plane, all as float. */
+typedef struct mlgpu_calibration_photon {
+    uint32_t reached_sensor;
+    uint32_t lixel_id;
+    float x_pos_on_principal_aperture;
+    float y_pos_on_principal_aperture;
+    float x_tilt_vs_optical_axis;
+    float y_tilt_vs_optical_axis;
+    float time_of_flight;
+} mlgpu_calibration_photon;
+
+/* The results of n calibration photons (Calibrator.cpp:48-102, one_photon):
+ * photon photon_id_offset + i was drawn by `source` (seed); its aperture
+ * point is the posed support (rot (x, y, 0) + pos), its incident direction
+ * the reverse
+ * of its travel direction, and a photon whose result has no error flag and
+ * a sensor_id in [0, num_lixel) reached that lixel, with time of flight
+ * results.time_of_flight - start_distance / c.  out: n device records.
+ * Asynchronous. */
+int mlgpu_calibration_photons(const mlgpu_source* source, uint64_t seed, uint64_t photon_id_offset,
+                              const mlgpu_results* results, int64_t n, int32_t num_lixel,
+                              mlgpu_calibration_photon* out, void* stream);
+
+/* Per-lixel statistics as plenoscope::calibration::LixelStatistic
+ * (calibration/LixelStatistics.h:16-24, 48 bytes; the records of
+ * lixel_statistics.bin). */
+typedef struct mlgpu_lixel_statistic {
+    float efficiency, efficiency_std;
+    float cx_mean, cx_std;
+    float cy_mean, cy_std;
+    float x_mean, x_std;
+    float y_mean, y_std;
+    float time_delay_mean, time_delay_std;
+} mlgpu_lixel_statistic;
+
+/* LixelStatisticsFiller (calibration/LixelStatisticsFiller.cpp:8-62) on the
+ * GPU: per lixel a count and the online statistics (OnlineStatistics.cpp:
+ * 15-24: n, sum, Welford mean and m2 in double) of cx, cy, x, y and the time
+ * delay.  fill_block adds the reached photons of a block of n device
+ * records in record order (lixel by lixel, so each lixel's sums are those
+ * of the reference bit for bit); statistics writes num_lixel records
+ * (device): efficiency count / (num_blocks photons_per_block / num_lixel),
+ * its Poisson error, each mean (sum / n) and standard deviation
+ * (sqrt(m2 / n), NaN below 2 samples).  A record with reached_sensor and a
+ * lixel_id >= num_lixel is an MLGPU_ERR_ARGUMENT (nothing is added). */
+typedef struct mlgpu_lixel_filler mlgpu_lixel_filler;
+int mlgpu_lixel_filler_create(int64_t num_lixel, int64_t num_blocks, int64_t photons_per_block,
+                              mlgpu_lixel_filler** out);
+int mlgpu_lixel_filler_fill_block(mlgpu_lixel_filler* filler, const mlgpu_calibration_photon* block,
+                                  int64_t n, void* stream);
+int mlgpu_lixel_filler_statistics(const mlgpu_lixel_filler* filler, mlgpu_lixel_statistic* out,
+                                  void* stream);
+void mlgpu_lixel_filler_destroy(mlgpu_lixel_filler* filler);
+
 /* Number of photons with a non-zero error flag in the last n results
  * (synchronises `stream`). */
 int mlgpu_count_errors(const mlgpu_results* results, int64_t n, void* stream,

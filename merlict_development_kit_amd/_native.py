@@ -246,6 +246,12 @@ SIGNATURES = {
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_void_p]),
     "mlgpu_nsb_light_rate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_double,
                                        C.POINTER(C.c_double)]),
+    "mlgpu_calibration_photons": (C.c_int, [C.POINTER(Source), C.c_uint64, C.c_uint64, C.POINTER(Results), C.c_int64,
+                                            C.c_int32, C.c_void_p, C.c_void_p]),
+    "mlgpu_lixel_filler_create": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]),
+    "mlgpu_lixel_filler_fill_block": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "mlgpu_lixel_filler_statistics": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlgpu_lixel_filler_destroy": (None, [C.c_void_p]),
     "mlgpu_corsika_photons": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(PhotonBuffers),
                                         C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]),
     "mlgpu_histogram_arrivals": (C.c_int, [C.POINTER(Results), C.c_int64, C.POINTER(Histogram), C.c_void_p]),

@@ -583,6 +583,26 @@ int mlgpu_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed, ui
     return MLGPU_OK;
 }
 
+int mlgpu_calibration_photons(const mlgpu_source* src, uint64_t seed, uint64_t photon_id_offset,
+                              const mlgpu_results* results, int64_t n, int32_t num_lixel,
+                              mlgpu_calibration_photon* out, void* stream) {
+    if (!src || !results || !out || n < 0 || num_lixel < 0) {
+        set_last_error("null argument or negative size");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (n == 0) return MLGPU_OK;
+    if (!results->sensor_id || !results->error || !results->time_of_flight) {
+        set_last_error("null result array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const int rc = check_source(src);
+    if (rc) return rc;
+    const hipError_t e = mlgpu::launch_calibration_photons(*src, seed, photon_id_offset, *results, n, num_lixel, out,
+                                                           static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "calibration photons launch");
+    return MLGPU_OK;
+}
+
 int mlgpu_corsika_photons(const float* bunches, int64_t n, uint64_t seed, uint64_t id_offset,
                           const mlgpu_photon_buffers* out, int32_t* truth_id, int64_t capacity, int64_t* num_photons,
                           void* stream) {

@@ -91,4 +91,9 @@ hipError_t launch_corsika_emit(const float* bunches, int64_t n, uint64_t seed, u
                                int64_t capacity, const mlgpu_photon_buffers& out, int32_t* truth,
                                hipStream_t stream);
 
+// Calibration photon records of n results of `src` photons (calibration.hip).
+hipError_t launch_calibration_photons(const mlgpu_source& src, uint64_t seed, uint64_t id_offset,
+                                      const mlgpu_results& r, int64_t n, int32_t num_lixel,
+                                      mlgpu_calibration_photon* out, hipStream_t stream);
+
 }  // namespace mlgpu

@@ -91,9 +91,10 @@ MLGPU_HD inline void sincos_turn(double u, double& sn, double& cs) {
     else { sn = -c; cs = s; }
 }
 
-// Photon id of the source (support s, direction d, wavelength)
-MLGPU_HD inline void source_photon(const mlgpu_source& src, const uint64_t seed, const uint64_t id, D3& s, D3& d,
-                                   double& lambda) {
+// Photon id of the source before the pose: the support (x, y) on the xy
+// plane, the direction d about +z, the wavelength
+MLGPU_HD inline void source_local(const mlgpu_source& src, const uint64_t seed, const uint64_t id, double& x,
+                                  double& y, D3& d, double& lambda) {
     uint32_t w[4];
     // direction about +z
     source_block(seed, id, 0u, w);
@@ -160,7 +161,8 @@ MLGPU_HD inline void source_photon(const mlgpu_source& src, const uint64_t seed,
     }
     // support: Generator::get_point_on_xy_disc_within_radius, with a
     // deterministic cap of kDiscTries (all failing: p < 1e-42) at the centre
-    double x = 0.0, y = 0.0;
+    x = 0.0;
+    y = 0.0;
     if (src.support == MLGPU_SUPPORT_DISC) {
         const double r = src.radius, r2 = r * r;
         for (int k = 0; k < kDiscTries; k++) {
@@ -174,6 +176,14 @@ MLGPU_HD inline void source_photon(const mlgpu_source& src, const uint64_t seed,
             }
         }
     }
+}
+
+// Photon id of the source (support s, direction d, wavelength): the local
+// photon started start_distance back along d, then posed
+MLGPU_HD inline void source_photon(const mlgpu_source& src, const uint64_t seed, const uint64_t id, D3& s, D3& d,
+                                   double& lambda) {
+    double x, y;
+    source_local(src, seed, id, x, y, d, lambda);
     const double g = src.start_distance;
     const D3 p{x - d.x * g, y - d.y * g, 0.0 - d.z * g};
     const double* R = src.rot;

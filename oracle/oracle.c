@@ -895,10 +895,11 @@ void oracle_sincos_turn(double u, double* s, double* c) {
     }
 }
 
-void oracle_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed,
-                             uint64_t photon_id_offset, double* const out[7]) {
-    for (int64_t i = 0; i < n; i++) {
-        const uint64_t id = photon_id_offset + (uint64_t)i;
+/* photon id of the source before the pose: support (px, py), direction
+   about +z, wavelength */
+static void src_local(const mlgpu_source* src, uint64_t seed, uint64_t id, double* px_out, double* py_out,
+                      double d[3], double* lambda_out) {
+    {
         uint32_t w[4];
         double dx = 0.0, dy = 0.0, dz = 1.0, lambda, px = 0.0, py = 0.0;
         src_block(seed, id, 0u, w);
@@ -954,6 +955,18 @@ void oracle_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed,
                 if (x * x + y * y <= r * r) { px = x; py = y; break; }
             }
         }
+        *px_out = px; *py_out = py;
+        d[0] = dx; d[1] = dy; d[2] = dz;
+        *lambda_out = lambda;
+    }
+}
+
+void oracle_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed,
+                             uint64_t photon_id_offset, double* const out[7]) {
+    for (int64_t i = 0; i < n; i++) {
+        double px, py, d[3], lambda;
+        src_local(src, seed, photon_id_offset + (uint64_t)i, &px, &py, d, &lambda);
+        const double dx = d[0], dy = d[1], dz = d[2];
         const double g = src->start_distance;
         const double sx = px - dx * g, sy = py - dy * g, sz = 0.0 - dz * g;
         const double* R = src->rot;
@@ -1565,5 +1578,78 @@ int64_t oracle_inject_nsb(const int64_t* offsets, const double* arrival_time, co
     rc = k;
 done:
     free(cdf); free(keys); free(bt); free(bl);
+    return rc;
+}
+
+/* ---------------------------------------------- light-field calibration */
+
+/* one_photon's result (Calibrator.cpp:48-102) from the photon's draw and its
+   propagation result */
+void oracle_calibration_photons(const mlgpu_source* src, uint64_t seed, uint64_t photon_id_offset,
+                                const int32_t* sensor_id, const uint8_t* error, const double* time_of_flight,
+                                int64_t n, int32_t num_lixel, mlgpu_calibration_photon* out) {
+    for (int64_t i = 0; i < n; i++) {
+        mlgpu_calibration_photon c;
+        memset(&c, 0, sizeof(c));
+        if (sensor_id[i] >= 0 && sensor_id[i] < num_lixel && error[i] == 0) {
+            double px, py, d[3], lambda;
+            src_local(src, seed, photon_id_offset + (uint64_t)i, &px, &py, d, &lambda);
+            const double* R = src->rot;
+            c.reached_sensor = 1u;
+            c.lixel_id = (uint32_t)sensor_id[i];
+            c.x_pos_on_principal_aperture = (float)(px * R[0] + py * R[1] + 0.0 * R[2] + src->pos[0]);
+            c.y_pos_on_principal_aperture = (float)(px * R[3] + py * R[4] + 0.0 * R[5] + src->pos[1]);
+            c.x_tilt_vs_optical_axis = (float)(-(d[0] * R[0] + d[1] * R[1] + d[2] * R[2]));
+            c.y_tilt_vs_optical_axis = (float)(-(d[0] * R[3] + d[1] * R[4] + d[2] * R[5]));
+            c.time_of_flight = (float)(time_of_flight[i] - src->start_distance / 299792458.0);
+        }
+        out[i] = c;
+    }
+}
+
+/* LixelStatisticsFiller over n records in order (LixelStatisticsFiller.cpp:
+   8-62, OnlineStatistics.cpp:15-45); -1 for a reached lixel id out of range */
+int oracle_lixel_statistics(const mlgpu_calibration_photon* rec, int64_t n, int64_t num_lixel, int64_t num_blocks,
+                            int64_t photons_per_block, mlgpu_lixel_statistic* out) {
+    const double emitted = (double)num_blocks * (double)photons_per_block / (double)num_lixel;
+    unsigned* count = (unsigned*)calloc((size_t)num_lixel, sizeof(unsigned));
+    double* acc = (double*)calloc((size_t)num_lixel * 15, sizeof(double)); /* (sum, mean, m2) x 5 */
+    int rc = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (!rec[i].reached_sensor) continue;
+        if (rec[i].lixel_id >= (uint64_t)num_lixel) { rc = -1; break; }
+        const uint32_t l = rec[i].lixel_id;
+        count[l]++;
+        const double ns = (double)count[l];
+        const double v[5] = {rec[i].x_tilt_vs_optical_axis, rec[i].y_tilt_vs_optical_axis,
+                             rec[i].x_pos_on_principal_aperture, rec[i].y_pos_on_principal_aperture,
+                             rec[i].time_of_flight};
+        for (int q = 0; q < 5; q++) {
+            double* a = acc + (size_t)l * 15 + 3 * q;
+            const double delta = v[q] - a[1];
+            a[1] += delta / ns;
+            a[2] += delta * (v[q] - a[1]);
+            a[0] += v[q];
+        }
+    }
+    for (int64_t l = 0; rc == 0 && l < num_lixel; l++) {
+        const double ns = (double)count[l];
+        float m[5], sd[5];
+        for (int q = 0; q < 5; q++) {
+            const double* a = acc + (size_t)l * 15 + 3 * q;
+            m[q] = (float)(a[0] / ns);
+            sd[q] = (float)sqrt(ns < 2.0 ? NAN : a[2] / ns);
+        }
+        mlgpu_lixel_statistic r;
+        r.efficiency = (float)((double)count[l] / emitted);
+        r.efficiency_std = (float)((double)sqrtf((float)count[l]) / emitted);
+        r.cx_mean = m[0]; r.cx_std = sd[0];
+        r.cy_mean = m[1]; r.cy_std = sd[1];
+        r.x_mean = m[2]; r.x_std = sd[2];
+        r.y_mean = m[3]; r.y_std = sd[3];
+        r.time_delay_mean = m[4]; r.time_delay_std = sd[4];
+        out[l] = r;
+    }
+    free(count); free(acc);
     return rc;
 }

@@ -140,6 +140,15 @@ int64_t oracle_inject_nsb(const int64_t* offsets, const double* arrival_time, co
                           const double* flux_y, int32_t flux_n, double rate, double exposure_time, uint64_t seed,
                           int64_t* out_offsets, double* out_time, double* out_wavelength, int32_t* out_truth,
                           int64_t capacity, double* start);
+/* one_photon's calibration result (Calibrator.cpp:48-102) for n photons of
+   `src` and their propagation results. */
+void oracle_calibration_photons(const mlgpu_source* src, uint64_t seed, uint64_t photon_id_offset,
+                                const int32_t* sensor_id, const uint8_t* error, const double* time_of_flight,
+                                int64_t n, int32_t num_lixel, mlgpu_calibration_photon* out);
+/* LixelStatisticsFiller over n records in order; -1 for a lixel id out of
+   range. */
+int oracle_lixel_statistics(const mlgpu_calibration_photon* rec, int64_t n, int64_t num_lixel, int64_t num_blocks,
+                            int64_t photons_per_block, mlgpu_lixel_statistic* out);
 /* night_sky_background::Light::rate (Light.cpp:9-28); -1 for a bad table */
 double oracle_nsb_light_rate(const double* flux_x, const double* flux_y, int32_t flux_n, double max_fov_radius,
                              double max_aperture_radius);

@@ -549,6 +549,56 @@ def ref_nsb(offsets, arrival_time, wavelength, truth_id, efficiencies, flux_x, f
             "max_aperture_radius": float(head[3]), "offsets": oo, "time": t, "wavelength": lam, "truth": tr}
 
 
+# ---------------------------------------------------- light-field calibration
+CALIBRATION_PHOTON = np.dtype([("reached_sensor", "<u4"), ("lixel_id", "<u4"),
+                               ("x_pos_on_principal_aperture", "<f4"), ("y_pos_on_principal_aperture", "<f4"),
+                               ("x_tilt_vs_optical_axis", "<f4"), ("y_tilt_vs_optical_axis", "<f4"),
+                               ("time_of_flight", "<f4")])
+LIXEL_STATISTIC = np.dtype([(k, "<f4") for k in (
+    "efficiency", "efficiency_std", "cx_mean", "cx_std", "cy_mean", "cy_std", "x_mean", "x_std", "y_mean", "y_std",
+    "time_delay_mean", "time_delay_std")])
+
+
+def calibration_photons(src: Source, seed: int, photon_id_offset: int, results: dict, num_lixel: int) -> np.ndarray:
+    """one_photon's records for oracle results (oracle_calibration_photons)."""
+    s = np.ascontiguousarray(results["sensor_id"], np.int32)
+    e = np.ascontiguousarray(results["error"], np.uint8)
+    t = np.ascontiguousarray(results["time_of_flight"], np.float64)
+    out = np.zeros(len(s), CALIBRATION_PHOTON)
+    L = lib()
+    L.oracle_calibration_photons.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, _P_I32, _P_U8, _P_F64, C.c_int64,
+                                             C.c_int32, C.c_void_p]
+    L.oracle_calibration_photons(C.byref(src), seed, photon_id_offset, _ptr(s, _P_I32), _ptr(e, _P_U8),
+                                 _ptr(t, _P_F64), len(s), num_lixel, out.ctypes.data)
+    return out
+
+
+def lixel_statistics(records: np.ndarray, num_lixel: int, num_blocks: int, photons_per_block: int) -> np.ndarray:
+    """LixelStatisticsFiller over the records in order (oracle_lixel_statistics)."""
+    r = np.ascontiguousarray(records, CALIBRATION_PHOTON)
+    out = np.zeros(num_lixel, LIXEL_STATISTIC)
+    L = lib()
+    L.oracle_lixel_statistics.restype = C.c_int
+    L.oracle_lixel_statistics.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_void_p]
+    rc = L.oracle_lixel_statistics(r.ctypes.data, len(r), num_lixel, num_blocks, photons_per_block, out.ctypes.data)
+    if rc != 0:
+        raise IndexError("a reached lixel id out of range")
+    return out
+
+
+def ref_lixelstats(records: np.ndarray, num_lixel: int, num_blocks: int, photons_per_block: int) -> bytes:
+    """The reference's LixelStatisticsFiller over the records in blocks of
+    photons_per_block, as calibration::write writes it (harness
+    `lixelstats`)."""
+    r = np.ascontiguousarray(records, CALIBRATION_PHOTON)
+    head = np.array([num_lixel, num_blocks, photons_per_block, len(r)], np.int64).tobytes()
+    with tempfile.TemporaryDirectory() as d:
+        i, o = Path(d) / "in.bin", Path(d) / "out.bin"
+        i.write_bytes(head + r.tobytes())
+        _run_ref(["lixelstats", i, o])
+        return o.read_bytes()
+
+
 def ref_signal(sensor_id, time_of_flight, wavelength, truth_id, num_sensors, qe_x, qe_y, dark_rate, crosstalk,
                exposure_time, seed, duration, std_dev) -> dict:
     """The reference's signal chain per sensor (harness `signal`)."""

@@ -15,6 +15,7 @@
 //   lightfield frames built by light_field_sensor::Factory
 //   bench      timing of the reference CPU path (chunked threads, or the
 //              reference's own merlict_multi_thread driver)
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -36,6 +37,7 @@
 #include "merlict_signal_processing/signal_processing.h"
 #include "merlict_portal_plenoscope/night_sky_background/Injector.h"
 #include "merlict_portal_plenoscope/night_sky_background/Light.h"
+#include "merlict_portal_plenoscope/calibration/LixelStatisticsFiller.h"
 namespace sp = signal_processing;
 
 namespace ml = merlict;
@@ -651,6 +653,29 @@ static int cmd_nsb(int argc, char** argv) {
     return 0;
 }
 
+// lixelstats <in> <out.bin>: LixelStatisticsFiller over blocks of
+// CalibrationPhotonResult records, written by calibration::write
+static int cmd_lixelstats(int argc, char** argv) {
+    if (argc != 4) return 2;
+    std::ifstream in(argv[2], std::ios::binary);
+    int64_t num_lixel = 0, num_blocks = 0, per_block = 0, n = 0;
+    in.read(reinterpret_cast<char*>(&num_lixel), 8);
+    in.read(reinterpret_cast<char*>(&num_blocks), 8);
+    in.read(reinterpret_cast<char*>(&per_block), 8);
+    in.read(reinterpret_cast<char*>(&n), 8);
+    static_assert(sizeof(plenoscope::CalibrationPhotonResult) == 28, "record size");
+    std::vector<plenoscope::CalibrationPhotonResult> rec(static_cast<size_t>(n));
+    in.read(reinterpret_cast<char*>(rec.data()), n * 28);
+    plenoscope::calibration::LixelStatisticsFiller filler(num_lixel, num_blocks, per_block);
+    for (int64_t b = 0; b * per_block < n; b++) {
+        const int64_t lo = b * per_block, hi = std::min(n, lo + per_block);
+        const std::vector<plenoscope::CalibrationPhotonResult> block(rec.begin() + lo, rec.begin() + hi);
+        filler.fill_in_block(block);
+    }
+    plenoscope::calibration::write(filler.get_lixel_statistics(), argv[3]);
+    return 0;
+}
+
 static int cmd_lensmaker(int argc, char** argv) {
     if (argc < 5) return 2;
     ml::lens_maker::Config c;
@@ -767,6 +792,7 @@ int main(int argc, char** argv) {
         else if (c == "corsika") rc = cmd_corsika(argc, argv);
         else if (c == "signal") rc = cmd_signal(argc, argv);
         else if (c == "nsb") rc = cmd_nsb(argc, argv);
+        else if (c == "lixelstats") rc = cmd_lixelstats(argc, argv);
         if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", c.c_str());
         return rc;
     } catch (const std::exception& e) {

@@ -5,7 +5,7 @@ oracle/Makefile, with the Philox stream plugged into merlict's
 random::Generator) on seeded inputs and stores inputs and outputs as small
 .npz files.  Run in the container that has the reference mounted:
 
-    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika | signal | nsb]
+    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika | signal | nsb | calibration]
 """
 from __future__ import annotations
 
@@ -190,9 +190,40 @@ def nsb_fixtures():
           "start", ref["start"], "| empty ->", int(z["offsets"][-1]), "start", z["start"])
 
 
+def calibration_records(num_lixel=60, n=1200, seed=29):
+    """Synthetic CalibrationPhotonResult records: about 80 % reached, lixel
+    ids skewed so that some lixels get none or one record, float values of
+    the calibration's scales."""
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, O.CALIBRATION_PHOTON)
+    r["reached_sensor"] = rng.uniform(size=n) < 0.8
+    r["lixel_id"] = np.minimum((rng.pareto(1.5, n) * 6).astype(np.int64), num_lixel - 1)
+    r["lixel_id"][~r["reached_sensor"].astype(bool)] = rng.integers(0, 1 << 31, (~r["reached_sensor"].astype(bool)).sum())
+    r["x_pos_on_principal_aperture"] = rng.uniform(-37, 37, n)
+    r["y_pos_on_principal_aperture"] = rng.uniform(-37, 37, n)
+    r["x_tilt_vs_optical_axis"] = rng.normal(0, 0.03, n)
+    r["y_tilt_vs_optical_axis"] = rng.normal(0, 0.03, n)
+    r["time_of_flight"] = rng.normal(5.5e-7, 3e-9, n)
+    return r
+
+
+def calibration_fixtures():
+    """The reference's LixelStatisticsFiller (3 blocks of 400 records, 60
+    lixels) and its calibration::write bytes."""
+    rec = calibration_records()
+    raw = O.ref_lixelstats(rec, 60, 3, 400)
+    np.savez_compressed(HERE / "lixel_statistics.npz", records=rec.view(np.uint8),
+                        config=np.array([60, 3, 400], np.int64), written=np.frombuffer(raw, np.uint8))
+    st = np.frombuffer(raw, O.LIXEL_STATISTIC)
+    print("lixel_statistics", len(rec), "records", len(st), "lixels", int(np.isnan(st["cx_std"]).sum()), "NaN std")
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/merlict_golden is not built")
+    if len(sys.argv) > 1 and sys.argv[1] == "calibration":
+        calibration_fixtures()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "nsb":
         nsb_fixtures()
         return

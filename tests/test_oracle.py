@@ -636,3 +636,99 @@ def test_oracle_nsb_laws():
     keep = ~nsb
     np.testing.assert_array_equal(np.sort(otr[keep]), tr)
     np.testing.assert_array_equal(ot[keep][np.argsort(otr[keep])], t - start)
+
+
+# ----------------------------------------------------- light-field calibration
+def _calibration_golden(golden):
+    g = golden("lixel_statistics")
+    rec = g["records"].view(O.CALIBRATION_PHOTON)
+    return rec, [int(v) for v in g["config"]], g["written"].tobytes()
+
+
+def test_oracle_lixel_statistics_matches_reference(golden):
+    """LixelStatisticsFiller + calibration::write of the reference on 1200
+    records in 3 blocks over 60 lixels (some with no or one record): the
+    oracle's records are the same bytes, NaNs included."""
+    rec, (nl, nb, pb), written = _calibration_golden(golden)
+    st = O.lixel_statistics(rec, nl, nb, pb)
+    assert st.tobytes() == written
+
+
+def test_oracle_lixel_statistics_laws(golden):
+    """Per lixel: efficiency = count / (blocks x photons per block / lixels),
+    its error sqrt(count) / the same, the means and population deviations of
+    the float values (numpy in double), NaN deviation below two records,
+    NaN means without records."""
+    rec, (nl, nb, pb), _ = _calibration_golden(golden)
+    st = O.lixel_statistics(rec, nl, nb, pb)
+    emitted = nb * pb / nl
+    reached = rec[rec["reached_sensor"] == 1]
+    for lx in range(nl):
+        r = reached[reached["lixel_id"] == lx]
+        assert st["efficiency"][lx] == np.float32(len(r) / emitted)
+        assert st["efficiency_std"][lx] == pytest.approx(math.sqrt(len(r)) / emitted, rel=1e-6)
+        for f, k in (("x_tilt_vs_optical_axis", "cx"), ("y_tilt_vs_optical_axis", "cy"),
+                     ("x_pos_on_principal_aperture", "x"), ("y_pos_on_principal_aperture", "y"),
+                     ("time_of_flight", "time_delay")):
+            v = r[f].astype(np.float64)
+            if len(v) == 0:
+                assert np.isnan(st[k + "_mean"][lx])
+            else:
+                assert st[k + "_mean"][lx] == pytest.approx(v.mean(), rel=1e-6, abs=1e-30)
+            if len(v) < 2:
+                assert np.isnan(st[k + "_std"][lx])
+            else:
+                assert st[k + "_std"][lx] == pytest.approx(v.std(), rel=1e-5, abs=1e-12 * abs(v.mean()))
+    with pytest.raises(IndexError):
+        bad = rec.copy()
+        bad["reached_sensor"][0], bad["lixel_id"][0] = 1, nl
+        O.lixel_statistics(bad, nl, nb, pb)
+
+
+def test_lixel_statistics_file_format(tmp_path, golden):
+    """write / read / read_efficiencies (LixelStatistics.cpp:18-80): the
+    reference's file read back, written again byte for byte; a trailing
+    partial record is ignored as the reference's reader does."""
+    from merlict_development_kit_amd import calibration as CAL
+    _, _, written = _calibration_golden(golden)
+    p = tmp_path / "lixel_statistics.bin"
+    p.write_bytes(written + b"\x01\x02")
+    st = CAL.read(p)
+    assert len(st) == 60 and st.dtype == CAL.LIXEL_STATISTIC and CAL.LIXEL_STATISTIC.itemsize == 48
+    q = tmp_path / "again.bin"
+    CAL.write(st, q)
+    assert q.read_bytes() == written
+    np.testing.assert_array_equal(CAL.read_efficiencies(p), st["efficiency"])
+    assert CAL.CALIBRATION_PHOTON == O.CALIBRATION_PHOTON and CAL.CALIBRATION_PHOTON.itemsize == 28
+
+
+def test_oracle_calibration_photons():
+    """one_photon's records (Calibrator.cpp:48-102) from the oracle's
+    calibration photons: the incident direction is the reverse of the travel
+    direction, the aperture point the start moved 1e4 m along it, the time of
+    flight less 1e4 m / c; only photons on a lixel without error count."""
+    from merlict_development_kit_amd import calibration as CAL
+    src = CAL.calibration_source(35.5, math.radians(3.25))
+    n = 5000
+    ph = O.generate_photons(src.to_c(O.Source), n, seed=3, photon_id_offset=100)
+    rng = np.random.default_rng(1)
+    res = {"sensor_id": rng.integers(-2, 50, n).astype(np.int32),
+           "error": (rng.uniform(size=n) < 0.05).astype(np.uint8),
+           "time_of_flight": 1e4 / 299792458.0 + rng.uniform(0, 1e-7, n)}
+    rec = O.calibration_photons(src.to_c(O.Source), 3, 100, res, 40)
+    ok = (res["sensor_id"] >= 0) & (res["sensor_id"] < 40) & (res["error"] == 0)
+    np.testing.assert_array_equal(rec["reached_sensor"], ok.astype(np.uint32))
+    np.testing.assert_array_equal(rec["lixel_id"][ok], res["sensor_id"][ok])
+    assert (rec["lixel_id"][~ok] == 0).all() and (rec["time_of_flight"][~ok] == 0).all()
+    np.testing.assert_array_equal(rec["x_tilt_vs_optical_axis"][ok], (-ph["direction_x"][ok]).astype(np.float32))
+    np.testing.assert_array_equal(rec["y_tilt_vs_optical_axis"][ok], (-ph["direction_y"][ok]).astype(np.float32))
+    ax = ph["support_x"] + ph["direction_x"] * 1e4
+    ay = ph["support_y"] + ph["direction_y"] * 1e4
+    np.testing.assert_allclose(rec["x_pos_on_principal_aperture"][ok], ax[ok], rtol=0, atol=1e-5)
+    np.testing.assert_allclose(rec["y_pos_on_principal_aperture"][ok], ay[ok], rtol=0, atol=1e-5)
+    assert np.hypot(ax, ay).max() <= 1.05 * 35.5 + 1e-9
+    assert np.abs(ph["support_z"] + ph["direction_z"] * 1e4).max() < 1e-9
+    tilt = np.arccos(np.clip(-ph["direction_z"], -1, 1))
+    assert tilt.max() <= 1.2 * math.radians(3.25) + 1e-12
+    np.testing.assert_array_equal(rec["time_of_flight"][ok],
+                                  (res["time_of_flight"][ok] - 1e4 / 299792458.0).astype(np.float32))
