@@ -297,6 +297,42 @@ def calibration_leg(dev, cpu_threads: int, num_blocks: int = 8, photons_per_bloc
     return out
 
 
+def render_leg(dev, reps: int = 5, cpu: bool = True) -> dict:
+    """Images of the FACT-like telescope with light guides as merlict_visual
+    renders them: a 1920 x 1080 pin-hole camera above the dish (the
+    reference Config's snapshot size), the Tracer with global illumination
+    (a shadow ray per surface pixel), every frame coloured.  GPU: whole
+    calls timed; CPU: the reference's own Tracer (oracle/_ref) on one thread
+    over a 320 x 180 view of the same camera."""
+    import math as _m
+    from merlict_development_kit_amd import visual as V
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    desc = SceneDescription.fact(light_guides=True)
+    scene = GpuScene(desc)
+    colors = np.random.default_rng(1).uniform(0, 255, (desc.num_frames, 6)).astype(np.float32)
+
+    def camera(cols, rows):
+        return V.PinHoleCamera((0.0, -6.0, 10.0), (_m.radians(149), 0.0, _m.radians(10)), _m.radians(40), cols, rows)
+
+    cfg = V.RenderConfig()
+    V.render(scene, camera(1920, 1080), cfg, colors, seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        V.render(scene, camera(1920, 1080), cfg, colors, seed=r)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    out = {"workload": "FACT-like telescope with light guides, 1920 x 1080 pin-hole view, global illumination",
+           "ms_per_image": sec * 1e3, "value": 1920 * 1080 / sec, "unit": "pixels/s"}
+    if cpu and O.ref_available():
+        _, s_cpu = O.ref_render(desc, (0.0, -6.0, 10.0), (_m.radians(149), 0.0, _m.radians(10)), _m.radians(40),
+                                320, 180, True, cfg.incoming_direction, cfg.sky_color, colors, 0, timing=True)
+        out["cpu_baseline"] = {"value": 320 * 180 / s_cpu, "unit": "pixels/s", "cores": 1, "kind": "reference",
+                               "sample": f"320 x 180 view of the same camera, one thread, {s_cpu:.2f} s"}
+    return out
+
+
 def plenoscope_bunches_leg(dev, batches: int = 4, bunches_per_batch: int = 256) -> dict:
     """Batched bunches on the plenoscope (1,860 facets, 8,443 lenslets,
     515,023 lixels): per batch 256 bunches with sizes log-uniform in
@@ -477,6 +513,7 @@ def main() -> int:
     ap.add_argument("--no-plenoscope", action="store_true")
     ap.add_argument("--no-signal", action="store_true")
     ap.add_argument("--no-calibration", action="store_true")
+    ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-stream", action="store_true", help="skip the full-scale hybrid stream (about 1e9 photons)")
     ap.add_argument("--hybrid-train-photons", type=int, default=1_000_000_000)
     ap.add_argument("--hybrid-max-entropy", type=float, default=float("inf"))
@@ -586,6 +623,8 @@ def main() -> int:
         line["signal_chain"] = signal_leg(out, photons, dev)
     if not args.no_plenoscope:
         line["plenoscope_bunches"] = plenoscope_bunches_leg(dev)
+    if not args.no_render:
+        line["render"] = render_leg(dev, cpu=not args.no_cpu_baseline)
     if not args.no_calibration:
         line["plenoscope_calibration"] = calibration_leg(dev, args.cpu_threads, cpu=not args.no_cpu_baseline)
     if not args.no_corsika:

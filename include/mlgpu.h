@@ -697,6 +697,46 @@ int mlgpu_lixel_filler_statistics(const mlgpu_lixel_filler* filler, mlgpu_lixel_
                                   void* stream);
 void mlgpu_lixel_filler_destroy(mlgpu_lixel_filler* filler);
 
+/* ------------------------------------------------------------- rendering
+ * Images of a scene as merlict_visual renders them: a pin-hole camera
+ * (PinHoleCamera.cpp:10-77: the camera frame from position and Tait-Bryan
+ * rotation, the principal point (num_cols / 2) / tan(fov / 2) along the
+ * optical axis, pixel (row, col) looking at principal point + x_cam
+ * (row - num_rows / 2) + y_cam (col - num_cols / 2)) and the Tracer
+ * (Tracer.cpp:21-145) per pixel: at the first hit a reflection when the
+ * facing reflectivity at 533 nm is >= a uniform, else through a
+ * transparent boundary by Fresnel (reflection when R > a uniform, else
+ * refraction into the restriction frame as the propagator does), else the
+ * facing colour of the surface, shaded against the global light when
+ * global_illumination is on; no hit: the sky colour.  Pixel p draws its
+ * uniforms from Philox stream 10 of (seed, p).  The reference recurses
+ * without a bound; here a ray that has been reflected or refracted
+ * max_interaction_depth times ends black. */
+typedef struct mlgpu_pinhole_camera {
+    double position[3];
+    double rotation[3];      /* Tait-Bryan x, y, z angles (Rot3) */
+    double field_of_view;    /* rad, in (0, pi] (CameraDevice.cpp:81-92) */
+    int32_t num_cols;
+    int32_t num_rows;
+} mlgpu_pinhole_camera;
+typedef struct mlgpu_render_config {
+    int32_t global_illumination;    /* 0 / 1 */
+    int32_t max_interaction_depth;  /* >= 1 (Config.cpp:7: 42) */
+    double incoming_direction[3];   /* of the global light (Config.cpp:21) */
+    float sky_color[3];             /* SkyDome background (Config.cpp:6) */
+    float pad;
+} mlgpu_render_config;
+/* image (device): num_rows x num_cols x 3 floats, row-major (the Colors of
+ * merlict's Image, 0..255); frame_colors (host): n_frames x 6 floats, the
+ * outer then the inner colour of every frame, NULL for merlict's default
+ * COLOR_DARK_GRAY (SurfaceEntity.cpp:42-43); n_frames must be the
+ * description's.  *num_errors (host) receives the pixels whose ray met a
+ * surface function that does not cover 533 nm (the reference throws; their
+ * colour is NaN).  Synchronises `stream`. */
+int mlgpu_render_pinhole(const mlgpu_scene* scene, int32_t world_frame, const mlgpu_pinhole_camera* camera,
+                         const mlgpu_render_config* config, const float* frame_colors, int32_t n_frames,
+                         uint64_t seed, float* image, int64_t* num_errors, void* stream);
+
 /* Number of photons with a non-zero error flag in the last n results
  * (synchronises `stream`). */
 int mlgpu_count_errors(const mlgpu_results* results, int64_t n, void* stream,

@@ -164,6 +164,16 @@ class Nsb(C.Structure):
                 ("exposure_time", C.c_double), ("seed", C.c_uint64)]
 
 
+class PinholeCamera(C.Structure):
+    _fields_ = [("position", C.c_double * 3), ("rotation", C.c_double * 3), ("field_of_view", C.c_double),
+                ("num_cols", C.c_int32), ("num_rows", C.c_int32)]
+
+
+class RenderConfig(C.Structure):
+    _fields_ = [("global_illumination", C.c_int32), ("max_interaction_depth", C.c_int32),
+                ("incoming_direction", C.c_double * 3), ("sky_color", C.c_float * 3), ("pad", C.c_float)]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("max_num_interactions_per_photon", C.c_uint32),
@@ -246,6 +256,9 @@ SIGNATURES = {
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_void_p]),
     "mlgpu_nsb_light_rate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_double,
                                        C.POINTER(C.c_double)]),
+    "mlgpu_render_pinhole": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(PinholeCamera), C.POINTER(RenderConfig),
+                                       C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.POINTER(C.c_int64),
+                                       C.c_void_p]),
     "mlgpu_calibration_photons": (C.c_int, [C.POINTER(Source), C.c_uint64, C.c_uint64, C.POINTER(Results), C.c_int64,
                                             C.c_int32, C.c_void_p, C.c_void_p]),
     "mlgpu_lixel_filler_create": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]),

@@ -583,6 +583,86 @@ int mlgpu_generate_photons(const mlgpu_source* src, int64_t n, uint64_t seed, ui
     return MLGPU_OK;
 }
 
+int mlgpu_render_pinhole(const mlgpu_scene* s, int32_t world_frame, const mlgpu_pinhole_camera* cam,
+                         const mlgpu_render_config* cfg, const float* frame_colors, int32_t n_frames, uint64_t seed,
+                         float* image, int64_t* num_errors, void* stream) {
+    if (!s || !cam || !cfg || !image || !num_errors) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    *num_errors = 0;
+    if (world_frame < 0 || world_frame >= static_cast<int32_t>(s->scope_of_frame.size()) ||
+        s->scope_of_frame[world_frame] < 0) {
+        set_last_error("world_frame must be frame 0 or a restriction frame");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (n_frames != static_cast<int32_t>(s->scope_of_frame.size())) {
+        set_last_error("n_frames must be the description's number of frames");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    // CameraDevice::assert_field_of_view_is_valid (CameraDevice.cpp:81-92)
+    if (!(cam->field_of_view > 0.0) || !(cam->field_of_view <= M_PI)) {
+        set_last_error("field_of_view must be in (0, pi]");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (cam->num_cols < 1 || cam->num_rows < 1 ||
+        static_cast<int64_t>(cam->num_cols) * cam->num_rows > (int64_t{1} << 31)) {
+        set_last_error("need 1 <= num_cols, num_rows and at most 2^31 pixels");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (cfg->max_interaction_depth < 1) { set_last_error("max_interaction_depth must be >= 1"); return MLGPU_ERR_ARGUMENT; }
+    // CameraDevice / PinHoleCamera::set_pos_rot_fov (CameraDevice.cpp:19-33,
+    // PinHoleCamera.cpp:10-50)
+    const double rot[4] = {cam->rotation[0], cam->rotation[1], cam->rotation[2], 0.0};
+    double T[12];
+    mlgpu::pose_matrix(cam->position, MLGPU_ROT_XYZ, rot, T);
+    auto orientation = [&](double x, double y, double z, double* o) {
+        o[0] = x * T[0] + y * T[1] + z * T[2];
+        o[1] = x * T[4] + y * T[5] + z * T[6];
+        o[2] = x * T[8] + y * T[9] + z * T[10];
+    };
+    mlgpu::RenderCamera rc{};
+    double axis[3];
+    orientation(0.0, 0.0, 1.0, axis);
+    const double an = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    for (int k = 0; k < 3; k++) axis[k] = axis[k] / an;  // Ray normalises the optical axis
+    orientation(0.0, 1.0, 0.0, rc.scol);
+    orientation(1.0, 0.0, 0.0, rc.srow);
+    const double dist = (static_cast<double>(static_cast<uint32_t>(cam->num_cols)) / 2.0) /
+                        std::tan(cam->field_of_view / 2.0);
+    for (int k = 0; k < 3; k++) {
+        rc.pos[k] = cam->position[k];
+        rc.pp[k] = axis[k] * dist;
+    }
+    rc.cols = cam->num_cols;
+    rc.rows = cam->num_rows;
+    mlgpu::RenderParams prm{};
+    prm.gi = cfg->global_illumination ? 1 : 0;
+    prm.max_depth = cfg->max_interaction_depth;
+    for (int k = 0; k < 3; k++) {
+        prm.light[k] = cfg->incoming_direction[k];
+        prm.sky[k] = cfg->sky_color[k];
+    }
+    // frame colours: merlict's default COLOR_DARK_GRAY (SurfaceEntity.cpp:42-43)
+    std::vector<float> colors(static_cast<size_t>(n_frames) * 6, 25.0f);
+    if (frame_colors) std::copy(frame_colors, frame_colors + colors.size(), colors.begin());
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    char* mem = nullptr;
+    const size_t cbytes = (colors.size() * sizeof(float) + 255) & ~static_cast<size_t>(255);
+    hipError_t e = hipMalloc(&mem, cbytes + 256);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc render");
+    float* dcolors = reinterpret_cast<float*>(mem);
+    auto* derr = reinterpret_cast<unsigned long long*>(mem + cbytes);
+    unsigned long long herr = 0;
+    e = hipMemcpyAsync(dcolors, colors.data(), colors.size() * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(derr, 0, sizeof(unsigned long long), st);
+    if (e == hipSuccess)
+        e = mlgpu::launch_render(s->dev, s->scope_of_frame[world_frame], rc, prm, dcolors, seed, image, derr, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&herr, derr, sizeof(herr), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(mem);
+    if (e != hipSuccess) return hip_fail(e, "render");
+    *num_errors = static_cast<int64_t>(herr);
+    return MLGPU_OK;
+}
+
 int mlgpu_calibration_photons(const mlgpu_source* src, uint64_t seed, uint64_t photon_id_offset,
                               const mlgpu_results* results, int64_t n, int32_t num_lixel,
                               mlgpu_calibration_photon* out, void* stream) {

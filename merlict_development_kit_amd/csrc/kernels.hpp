@@ -91,6 +91,24 @@ hipError_t launch_corsika_emit(const float* bunches, int64_t n, uint64_t seed, u
                                int64_t capacity, const mlgpu_photon_buffers& out, int32_t* truth,
                                hipStream_t stream);
 
+// Pin-hole camera in the world frame (PinHoleCamera::set_pos_rot_fov):
+// position, principal point, the sensor's row and column steps.
+struct RenderCamera {
+    double pos[3], pp[3], srow[3], scol[3];
+    int32_t cols, rows;
+};
+struct RenderParams {
+    int32_t gi;         // global illumination on
+    int32_t max_depth;  // reflections + refractions before a ray ends black
+    double light[3];    // incoming direction of the global light
+    float sky[3];
+    float pad;
+};
+// One pixel per lane (render.hip): image rows x cols x 3 floats.
+hipError_t launch_render(const DeviceScene& sc, int32_t world_scope, const RenderCamera& cam,
+                         const RenderParams& prm, const float* colors, uint64_t seed, float* image,
+                         unsigned long long* errors, hipStream_t stream);
+
 // Calibration photon records of n results of `src` photons (calibration.hip).
 hipError_t launch_calibration_photons(const mlgpu_source& src, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& r, int64_t n, int32_t num_lixel,

@@ -798,6 +798,12 @@ void build_host_scene(const mlgpu_scene_desc& d, HostScene* hs) {
     b.run();
 }
 
+void pose_matrix(const double* pos, int mode, const double* rot, double* out) {
+    const M34 m = pose(pos, mode, rot);
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 4; c++) out[4 * r + c] = m.T[r][c];
+}
+
 void frame_poses(const mlgpu_scene_desc& d, double* out) {
     validate_desc(d);
     std::vector<double> radius(d.n_frames);

@@ -34,5 +34,8 @@ void build_host_scene(const mlgpu_scene_desc& d, HostScene* hs);
 // frame2world of every frame (12 doubles each, row-major 3 x 4), as merlict
 // computes them after clustering (throws std::invalid_argument on bad input).
 void frame_poses(const mlgpu_scene_desc& d, double* out);
+// HomTra3::set_transformation(Rot3, Vec3) of one pose (merlict/HomTra3.cpp:
+// 43-112): out holds the row-major 3 x 4 transform.
+void pose_matrix(const double* pos, int mode, const double* rot, double* out);
 
 }  // namespace mlgpu

@@ -27,6 +27,7 @@
 #include "kernels.hpp"
 #include "propagate_core.hpp"
 #include "signal.hpp"
+#include "stream_rng.hpp"
 
 namespace mlgpu {
 namespace {
@@ -41,35 +42,6 @@ constexpr int32_t kCrosstalk = -202;   // simulation_truth.h:17
 constexpr int32_t kAccidental = -201;  // simulation_truth.h:16
 constexpr int32_t kNumTimeSlices = 100;  // pulse_extraction.h:13
 constexpr int32_t kMaxChain = 1 << 20;
-
-// sequential 24-bit uniforms of one Philox stream (as the reference's
-// generator is driven through random::Generator::uniform)
-struct StreamRng {
-    uint32_t k0, k1, id_lo, id_hi, stream, next;
-    uint32_t b[4];
-    __device__ void init(uint64_t seed, uint64_t id, uint32_t s) {
-        k0 = static_cast<uint32_t>(seed);
-        k1 = static_cast<uint32_t>(seed >> 32);
-        id_lo = static_cast<uint32_t>(id);
-        id_hi = static_cast<uint32_t>(id >> 32);
-        stream = s;
-        next = 0;
-    }
-    __device__ double uniform() {
-        if ((next & 3u) == 0u) philox4x32_10(next >> 2, stream, id_lo, id_hi, k0, k1, b[0], b[1], b[2], b[3]);
-        const uint32_t w = b[next & 3u];
-        next++;
-        return static_cast<double>(w >> 8) * (1.0 / 16777216.0);
-    }
-    // Generator::expovariate (merlict/random/Generator.cpp:36-39)
-    __device__ double expovariate(double rate) { return -log(uniform()) / rate; }
-    // normal(mean, std) as the reference's Philox generator draws it:
-    // Box-Muller on 1 - u1 and u2
-    __device__ double normal(double mean, double std_dev) {
-        const double u1 = 1.0 - uniform(), u2 = uniform();
-        return mean + std_dev * sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
-    }
-};
 
 // Func1::evaluate (merlict/function/Func1.cpp:66-71) on a table sorted by
 // x with the slopes computed before sorting; false outside [x0, xN)

@@ -1653,3 +1653,122 @@ int oracle_lixel_statistics(const mlgpu_calibration_photon* rec, int64_t n, int6
     free(count); free(acc);
     return rc;
 }
+
+/* -------------------------------------------------------------- rendering */
+
+static v3 mirror_v(v3 n, v3 v) {
+    /* Vec3::mirror (Vec3.cpp:59-113) */
+    const double x = n.x, y = n.y, z = n.z;
+    return V((1. - 2. * x * x) * v.x + -2. * x * y * v.y + -2. * x * z * v.z,
+             -2. * x * y * v.x + (1. - 2. * y * y) * v.y + -2. * y * z * v.z,
+             -2. * x * z * v.x + -2. * y * z * v.y + (1. - 2. * z * z) * v.z);
+}
+
+/* PinHoleCamera (PinHoleCamera.cpp:10-77) and Tracer (Tracer.cpp:21-145)
+   per pixel, pixel p drawing from Philox stream 10 of (seed, p); the ray
+   ends black after max_interaction_depth reflections and refractions.
+   image: rows x cols x 3; returns the number of pixels whose ray met a
+   function that does not cover 533 nm (NaN colour), -1 on bad arguments. */
+int64_t oracle_render_pinhole(const oracle_scene* s, int32_t world, const mlgpu_pinhole_camera* cam,
+                              const mlgpu_render_config* cfg, const float* colors, uint64_t seed, float* image) {
+    if (world < 0 || world >= s->n || cam->num_cols < 1 || cam->num_rows < 1) return -1;
+    const double lam = 533e-9;
+    const double rot[4] = {cam->rotation[0], cam->rotation[1], cam->rotation[2], 0.0};
+    const homtra T = ht_from_pos_rot(V(cam->position[0], cam->position[1], cam->position[2]), MLGPU_ROT_XYZ, rot);
+    const v3 axis = normalized(ht_orient(&T, V(0.0, 0.0, 1.0)));
+    const v3 scol = ht_orient(&T, V(0.0, 1.0, 0.0));
+    const v3 srow = ht_orient(&T, V(1.0, 0.0, 0.0));
+    const double dist = ((double)(unsigned)cam->num_cols / 2.0) / tan(cam->field_of_view / 2.0);
+    const v3 pp = mul(axis, dist);
+    const v3 pos = V(cam->position[0], cam->position[1], cam->position[2]);
+    const v3 light = V(cfg->incoming_direction[0], cfg->incoming_direction[1], cfg->incoming_direction[2]);
+    int64_t errors = 0;
+    const int64_t npix = (int64_t)cam->num_rows * cam->num_cols;
+    for (int64_t p = 0; p < npix; p++) {
+        const int row = (int)(p / cam->num_cols), col = (int)(p % cam->num_cols);
+        v3 dir = normalized(add(add(pp, mul(srow, (double)(row - cam->num_rows / 2))),
+                                mul(scol, (double)(col - cam->num_cols / 2))));
+        v3 sup = pos;
+        int root = world;
+        sig_gen g;
+        sig_init(&g, seed, (uint64_t)p, 10u);
+        float rgb[3] = {0.0f, 0.0f, 0.0f};
+        int bad = 0, bounces = 0;
+        for (;;) {
+            const isect is = first_hit(s, root, sup, dir, NULL, NULL);
+            if (is.frame < 0) {
+                for (int k = 0; k < 3; k++) rgb[k] = cfg->sky_color[k];
+                break;
+            }
+            const oframe* f = &s->f[is.frame];
+            const v3 inc = ht_orient_inv(&f->t2w, dir);
+            const v3 hit_world = ht_pos(&f->t2w, is.position);
+            double refl;
+            if (func1_eval(&s->funcs[f->surf[is.outside_in ? MLGPU_OUTER_REFLECTION : MLGPU_INNER_REFLECTION]], lam,
+                           &refl)) { bad = 1; break; }
+            int reflect = 0;
+            v3 refr = V(0.0, 0.0, 0.0);
+            int new_root = root;
+            if (refl >= sig_uniform(&g)) {
+                reflect = 1;
+            } else if (f->surf[MLGPU_OUTER_REFRACTION] == s->def_refr && f->surf[MLGPU_INNER_REFRACTION] == s->def_refr) {
+                const float* fc = colors ? colors + 6 * is.frame + (is.outside_in ? 0 : 3) : NULL;
+                for (int k = 0; k < 3; k++) rgb[k] = fc ? fc[k] : 25.0f;
+                if (cfg->global_illumination) {
+                    const v3 spec = ht_orient(&f->t2w, mirror_v(is.normal, ht_orient_inv(&f->t2w, light)));
+                    double dk = dot(spec, dir);
+                    if (dk < 0.0) dk = 0.0;
+                    if (dk > 0.750) dk = 0.750;
+                    const isect sh = first_hit(s, root, hit_world, normalized(light), NULL, NULL);
+                    const v3 nr = ht_orient(&f->t2w, is.normal);
+                    double pr = dot(nr, light);
+                    if (dot(nr, dir) > 0.0) pr = pr * -1.0;
+                    if (sh.frame >= 0 || pr < 0.0) dk = dk * 0.25;
+                    const double m = 0.750 - dk;
+                    for (int k = 0; k < 3; k++) rgb[k] = (float)((1. - m) * rgb[k] + m * 0.0f);
+                }
+                break;
+            } else {
+                double n_from, n_to;
+                int ff = f->surf[is.outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION];
+                int ft = f->surf[is.outside_in ? MLGPU_INNER_REFRACTION : MLGPU_OUTER_REFRACTION];
+                if (func1_eval(&s->funcs[ff], lam, &n_from) || func1_eval(&s->funcs[ft], lam, &n_to)) { bad = 1; break; }
+                const v3 nrm = is.outside_in ? is.normal : mul(is.normal, -1.0);
+                const double cosI = dot(nrm, inc) * (-1.0);
+                const double eta = n_from / n_to;
+                const double sinT2 = eta * eta * (1.0 - cosI * cosI);
+                const double cosT = sqrt(1.0 - sinT2);
+                double R;
+                if (sinT2 > 1.0) {
+                    R = 1.0;
+                } else {
+                    const double ro = (n_from * cosI - n_to * cosT) / (n_from * cosI + n_to * cosT);
+                    const double rp = (n_to * cosI - n_from * cosT) / (n_to * cosI + n_from * cosT);
+                    R = (ro * ro + rp * rp) / 2.0;
+                }
+                if (R > sig_uniform(&g)) {
+                    reflect = 1;
+                } else {
+                    const int to_default = is.outside_in ? f->surf[MLGPU_INNER_REFRACTION] == s->def_refr
+                                                         : f->surf[MLGPU_OUTER_REFRACTION] == s->def_refr;
+                    new_root = (f->allowed >= 0 && !to_default) ? f->allowed : f->root;
+                    refr = add(mul(inc, eta), mul(nrm, eta * cosI - cosT));
+                }
+            }
+            if (bounces == cfg->max_interaction_depth) {
+                rgb[0] = rgb[1] = rgb[2] = 0.0f;
+                break;
+            }
+            bounces++;
+            dir = normalized(ht_orient(&f->t2w, reflect ? mirror_v(is.normal, inc) : refr));
+            sup = hit_world;
+            root = new_root;
+        }
+        if (bad) {
+            rgb[0] = rgb[1] = rgb[2] = NAN;
+            errors++;
+        }
+        for (int k = 0; k < 3; k++) image[3 * p + k] = rgb[k];
+    }
+    return errors;
+}

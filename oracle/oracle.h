@@ -149,6 +149,11 @@ void oracle_calibration_photons(const mlgpu_source* src, uint64_t seed, uint64_t
    range. */
 int oracle_lixel_statistics(const mlgpu_calibration_photon* rec, int64_t n, int64_t num_lixel, int64_t num_blocks,
                             int64_t photons_per_block, mlgpu_lixel_statistic* out);
+/* merlict_visual's PinHoleCamera + Tracer (Tracer.cpp:21-145) per pixel on
+   Philox stream 10 of (seed, pixel); image rows x cols x 3 floats; returns
+   the pixels with a 533 nm lookup outside a table, -1 on bad arguments. */
+int64_t oracle_render_pinhole(const oracle_scene* s, int32_t world, const mlgpu_pinhole_camera* cam,
+                              const mlgpu_render_config* cfg, const float* colors, uint64_t seed, float* image);
 /* night_sky_background::Light::rate (Light.cpp:9-28); -1 for a bad table */
 double oracle_nsb_light_rate(const double* flux_x, const double* flux_y, int32_t flux_n, double max_fov_radius,
                              double max_aperture_radius);

@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+import json
 import tempfile
 from pathlib import Path
 
@@ -129,6 +130,22 @@ class OracleScene:
     @property
     def num_frames(self) -> int:
         return lib().oracle_num_frames(self._h)
+
+    def render_pinhole(self, camera, config, frame_colors=None, seed: int = 0, world: int = 0):
+        """(image rows x cols x 3 float32, errors) of oracle_render_pinhole;
+        camera / config: ctypes structs laid out as mlgpu_pinhole_camera /
+        mlgpu_render_config."""
+        img = np.zeros((camera.num_rows, camera.num_cols, 3), np.float32)
+        cols = None if frame_colors is None else np.ascontiguousarray(frame_colors, np.float32)
+        L = lib()
+        L.oracle_render_pinhole.restype = C.c_int64
+        L.oracle_render_pinhole.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                            C.c_void_p]
+        k = L.oracle_render_pinhole(self._h, world, C.addressof(camera), C.addressof(config),
+                                    cols.ctypes.data if cols is not None else None, seed, img.ctypes.data)
+        if k < 0:
+            raise ValueError("bad render arguments")
+        return img, int(k)
 
     def tree_dump(self) -> dict:
         n = self.num_frames
@@ -597,6 +614,26 @@ def ref_lixelstats(records: np.ndarray, num_lixel: int, num_blocks: int, photons
         i.write_bytes(head + r.tobytes())
         _run_ref(["lixelstats", i, o])
         return o.read_bytes()
+
+
+def ref_render(desc, position, rotation, fov, cols, rows, global_illumination, light, sky, frame_colors,
+               seed, timing: bool = False):
+    """The reference's PinHoleCamera + Tracer per pixel (harness `render`,
+    one thread): image rows x cols x 3 float32 (and the seconds of the pixel
+    loop with timing=True)."""
+    cols_arr = np.ascontiguousarray(frame_colors, np.float32)
+    parts = [np.array(position, np.float64).tobytes(), np.array(rotation, np.float64).tobytes(),
+             np.array([fov], np.float64).tobytes(), np.array([cols, rows, int(global_illumination), 42],
+                                                              np.int32).tobytes(),
+             np.array(light, np.float64).tobytes(), np.array(sky, np.float32).tobytes(),
+             np.array([len(cols_arr)], np.int32).tobytes(), cols_arr.tobytes(), np.array([seed], np.uint64).tobytes()]
+    with tempfile.TemporaryDirectory() as d:
+        sp, i, o = Path(d) / "s.bin", Path(d) / "in.bin", Path(d) / "out.bin"
+        desc.save(str(sp))
+        i.write_bytes(b"".join(parts))
+        out = _run_ref(["render", sp, i, o])
+        img = np.frombuffer(o.read_bytes(), np.float32).reshape(rows, cols, 3).copy()
+    return (img, json.loads(out.strip().splitlines()[-1])["seconds"]) if timing else img
 
 
 def ref_signal(sensor_id, time_of_flight, wavelength, truth_id, num_sensors, qe_x, qe_y, dark_rate, crosstalk,

@@ -38,6 +38,8 @@
 #include "merlict_portal_plenoscope/night_sky_background/Injector.h"
 #include "merlict_portal_plenoscope/night_sky_background/Light.h"
 #include "merlict_portal_plenoscope/calibration/LixelStatisticsFiller.h"
+#include "merlict_visual/PinHoleCamera.h"
+#include "merlict_visual/Tracer.h"
 namespace sp = signal_processing;
 
 namespace ml = merlict;
@@ -676,6 +678,77 @@ static int cmd_lixelstats(int argc, char** argv) {
     return 0;
 }
 
+// The Tracer's generator: an Mt19937 whose uniforms come from Philox stream
+// `stream` of (seed, id).
+class PhiloxMt : public ml::random::Mt19937 {
+    PhiloxGenerator g_;
+
+ public:
+    PhiloxMt(uint64_t seed, uint64_t id, uint32_t stream) : ml::random::Mt19937(0), g_(seed, id, stream) {}
+    double uniform() override { return g_.uniform(); }
+};
+
+// render <scene> <in> <out>: the reference's PinHoleCamera rays and Tracer
+// colours per pixel, pixel p on Philox stream 10 of (seed, p)
+static int cmd_render(int argc, char** argv) {
+    if (argc != 5) return 2;
+    Desc d = load_desc(argv[2]);
+    RefScene rs;
+    build(d, &rs);
+    std::ifstream in(argv[3], std::ios::binary);
+    double pos[3], rot[3], fov, light[3];
+    int32_t cols = 0, rows = 0, gi = 0, depth = 0, nf = 0;
+    float sky[3];
+    uint64_t seed = 0;
+    in.read(reinterpret_cast<char*>(pos), 24);
+    in.read(reinterpret_cast<char*>(rot), 24);
+    in.read(reinterpret_cast<char*>(&fov), 8);
+    in.read(reinterpret_cast<char*>(&cols), 4);
+    in.read(reinterpret_cast<char*>(&rows), 4);
+    in.read(reinterpret_cast<char*>(&gi), 4);
+    in.read(reinterpret_cast<char*>(&depth), 4);
+    in.read(reinterpret_cast<char*>(light), 24);
+    in.read(reinterpret_cast<char*>(sky), 12);
+    in.read(reinterpret_cast<char*>(&nf), 4);
+    std::vector<float> col(static_cast<size_t>(nf) * 6);
+    in.read(reinterpret_cast<char*>(col.data()), nf * 24);
+    in.read(reinterpret_cast<char*>(&seed), 8);
+    if (nf != d.n) { std::fprintf(stderr, "need %d frame colours\n", d.n); return 1; }
+    std::vector<ml::Color> palette;
+    palette.reserve(static_cast<size_t>(nf) * 2);
+    for (int32_t i = 0; i < nf; i++) {
+        auto* s = dynamic_cast<ml::SurfaceEntity*>(rs.frames[i]);
+        palette.emplace_back(col[6 * i], col[6 * i + 1], col[6 * i + 2]);
+        palette.emplace_back(col[6 * i + 3], col[6 * i + 4], col[6 * i + 5]);
+        if (s) {
+            s->outer_color = &palette[2 * i];
+            s->inner_color = &palette[2 * i + 1];
+        }
+    }
+    ml::visual::Config cfg;
+    cfg.global_illumination.on = gi != 0;
+    cfg.global_illumination.incoming_direction = ml::Vec3(light[0], light[1], light[2]);
+    cfg.sky_dome = ml::visual::SkyDome(ml::Color(sky[0], sky[1], sky[2]));
+    ml::visual::PinHoleCamera cam("cam", static_cast<unsigned>(cols), static_cast<unsigned>(rows));
+    cam.set_pos_rot_fov(ml::Vec3(pos[0], pos[1], pos[2]), ml::Rot3(rot[0], rot[1], rot[2]), fov);
+    std::vector<float> img(static_cast<size_t>(cols) * rows * 3);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int64_t p = 0; p < static_cast<int64_t>(cols) * rows; p++) {
+        const int row = static_cast<int>(p / cols), c = static_cast<int>(p % cols);
+        ml::visual::CameraRay ray = cam.get_ray_for_pixel_in_row_and_col(row, c);
+        PhiloxMt gen(seed, static_cast<uint64_t>(p), 10u);
+        ml::visual::Tracer tracer(&ray, &rs.scenery.root, &cfg, &gen);
+        img[3 * p] = tracer.color.r;
+        img[3 * p + 1] = tracer.color.g;
+        img[3 * p + 2] = tracer.color.b;
+    }
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("{\"seconds\": %.6f}\n", sec);
+    std::ofstream o(argv[4], std::ios::binary);
+    o.write(reinterpret_cast<const char*>(img.data()), static_cast<std::streamsize>(img.size() * 4));
+    return 0;
+}
+
 static int cmd_lensmaker(int argc, char** argv) {
     if (argc < 5) return 2;
     ml::lens_maker::Config c;
@@ -793,6 +866,7 @@ int main(int argc, char** argv) {
         else if (c == "signal") rc = cmd_signal(argc, argv);
         else if (c == "nsb") rc = cmd_nsb(argc, argv);
         else if (c == "lixelstats") rc = cmd_lixelstats(argc, argv);
+        else if (c == "render") rc = cmd_render(argc, argv);
         if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", c.c_str());
         return rc;
     } catch (const std::exception& e) {

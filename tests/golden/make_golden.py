@@ -5,7 +5,7 @@ oracle/Makefile, with the Philox stream plugged into merlict's
 random::Generator) on seeded inputs and stores inputs and outputs as small
 .npz files.  Run in the container that has the reference mounted:
 
-    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika | signal | nsb | calibration]
+    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika | signal | nsb | calibration | render]
 """
 from __future__ import annotations
 
@@ -218,9 +218,26 @@ def calibration_fixtures():
     print("lixel_statistics", len(rec), "records", len(st), "lixels", int(np.isnan(st["cx_std"]).sum()), "NaN std")
 
 
+def render_fixtures():
+    """The reference's PinHoleCamera + Tracer (merlict_visual) per pixel."""
+    from merlict_development_kit_amd import visual as V
+    from render_cases import render_cases, render_colors
+    arrays = {}
+    for name, desc, cam, gi, seed in render_cases():
+        cfg = V.RenderConfig(global_illumination=gi)
+        col = render_colors(desc.num_frames)
+        arrays[name] = O.ref_render(desc, cam.position, cam.rotation, cam.field_of_view, cam.num_cols,
+                                    cam.num_rows, gi, cfg.incoming_direction, cfg.sky_color, col, seed)
+        print("render", name, arrays[name].shape, len(np.unique(arrays[name].reshape(-1, 3), axis=0)), "colours")
+    np.savez_compressed(HERE / "render.npz", **arrays)
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/merlict_golden is not built")
+    if len(sys.argv) > 1 and sys.argv[1] == "render":
+        render_fixtures()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "calibration":
         calibration_fixtures()
         return

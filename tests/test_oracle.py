@@ -732,3 +732,66 @@ def test_oracle_calibration_photons():
     assert tilt.max() <= 1.2 * math.radians(3.25) + 1e-12
     np.testing.assert_array_equal(rec["time_of_flight"][ok],
                                   (res["time_of_flight"][ok] - 1e4 / 299792458.0).astype(np.float32))
+
+
+# ------------------------------------------------------------------- rendering
+def test_oracle_render_matches_reference(golden):
+    """PinHoleCamera + Tracer of the reference (merlict_visual, driven per
+    pixel with Philox stream 10) on the zoo, the FACT-like telescope and the
+    lens bench: the oracle's colours are the same floats."""
+    from render_cases import render_cases, render_colors
+    from merlict_development_kit_amd import visual as V
+    g = golden("render")
+    for name, desc, cam, gi, seed in render_cases():
+        cfg = V.RenderConfig(global_illumination=gi)
+        img, err = O.OracleScene(desc).render_pinhole(cam.to_c(), cfg.to_c(), render_colors(desc.num_frames), seed)
+        assert err == 0
+        np.testing.assert_array_equal(img, g[name], err_msg=name)
+        assert len(np.unique(g[name].reshape(-1, 3), axis=0)) > 10
+
+
+def test_oracle_render_laws():
+    """A disc of radius 0.5 seen from 4 m straight above and below: pixels
+    whose ray meets z = 0 well inside the disc (pin-hole projection,
+    computed here) take the facing colour (outer above, inner below), those
+    well outside the sky's; with reflectivity 0.5 about half of the disc
+    pixels show the sky; with global illumination the light from above
+    leaves the facing colour unchanged (darkening clamped at 0.75)."""
+    from render_cases import one_disc
+    from merlict_development_kit_amd import visual as V
+    cols, rows, fov = 96, 80, math.radians(30)
+    colors = np.array([[0, 0, 0, 0, 0, 0], [200, 100, 50, 10, 220, 30]], np.float32)
+    sky = np.array(V.COLOR_SKY_BLUE, np.float32)
+    f_pix = (cols / 2.0) / math.tan(fov / 2.0)
+    r = np.arange(rows)[:, None] - rows // 2
+    c = np.arange(cols)[None, :] - cols // 2
+    rad = 4.0 * np.hypot(r, c) / f_pix  # where the pixel's ray meets z = 0
+    inside, outside = rad < 0.5 * 0.98, rad > 0.5 * 1.02
+    for above in (True, False):
+        cam = V.PinHoleCamera((0.0, 0.0, 4.0 if above else -4.0), (math.pi if above else 0.0, 0.0, 0.0), fov, cols,
+                              rows)
+        img, err = O.OracleScene(one_disc(0.0)).render_pinhole(cam.to_c(), V.RenderConfig(False).to_c(), colors, 3)
+        assert err == 0
+        want = colors[1, :3] if above else colors[1, 3:]
+        assert (img[inside] == want).all() and (img[outside] == sky).all()
+        assert 0.9 < inside.sum() / (math.pi * (0.5 * f_pix / 4.0) ** 2) < 1.0
+    cam = V.PinHoleCamera((0.0, 0.0, 4.0), (math.pi, 0.0, 0.0), fov, cols, rows)
+    img, _ = O.OracleScene(one_disc(0.5)).render_pinhole(cam.to_c(), V.RenderConfig(False).to_c(), colors, 4)
+    k = inside.sum()
+    shown_sky = (img[inside] == sky).all(axis=1).sum()
+    assert abs(shown_sky - 0.5 * k) < 5 * math.sqrt(0.25 * k)
+    assert ((img[inside] == sky).all(axis=1) | (img[inside] == colors[1, :3]).all(axis=1)).all()
+    img, _ = O.OracleScene(one_disc(0.0)).render_pinhole(cam.to_c(), V.RenderConfig(True).to_c(), colors, 4)
+    assert (img[rows // 2, cols // 2] == colors[1, :3]).all()
+
+
+def test_ppm_format(tmp_path):
+    """write_ppm / read_ppm (PortablePixMap.cpp): the reference's header,
+    8-bit truncation of the float colours, rows then columns."""
+    from merlict_development_kit_amd import visual as V
+    img = np.array([[[0.0, 127.9, 255.0], [1.5, 2.5, 254.99]]], np.float32)
+    p = tmp_path / "i.ppm"
+    V.write_ppm(img, p)
+    raw = p.read_bytes()
+    assert raw == b"P6\n# CREATOR: MERLICT\n2 1\n255\n" + bytes([0, 127, 255, 1, 2, 254])
+    np.testing.assert_array_equal(V.read_ppm(p), np.trunc(img))
