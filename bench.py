@@ -299,11 +299,12 @@ def calibration_leg(dev, cpu_threads: int, num_blocks: int = 8, photons_per_bloc
 
 def render_leg(dev, reps: int = 5, cpu: bool = True) -> dict:
     """Images of the FACT-like telescope with light guides as merlict_visual
-    renders them: a 1920 x 1080 pin-hole camera above the dish (the
-    reference Config's snapshot size), the Tracer with global illumination
-    (a shadow ray per surface pixel), every frame coloured.  GPU: whole
-    calls timed; CPU: the reference's own Tracer (oracle/_ref) on one thread
-    over a 320 x 180 view of the same camera."""
+    renders them: a 1920 x 1080 pin-hole camera 8 m above the dish looking
+    down (the reference Config's snapshot size; the facets reflect most
+    pixel rays on towards the camera housing and the sky), the Tracer with
+    global illumination (a shadow ray per surface pixel), every frame
+    coloured.  GPU: whole calls timed; CPU: the reference's own Tracer
+    (oracle/_ref) on one thread over a 1280 x 720 view of the same camera."""
     import math as _m
     from merlict_development_kit_amd import visual as V
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -313,7 +314,7 @@ def render_leg(dev, reps: int = 5, cpu: bool = True) -> dict:
     colors = np.random.default_rng(1).uniform(0, 255, (desc.num_frames, 6)).astype(np.float32)
 
     def camera(cols, rows):
-        return V.PinHoleCamera((0.0, -6.0, 10.0), (_m.radians(149), 0.0, _m.radians(10)), _m.radians(40), cols, rows)
+        return V.PinHoleCamera((0.0, 0.0, 8.0), (_m.pi, 0.0, 0.0), _m.radians(40), cols, rows)
 
     cfg = V.RenderConfig()
     V.render(scene, camera(1920, 1080), cfg, colors, seed=0)
@@ -323,13 +324,15 @@ def render_leg(dev, reps: int = 5, cpu: bool = True) -> dict:
         V.render(scene, camera(1920, 1080), cfg, colors, seed=r)
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / reps
-    out = {"workload": "FACT-like telescope with light guides, 1920 x 1080 pin-hole view, global illumination",
+    out = {"workload": "FACT-like telescope with light guides, 1920 x 1080 pin-hole view from 8 m above, "
+                       "global illumination",
            "ms_per_image": sec * 1e3, "value": 1920 * 1080 / sec, "unit": "pixels/s"}
     if cpu and O.ref_available():
-        _, s_cpu = O.ref_render(desc, (0.0, -6.0, 10.0), (_m.radians(149), 0.0, _m.radians(10)), _m.radians(40),
-                                320, 180, True, cfg.incoming_direction, cfg.sky_color, colors, 0, timing=True)
-        out["cpu_baseline"] = {"value": 320 * 180 / s_cpu, "unit": "pixels/s", "cores": 1, "kind": "reference",
-                               "sample": f"320 x 180 view of the same camera, one thread, {s_cpu:.2f} s"}
+        c = camera(1280, 720)
+        _, s_cpu = O.ref_render(desc, c.position, c.rotation, c.field_of_view, 1280, 720, True,
+                                cfg.incoming_direction, cfg.sky_color, colors, 0, timing=True)
+        out["cpu_baseline"] = {"value": 1280 * 720 / s_cpu, "unit": "pixels/s", "cores": 1, "kind": "reference",
+                               "sample": f"1280 x 720 view of the same camera, one thread, {s_cpu:.2f} s"}
     return out
 
 
