@@ -7,11 +7,13 @@
 //      collapse of the weight to a photon count (:8-24; floor(w), one more
 //      when a uniform from Philox stream 5 of (seed, id) is <= fmod(w, 1));
 //      a block-local exclusive scan of the counts;
-//   2. corsika_scan_kernel: one block scans the block totals;
+//   2. a device-wide exclusive scan (rocPRIM) of the block totals;
 //   3. corsika_emit_kernel: each bunch writes its photons at its offset:
 //      support, direction and wavelength computed in FP64 in the
 //      reference's operation order (:30-136, Ray.cpp:16-20, Vec3.cpp:20-26).
 #include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -91,36 +93,6 @@ __global__ __launch_bounds__(kScanBlock) void corsika_count_kernel(const float* 
     if (t == 0) block_total[blockIdx.x] = total;
 }
 
-// Exclusive scan of the block totals in place (one block, chunks of 256),
-// and the grand total in *total.
-__global__ __launch_bounds__(kScanBlock) void corsika_scan_kernel(uint64_t* __restrict__ block_total,
-                                                                  const int64_t blocks,
-                                                                  int64_t* __restrict__ total) {
-    __shared__ uint64_t s[kScanBlock];
-    __shared__ uint64_t carry;
-    const int t = static_cast<int>(threadIdx.x);
-    if (t == 0) carry = 0;
-    __syncthreads();
-    for (int64_t base = 0; base < blocks; base += kScanBlock) {
-        const int64_t i = base + t;
-        const uint64_t v = i < blocks ? block_total[i] : 0u;
-        s[t] = v;
-        __syncthreads();
-        for (int off = 1; off < kScanBlock; off <<= 1) {
-            const uint64_t u = t >= off ? s[t - off] : 0u;
-            __syncthreads();
-            s[t] += u;
-            __syncthreads();
-        }
-        const uint64_t c = carry;
-        if (i < blocks) block_total[i] = c + s[t] - v;
-        __syncthreads();
-        if (t == kScanBlock - 1) carry = c + s[t];
-        __syncthreads();
-    }
-    if (t == 0) *total = static_cast<int64_t>(carry);
-}
-
 __global__ __launch_bounds__(256) void corsika_emit_kernel(const float* __restrict__ bunches, const int64_t n,
                                                            const uint64_t seed, const uint64_t id_offset,
                                                            const uint32_t* __restrict__ local,
@@ -165,33 +137,72 @@ __global__ __launch_bounds__(256) void corsika_emit_kernel(const float* __restri
     }
 }
 
+// Workspace: per bunch its offset in its block (4 B), per block its total
+// (blocks + 1 entries, the last 0) and its exclusive offset, the scan's
+// temporary storage.
+struct CorsikaWs {
+    uint32_t* local;
+    uint64_t* block_total;
+    uint64_t* block_offset;
+    void* temp;
+    size_t temp_bytes;
+};
+
+size_t ws_align(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+size_t scan_temp_bytes(int64_t blocks) {
+    size_t t = 0;
+    (void)rocprim::exclusive_scan(nullptr, t, static_cast<uint64_t*>(nullptr), static_cast<uint64_t*>(nullptr),
+                                  uint64_t{0}, static_cast<size_t>(blocks + 1), rocprim::plus<uint64_t>(), 0);
+    return t;
+}
+
+CorsikaWs carve(void* ws, int64_t n) {
+    const int64_t blocks = (n + kScanBlock - 1) / kScanBlock;
+    char* q = static_cast<char*>(ws);
+    CorsikaWs w;
+    w.local = reinterpret_cast<uint32_t*>(q);
+    q += ws_align(static_cast<size_t>(n) * 4);
+    w.block_total = reinterpret_cast<uint64_t*>(q);
+    q += ws_align(static_cast<size_t>(blocks + 1) * 8);
+    w.block_offset = reinterpret_cast<uint64_t*>(q);
+    q += ws_align(static_cast<size_t>(blocks + 1) * 8);
+    w.temp = q;
+    w.temp_bytes = scan_temp_bytes(blocks);
+    return w;
+}
+
 }  // namespace
 
 size_t corsika_workspace_bytes(int64_t n) {
     const int64_t blocks = (n + kScanBlock - 1) / kScanBlock;
-    return static_cast<size_t>(n) * 4 + static_cast<size_t>(blocks) * 8 + 3 * 256;
+    return ws_align(static_cast<size_t>(n) * 4) + 2 * ws_align(static_cast<size_t>(blocks + 1) * 8) +
+           ws_align(scan_temp_bytes(blocks));
 }
 
 hipError_t launch_corsika_count(const float* bunches, int64_t n, uint64_t seed, uint64_t id_offset, void* ws,
                                 int32_t* bad, int64_t* total, hipStream_t stream) {
     const int64_t blocks = (n + kScanBlock - 1) / kScanBlock;
-    uint32_t* local = static_cast<uint32_t*>(ws);
-    uint64_t* block_total = reinterpret_cast<uint64_t*>(
-        static_cast<char*>(ws) + ((static_cast<size_t>(n) * 4 + 255) & ~static_cast<size_t>(255)));
+    CorsikaWs w = carve(ws, n);
     hipLaunchKernelGGL(corsika_count_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kScanBlock), 0, stream,
-                       bunches, n, seed, id_offset, local, block_total, bad);
-    hipLaunchKernelGGL(corsika_scan_kernel, dim3(1), dim3(kScanBlock), 0, stream, block_total, blocks, total);
-    return hipGetLastError();
+                       bunches, n, seed, id_offset, w.local, w.block_total, bad);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemsetAsync(w.block_total + blocks, 0, 8, stream);
+    // exclusive offsets of the blocks; entry `blocks` is the grand total
+    if (e == hipSuccess)
+        e = rocprim::exclusive_scan(w.temp, w.temp_bytes, w.block_total, w.block_offset, uint64_t{0},
+                                    static_cast<size_t>(blocks + 1), rocprim::plus<uint64_t>(), stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(total, w.block_offset + blocks, 8, hipMemcpyDeviceToDevice, stream);
+    return e;
 }
 
 hipError_t launch_corsika_emit(const float* bunches, int64_t n, uint64_t seed, uint64_t id_offset, const void* ws,
                                int64_t capacity, const mlgpu_photon_buffers& out, int32_t* truth,
                                hipStream_t stream) {
-    const uint32_t* local = static_cast<const uint32_t*>(ws);
-    const uint64_t* block_offset = reinterpret_cast<const uint64_t*>(
-        static_cast<const char*>(ws) + ((static_cast<size_t>(n) * 4 + 255) & ~static_cast<size_t>(255)));
+    const CorsikaWs w = carve(const_cast<void*>(ws), n);
     hipLaunchKernelGGL(corsika_emit_kernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, stream,
-                       bunches, n, seed, id_offset, local, block_offset, capacity, out, truth);
+                       bunches, n, seed, id_offset, w.local, w.block_offset, capacity, out, truth);
     return hipGetLastError();
 }
 

@@ -18,6 +18,8 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -63,28 +65,23 @@ __device__ __forceinline__ uint64_t time_key(double t) {
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
 
-__global__ void select_kernel(const mlgpu_results r, const int64_t n, const int32_t num_sensors,
-                              uint32_t* __restrict__ sensor_key, uint64_t* __restrict__ tkey,
-                              int32_t* __restrict__ index, int32_t* __restrict__ count) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    bool det = false;
-    if (i < n) {
-        const int32_t s = r.sensor_id[i];
-        det = s >= 0 && s < num_sensors && r.error[i] == 0;
+// photon i is detected: a sensor in range and no error flag
+struct Detected {
+    const int32_t* sensor;
+    const uint8_t* error;
+    int32_t num_sensors;
+    __device__ bool operator()(const int32_t i) const {
+        const int32_t s = sensor[i];
+        return s >= 0 && s < num_sensors && error[i] == 0;
     }
-    const uint64_t mask = __ballot(det);
-    if (mask == 0) return;
-    const int lane = static_cast<int>(threadIdx.x & 63u);
-    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(mask));
-    base = __shfl(base, leader);
-    if (det) {
-        const int k = base + __popcll(mask & ((1ull << lane) - 1ull));
-        sensor_key[k] = static_cast<uint32_t>(r.sensor_id[i]);
-        tkey[k] = time_key(r.time_of_flight[i]);
-        index[k] = static_cast<int32_t>(i);
-    }
+};
+
+// time keys of the detected photons (index order)
+__global__ void keys_kernel(const int32_t* __restrict__ index, const int64_t m, const mlgpu_results r,
+                            uint64_t* __restrict__ tkey) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    tkey[k] = time_key(r.time_of_flight[index[k]]);
 }
 
 // sensor of each photon in time order
@@ -497,7 +494,7 @@ hipError_t build_pipelines(const mlgpu_results& r, const double* wavelength, con
     *too_small = 0;
     const size_t cap = static_cast<size_t>(n);
     // keys and values, doubled for the sort's alternate buffers
-    size_t temp_bytes = 0, temp2 = 0;
+    size_t temp_bytes = 0, temp2 = 0, temp3 = 0;
     hipError_t e = rocprim::radix_sort_pairs(nullptr, temp_bytes, static_cast<uint64_t*>(nullptr),
                                              static_cast<uint64_t*>(nullptr), static_cast<int32_t*>(nullptr),
                                              static_cast<int32_t*>(nullptr), cap, 0, 64, stream);
@@ -505,7 +502,12 @@ hipError_t build_pipelines(const mlgpu_results& r, const double* wavelength, con
     e = rocprim::radix_sort_pairs(nullptr, temp2, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
                                   static_cast<int32_t*>(nullptr), static_cast<int32_t*>(nullptr), cap, 0, 32, stream);
     if (e != hipSuccess) return e;
+    const Detected detected{r.sensor_id, r.error, num_sensors};
+    e = rocprim::select(nullptr, temp3, rocprim::counting_iterator<int32_t>(0), static_cast<int32_t*>(nullptr),
+                        static_cast<int32_t*>(nullptr), cap, detected, stream);
+    if (e != hipSuccess) return e;
     temp_bytes = temp_bytes > temp2 ? temp_bytes : temp2;
+    temp_bytes = temp_bytes > temp3 ? temp_bytes : temp3;
     const size_t bytes = 2 * align_up(cap * 4) + 2 * align_up(cap * 8) + 2 * align_up(cap * 4) + align_up(temp_bytes) +
                          256;
     char* mem = nullptr;
@@ -522,17 +524,23 @@ hipError_t build_pipelines(const mlgpu_results& r, const double* wavelength, con
     void* temp = take(temp_bytes);
     auto* cnt = reinterpret_cast<int32_t*>(take(4));
     int32_t m32 = 0;
-    e = hipMemsetAsync(cnt, 0, 4, stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(select_kernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, stream, r, n,
-                           num_sensors, skey, tkey, idx, cnt);
-        e = hipGetLastError();
+    {
+        // the detected photons' indices in index order (an order-preserving
+        // select), so equal arrival times keep photon order through the
+        // stable sorts
+        size_t tb = temp_bytes;
+        e = rocprim::select(temp, tb, rocprim::counting_iterator<int32_t>(0), idx, cnt, cap, detected, stream);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&m32, cnt, 4, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     const size_t m = static_cast<size_t>(m32);
     *num_photons = m32;
     if (e == hipSuccess && static_cast<int64_t>(m) > out.capacity) *too_small = 1;
+    if (e == hipSuccess && !*too_small && m > 0) {
+        hipLaunchKernelGGL(keys_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream, idx,
+                           static_cast<int64_t>(m), r, tkey);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess && !*too_small && m > 0) {
         // by time, then stably by sensor: (sensor, time, photon index)
         size_t tb = temp_bytes;

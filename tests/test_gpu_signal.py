@@ -171,3 +171,22 @@ def test_gpu_nsb_errors_and_edges(cuda, golden):
                   *(torch.zeros(0, dtype=d, device=cuda) for d in (torch.float64, torch.float64, torch.int32)))
     out, start = inject_nsb(e, 0, torch.zeros(0, device=cuda), g["flux_x"], g["flux_y"], 1e9, 50e-9)
     assert start == 0.0 and len(out.arrival_time) == 0
+
+
+def test_gpu_pipelines_keep_photon_order_on_equal_times(cuda):
+    """100k photons on 5 sensors with only 8 distinct arrival times: within
+    a time the pipeline keeps photon order (the oracle's (sensor, time,
+    index) order) -- the compaction before the stable sorts preserves it."""
+    rng = np.random.default_rng(9)
+    n = 100_000
+    sensor = rng.integers(-1, 5, n).astype(np.int32)
+    err = (rng.uniform(size=n) < 0.01).astype(np.uint8)
+    t = rng.integers(0, 8, n).astype(np.float64) * 1e-9
+    lam = np.full(n, 433e-9)
+    T = lambda x, dt: torch.tensor(x, dtype=dt, device=cuda)
+    pipes = sensor_pipelines(T(sensor, torch.int32), T(err, torch.uint8), T(t, torch.float64),
+                             T(lam, torch.float64), 5)
+    off, ot, _, otr = O.sensor_pipelines(sensor, err, t, lam, None, 5)
+    np.testing.assert_array_equal(pipes.offsets.cpu().numpy(), off)
+    np.testing.assert_array_equal(pipes.arrival_time.cpu().numpy(), ot)
+    np.testing.assert_array_equal(pipes.truth_id.cpu().numpy(), otr)
