@@ -656,10 +656,28 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
     float* O = reinterpret_cast<float*>(H);
     const int t = static_cast<int>(threadIdx.x), lane = t & 63, w = t >> 6;
     const int lr = lane & 31, lk = (lane >> 5) * 8, lq = (lane >> 5) * 4;
-    const DrawConsts dk = draw_consts(a.s);
-    const FeatureConsts fc = feature_consts(a.s.dom);
+    // the per-call constants of the features and draws live in LDS, read
+    // where they are used, so that no register holds them across the
+    // matrix loops
+    __shared__ DrawConsts dk_lds;
+    __shared__ FeatureConsts fc_lds;
+    if (t == 0) {
+        dk_lds = draw_consts(a.s);
+        fc_lds = feature_consts(a.s.dom);
+    }
+    __syncthreads();
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * kTileP; base < n;
          base += static_cast<int64_t>(gridDim.x) * kTileP) {
+        // weight fragments of the first k-steps of the next layer, loaded
+        // ahead of the barrier that precedes it (here layer 0: one k-step)
+        bf16x8 pf0, pf1, pf2, pf3;
+        {
+            const bf16x8* P = reinterpret_cast<const bf16x8*>(a.wf[0]) + (2 * w) * 64 + lane;
+            pf0 = P[0];
+            pf1 = P[64];
+            pf2 = pf0;
+            pf3 = pf1;
+        }
         if (t < kTileP) {
             const int64_t i = base + t;
             float* f = F + t * kIn;
@@ -668,6 +686,7 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
 #pragma unroll
                     for (int j = 0; j < kIn; j++) f[j] = a.features[i * kIn + j];
                 } else {
+                    const FeatureConsts fc = fc_lds;
                     photon_features(fc, a.ph, i, f);
                 }
             } else {
@@ -694,65 +713,89 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
             const bf16x8* W1 = W0 + ksteps * 64;
             const uint16_t* src = l == 0 ? F16 : H;
             const int stride = l == 0 ? kFStride : kHStride;
-            bf16x8 wc0 = W0[0], wc1 = W1[0];
+            bf16x8 wc0 = pf0, wc1 = pf1;
             static_assert(kHidden / 16 % 2 == 0, "the k-loop takes k-steps in pairs");
-            // bias in accumulator layout: the C operand of the first k-step
-            f32x16 bt[2];
-#pragma unroll
-            for (int ot = 0; ot < 2; ot++) {
-#pragma unroll
-                for (int g = 0; g < 4; g++) {
-                    const float4 bias = *reinterpret_cast<const float4*>(a.b[l] + w * 64 + ot * 32 + g * 8 + lq);
-                    bt[ot][4 * g + 0] = bias.x;
-                    bt[ot][4 * g + 1] = bias.y;
-                    bt[ot][4 * g + 2] = bias.z;
-                    bt[ot][4 * g + 3] = bias.w;
-                }
-            }
+            // the accumulators start at zero; the bias is added when they are
+            // packed into the next layer's activations
+            const f32x16 zero = {};
             f32x16 acc[2][4];
-            // one k-step: the eight instructions of k-step ks on the weights (w0, w1)
-            auto kstep = [&](const int ks, const bf16x8& w0, const bf16x8& w1, const bool first) {
+            // activation fragments of half `h` (photon tiles 2 h, 2 h + 1) of
+            // k-step ks
+            auto load_h = [&](const int ks, const int h, bf16x8 (&hv)[2]) {
 #pragma unroll
-                for (int half = 0; half < 2; half++) {
-                    bf16x8 hv[2];
+                for (int q = 0; q < 2; q++)
+                    hv[q] = *reinterpret_cast<const bf16x8*>(src + ((2 * h + q) * 32 + lr) * stride + ks * 16 + lk);
+            };
+            // the four instructions of half h of a k-step on the weights (w0, w1)
+            auto mma = [&](const bf16x8& w0, const bf16x8& w1, const bf16x8 (&hv)[2], const int h, const bool first) {
 #pragma unroll
-                    for (int q = 0; q < 2; q++)
-                        hv[q] = *reinterpret_cast<const bf16x8*>(src + ((2 * half + q) * 32 + lr) * stride + ks * 16 + lk);
-#pragma unroll
-                    for (int q = 0; q < 2; q++) {
-                        const int pt = 2 * half + q;
-                        acc[0][pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, hv[q], first ? bt[0] : acc[0][pt], 0, 0, 0);
-                        acc[1][pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, hv[q], first ? bt[1] : acc[1][pt], 0, 0, 0);
-                    }
+                for (int q = 0; q < 2; q++) {
+                    const int pt = 2 * h + q;
+                    acc[0][pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, hv[q], first ? zero : acc[0][pt], 0, 0, 0);
+                    acc[1][pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, hv[q], first ? zero : acc[1][pt], 0, 0, 0);
                 }
             };
+            // The activation fragments of the next half k-step are read from
+            // LDS while the current half's instructions run (two sets of two
+            // fragments), the weights a k-step ahead (two sets); the
+            // scheduling barriers keep the compiler from sinking the loads
+            // to their use.
+            bf16x8 ha[2], hb[2];
+            load_h(0, 0, ha);
             if (ksteps == 1) {
-                kstep(0, wc0, wc1, true);
+                load_h(0, 1, hb);
+                __builtin_amdgcn_sched_barrier(0);
+                mma(wc0, wc1, ha, 0, true);
+                mma(wc0, wc1, hb, 1, true);
             } else {
-                {
-                    // k-steps 0 and 1
-                    const bf16x8 wb0 = W0[64], wb1 = W1[64];
-                    __builtin_amdgcn_sched_barrier(0);
-                    kstep(0, wc0, wc1, true);
-                    wc0 = W0[2 * 64];
-                    wc1 = W1[2 * 64];
-                    __builtin_amdgcn_sched_barrier(0);
-                    kstep(1, wb0, wb1, false);
-                }
-                // two register sets: the loads of one k-step are issued before
-                // the instructions of the previous one (the scheduling
-                // barriers keep the compiler from sinking them to their use)
+                bf16x8 wb0 = pf2, wb1 = pf3;
+                load_h(0, 1, hb);
+                __builtin_amdgcn_sched_barrier(0);
+                mma(wc0, wc1, ha, 0, true);
+                load_h(1, 0, ha);
+                __builtin_amdgcn_sched_barrier(0);
+                mma(wc0, wc1, hb, 1, true);
+                const int k2 = ksteps > 2 ? 2 : 1;
+                wc0 = W0[k2 * 64];
+                wc1 = W1[k2 * 64];
+                load_h(1, 1, hb);
+                __builtin_amdgcn_sched_barrier(0);
+                mma(wb0, wb1, ha, 0, false);
+                load_h(k2, 0, ha);
+                __builtin_amdgcn_sched_barrier(0);
+                mma(wb0, wb1, hb, 1, false);
 #pragma unroll 1
                 for (int ks = 2; ks < ksteps; ks += 2) {
-                    const bf16x8 wb0 = W0[(ks + 1) * 64], wb1 = W1[(ks + 1) * 64];
+                    wb0 = W0[(ks + 1) * 64];
+                    wb1 = W1[(ks + 1) * 64];
+                    load_h(ks, 1, hb);
                     __builtin_amdgcn_sched_barrier(0);
-                    kstep(ks, wc0, wc1, false);
+                    mma(wc0, wc1, ha, 0, false);
+                    load_h(ks + 1, 0, ha);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mma(wc0, wc1, hb, 1, false);
+                    // past the last k-step the loads repeat it (no branch)
                     const int kn = ks + 2 < ksteps ? ks + 2 : ks + 1;
                     wc0 = W0[kn * 64];
                     wc1 = W1[kn * 64];
+                    load_h(ks + 1, 1, hb);
                     __builtin_amdgcn_sched_barrier(0);
-                    kstep(ks + 1, wb0, wb1, false);
+                    mma(wb0, wb1, ha, 0, false);
+                    load_h(kn, 0, ha);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mma(wb0, wb1, hb, 1, false);
                 }
+            }
+            if (l + 1 < kLayers - 1) {
+                const bf16x8* P = reinterpret_cast<const bf16x8*>(a.wf[l + 1]) + (2 * w) * (kHidden / 16) * 64 + lane;
+                pf0 = P[0];
+                pf1 = P[(kHidden / 16) * 64];
+                pf2 = P[64];
+                pf3 = P[(kHidden / 16) * 64 + 64];
+            } else {
+                const bf16x8* P = reinterpret_cast<const bf16x8*>(a.wf[kLayers - 1]) + (w & 1) * (kHidden / 16) * 64 + lane;
+                pf0 = P[0];
+                pf2 = P[64];
             }
             __syncthreads();
 #pragma unroll
@@ -760,12 +803,13 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
 #pragma unroll
                 for (int g = 0; g < 4; g++) {
                     const int col = w * 64 + ot * 32 + g * 8 + lq;
+                    const float4 bias = *reinterpret_cast<const float4*>(a.b[l] + col);
 #pragma unroll
                     for (int pt = 0; pt < 4; pt++) {
                         const int row = pt * 32 + lr;
                         uint2 packed;
-                        packed.x = relu_bf16x2(pack_bf16(acc[ot][pt][4 * g + 0], acc[ot][pt][4 * g + 1]));
-                        packed.y = relu_bf16x2(pack_bf16(acc[ot][pt][4 * g + 2], acc[ot][pt][4 * g + 3]));
+                        packed.x = relu_bf16x2(pack_bf16(acc[ot][pt][4 * g + 0] + bias.x, acc[ot][pt][4 * g + 1] + bias.y));
+                        packed.y = relu_bf16x2(pack_bf16(acc[ot][pt][4 * g + 2] + bias.z, acc[ot][pt][4 * g + 3] + bias.w));
                         *reinterpret_cast<uint2*>(H + row * kHStride + col) = packed;
                     }
                 }
@@ -785,32 +829,39 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
                 bt[4 * g + 3] = bias.w;
             }
             f32x16 acc[2];
-            auto kstep = [&](const int ks, const bf16x8& wv, const bool first) {
-                bf16x8 hv[2];
+            auto load_o = [&](const int ks, bf16x8 (&hv)[2]) {
 #pragma unroll
                 for (int p = 0; p < 2; p++)
                     hv[p] = *reinterpret_cast<const bf16x8*>(H + ((pt0 + p) * 32 + lr) * kHStride + ks * 16 + lk);
+            };
+            auto mma = [&](const bf16x8& wv, const bf16x8 (&hv)[2], const bool first) {
 #pragma unroll
                 for (int p = 0; p < 2; p++)
                     acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, hv[p], first ? bt : acc[p], 0, 0, 0);
             };
-            bf16x8 wc = W[0];
-            {
-                const bf16x8 wb = W[64];
-                __builtin_amdgcn_sched_barrier(0);
-                kstep(0, wc, true);
-                wc = W[2 * 64];
-                __builtin_amdgcn_sched_barrier(0);
-                kstep(1, wb, false);
-            }
+            // weights and activations of the next k-step in flight while the
+            // current k-step's instructions run (two register sets each)
+            bf16x8 wc = pf0, wb = pf2;
+            bf16x8 ha[2], hb[2];
+            load_o(0, ha);
+            load_o(1, hb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wc, ha, true);
+            wc = W[2 * 64];
+            load_o(2, ha);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wb, hb, false);
 #pragma unroll 1
             for (int ks = 2; ks < kHidden / 16; ks += 2) {
-                const bf16x8 wb = W[(ks + 1) * 64];
+                wb = W[(ks + 1) * 64];
+                load_o(ks + 1, hb);
                 __builtin_amdgcn_sched_barrier(0);
-                kstep(ks, wc, false);
-                wc = W[(ks + 2 < kHidden / 16 ? ks + 2 : ks + 1) * 64];
+                mma(wc, ha, false);
+                const int kn = ks + 2 < kHidden / 16 ? ks + 2 : ks + 1;
+                wc = W[kn * 64];
+                load_o(kn, ha);
                 __builtin_amdgcn_sched_barrier(0);
-                kstep(ks + 1, wb, false);
+                mma(wb, hb, false);
             }
             __syncthreads();
 #pragma unroll
@@ -837,6 +888,7 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
         }
         if (a.sample && w < 2) {
             const int64_t i = base + t;
+            const DrawConsts dk = dk_lds;
             const bool fallback = i < n && sample_one(a.s, dk, F + t * kIn, O + t * kOStride, i);
             append_fallback(a.s, fallback, i);
         }
