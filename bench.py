@@ -420,14 +420,14 @@ def signal_leg(result, photons, dev, reps: int = 5) -> dict:
     extraction (the card's slices and time resolution), each timed on the
     GPU; the oracle (C, one thread) times the same chain on the host."""
     from merlict_development_kit_amd.signal_chain import (extract_pulses, inject_nsb, nsb_light_rate,
-                                                          photoelectric_convert, sensor_pipelines)
+                                                          photoelectric_convert, photon_stream, sensor_pipelines)
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
     c = signal_card()
     rate = nsb_light_rate(c["flux_x"], c["flux_y"], math.radians(2.2), 2.1)
     eff = torch.ones(1440, dtype=torch.float32, device=dev)
     conv = (c["qe_x"], c["qe_y"], c["dark_rate"], c["crosstalk"], c["exposure"])
-    times = {"pipelines": 0.0, "nsb": 0.0, "convert": 0.0, "extract": 0.0}
+    times = {"pipelines": 0.0, "nsb": 0.0, "convert": 0.0, "extract": 0.0, "photon_stream": 0.0}
     for r in range(reps + 1):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -440,11 +440,14 @@ def signal_leg(result, photons, dev, reps: int = 5) -> dict:
         pulses = photoelectric_convert(lit, 1440, *conv, seed=r)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
-        extract_pulses(pulses, 1440, c["slice"], c["spread"], seed=r)
+        sl = extract_pulses(pulses, 1440, c["slice"], c["spread"], seed=r)
         torch.cuda.synchronize()
         t4 = time.perf_counter()
+        sym, _ = photon_stream(pulses, sl, 1440)
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
         if r > 0:
-            for k, dt in zip(times, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
+            for k, dt in zip(times, (t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4)):
                 times[k] += dt / reps
     n = len(result.sensor_id)
     m = len(pipes.arrival_time)

@@ -602,6 +602,18 @@ int mlgpu_extract_pulses(const mlgpu_pulses* pulses, int32_t num_sensors,
                          double time_slice_duration, double arrival_time_std,
                          uint64_t seed, int16_t* slice, void* stream);
 
+/* The photon stream of the extracted pulses (merlict_signal_processing/
+ * PhotonStream.cpp:15-58): channel by channel the time slices of the kept
+ * pulses (slice >= 0) in pulse order, a NEXT_CHANNEL_MARKER (255) after
+ * every channel but the last, as PhotonStream::write stores them after its
+ * 16-byte header; truth (device, may be NULL) receives the kept pulses'
+ * truth ids (write_simulation_truth).  *num_symbols (host) = kept pulses +
+ * channels - 1 (0 without either); a capacity below it is an
+ * MLGPU_ERR_ARGUMENT with the count set.  Synchronises `stream`. */
+int mlgpu_photon_stream(const mlgpu_pulses* pulses, const int16_t* slice, int32_t num_sensors,
+                        uint8_t* symbols, int32_t* truth, int64_t capacity, int64_t* num_symbols,
+                        void* stream);
+
 /* Night-sky-background injection (merlict_portal_plenoscope/
  * night_sky_background/Injector.cpp:11-107) into the pipelines `in`: the
  * exposure [start, start + exposure_time) is centred on the mode of the

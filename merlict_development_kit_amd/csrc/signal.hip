@@ -20,6 +20,7 @@
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -483,6 +484,35 @@ __global__ void add_offsets_kernel(const int64_t* __restrict__ a, const int64_t*
     if (s <= n) out[s] = a[s] + b[s];
 }
 
+// ------------------------------------------------------ photon stream
+
+constexpr uint8_t kNextChannelMarker = 255;  // PhotonStream.h:15
+
+struct Kept {
+    const int16_t* slice;
+    __device__ int64_t operator()(const int64_t k) const { return slice[k] >= 0 ? 1 : 0; }
+};
+
+// kept pulse k of channel s goes to symbol kscan[k] + s (the kept pulses
+// of the channels before it and their markers come first)
+__global__ void stream_pulses_kernel(const mlgpu_pulses p, const int16_t* __restrict__ slice, const int64_t count,
+                                     const int32_t num_sensors, const int64_t* __restrict__ kscan,
+                                     uint8_t* __restrict__ symbols, int32_t* __restrict__ truth) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= count || slice[k] < 0) return;
+    const int32_t s = sensor_of(p.offsets, num_sensors, k);
+    symbols[kscan[k] + s] = static_cast<uint8_t>(slice[k]);
+    if (truth) truth[kscan[k]] = p.truth_id[k];
+}
+
+// the marker after channel s (s < num_sensors - 1)
+__global__ void stream_markers_kernel(const mlgpu_pulses p, const int32_t num_sensors,
+                                      const int64_t* __restrict__ kscan, uint8_t* __restrict__ symbols) {
+    const int32_t s = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= num_sensors - 1) return;
+    symbols[kscan[p.offsets[s + 1]] + s] = kNextChannelMarker;
+}
+
 size_t align_up(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
 }  // namespace
@@ -748,6 +778,55 @@ hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m,
         }
     }
     if (nsb_mem) (void)hipFree(nsb_mem);
+    (void)hipFree(mem);
+    return e;
+}
+
+hipError_t photon_stream(const mlgpu_pulses& p, const int16_t* slice, int32_t num_sensors, int64_t count,
+                         uint8_t* symbols, int32_t* truth, int64_t capacity, int64_t* num_symbols, int* too_small,
+                         hipStream_t stream) {
+    *num_symbols = 0;
+    *too_small = 0;
+    const size_t m = static_cast<size_t>(count);
+    size_t scan_bytes = 0;
+    const auto kept = rocprim::make_transform_iterator(rocprim::counting_iterator<int64_t>(0), Kept{slice});
+    hipError_t e = rocprim::exclusive_scan(nullptr, scan_bytes, kept, static_cast<int64_t*>(nullptr), int64_t{0},
+                                           m + 1, rocprim::plus<int64_t>(), stream);
+    if (e != hipSuccess) return e;
+    char* mem = nullptr;
+    e = hipMalloc(&mem, align_up((m + 2) * 8) + align_up(scan_bytes));
+    if (e != hipSuccess) return e;
+    auto* kscan = reinterpret_cast<int64_t*>(mem);
+    void* temp = mem + align_up((m + 2) * 8);
+    int64_t total = 0;
+    // kscan[k]: kept pulses before pulse k; kscan[count] (the total) is
+    // filled in from the last pulse below
+    e = rocprim::exclusive_scan(temp, scan_bytes, kept, kscan, int64_t{0}, m, rocprim::plus<int64_t>(), stream);
+    int64_t last = 0;
+    int16_t last_slice = -1;
+    if (e == hipSuccess && m > 0) e = hipMemcpyAsync(&last, kscan + m - 1, 8, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess && m > 0) e = hipMemcpyAsync(&last_slice, slice + m - 1, 2, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) {
+        total = m > 0 ? last + (last_slice >= 0 ? 1 : 0) : 0;  // kept pulses
+        e = hipMemcpyAsync(kscan + m, &total, 8, hipMemcpyHostToDevice, stream);
+    }
+    if (e == hipSuccess) {
+        // PhotonStream::num_symbols_to_represent (PhotonStream.cpp:181-193)
+        const int64_t sym = total + num_sensors > 0 ? total + num_sensors - 1 : 0;
+        *num_symbols = sym;
+        if (sym > capacity) *too_small = 1;
+    }
+    if (e == hipSuccess && !*too_small) {
+        if (m > 0)
+            hipLaunchKernelGGL(stream_pulses_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0,
+                               stream, p, slice, count, num_sensors, kscan, symbols, truth);
+        if (num_sensors > 1)
+            hipLaunchKernelGGL(stream_markers_kernel, dim3(static_cast<uint32_t>((num_sensors + 255) / 256)),
+                               dim3(256), 0, stream, p, num_sensors, kscan, symbols);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    }
     (void)hipFree(mem);
     return e;
 }

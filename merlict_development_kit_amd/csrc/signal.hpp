@@ -42,4 +42,11 @@ hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m,
                       uint64_t seed, const mlgpu_pipelines& out, int64_t* num_photons, double* start, int* bad,
                       int* too_small, hipStream_t stream);
 
+// The photon-stream symbols of count pulses of num_sensors channels: kept
+// pulses' slices with a marker between channels.  Synchronises;
+// *num_symbols receives their number, *too_small when it exceeds capacity.
+hipError_t photon_stream(const mlgpu_pulses& p, const int16_t* slice, int32_t num_sensors, int64_t count,
+                         uint8_t* symbols, int32_t* truth, int64_t capacity, int64_t* num_symbols, int* too_small,
+                         hipStream_t stream);
+
 }  // namespace mlgpu

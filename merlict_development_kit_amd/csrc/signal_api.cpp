@@ -229,6 +229,34 @@ int mlgpu_extract_pulses(const mlgpu_pulses* p, int32_t num_sensors, double dura
     return MLGPU_OK;
 }
 
+int mlgpu_photon_stream(const mlgpu_pulses* p, const int16_t* slice, int32_t num_sensors, uint8_t* symbols,
+                        int32_t* truth, int64_t capacity, int64_t* num_symbols, void* stream) {
+    if (!p || !num_symbols || num_sensors < 0 || capacity < 0 || !p->offsets) {
+        set_last_error("null argument or negative size");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    *num_symbols = 0;
+    if (num_sensors == 0) return MLGPU_OK;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    int64_t count = 0;
+    hipError_t e = hipMemcpyAsync(&count, p->offsets + num_sensors, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "photon stream");
+    if (count > 0 && (!slice || (truth && !p->truth_id))) {
+        set_last_error("null slice or truth array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (!symbols && count + num_sensors > 1) {
+        set_last_error("null symbols");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    int too_small = 0;
+    e = mlgpu::photon_stream(*p, slice, num_sensors, count, symbols, truth, capacity, num_symbols, &too_small, st);
+    if (e != hipSuccess) return hip_fail(e, "photon stream");
+    if (too_small) { set_last_error("capacity below the number of symbols"); return MLGPU_ERR_ARGUMENT; }
+    return MLGPU_OK;
+}
+
 int mlgpu_nsb_light_rate(const double* flux_x, const double* flux_y, int32_t flux_n, double max_fov_radius,
                          double max_aperture_radius, double* rate) {
     if (!rate) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }

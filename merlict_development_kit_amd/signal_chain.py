@@ -7,6 +7,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 from dataclasses import dataclass
+from pathlib import Path
 from typing import Optional
 
 import numpy as np
@@ -149,3 +150,56 @@ def inject_nsb(pipes: Pipelines, num_sensors: int, efficiencies: torch.Tensor, f
             continue
         N.check(rc, "mlgpu_inject_nsb")
     N.check(rc, "mlgpu_inject_nsb")
+
+
+NEXT_CHANNEL_MARKER = 255   # merlict_signal_processing/PhotonStream.h:15
+NUMBER_TIME_SLICES = 100    # pulse_extraction.h:13
+
+
+def photon_stream(pulses: Pulses, slices: torch.Tensor, num_sensors: int, stream=None):
+    """(symbols uint8, truth int32) on the GPU: the kept pulses' slices
+    channel by channel with a marker between channels, and their truth ids
+    (mlgpu_photon_stream)."""
+    dev = pulses.offsets.device
+    if slices.dtype != torch.int16 or len(slices) != len(pulses.arrival_time):
+        raise ValueError("slices must be one int16 per pulse")
+    cap = len(slices) + num_sensors
+    sym = torch.empty(max(cap, 1), dtype=torch.uint8, device=dev)
+    truth = torch.empty(max(len(slices), 1), dtype=torch.int32, device=dev)
+    k = C.c_int64()
+    N.check(N.lib().mlgpu_photon_stream(C.byref(pulses.c_struct()), C.c_void_p(slices.data_ptr() if len(slices) else None),
+                                        num_sensors, C.c_void_p(sym.data_ptr()), C.c_void_p(truth.data_ptr()), cap,
+                                        C.byref(k), _st(stream, dev)), "mlgpu_photon_stream")
+    n = k.value
+    kept = n - num_sensors + 1 if n > 0 else 0
+    return sym[:n], truth[:kept]
+
+
+def write_photon_stream(path, symbols, num_channels: int, time_slice_duration: float) -> None:
+    """PhotonStream::write (PhotonStream.cpp:15-58): float32 slice duration,
+    uint32 channels, uint32 100 slices, uint32 symbols, then the symbols."""
+    sym = symbols.cpu().numpy() if isinstance(symbols, torch.Tensor) else np.asarray(symbols, np.uint8)
+    head = np.array([time_slice_duration], "<f4").tobytes() + np.array(
+        [num_channels, NUMBER_TIME_SLICES, len(sym)], "<u4").tobytes()
+    Path(path).write_bytes(head + np.ascontiguousarray(sym, np.uint8).tobytes())
+
+
+def write_simulation_truth(path, truth) -> None:
+    """PhotonStream::write_simulation_truth (PhotonStream.cpp:60-78): int32 per pulse."""
+    t = truth.cpu().numpy() if isinstance(truth, torch.Tensor) else np.asarray(truth)
+    Path(path).write_bytes(np.ascontiguousarray(t, "<i4").tobytes())
+
+
+def read_photon_stream(path):
+    """PhotonStream::read (PhotonStream.cpp:80-130): (time slice duration,
+    list of per-channel uint8 slice arrays)."""
+    raw = Path(path).read_bytes()
+    dur = float(np.frombuffer(raw[:4], "<f4")[0])
+    _, slices, n = (int(v) for v in np.frombuffer(raw[4:16], "<u4"))
+    if slices != NUMBER_TIME_SLICES:
+        raise ValueError(f"expected {NUMBER_TIME_SLICES} time slices, got {slices}")
+    sym = np.frombuffer(raw[16:16 + n], np.uint8)
+    cut = np.nonzero(sym == NEXT_CHANNEL_MARKER)[0]
+    channels = np.split(sym, cut)
+    channels = [channels[0]] + [c[1:] for c in channels[1:]] if n > 0 else []
+    return dur, channels

@@ -670,4 +670,7 @@ def ref_signal(sensor_id, time_of_flight, wavelength, truth_id, num_sensors, qe_
     out["extracted_offsets"] = take(np.int64, num_sensors + 1)
     x = int(out["extracted_offsets"][-1])
     out["extracted_slice"], out["extracted_truth"] = take(np.int16, x), take(np.int32, x)
+    for name in ("photon_stream", "photon_stream_truth"):
+        nb = int(take(np.int64, 1)[0])
+        out[name] = take(np.uint8, nb)
     return out

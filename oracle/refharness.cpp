@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -518,6 +519,7 @@ static int cmd_signal(int argc, char** argv) {
     // pipeline (seed, s, stream 8: its accidental pulses) and one pulse
     // (seed, k, stream 7) at a time.
     int64_t j = 0, k = 0;
+    std::vector<std::vector<sp::ExtractedPulse>> channels(static_cast<size_t>(ns));
     for (int32_t s = 0; s < ns; s++) {
         sp::sort_photon_pipelines_arrival_time(&pipes[s]);
         for (const auto& ph : pipes[s]) { pt.push_back(ph.arrival_time); pl.push_back(ph.wavelength); ptr.push_back(ph.simulation_truth_id); }
@@ -535,7 +537,11 @@ static int cmd_signal(int argc, char** argv) {
         for (const auto& pu : pulses) {
             PhiloxGenerator g7(seed, static_cast<uint64_t>(k++), 7u);
             const std::vector<sp::ElectricPulse> one{pu};
-            for (const auto& x : sp::extract_pulses(one, dur, sd, &g7)) { xs.push_back(static_cast<int16_t>(x.arrival_time_slice)); xtr.push_back(x.simulation_truth_id); }
+            for (const auto& x : sp::extract_pulses(one, dur, sd, &g7)) {
+                xs.push_back(static_cast<int16_t>(x.arrival_time_slice));
+                xtr.push_back(x.simulation_truth_id);
+                channels[s].push_back(x);
+            }
         }
         xo.push_back(static_cast<int64_t>(xs.size()));
     }
@@ -544,6 +550,17 @@ static int cmd_signal(int argc, char** argv) {
     w(po.data(), po.size() * 8); w(pt.data(), pt.size() * 8); w(pl.data(), pl.size() * 8); w(ptr.data(), ptr.size() * 4);
     w(eo.data(), eo.size() * 8); w(et.data(), et.size() * 8); w(etr.data(), etr.size() * 4);
     w(xo.data(), xo.size() * 8); w(xs.data(), xs.size() * 2); w(xtr.data(), xtr.size() * 4);
+    // the photon stream and its truth as the reference writes them
+    const std::string phs = std::string(argv[3]) + ".phs", tru = std::string(argv[3]) + ".truth";
+    sp::PhotonStream::write(channels, static_cast<float>(dur), phs);
+    sp::PhotonStream::write_simulation_truth(channels, tru);
+    for (const std::string& path : {phs, tru}) {
+        std::ifstream f(path, std::ios::binary);
+        const std::vector<char> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        const int64_t nb = static_cast<int64_t>(bytes.size());
+        w(&nb, 8);
+        w(bytes.data(), bytes.size());
+    }
     return 0;
 }
 

@@ -11,7 +11,8 @@ import pyoracle as O
 from merlict_development_kit_amd._native import MlgpuError
 from merlict_development_kit_amd import GpuScene, PropagationConfig, SceneDescription, sources
 from merlict_development_kit_amd.signal_chain import (Pipelines, extract_pulses, inject_nsb, photoelectric_convert,
-                                                      sensor_pipelines)
+                                                      photon_stream, sensor_pipelines, write_photon_stream,
+                                                      write_simulation_truth)
 
 pytestmark = pytest.mark.gpu
 
@@ -23,7 +24,7 @@ def cuda():
     return torch.device("cuda:0")
 
 
-def test_gpu_signal_chain_matches_reference(cuda, golden):
+def test_gpu_signal_chain_matches_reference(cuda, golden, tmp_path):
     g = golden("signal_chain")
     a = {k[3:]: g[k] for k in g if k.startswith("in_")}
     ref = {k[4:]: g[k] for k in g if k.startswith("out_")}
@@ -46,11 +47,20 @@ def test_gpu_signal_chain_matches_reference(cuda, golden):
     np.testing.assert_array_equal(t[photon], ref["pulse_time"][photon])
     # accidental times: -log(u) / rate, device log against glibc's (both < 1 ulp)
     np.testing.assert_allclose(t, ref["pulse_time"], rtol=1e-14, atol=0)
-    sl = extract_pulses(pulses, ns, float(a["duration"][0]), float(a["std_dev"][0]), int(a["seed"][0])).cpu().numpy()
+    slt = extract_pulses(pulses, ns, float(a["duration"][0]), float(a["std_dev"][0]), int(a["seed"][0]))
+    sl = slt.cpu().numpy()
     kept = sl >= 0
     assert kept.sum() == len(ref["extracted_slice"])
     np.testing.assert_array_equal(sl[kept], ref["extracted_slice"])
     np.testing.assert_array_equal(tr[kept], ref["extracted_truth"])
+    # the photon stream file and its truth, byte for byte as the reference writes them
+    sym, truth = photon_stream(pulses, slt, ns)
+    p = tmp_path / "raw.phs"
+    write_photon_stream(p, sym, ns, float(a["duration"][0]))
+    assert p.read_bytes() == ref["photon_stream"].tobytes()
+    q = tmp_path / "truth.bin"
+    write_simulation_truth(q, truth)
+    assert q.read_bytes() == ref["photon_stream_truth"].tobytes()
 
 
 def test_gpu_pipelines_of_propagated_photons_match_oracle(cuda):
@@ -190,3 +200,16 @@ def test_gpu_pipelines_keep_photon_order_on_equal_times(cuda):
     np.testing.assert_array_equal(pipes.offsets.cpu().numpy(), off)
     np.testing.assert_array_equal(pipes.arrival_time.cpu().numpy(), ot)
     np.testing.assert_array_equal(pipes.truth_id.cpu().numpy(), otr)
+
+
+def test_gpu_photon_stream_edges(cuda):
+    """Channels without kept pulses still get their markers; one channel
+    without pulses has no symbol (PhotonStream::num_symbols_to_represent)."""
+    from merlict_development_kit_amd.signal_chain import Pulses
+    T = lambda x, dt: torch.tensor(x, dtype=dt, device=cuda)
+    p = Pulses(T([0, 2, 2, 3], torch.int64), T([1e-9, 2e-9, 3e-9], torch.float64), T([5, 6, 7], torch.int32))
+    sym, truth = photon_stream(p, T([4, -1, 99], torch.int16), 3)
+    assert sym.cpu().tolist() == [4, 255, 255, 99] and truth.cpu().tolist() == [5, 7]
+    e = Pulses(T([0, 0], torch.int64), T([], torch.float64), T([], torch.int32))
+    sym, truth = photon_stream(e, T([], torch.int16), 1)
+    assert len(sym) == 0 and len(truth) == 0

@@ -795,3 +795,29 @@ def test_ppm_format(tmp_path):
     raw = p.read_bytes()
     assert raw == b"P6\n# CREATOR: MERLICT\n2 1\n255\n" + bytes([0, 127, 255, 1, 2, 254])
     np.testing.assert_array_equal(V.read_ppm(p), np.trunc(img))
+
+
+def test_photon_stream_format(golden, tmp_path):
+    """PhotonStream::write and write_simulation_truth (PhotonStream.cpp:15-78)
+    of the reference's extracted channels: the writer here, fed the same
+    channels, produces the same bytes, and the reader gives them back."""
+    from merlict_development_kit_amd import signal_chain as SC
+    _, ref = _signal_golden(golden)
+    off, sl, tr = ref["extracted_offsets"], ref["extracted_slice"], ref["extracted_truth"]
+    ns = len(off) - 1
+    parts = []
+    for s_ in range(ns):
+        parts.append(sl[off[s_]:off[s_ + 1]].astype(np.uint8))
+        if s_ < ns - 1:
+            parts.append(np.array([SC.NEXT_CHANNEL_MARKER], np.uint8))
+    sym = np.concatenate(parts)
+    p = tmp_path / "raw.phs"
+    SC.write_photon_stream(p, sym, ns, 0.5e-9)
+    assert p.read_bytes() == ref["photon_stream"].tobytes()
+    q = tmp_path / "truth.bin"
+    SC.write_simulation_truth(q, tr)
+    assert q.read_bytes() == ref["photon_stream_truth"].tobytes()
+    dur, channels = SC.read_photon_stream(p)
+    assert dur == np.float32(0.5e-9) and len(channels) == ns
+    for s_ in range(ns):
+        np.testing.assert_array_equal(channels[s_], sl[off[s_]:off[s_ + 1]])
