@@ -271,6 +271,7 @@ def calibration_leg(dev, cpu_threads: int, num_blocks: int = 8, photons_per_bloc
                              seed=2)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
+    _PLENOSCOPE["efficiencies"] = st["efficiency"].copy()
     # the statistics part alone, on one propagated block
     ph = src.generate(photons_per_block, seed=3, device=dev)
     res = scene.propagate(ph, PropagationConfig(5, seed=3))
@@ -334,6 +335,49 @@ def render_leg(dev, reps: int = 5, cpu: bool = True) -> dict:
         out["cpu_baseline"] = {"value": 1280 * 720 / s_cpu, "unit": "pixels/s", "cores": 1, "kind": "reference",
                                "sample": f"1280 x 720 view of the same camera, one thread, {s_cpu:.2f} s"}
     return out
+
+
+def event_leg(dev, events: int = 4, bunches: int = 1_000_000) -> dict:
+    """Plenoscope events as merlict's plenoscope propagation app runs them
+    (event.propagate_event): per event 1e6 CORSIKA-like photon bunches
+    (weights 0.5-3, within 30 m of the axis), propagation, pipelines of the
+    515,023 lixels, the night sky of the example card at its full rate over
+    the plenoscope (about 1.1e7 background photons per event) with the
+    calibration leg's lixel efficiencies (0.8 each without it), converter,
+    extraction, photon stream; whole events timed."""
+    from merlict_development_kit_amd import event as EV
+    _, scene = plenoscope_scene()
+    card = EV.EventCard.from_json(ROOT / "tests" / "golden" / "plenoscope_propagation_input_card.json")
+    card = card.with_padded_quantum_efficiency()
+    rate = EV.plenoscope_nsb_rate(card)
+    eff_np = _PLENOSCOPE.get("efficiencies")
+    eff = torch.tensor(eff_np, device=dev) if eff_np is not None else torch.full((515_023,), 0.8, device=dev)
+    rng = np.random.default_rng(11)
+
+    def make(n):
+        r, phi = 3000.0 * np.sqrt(rng.uniform(0, 1, n)), rng.uniform(0, 2 * np.pi, n)
+        b = np.empty((n, 8), np.float32)
+        b[:, 0], b[:, 1] = r * np.cos(phi), r * np.sin(phi)
+        b[:, 2], b[:, 3] = rng.normal(0, 0.004, n), rng.normal(0, 0.004, n)
+        b[:, 4], b[:, 5], b[:, 6], b[:, 7] = rng.uniform(0, 5, n), 1e6, rng.uniform(0.5, 3.0, n), -rng.uniform(300, 550, n)
+        return torch.tensor(b, device=dev)
+
+    batches = [make(bunches) for _ in range(events)]
+    EV.propagate_event(scene, batches[0], 515_023, eff, rate, card, seed=100)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [EV.propagate_event(scene, b, 515_023, eff, rate, card, seed=e) for e, b in enumerate(batches)]
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    photons = sum(r.num_photons for r in stats)
+    return {"workload": f"plenoscope events, {bunches} CORSIKA-like bunches each, full example-card night sky "
+                        "(about 1.1e7 background photons), 515,023 lixels, card converter and photon stream",
+            "events": events, "ms_per_event": sec / events * 1e3, "value": photons / sec,
+            "unit": "Cherenkov photons/s through the whole event", "photons_per_event": photons / events,
+            "detected_per_event": sum(r.num_detected for r in stats) / events,
+            "pulses_per_event": sum(r.num_pulses for r in stats) / events,
+            "symbols_per_event": sum(len(r.symbols) for r in stats) / events,
+            "efficiencies": "calibration leg" if eff_np is not None else "0.8 each"}
 
 
 def plenoscope_bunches_leg(dev, batches: int = 4, bunches_per_batch: int = 256) -> dict:
@@ -520,6 +564,7 @@ def main() -> int:
     ap.add_argument("--no-signal", action="store_true")
     ap.add_argument("--no-calibration", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--no-event", action="store_true")
     ap.add_argument("--no-stream", action="store_true", help="skip the full-scale hybrid stream (about 1e9 photons)")
     ap.add_argument("--hybrid-train-photons", type=int, default=1_000_000_000)
     ap.add_argument("--hybrid-max-entropy", type=float, default=float("inf"))
@@ -633,6 +678,8 @@ def main() -> int:
         line["render"] = render_leg(dev, cpu=not args.no_cpu_baseline)
     if not args.no_calibration:
         line["plenoscope_calibration"] = calibration_leg(dev, args.cpu_threads, cpu=not args.no_cpu_baseline)
+    if not args.no_event:
+        line["plenoscope_event"] = event_leg(dev)
     if not args.no_corsika:
         line["corsika_ingestion"] = corsika_leg(dev)
     if not args.no_cpu_baseline:
