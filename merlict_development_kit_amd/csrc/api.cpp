@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "pool.hpp"
 #include "scene_build.hpp"
 
 struct mlgpu_scene {
@@ -644,10 +645,11 @@ int mlgpu_render_pinhole(const mlgpu_scene* s, int32_t world_frame, const mlgpu_
     std::vector<float> colors(static_cast<size_t>(n_frames) * 6, 25.0f);
     if (frame_colors) std::copy(frame_colors, frame_colors + colors.size(), colors.begin());
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    char* mem = nullptr;
     const size_t cbytes = (colors.size() * sizeof(float) + 255) & ~static_cast<size_t>(255);
-    hipError_t e = hipMalloc(&mem, cbytes + 256);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc render");
+    mlgpu::PoolBlock block;
+    hipError_t e = block.get(cbytes + 256);
+    if (e != hipSuccess) return hip_fail(e, "render workspace");
+    char* mem = block.as();
     float* dcolors = reinterpret_cast<float*>(mem);
     auto* derr = reinterpret_cast<unsigned long long*>(mem + cbytes);
     unsigned long long herr = 0;
@@ -657,7 +659,8 @@ int mlgpu_render_pinhole(const mlgpu_scene* s, int32_t world_frame, const mlgpu_
         e = mlgpu::launch_render(s->dev, s->scope_of_frame[world_frame], rc, prm, dcolors, seed, image, derr, st);
     if (e == hipSuccess) e = hipMemcpyAsync(&herr, derr, sizeof(herr), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(mem);
+    (void)hipStreamSynchronize(st);  // the borrowed block is idle when returned
+    block.release();
     if (e != hipSuccess) return hip_fail(e, "render");
     *num_errors = static_cast<int64_t>(herr);
     return MLGPU_OK;
@@ -707,15 +710,11 @@ int mlgpu_corsika_photons(const float* bunches, int64_t n, uint64_t seed, uint64
     }
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const size_t ws_bytes = mlgpu::corsika_workspace_bytes(n);
-    void* ws = nullptr;
-    hipError_t e = hipMalloc(&ws, ws_bytes + 256);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc CORSIKA workspace");
-    int64_t* host = nullptr;  // {bad, total}
-    e = hipHostMalloc(reinterpret_cast<void**>(&host), 2 * sizeof(int64_t));
-    if (e != hipSuccess) {
-        (void)hipFree(ws);
-        return hip_fail(e, "hipHostMalloc");
-    }
+    mlgpu::PoolBlock block;
+    hipError_t e = block.get(ws_bytes + 256);
+    if (e != hipSuccess) return hip_fail(e, "CORSIKA workspace");
+    void* ws = block.as<void>();
+    int64_t host[2] = {0, 0};  // {bad, total}
     char* flags = static_cast<char*>(ws) + ws_bytes;  // int32 bad at +0, int64 total at +8
     int32_t* bad = reinterpret_cast<int32_t*>(flags);
     int64_t* total = reinterpret_cast<int64_t*>(flags + 8);
@@ -740,8 +739,7 @@ int mlgpu_corsika_photons(const float* bunches, int64_t n, uint64_t seed, uint64
             if (e != hipSuccess) rc = hip_fail(e, "CORSIKA emit");
         }
     }
-    (void)hipHostFree(host);
-    (void)hipFree(ws);
+    (void)hipStreamSynchronize(st);  // the borrowed block is idle when returned
     return rc;
 }
 

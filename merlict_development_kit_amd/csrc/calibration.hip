@@ -19,6 +19,7 @@
 
 #include "../../include/mlgpu.h"
 #include "kernels.hpp"
+#include "pool.hpp"
 #include "propagate_core.hpp"
 #include "sources_core.hpp"
 
@@ -263,9 +264,10 @@ int mlgpu_lixel_filler_fill_block(mlgpu_lixel_filler* f, const mlgpu_calibration
     if (e != hipSuccess) return mlgpu::hip_fail(e, "lixel sort");
     const size_t bytes = 2 * mlgpu::align_up(N * 4) + 2 * mlgpu::align_up(N * 4) + mlgpu::align_up((L + 1) * 8) +
                          mlgpu::align_up(temp_bytes) + mlgpu::align_up(4);
-    char* mem = nullptr;
-    e = hipMalloc(&mem, bytes);
-    if (e != hipSuccess) return mlgpu::hip_fail(e, "hipMalloc lixel block");
+    mlgpu::PoolBlock pool_block;
+    e = pool_block.get(bytes);
+    if (e != hipSuccess) return mlgpu::hip_fail(e, "lixel block workspace");
+    char* mem = pool_block.as();
     char* q = mem;
     auto take = [&](size_t b) { char* x = q; q += mlgpu::align_up(b); return x; };
     auto* key = reinterpret_cast<uint32_t*>(take(N * 4));
@@ -296,7 +298,8 @@ int mlgpu_lixel_filler_fill_block(mlgpu_lixel_filler* f, const mlgpu_calibration
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
-    (void)hipFree(mem);
+    (void)hipStreamSynchronize(st);  // the borrowed block is idle when returned
+    pool_block.release();
     if (e != hipSuccess) return mlgpu::hip_fail(e, "lixel statistics fill");
     if (host_bad) {
         mlgpu::set_last_error("a record reached a lixel_id >= num_lixel");

@@ -29,6 +29,7 @@
 
 #include "kernels.hpp"
 #include "propagate_core.hpp"
+#include "pool.hpp"
 #include "signal.hpp"
 #include "stream_rng.hpp"
 
@@ -540,9 +541,10 @@ hipError_t build_pipelines(const mlgpu_results& r, const double* wavelength, con
     temp_bytes = temp_bytes > temp3 ? temp_bytes : temp3;
     const size_t bytes = 2 * align_up(cap * 4) + 2 * align_up(cap * 8) + 2 * align_up(cap * 4) + align_up(temp_bytes) +
                          256;
-    char* mem = nullptr;
-    e = hipMalloc(&mem, bytes);
+    PoolBlock block;
+    e = block.get(bytes);
     if (e != hipSuccess) return e;
+    char* mem = block.as();
     char* q = mem;
     auto take = [&](size_t b) { char* x = q; q += align_up(b); return x; };
     auto* skey = reinterpret_cast<uint32_t*>(take(cap * 4));
@@ -600,7 +602,7 @@ hipError_t build_pipelines(const mlgpu_results& r, const double* wavelength, con
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    (void)hipFree(mem);
+    (void)hipStreamSynchronize(stream);  // the borrowed blocks are idle when returned
     return e;
 }
 
@@ -622,9 +624,10 @@ hipError_t convert_pulses(const mlgpu_pipelines& p, int32_t num_sensors, int64_t
     if (e != hipSuccess) return e;
     const size_t scan_bytes = scan_a > scan_b ? scan_a : scan_b;
     const size_t bytes = 2 * align_up((m + 1) * 8) + align_up((ns + 1) * 8) + align_up(scan_bytes) + 256;
-    char* mem = nullptr;
-    e = hipMalloc(&mem, bytes);
+    PoolBlock block;
+    e = block.get(bytes);
     if (e != hipSuccess) return e;
+    char* mem = block.as();
     char* q = mem;
     auto take = [&](size_t b) { char* x = q; q += align_up(b); return x; };
     auto* pcount = reinterpret_cast<int64_t*>(take((m + 1) * 8));
@@ -669,7 +672,7 @@ hipError_t convert_pulses(const mlgpu_pipelines& p, int32_t num_sensors, int64_t
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
     }
-    (void)hipFree(mem);
+    (void)hipStreamSynchronize(stream);  // the borrowed blocks are idle when returned
     return e;
 }
 
@@ -701,9 +704,10 @@ hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m,
     if (e != hipSuccess) return e;
     const size_t bytes = align_up(16) + align_up(static_cast<size_t>(count) * 4) + 2 * align_up((ns + 1) * 8) +
                          align_up(scan_bytes) + align_up(8) + align_up(4) + 256;
-    char* mem = nullptr;
-    e = hipMalloc(&mem, bytes);
+    PoolBlock block;
+    e = block.get(bytes);
     if (e != hipSuccess) return e;
+    char* mem = block.as();
     char* q = mem;
     auto take = [&](size_t b) { char* x = q; q += align_up(b); return x; };
     auto* range = reinterpret_cast<unsigned long long*>(take(16));
@@ -713,6 +717,7 @@ hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m,
     void* temp = take(scan_bytes);
     auto* dstart = reinterpret_cast<double*>(take(8));
     auto* dbad = reinterpret_cast<int32_t*>(take(4));
+    PoolBlock nsb_block;
     char* nsb_mem = nullptr;
     int64_t total = 0;
     int32_t host_bad = 0;
@@ -751,7 +756,8 @@ hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m,
     }
     if (e == hipSuccess && !*bad && !*too_small) {
         const size_t nt = static_cast<size_t>(total > 0 ? total : 1);
-        e = hipMalloc(&nsb_mem, 2 * align_up(nt * 8));
+        e = nsb_block.get(2 * align_up(nt * 8));
+        nsb_mem = nsb_block.as();
         double* nsb_time = reinterpret_cast<double*>(nsb_mem);
         double* nsb_lambda = reinterpret_cast<double*>(nsb_mem + align_up(nt * 8));
         if (e == hipSuccess && total > 0) {
@@ -777,8 +783,7 @@ hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m,
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
         }
     }
-    if (nsb_mem) (void)hipFree(nsb_mem);
-    (void)hipFree(mem);
+    (void)hipStreamSynchronize(stream);  // the borrowed blocks are idle when returned
     return e;
 }
 
@@ -793,9 +798,10 @@ hipError_t photon_stream(const mlgpu_pulses& p, const int16_t* slice, int32_t nu
     hipError_t e = rocprim::exclusive_scan(nullptr, scan_bytes, kept, static_cast<int64_t*>(nullptr), int64_t{0},
                                            m + 1, rocprim::plus<int64_t>(), stream);
     if (e != hipSuccess) return e;
-    char* mem = nullptr;
-    e = hipMalloc(&mem, align_up((m + 2) * 8) + align_up(scan_bytes));
+    PoolBlock block;
+    e = block.get(align_up((m + 2) * 8) + align_up(scan_bytes));
     if (e != hipSuccess) return e;
+    char* mem = block.as();
     auto* kscan = reinterpret_cast<int64_t*>(mem);
     void* temp = mem + align_up((m + 2) * 8);
     int64_t total = 0;
@@ -827,7 +833,7 @@ hipError_t photon_stream(const mlgpu_pulses& p, const int16_t* slice, int32_t nu
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
     }
-    (void)hipFree(mem);
+    (void)hipStreamSynchronize(stream);  // the borrowed blocks are idle when returned
     return e;
 }
 

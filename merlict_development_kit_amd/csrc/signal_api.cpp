@@ -9,6 +9,7 @@
 
 #include "../../include/mlgpu.h"
 #include "device_scene.hpp"
+#include "pool.hpp"
 #include "signal.hpp"
 
 namespace mlgpu {
@@ -185,16 +186,16 @@ int mlgpu_photoelectric_convert(const mlgpu_pipelines* p, int32_t num_sensors, c
     for (int i = 0; i + 1 < n; i++)
         if (pts[i].x == pts[i + 1].x) { set_last_error("quantum efficiency table has a duplicate wavelength"); return MLGPU_ERR_ARGUMENT; }
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    mlgpu::FuncPoint* qe = nullptr;
-    hipError_t e = hipMalloc(&qe, sizeof(mlgpu::FuncPoint) * pts.size());
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc quantum efficiency");
+    mlgpu::PoolBlock qe_block;
+    hipError_t e = qe_block.get(sizeof(mlgpu::FuncPoint) * pts.size());
+    if (e != hipSuccess) return hip_fail(e, "quantum efficiency table");
+    auto* qe = qe_block.as<mlgpu::FuncPoint>();
     e = hipMemcpyAsync(qe, pts.data(), sizeof(mlgpu::FuncPoint) * pts.size(), hipMemcpyHostToDevice, st);
     int bad = 0, too_small = 0;
     int64_t m = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&m, p->offsets + num_sensors, sizeof(int64_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess && m > 0 && (!p->arrival_time || !p->wavelength || !p->truth_id)) {
-        (void)hipFree(qe);
         set_last_error("null pipeline array");
         return MLGPU_ERR_ARGUMENT;
     }
@@ -202,7 +203,6 @@ int mlgpu_photoelectric_convert(const mlgpu_pipelines* p, int32_t num_sensors, c
         e = mlgpu::convert_pulses(*p, num_sensors, m, qe, n, c->dark_rate, c->crosstalk, c->exposure_time, c->seed,
                                   *out, num_pulses, &bad, &too_small, st);
     (void)hipStreamSynchronize(st);
-    (void)hipFree(qe);
     if (e != hipSuccess) return hip_fail(e, "photo-electric conversion");
     if (bad) { set_last_error("a wavelength outside the quantum efficiency table"); return MLGPU_ERR_ARGUMENT; }
     if (too_small) { set_last_error("capacity below the number of pulses"); return MLGPU_ERR_ARGUMENT; }
@@ -307,9 +307,10 @@ int mlgpu_inject_nsb(const mlgpu_pipelines* in, int32_t num_sensors, const float
         set_last_error("null pipeline array");
         return MLGPU_ERR_ARGUMENT;
     }
-    mlgpu::FuncPoint* dcdf = nullptr;
-    e = hipMalloc(&dcdf, sizeof(mlgpu::FuncPoint) * cdf.p.size());
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc night-sky table");
+    mlgpu::PoolBlock cdf_block;
+    e = cdf_block.get(sizeof(mlgpu::FuncPoint) * cdf.p.size());
+    if (e != hipSuccess) return hip_fail(e, "night-sky table");
+    auto* dcdf = cdf_block.as<mlgpu::FuncPoint>();
     e = hipMemcpyAsync(dcdf, cdf.p.data(), sizeof(mlgpu::FuncPoint) * cdf.p.size(), hipMemcpyHostToDevice, st);
     int bad = 0, too_small = 0;
     if (e == hipSuccess)
@@ -317,7 +318,6 @@ int mlgpu_inject_nsb(const mlgpu_pipelines* in, int32_t num_sensors, const float
                               nsb->rate, nsb->exposure_time, nsb->seed, *out, num_photons, exposure_start_time, &bad,
                               &too_small, st);
     (void)hipStreamSynchronize(st);
-    (void)hipFree(dcdf);
     if (e != hipSuccess) return hip_fail(e, "night-sky background");
     if (bad) {
         set_last_error("night-sky background: a negative sensor rate, more than 1e7 expected photons in one sensor, "
