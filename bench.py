@@ -463,8 +463,9 @@ def signal_leg(result, photons, dev, reps: int = 5) -> dict:
     efficiency, dark rate and crosstalk over the exposure) and pulse
     extraction (the card's slices and time resolution), each timed on the
     GPU; the oracle (C, one thread) times the same chain on the host."""
-    from merlict_development_kit_amd.signal_chain import (extract_pulses, inject_nsb, nsb_light_rate,
-                                                          photoelectric_convert, photon_stream, sensor_pipelines)
+    from merlict_development_kit_amd.signal_chain import (extract_pulses, fact_photon_stream, inject_nsb,
+                                                          nsb_light_rate, photoelectric_convert, photon_stream,
+                                                          sensor_pipelines)
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
     c = signal_card()
@@ -493,6 +494,16 @@ def signal_leg(result, photons, dev, reps: int = 5) -> dict:
         if r > 0:
             for k, dt in zip(times, (t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4)):
                 times[k] += dt / reps
+    # FACT's own event format on the same pulses (in place of extraction and
+    # the plenoscope's photon stream), timed apart from the chain
+    fact_ms = 0.0
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        raw = fact_photon_stream(pulses, 1440)
+        torch.cuda.synchronize()
+        if r > 0:
+            fact_ms += (time.perf_counter() - t0) * 1e3 / reps
     n = len(result.sensor_id)
     m = len(pipes.arrival_time)
     sample = min(n, 2_000_000)
@@ -509,6 +520,7 @@ def signal_leg(result, photons, dev, reps: int = 5) -> dict:
     return {"photons": n, "detected": m, "with_background": len(lit.arrival_time),
             "pulses": len(pulses.arrival_time), "nsb_rate_hz": rate,
             "ms": {k: v * 1e3 for k, v in times.items()},
+            "fact_photon_stream_ms": fact_ms, "fact_photon_stream_bytes": len(raw),
             "value": n / gpu, "unit": "photons/s (whole chain on one pass's results)",
             "cpu_baseline": {"value": sample / cpu, "unit": "photons/s", "cores": 1, "kind": "port",
                              "sample": f"the first {sample} photon results, oracle restatement"}}

@@ -614,6 +614,18 @@ int mlgpu_photon_stream(const mlgpu_pulses* pulses, const int16_t* slice, int32_
                         uint8_t* symbols, int32_t* truth, int64_t capacity, int64_t* num_symbols,
                         void* stream);
 
+/* FACT's photon stream of an event (merlict_fact/apps/fact_propagation.cpp:
+ * 34-94, electric_pipelines_to_photon_stream on the converter's pulses):
+ * every pulse's slice round((t - median + trigger_time) / 0.5 ns), with
+ * the median the sorted arrival time at floor(pulses / 2) of the whole
+ * event, kept within [30, 130) (photon_stream.h:29-34), pixel by pixel,
+ * each pixel ended by NEXT_PIXEL_MARKER (255).  raw (device) receives the
+ * bytes of the event's PhotonStream (photon_stream.h:284-308); *num_raw
+ * (host) = kept pulses + num_pixels; a capacity below it is an
+ * MLGPU_ERR_ARGUMENT with the count set.  Synchronises `stream`. */
+int mlgpu_fact_photon_stream(const mlgpu_pulses* pulses, int32_t num_pixels, double trigger_time,
+                             uint8_t* raw, int64_t capacity, int64_t* num_raw, void* stream);
+
 /* Night-sky-background injection (merlict_portal_plenoscope/
  * night_sky_background/Injector.cpp:11-107) into the pipelines `in`: the
  * exposure [start, start + exposure_time) is centred on the mode of the

@@ -254,6 +254,8 @@ SIGNATURES = {
                                        C.c_void_p]),
     "mlgpu_photon_stream": (C.c_int, [C.POINTER(Pulses), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
                                       C.POINTER(C.c_int64), C.c_void_p]),
+    "mlgpu_fact_photon_stream": (C.c_int, [C.POINTER(Pulses), C.c_int32, C.c_double, C.c_void_p, C.c_int64,
+                                           C.POINTER(C.c_int64), C.c_void_p]),
     "mlgpu_inject_nsb": (C.c_int, [C.POINTER(Pipelines), C.c_int32, C.c_void_p, C.POINTER(Nsb), C.POINTER(Pipelines),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_void_p]),
     "mlgpu_nsb_light_rate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_double,

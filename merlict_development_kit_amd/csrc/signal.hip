@@ -487,7 +487,8 @@ __global__ void add_offsets_kernel(const int64_t* __restrict__ a, const int64_t*
 
 // ------------------------------------------------------ photon stream
 
-constexpr uint8_t kNextChannelMarker = 255;  // PhotonStream.h:15
+constexpr uint8_t kNextChannelMarker = 255;  // PhotonStream.h:15; FACT's NEXT_PIXEL_MARKER
+constexpr int32_t kFactSliceOffset = 30;     // merlict_fact/apps/photon_stream.h:10
 
 struct Kept {
     const int16_t* slice;
@@ -506,12 +507,38 @@ __global__ void stream_pulses_kernel(const mlgpu_pulses p, const int16_t* __rest
     if (truth) truth[kscan[k]] = p.truth_id[k];
 }
 
-// the marker after channel s (s < num_sensors - 1)
-__global__ void stream_markers_kernel(const mlgpu_pulses p, const int32_t num_sensors,
+// the marker after channel s (s < num_markers: every channel but the last,
+// or every channel)
+__global__ void stream_markers_kernel(const mlgpu_pulses p, const int32_t num_markers,
                                       const int64_t* __restrict__ kscan, uint8_t* __restrict__ symbols) {
     const int32_t s = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (s >= num_sensors - 1) return;
+    if (s >= num_markers) return;
     symbols[kscan[p.offsets[s + 1]] + s] = kNextChannelMarker;
+}
+
+// FACT's photon stream (merlict_fact/apps/fact_propagation.cpp:34-94): the
+// slice of pulse k relative to the median arrival time of the event, kept
+// within [30, 130), else -1
+__global__ void fact_slices_kernel(const double* __restrict__ t, const int64_t count,
+                                   const double* __restrict__ median, const double trigger_time,
+                                   const double slice_duration, int16_t* __restrict__ slice) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double x = round((t[k] - *median + trigger_time) / slice_duration);
+    int16_t out = -1;
+    if (x >= static_cast<double>(kFactSliceOffset) && x < static_cast<double>(kFactSliceOffset + kNumTimeSlices))
+        out = static_cast<int16_t>(x);
+    slice[k] = out;
+}
+
+__global__ void time_keys_kernel(const double* __restrict__ t, const int64_t count, uint64_t* __restrict__ keys) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < count) keys[k] = time_key(t[k]);
+}
+
+// the median arrival time: the sorted time key at floor(count / 2)
+__global__ void key_at_kernel(const uint64_t* __restrict__ sorted, const int64_t at, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = key_time(sorted[at]);
 }
 
 size_t align_up(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
@@ -789,7 +816,7 @@ hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m,
 
 hipError_t photon_stream(const mlgpu_pulses& p, const int16_t* slice, int32_t num_sensors, int64_t count,
                          uint8_t* symbols, int32_t* truth, int64_t capacity, int64_t* num_symbols, int* too_small,
-                         hipStream_t stream) {
+                         hipStream_t stream, bool trailing_marker) {
     *num_symbols = 0;
     *too_small = 0;
     const size_t m = static_cast<size_t>(count);
@@ -818,8 +845,10 @@ hipError_t photon_stream(const mlgpu_pulses& p, const int16_t* slice, int32_t nu
         e = hipMemcpyAsync(kscan + m, &total, 8, hipMemcpyHostToDevice, stream);
     }
     if (e == hipSuccess) {
-        // PhotonStream::num_symbols_to_represent (PhotonStream.cpp:181-193)
-        const int64_t sym = total + num_sensors > 0 ? total + num_sensors - 1 : 0;
+        // PhotonStream::num_symbols_to_represent (PhotonStream.cpp:181-193);
+        // FACT's stream ends every pixel with a marker
+        const int64_t sym = trailing_marker ? total + num_sensors
+                                            : (total + num_sensors > 0 ? total + num_sensors - 1 : 0);
         *num_symbols = sym;
         if (sym > capacity) *too_small = 1;
     }
@@ -827,13 +856,47 @@ hipError_t photon_stream(const mlgpu_pulses& p, const int16_t* slice, int32_t nu
         if (m > 0)
             hipLaunchKernelGGL(stream_pulses_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0,
                                stream, p, slice, count, num_sensors, kscan, symbols, truth);
-        if (num_sensors > 1)
-            hipLaunchKernelGGL(stream_markers_kernel, dim3(static_cast<uint32_t>((num_sensors + 255) / 256)),
-                               dim3(256), 0, stream, p, num_sensors, kscan, symbols);
+        const int32_t markers = trailing_marker ? num_sensors : num_sensors - 1;
+        if (markers > 0)
+            hipLaunchKernelGGL(stream_markers_kernel, dim3(static_cast<uint32_t>((markers + 255) / 256)),
+                               dim3(256), 0, stream, p, markers, kscan, symbols);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
     }
     (void)hipStreamSynchronize(stream);  // the borrowed blocks are idle when returned
+    return e;
+}
+
+hipError_t fact_slices(const mlgpu_pulses& p, int64_t count, double trigger_time, double slice_duration,
+                       int16_t* slice, hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    const size_t m = static_cast<size_t>(count);
+    size_t sort_bytes = 0;
+    hipError_t e = rocprim::radix_sort_keys(nullptr, sort_bytes, static_cast<uint64_t*>(nullptr),
+                                            static_cast<uint64_t*>(nullptr), m, 0, 64, stream);
+    if (e != hipSuccess) return e;
+    PoolBlock block;
+    e = block.get(2 * align_up(m * 8) + align_up(sort_bytes) + 256);
+    if (e != hipSuccess) return e;
+    char* q = block.as();
+    auto* keys = reinterpret_cast<uint64_t*>(q);
+    auto* sorted = reinterpret_cast<uint64_t*>(q + align_up(m * 8));
+    void* temp = q + 2 * align_up(m * 8);
+    auto* median = reinterpret_cast<double*>(q + 2 * align_up(m * 8) + align_up(sort_bytes));
+    hipLaunchKernelGGL(time_keys_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream,
+                       p.arrival_time, count, keys);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = rocprim::radix_sort_keys(temp, sort_bytes, keys, sorted, m, 0, 64, stream);
+    // estimate_median_arrival_time (fact_propagation.cpp:33-57): the sorted
+    // time at unsigned(count * 0.5)
+    const int64_t half = static_cast<int64_t>(static_cast<uint32_t>(static_cast<double>(count) * 0.5));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(key_at_kernel, dim3(1), dim3(64), 0, stream, sorted, half, median);
+        hipLaunchKernelGGL(fact_slices_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream,
+                           p.arrival_time, count, median, trigger_time, slice_duration, slice);
+        e = hipGetLastError();
+    }
+    (void)hipStreamSynchronize(stream);  // the borrowed block is idle when returned
     return e;
 }
 

@@ -47,6 +47,11 @@ hipError_t inject_nsb(const mlgpu_pipelines& in, int32_t num_sensors, int64_t m,
 // *num_symbols receives their number, *too_small when it exceeds capacity.
 hipError_t photon_stream(const mlgpu_pulses& p, const int16_t* slice, int32_t num_sensors, int64_t count,
                          uint8_t* symbols, int32_t* truth, int64_t capacity, int64_t* num_symbols, int* too_small,
-                         hipStream_t stream);
+                         hipStream_t stream, bool trailing_marker = false);
+
+// FACT's slices of count pulses: round((t - median + trigger_time) /
+// slice_duration) within [30, 130), else -1.  Synchronises.
+hipError_t fact_slices(const mlgpu_pulses& p, int64_t count, double trigger_time, double slice_duration,
+                       int16_t* slice, hipStream_t stream);
 
 }  // namespace mlgpu

@@ -257,6 +257,37 @@ int mlgpu_photon_stream(const mlgpu_pulses* p, const int16_t* slice, int32_t num
     return MLGPU_OK;
 }
 
+int mlgpu_fact_photon_stream(const mlgpu_pulses* p, int32_t num_pixels, double trigger_time, uint8_t* raw,
+                             int64_t capacity, int64_t* num_raw, void* stream) {
+    if (!p || !num_raw || num_pixels < 0 || capacity < 0 || !p->offsets) {
+        set_last_error("null argument or negative size");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    *num_raw = 0;
+    if (num_pixels == 0) return MLGPU_OK;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    int64_t count = 0;
+    hipError_t e = hipMemcpyAsync(&count, p->offsets + num_pixels, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "FACT photon stream");
+    if (count > 0 && !p->arrival_time) { set_last_error("null pulse array"); return MLGPU_ERR_ARGUMENT; }
+    if (!raw) { set_last_error("null raw"); return MLGPU_ERR_ARGUMENT; }
+    mlgpu::PoolBlock slices;
+    e = slices.get(static_cast<size_t>(count > 0 ? count : 1) * sizeof(int16_t));
+    if (e != hipSuccess) return hip_fail(e, "FACT photon stream");
+    // photon_stream.h:16: the slice duration is the float 0.5e-9
+    const double slice_duration = static_cast<double>(0.5e-9f);
+    e = mlgpu::fact_slices(*p, count, trigger_time, slice_duration, slices.as<int16_t>(), st);
+    int too_small = 0;
+    if (e == hipSuccess)
+        e = mlgpu::photon_stream(*p, slices.as<int16_t>(), num_pixels, count, raw, nullptr, capacity, num_raw,
+                                 &too_small, st, true);
+    (void)hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "FACT photon stream");
+    if (too_small) { set_last_error("capacity below the stream's length"); return MLGPU_ERR_ARGUMENT; }
+    return MLGPU_OK;
+}
+
 int mlgpu_nsb_light_rate(const double* flux_x, const double* flux_y, int32_t flux_n, double max_fov_radius,
                          double max_aperture_radius, double* rate) {
     if (!rate) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }

@@ -203,3 +203,60 @@ def read_photon_stream(path):
     channels = np.split(sym, cut)
     channels = [channels[0]] + [c[1:] for c in channels[1:]] if n > 0 else []
     return dur, channels
+
+
+# merlict_fact/apps/photon_stream.h:29-37
+FACT_SLICE_OFFSET = 30
+FACT_PASS_VERSION = 4
+FACT_SIMULATION_EVENT = 1
+
+
+def fact_photon_stream(pulses: Pulses, num_pixels: int = 1440, trigger_time: float = 25e-9,
+                       stream=None) -> torch.Tensor:
+    """The raw bytes of FACT's photon stream of one event (uint8 on the GPU):
+    the pulses' slices relative to the event's median arrival time, pixel
+    by pixel, each pixel ended by 255 (mlgpu_fact_photon_stream)."""
+    dev = pulses.offsets.device
+    cap = len(pulses.arrival_time) + num_pixels
+    raw = torch.empty(max(cap, 1), dtype=torch.uint8, device=dev)
+    k = C.c_int64()
+    N.check(N.lib().mlgpu_fact_photon_stream(C.byref(pulses.c_struct()), num_pixels, trigger_time,
+                                             C.c_void_p(raw.data_ptr()), cap, C.byref(k), _st(stream, dev)),
+            "mlgpu_fact_photon_stream")
+    return raw[:k.value]
+
+
+def write_fact_simulation_event(fout, raw, run: int, event: int, reuse: int = 0, zd: float = 0.0,
+                                az: float = 0.0) -> None:
+    """append_SimulationEvent_to_file (merlict_fact/apps/photon_stream.h:398-406):
+    descriptor 'phs', pass version, event type; run, event, reuse; zenith
+    distance and azimuth; the stream's length, its bytes and no saturated
+    pixels."""
+    r = raw.cpu().numpy() if isinstance(raw, torch.Tensor) else np.asarray(raw, np.uint8)
+    fout.write(b"phs" + bytes([FACT_PASS_VERSION, FACT_SIMULATION_EVENT]))
+    fout.write(np.array([run, event, reuse], "<u4").tobytes())
+    fout.write(np.array([zd, az], "<f4").tobytes())
+    fout.write(np.array([len(r)], "<u4").tobytes())
+    fout.write(np.ascontiguousarray(r, np.uint8).tobytes())
+    fout.write(np.array([0], "<u2").tobytes())
+
+
+def read_fact_simulation_events(path) -> list:
+    """read_SimulationEvent_from_file (merlict_fact/apps/photon_stream.h:386-396)
+    over a whole file: one dict per event."""
+    raw = Path(path).read_bytes()
+    events, pos = [], 0
+    while pos < len(raw):
+        if raw[pos:pos + 3] != b"phs" or raw[pos + 4] != FACT_SIMULATION_EVENT:
+            raise ValueError("not a FACT simulation event")
+        run, event, reuse = (int(v) for v in np.frombuffer(raw[pos + 5:pos + 17], "<u4"))
+        zd, az = (float(v) for v in np.frombuffer(raw[pos + 17:pos + 25], "<f4"))
+        n = int(np.frombuffer(raw[pos + 25:pos + 29], "<u4")[0])
+        stream = np.frombuffer(raw[pos + 29:pos + 29 + n], np.uint8).copy()
+        sat = int(np.frombuffer(raw[pos + 29 + n:pos + 31 + n], "<u2")[0])
+        events.append({"pass_version": raw[pos + 3], "run": run, "event": event, "reuse": reuse, "zd": zd,
+                       "az": az, "raw": stream})
+        # append_PhotonStream_to_file writes `sat` bytes of the saturated pixels'
+        # list (photon_stream.h:336-340)
+        pos += 31 + n + sat
+    return events

@@ -1772,3 +1772,40 @@ int64_t oracle_render_pinhole(const oracle_scene* s, int32_t world, const mlgpu_
     }
     return errors;
 }
+
+/* ------------------------------------------------- FACT photon stream */
+static int cmp_double(const void* a, const void* b) {
+    const double x = *(const double*)a, y = *(const double*)b;
+    return x < y ? -1 : (x > y);
+}
+
+/* electric_pipelines_to_photon_stream (merlict_fact/apps/fact_propagation.cpp:
+   34-94): the pulses' slices relative to the event's median arrival time,
+   pixel by pixel, each pixel ended by 255; returns the length, -1 when
+   capacity is too small */
+int64_t oracle_fact_photon_stream(const int64_t* offsets, const double* arrival_time, int32_t num_pixels,
+                                  double trigger_time, uint8_t* raw, int64_t capacity) {
+    const int64_t n = offsets[num_pixels];
+    double median = NAN;
+    if (n > 0) {
+        double* t = (double*)malloc(sizeof(double) * (size_t)n);
+        memcpy(t, arrival_time, sizeof(double) * (size_t)n);
+        qsort(t, (size_t)n, sizeof(double), cmp_double);
+        median = t[(unsigned)((double)n * 0.5)];
+        free(t);
+    }
+    const double dur = (double)0.5e-9f;
+    int64_t k = 0;
+    for (int32_t s = 0; s < num_pixels; s++) {
+        for (int64_t j = offsets[s]; j < offsets[s + 1]; j++) {
+            const double x = round((arrival_time[j] - median + trigger_time) / dur);
+            if (x >= 30.0 && x < 130.0) {
+                if (k >= capacity) return -1;
+                raw[k++] = (uint8_t)(int)x;
+            }
+        }
+        if (k >= capacity) return -1;
+        raw[k++] = 255;
+    }
+    return k;
+}

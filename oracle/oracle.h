@@ -154,6 +154,10 @@ int oracle_lixel_statistics(const mlgpu_calibration_photon* rec, int64_t n, int6
    the pixels with a 533 nm lookup outside a table, -1 on bad arguments. */
 int64_t oracle_render_pinhole(const oracle_scene* s, int32_t world, const mlgpu_pinhole_camera* cam,
                               const mlgpu_render_config* cfg, const float* colors, uint64_t seed, float* image);
+/* FACT's photon stream of an event (fact_propagation.cpp:34-94); the length
+   or -1 when capacity is too small. */
+int64_t oracle_fact_photon_stream(const int64_t* offsets, const double* arrival_time, int32_t num_pixels,
+                                  double trigger_time, uint8_t* raw, int64_t capacity);
 /* night_sky_background::Light::rate (Light.cpp:9-28); -1 for a bad table */
 double oracle_nsb_light_rate(const double* flux_x, const double* flux_y, int32_t flux_n, double max_fov_radius,
                              double max_aperture_radius);

@@ -616,6 +616,35 @@ def ref_lixelstats(records: np.ndarray, num_lixel: int, num_blocks: int, photons
         return o.read_bytes()
 
 
+def fact_photon_stream(offsets, arrival_time, num_pixels, trigger_time=25e-9) -> np.ndarray:
+    """FACT's photon stream of one event (oracle_fact_photon_stream)."""
+    off = np.ascontiguousarray(offsets, np.int64)
+    t = np.ascontiguousarray(arrival_time, np.float64)
+    cap = len(t) + num_pixels
+    raw = np.zeros(max(cap, 1), np.uint8)
+    L = lib()
+    L.oracle_fact_photon_stream.restype = C.c_int64
+    L.oracle_fact_photon_stream.argtypes = [C.POINTER(C.c_int64), _P_F64, C.c_int32, C.c_double, C.c_void_p,
+                                            C.c_int64]
+    k = L.oracle_fact_photon_stream(off.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(t, _P_F64), num_pixels,
+                                    trigger_time, raw.ctypes.data, cap)
+    return raw[:k]
+
+
+def ref_read_fact_events(path) -> list:
+    """The reference's read_SimulationEvent_from_file over a file (harness
+    `factphs`): per event (descriptor valid, run, event, reuse, zd, az,
+    number_of_photons, raw bytes)."""
+    out = _run_ref(["factphs", path])
+    events = []
+    for line in out.strip().splitlines():
+        f = line.split()
+        events.append({"valid": int(f[0]), "pass_version": int(f[1]), "event_type": int(f[2]), "run": int(f[3]),
+                       "event": int(f[4]), "reuse": int(f[5]), "zd": float(f[6]), "az": float(f[7]),
+                       "number_of_photons": int(f[8]), "raw": np.array([int(v) for v in f[9:]], np.uint8)})
+    return events
+
+
 def ref_render(desc, position, rotation, fov, cols, rows, global_illumination, light, sky, frame_colors,
                seed, timing: bool = False):
     """The reference's PinHoleCamera + Tracer per pixel (harness `render`,

@@ -41,6 +41,7 @@
 #include "merlict_portal_plenoscope/calibration/LixelStatisticsFiller.h"
 #include "merlict_visual/PinHoleCamera.h"
 #include "merlict_visual/Tracer.h"
+#include "merlict_fact/apps/photon_stream.h"
 namespace sp = signal_processing;
 
 namespace ml = merlict;
@@ -766,6 +767,26 @@ static int cmd_render(int argc, char** argv) {
     return 0;
 }
 
+// factphs <file>: the reference's reader of FACT simulation events; one
+// line per event: valid, pass version, type, run, event, reuse, zd, az,
+// number_of_photons, raw bytes
+static int cmd_factphs(int argc, char** argv) {
+    if (argc != 3) return 2;
+    std::ifstream in(argv[2], std::ios::binary);
+    in.seekg(0, std::ios::end);
+    const std::streamoff size = in.tellg();
+    in.seekg(0, std::ios::beg);
+    while (in.tellg() < size) {
+        photon_stream::SimulationEvent e = photon_stream::read_SimulationEvent_from_file(in);
+        std::printf("%d %u %u %u %u %u %.9g %.9g %u", e.descriptor.is_valid() ? 1 : 0, e.descriptor.pass_version,
+                    e.descriptor.event_type, e.id.run, e.id.event, e.id.reuse, e.pointing.zd, e.pointing.az,
+                    e.photon_stream.number_of_photons());
+        for (uint8_t b : e.photon_stream.raw) std::printf(" %u", b);
+        std::printf("\n");
+    }
+    return 0;
+}
+
 static int cmd_lensmaker(int argc, char** argv) {
     if (argc < 5) return 2;
     ml::lens_maker::Config c;
@@ -884,6 +905,7 @@ int main(int argc, char** argv) {
         else if (c == "nsb") rc = cmd_nsb(argc, argv);
         else if (c == "lixelstats") rc = cmd_lixelstats(argc, argv);
         else if (c == "render") rc = cmd_render(argc, argv);
+        else if (c == "factphs") rc = cmd_factphs(argc, argv);
         if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", c.c_str());
         return rc;
     } catch (const std::exception& e) {

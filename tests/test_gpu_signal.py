@@ -213,3 +213,26 @@ def test_gpu_photon_stream_edges(cuda):
     e = Pulses(T([0, 0], torch.int64), T([], torch.float64), T([], torch.int32))
     sym, truth = photon_stream(e, T([], torch.int16), 1)
     assert len(sym) == 0 and len(truth) == 0
+
+
+def test_gpu_fact_photon_stream_matches_oracle(cuda):
+    """FACT's photon stream of the pulses of 1e6 propagated FACT-like photons
+    (converter with dark counts and crosstalk over 100 ns) and of a few
+    hand-placed pulses: the GPU's bytes equal the oracle's."""
+    from merlict_development_kit_amd.signal_chain import Pulses, fact_photon_stream
+    gs = GpuScene(SceneDescription.fact(light_guides=True))
+    ph = sources.fact_bunch().generate(1_000_000, seed=7, device=cuda)
+    res = gs.propagate(ph, PropagationConfig(5, seed=8))
+    pipes = sensor_pipelines(res.sensor_id, res.error, res.time_of_flight, ph.wavelength, 1440)
+    qx, qy = np.array([200e-9, 400e-9, 1200e-9]), np.array([0.3, 0.35, 0.1])
+    pulses = photoelectric_convert(pipes, 1440, qx, qy, 5e6, 0.1, 100e-9, seed=9)
+    raw = fact_photon_stream(pulses, 1440).cpu().numpy()
+    ref = O.fact_photon_stream(pulses.offsets.cpu().numpy(), pulses.arrival_time.cpu().numpy(), 1440)
+    np.testing.assert_array_equal(raw, ref)
+    assert (raw == 255).sum() == 1440 and len(raw) > 1440 + 10_000
+    T = lambda x, dt: torch.tensor(x, dtype=dt, device=cuda)
+    dur = float(np.float32(0.5e-9))
+    t0 = 40e-9
+    times = [t0 + k * dur for k in (-1, 0, 1, 7)] + [t0 - 30 * dur, t0 + 90 * dur, t0 - 19 * dur, t0 + 79 * dur]
+    p = Pulses(T([0, 4, 4, 8], torch.int64), T(times, torch.float64), T(list(range(8)), torch.int32))
+    assert fact_photon_stream(p, 3).cpu().tolist() == [48, 49, 50, 56, 255, 255, 30, 128, 255]

@@ -821,3 +821,49 @@ def test_photon_stream_format(golden, tmp_path):
     assert dur == np.float32(0.5e-9) and len(channels) == ns
     for s_ in range(ns):
         np.testing.assert_array_equal(channels[s_], sl[off[s_]:off[s_ + 1]])
+
+
+def test_oracle_fact_photon_stream_laws():
+    """FACT's photon stream (fact_propagation.cpp:34-94): the median pulse
+    lands in slice round(25 ns / 0.5 ns) = 50, a pulse k slices later in
+    50 + k; slices outside [30, 130) are dropped; every pixel ends with 255."""
+    dur = float(np.float32(0.5e-9))
+    t0 = 123.4e-9
+    # pixel 0: the median and its neighbours; pixel 1: empty; pixel 2: far pulses
+    times = [t0 + k * dur for k in (-1, 0, 1, 7)] + [t0 - 30 * dur, t0 + 90 * dur, t0 - 19 * dur, t0 + 79 * dur]
+    off = np.array([0, 4, 4, 8], np.int64)
+    raw = O.fact_photon_stream(off, np.array(times), 3)
+    # sorted offsets from t0 (in slices): -30 -19 -1 0 1 7 79 90; the median
+    # at index floor(8 / 2) = 4 is t0 + 1 slice: t0 - 30 slices lands in 19
+    # and t0 + 90 in 139, both dropped; t0 - 19 in 30, t0 + 79 in 128
+    assert raw.tolist() == [48, 49, 50, 56, 255, 255, 30, 128, 255]
+    assert len(O.fact_photon_stream(np.zeros(4, np.int64), np.zeros(0), 3)) == 3
+
+
+def test_fact_simulation_event_records(tmp_path):
+    """write_fact_simulation_event: the reference's own reader
+    (photon_stream.h read_SimulationEvent_from_file) reads the records back
+    with a valid descriptor, the ids, pointing and stream; so does ours."""
+    from merlict_development_kit_amd import signal_chain as SC
+    if not O.ref_available():
+        pytest.skip("oracle/_ref is not built")
+    rng = np.random.default_rng(2)
+    streams = []
+    p = tmp_path / "events.phs"
+    with open(p, "wb") as f:
+        for e in range(3):
+            raw = np.concatenate([np.append(rng.integers(30, 130, rng.integers(0, 4)), 255) for _ in range(1440)])
+            raw = raw.astype(np.uint8)
+            streams.append(raw)
+            SC.write_fact_simulation_event(f, raw, run=7, event=e + 1, reuse=0, zd=0.5, az=1.25)
+    ref = O.ref_read_fact_events(p)
+    mine = SC.read_fact_simulation_events(p)
+    assert len(ref) == len(mine) == 3
+    for e in range(3):
+        r = ref[e]
+        assert r["valid"] == 1 and r["pass_version"] == 4 and r["event_type"] == 1
+        assert (r["run"], r["event"], r["reuse"], r["zd"], r["az"]) == (7, e + 1, 0, 0.5, 1.25)
+        assert r["number_of_photons"] == len(streams[e]) - 1440
+        np.testing.assert_array_equal(r["raw"], streams[e])
+        np.testing.assert_array_equal(mine[e]["raw"], streams[e])
+        assert mine[e]["event"] == e + 1 and mine[e]["zd"] == 0.5
