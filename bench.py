@@ -110,37 +110,46 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
 
     plain = plain_mean(range(accuracy_passes))
     again = plain_mean(range(1000, 1000 + accuracy_passes))
-    acc = binning.empty(dev)
-    for s in range(accuracy_passes):
-        SG.propagate_hybrid(scene, sur, dom, cells, photons, binning, PropagationConfig(5, seed=s),
-                            max_entropy=max_entropy, seed=s, counts=acc)
-    hy = acc.cpu().numpy().astype(np.float64) / accuracy_passes
-
-    counts = binning.empty(dev)
-    fallback = 0
-    for step in range(warmup + steps):
-        if step == warmup:
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            fallback = 0
-        _, f = SG.propagate_hybrid(scene, sur, dom, cells, photons, binning, PropagationConfig(5, seed=step),
-                                   max_entropy=max_entropy, seed=step, counts=counts)
-        fallback += f
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
 
     def l1(a, b):
         return float(np.abs(a - b).sum() / b.sum())
 
-    value = n * steps / wall
+    def measure():
+        """histogram accuracy over the accuracy passes, then the timed passes"""
+        acc = binning.empty(dev)
+        for s in range(accuracy_passes):
+            SG.propagate_hybrid(scene, sur, dom, cells, photons, binning, PropagationConfig(5, seed=s),
+                                max_entropy=max_entropy, seed=s, counts=acc)
+        hy = acc.cpu().numpy().astype(np.float64) / accuracy_passes
+        counts = binning.empty(dev)
+        fallback = 0
+        for step in range(warmup + steps):
+            if step == warmup:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fallback = 0
+            _, f = SG.propagate_hybrid(scene, sur, dom, cells, photons, binning, PropagationConfig(5, seed=step),
+                                       max_entropy=max_entropy, seed=step, counts=counts)
+            fallback += f
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return hy, n * steps / wall, wall / steps * 1e3, fallback / (n * steps)
+
+    flops = 2 * 212_480  # multiply-adds of the network per photon, twice
+    sur.inference = "bf16"
+    hy, value, ms, fb = measure()
     cell_l1 = l1(hy.sum(axis=1), plain.sum(axis=1))
+    sur.inference = "fp8"
+    hy8, value8, ms8, fb8 = measure()
+    cell_l1_8 = l1(hy8.sum(axis=1), plain.sum(axis=1))
+    sur.inference = "bf16"
     trained = (sur, dom, cells)
     return trained, {
         "value": value,
         "unit": "photons/s",
-        "ms_per_step": wall / steps * 1e3,
+        "ms_per_step": ms,
         "max_entropy": max_entropy,
-        "fallback_fraction": fallback / (n * steps),
+        "fallback_fraction": fb,
         "accuracy_passes": accuracy_passes,
         "pixel_l1": cell_l1,
         "pixel_time_l1": l1(hy, plain),
@@ -150,8 +159,21 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
         "trained_photons": train_photons,
         "train_seconds": train_s,
         "network": "MLP 5-256-256-256-256-57, bf16 on MFMA 32x32x16, fp32 accumulation",
-        "mfma_tflops": value * 2 * 212_480 / 1e12,
-        "mfma_frac_of_bf16_dense_peak": value * 2 * 212_480 / 2.5e15,
+        "mfma_tflops": value * flops / 1e12,
+        "mfma_frac_of_bf16_dense_peak": value * flops / 2.5e15,
+        "fp8": {
+            "network": "the same trained network, layers 1-4 e4m3 on MFMA 32x32x64 (per-row power-of-two weight "
+                       "scales, e4m3 activations), layer 0 bf16, fp32 accumulation",
+            "value": value8,
+            "unit": "photons/s",
+            "ms_per_step": ms8,
+            "fallback_fraction": fb8,
+            "pixel_l1": cell_l1_8,
+            "pixel_time_l1": l1(hy8, plain),
+            "meets_1pct_l1": cell_l1_8 <= 0.01,
+            "mfma_tflops": value8 * flops / 1e12,
+            "mfma_frac_of_fp8_dense_peak": value8 * flops / 5.0e15,
+        },
     }
 
 
@@ -680,6 +702,7 @@ def main() -> int:
         trained, line["hybrid"] = hybrid_leg(desc, scene, photons, args.steps, args.warmup,
                                              args.hybrid_train_photons, args.hybrid_max_entropy)
         line["hybrid"]["speedup_vs_plain"] = line["hybrid"]["value"] / value
+        line["hybrid"]["fp8"]["speedup_vs_plain"] = line["hybrid"]["fp8"]["value"] / value
         if not args.no_stream:
             line["hybrid_stream"] = hybrid_stream_leg(scene, trained, dev)
     if not args.no_signal:

@@ -422,6 +422,18 @@ void mlgpu_surrogate_destroy(mlgpu_surrogate* s);
 int mlgpu_surrogate_get_params(const mlgpu_surrogate* s, float* dst, void* stream);
 /* also resets the optimiser state */
 int mlgpu_surrogate_set_params(mlgpu_surrogate* s, const float* src, void* stream);
+/* Inference precision of mlgpu_surrogate_forward and
+ * mlgpu_propagate_hybrid (training stays bf16 with fp32 masters):
+ * MLGPU_SURROGATE_BF16 (the default) or MLGPU_SURROGATE_FP8, where layers
+ * 1.. run on the fp8 matrix instructions (e4m3 weights, each row scaled by
+ * a power of two to the e4m3 range; e4m3 ReLU activations clamped to 448;
+ * fp32 accumulation) and layer 0 stays bf16.  The fp8 weight copies are
+ * refreshed with the bf16 ones (create, set_params, train_step).  No
+ * counterpart in the reference. */
+#define MLGPU_SURROGATE_BF16 0
+#define MLGPU_SURROGATE_FP8 1
+int mlgpu_surrogate_set_inference(mlgpu_surrogate* s, int32_t precision);
+int32_t mlgpu_surrogate_get_inference(const mlgpu_surrogate* s);
 /* out: n x 57 fp32 */
 int mlgpu_surrogate_forward(mlgpu_surrogate* s, const float* features, int64_t n,
                             float* out, void* stream);

@@ -226,6 +226,8 @@ SIGNATURES = {
     "mlgpu_surrogate_destroy": (None, [C.c_void_p]),
     "mlgpu_surrogate_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mlgpu_surrogate_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlgpu_surrogate_set_inference": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlgpu_surrogate_get_inference": (C.c_int32, [C.c_void_p]),
     "mlgpu_surrogate_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "mlgpu_surrogate_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),

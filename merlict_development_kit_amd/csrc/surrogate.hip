@@ -896,6 +896,342 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
     }
 }
 
+// ------------------------------------------------- fp8 fused inference
+// The same tile, waves and epilogues as fused_kernel with layers 1.. on
+// v_mfma_scale_f32_32x32x64_f8f6f4: weights and activations e4m3 (OCP),
+// fp32 accumulation.  Layer 0 (K = 16) stays bf16.  The activations are
+// kept in LDS as e4m3, photon-major (kTileP rows of kQStride bytes): the
+// ReLU outputs clamped to 448, the largest e4m3 value.  Row r of a
+// weight matrix is stored as W[r] * 2^s (quantize_weights_kernel) and the
+// instruction's A scale 2^-s (an e8m0 byte, per lane: every lane holds one
+// row) undoes it in the product, so the accumulators hold the plain sums.
+// Lane l of the instruction supplies row (or photon) l % 32 and 32 bytes
+// of K; the K order inside a k-step of 64 is the same for A and B, which
+// is all the sum needs: byte j of lane l is K position 64 ks + 32 (l / 32) + j.
+// The activations are stored in a permuted order (act_position) that puts
+// the 16 outputs a lane holds for one photon in 16 consecutive bytes, one
+// 16-byte LDS write; the weights' K positions follow the same permutation.
+// The biases live in LDS; the outputs are stored with an odd row stride
+// (kO8Stride floats) so that the draws' per-photon reads do not collide in
+// the LDS banks.
+typedef int i32x8 __attribute__((__vector_size__(8 * sizeof(int))));
+constexpr int kQStride = kHidden + 16;
+constexpr int kQSteps = kHidden / 64;  // k-steps of the fp8 layers
+constexpr int kO8Stride = kOut + 8;    // 65
+constexpr int kBiasFloats = (kLayers - 1) * kHidden + kOutPad;
+constexpr float kFp8Max = 448.0f;
+constexpr int kFp8Tile = 128;  // photons per block of the fp8 kernel (128 or 256)
+static_assert(kTileP * kO8Stride * 4 <= kTileP * kQStride, "the outputs reuse the activations' LDS");
+static_assert(kO8Stride % 2 == 1, "odd output stride");
+static_assert(kQSteps == 4, "the fp8 k-loop is written out for four k-steps");
+
+// Stored K position of input feature c of layers 1..: inside each block of
+// 32, feature 8 a + 4 h + b (the accumulator 4 a + b of the lanes of half
+// h) goes to 16 h + 4 a + b.
+__device__ __forceinline__ int act_position(int c) {
+    return (c & ~31) | (((c >> 2) & 1) << 4) | (((c >> 3) & 3) << 2) | (c & 3);
+}
+
+// the largest s with m 2^s <= 448 = 1.75 2^8, for m > 0 (0 otherwise),
+// within [-100, 100]
+__device__ __forceinline__ int fp8_row_shift(float m) {
+    if (!(m > 0.0f)) return 0;
+    int e;
+    const float f = frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+    const int s = (2.0f * f <= 1.75f ? 8 : 7) - (e - 1);
+    return s < -100 ? -100 : (s > 100 ? 100 : s);
+}
+
+// four floats to four e4m3 bytes (round to nearest even), the first in the
+// lowest byte
+__device__ __forceinline__ uint32_t pack_fp8x4(float x0, float x1, float x2, float x3) {
+    int p = __builtin_amdgcn_cvt_pk_fp8_f32(x0, x1, 0, false);
+    p = __builtin_amdgcn_cvt_pk_fp8_f32(x2, x3, p, true);
+    return static_cast<uint32_t>(p);
+}
+
+// ReLU clamped to the e4m3 range
+__device__ __forceinline__ float act_fp8(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, kFp8Max); }
+
+// C + sum over a k-step of (A 2^(scale_a - 127)) B; both operands e4m3
+__device__ __forceinline__ f32x16 mfma_fp8(const i32x8& a, const i32x8& b, const f32x16& c, int scale_a) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, scale_a, 0, 127);
+}
+
+// One wave per row of layers 1..: the row's largest magnitude, its shift s
+// and the row's e4m3 values W[r][c] 2^s at their fragment positions
+// (element ((ot * kQSteps + ks) * 64 + l) * 32 + j holds W[32 ot + l % 32][c]
+// with act_position(c) = 64 ks + 32 (l / 32) + j); padded rows are zero.
+__global__ __launch_bounds__(256) void quantize_weights_kernel(const float* __restrict__ params,
+                                                               const PaddedWeights pw) {
+    int row = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    int l = 1;
+    while (l < kLayers && row >= kPadOut[l]) {
+        row -= kPadOut[l];
+        l++;
+    }
+    if (l >= kLayers) return;
+    static_assert(kHidden == 4 * 64, "one wave quantizes a row of 256 in fours");
+    const int c = 4 * lane;
+    float v[4];
+    float m = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        v[j] = row < kLayerOut[l] ? params[weight_offset(l) + static_cast<int64_t>(row) * kLayerIn[l] + c + j] : 0.0f;
+        m = fmaxf(m, fabsf(v[j]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    const int s = fp8_row_shift(m);
+    const uint32_t q = pack_fp8x4(ldexpf(v[0], s), ldexpf(v[1], s), ldexpf(v[2], s), ldexpf(v[3], s));
+    const int pc = act_position(c);  // c % 4 == 0: the four features stay consecutive
+    const int ot = row / 32, ks = pc / 64, fl = row % 32 + 32 * ((pc % 64) / 32), j = pc % 32;
+    *reinterpret_cast<uint32_t*>(pw.wq[l] + ((ot * kQSteps + ks) * 64 + fl) * 32 + j) = q;
+    if (lane == 0) pw.wscale[l][row] = static_cast<uint8_t>(127 - s);
+}
+
+// TP photons per block of 2 TP threads: waves w and w + 4 share neuron
+// group w % 4 (the same weight fragments, close in time) on photon halves
+// w / 4.
+template <int TP>
+__global__ __launch_bounds__(2 * TP, 256 / TP) void fused_fp8_kernel(const FusedArgs a, const int64_t n) {
+    constexpr int kThreads = 2 * TP;
+    __shared__ __attribute__((aligned(16))) uint8_t Q[TP * kQStride];
+    __shared__ __attribute__((aligned(16))) uint16_t F16[TP * kFStride];
+    __shared__ float F[2][TP * kIn];  // features of the current and the next tile
+    __shared__ __attribute__((aligned(16))) float Bs[kBiasFloats];  // the biases, layer by layer
+    float* O = reinterpret_cast<float*>(Q);
+    const int t = static_cast<int>(threadIdx.x), lane = t & 63, w = t >> 6;
+    const int lr = lane & 31, lk = (lane >> 5) * 8, lq = (lane >> 5) * 4, lh = (lane >> 5) * 32;
+    const int ng = w & 3, ph = w >> 2;  // neuron group, photon half
+    __shared__ DrawConsts dk_lds;
+    __shared__ FeatureConsts fc_lds;
+    if (t == 0) {
+        dk_lds = draw_consts(a.s);
+        fc_lds = feature_consts(a.s.dom);
+    }
+    for (int e = t; e < kBiasFloats; e += kThreads) {
+        const int l = e / kHidden < kLayers - 1 ? e / kHidden : kLayers - 1;
+        Bs[e] = a.b[l][e - l * kHidden];
+    }
+    __syncthreads();
+    const f32x16 zero = {};
+    // features of the tile at `base` into F[buf] and F16, by one thread per photon
+    auto features = [&](const int64_t base, const int p, float* Fb) {
+        const int64_t i = base + p;
+        float* f = Fb + p * kIn;
+        if (i < n) {
+            if (a.features) {
+#pragma unroll
+                for (int j = 0; j < kIn; j++) f[j] = a.features[i * kIn + j];
+            } else {
+                const FeatureConsts fc = fc_lds;
+                photon_features(fc, a.ph, i, f);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kIn; j++) f[j] = 0.0f;
+        }
+        uint32_t* row = reinterpret_cast<uint32_t*>(F16 + p * kFStride);
+        row[0] = pack_bf16(f[0], f[1]);
+        row[1] = pack_bf16(f[2], f[3]);
+        row[2] = pack_bf16(f[4], 0.0f);
+#pragma unroll
+        for (int j = 3; j < 8; j++) row[j] = 0u;
+    };
+    // Waves 2 and 3 compute a tile's features while waves 0 and 1 draw the
+    // previous tile's photons (F double-buffered; F16 is free once layer 0
+    // has read it).
+    const int64_t step = static_cast<int64_t>(gridDim.x) * TP;
+    int buf = 0;
+    if (t >= TP && static_cast<int64_t>(blockIdx.x) * TP < n)
+        features(static_cast<int64_t>(blockIdx.x) * TP, t - TP, F[0]);
+    __syncthreads();
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * TP; base < n; base += step, buf ^= 1) {
+        bf16x8 w00, w01;
+        {
+            const bf16x8* P = reinterpret_cast<const bf16x8*>(a.wf[0]) + (2 * ng) * 64 + lane;
+            w00 = P[0];
+            w01 = P[64];
+        }
+        // layer 0 in bf16 (one k-step of 16)
+        f32x16 acc[2][4];
+        {
+            bf16x8 hv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) hv[q] = *reinterpret_cast<const bf16x8*>(F16 + ((4 * ph + q) * 32 + lr) * kFStride + lk);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                acc[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w00, hv[q], zero, 0, 0, 0);
+                acc[1][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w01, hv[q], zero, 0, 0, 0);
+            }
+        }
+        // Layer l's outputs (acc, without the bias) become layer l + 1's
+        // inputs in Q, then layer l + 1 runs.  The next layer's first weight
+        // fragments and its scales load before the barriers.
+        i32x8 wc0, wc1;
+        int sc0, sc1;
+#pragma unroll 1
+        for (int l = 0; l < kLayers - 1; l++) {
+            const int nl = l + 1;
+            if (nl < kLayers - 1) {
+                const i32x8* P = reinterpret_cast<const i32x8*>(a.wq[nl]) + (2 * ng) * kQSteps * 64 + lane;
+                wc0 = P[0];
+                wc1 = P[kQSteps * 64];
+                sc0 = a.wscale[nl][ng * 64 + lr];
+                sc1 = a.wscale[nl][ng * 64 + 32 + lr];
+            } else {
+                const int ot = w & 1;
+                wc0 = reinterpret_cast<const i32x8*>(a.wq[nl])[ot * kQSteps * 64 + lane];
+                wc1 = wc0;
+                sc0 = a.wscale[nl][ot * 32 + lr];
+                sc1 = sc0;
+            }
+            __syncthreads();
+            // the 16 outputs of a lane for photon tile pt and output tile ot
+            // as 16 consecutive e4m3 bytes (act_position)
+            const float* bias_l = Bs + l * kHidden + ng * 64;
+#pragma unroll
+            for (int ot = 0; ot < 2; ot++) {
+#pragma unroll
+                for (int pt = 0; pt < 4; pt++) {
+                    uint4 v;
+                    uint32_t* vv = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+                    for (int g = 0; g < 4; g++) {
+                        const float4 bias = *reinterpret_cast<const float4*>(bias_l + ot * 32 + g * 8 + lq);
+                        vv[g] = pack_fp8x4(act_fp8(acc[ot][pt][4 * g + 0] + bias.x), act_fp8(acc[ot][pt][4 * g + 1] + bias.y),
+                                           act_fp8(acc[ot][pt][4 * g + 2] + bias.z), act_fp8(acc[ot][pt][4 * g + 3] + bias.w));
+                    }
+                    *reinterpret_cast<uint4*>(Q + ((4 * ph + pt) * 32 + lr) * kQStride + ng * 64 + ot * 32 + (lane >> 5) * 16) = v;
+                }
+            }
+            __syncthreads();
+            if (nl == kLayers - 1) break;
+            const i32x8* W0 = reinterpret_cast<const i32x8*>(a.wq[nl]) + (2 * ng) * kQSteps * 64 + lane;
+            const i32x8* W1 = W0 + kQSteps * 64;
+            // activation fragments of half h (photon tiles 2 h, 2 h + 1) of k-step ks
+            auto load_h = [&](const int ks, const int h, i32x8 (&hv)[2]) {
+#pragma unroll
+                for (int q = 0; q < 2; q++)
+                    hv[q] = *reinterpret_cast<const i32x8*>(Q + ((4 * ph + 2 * h + q) * 32 + lr) * kQStride + ks * 64 + lh);
+            };
+            auto mma = [&](const i32x8& x0, const i32x8& x1, const i32x8 (&hv)[2], const int h, const bool first) {
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const int pt = 2 * h + q;
+                    acc[0][pt] = mfma_fp8(x0, hv[q], first ? zero : acc[0][pt], sc0);
+                    acc[1][pt] = mfma_fp8(x1, hv[q], first ? zero : acc[1][pt], sc1);
+                }
+            };
+            // the next half k-step's activations and the next k-step's
+            // weights are in flight while the current instructions run
+            i32x8 ha[2], hb[2], wn0, wn1;
+            load_h(0, 0, ha);
+            wn0 = W0[64];
+            wn1 = W1[64];
+            load_h(0, 1, hb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wc0, wc1, ha, 0, true);
+            load_h(1, 0, ha);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wc0, wc1, hb, 1, true);
+            wc0 = W0[2 * 64];
+            wc1 = W1[2 * 64];
+            load_h(1, 1, hb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wn0, wn1, ha, 0, false);
+            load_h(2, 0, ha);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wn0, wn1, hb, 1, false);
+            wn0 = W0[3 * 64];
+            wn1 = W1[3 * 64];
+            load_h(2, 1, hb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wc0, wc1, ha, 0, false);
+            load_h(3, 0, ha);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wc0, wc1, hb, 1, false);
+            load_h(3, 1, hb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wn0, wn1, ha, 0, false);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wn0, wn1, hb, 1, false);
+        }
+        {
+            // output layer: wave w, output tile w % 2 of photon tiles 2 (w / 2), + 1
+            const int ot = w & 1, pt0 = (w >> 1) * 2;
+            const i32x8* W = reinterpret_cast<const i32x8*>(a.wq[kLayers - 1]) + ot * kQSteps * 64 + lane;
+            f32x16 bt;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const float4 bias = *reinterpret_cast<const float4*>(Bs + (kLayers - 1) * kHidden + ot * 32 + g * 8 + lq);
+                bt[4 * g + 0] = bias.x;
+                bt[4 * g + 1] = bias.y;
+                bt[4 * g + 2] = bias.z;
+                bt[4 * g + 3] = bias.w;
+            }
+            f32x16 o2[2];
+            auto load_o = [&](const int ks, i32x8 (&hv)[2]) {
+#pragma unroll
+                for (int p = 0; p < 2; p++)
+                    hv[p] = *reinterpret_cast<const i32x8*>(Q + ((pt0 + p) * 32 + lr) * kQStride + ks * 64 + lh);
+            };
+            auto mma = [&](const i32x8& wv, const i32x8 (&hv)[2], const bool first) {
+#pragma unroll
+                for (int p = 0; p < 2; p++) o2[p] = mfma_fp8(wv, hv[p], first ? bt : o2[p], sc0);
+            };
+            i32x8 ha[2], hb[2];
+            i32x8 wn = W[64];
+            load_o(0, ha);
+            load_o(1, hb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wc0, ha, true);
+            wc0 = W[2 * 64];
+            load_o(2, ha);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wn, hb, false);
+            wn = W[3 * 64];
+            load_o(3, hb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wc0, ha, false);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(wn, hb, false);
+            __syncthreads();
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int col = ot * 32 + g * 8 + lq;
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    const int row = (pt0 + p) * 32 + lr;
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        if (col + e < kOut) O[row * kO8Stride + col + e] = o2[p][4 * g + e];
+                }
+            }
+            __syncthreads();
+        }
+        if (a.out) {
+            for (int e = t; e < TP * kOut; e += kThreads) {
+                const int row = e / kOut, col = e % kOut;
+                if (base + row < n) a.out[(base + row) * kOut + col] = O[row * kO8Stride + col];
+            }
+        }
+        if (t < TP) {
+            if (a.sample) {
+                const int64_t i = base + t;
+                const DrawConsts dk = dk_lds;
+                const bool fallback = i < n && sample_one(a.s, dk, F[buf] + t * kIn, O + t * kO8Stride, i);
+                append_fallback(a.s, fallback, i);
+            }
+        } else if (base + step < n) {
+            features(base + step, t - TP, F[buf ^ 1]);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void gather_photons_kernel(const mlgpu_photons src, const int32_t* __restrict__ idx,
                                       const int32_t* __restrict__ count, const int64_t n,
                                       const mlgpu_photon_buffers dst) {
@@ -1008,8 +1344,17 @@ hipError_t launch_fused(const FusedArgs& a, int64_t n, int num_cus, hipStream_t 
     if (n <= 0) return hipSuccess;
     const int64_t tiles = (n + kTileP - 1) / kTileP;
     const int64_t cap = static_cast<int64_t>(num_cus) * 2;
-    hipLaunchKernelGGL(fused_kernel, dim3(static_cast<uint32_t>(tiles < cap ? tiles : cap)), dim3(256), 0, stream, a,
-                       n);
+    const dim3 grid(static_cast<uint32_t>(tiles < cap ? tiles : cap));
+    if (a.fp8) {
+        for (int l = 1; l < kLayers; l++)
+            if (!a.wq[l] || !a.wscale[l]) return hipErrorInvalidValue;
+        const int64_t tiles8 = (n + kFp8Tile - 1) / kFp8Tile;
+        const int64_t cap8 = static_cast<int64_t>(num_cus) * (256 / kFp8Tile);
+        hipLaunchKernelGGL(fused_fp8_kernel<kFp8Tile>, dim3(static_cast<uint32_t>(tiles8 < cap8 ? tiles8 : cap8)),
+                           dim3(2 * kFp8Tile), 0, stream, a, n);
+    } else {
+        hipLaunchKernelGGL(fused_kernel, grid, dim3(256), 0, stream, a, n);
+    }
     return hipGetLastError();
 }
 
@@ -1025,6 +1370,12 @@ hipError_t launch_pad_weights(const float* params, const PaddedWeights& pw, hipS
     const int64_t total = padded_elements();
     hipLaunchKernelGGL(pad_weights_kernel, dim3(static_cast<uint32_t>((total + 255) / 256)), dim3(256), 0, stream,
                        params, pw);
+    if (pw.wq[1]) {
+        int rows = 0;
+        for (int l = 1; l < kLayers; l++) rows += kPadOut[l];
+        hipLaunchKernelGGL(quantize_weights_kernel, dim3(static_cast<uint32_t>((rows + 3) / 4)), dim3(256), 0, stream,
+                           params, pw);
+    }
     return hipGetLastError();
 }
 

@@ -107,6 +107,12 @@ struct PaddedWeights {
     uint16_t* w[kLayers];
     float* b[kLayers];
     uint16_t* wf[kLayers];   // fragment-ordered copy for the fused inference kernel
+    // fp8 (e4m3) fragment-ordered copies of layers 1.. for the fp8 inference
+    // kernel, row r of layer l stored as W[r] * 2^s (the largest s with
+    // max |W[r]| * 2^s <= 448), and the rows' scale bytes 127 - s (layer 0:
+    // null)
+    uint8_t* wq[kLayers];
+    uint8_t* wscale[kLayers];
 };
 hipError_t launch_pad_weights(const float* params, const PaddedWeights& pw, hipStream_t stream);
 
@@ -146,6 +152,9 @@ struct FusedArgs {
     float* out;                // n x kOut, or null
     const uint16_t* wf[kLayers];  // fragment-ordered bf16 weights
     const float* b[kLayers];
+    bool fp8;                     // layers 1.. on the fp8 copies (PaddedWeights::wq, wscale)
+    const uint8_t* wq[kLayers];
+    const uint8_t* wscale[kLayers];
 };
 hipError_t launch_fused(const FusedArgs& a, int64_t n, int num_cus, hipStream_t stream);
 

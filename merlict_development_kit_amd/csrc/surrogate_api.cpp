@@ -22,6 +22,7 @@ struct mlgpu_surrogate {
     PaddedWeights pw{};
     void* weights_mem = nullptr;
     int64_t step = 0;
+    bool fp8 = false;  // inference precision of layers 1.. (mlgpu_surrogate_set_inference)
     std::mutex mutex;  // guards the batch workspace, the step counter and the region
     // truncation region of the training loss (mlgpu_surrogate_set_region)
     bool region_on = false;
@@ -227,7 +228,10 @@ FusedArgs fused_args(const mlgpu_surrogate* s) {
     for (int l = 0; l < kLayers; l++) {
         a.wf[l] = s->pw.wf[l];
         a.b[l] = s->pw.b[l];
+        a.wq[l] = s->pw.wq[l];
+        a.wscale[l] = s->pw.wscale[l];
     }
+    a.fp8 = s->fp8;
     return a;
 }
 
@@ -264,7 +268,8 @@ int mlgpu_surrogate_create(uint64_t seed, mlgpu_surrogate** out) {
     size_t wbytes = 0;
     for (int l = 0; l < kLayers; l++)
         wbytes += align256(static_cast<size_t>(kPadOut[l]) * kPadIn[l] * 2) + align256(static_cast<size_t>(kPadOut[l]) * 4) +
-                  align256(fragment_elements(l) * 2);
+                  align256(fragment_elements(l) * 2) +
+                  (l > 0 ? align256(static_cast<size_t>(kPadOut[l]) * kPadIn[l]) + align256(kPadOut[l]) : 0);
     e = hipMalloc(&s->weights_mem, wbytes);
     if (e != hipSuccess) { (void)hipFree(s->params); delete s; return hip_fail(e, "hipMalloc surrogate weights"); }
     char* q = static_cast<char*>(s->weights_mem);
@@ -275,6 +280,12 @@ int mlgpu_surrogate_create(uint64_t seed, mlgpu_surrogate** out) {
         q += align256(static_cast<size_t>(kPadOut[l]) * 4);
         s->pw.wf[l] = reinterpret_cast<uint16_t*>(q);
         q += align256(fragment_elements(l) * 2);
+        if (l > 0) {
+            s->pw.wq[l] = reinterpret_cast<uint8_t*>(q);
+            q += align256(static_cast<size_t>(kPadOut[l]) * kPadIn[l]);
+            s->pw.wscale[l] = reinterpret_cast<uint8_t*>(q);
+            q += align256(kPadOut[l]);
+        }
     }
     e = launch_init_params(s->params, seed, nullptr);
     if (e == hipSuccess) e = hipMemsetAsync(s->adam_m, 0, pbytes, nullptr);
@@ -317,6 +328,20 @@ int mlgpu_surrogate_set_params(mlgpu_surrogate* s, const float* src, void* strea
     if (e != hipSuccess) return hip_fail(e, "set parameters");
     s->step = 0;
     return MLGPU_OK;
+}
+
+int mlgpu_surrogate_set_inference(mlgpu_surrogate* s, int32_t precision) {
+    if (!s || (precision != MLGPU_SURROGATE_BF16 && precision != MLGPU_SURROGATE_FP8)) {
+        set_last_error("null surrogate or unknown precision");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(s->mutex);
+    s->fp8 = precision == MLGPU_SURROGATE_FP8;
+    return MLGPU_OK;
+}
+
+int32_t mlgpu_surrogate_get_inference(const mlgpu_surrogate* s) {
+    return s && s->fp8 ? MLGPU_SURROGATE_FP8 : MLGPU_SURROGATE_BF16;
 }
 
 int mlgpu_surrogate_forward(mlgpu_surrogate* s, const float* features, int64_t n, float* out, void* stream) {

@@ -68,6 +68,21 @@ class Surrogate:
                                     or not targets.is_contiguous() or targets.device != features.device):
             raise ValueError("targets must be a contiguous (n, 4) float32 tensor on the features' device")
 
+    @property
+    def inference(self) -> str:
+        """Inference precision of forward and the hybrid: "bf16" (the
+        default) or "fp8" (layers 1.. on the fp8 matrix instructions with
+        e4m3 weights scaled per row by a power of two and e4m3 activations;
+        layer 0 in bf16; fp32 accumulation)."""
+        return "fp8" if N.lib().mlgpu_surrogate_get_inference(self._h) == 1 else "bf16"
+
+    @inference.setter
+    def inference(self, precision: str) -> None:
+        codes = {"bf16": 0, "fp8": 1}
+        if precision not in codes:
+            raise ValueError(f"precision must be one of {sorted(codes)}")
+        N.check(N.lib().mlgpu_surrogate_set_inference(self._h, codes[precision]), "mlgpu_surrogate_set_inference")
+
     def forward(self, features: torch.Tensor, stream=None) -> torch.Tensor:
         self._check(features)
         out = torch.empty((len(features), NUM_OUTPUTS), dtype=torch.float32, device=features.device)

@@ -65,6 +65,51 @@ def forward(params: np.ndarray, features: np.ndarray, bf16_storage: bool = True)
     raise AssertionError
 
 
+E4M3_MAX = 448.0
+
+
+def e4m3(x) -> np.ndarray:
+    """Round values to OCP fp8 e4m3 (4 exponent bits of bias 7, 3 mantissa
+    bits, subnormals down to 2^-9, largest finite 448) to nearest even,
+    returned as float64; magnitudes beyond 448 saturate to 448."""
+    a = np.asarray(x, dtype=np.float64)
+    m = np.abs(a)
+    _, e = np.frexp(m)                       # m = f 2^e, f in [0.5, 1)
+    ulp = np.ldexp(1.0, np.maximum(e - 1, -6) - 3)
+    r = np.round(m / ulp) * ulp              # numpy rounds halves to even
+    return np.copysign(np.minimum(r, E4M3_MAX), a)
+
+
+def fp8_row_shift(m) -> np.ndarray:
+    """The largest s with m 2^s <= 448 = 1.75 2^8 (m > 0; 0 for m = 0),
+    clamped to [-100, 100]: the power-of-two row scale of the fp8 weights."""
+    m = np.asarray(m, dtype=np.float64)
+    f, e = np.frexp(m)
+    s = np.where(2.0 * f <= 1.75, 8, 7) - (e - 1)
+    return np.where(m > 0, np.clip(s, -100, 100), 0).astype(np.int64)
+
+
+def forward_fp8(params: np.ndarray, features: np.ndarray) -> np.ndarray:
+    """Outputs (n x 57) of the fp8 inference path (mlgpu_surrogate_set_inference
+    MLGPU_SURROGATE_FP8): layer 0 on bf16 features and weights; every later
+    layer on e4m3 weights W[r] 2^s rounded and scaled back by 2^-s (s per
+    row, fp8_row_shift of the row's largest fp32 magnitude) and e4m3
+    activations relu(z) clamped to 448; fp32 biases added after the sums."""
+    layers = unpack(np.asarray(params, dtype=np.float32).astype(np.float64))
+    qb = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)
+    W0, b0 = layers[0]
+    h = e4m3(np.clip(bf16(features) @ bf16(W0).T + qb(b0), 0.0, E4M3_MAX))
+    for i, (W, b) in enumerate(layers[1:]):
+        shift = fp8_row_shift(np.abs(W).max(axis=1))[:, None]
+        Wq = np.ldexp(e4m3(np.ldexp(W, shift)), -shift)
+        z = h @ Wq.T + qb(b)
+        if i < len(layers) - 2:
+            h = e4m3(np.clip(z, 0.0, E4M3_MAX))
+        else:
+            return z
+    raise AssertionError
+
+
 def mixture_loss(out: np.ndarray, targets: np.ndarray):
     """Per-photon loss and its gradient wrt the 57 outputs."""
     n = len(out)

@@ -78,6 +78,51 @@ def test_oracle_bf16_rounding():
     assert abs(r[3] + 3.140625) < 1e-12 and r[4] == 0.0
 
 
+def test_oracle_e4m3_rounding():
+    """The e4m3 grid by hand (ties to even in the normal and the subnormal
+    range, the largest value, saturation) and against torch's float8_e4m3fn
+    cast on random values over the whole range."""
+    x = np.array([1.0, 1.0 + 2 ** -5, 1.0 + 3 * 2 ** -5, 1.0625, 448.0, 460.0, 2 ** -9, 2 ** -10, 3 * 2 ** -10,
+                  -0.3, 1.9375 + 2 ** -5, 2 ** -6 + 2 ** -10, 0.0, 1.1875])
+    r = SO.e4m3(x)
+    # three mantissa bits: neighbours of 1 are 1 and 1.125
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.125
+    assert r[3] == 1.0 and r[13] == 1.25              # ties to even, down and up
+    assert r[4] == 448.0 and r[5] == 448.0
+    assert r[6] == 2 ** -9 and r[7] == 0.0             # smallest subnormal; its half ties to zero
+    assert r[8] == 2 ** -8                             # subnormal tie to even
+    assert r[9] == -0.3125 and r[10] == 2.0 and r[11] == 2 ** -6 and r[12] == 0.0
+    rng = np.random.default_rng(4)
+    v = (rng.uniform(-1, 1, 100_000) * np.exp2(rng.uniform(-12, 8.7, 100_000))).astype(np.float32)
+    t = torch.from_numpy(v).to(torch.float8_e4m3fn).to(torch.float64).numpy()
+    np.testing.assert_array_equal(SO.e4m3(v), t)
+
+
+def test_oracle_fp8_row_shift():
+    """The shift puts the row's largest magnitude in (224, 448]."""
+    m = np.array([448.0, 448.5, 224.0, 224.5, 1.0, 1.75, 1.76, 0.1, 3e-5, 0.0])
+    s = SO.fp8_row_shift(m)
+    assert s.tolist()[:4] == [0, -1, 1, 0] and s[4] == 8 and s[5] == 8 and s[6] == 7 and s[-1] == 0
+    y = np.ldexp(m[:-1], s[:-1])
+    assert np.all(y <= 448.0) and np.all(y > 224.0)
+
+
+def test_oracle_fp8_forward_near_fp64():
+    """The fp8 restatement stays near the FP64 network, within the e4m3
+    rounding of four layers (about 2^-4 relative per weight and activation,
+    a few per cent of the outputs), and nearer than a zeroed layer would."""
+    p = _params(7)
+    f, _ = _batch(500, 8)
+    want, _ = SO.forward(p, f, bf16_storage=False)
+    got = SO.forward_fp8(p, f)
+    err = np.abs(got - want)
+    scale = np.abs(want).mean()
+    assert 1e-3 * scale < err.mean() < 0.1 * scale
+    q = p.copy()
+    q[SO.NUM_PARAMS - 57 * 256 - 57:SO.NUM_PARAMS - 57] = 0.0  # a dropped output layer
+    assert np.abs(SO.forward_fp8(q, f) - want).mean() > 0.5 * scale
+
+
 # ------------------------------------------------------------------ GPU
 @pytest.fixture(scope="module")
 def cuda():
@@ -100,6 +145,84 @@ def test_gpu_surrogate_forward_matches_oracle(cuda):
     scale = np.abs(want).mean()
     print(f"forward: max abs err {err.max():.3e}, mean abs err {err.mean():.3e}, mean |out| {scale:.3e}")
     assert err.mean() < 1e-2 * scale and err.max() < 0.1 * scale
+
+
+def _integer_params(seed):
+    """A network whose sums are small integers: features and layer-0 weights
+    in {-1, 0, 1}, later rows with at most six weights of -1, 1, 2 or 4,
+    integer biases.
+    Every sum is exact at any precision of the matrix instructions' adders;
+    activations past 16 exercise e4m3 ties between integers and past 448
+    the clamp."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for li, (out, inp) in enumerate(SO.SIZES):
+        if li == 0:
+            W = rng.integers(-1, 2, (out, inp)).astype(np.float64)
+        else:
+            W = np.zeros((out, inp))
+            for r in range(out):
+                k = rng.integers(0, 7)
+                W[r, rng.choice(inp, k, replace=False)] = rng.choice([-1.0, 1.0, 2.0, 4.0], k)
+        parts += [W.reshape(-1), rng.integers(0, 3 if li < 4 else 2, out).astype(np.float64) * (1 if li < 4 else -1)]
+    return np.concatenate(parts)
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_fp8_exact_on_integer_network(cuda):
+    """fp8 inference on a network of exact integer sums equals the oracle's
+    restatement bit for bit: fragment layouts, row scales, biases, ReLU,
+    the clamp to 448 and e4m3 rounding to nearest even."""
+    from merlict_development_kit_amd.surrogate import Surrogate
+    s = Surrogate(seed=5)
+    p = _integer_params(11)
+    s.set_params(torch.tensor(p, dtype=torch.float32, device=cuda))
+    s.inference = "fp8"
+    f = np.random.default_rng(12).integers(-1, 2, (777, 5)).astype(np.float32)
+    got = s.forward(torch.tensor(f, device=cuda)).cpu().numpy().astype(np.float64)
+    want = SO.forward_fp8(p, f)
+    hidden = [np.abs(want).max(), (want == np.round(want)).mean()]
+    print(f"integer network: outputs up to {hidden[0]:.0f}, integral fraction {hidden[1]:.3f}")
+    assert np.abs(want).max() > 448.0 * 0.5
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_fp8_forward_matches_oracle(cuda):
+    """The fp8 inference path against the oracle's restatement of its
+    arithmetic (e4m3 weights with per-row power-of-two scales, e4m3
+    activations, exact sums) on a He-initialised network.  The fp8 matrix
+    instruction's sums are not fp32-exact (the typical photon deviates by
+    about 4e-5 relative), and a deviation that moves an activation across
+    an e4m3 rounding boundary (2^-4 relative) carries into the later
+    layers: so the bulk must agree closely and a few per cent of the
+    photons may differ by a few per cent.  Switching back to bf16 gives the
+    bf16 outputs again."""
+    from merlict_development_kit_amd.surrogate import Surrogate
+    s = Surrogate(seed=5)
+    p = _params(7)
+    s.set_params(torch.tensor(p, dtype=torch.float32, device=cuda))
+    f, _ = _batch(1000, 8)
+    ft = torch.tensor(f, dtype=torch.float32, device=cuda)
+    bf = s.forward(ft).cpu().numpy()
+    assert s.inference == "bf16"
+    s.inference = "fp8"
+    got = s.forward(ft).cpu().numpy().astype(np.float64)
+    want = SO.forward_fp8(p, f.astype(np.float32))
+    err = np.abs(got - want)
+    scale = np.abs(want).mean()
+    exact, _ = SO.forward(p.astype(np.float32).astype(np.float64), f.astype(np.float32), bf16_storage=False)
+    q = np.abs(got - exact).mean()
+    print(f"fp8 forward: max abs err {err.max():.3e}, mean {err.mean():.3e}, mean |out| {scale:.3e}; "
+          f"vs fp64 network {q:.3e}")
+    row = err.max(axis=1)
+    print(f"photon max error quantiles 50/90/99 %: {np.quantile(row, [0.5, 0.9, 0.99]) / scale}")
+    assert np.median(row) < 2e-4 * scale and np.quantile(row, 0.9) < 5e-3 * scale
+    assert err.mean() < 2e-3 * scale and (row > 0.02 * scale).mean() < 0.1
+    s.inference = "bf16"
+    np.testing.assert_array_equal(s.forward(ft).cpu().numpy(), bf)
+    with pytest.raises(Exception):
+        s.inference = "fp4"
 
 
 @pytest.mark.gpu
@@ -296,10 +419,11 @@ def test_gpu_trained_surrogate_gains_over_untrained(cuda):
     """What training buys: on the FACT-like telescope, the hybrid's error
     against the plain propagator, per pixel and per pixel and time bin
     (means over 8 passes of 1e7 photons each), drops well below that of the
-    untrained network.  The untrained network draws every photon at its
-    nominal image point with a 1 cm spread and survival 1/2, the
-    geometric-optics guess.  Every photon is in the domain and the entropy
-    bound is off, so none falls back."""
+    untrained network, also with the trained network's inference in fp8.
+    The untrained network draws every photon at its nominal image point
+    with a 1 cm spread and survival 1/2, the geometric-optics guess.  Every
+    photon is in the domain and the entropy bound is off, so none falls
+    back."""
     from merlict_development_kit_amd import (ArrivalBinning, GpuScene, PropagationConfig, SceneDescription,
                                              histogram_arrivals)
     from merlict_development_kit_amd import surrogate as SG
@@ -331,8 +455,12 @@ def test_gpu_trained_surrogate_gains_over_untrained(cuda):
 
     pu, tu, fu = errors(untrained)
     pt, tt, ft = errors(trained)
-    print(f"L1 per pixel: untrained {pu:.4f}, trained {pt:.4f}; per pixel and time: untrained {tu:.4f}, "
-          f"trained {tt:.4f}; fallbacks {fu}, {ft}")
-    assert fu == 0 and ft == 0
+    trained.inference = "fp8"
+    p8, t8, f8 = errors(trained)
+    print(f"L1 per pixel: untrained {pu:.4f}, trained {pt:.4f}, trained in fp8 {p8:.4f}; per pixel and time: "
+          f"untrained {tu:.4f}, trained {tt:.4f}, fp8 {t8:.4f}; fallbacks {fu}, {ft}, {f8}")
+    assert fu == 0 and ft == 0 and f8 == 0
     assert pt < 0.7 * pu and pt < 0.02
     assert tt < 0.6 * tu
+    # the fp8 inference of the same network keeps what training bought
+    assert p8 < 0.02 and p8 < pt + 0.01 and t8 < 0.6 * tu
