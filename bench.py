@@ -185,7 +185,8 @@ def hybrid_stream_leg(scene: GpuScene, trained, dev, events: int = 4, bunches_pe
     280-700 nm) and 1e8 night-sky photons (uniform within 2 deg of the axis,
     lambda uniform on 300-600 nm); four events (about 1e9 photons).  The
     same photons go through the plain propagator and through the surrogate
-    + fallback hybrid (both timed whole, photon generation included);
+    + fallback hybrid, bf16 and fp8 inference (each timed whole, photon
+    generation included);
     per-pixel and per-pixel-and-time histograms are compared, with the noise
     floor from a second plain pass under other propagation seeds."""
     from merlict_development_kit_amd import ArrivalBinning, histogram_arrivals
@@ -238,7 +239,11 @@ def hybrid_stream_leg(scene: GpuScene, trained, dev, events: int = 4, bunches_pe
         return time.perf_counter() - t0, counts.cpu().numpy().astype(np.float64), fallback
 
     t_plain, plain, _ = run("plain", 0)
+    sur.inference = "bf16"
     t_hy, hy, fallback = run("hybrid", 0)
+    sur.inference = "fp8"
+    t_hy8, hy8, fallback8 = run("hybrid", 0)
+    sur.inference = "bf16"
     _, again, _ = run("plain", 100_000)
 
     def l1(a, b):
@@ -252,6 +257,9 @@ def hybrid_stream_leg(scene: GpuScene, trained, dev, events: int = 4, bunches_pe
         "hybrid_photons_per_s": total / t_hy,
         "speedup": t_plain / t_hy,
         "fallback_fraction": fallback / total,
+        "fp8": {"hybrid_photons_per_s": total / t_hy8, "speedup": t_plain / t_hy8,
+                "fallback_fraction": fallback8 / total, "pixel_l1": l1(hy8.sum(axis=1), plain.sum(axis=1)),
+                "pixel_time_l1": l1(hy8, plain)},
         "pixel_l1": l1(hy.sum(axis=1), plain.sum(axis=1)),
         "pixel_time_l1": l1(hy, plain),
         "noise_floor_pixel_l1": l1(again.sum(axis=1), plain.sum(axis=1)),

@@ -484,7 +484,8 @@ int mlgpu_propagate_hybrid(const mlgpu_scene* scene, mlgpu_surrogate* s, const m
         const int64_t k = *s->hy_count_host;
         if (k == 0) continue;
         fallbacks += k;
-        e = launch_gather_photons(chunk, s->hy_fallback, s->hy_count, m, s->hy_photons, st);
+        // k (<= m) is known here: one thread per fallback photon
+        e = launch_gather_photons(chunk, s->hy_fallback, s->hy_count, k, s->hy_photons, st);
         if (e != hipSuccess) return hip_fail(e, "hybrid gather");
         const mlgpu_photons fb{s->hy_photons.support_x, s->hy_photons.support_y, s->hy_photons.support_z,
                                s->hy_photons.direction_x, s->hy_photons.direction_y, s->hy_photons.direction_z,
