@@ -28,7 +28,8 @@ enum mlgpu_status {
     MLGPU_ERR_HIP = 3,         /* a HIP runtime call failed                */
     MLGPU_ERR_NO_DEVICE = 4,   /* no usable GPU                            */
     MLGPU_ERR_PHOTON = 5,      /* at least one photon carries an error flag */
-    MLGPU_ERR_UNSUPPORTED = 6
+    MLGPU_ERR_UNSUPPORTED = 6,
+    MLGPU_ERR_FORMAT = 7       /* malformed input file                     */
 };
 
 /* Text of the last error raised on the calling thread. */
@@ -529,6 +530,32 @@ int mlgpu_propagate_hybrid(const mlgpu_scene* scene, mlgpu_surrogate* surrogate,
                            const mlgpu_hybrid_config* hybrid,
                            const mlgpu_histogram* hist, int64_t* num_fallback,
                            void* stream);
+
+/* CORSIKA EventIO runs on the host: replaces eventio::Run
+ * (merlict_corsika/eventio.h:124-146, eventio.cpp:282-412), the reader whose
+ * bunches the reference's apps turn into photons.  Open reads the run
+ * header (the 273-float block, the input card, the telescope positions as
+ * (x, y, z, r) records) and looks ahead to the first event's bunches.
+ * has_events is 1 until the run end has been read (has_still_events_left);
+ * next_event moves to the next event (its 273-float header, the number of
+ * telescope offsets and of bunches), event_data copies its offsets
+ * (toff, xoff, yoff, weight records) and bunches (n x 8 float32: x, y
+ * [cm], cx, cy, time [ns], emission height [cm], weight, wavelength [nm];
+ * compact bunches scaled as the reference does), run_end the run end
+ * block once read.  Buffers are host memory.  A file the reader can not
+ * follow is an MLGPU_ERR_FORMAT (the reference throws), next_event past the
+ * run end an MLGPU_ERR_ARGUMENT. */
+typedef struct mlgpu_eventio_run mlgpu_eventio_run;
+int mlgpu_eventio_open(const char* path, mlgpu_eventio_run** out);
+void mlgpu_eventio_close(mlgpu_eventio_run* run);
+int mlgpu_eventio_run_header(const mlgpu_eventio_run* run, float* raw273, int32_t* num_telescopes,
+                             int64_t* input_card_length);
+int mlgpu_eventio_run_info(const mlgpu_eventio_run* run, float* telescope_positions, char* input_card);
+int mlgpu_eventio_has_events(const mlgpu_eventio_run* run);
+int mlgpu_eventio_next_event(mlgpu_eventio_run* run, float* header273, int32_t* num_offsets,
+                             int64_t* num_bunches);
+int mlgpu_eventio_event_data(const mlgpu_eventio_run* run, float* offsets, float* bunches);
+int mlgpu_eventio_run_end(const mlgpu_eventio_run* run, float* raw273);
 
 /* CORSIKA photon bunches to photons.  Replaces merlict::EventIoPhotonFactory
  * (merlict_corsika/PhotonFactory.cpp:8-148) as the reference's apps apply it

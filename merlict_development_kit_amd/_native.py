@@ -226,6 +226,14 @@ SIGNATURES = {
     "mlgpu_surrogate_destroy": (None, [C.c_void_p]),
     "mlgpu_surrogate_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mlgpu_surrogate_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlgpu_eventio_open": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "mlgpu_eventio_close": (None, [C.c_void_p]),
+    "mlgpu_eventio_run_header": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "mlgpu_eventio_run_info": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlgpu_eventio_has_events": (C.c_int, [C.c_void_p]),
+    "mlgpu_eventio_next_event": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "mlgpu_eventio_event_data": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlgpu_eventio_run_end": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mlgpu_surrogate_set_inference": (C.c_int, [C.c_void_p, C.c_int32]),
     "mlgpu_surrogate_get_inference": (C.c_int32, [C.c_void_p]),
     "mlgpu_surrogate_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),

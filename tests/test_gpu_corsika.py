@@ -84,3 +84,22 @@ def test_gpu_corsika_photons_land_on_their_ground_point_at_their_time(cuda, gold
     t_ref = bunches[tr, 4].astype(np.float64) * 1e-9
     t = res["time_of_flight"]
     np.testing.assert_allclose(t - t.mean(), t_ref - t_ref.mean(), rtol=0, atol=1e-11)
+
+
+def test_gpu_corsika_eventio_run_to_photons(cuda):
+    """The reference's test run (telescope.dat) read by the EventIO reader
+    and turned into photons on the GPU, as the reference's apps chain
+    eventio::Run and EventIoPhotonFactory: all 42,629 bunches of the event,
+    photons and truth ids bit for bit as the oracle's."""
+    from pathlib import Path
+    from merlict_development_kit_amd.corsika import EventIoRun
+    run = EventIoRun(Path(__file__).parent / "golden" / "corsika_telescope.dat")
+    ev = run.next_event()
+    b = np.ascontiguousarray(ev.bunches)
+    ph, truth = photons_from_bunches(torch.tensor(b, device=cuda), seed=5, id_offset=0)
+    want, want_truth = O.corsika_photons(b, seed=5, id_offset=0)
+    np.testing.assert_array_equal(truth.cpu().numpy(), want_truth)
+    got = ph.numpy()
+    for k in O.PHOTON_FIELDS:
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    assert len(want_truth) > 40_000 and not run.has_events()
