@@ -557,6 +557,23 @@ int mlgpu_eventio_next_event(mlgpu_eventio_run* run, float* header273, int32_t* 
 int mlgpu_eventio_event_data(const mlgpu_eventio_run* run, float* offsets, float* bunches);
 int mlgpu_eventio_run_end(const mlgpu_eventio_run* run, float* raw273);
 
+/* CORSIKA event tapes on the host: replaces event_tape::Run
+ * (merlict_corsika/event_tape.h:23-36, event_tape.cpp:10-45), a tar of
+ * ddddddddd/RUNH.float32, then per event ddddddddd/ddddddddd/EVTH.float32
+ * and its bunch blocks ddddddddd/ddddddddd/ddddddddd.cer.x8.float32.  Open
+ * reads the run header (a tape without one is an MLGPU_ERR_FORMAT) and the
+ * first event header; next_event reads the event's bunches (n x 8
+ * float32, host memory) and the next event header; has_events is 0 once
+ * no event header follows (a header or block that fails the reference's
+ * checks ends the run, as in the reference). */
+typedef struct mlgpu_event_tape mlgpu_event_tape;
+int mlgpu_event_tape_open(const char* path, mlgpu_event_tape** out);
+void mlgpu_event_tape_close(mlgpu_event_tape* tape);
+int mlgpu_event_tape_run_header(const mlgpu_event_tape* tape, float* raw273);
+int mlgpu_event_tape_has_events(const mlgpu_event_tape* tape);
+int mlgpu_event_tape_next_event(mlgpu_event_tape* tape, float* header273, int64_t* num_bunches);
+int mlgpu_event_tape_event_bunches(const mlgpu_event_tape* tape, float* bunches);
+
 /* CORSIKA photon bunches to photons.  Replaces merlict::EventIoPhotonFactory
  * (merlict_corsika/PhotonFactory.cpp:8-148) as the reference's apps apply it
  * to every bunch of an event (plenoscope_propagation.cpp:196-203).

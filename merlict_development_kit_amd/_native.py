@@ -226,6 +226,12 @@ SIGNATURES = {
     "mlgpu_surrogate_destroy": (None, [C.c_void_p]),
     "mlgpu_surrogate_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mlgpu_surrogate_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlgpu_event_tape_open": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "mlgpu_event_tape_close": (None, [C.c_void_p]),
+    "mlgpu_event_tape_run_header": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mlgpu_event_tape_has_events": (C.c_int, [C.c_void_p]),
+    "mlgpu_event_tape_next_event": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
+    "mlgpu_event_tape_event_bunches": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mlgpu_eventio_open": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "mlgpu_eventio_close": (None, [C.c_void_p]),
     "mlgpu_eventio_run_header": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),

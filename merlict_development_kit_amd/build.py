@@ -38,7 +38,7 @@ HIP_FLAGS = [
 
 SOURCES = ["scene_desc.cpp", "builders.cpp", "scene_build.cpp", "api.cpp", "propagate.hip", "sources.hip", "arrivals.hip", "corsika.hip", "surrogate.hip",
            "surrogate_api.cpp", "signal.hip", "signal_api.cpp", "calibration.hip", "render.hip", "pool.cpp",
-           "eventio.cpp"]
+           "eventio.cpp", "event_tape.cpp"]
 
 
 def _hipcc() -> str:

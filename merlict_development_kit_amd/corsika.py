@@ -162,3 +162,50 @@ class EventIoRun:
         out = self._np.zeros(273, self._np.float32)
         N.check(N.lib().mlgpu_eventio_run_end(self._h, out.ctypes.data), "mlgpu_eventio_run_end")
         return out
+
+
+class EventTapeRun:
+    """A CORSIKA event tape (a tar of RUNH, EVTH and Cherenkov bunch blocks)
+    read event by event (mlgpu_event_tape_*, event_tape::Run in
+    merlict_corsika/event_tape.h:23-36): ``header`` (273 floats),
+    ``has_events()`` / ``next_event()`` (header and (n, 8) float32 bunches)
+    or iteration."""
+
+    def __init__(self, path):
+        import numpy as np
+        self._np = np
+        self._h = C.c_void_p()
+        N.check(N.lib().mlgpu_event_tape_open(str(path).encode(), C.byref(self._h)), "mlgpu_event_tape_open")
+        self.header = np.zeros(273, np.float32)
+        N.check(N.lib().mlgpu_event_tape_run_header(self._h, self.header.ctypes.data), "mlgpu_event_tape_run_header")
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().mlgpu_event_tape_close(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def has_events(self) -> bool:
+        return bool(N.lib().mlgpu_event_tape_has_events(self._h))
+
+    def next_event(self) -> EventIoEvent:
+        np = self._np
+        header = np.zeros(273, np.float32)
+        nb = C.c_int64()
+        N.check(N.lib().mlgpu_event_tape_next_event(self._h, header.ctypes.data, C.byref(nb)),
+                "mlgpu_event_tape_next_event")
+        bunches = np.zeros((nb.value, 8), np.float32)
+        if nb.value:
+            N.check(N.lib().mlgpu_event_tape_event_bunches(self._h, bunches.ctypes.data),
+                    "mlgpu_event_tape_event_bunches")
+        return EventIoEvent(header, np.zeros((0, 4), np.float32), bunches)
+
+    def __iter__(self):
+        while self.has_events():
+            yield self.next_event()

@@ -114,3 +114,93 @@ def test_eventio_matches_reference_reader():
     got = np.concatenate([e.bunches for e in _run()])
     assert got.shape == want.shape
     np.testing.assert_array_equal(got.view(np.uint32), np.ascontiguousarray(want, np.float32).view(np.uint32))
+
+
+# ------------------------------------------------------------ event tapes
+TAPE = Path(__file__).parent / "golden" / "corsika_event_tape.tar"
+
+
+def _tape(path=TAPE):
+    from merlict_development_kit_amd.corsika import EventTapeRun
+    return EventTapeRun(path)
+
+
+def _tar_members(path):
+    """The tape's members as Python's tarfile reads them: name -> bytes."""
+    import tarfile
+    with tarfile.open(path) as tf:
+        return {m.name: tf.extractfile(m).read() for m in tf.getmembers() if m.isfile()}
+
+
+def test_event_tape_read_one_by_one():
+    """EventTapeTest.cpp:10-27: RUNH, then three events whose headers start
+    with 'EVTH', and no fourth."""
+    r = _tape()
+    assert r.header[0] == np.frombuffer(b"RUNH", np.float32)[0]
+    evth = np.frombuffer(b"EVTH", np.float32)[0]
+    for k in range(3):
+        assert r.has_events()
+        assert r.next_event().header[0] == evth
+    assert not r.has_events()
+    from merlict_development_kit_amd._native import MlgpuError
+    with pytest.raises(MlgpuError):
+        r.next_event()
+
+
+def test_event_tape_matches_tar_members():
+    """Headers and bunches bit for bit as Python's tarfile reads the
+    members (RUNH, each EVTH, each event's cer.x8.float32 blocks in order)."""
+    m = _tar_members(TAPE)
+    r = _tape()
+    assert r.header.tobytes() == m["000000001/RUNH.float32"]
+    for k, ev in enumerate(r, start=1):
+        d = f"000000001/{k:09d}"
+        assert ev.header.tobytes() == m[f"{d}/EVTH.float32"]
+        blocks = sorted(n for n in m if n.startswith(d + "/") and n.endswith(".cer.x8.float32"))
+        assert ev.bunches.tobytes() == b"".join(m[n] for n in blocks) and len(ev.bunches) > 0
+    assert k == 3
+
+
+def test_event_tape_photons_point_down():
+    """EventTapeTest.cpp:29-56: the photons of every event's bunches (the
+    oracle's EventIoPhotonFactory restatement) have wavelengths within
+    190-700 nm and directions down, steeper than |z| > 0.7."""
+    for ev in _tape():
+        ph, _ = O.corsika_photons(np.ascontiguousarray(ev.bunches), seed=0, id_offset=0)
+        assert len(ph["wavelength"]) > 0
+        assert np.all((ph["wavelength"] > 190e-9) & (ph["wavelength"] < 700e-9))
+        assert np.all(ph["direction_z"] < 0) and np.all(np.abs(ph["direction_z"]) > 0.7)
+
+
+def test_event_tape_errors(tmp_path):
+    """A tape that does not start with ddddddddd/RUNH.float32, a damaged
+    first header (bad checksum) and an empty file are errors; a tape cut
+    after its first event ends the run after it."""
+    import tarfile
+    import io
+    from merlict_development_kit_amd._native import MlgpuError
+    m = _tar_members(TAPE)
+    wrong = tmp_path / "wrong.tar"
+    with tarfile.open(wrong, "w", format=tarfile.USTAR_FORMAT) as tf:
+        ti = tarfile.TarInfo("000000001/EVTH.float32")
+        ti.size = 1092
+        tf.addfile(ti, io.BytesIO(m["000000001/000000001/EVTH.float32"]))
+    raw = bytearray(TAPE.read_bytes())
+    raw[0] ^= 1  # the name changes, the stored checksum does not
+    bad = tmp_path / "bad.tar"
+    bad.write_bytes(bytes(raw))
+    empty = tmp_path / "empty.tar"
+    empty.write_bytes(b"")
+    for p in (wrong, bad, empty, tmp_path / "missing.tar"):
+        with pytest.raises(MlgpuError):
+            _tape(p)
+    one = tmp_path / "one.tar"
+    with tarfile.open(one, "w", format=tarfile.USTAR_FORMAT) as tf:
+        for name in ("000000001/RUNH.float32", "000000001/000000001/EVTH.float32",
+                     "000000001/000000001/000000001.cer.x8.float32"):
+            ti = tarfile.TarInfo(name)
+            ti.size = len(m[name])
+            tf.addfile(ti, io.BytesIO(m[name]))
+    r = _tape(one)
+    events = list(r)
+    assert len(events) == 1 and events[0].bunches.tobytes() == m["000000001/000000001/000000001.cer.x8.float32"]
