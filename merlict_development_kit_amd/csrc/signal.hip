@@ -541,6 +541,53 @@ __global__ void key_at_kernel(const uint64_t* __restrict__ sorted, const int64_t
     if (threadIdx.x == 0 && blockIdx.x == 0) *out = key_time(sorted[at]);
 }
 
+// Radix selection of the k-th smallest of m keys, 8 bits a pass from the
+// top: the histogram of the next digit over the keys that share the bits
+// chosen so far, then the bucket that holds rank k.  After eight passes
+// `prefix` is the key itself.
+struct SelectState {
+    unsigned long long prefix, mask;
+    long long k;
+    unsigned int hist[256];
+};
+
+__global__ void select_init_kernel(SelectState* st, const long long k) {
+    st->hist[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) {
+        st->prefix = 0ull;
+        st->mask = 0ull;
+        st->k = k;
+    }
+}
+
+__global__ __launch_bounds__(256) void select_hist_kernel(const uint64_t* __restrict__ keys, const int64_t m,
+                                                          SelectState* __restrict__ st, const int shift) {
+    __shared__ unsigned int h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const unsigned long long prefix = st->prefix, mask = st->mask;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < m;
+         i += static_cast<int64_t>(gridDim.x) * 256) {
+        const unsigned long long key = keys[i];
+        if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 255ull], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void select_pick_kernel(SelectState* st, const int shift) {
+    if (threadIdx.x == 0) {
+        long long k = st->k, below = 0;
+        int b = 0;
+        while (b < 255 && below + static_cast<long long>(st->hist[b]) <= k) below += st->hist[b++];
+        st->k = k - below;
+        st->prefix |= static_cast<unsigned long long>(b) << shift;
+        st->mask |= 255ull << shift;
+    }
+    __syncthreads();
+    st->hist[threadIdx.x] = 0u;
+}
+
 size_t align_up(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
 }  // namespace
@@ -871,31 +918,29 @@ hipError_t fact_slices(const mlgpu_pulses& p, int64_t count, double trigger_time
                        int16_t* slice, hipStream_t stream) {
     if (count <= 0) return hipSuccess;
     const size_t m = static_cast<size_t>(count);
-    size_t sort_bytes = 0;
-    hipError_t e = rocprim::radix_sort_keys(nullptr, sort_bytes, static_cast<uint64_t*>(nullptr),
-                                            static_cast<uint64_t*>(nullptr), m, 0, 64, stream);
-    if (e != hipSuccess) return e;
     PoolBlock block;
-    e = block.get(2 * align_up(m * 8) + align_up(sort_bytes) + 256);
+    hipError_t e = block.get(align_up(m * 8) + align_up(sizeof(SelectState)) + 256);
     if (e != hipSuccess) return e;
     char* q = block.as();
     auto* keys = reinterpret_cast<uint64_t*>(q);
-    auto* sorted = reinterpret_cast<uint64_t*>(q + align_up(m * 8));
-    void* temp = q + 2 * align_up(m * 8);
-    auto* median = reinterpret_cast<double*>(q + 2 * align_up(m * 8) + align_up(sort_bytes));
+    auto* st = reinterpret_cast<SelectState*>(q + align_up(m * 8));
+    auto* median = reinterpret_cast<double*>(q + align_up(m * 8) + align_up(sizeof(SelectState)));
+    // estimate_median_arrival_time (fact_propagation.cpp:34-60): the sorted
+    // time at unsigned(count * 0.5), found by radix selection (no sort)
+    const int64_t half = static_cast<int64_t>(static_cast<uint32_t>(static_cast<double>(count) * 0.5));
+    const uint32_t grid = static_cast<uint32_t>(std::min<size_t>((m + 1023) / 1024, 2048));
     hipLaunchKernelGGL(time_keys_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream,
                        p.arrival_time, count, keys);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = rocprim::radix_sort_keys(temp, sort_bytes, keys, sorted, m, 0, 64, stream);
-    // estimate_median_arrival_time (fact_propagation.cpp:33-57): the sorted
-    // time at unsigned(count * 0.5)
-    const int64_t half = static_cast<int64_t>(static_cast<uint32_t>(static_cast<double>(count) * 0.5));
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(key_at_kernel, dim3(1), dim3(64), 0, stream, sorted, half, median);
-        hipLaunchKernelGGL(fact_slices_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream,
-                           p.arrival_time, count, median, trigger_time, slice_duration, slice);
-        e = hipGetLastError();
+    hipLaunchKernelGGL(select_init_kernel, dim3(1), dim3(256), 0, stream, st, static_cast<long long>(half));
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hipLaunchKernelGGL(select_hist_kernel, dim3(grid), dim3(256), 0, stream, keys, count, st, shift);
+        hipLaunchKernelGGL(select_pick_kernel, dim3(1), dim3(256), 0, stream, st, shift);
     }
+    hipLaunchKernelGGL(key_at_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<const uint64_t*>(&st->prefix),
+                       int64_t{0}, median);
+    hipLaunchKernelGGL(fact_slices_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)), dim3(256), 0, stream,
+                       p.arrival_time, count, median, trigger_time, slice_duration, slice);
+    e = hipGetLastError();
     (void)hipStreamSynchronize(stream);  // the borrowed block is idle when returned
     return e;
 }
