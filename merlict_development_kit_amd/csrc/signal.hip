@@ -576,16 +576,27 @@ __global__ __launch_bounds__(256) void select_hist_kernel(const uint64_t* __rest
 }
 
 __global__ void select_pick_kernel(SelectState* st, const int shift) {
-    if (threadIdx.x == 0) {
-        long long k = st->k, below = 0;
-        int b = 0;
-        while (b < 255 && below + static_cast<long long>(st->hist[b]) <= k) below += st->hist[b++];
-        st->k = k - below;
-        st->prefix |= static_cast<unsigned long long>(b) << shift;
+    // inclusive prefix sums of the 256 counts in LDS; the thread whose
+    // bucket holds rank k (the last bucket if none does) records it
+    __shared__ long long c[256];
+    const int t = static_cast<int>(threadIdx.x);
+    const long long k = st->k;
+    const long long v = st->hist[t];
+    st->hist[t] = 0u;
+    c[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const long long x = t >= off ? c[t - off] : 0;
+        __syncthreads();
+        c[t] += x;
+        __syncthreads();
+    }
+    const long long incl = c[t], excl = incl - v;
+    if (excl <= k && (k < incl || t == 255)) {
+        st->k = k - excl;
+        st->prefix |= static_cast<unsigned long long>(t) << shift;
         st->mask |= 255ull << shift;
     }
-    __syncthreads();
-    st->hist[threadIdx.x] = 0u;
 }
 
 size_t align_up(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
