@@ -615,8 +615,8 @@ def main() -> int:
                          "traffic, for scaling rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the photon propagator runs on a single MI355X here; "
-              "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+        print("bench.py measures one GPU: the photon propagator runs on a single MI355X; "
+              "sharding photons or bunches across GPUs is not implemented.", file=sys.stderr)
         return 2
     dev = torch.device("cuda:0")
     if args.copy_calibration_mib > 0:
