@@ -515,9 +515,14 @@ int mlgpu_surrogate_set_region(mlgpu_surrogate* s, const mlgpu_surrogate_domain*
  * surrogate (survival, component, position, time from Philox stream 3 of
  * (seed, config photon_id_offset + i)) and binned into hist at the cell
  * its position falls in, if any; every other photon is propagated by the
- * plain propagator and binned from its result.  *num_fallback (host, may be
- * NULL) receives the number of propagated photons.  Synchronises the
- * stream once per chunk of 2^24 photons. */
+ * plain propagator and binned from its result, under its own photon id
+ * (config photon_id_offset + i for propagation, hist photon_id_offset + i
+ * for the jitter), so it gets exactly what mlgpu_propagate and
+ * mlgpu_histogram_arrivals give photon i of the same batch, and the
+ * histogram is the same from run to run.  A max_entropy of -inf sends every
+ * photon to the propagator.  *num_fallback (host, may be NULL) receives the
+ * number of propagated photons.  Synchronises the stream once per chunk of
+ * 2^24 photons. */
 typedef struct mlgpu_hybrid_config {
     double max_entropy;
     uint64_t seed;

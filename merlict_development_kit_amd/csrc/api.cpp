@@ -205,9 +205,9 @@ struct Carver {
 };
 
 int launch_fused(const mlgpu_scene* s, int32_t scope, const mlgpu_photons& ph, int64_t n, const mlgpu_config& cfg,
-                 const mlgpu_results& r, hipStream_t st) {
+                 const mlgpu_results& r, hipStream_t st, const int32_t* ids = nullptr) {
     const hipError_t e = mlgpu::launch_propagate(s->dev, scope, ph, n, cfg.max_num_interactions_per_photon, cfg.seed,
-                                                 cfg.photon_id_offset, r, st);
+                                                 cfg.photon_id_offset, r, st, ids);
     if (e != hipSuccess) return hip_fail(e, "propagate launch");
     return MLGPU_OK;
 }
@@ -243,7 +243,7 @@ mlgpu::Workspace carve_workspace(void* mem, int64_t capacity) {
 
 // Wavefront propagation; the caller holds ws_mutex and began shared use.
 int launch_wavefront(mlgpu_scene* ms, int32_t scope, const mlgpu_photons& ph, int64_t n, const mlgpu_config& cfg,
-                     const mlgpu_results& r, hipStream_t st) {
+                     const mlgpu_results& r, hipStream_t st, const int32_t* ids = nullptr) {
     if (ms->ws_capacity < n) {
         const int64_t cap = n + n / 8;
         const int rc = grow(ms, &ms->ws, mlgpu::workspace_bytes(cap));
@@ -253,7 +253,7 @@ int launch_wavefront(mlgpu_scene* ms, int32_t scope, const mlgpu_photons& ph, in
     const mlgpu::Workspace w = carve_workspace(ms->ws, ms->ws_capacity);
     const hipError_t e = mlgpu::launch_propagate_wavefront(
         ms->dev, scope, ph, n, cfg.max_num_interactions_per_photon, cfg.seed, cfg.photon_id_offset, r, w,
-        ms->tuning.blocks_per_cu * ms->num_cus, ms->tuning.trace_budget, st);
+        ms->tuning.blocks_per_cu * ms->num_cus, ms->tuning.trace_budget, st, nullptr, ids);
     if (e != hipSuccess) return hip_fail(e, "propagate launch");
     return MLGPU_OK;
 }
@@ -328,6 +328,47 @@ int run_bunches(mlgpu_scene* ms, int32_t scope, const mlgpu_bunch* bunches, int3
 }
 
 }  // namespace
+
+namespace mlgpu {
+
+int propagate_gathered(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photons* photons, int64_t n,
+                       const mlgpu_config* cfg, mlgpu_results* r, const int32_t* ids, void* stream) {
+    if (!s || !cfg || !r || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (n == 0) return MLGPU_OK;
+    if (!photons_ok(photons)) { set_last_error("null photon array"); return MLGPU_ERR_ARGUMENT; }
+    int rc;
+    if ((rc = check_results(r)) || (rc = check_world(s, world_frame)) || (rc = check_config(cfg))) return rc;
+    if (n > INT32_MAX) { set_last_error("at most 2^31 - 1 photons per call"); return MLGPU_ERR_ARGUMENT; }
+    auto* ms = const_cast<mlgpu_scene*>(s);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (schedule_for(s, n) == MLGPU_SCHEDULE_FUSED)
+        return launch_fused(s, s->scope_of_frame[world_frame], *photons, n, *cfg, *r, st, ids);
+    std::lock_guard<std::mutex> lock(ms->ws_mutex);
+    if ((rc = begin_shared_use(ms, st))) return rc;
+    rc = launch_wavefront(ms, s->scope_of_frame[world_frame], *photons, n, *cfg, *r, st, ids);
+    end_shared_use(ms, st);
+    return rc;
+}
+
+int histogram_gathered(const mlgpu_results* r, int64_t n, const mlgpu_histogram* h, const int32_t* ids,
+                       void* stream) {
+    if (!r || !h || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (n == 0) return MLGPU_OK;
+    if (!r->sensor_id || !r->error || !r->time_of_flight) {
+        set_last_error("null array");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    const int rc = check_histogram(h);
+    if (rc) return rc;
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    const hipError_t e = mlgpu::launch_histogram(*r, n, *h, cus, static_cast<hipStream_t>(stream), nullptr, ids);
+    if (e != hipSuccess) return hip_fail(e, "histogram launch");
+    return MLGPU_OK;
+}
+
+}  // namespace mlgpu
 
 extern "C" {
 
@@ -494,21 +535,7 @@ int mlgpu_scene_stats(const mlgpu_scene* s, int64_t* num_nodes, int64_t* num_lea
 
 int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photons* photons, int64_t n,
                     const mlgpu_config* cfg, mlgpu_results* r, void* stream) {
-    if (!s || !cfg || !r || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
-    if (n == 0) return MLGPU_OK;
-    if (!photons_ok(photons)) { set_last_error("null photon array"); return MLGPU_ERR_ARGUMENT; }
-    int rc;
-    if ((rc = check_results(r)) || (rc = check_world(s, world_frame)) || (rc = check_config(cfg))) return rc;
-    if (n > INT32_MAX) { set_last_error("at most 2^31 - 1 photons per call"); return MLGPU_ERR_ARGUMENT; }
-    auto* ms = const_cast<mlgpu_scene*>(s);
-    const hipStream_t st = static_cast<hipStream_t>(stream);
-    if (schedule_for(s, n) == MLGPU_SCHEDULE_FUSED)
-        return launch_fused(s, s->scope_of_frame[world_frame], *photons, n, *cfg, *r, st);
-    std::lock_guard<std::mutex> lock(ms->ws_mutex);
-    if ((rc = begin_shared_use(ms, st))) return rc;
-    rc = launch_wavefront(ms, s->scope_of_frame[world_frame], *photons, n, *cfg, *r, st);
-    end_shared_use(ms, st);
-    return rc;
+    return mlgpu::propagate_gathered(s, world_frame, photons, n, cfg, r, nullptr, stream);
 }
 
 int mlgpu_propagate_bunches(const mlgpu_scene* s, int32_t world_frame, const mlgpu_bunch* bunches,
@@ -744,20 +771,7 @@ int mlgpu_corsika_photons(const float* bunches, int64_t n, uint64_t seed, uint64
 }
 
 int mlgpu_histogram_arrivals(const mlgpu_results* r, int64_t n, const mlgpu_histogram* h, void* stream) {
-    if (!r || !h || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
-    if (n == 0) return MLGPU_OK;
-    if (!r->sensor_id || !r->error || !r->time_of_flight) {
-        set_last_error("null array");
-        return MLGPU_ERR_ARGUMENT;
-    }
-    const int rc = check_histogram(h);
-    if (rc) return rc;
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-    const hipError_t e = mlgpu::launch_histogram(*r, n, *h, cus, static_cast<hipStream_t>(stream));
-    if (e != hipSuccess) return hip_fail(e, "histogram launch");
-    return MLGPU_OK;
+    return mlgpu::histogram_gathered(r, n, h, nullptr, stream);
 }
 
 int mlgpu_count_errors(const mlgpu_results* r, int64_t n, void* stream, int64_t* num_errors) {

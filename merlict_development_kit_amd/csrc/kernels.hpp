@@ -48,16 +48,19 @@ inline size_t workspace_bytes(int64_t cap) {
     return m * per + 4 * kNumCounts + 40 * 256;
 }
 
+// Photon i of a batch draws from Philox stream id id_offset + i, or
+// id_offset + ids[i] when ids is given (a batch gathered from a larger one).
 hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in, int64_t n,
                             uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
-                            const mlgpu_results& out, hipStream_t stream);
+                            const mlgpu_results& out, hipStream_t stream, const int32_t* ids = nullptr);
 
 // n (and seed, id_offset) size the grids; with cp, the kernels take the
 // photon count, seed and id offset from *cp (n is then the capacity)
 hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& out, const Workspace& ws, int grid_blocks,
-                                      int32_t trace_budget, hipStream_t stream, const CallParams* cp = nullptr);
+                                      int32_t trace_budget, hipStream_t stream, const CallParams* cp = nullptr,
+                                      const int32_t* ids = nullptr);
 
 hipError_t launch_first_hit(const DeviceScene& sc, int32_t root_scope, const mlgpu_photons& rays, int64_t n,
                             const mlgpu_hits& out, hipStream_t stream);
@@ -77,8 +80,10 @@ hipError_t launch_generate_bunches(const BunchDev* table, int32_t num_bunches, i
 hipError_t launch_generate(const mlgpu_source& src, int64_t n, uint64_t seed, uint64_t id_offset,
                            const mlgpu_photon_buffers& out, hipStream_t stream);
 
+// Result i is jittered on the stream of photon id h.photon_id_offset + i, or
+// h.photon_id_offset + ids[i] when ids is given.
 hipError_t launch_histogram(const mlgpu_results& r, int64_t n, const mlgpu_histogram& h, int num_cus,
-                            hipStream_t stream, const CallParams* cp = nullptr);
+                            hipStream_t stream, const CallParams* cp = nullptr, const int32_t* ids = nullptr);
 
 hipError_t launch_count_errors(const uint8_t* err, int64_t n, unsigned long long* counter, hipStream_t stream);
 

@@ -24,6 +24,15 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Philox stream id of photon i of a batch: id_offset + i, or id_offset +
+// ids[i] when the batch was gathered from a larger one (ids: the photons'
+// indices there), so a photon draws the same numbers wherever it is
+// propagated.
+__device__ __forceinline__ uint64_t photon_id(const uint64_t id_offset, const int32_t* __restrict__ ids,
+                                              const int64_t i) {
+    return id_offset + static_cast<uint64_t>(ids ? static_cast<int64_t>(ids[i]) : i);
+}
+
 // The per-lane traversal stacks of a block: column threadIdx.x.
 #define MLGPU_LANE_STACK(name)                                          \
     __shared__ int32_t name##_mem[kMaxTreeDepth * kBlock];              \
@@ -32,11 +41,12 @@ constexpr int kBlock = 256;
 __global__ __launch_bounds__(kBlock) void propagate_kernel(const DeviceScene sc, const int32_t world_scope,
                                                            const mlgpu_photons in, const int64_t n,
                                                            const uint32_t max_interactions, const uint64_t seed,
-                                                           const uint64_t id_offset, const mlgpu_results out) {
+                                                           const uint64_t id_offset, const mlgpu_results out,
+                                                           const int32_t* __restrict__ ids) {
     MLGPU_LANE_STACK(stk);
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    propagate_photon(sc, world_scope, in, i, max_interactions, seed, id_offset, out, stk);
+    propagate_photon(sc, world_scope, in, i, max_interactions, seed, photon_id(id_offset, ids, i), out, stk);
 }
 
 // ------------------------------------------------------- wavefront schedule
@@ -209,7 +219,8 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
                                                           const Workspace ws, const int level,
                                                           const uint32_t max_interactions, const uint64_t seed_arg,
                                                           const uint64_t id_offset_arg, const mlgpu_results out,
-                                                          const CallParams* __restrict__ cp) {
+                                                          const CallParams* __restrict__ cp,
+                                                          const int32_t* __restrict__ ids) {
     const uint64_t seed = cp ? cp->seed : seed_arg;
     const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
     __shared__ TileQueue<kTile> q;
@@ -235,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
                 st.n_hist = ws.n_hist[i];
                 st.err = MLGPU_PHOTON_OK;
                 Rng rng;
-                rng.init(seed, id_offset + static_cast<uint64_t>(i), ws.draws[i]);
+                rng.init(seed, photon_id(id_offset, ids, i), ws.draws[i]);
                 go = photon_bounce(sc, st, ws.hit_leaf[slot], ws.hit_t[slot], in.wavelength[i], rng,
                                    max_interactions);
                 if (go) {
@@ -295,19 +306,20 @@ __global__ void count_errors_kernel(const uint8_t* err, int64_t n, unsigned long
 
 hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in, int64_t n,
                             uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
-                            const mlgpu_results& out, hipStream_t stream) {
+                            const mlgpu_results& out, hipStream_t stream, const int32_t* ids) {
     if (n <= 0) return hipSuccess;
     const int block = kBlock;
     const int64_t grid = (n + block - 1) / block;
     hipLaunchKernelGGL(propagate_kernel, dim3(static_cast<uint32_t>(grid)), dim3(block), 0, stream, sc, world_scope,
-                       in, n, max_interactions, seed, id_offset, out);
+                       in, n, max_interactions, seed, id_offset, out, ids);
     return hipGetLastError();
 }
 
 hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& out, const Workspace& ws, int grid_blocks,
-                                      int32_t trace_budget, hipStream_t stream, const CallParams* cp) {
+                                      int32_t trace_budget, hipStream_t stream, const CallParams* cp,
+                                      const int32_t* ids) {
     if (n <= 0) return hipSuccess;
     const int levels = static_cast<int>(max_interactions) + 1;
     if (levels + 1 > kDeferredCounts) return hipErrorInvalidValue;
@@ -326,7 +338,7 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
             hipLaunchKernelGGL(wf_resume_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, ws, level);
         }
         hipLaunchKernelGGL(wf_shade_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, sc, in, ws, level,
-                           max_interactions, seed, id_offset, out, cp);
+                           max_interactions, seed, id_offset, out, cp, ids);
     }
     return hipGetLastError();
 }

@@ -679,15 +679,15 @@ MLGPU_HD inline void photon_finish(const PhotonState& st, const mlgpu_results& o
 }
 
 // Propagator (merlict/PhotonAndFrame.cpp:6-124) for photon i of the batch,
-// all bounces in one lane.
+// all bounces in one lane, on the Philox stream of photon id `id`.
 MLGPU_HD inline void propagate_photon(const DeviceScene& sc, const int32_t world_scope, const mlgpu_photons& in,
                                       const int64_t i, const uint32_t max_interactions, const uint64_t seed,
-                                      const uint64_t id_offset, const mlgpu_results& out, const LaneStack stk) {
+                                      const uint64_t id, const mlgpu_results& out, const LaneStack stk) {
     PhotonState st;
     photon_init(st, in, i, world_scope);
     const double lambda = in.wavelength[i];
     Rng rng;
-    rng.init(seed, id_offset + static_cast<uint64_t>(i), 0u);
+    rng.init(seed, id, 0u);
     bool go = st.err == MLGPU_PHOTON_OK && st.n_hist <= max_interactions;
     while (go) {
         Closest c;

@@ -131,4 +131,13 @@ void set_last_error(const std::string& msg);
 // std::invalid_argument with the first problem found.
 void validate_desc(const mlgpu_scene_desc& d);
 
+// mlgpu_propagate and mlgpu_histogram_arrivals for a batch gathered from a
+// larger one: entry i is photon ids[i] of that batch, so it draws (and is
+// jittered) on the Philox stream of photon id offset + ids[i].  Null ids:
+// the public entry points.
+int propagate_gathered(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photons* photons, int64_t n,
+                       const mlgpu_config* cfg, mlgpu_results* r, const int32_t* ids, void* stream);
+int histogram_gathered(const mlgpu_results* r, int64_t n, const mlgpu_histogram* h, const int32_t* ids,
+                       void* stream);
+
 }  // namespace mlgpu

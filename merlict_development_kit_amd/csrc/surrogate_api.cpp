@@ -490,12 +490,19 @@ int mlgpu_propagate_hybrid(const mlgpu_scene* scene, mlgpu_surrogate* s, const m
         const mlgpu_photons fb{s->hy_photons.support_x, s->hy_photons.support_y, s->hy_photons.support_z,
                                s->hy_photons.direction_x, s->hy_photons.direction_y, s->hy_photons.direction_z,
                                s->hy_photons.wavelength};
+        // every fallback photon is propagated and binned under its own id
+        // (photon_id_offset + its index in the call), whatever slot the
+        // fallback list gave it: its draws and jitter are those of the same
+        // photon under mlgpu_propagate, and the histogram does not depend on
+        // the order in which the list filled
         mlgpu_config fcfg = *cfg;
         fcfg.photon_id_offset = cfg->photon_id_offset + static_cast<uint64_t>(c0);
-        if ((rc = mlgpu_propagate(scene, world_frame, &fb, k, &fcfg, &s->hy_results, stream))) return rc;
+        if ((rc = mlgpu::propagate_gathered(scene, world_frame, &fb, k, &fcfg, &s->hy_results, s->hy_fallback,
+                                            stream)))
+            return rc;
         mlgpu_histogram fh = *hist;
         fh.photon_id_offset = hist->photon_id_offset + static_cast<uint64_t>(c0);
-        if ((rc = mlgpu_histogram_arrivals(&s->hy_results, k, &fh, stream))) return rc;
+        if ((rc = mlgpu::histogram_gathered(&s->hy_results, k, &fh, s->hy_fallback, stream))) return rc;
     }
     if (num_fallback) *num_fallback = fallbacks;
     return MLGPU_OK;
