@@ -275,6 +275,22 @@ int mlgpu_propagate(const mlgpu_scene* scene, int32_t world_frame,
                     const mlgpu_config* config, mlgpu_results* results,
                     void* stream);
 
+/* mlgpu_propagate without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The wavefront
+ * workspace is a static device array of the library (about 58 MB, held on
+ * every GPU the library runs on until the process ends) instead of the
+ * scene's grown workspace; results equal mlgpu_propagate's bit for bit.
+ * At most MLGPU_NOALLOC_MAX_PHOTONS photons per call (larger n: an argument
+ * error).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other streams, other threads) race; calls on one stream are
+ * ordered and safe. */
+#define MLGPU_NOALLOC_MAX_PHOTONS 262144
+int mlgpu_propagate_noalloc(const mlgpu_scene* scene, int32_t world_frame,
+                            const mlgpu_photons* photons, int64_t n,
+                            const mlgpu_config* config,
+                            mlgpu_results* results, void* stream);
+
 /* ------------------------------------------------------------ photon sources
  * Synthetic photons drawn on the GPU from Philox4x32-10 stream 1 of
  * (seed, photon id) -- propagation uses stream 0 -- with only +, -, *, /

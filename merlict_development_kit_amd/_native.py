@@ -217,6 +217,8 @@ SIGNATURES = {
     "mlgpu_scene_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlgpu_propagate": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64, C.POINTER(Config),
                                   C.POINTER(Results), C.c_void_p]),
+    "mlgpu_propagate_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64,
+                                          C.POINTER(Config), C.POINTER(Results), C.c_void_p]),
     "mlgpu_generate_photons": (C.c_int, [C.POINTER(Source), C.c_int64, C.c_uint64, C.c_uint64,
                                          C.POINTER(PhotonBuffers), C.c_void_p]),
     "mlgpu_propagate_bunches": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Bunch), C.c_int32, C.POINTER(Config),

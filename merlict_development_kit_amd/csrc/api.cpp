@@ -538,6 +538,31 @@ int mlgpu_propagate(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photo
     return mlgpu::propagate_gathered(s, world_frame, photons, n, cfg, r, nullptr, stream);
 }
 
+int mlgpu_propagate_noalloc(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photons* photons, int64_t n,
+                            const mlgpu_config* cfg, mlgpu_results* r, void* stream) {
+    if (!s || !cfg || !r || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (n == 0) return MLGPU_OK;
+    if (!photons_ok(photons)) { set_last_error("null photon array"); return MLGPU_ERR_ARGUMENT; }
+    int rc;
+    if ((rc = check_results(r)) || (rc = check_world(s, world_frame)) || (rc = check_config(cfg))) return rc;
+    if (n > mlgpu::kNoAllocMaxPhotons) {
+        set_last_error("mlgpu_propagate_noalloc: at most MLGPU_NOALLOC_MAX_PHOTONS (262144) photons per call");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    void* mem = nullptr;
+    hipError_t e = mlgpu::noalloc_workspace(&mem);
+    if (e != hipSuccess) return hip_fail(e, "no-alloc workspace address");
+    // the wavefront schedule on the library's static workspace; the kernels
+    // clear their counters themselves, so nothing needs zeroing beforehand
+    const mlgpu::Workspace w = carve_workspace(mem, mlgpu::kNoAllocMaxPhotons);
+    e = mlgpu::launch_propagate_wavefront(s->dev, s->scope_of_frame[world_frame], *photons, n,
+                                          cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset, *r,
+                                          w, s->tuning.blocks_per_cu * s->num_cus, s->tuning.trace_budget,
+                                          static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "propagate launch");
+    return MLGPU_OK;
+}
+
 int mlgpu_propagate_bunches(const mlgpu_scene* s, int32_t world_frame, const mlgpu_bunch* bunches,
                             int32_t num_bunches, const mlgpu_config* cfg, const mlgpu_histogram* hist,
                             mlgpu_results* results, void* stream) {

@@ -42,11 +42,18 @@ constexpr int kDeferredCounts = 256;
 constexpr int kNumCounts = 2 * kDeferredCounts;
 
 // bytes of workspace for cap photons (each array 256-byte aligned)
-inline size_t workspace_bytes(int64_t cap) {
+constexpr size_t workspace_bytes(int64_t cap) {
     const size_t m = static_cast<size_t>(cap);
     const size_t per = 8 * 8 + 4 + 4 + 4 + 4 + 8 + 4 + 4 + 4 * kMaxTreeDepth + 4 + 4 + 4;
     return m * per + 4 * kNumCounts + 40 * 256;
 }
+
+// mlgpu_propagate_noalloc: the most photons per call, whose wavefront
+// workspace lives in a file-scope device array (propagate.hip), and the
+// device address of this GPU's copy of that array (looked up once per
+// device, with that device current).
+constexpr int64_t kNoAllocMaxPhotons = int64_t{1} << 18;
+hipError_t noalloc_workspace(void** ptr);
 
 // Photon i of a batch draws from Philox stream id id_offset + i, or
 // id_offset + ids[i] when ids is given (a batch gathered from a larger one).

@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "kernels.hpp"
 #include "propagate_core.hpp"
@@ -303,6 +304,33 @@ __global__ void count_errors_kernel(const uint8_t* err, int64_t n, unsigned long
 }
 
 }  // namespace
+
+// The wavefront workspace of mlgpu_propagate_noalloc (workspace_bytes of
+// kNoAllocMaxPhotons, about 58 MB): every GPU the library runs on holds its
+// own copy of this array until the process ends, and every call on that GPU
+// shares it.  Host code never uses this name as an address: it takes the
+// address hipGetSymbolAddress gives for the current device.
+static __device__ __align__(256) unsigned char scratch[workspace_bytes(kNoAllocMaxPhotons)];
+
+hipError_t noalloc_workspace(void** ptr) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mutex;
+    static void* address[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mutex);
+    if (!address[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        address[dev] = p;
+    }
+    *ptr = address[dev];
+    return hipSuccess;
+}
 
 hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in, int64_t n,
                             uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
