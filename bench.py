@@ -1,6 +1,6 @@
 """Benchmark: photons propagated per second on one MI355X.
 
-Workload: the FACT-like telescope (30-facet Davies-Cotton dish, f = 4.889 m,
+Workload: the FACT telescope (30-facet Davies-Cotton dish, f = 4.889 m,
 1440 hexagonal pixels of 9.5 mm pitch behind PMMA light guides) and a
 synthetic bunch of 1e7 photons per step, drawn on the GPU from Philox:
 supports uniform on a 4.5 m disc, x and y tilts N(0, 0.5 deg) about the
@@ -29,7 +29,7 @@ from merlict_development_kit_amd import GpuScene, PropagationConfig, Propagation
 
 METRIC = "photons propagated/sec on 1 MI355X, plain vs surrogate hybrid at ≤1% histogram L1"
 HBM_PEAK = 8.0e12
-# HBM bytes per photon of the default workload (FACT-like with light guides, 1e7
+# HBM bytes per photon of the default workload (FACT with light guides, 1e7
 # photons, default tuning), rocprofv3 FETCH_SIZE + WRITE_SIZE of one propagation
 # pass scaled by a 1 GiB copy: profiles/propagation_pass_fact_lg_1e7_counters.md
 TRAFFIC_BYTES_PER_PHOTON = 1494.0
@@ -179,7 +179,7 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
 
 def hybrid_stream_leg(scene: GpuScene, trained, dev, events: int = 4, bunches_per_event: int = 1024,
                       nsb_per_event: int = 100_000_000, chunk: int = 1 << 24) -> dict:
-    """The full-scale hybrid stream on the FACT-like telescope: per event
+    """The full-scale hybrid stream on the FACT telescope: per event
     1,024 shower-like bunches (Cherenkov rings of radius U[0.5, 1.3] deg,
     width 0.1 deg, sizes log-uniform in [1e3, 1e6], lambda ~ lambda^-2 on
     280-700 nm) and 1e8 night-sky photons (uniform within 2 deg of the axis,
@@ -251,7 +251,7 @@ def hybrid_stream_leg(scene: GpuScene, trained, dev, events: int = 4, bunches_pe
 
     return {
         "workload": f"{events} events x ({bunches_per_event} Cherenkov-ring bunches + {nsb_per_event:.0e} night-sky "
-                    "photons), FACT-like telescope with light guides",
+                    "photons), FACT telescope with light guides",
         "photons": int(total),
         "plain_photons_per_s": total / t_plain,
         "hybrid_photons_per_s": total / t_hy,
@@ -329,7 +329,7 @@ def calibration_leg(dev, cpu_threads: int, num_blocks: int = 8, photons_per_bloc
 
 
 def render_leg(dev, reps: int = 5, cpu: bool = True) -> dict:
-    """Images of the FACT-like telescope with light guides as merlict_visual
+    """Images of the FACT telescope with light guides as merlict_visual
     renders them: a 1920 x 1080 pin-hole camera 8 m above the dish looking
     down (the reference Config's snapshot size; the facets reflect most
     pixel rays on towards the camera housing and the sky), the Tracer with
@@ -355,7 +355,7 @@ def render_leg(dev, reps: int = 5, cpu: bool = True) -> dict:
         V.render(scene, camera(1920, 1080), cfg, colors, seed=r)
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / reps
-    out = {"workload": "FACT-like telescope with light guides, 1920 x 1080 pin-hole view from 8 m above, "
+    out = {"workload": "FACT telescope with light guides, 1920 x 1080 pin-hole view from 8 m above, "
                        "global illumination",
            "ms_per_image": sec * 1e3, "value": 1920 * 1080 / sec, "unit": "pixels/s"}
     if cpu and O.ref_available():
@@ -485,7 +485,7 @@ def signal_card() -> dict:
 
 
 def signal_leg(result, photons, dev, reps: int = 5) -> dict:
-    """The sensor signal chain on the detected photons of one FACT-like
+    """The sensor signal chain on the detected photons of one FACT
     propagation pass (1440 pixels) with the plenoscope example card's
     settings: pipelines (sorted by arrival time), night-sky background (the
     card's flux through a 2.1 m aperture over a 2.2 deg FoV radius, every
@@ -672,8 +672,8 @@ def main() -> int:
         "dtype": "f32 traversal, f64 leaf solves and physics",
         "data": "synthetic",
         "config": {
-            "workload": "FACT-like telescope with PMMA light guides (plain propagator)"
-            if not args.no_light_guides else "FACT-like telescope without light guides (plain propagator)",
+            "workload": "FACT telescope with PMMA light guides (plain propagator)"
+            if not args.no_light_guides else "FACT telescope without light guides (plain propagator)",
             "photons_per_step": n,
             "leaves": scene.stats()["leaves"],
             "max_num_interactions_per_photon": 5,

@@ -122,9 +122,13 @@ int mlgpu_desc_load(const char* path, mlgpu_desc** out);
 int mlgpu_build_lens_scene(double focal_length, double aperture_radius,
                            double refractive_index, double cell_pitch,
                            mlgpu_desc** out);
-/* FACT-like telescope: segmented_imaging_reflector (30 facets, f = 4.889 m),
- * 1440 hexagonal pixels of 9.5 mm pitch at the focal plane and a shield disc;
- * camera 1 adds a hexagonal PMMA light guide (15 mm) in front of each pixel;
+/* FACT telescope as merlict_fact/apps/fact_propagation.cpp:282-341 builds
+ * it: segmented_imaging_reflector (30 facets, f = 4.889 m) with the app's
+ * fact_mirror_reflectivity() table (fact_propagation.cpp:145-259) on the
+ * facets' outer side, a shield disc, and the 1440 hexagonal pixels of the
+ * app's pixel table at the focal plane, sensor id = FACT chid;
+ * camera 1 adds a hexagonal PMMA light guide (15 mm) in front of each pixel
+ * (entry hexagons as wide as the tightest pair of neighbouring pixels allows);
  * camera 0 has bare pixels; camera 2 replaces the camera by one absorbing
  * sensor disc (sensor 0, radius 0.33 m) at the guides' entrance plane, 15 mm
  * in front of the pixels: the same dish and shield, so a photon reaches the
@@ -502,15 +506,28 @@ int mlgpu_surrogate_prepare(const mlgpu_surrogate_domain* domain,
                             const double* focal_poses, int64_t n,
                             float* features, float* targets, void* stream);
 
-/* Sensor cells on a hexagonal lattice in the sensor plane: the cell whose
- * centre origin + i a1 + j a2 is nearest to a point, cell id
- * cell[(i + radius) * (2 radius + 1) + (j + radius)] (device; -1: none). */
+/* Sensor cells on a hexagonal lattice in the sensor plane.  Lattice site
+ * (i, j) is origin + i a1 + j a2 with cell id
+ * cell[(i + radius) * (2 radius + 1) + (j + radius)] (device; -1: none).
+ * Without centres (centre NULL) a point belongs to the cell of its nearest
+ * site.  With centres (device, 2 floats per cell id: the cell's own centre,
+ * which may sit off its site, e.g. the FACT camera's pixel table) a point
+ * belongs to the cell whose centre is nearest among the cells of its
+ * nearest site and that site's six neighbours whose hexagon holds it
+ * (inner radius inner_radius, flat sides facing a1 and a1 turned by 60 and
+ * 120 degrees; inner_radius 0: every one of them); a point in no such
+ * hexagon belongs to none.  Hexagons that do not overlap give cells with
+ * gaps between them; hexagons a little larger than half the widest spacing
+ * give the nearest-centre cells, closed inside the lattice and ending at
+ * its rim. */
 typedef struct mlgpu_hex_cells {
     double a1[2];
     double a2[2];
     double origin[2];
     int32_t radius;
     const int32_t* cell;
+    const float* centre;
+    double inner_radius;
 } mlgpu_hex_cells;
 
 /* The sensor cells of the surrogate's draws (mlgpu_propagate_hybrid: a

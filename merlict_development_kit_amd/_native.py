@@ -118,6 +118,8 @@ class HexCells(C.Structure):
         ("origin", C.c_double * 2),
         ("radius", C.c_int32),
         ("cell", C.c_void_p),
+        ("centre", C.c_void_p),
+        ("inner_radius", C.c_double),
     ]
 
 

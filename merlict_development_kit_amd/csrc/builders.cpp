@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "fact_tables.inc"
 #include "scene.hpp"
 
 #ifndef M_PI
@@ -323,11 +324,12 @@ int mlgpu_build_fact_scene(int camera, mlgpu_desc** out) {
     try {
         auto* d = new mlgpu_desc();
         const int32_t root = d->add_frame(-1, v3(0, 0, 0), Rot::xyz(0, 0, 0), MLGPU_KIND_FRAME);
-        // facet reflectivity of an aluminium + SiO2 mirror (synthetic table)
-        const std::vector<double> rx = {200e-9, 250e-9, 280e-9, 300e-9, 350e-9, 400e-9, 450e-9, 500e-9,
-                                        550e-9, 600e-9, 650e-9, 700e-9, 800e-9, 900e-9, 1000e-9, 1200e-9};
-        const std::vector<double> ry = {0.60, 0.78, 0.82, 0.84, 0.87, 0.885, 0.89, 0.885,
-                                        0.875, 0.865, 0.855, 0.845, 0.83, 0.85, 0.87, 0.90};
+        // the FACT mirror reflectivity (fact_propagation.cpp:145-259)
+        std::vector<double> rx, ry;
+        for (int k = 0; k < kFactMirrorReflectivityPoints; k++) {
+            rx.push_back(kFactMirrorReflectivity[k][0]);
+            ry.push_back(kFactMirrorReflectivity[k][1]);
+        }
         const int32_t mirror_fn = d->add_function(rx, ry);
         ReflectorConfig rc;
         rc.focal_length = 4.889;
@@ -342,22 +344,14 @@ int mlgpu_build_fact_scene(int camera, mlgpu_desc** out) {
         const int32_t cam = d->add_frame(root, v3(0, 0, 4.889), Rot::xyz(0, 0, 0), MLGPU_KIND_FRAME);
         d->add_frame(cam, v3(0, 0, 0.001), Rot::xyz(0, 0, 0), MLGPU_KIND_DISC, {0.33});
 
-        // 1440 hexagonal pixels on a 9.5 mm hexagonal grid, the cells
-        // closest to the optical axis (FACT-like camera).
-        const double pitch = 9.5e-3;
-        struct Cell { double x, y, r2, phi; };
-        std::vector<Cell> cand;
-        for (int j = -30; j <= 30; j++) {
-            for (int i = -30; i <= 30; i++) {
-                const double x = pitch * (i + 0.5 * j);
-                const double y = pitch * (std::sqrt(3.0) / 2.0) * j;
-                cand.push_back(Cell{x, y, std::round((x * x + y * y) * 1e12), std::atan2(y, x)});
-            }
-        }
-        std::sort(cand.begin(), cand.end(), [](const Cell& a, const Cell& b) {
-            if (a.r2 != b.r2) return a.r2 < b.r2;
-            return a.phi < b.phi;
-        });
+        // the 1440 pixels of the FACT camera in chid order
+        // (fact_propagation.cpp:316-341): the app's float millimetres widened
+        // to double and divided by 1e3, hexagons of outer radius 4.95 mm
+        // turned by 30 deg; sensor id = chid
+        std::vector<Vec3d> centre(kFactNumPixels);
+        for (int k = 0; k < kFactNumPixels; k++)
+            centre[k] = v3(static_cast<double>(kFactPixelXmm[k]) / 1e3, static_cast<double>(kFactPixelYmm[k]) / 1e3,
+                           0.0);
         const double pixel_r = 0.00495;
         const double guide_h = 0.015;
         if (camera == 2) {
@@ -367,7 +361,22 @@ int mlgpu_build_fact_scene(int camera, mlgpu_desc** out) {
             *out = d;
             return MLGPU_OK;
         }
-        const double entry_r = pitch / std::sqrt(3.0) * 0.995;
+        // light guides: entry hexagons (flat sides facing +-x, +-60 deg,
+        // +-120 deg, like the pixels) as wide as the tightest pair of
+        // neighbouring pixels allows, so that no two guides overlap
+        double min_gap = 1.0;
+        for (int a = 0; a < kFactNumPixels; a++)
+            for (int b = a + 1; b < kFactNumPixels; b++) {
+                const double dx = centre[b].x - centre[a].x, dy = centre[b].y - centre[a].y;
+                if (dx * dx + dy * dy > 4.0 * pixel_r * pixel_r * 4.0) continue;
+                double sep = 0.0;  // separation along the flat normals
+                for (int e = 0; e < 3; e++) {
+                    const double ang = e * M_PI / 3.0;
+                    sep = std::max(sep, std::fabs(dx * std::cos(ang) + dy * std::sin(ang)));
+                }
+                min_gap = std::min(min_gap, sep);
+            }
+        const double entry_r = 0.5 * min_gap / std::cos(M_PI / 6.0) * 0.995;
         int32_t n_index = -1, n_abs = -1;
         if (with_light_guides) {
             const std::vector<double> lx = wavelengths(200e-9, 1200e-9, 51);
@@ -376,8 +385,8 @@ int mlgpu_build_fact_scene(int camera, mlgpu_desc** out) {
             n_index = d->add_function(lx, ny);
             n_abs = d->add_function(lx, ay);
         }
-        for (int32_t k = 0; k < 1440; k++) {
-            const Vec3d p = v3(cand[k].x, cand[k].y, 0.0);
+        for (int32_t k = 0; k < kFactNumPixels; k++) {
+            const Vec3d& p = centre[k];
             if (!with_light_guides) {
                 d->add_frame(cam, p, Rot::xyz(0, 0, M_PI / 6.0), MLGPU_KIND_HEX_PLANE, {pixel_r},
                              Surface(), MLGPU_NO_FRAME, k);

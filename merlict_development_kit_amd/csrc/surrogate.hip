@@ -460,7 +460,59 @@ __global__ void prepare_kernel(const mlgpu_surrogate_domain dom, const mlgpu_pho
     }
 }
 
-// nearest lattice cell of a sensor-plane point (cube rounding), or -1
+// The cell of a sensor-plane point (mlgpu_hex_cells), or -1: the nearest
+// lattice site by cube rounding, then, with cell centres, among that site
+// and its six neighbours the cell whose centre is nearest of those whose
+// hexagon holds the point (every one for inner radius 0).
+template <typename T>
+__device__ __forceinline__ int32_t cell_at(const mlgpu_hex_cells& c, const int ii, const int jj) {
+    if (ii < -c.radius || ii > c.radius || jj < -c.radius || jj > c.radius) return -1;
+    return c.cell[(ii + c.radius) * (2 * c.radius + 1) + (jj + c.radius)];
+}
+
+// does the hexagon of inner radius r about (cx, cy), flat sides facing
+// n1, n2, n3, hold (x, y)?
+template <typename T>
+__device__ __forceinline__ bool in_cell_hexagon(const T x, const T y, const T cx, const T cy, const T* n, const T r) {
+    const T dx = x - cx, dy = y - cy;
+    const T u = dx * n[0] + dy * n[1], v = dx * n[2] + dy * n[3], w = dx * n[4] + dy * n[5];
+    return u < r && u > -r && v < r && v > -r && w < r && w > -r;
+}
+
+template <typename T>
+__device__ __forceinline__ int32_t cell_of_site(const mlgpu_hex_cells& c, const int ii, const int jj, const T x,
+                                                const T y, const T* n, const T r) {
+    if (!c.centre) return cell_at<T>(c, ii, jj);
+    constexpr int kNi[7] = {0, 1, -1, 0, 0, 1, -1};
+    constexpr int kNj[7] = {0, 0, 0, 1, -1, -1, 1};
+    int32_t best = -1;
+    T best_d2 = static_cast<T>(INFINITY);
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        const int32_t id = cell_at<T>(c, ii + kNi[q], jj + kNj[q]);
+        if (id < 0) continue;
+        const T cx = static_cast<T>(c.centre[2 * id]), cy = static_cast<T>(c.centre[2 * id + 1]);
+        if (r > static_cast<T>(0) && !in_cell_hexagon<T>(x, y, cx, cy, n, r)) continue;
+        const T d2 = (x - cx) * (x - cx) + (y - cy) * (y - cy);
+        if (d2 < best_d2) { best_d2 = d2; best = id; }
+    }
+    return best;
+}
+
+// flat-side normals of the cells: a1 turned by 0, 60 and 120 degrees
+template <typename T>
+__device__ __forceinline__ void cell_normals(const mlgpu_hex_cells& c, T* n) {
+    const double l = sqrt(c.a1[0] * c.a1[0] + c.a1[1] * c.a1[1]);
+    const double e0 = c.a1[0] / l, e1 = c.a1[1] / l;
+    const double h = 0.86602540378443865;  // sin 60
+    n[0] = static_cast<T>(e0);
+    n[1] = static_cast<T>(e1);
+    n[2] = static_cast<T>(0.5 * e0 - h * e1);
+    n[3] = static_cast<T>(h * e0 + 0.5 * e1);
+    n[4] = static_cast<T>(-0.5 * e0 - h * e1);
+    n[5] = static_cast<T>(h * e0 - 0.5 * e1);
+}
+
 __device__ int32_t hex_cell(const mlgpu_hex_cells& c, const double x, const double y) {
     const double px = x - c.origin[0], py = y - c.origin[1];
     const double det = c.a1[0] * c.a2[1] - c.a1[1] * c.a2[0];
@@ -471,16 +523,19 @@ __device__ int32_t hex_cell(const mlgpu_hex_cells& c, const double x, const doub
     const double di = fabs(ri - fi), dj = fabs(rj - fj), dk = fabs(rk - fk);
     if (di > dj && di > dk) ri = -rj - rk;
     else if (dj > dk) rj = -ri - rk;
-    const int ii = static_cast<int>(ri), jj = static_cast<int>(rj);
-    if (ii < -c.radius || ii > c.radius || jj < -c.radius || jj > c.radius) return -1;
-    return c.cell[(ii + c.radius) * (2 * c.radius + 1) + (jj + c.radius)];
+    double n[6];
+    if (c.centre) cell_normals<double>(c, n);
+    return cell_of_site<double>(c, static_cast<int>(ri), static_cast<int>(rj), x, y, n, c.inner_radius);
 }
 
 // float form of the draws' constants: the lattice's inverse basis, the
-// image-point and position scales
+// cells' flat-side normals and inner radius, the image-point and position
+// scales
 struct DrawConsts {
     float m00, m01, m10, m11;  // inverse of [a1 a2]
     float ox, oy;              // lattice origin
+    float n[6];                // flat-side normals of the cells
+    float r_in;                // their inner radius
     float sin_angle, image_scale, position_scale;
 };
 
@@ -493,6 +548,8 @@ __device__ __forceinline__ DrawConsts draw_consts(const SampleArgs& a) {
     k.m11 = static_cast<float>(a.cells.a1[0] / det);
     k.ox = static_cast<float>(a.cells.origin[0]);
     k.oy = static_cast<float>(a.cells.origin[1]);
+    cell_normals<float>(a.cells, k.n);
+    k.r_in = static_cast<float>(a.cells.inner_radius);
     k.sin_angle = static_cast<float>(sin(a.dom.max_angle));
     k.image_scale = static_cast<float>(a.dom.image_scale);
     k.position_scale = static_cast<float>(a.dom.position_scale);
@@ -510,9 +567,7 @@ __device__ __forceinline__ int32_t hex_cell_f(const mlgpu_hex_cells& c, const Dr
     const float di = fabsf(ri - fi), dj = fabsf(rj - fj), dk = fabsf(rk - fk);
     if (di > dj && di > dk) ri = -rj - rk;
     else if (dj > dk) rj = -ri - rk;
-    const int ii = static_cast<int>(ri), jj = static_cast<int>(rj);
-    if (ii < -c.radius || ii > c.radius || jj < -c.radius || jj > c.radius) return -1;
-    return c.cell[(ii + c.radius) * (2 * c.radius + 1) + (jj + c.radius)];
+    return cell_of_site<float>(c, static_cast<int>(ri), static_cast<int>(rj), x, y, k.n, k.r_in);
 }
 
 // uniform in (0, 1] from the top 24 bits of a word

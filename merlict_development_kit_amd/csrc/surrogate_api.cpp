@@ -249,6 +249,15 @@ int check_batch(const mlgpu_surrogate* s, const float* features, int64_t n) {
 
 }  // namespace
 
+namespace {
+
+// a lattice table, and a hexagon radius >= 0 (0: nearest centre)
+bool cells_ok(const mlgpu_hex_cells& c) {
+    return c.cell && c.radius >= 0 && c.inner_radius >= 0.0 && std::isfinite(c.inner_radius);
+}
+
+}  // namespace
+
 extern "C" {
 
 int64_t mlgpu_surrogate_num_params(void) { return kNumParams; }
@@ -418,7 +427,7 @@ int mlgpu_surrogate_set_region(mlgpu_surrogate* s, const mlgpu_surrogate_domain*
         s->region_on = false;
         return MLGPU_OK;
     }
-    if (!domain_ok(dom) || !cells->cell || cells->radius < 0) {
+    if (!domain_ok(dom) || !cells_ok(*cells)) {
         set_last_error("bad domain or cells");
         return MLGPU_ERR_ARGUMENT;
     }
@@ -445,7 +454,7 @@ int mlgpu_propagate_hybrid(const mlgpu_scene* scene, mlgpu_surrogate* s, const m
                            const mlgpu_hex_cells* cells, int32_t world_frame, const mlgpu_photons* ph, int64_t n,
                            const mlgpu_config* cfg, const mlgpu_hybrid_config* hc, const mlgpu_histogram* hist,
                            int64_t* num_fallback, void* stream) {
-    if (!scene || !s || !domain_ok(dom) || !cells || !cells->cell || cells->radius < 0 || !ph || !cfg || !hc ||
+    if (!scene || !s || !domain_ok(dom) || !cells || !cells_ok(*cells) || !ph || !cfg || !hc ||
         !hist || !hist->counts || n < 0) {
         set_last_error("bad domain or null argument");
         return MLGPU_ERR_ARGUMENT;

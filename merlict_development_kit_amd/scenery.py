@@ -40,8 +40,11 @@ class SceneDescription:
 
     @classmethod
     def fact(cls, light_guides: bool = True, focal_plane: bool = False):
-        """FACT-like telescope: 30-facet Davies-Cotton dish, 1440-pixel camera
-        (with PMMA light guides in front of the pixels, or bare pixels).
+        """FACT telescope as merlict's FACT propagation app builds it
+        (fact_propagation.cpp:282-341): 30-facet Davies-Cotton dish with
+        FACT's mirror reflectivity, the 1440 pixels of FACT's pixel table
+        (sensor id = chid), with PMMA light guides in front of the pixels or
+        bare pixels.
         focal_plane: the same dish with one sensor disc at the light guides'
         entrance plane instead of the camera (mlgpu_build_fact_scene)."""
         h = C.c_void_p()

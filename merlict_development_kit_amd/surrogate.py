@@ -159,7 +159,7 @@ class SurrogateDomain:
 
 
 def fact_domain() -> SurrogateDomain:
-    """The FACT-like telescope's domain: supports within 1.1 x the 2.1 m
+    """The FACT telescope's domain: supports within 1.1 x the 2.1 m
     mirror radius (fact_propagation.cpp:295), directions within 1.2 x the
     camera's 2.2 deg field radius, lambda in [280, 700] nm; positions
     relative to the nominal image point in units of 1 cm.  The image scale
@@ -168,11 +168,24 @@ def fact_domain() -> SurrogateDomain:
 
 
 class HexCells:
-    """mlgpu_hex_cells of a scene's sensors: their world centres (frame
-    poses) must sit on a hexagonal lattice; the lattice and the lookup table
-    of axial coordinates are derived from them."""
+    """mlgpu_hex_cells of a scene's sensors.  Their world centres (frame
+    poses) must lie near the sites of a hexagonal lattice (within a fifth of
+    a lattice step; the FACT camera's pixel table deviates by up to 0.4 mm
+    from a 9.5 mm lattice), one sensor per site; the cells are centred on
+    the sensors.  mode "hexagon": a cell is the hexagon a photon must enter
+    to reach its sensor, the entry hexagon of the light guide in front of a
+    pixel (a HexPlane sibling without a sensor id) or the bare pixel's own
+    HexPlane, and a photon in a gap between them reaches none.  mode
+    "nearest" (the default): a cell holds the points nearest to its centre
+    within a hexagon just wider than half the widest spacing of neighbouring
+    sensors, so the cells close every gap inside the camera (a photon that
+    falls between two light-guide entries meets a side wall and is refracted
+    into a guide) and end about half a pitch beyond its rim.  Sensors that
+    are not hexagons of one size get the cells of their lattice sites."""
 
-    def __init__(self, desc: SceneDescription, device="cuda"):
+    def __init__(self, desc: SceneDescription, device="cuda", mode: str = "nearest"):
+        if mode not in ("hexagon", "nearest"):
+            raise ValueError("mode must be 'hexagon' or 'nearest'")
         frames = desc.sensor_frames()
         poses = desc.frame_poses()
         centres = poses[frames][:, :2, 3]
@@ -180,9 +193,10 @@ class HexCells:
         dist = np.hypot(d[..., 0], d[..., 1])
         np.fill_diagonal(dist, np.inf)
         pitch = float(np.median(dist.min(axis=1)))
-        # first lattice vector: a nearest-neighbour offset of the most central cell
+        # first lattice vector: the nearest-neighbour offset of the most
+        # central cell at the smallest angle
         c0 = int(np.argmin(np.hypot(centres[:, 0], centres[:, 1])))
-        nb = np.nonzero(np.abs(dist[c0] - pitch) < 1e-3 * pitch)[0]
+        nb = np.nonzero(np.abs(dist[c0] - pitch) < 0.05 * pitch)[0]
         vecs = centres[nb] - centres[c0]
         ang = np.mod(np.arctan2(vecs[:, 1], vecs[:, 0]), 2 * np.pi)
         a1 = vecs[int(np.argmin(ang))]
@@ -192,15 +206,50 @@ class HexCells:
         A = np.stack([a1, a2], axis=1)
         ij = np.linalg.solve(A, (centres - origin).T).T
         rij = np.rint(ij)
-        if np.abs(ij - rij).max() > 1e-3:
-            raise ValueError("sensor centres are not on a hexagonal lattice")
+        if np.abs(ij - rij).max() > 0.2:
+            raise ValueError("sensor centres are not near the sites of a hexagonal lattice")
         radius = int(np.abs(rij).max())
         table = np.full((2 * radius + 1) ** 2, -1, dtype=np.int32)
         idx = ((rij[:, 0] + radius) * (2 * radius + 1) + (rij[:, 1] + radius)).astype(np.int64)
+        if len(np.unique(idx)) != len(idx):
+            raise ValueError("two sensors share a lattice site")
         table[idx] = np.arange(len(frames), dtype=np.int32)
         self.a1, self.a2, self.origin, self.radius = a1, a2, origin, radius
         self.table = torch.tensor(table, device=device)
         self.pitch = pitch
+        r_in = self._hexagon_inner_radius(desc, frames)
+        self.mode = mode if r_in > 0 else "lattice"
+        if self.mode == "nearest":
+            # half the widest separation of neighbouring centres along the
+            # hexagons' flat-side normals, plus 0.1 %
+            nrm = np.stack([a1, rot @ a1, rot @ rot @ a1]) / np.linalg.norm(a1)
+            close = (dist < 1.5 * pitch) & np.isfinite(dist)
+            sep = np.abs(np.einsum("ijk,lk->ijl", d, nrm)).max(axis=2)
+            r_in = 0.5 * float(sep[close].max()) * 1.001
+        self.inner_radius = r_in if self.mode != "lattice" else 0.0
+        self.centres = (torch.tensor(centres.astype(np.float32).reshape(-1), device=device)
+                        if self.mode != "lattice" else None)
+
+    @staticmethod
+    def _hexagon_inner_radius(desc: SceneDescription, frames: np.ndarray) -> float:
+        """Inner radius of the hexagon a photon enters to reach a sensor, the
+        same for every sensor (light-guide entries, else bare HexPlane
+        pixels), or 0."""
+        a = desc.arrays()
+        kind, mother, sensor, params = a["kind"], a["mother"], a["sensor_id"], a["params"]
+        hexplane = N.KIND["HexPlane"]
+        entry = (kind == hexplane) & (sensor < 0) & (mother >= 0)
+        count = np.zeros(len(kind), np.int64)
+        np.add.at(count, mother[entry], 1)
+        if np.all(count[mother[frames]] == 1):
+            first = np.full(len(kind), -1, np.int64)
+            first[mother[entry]] = np.nonzero(entry)[0]
+            outer = params[first[mother[frames]], 0]
+        elif np.all(kind[frames] == hexplane):
+            outer = params[frames, 0]
+        else:
+            return 0.0
+        return float(outer[0] * math.sqrt(3) / 2) if np.all(outer == outer[0]) else 0.0
 
     def c_struct(self) -> N.HexCells:
         c = N.HexCells()
@@ -208,6 +257,8 @@ class HexCells:
             c.a1[k], c.a2[k], c.origin[k] = float(self.a1[k]), float(self.a2[k]), float(self.origin[k])
         c.radius = self.radius
         c.cell = self.table.data_ptr()
+        c.centre = self.centres.data_ptr() if self.centres is not None else None
+        c.inner_radius = self.inner_radius
         return c
 
 

@@ -5,7 +5,7 @@ oracle/Makefile, with the Philox stream plugged into merlict's
 random::Generator) on seeded inputs and stores inputs and outputs as small
 .npz files.  Run in the container that has the reference mounted:
 
-    make -C oracle ref && python tests/golden/make_golden.py [plenoscope | zoo | corsika | signal | nsb | calibration | render]
+    make -C oracle ref && python tests/golden/make_golden.py [fact | plenoscope | zoo | corsika | signal | nsb | calibration | render]
 """
 from __future__ import annotations
 
@@ -54,6 +54,29 @@ def probe_rays(n, seed):
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     return {"support_x": sx, "support_y": sy, "support_z": sz, "direction_x": d[:, 0], "direction_y": d[:, 1],
             "direction_z": d[:, 2], "wavelength": np.full(n, 433e-9)}
+
+
+def fact_fixtures():
+    """The FACT telescope (pixel table and mirror reflectivity of the FACT
+    propagation app): propagation with bare pixels and with light guides,
+    the clustered tree, probe-ray first hits."""
+    fact = SceneDescription.fact(light_guides=False)
+    save_propagation("fact_propagation", fact, sources.fact_bunch(2000, seed=3).numpy(), seed=11)
+    fact_lg = SceneDescription.fact(light_guides=True)
+    save_propagation("fact_lg_propagation", fact_lg, sources.fact_bunch(2000, seed=5).numpy(), seed=13)
+    # deeper histories: a higher interaction limit on the light-guide camera
+    save_propagation("fact_lg_max10_propagation", fact_lg, sources.fact_bunch(1000, seed=6).numpy(), seed=17,
+                     max_int=10)
+
+    tree = O.ref_tree(fact)
+    np.savez_compressed(HERE / "fact_tree.npz", **tree)
+    print("fact tree", len(tree["frame"]))
+
+    rays = probe_rays(2000, seed=21)
+    hits = O.ref_first_hit(fact, rays)
+    np.savez_compressed(HERE / "fact_first_hit.npz", **{f"in_{k}": rays[k] for k in PHOTON_KEYS},
+                        out_frame=hits["frame"], out_distance=hits["distance"], out_position=hits["position"])
+    print("probe rays", len(rays["support_x"]), "hits", int((hits["frame"] >= 0).sum()))
 
 
 def plenoscope_fixtures():
@@ -256,25 +279,13 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "zoo":
         zoo_fixtures()
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "fact":
+        fact_fixtures()
+        render_fixtures()
+        return
     lens = SceneDescription.lens()
     save_propagation("lens_propagation", lens, sources.lens_beam(2000, seed=1).numpy(), seed=7)
-    fact = SceneDescription.fact(light_guides=False)
-    save_propagation("fact_propagation", fact, sources.fact_bunch(2000, seed=3).numpy(), seed=11)
-    fact_lg = SceneDescription.fact(light_guides=True)
-    save_propagation("fact_lg_propagation", fact_lg, sources.fact_bunch(2000, seed=5).numpy(), seed=13)
-    # deeper histories: a higher interaction limit on the light-guide camera
-    save_propagation("fact_lg_max10_propagation", fact_lg, sources.fact_bunch(1000, seed=6).numpy(), seed=17,
-                     max_int=10)
-
-    tree = O.ref_tree(fact)
-    np.savez_compressed(HERE / "fact_tree.npz", **tree)
-    print("fact tree", len(tree["frame"]))
-
-    rays = probe_rays(2000, seed=21)
-    hits = O.ref_first_hit(fact, rays)
-    np.savez_compressed(HERE / "fact_first_hit.npz", **{f"in_{k}": rays[k] for k in PHOTON_KEYS},
-                        out_frame=hits["frame"], out_distance=hits["distance"], out_position=hits["position"])
-    print("probe rays", len(rays["support_x"]), "hits", int((hits["frame"] >= 0).sum()))
+    fact_fixtures()
 
     fact_facets = O.ref_reflector(4.889, 0.5, 2.1, 0.25, 0, 0.3, 0.01)
     pleno_facets = O.ref_reflector(106.5, 0.0, 35.5, 3.5, 0, 0.75, 0.02)

@@ -10,7 +10,7 @@ from merlict_development_kit_amd import visual as V
 
 def render_cases():
     """(name, description, camera, global illumination, seed): the primitive
-    zoo, the FACT-like telescope from above, the lens bench without global
+    zoo, the FACT telescope from above, the lens bench without global
     illumination."""
     return [
         ("zoo", scene_zoo.zoo(), V.PinHoleCamera((0.0, -3.0, 3.0), (math.radians(135), 0.0, 0.0), math.radians(60),

@@ -24,7 +24,7 @@ def cuda():
 
 
 def test_gpu_render_matches_reference(cuda, golden):
-    """The zoo, the FACT-like telescope and the lens bench: every pixel's
+    """The zoo, the FACT telescope and the lens bench: every pixel's
     colour equals the reference Tracer's."""
     g = golden("render")
     for name, desc, cam, gi, seed in render_cases():
@@ -34,7 +34,7 @@ def test_gpu_render_matches_reference(cuda, golden):
 
 
 def test_gpu_render_matches_oracle_large(cuda):
-    """640 x 480 renders of the zoo and the FACT-like telescope from other
+    """640 x 480 renders of the zoo and the FACT telescope from other
     cameras: GPU and oracle agree on every pixel."""
     zoo_desc = render_cases()[0][1]
     fact_desc = SceneDescription.fact(light_guides=True)

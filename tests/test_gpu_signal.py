@@ -64,7 +64,7 @@ def test_gpu_signal_chain_matches_reference(cuda, golden, tmp_path):
 
 
 def test_gpu_pipelines_of_propagated_photons_match_oracle(cuda):
-    """Pipelines of 1e6 propagated FACT-like photons (1440 pixels, many
+    """Pipelines of 1e6 propagated FACT photons (1440 pixels, many
     photons per pixel): the GPU sort equals the oracle's bit for bit, and the
     converter agrees too."""
     gs = GpuScene(SceneDescription.fact(light_guides=True))
@@ -135,7 +135,7 @@ def test_gpu_nsb_matches_reference(cuda, golden):
 
 
 def test_gpu_nsb_of_propagated_pipelines_matches_oracle(cuda, golden):
-    """Background injected into the pipelines of 1e6 propagated FACT-like
+    """Background injected into the pipelines of 1e6 propagated FACT
     photons (1440 pixels, ~2,300 photons each, ~400 background photons per
     pixel): GPU and oracle agree bit for bit, and the converter runs on the
     result."""
@@ -216,7 +216,7 @@ def test_gpu_photon_stream_edges(cuda):
 
 
 def test_gpu_fact_photon_stream_matches_oracle(cuda):
-    """FACT's photon stream of the pulses of 1e6 propagated FACT-like photons
+    """FACT's photon stream of the pulses of 1e6 propagated FACT photons
     (converter with dark counts and crosstalk over 100 ns) and of a few
     hand-placed pulses: the GPU's bytes equal the oracle's."""
     from merlict_development_kit_amd.signal_chain import Pulses, fact_photon_stream

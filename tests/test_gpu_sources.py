@@ -56,7 +56,7 @@ def test_gpu_source_rejects_bad_parameters(cuda):
 
 
 def test_gpu_arrival_histograms_match_oracle(cuda):
-    """Arrival histograms of a propagated FACT-like bunch: the GPU counts equal
+    """Arrival histograms of a propagated FACT bunch: the GPU counts equal
     the oracle's binning of the same results, with and without jitter, for
     the LDS (1440 pixels x 8 bins) and the global-atomic (x 64 bins) paths."""
     from merlict_development_kit_amd import (ArrivalBinning, GpuScene, PropagationConfig, SceneDescription,

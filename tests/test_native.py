@@ -65,11 +65,36 @@ def test_scene_sizes():
     assert int((lg["kind"] == 10).sum()) == 1440 * 12          # two triangles per guide wall
 
 
-def test_pixel_grid_is_hexagonal_and_unique():
-    a = SceneDescription.fact(light_guides=False).arrays()
-    pix = a["pos"][a["sensor_id"] >= 0][:, :2]
-    assert len(np.unique(np.round(pix, 9), axis=0)) == 1440
-    # nearest-neighbour distance is the 9.5 mm pitch
-    from scipy.spatial import cKDTree
-    d, _ = cKDTree(pix).query(pix, k=2)
-    np.testing.assert_allclose(d[:, 1], 9.5e-3, rtol=1e-9)
+def test_fact_pixels_are_the_fact_camera_table():
+    """Sensor id = FACT chid at the FACT propagation app's pixel position
+    (fact_propagation.cpp:316-341: float millimetres / 1e3), spot-checked
+    against values of the app's table; positions unique, neighbours 9.19 to
+    9.5 mm apart (the camera's modules sit off a regular lattice)."""
+    for lg in (False, True):
+        a = SceneDescription.fact(light_guides=lg).arrays()
+        frames = np.nonzero(a["sensor_id"] >= 0)[0]
+        order = frames[np.argsort(a["sensor_id"][frames])]
+        holder = a["mother"][order] if lg else order
+        pix = a["pos"][holder][:, :2]
+        assert len(pix) == 1440 and np.array_equal(a["sensor_id"][order], np.arange(1440))
+        for chid, (x, y) in {0: (28.5, 172.805), 9: (-0.0, 172.805), 36: (114.0, 123.805),
+                             1439: (-4.75, 164.54)}.items():
+            assert pix[chid, 0] == float(np.float32(x)) / 1e3 and pix[chid, 1] == float(np.float32(y)) / 1e3, chid
+        assert len(np.unique(np.round(pix, 9), axis=0)) == 1440
+        from scipy.spatial import cKDTree
+        d, _ = cKDTree(pix).query(pix, k=2)
+        assert 9.18e-3 < d[:, 1].min() and d[:, 1].max() <= 9.5e-3 + 1e-12
+
+
+def test_fact_tables_match_the_reference_source():
+    """fact_tables.inc is what gen_fact_tables.py writes from the reference's
+    fact_propagation.cpp (skipped where the reference sources are absent)."""
+    import subprocess
+    import sys
+    src = Path("/root/reference/merlict_fact/apps/fact_propagation.cpp")
+    if not src.exists():
+        pytest.skip("reference sources absent")
+    csrc = Path(__file__).resolve().parent.parent / "merlict_development_kit_amd" / "csrc"
+    before = (csrc / "fact_tables.inc").read_text()
+    subprocess.run([sys.executable, str(csrc / "gen_fact_tables.py"), str(src)], check=True, capture_output=True)
+    assert (csrc / "fact_tables.inc").read_text() == before

@@ -243,11 +243,13 @@ def test_time_of_flight_through_lens(oracle_threads):
 def test_func1_limits_and_interpolation():
     d = SceneDescription.fact(light_guides=False)
     o = O.OracleScene(d)
-    # table 0 is the mirror reflectivity; lower bound inclusive, upper exclusive (Limits.cpp:37-46)
-    assert o.func_evaluate(0, 200e-9) == pytest.approx(0.60)
-    assert o.func_evaluate(0, 1200e-9) is None
+    # table 0 is FACT's mirror reflectivity (fact_propagation.cpp:145-259, 200 to
+    # 950 nm); lower bound inclusive, upper exclusive (Limits.cpp:37-46)
+    assert o.func_evaluate(0, 200e-9) == pytest.approx(0.5271)
+    assert o.func_evaluate(0, 950e-9) is None
+    assert o.func_evaluate(0, 949.9e-9) == pytest.approx(0.6634, abs=1e-4)
     assert o.func_evaluate(0, 199e-9) is None
-    assert o.func_evaluate(0, 425e-9) == pytest.approx(0.5 * (0.885 + 0.89))
+    assert o.func_evaluate(0, 207.5e-9) == pytest.approx(0.5 * (0.5988 + 0.6569))
     assert o.func_evaluate(-2, 433e-9) == 1.0  # DEFAULT_REFRACTION
 
 
@@ -737,7 +739,7 @@ def test_oracle_calibration_photons():
 # ------------------------------------------------------------------- rendering
 def test_oracle_render_matches_reference(golden):
     """PinHoleCamera + Tracer of the reference (merlict_visual, driven per
-    pixel with Philox stream 10) on the zoo, the FACT-like telescope and the
+    pixel with Philox stream 10) on the zoo, the FACT telescope and the
     lens bench: the oracle's colours are the same floats."""
     from render_cases import render_cases, render_colors
     from merlict_development_kit_amd import visual as V

@@ -326,24 +326,33 @@ def test_gpu_region_covering_every_draw_changes_nothing(cuda):
 
 @pytest.mark.gpu
 def test_gpu_region_mass_matches_monte_carlo(cuda):
-    """On the FACT-like camera's 1440 cells, what the region adds to the loss
+    """On the FACT camera's 1440 cells (nearest-centre cells of the pixel
+    table), what the region adds to the loss
     (over the photons not detected, -log(1 - sigmoid(s) Z) instead of
     -log(1 - sigmoid(s))) agrees with the same term from a Monte Carlo
     estimate of the mixture's mass Z on the cells, computed on the CPU from
     the network's outputs with a brute-force nearest-cell search (512 draws
     per component)."""
+    from scipy.spatial import cKDTree
+
     from merlict_development_kit_amd import SceneDescription
+    from merlict_development_kit_amd import surrogate as SG
     from merlict_development_kit_amd.surrogate import Surrogate
     desc = SceneDescription.fact(light_guides=True)
-    cent = desc.frame_poses()[desc.sensor_frames()][:, :2, 3]
-    pitch = 9.5e-3
-    a1, a2 = np.array([pitch, 0.0]), np.array([pitch / 2, pitch * np.sqrt(3) / 2])
-    ij = np.rint(np.linalg.solve(np.stack([a1, a2], axis=1), cent.T).T).astype(int)
-    r = int(np.abs(ij).max())
-    mask = np.zeros((2 * r + 1, 2 * r + 1), bool)
-    mask[ij[:, 0] + r, ij[:, 1] + r] = True
-    assert mask.sum() == 1440
-    lat = _Lattice(a1, a2, mask, cuda)
+    lat = SG.HexCells(desc, cuda)
+    cent = lat.centres.cpu().numpy().reshape(-1, 2).astype(np.float64)
+    tree = cKDTree(cent)
+    normals = np.stack([np.cos(np.radians([0.0, 60.0, 120.0])), np.sin(np.radians([0.0, 60.0, 120.0]))], axis=1)
+
+    def on_cells(x, y):
+        """brute force: some pixel's hexagon of the cells' inner radius holds the point"""
+        _, near = tree.query(np.stack([x.ravel(), y.ravel()], axis=1), k=3)
+        hit = np.zeros(x.size, bool)
+        for k in range(3):
+            rel = np.stack([x.ravel() - cent[near[:, k], 0], y.ravel() - cent[near[:, k], 1]], axis=1)
+            hit |= np.abs(rel @ normals.T).max(axis=1) < lat.inner_radius
+        return hit.reshape(x.shape)
+
     dom = _region_domain(-4.95)
     s = Surrogate(seed=6)
     p = _params(9)
@@ -375,7 +384,7 @@ def test_gpu_region_mass_matches_monte_carlo(cuda):
             e = rng.normal(size=(512, 2))
             x = px[b] + (mu[0] + sig[0] * e[:, 0]) * dom.position_scale
             y = py[b] + (mu[1] + sig[1] * e[:, 1]) * dom.position_scale
-            Z += pi[k] * lat.inside(x, y).mean()
+            Z += pi[k] * on_cells(x, y).mean()
         straddling += 0.05 < Z < 0.95
         sg, om = 1.0 / (1.0 + np.exp(-o[0])), 1.0 / (1.0 + np.exp(o[0]))
         total += -np.log(om + sg * (1.0 - Z)) + np.log(om)
@@ -416,7 +425,7 @@ def test_gpu_surrogate_training_lowers_the_loss(cuda):
 
 @pytest.mark.gpu
 def test_gpu_trained_surrogate_gains_over_untrained(cuda):
-    """What training buys: on the FACT-like telescope, the hybrid's error
+    """What training buys: on the FACT telescope, the hybrid's error
     against the plain propagator, per pixel and per pixel and time bin
     (means over 8 passes of 1e7 photons each), drops well below that of the
     untrained network, also with the trained network's inference in fp8.
