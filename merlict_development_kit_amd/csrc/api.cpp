@@ -396,7 +396,7 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
     if ((rc = upload(s, hs.pairs, &s->dev.pairs)) || (rc = upload(s, hs.scopes, &s->dev.scopes)) ||
         (rc = upload(s, hs.geo, &s->dev.geo)) || (rc = upload(s, hs.boxes, &s->dev.boxes)) ||
         (rc = upload(s, hs.phys, &s->dev.phys)) || (rc = upload(s, hs.funcs, &s->dev.funcs)) ||
-        (rc = upload(s, hs.points, &s->dev.points))) {
+        (rc = upload(s, hs.points, &s->dev.points)) || (rc = upload(s, hs.buckets, &s->dev.buckets))) {
         mlgpu_scene_destroy(s);
         return rc;
     }

@@ -99,11 +99,18 @@ struct alignas(16) LeafPhys {
 };
 
 // Func1 table: points sorted by x with the slope merlict computes before
-// sorting (merlict/function/Func1.cpp:25-36).
+// sorting (merlict/function/Func1.cpp:25-36), and a bucket index over
+// [lower, upper): bucket b = floor((x - lower) * scale) holds the segment
+// that contains the bucket's lower edge, and buckets are no wider than the
+// narrowest segment, so the segment of x is found in a step or two from
+// there instead of by a binary search.
 struct alignas(16) FuncHead {
     double lower, upper;  // valid arguments: lower <= x < upper
+    double scale;         // buckets per unit of x
     int32_t start, n;
     int32_t flags;        // FUNC_DEFAULT_* identity of merlict's defaults
+    int32_t bucket_start; // first bucket in DeviceScene::buckets
+    int32_t n_buckets;
     int32_t pad;
 };
 
@@ -128,6 +135,7 @@ struct DeviceScene {
     const LeafPhys* phys;
     const FuncHead* funcs;
     const FuncPoint* points;
+    const int32_t* buckets;  // segment index per bucket of every function
     int32_t num_pairs;
     int32_t num_leaves;
     int32_t root_scope;          // scope of frame 0

@@ -64,6 +64,7 @@ __global__ __launch_bounds__(kBlock) void propagate_kernel(const DeviceScene sc,
 // 1.5e5 atomics on one counter for 1e7 photons, which costs milliseconds.
 
 constexpr int kTile = 8 * kBlock;
+constexpr int kShadeTile = 8 * kBlock;  // 2, 4 and 8 blocks' worth measured alike
 
 template <int Cap>
 struct TileQueue {
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(kBlock) void wf_init_kernel(const int32_t world_sco
     __syncthreads();
     for (int64_t tile = static_cast<int64_t>(blockIdx.x) * kTile; tile < n;
          tile += static_cast<int64_t>(gridDim.x) * kTile) {
+        #pragma unroll 1
         for (int k = 0; k < kTile; k += kBlock) {
             const int64_t i = tile + k + threadIdx.x;
             bool live = false;
@@ -182,6 +184,7 @@ __global__ __launch_bounds__(kBlock) void wf_defer_kernel(const Workspace ws, co
     const int32_t count = ws.counts[level];
     for (int64_t tile = static_cast<int64_t>(blockIdx.x) * kTile; tile < count;
          tile += static_cast<int64_t>(gridDim.x) * kTile) {
+        #pragma unroll 1
         for (int k = 0; k < kTile; k += kBlock) {
             const int32_t slot = static_cast<int32_t>(tile + k + threadIdx.x);
             tile_push(q, slot < count && ws.tr_depth[slot] >= 0, slot);
@@ -224,15 +227,16 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
                                                           const int32_t* __restrict__ ids) {
     const uint64_t seed = cp ? cp->seed : seed_arg;
     const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
-    __shared__ TileQueue<kTile> q;
+    __shared__ TileQueue<kShadeTile> q;
     if (threadIdx.x == 0) q.n = 0;
     __syncthreads();
     const int32_t count = ws.counts[level];
     const int32_t* queue = ws.queue[level & 1];
     int32_t* next = ws.queue[(level + 1) & 1];
-    for (int64_t tile = static_cast<int64_t>(blockIdx.x) * kTile; tile < count;
-         tile += static_cast<int64_t>(gridDim.x) * kTile) {
-        for (int k = 0; k < kTile; k += kBlock) {
+    for (int64_t tile = static_cast<int64_t>(blockIdx.x) * kShadeTile; tile < count;
+         tile += static_cast<int64_t>(gridDim.x) * kShadeTile) {
+        #pragma unroll 1
+        for (int k = 0; k < kShadeTile; k += kBlock) {
             const int64_t slot = tile + k + threadIdx.x;
             bool go = false;
             int32_t i = 0;
@@ -354,6 +358,8 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
     hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(256), 0, stream, ws.counts, kNumCounts);
     const int64_t tiles = (n + kTile - 1) / kTile;
     const int tile_grid = static_cast<int>(tiles < grid_blocks ? tiles : grid_blocks);
+    const int64_t shade_tiles = (n + kShadeTile - 1) / kShadeTile;
+    const int shade_grid = static_cast<int>(shade_tiles < grid_blocks ? shade_tiles : grid_blocks);
     const int64_t blocks = (n + kBlock - 1) / kBlock;
     const int grid = static_cast<int>(blocks < grid_blocks ? blocks : grid_blocks);
     hipLaunchKernelGGL(wf_init_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, world_scope, in, n,
@@ -365,7 +371,7 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
             hipLaunchKernelGGL(wf_defer_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, ws, level);
             hipLaunchKernelGGL(wf_resume_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, ws, level);
         }
-        hipLaunchKernelGGL(wf_shade_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, sc, in, ws, level,
+        hipLaunchKernelGGL(wf_shade_kernel, dim3(shade_grid), dim3(kBlock), 0, stream, sc, in, ws, level,
                            max_interactions, seed, id_offset, out, cp, ids);
     }
     return hipGetLastError();

@@ -78,19 +78,23 @@ struct Rng {
 
 // ------------------------------------------------------------------- Func1
 // Func1::evaluate with Limits::assert_contains (merlict/function/Func1.cpp:66-71,
-// Limits.cpp:37-46); false when x is outside [lower, upper).
+// Limits.cpp:37-46); false when x is outside [lower, upper).  The segment is
+// the one upper_bound picks (x_k <= x < x_k+1, Func1.cpp:118-130), found from
+// the bucket index: the bucket's segment, moved down or up until it holds x
+// (a step at most when the rounding of the bucket put x next door).
 MLGPU_HD inline bool func_eval(const DeviceScene& sc, int32_t f, double x, double& y) {
     const FuncHead h = sc.funcs[f];
     if (!(x >= h.lower) || !(x < h.upper)) return false;
     if (h.flags & FUNC_DEFAULT_ABSORPTION) { y = HUGE_VAL; return true; }
     if (h.flags & FUNC_DEFAULT_REFRACTION) { y = 1.0; return true; }
     if (h.flags & FUNC_DEFAULT_REFLECTION) { y = 0.0; return true; }
-    int32_t lo = 0, hi = h.n;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (sc.points[h.start + mid].x > x) hi = mid; else lo = mid + 1;
-    }
-    const FuncPoint p0 = sc.points[h.start + lo - 1];
+    int32_t b = static_cast<int32_t>((x - h.lower) * h.scale);
+    b = b < 0 ? 0 : (b >= h.n_buckets ? h.n_buckets - 1 : b);
+    int32_t k = sc.buckets[h.bucket_start + b];
+    const FuncPoint* pt = sc.points + h.start;
+    while (k > 0 && pt[k].x > x) k--;
+    while (k + 2 < h.n && pt[k + 1].x <= x) k++;
+    const FuncPoint p0 = pt[k];
     y = p0.slope * x + p0.y - p0.slope * p0.x;
     return true;
 }
@@ -551,6 +555,13 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
         st.err = MLGPU_PHOTON_OUT_OF_TABLE;
         return false;
     }
+    // the refractive index on the side the photon comes from: Fresnel's n1
+    // and the index of the leg's time of flight (Photon.cpp:44-56)
+    double n_from;
+    if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION], lambda, n_from)) {
+        st.err = MLGPU_PHOTON_OUT_OF_TABLE;
+        return false;
+    }
     const double survival = exp(-t / ell);
     if (rng.uniform() > survival) {
         interaction = MLGPU_ABSORPTION_IN_VOID;
@@ -572,10 +583,9 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
                 interaction = MLGPU_ABSORPTION_ON_SURFACE;
             } else {
                 // fresnel_refraction_and_reflection (PhotonAndFrame.cpp:75-87), Fresnel.cpp:7-38
-                double n_from, n_to;
-                const int32_t f_from = ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION];
+                double n_to;
                 const int32_t f_to = ph.surf[outside_in ? MLGPU_INNER_REFRACTION : MLGPU_OUTER_REFRACTION];
-                if (!func_eval(sc, f_from, lambda, n_from) || !func_eval(sc, f_to, lambda, n_to)) {
+                if (!func_eval(sc, f_to, lambda, n_to)) {
                     st.err = MLGPU_PHOTON_OUT_OF_TABLE;
                     return false;
                 }
@@ -620,11 +630,6 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
         }
     }
     // push_back_intersection_and_interaction; Photon::time_of_flight (Photon.cpp:44-56)
-    double n_from;
-    if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION], lambda, n_from)) {
-        st.err = MLGPU_PHOTON_OUT_OF_TABLE;
-        return false;
-    }
     st.tof += n_from * t / kC;
     st.dist += t;
     st.n_hist++;

@@ -712,6 +712,24 @@ class Builder {
             fp.x = p.x; fp.y = p.y; fp.slope = p.slope; fp.pad = 0.0;
             hs_->points.push_back(fp);
         }
+        // bucket index: buckets no wider than the narrowest segment (at most
+        // 4096 per function; wider buckets only cost more steps), each
+        // pointing at the segment of its lower edge
+        double narrowest = INFINITY;
+        for (int i = 0; i + 1 < n; i++) narrowest = std::min(narrowest, pts[i + 1].x - pts[i].x);
+        const double range = h.upper - h.lower;
+        int32_t nb = 1;
+        if (n > 2 && std::isfinite(range) && range > 0.0)
+            nb = static_cast<int32_t>(std::min(4096.0, std::max(1.0, std::ceil(range / narrowest))));
+        h.n_buckets = nb;
+        h.scale = (range > 0.0 && std::isfinite(range)) ? nb / range : 0.0;
+        h.bucket_start = static_cast<int32_t>(hs_->buckets.size());
+        int32_t seg = 0;
+        for (int32_t b = 0; b < nb; b++) {
+            const double edge = h.lower + b / h.scale;
+            while (seg + 2 < n && pts[seg + 1].x <= edge) seg++;
+            hs_->buckets.push_back(n >= 2 ? seg : 0);
+        }
         hs_->funcs.push_back(h);
     }
 

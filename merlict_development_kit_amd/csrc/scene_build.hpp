@@ -21,6 +21,7 @@ struct HostScene {
     std::vector<int32_t> leaf_frame;
     std::vector<FuncHead> funcs;
     std::vector<FuncPoint> points;
+    std::vector<int32_t> buckets;
     std::vector<int32_t> scope_of_frame;  // per description frame, -1 if not a scope
     int32_t root_scope = -1;
     int32_t default_refraction = -1;
