@@ -42,38 +42,187 @@ FP32_PEAK, FP64_PEAK = 157.3e12, 78.6e12
 SIMDS, CLOCK_HZ = 256 * 4, 2.4e9
 IN_BYTES = 7 * 8            # support xyz, direction xyz, wavelength (float64)
 OUT_BYTES = 4 + 4 + 1 + 1 + 1 + 8 + 4 * 4 + 8  # frame, sensor, interaction, n, error, tof, x y cx cy, distance
+# algorithmic bytes per photon of the propagation: support, direction,
+# wavelength and id in (32 B as float32 / int32) and sensor id, time, x, y,
+# cx, cy, flags out (32 B)
+ALG_BYTES_PER_PHOTON = 64
+# floating-point operations per event of the reference's algorithm, counted
+# by the instrumented oracle on the bench's photons (oracle_work)
+OP_COSTS = {"sphere_tests": 20, "leaf_transform": 30, "leaf_cap": 60, "leaf_planar": 25, "leaf_cylinder": 40,
+            "leaf_sphere": 40, "fresnel": 40, "func_lookup": 8, "func_compare": 1}
 
 
-def cpu_baseline(desc: SceneDescription, photons, threads: int, target_seconds: float = 12.0) -> dict:
-    """The reference CPU path (oracle/_ref, compiled from the reference's own
-    sources) or, where it is absent, the oracle, on a bounded sample of the
-    same workload: the first photons of the bench's own bunch."""
+def strict_json(x):
+    """x with every non-finite float replaced by None (strict JSON)."""
+    if isinstance(x, dict):
+        return {k: strict_json(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [strict_json(v) for v in x]
+    if isinstance(x, float) and not math.isfinite(x):
+        return None
+    return x
+
+
+def algorithmic_work(desc: SceneDescription, photons, n: int = 20_000) -> dict:
+    """Work of the reference's algorithm per photon on the first n photons
+    of the bunch: the oracle's event counts (bounding-sphere tests, leaf
+    intersections by kind, each with its object-frame transform, Func1
+    lookups and their binary-search compares, Fresnel evaluations) times the
+    per-event operation costs OP_COSTS."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    n = min(n, len(photons))
+    w = O.OracleScene(desc).propagate(photons.slice(0, n).numpy(), seed=PARITY_SEED,
+                                      threads=min(8, os.cpu_count() or 1))["work"]
+    leaves = w["leaf_cap"] + w["leaf_planar"] + w["leaf_cylinder"] + w["leaf_sphere"]
+    flops = (OP_COSTS["sphere_tests"] * w["sphere_tests"] + OP_COSTS["leaf_transform"] * leaves +
+             sum(OP_COSTS[k] * w[k] for k in ("leaf_cap", "leaf_planar", "leaf_cylinder", "leaf_sphere")) +
+             OP_COSTS["fresnel"] * w["fresnel"] + OP_COSTS["func_lookup"] * w["func_lookups"] +
+             OP_COSTS["func_compare"] * w["func_compares"])
+    return {"flops_per_photon": flops / n, "events_per_photon": {k: v / n for k, v in w.items()},
+            "op_costs": OP_COSTS, "sample_photons": n}
+
+
+def host_cpus() -> dict:
+    """The host's cores as this process sees them: os.cpu_count(), the
+    scheduler affinity, the cgroup CPU quota (cores; None when unlimited)
+    and the CPU model."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = aff if quota is None else max(1, min(aff, int(math.floor(quota))))
+    return {"cpu_count": os.cpu_count(), "affinity": aff, "cgroup_quota_cores": quota, "usable": usable,
+            "model": model}
+
+
+def cpu_baseline(desc: SceneDescription, photons, threads: int, seed: int, target_seconds: float = 20.0,
+                 multi_thread_seconds: float = 8.0) -> dict:
+    """The reference CPU path (oracle/_ref: the reference's own sources
+    compiled by oracle/Makefile, driven by oracle/refharness.cpp) on the
+    host's usable cores, in both of its modes, on the bench's own photons:
+    chunked (contiguous ranges per thread, one Philox stream per photon, the
+    same streams as the GPU pass under `seed`; its per-photon results are
+    kept for the parity check) on as many of the photons as fit in about
+    `target_seconds`, and merlict_multi_thread's own driver
+    (merlict_multi_thread.cpp:28-55: one task and one Mt19937 per photon) on
+    a sample of about `multi_thread_seconds`.  Falls back to the oracle
+    restatement where the reference build is absent (kind "port")."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
 
+    out = {"unit": "photons/s", "cores": threads, "host": host_cpus()}
     with tempfile.TemporaryDirectory() as d:
         sp = Path(d) / "scene.bin"
         desc.save(str(sp))
-
-        def run(n):
-            ph = photons.slice(0, n).numpy()
-            if O.ref_available():
-                pp = Path(d) / "p.bin"
-                O.write_photons(pp, ph)
-                out = json.loads(O.ref_bench(sp, pp, threads, "chunked").strip().splitlines()[-1])
-                return out["seconds"], "reference"
+        if not O.ref_available():
+            n = min(len(photons), 200_000)
             t0 = time.perf_counter()
-            O.OracleScene(desc).propagate(ph, threads=threads)
-            return time.perf_counter() - t0, "port"
+            res = O.OracleScene(desc).propagate(photons.slice(0, n).numpy(), seed=seed, threads=threads)
+            sec = time.perf_counter() - t0
+            out.update({"value": n / sec, "kind": "port", "mode": "chunked", "results": res, "photons": n,
+                        "sample": f"the first {n} photons of the bench bunch, oracle restatement, {threads} threads, "
+                                  f"{sec:.1f} s"})
+            return out
 
-        n = 50_000
-        sec, kind = run(n)
-        n2 = int(min(len(photons), max(n, n * target_seconds / max(sec, 1e-3))))
-        sec, kind = run(n2)
-        return {"value": n2 / sec, "unit": "photons/s", "cores": threads, "kind": kind,
-                "sample": f"the first {n2} photons of the same bunch (seed 2024), "
-                          f"{threads} threads, contiguous chunks, one Philox stream per photon, "
-                          f"{sec:.1f} s wall"}
+        def run(n, mode, keep=False):
+            pp = Path(d) / "p.bin"
+            rp = Path(d) / "r.bin"
+            O.write_photons(pp, photons.slice(0, n).numpy())
+            line = O.ref_bench(sp, pp, threads, mode, seed, rp if keep else None)
+            info = json.loads(line.strip().splitlines()[-1])
+            return info["seconds"], (O.read_results(rp) if keep else None)
+
+        sec, _ = run(200_000, "chunked")
+        n = int(min(len(photons), max(200_000, 200_000 * target_seconds / max(sec, 1e-3))))
+        sec, res = run(n, "chunked", keep=True)
+        s1, _ = run(20_000, "multi_thread")
+        m = int(min(len(photons), max(20_000, 20_000 * multi_thread_seconds / max(s1, 1e-3))))
+        s_mt, _ = run(m, "multi_thread")
+        chunked, mt = n / sec, m / s_mt
+        out.update({
+            "value": max(chunked, mt), "kind": "reference",
+            "mode": "chunked" if chunked >= mt else "multi_thread",
+            "chunked": {"value": chunked, "photons": n, "seconds": sec, "threads": threads},
+            "multi_thread": {"value": mt, "photons": m, "seconds": s_mt,
+                             "threads": "std::thread::hardware_concurrency()"},
+            "results": res, "photons": n,
+            "sample": f"the first {n} photons of the bench bunch (seed 2024) chunked over {threads} threads "
+                      f"({sec:.1f} s), and the first {m} through merlict_multi_thread ({s_mt:.1f} s); value is "
+                      "the faster mode"})
+    return out
+
+
+PARITY_SEED = 77
+
+
+def scene_pixels(desc: SceneDescription) -> int:
+    return int(desc.arrays()["sensor_id"].max()) + 1
+
+
+def parity_check(scene, photons, ref: dict, seed: int, other_seed: int, num_pixels: int, dev) -> dict:
+    """The timed GPU path against the reference CPU path: per-photon final
+    frame and sensor on the same photons under the same Philox streams (the
+    flip rate; time of flight compared where both hit a surface), and, as two
+    independent samples of the same law (the GPU on as many fresh photons of
+    the bunch's source, drawn under another seed, propagated under seed
+    `other_seed`), the per-pixel counts and the arrival times of detected
+    photons by a two-sample chi-square test of the histograms and a
+    two-sample Kolmogorov-Smirnov test."""
+    from scipy import stats
+    n = len(ref["sensor_id"])
+    sub = photons.slice(0, n)
+    same = scene.propagate(sub, PropagationConfig(5, seed=seed)).numpy()
+    fresh = sources.fact_bunch().generate(n, seed=2025, photon_id_offset=n, device=dev)
+    other = scene.propagate(fresh, PropagationConfig(5, seed=other_seed)).numpy()
+    del fresh
+    torch.cuda.synchronize()
+    flips = (same["final_frame"] != ref["final_frame"]) | (same["sensor_id"] != ref["sensor_id"])
+    hit = ~flips & (ref["final_frame"] >= 0)
+    rel = np.abs(same["time_of_flight"][hit] - ref["time_of_flight"][hit]) / ref["time_of_flight"][hit]
+
+    def counts(r):
+        return np.bincount(r["sensor_id"][r["sensor_id"] >= 0], minlength=num_pixels).astype(np.float64)
+
+    a, b = counts(other), counts(ref)
+    keep = (a + b) > 0
+    # two-sample chi-square for equal totals expected under the same law
+    ka, kb = np.sqrt(b.sum() / a.sum()), np.sqrt(a.sum() / b.sum())
+    chi2 = float((((ka * a - kb * b)[keep]) ** 2 / (a + b)[keep]).sum())
+    dof = int(keep.sum()) - 1
+    ta = other["time_of_flight"][other["sensor_id"] >= 0]
+    tb = ref["time_of_flight"][ref["sensor_id"] >= 0]
+    edges = np.quantile(tb, np.linspace(0, 1, 65))
+    ha, _ = np.histogram(ta, edges)
+    hb, _ = np.histogram(tb, edges)
+    keep_t = (ha + hb) > 0
+    chi2_t = float((((ka * ha - kb * hb)[keep_t]) ** 2 / (ha + hb)[keep_t]).sum())
+    ks = stats.ks_2samp(ta, tb)
+    return {
+        "photons": int(n),
+        "same_streams": {"flips": int(flips.sum()), "flip_rate": float(flips.mean()),
+                         "max_rel_time_of_flight_diff": float(rel.max()) if rel.size else 0.0},
+        "independent_streams": {
+            "pixel_counts_chi2": chi2, "pixel_counts_dof": dof,
+            "pixel_counts_p": float(stats.chi2.sf(chi2, dof)),
+            "arrival_time_chi2": chi2_t, "arrival_time_dof": int(keep_t.sum()) - 1,
+            "arrival_time_p": float(stats.chi2.sf(chi2_t, int(keep_t.sum()) - 1)),
+            "arrival_time_ks": float(ks.statistic), "arrival_time_ks_p": float(ks.pvalue),
+            "detected_gpu": int(a.sum()), "detected_reference": int(b.sum())},
+        "reference": "oracle/_ref, the reference's own sources, chunked mode",
+    }
 
 
 def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, warmup: int, train_photons: int,
@@ -135,10 +284,41 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
         wall = time.perf_counter() - t0
         return hy, n * steps / wall, wall / steps * 1e3, fallback / (n * steps)
 
+    def domain_accuracy(passes: int = 10, n_domain: int = 10_000_000) -> dict:
+        """The surrogate on fresh photons spanning its whole training domain
+        (supports uniform within 1.1 x the mirror radius, directions
+        uniform within 2.64 deg, lambda ~ lambda^-2 on 280-700 nm): passes x
+        n_domain photon-propagations through the hybrid and through the
+        plain propagator under other seeds than training, per-pixel and
+        per-pixel-and-time L1, noise floor from a second set of plain
+        passes."""
+        ph = dom.source(10.0).generate(n_domain, seed=4242, photon_id_offset=10 ** 10, device=dev)
+
+        def plain_d(s0):
+            acc = binning.empty(dev)
+            for s in range(passes):
+                histogram_arrivals(scene.propagate(ph, PropagationConfig(5, seed=s0 + s)), binning, counts=acc)
+            return acc.cpu().numpy().astype(np.float64)
+
+        ref, again = plain_d(2000), plain_d(3000)
+        acc = binning.empty(dev)
+        fbk = 0
+        for s in range(passes):
+            _, f = SG.propagate_hybrid(scene, sur, dom, cells, ph, binning, PropagationConfig(5, seed=2000 + s),
+                                       max_entropy=max_entropy, seed=2000 + s, counts=acc)
+            fbk += f
+        hyd = acc.cpu().numpy().astype(np.float64)
+        return {"photons": passes * n_domain, "pixel_l1": l1(hyd.sum(axis=1), ref.sum(axis=1)),
+                "pixel_time_l1": l1(hyd, ref), "noise_floor_pixel_l1": l1(again.sum(axis=1), ref.sum(axis=1)),
+                "noise_floor_pixel_time_l1": l1(again, ref), "detected_ratio": float(hyd.sum() / ref.sum()),
+                "fallback_fraction": fbk / (passes * n_domain),
+                "meets_1pct_l1": l1(hyd.sum(axis=1), ref.sum(axis=1)) <= 0.01}
+
     flops = 2 * 212_480  # multiply-adds of the network per photon, twice
     sur.inference = "bf16"
     hy, value, ms, fb = measure()
     cell_l1 = l1(hy.sum(axis=1), plain.sum(axis=1))
+    c4 = domain_accuracy()
     sur.inference = "fp8"
     hy8, value8, ms8, fb8 = measure()
     cell_l1_8 = l1(hy8.sum(axis=1), plain.sum(axis=1))
@@ -148,7 +328,7 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
         "value": value,
         "unit": "photons/s",
         "ms_per_step": ms,
-        "max_entropy": max_entropy,
+        "max_entropy": max_entropy if math.isfinite(max_entropy) else None,  # None: the bound is off
         "fallback_fraction": fb,
         "accuracy_passes": accuracy_passes,
         "pixel_l1": cell_l1,
@@ -156,6 +336,7 @@ def hybrid_leg(desc: SceneDescription, scene: GpuScene, photons, steps: int, war
         "noise_floor_pixel_l1": l1(again.sum(axis=1), plain.sum(axis=1)),
         "noise_floor_pixel_time_l1": l1(again, plain),
         "meets_1pct_l1": cell_l1 <= 0.01,
+        "domain_wide": c4,
         "trained_photons": train_photons,
         "train_seconds": train_s,
         "network": "MLP 5-256-256-256-256-57, bf16 on MFMA 32x32x16, fp32 accumulation",
@@ -322,9 +503,10 @@ def calibration_leg(dev, cpu_threads: int, num_blocks: int = 8, photons_per_bloc
            "lixels_seen": int((st["efficiency"] > 0).sum()),
            "mean_efficiency": float(st["efficiency"].astype(np.float64).mean())}
     if cpu:
-        out["cpu_baseline"] = cpu_baseline(desc, ph, cpu_threads, target_seconds=10.0)
-        out["cpu_baseline"]["sample"] = out["cpu_baseline"]["sample"].replace(
-            "of the same bunch (seed 2024)", "of a calibration block (seed 3)")
+        cb = cpu_baseline(desc, ph, cpu_threads, seed=3, target_seconds=10.0, multi_thread_seconds=0.0)
+        cb.pop("results", None)
+        cb["sample"] = cb["sample"].replace("of the bench bunch (seed 2024)", "of a calibration block (seed 3)")
+        out["cpu_baseline"] = cb
     return out
 
 
@@ -594,7 +776,8 @@ def main() -> int:
     ap.add_argument("--photons", type=int, default=10_000_000)
     ap.add_argument("--no-light-guides", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="host threads of the CPU baseline (default: every usable core of this process)")
     ap.add_argument("--schedule", default="auto", choices=["auto", "fused", "wavefront"])
     ap.add_argument("--trace-budget", type=int, default=None)
     ap.add_argument("--blocks-per-cu", type=int, default=None)
@@ -614,6 +797,8 @@ def main() -> int:
                     help="first copy a float32 buffer of this size (a plain copy kernel of known "
                          "traffic, for scaling rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.cpu_threads is None:
+        args.cpu_threads = host_cpus()["usable"]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the photon propagator runs on a single MI355X; "
               "sharding photons or bunches across GPUs is not implemented.", file=sys.stderr)
@@ -655,7 +840,7 @@ def main() -> int:
     detected = int((out.sensor_id >= 0).sum().item())
     errors = scene.count_errors(out)
     value = n * args.steps / wall
-    achieved = n * (IN_BYTES + OUT_BYTES) / (ms_kernel * 1e-3)
+    achieved = n * ALG_BYTES_PER_PHOTON / (ms_kernel * 1e-3)
     default_workload = (n == 10_000_000 and not args.no_light_guides and args.trace_budget is None
                         and args.blocks_per_cu is None and args.schedule == "auto" and not args.autotune)
     line = {
@@ -692,12 +877,31 @@ def main() -> int:
             "kernel": "propagation pass: the kernels of one mlgpu_propagate call on one stream "
                       "(wavefront: wf_init, then per level wf_trace, wf_defer, wf_resume, wf_shade)",
             "kernel_ms": ms_kernel,
-            "algorithmic_bytes_per_photon": IN_BYTES + OUT_BYTES,
+            "algorithmic_bytes_per_photon": ALG_BYTES_PER_PHOTON,
+            "interface_bytes_per_photon": IN_BYTES + OUT_BYTES,
+            "interface_frac": n * (IN_BYTES + OUT_BYTES) / (ms_kernel * 1e-3) / HBM_PEAK,
+            "measured_limiter": "neither HBM nor VALU throughput: the traversal kernels issue their "
+                                "instructions at low lane utilisation (divergent walks) and wait on "
+                                "dependent node and leaf fetches (profiles/propagation_pass_fact_lg_1e7_counters.md)",
         },
     }
+    sec = ms_kernel * 1e-3
+    alg = algorithmic_work(desc, photons)
+    line["valu"] = {
+        "algorithmic_flops_per_photon": alg["flops_per_photon"],
+        "algorithmic_tflops": alg["flops_per_photon"] * n / sec / 1e12,
+        "algorithmic_frac_of_fp32_peak": alg["flops_per_photon"] * n / sec / FP32_PEAK,
+        "events_per_photon": alg["events_per_photon"],
+        "op_costs": alg["op_costs"],
+        "source": f"the oracle's event counts of the reference's algorithm on the first {alg['sample_photons']} "
+                  "photons of the bunch, times op_costs, over the measured pass time",
+    }
     if default_workload:
-        sec = ms_kernel * 1e-3
-        line["valu"] = {
+        line["valu"]["issued"] = {
+            "fp32_flops_per_photon": VALU_FP32_FLOPS_PER_PHOTON,
+            "fp64_flops_per_photon": VALU_FP64_FLOPS_PER_PHOTON,
+            "issued_over_algorithmic":
+                (VALU_FP32_FLOPS_PER_PHOTON + VALU_FP64_FLOPS_PER_PHOTON) / alg["flops_per_photon"],
             "fp32_tflops": VALU_FP32_FLOPS_PER_PHOTON * n / sec / 1e12,
             "fp32_frac": VALU_FP32_FLOPS_PER_PHOTON * n / sec / FP32_PEAK,
             "fp64_tflops": VALU_FP64_FLOPS_PER_PHOTON * n / sec / 1e12,
@@ -726,9 +930,12 @@ def main() -> int:
     if not args.no_corsika:
         line["corsika_ingestion"] = corsika_leg(dev)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(desc, photons, args.cpu_threads)
-        line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
-    print(json.dumps(line), flush=True)
+        cb = cpu_baseline(desc, photons, args.cpu_threads, seed=PARITY_SEED)
+        ref = cb.pop("results")
+        line["cpu_baseline"] = cb
+        line["gpu_over_cpu"] = value / cb["value"]
+        line["parity"] = parity_check(scene, photons, ref, PARITY_SEED, PARITY_SEED + 1, scene_pixels(desc), dev)
+    print(json.dumps(strict_json(line), allow_nan=False), flush=True)
     return 0
 
 

@@ -154,8 +154,18 @@ static int func1_init(func1* f, const double* x, const double* y, int n) {
     return 0;
 }
 
+/* the work counts of the calling worker thread (NULL: not counted) */
+static __thread oracle_work* g_work = NULL;
+static __thread oracle_work g_last_work;
+
 /* Func1::evaluate (Func1.cpp:66-71,118-130), Limits::assert_contains */
 static int func1_eval(const func1* f, double x, double* out) {
+    if (g_work) {
+        int c = 0;
+        while ((1 << c) < f->n) c++;
+        g_work->func_lookups++;
+        g_work->func_compares += c;
+    }
     if (x < f->lower || x >= f->upper) return 1;
     /* std::upper_bound: first point with P.x > x */
     int lo = 0, hi = f->n;
@@ -705,6 +715,16 @@ static int intersect_sphere(const oframe* f, int fi, v3 s, v3 d, isect* out) {
 }
 
 static int intersect_leaf(const oframe* f, int fi, v3 s, v3 d, isect* out) {
+    if (g_work) {
+        switch (f->kind) {
+        case MLGPU_KIND_SPHERE_CAP_HEX:
+        case MLGPU_KIND_SPHERE_CAP_CYL:
+        case MLGPU_KIND_SPHERE_CAP_RECT: g_work->leaf_cap++; break;
+        case MLGPU_KIND_CYLINDER: g_work->leaf_cylinder++; break;
+        case MLGPU_KIND_SPHERE: g_work->leaf_sphere++; break;
+        default: g_work->leaf_planar++;
+        }
+    }
     switch (f->kind) {
     case MLGPU_KIND_SPHERE_CAP_HEX:
     case MLGPU_KIND_SPHERE_CAP_CYL:
@@ -746,6 +766,7 @@ typedef struct {
 static void dfs(query* q, int fi) {
     const oframe* f = &q->s->f[fi];
     q->n_sphere++;
+    if (g_work) g_work->sphere_tests++;
     if (!sphere_hit(f, q->sup, q->dir)) return;
     if (f->n_children > 0) {
         for (int i = 0; i < f->n_children; i++) dfs(q, f->children[i]);
@@ -761,6 +782,7 @@ static void dfs(query* q, int fi) {
 }
 
 static isect first_hit(const oracle_scene* s, int root, v3 sup, v3 dir, int64_t* ns, int64_t* nl) {
+    if (g_work) g_work->queries++;
     query q;
     q.s = s; q.sup = sup; q.dir = dir; q.have = 0; q.n_sphere = 0; q.n_leaf = 0;
     dfs(&q, root);
@@ -1047,6 +1069,7 @@ static void propagate_one(const oracle_scene* s, int world, v3 sup, v3 dir, doub
 
     /* Propagator::propagate loop (PhotonAndFrame.cpp:13-21) */
     while (!err && (uint32_t)n_hist <= max_int) {
+        if (g_work) g_work->bounces++;
         isect is = first_hit(s, root, sup, dir, ns, nl);
         int interaction;
         int stop = 1;
@@ -1097,6 +1120,7 @@ static void propagate_one(const oracle_scene* s, int world, v3 sup, v3 dir, doub
                         const v3 inc = ht_orient_inv(&f->t2w, dir);
                         const v3 nrm = is.outside_in ? is.normal : mul(is.normal, -1.0);
                         /* Fresnel.cpp:7-38 */
+                        if (g_work) g_work->fresnel++;
                         const double cosI = dot(nrm, inc) * (-1.0);
                         const double eta = n_from / n_to;
                         const double sinT2 = eta * eta * (1.0 - cosI * cosI);
@@ -1199,18 +1223,23 @@ typedef struct {
     uint64_t seed, id0;
     oracle_results* out;
     int64_t ns, nl;
+    oracle_work wk;
 } work;
 
 static void* worker(void* arg) {
     work* w = (work*)arg;
+    g_work = &w->wk;
     for (int64_t i = w->begin; i < w->end; i++) {
         v3 sup = V(w->ph->support_x[i], w->ph->support_y[i], w->ph->support_z[i]);
         v3 dir = V(w->ph->direction_x[i], w->ph->direction_y[i], w->ph->direction_z[i]);
         propagate_one(w->s, w->world, sup, dir, w->ph->wavelength[i], w->max_int, w->seed,
                       w->id0 + (uint64_t)i, i, w->out, &w->ns, &w->nl);
     }
+    g_work = NULL;
     return NULL;
 }
+
+void oracle_last_work(oracle_work* out) { *out = g_last_work; }
 
 int oracle_propagate(const oracle_scene* s, int32_t world, const mlgpu_photons* ph, int64_t n,
                      uint32_t max_int, uint64_t seed, uint64_t id0, oracle_results* out,
@@ -1235,7 +1264,16 @@ int oracle_propagate(const oracle_scene* s, int32_t world, const mlgpu_photons* 
         for (int t = 0; t < num_threads; t++) pthread_join(th[t], NULL);
     }
     int64_t ns = 0, nl = 0;
-    for (int t = 0; t < num_threads; t++) { ns += w[t].ns; nl += w[t].nl; }
+    memset(&g_last_work, 0, sizeof(g_last_work));
+    for (int t = 0; t < num_threads; t++) {
+        ns += w[t].ns; nl += w[t].nl;
+        const oracle_work* k = &w[t].wk;
+        g_last_work.queries += k->queries; g_last_work.sphere_tests += k->sphere_tests;
+        g_last_work.leaf_cap += k->leaf_cap; g_last_work.leaf_planar += k->leaf_planar;
+        g_last_work.leaf_cylinder += k->leaf_cylinder; g_last_work.leaf_sphere += k->leaf_sphere;
+        g_last_work.func_lookups += k->func_lookups; g_last_work.func_compares += k->func_compares;
+        g_last_work.fresnel += k->fresnel; g_last_work.bounces += k->bounces;
+    }
     if (sphere_tests) *sphere_tests = ns;
     if (leaf_tests) *leaf_tests = nl;
     free(w);

@@ -69,6 +69,22 @@ typedef struct oracle_results {
 /* Propagate photons (host arrays, double SoA).  One Philox generator per
  * photon, as merlict_multi_thread.cpp:12-26 uses one Mt19937 per photon.
  * num_threads <= 1 runs single threaded; otherwise contiguous chunks. */
+/* Work counts of the reference's algorithm over one oracle_propagate call,
+ * per event (the instrumented restatement behind the VALU roofline):
+ * first-intersection queries, bounding-sphere tests, leaf intersections by
+ * kind, Func1 lookups (and the sum over them of ceil(log2(points)), the
+ * compares of their binary search), Fresnel evaluations. */
+typedef struct oracle_work {
+    int64_t queries;
+    int64_t sphere_tests;
+    int64_t leaf_cap, leaf_planar, leaf_cylinder, leaf_sphere;
+    int64_t func_lookups, func_compares;
+    int64_t fresnel;
+    int64_t bounces;
+} oracle_work;
+/* the counts of the last oracle_propagate call on the calling thread */
+void oracle_last_work(oracle_work* out);
+
 int oracle_propagate(const oracle_scene* s, int32_t world_frame,
                      const mlgpu_photons* photons, int64_t n,
                      uint32_t max_num_interactions, uint64_t seed,

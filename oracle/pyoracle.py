@@ -175,6 +175,9 @@ class OracleScene:
             out["distance"][i] = dist.value
             out["position"][i] = pos
         out["sphere_tests"], out["leaf_tests"] = st.value, lt.value
+        w = _Work()
+        lib().oracle_last_work(C.byref(w))
+        out["work"] = {name: int(getattr(w, name)) for name, _ in _Work._fields_}
         return out
 
     def propagate(self, photons: dict, max_num_interactions: int = 5, seed: int = 0, photon_id_offset: int = 0,
@@ -199,12 +202,22 @@ class OracleScene:
         if rc:
             raise RuntimeError(lib().oracle_last_error().decode())
         out["sphere_tests"], out["leaf_tests"] = st.value, lt.value
+        w = _Work()
+        lib().oracle_last_work(C.byref(w))
+        out["work"] = {name: int(getattr(w, name)) for name, _ in _Work._fields_}
         return out
 
     def func_evaluate(self, f: int, x: float):
         y = C.c_double()
         rc = lib().oracle_func_evaluate(self._h, f, x, C.byref(y))
         return (None if rc else y.value)
+
+
+class _Work(C.Structure):
+    """oracle_work: event counts of the reference's algorithm (oracle.h)"""
+    _fields_ = [(name, C.c_int64) for name in (
+        "queries", "sphere_tests", "leaf_cap", "leaf_planar", "leaf_cylinder", "leaf_sphere", "func_lookups",
+        "func_compares", "fresnel", "bounces")]
 
 
 def generate_photons(src: Source, n: int, seed: int = 0, photon_id_offset: int = 0) -> dict:
@@ -418,8 +431,11 @@ def ref_lightfield(f, aperture, fov_deg, pixel_fov_deg, n_paxel, lens_n, bin_ref
     return meta, rows
 
 
-def ref_bench(scene_path, photons_path, threads, mode, seed=0) -> str:
-    return _run_ref(["bench", scene_path, photons_path, threads, mode, seed])
+def ref_bench(scene_path, photons_path, threads, mode, seed=0, results_path=None) -> str:
+    args = ["bench", scene_path, photons_path, threads, mode, seed]
+    if results_path is not None:
+        args.append(results_path)
+    return _run_ref(args)
 
 
 # ------------------------------------------------------------ signal chain

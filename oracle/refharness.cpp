@@ -839,7 +839,9 @@ static int cmd_lightfield(int argc, char** argv) {
     return 0;
 }
 
-// bench <scene> <photons> <threads> <mode: chunked|multi_thread> <seed>
+// bench <scene> <photons> <threads> <mode: chunked|multi_thread> <seed> [results]
+// (chunked: one Philox stream per photon, ids 0 .. n-1, max 5 interactions;
+// with a results path the per-photon results are written as by propagate)
 static int cmd_bench(int argc, char** argv) {
     if (argc < 7) return 2;
     Desc d = load_desc(argv[2]);
@@ -881,6 +883,14 @@ static int cmd_bench(int argc, char** argv) {
     for (int64_t i = 0; i < p.n; i++) detected += r.sensor[i] >= 0;
     std::printf("{\"photons\": %lld, \"seconds\": %.6f, \"threads\": %d, \"mode\": \"chunked\", \"detected\": %lld}\n",
                 static_cast<long long>(p.n), s, threads, static_cast<long long>(detected));
+    if (argc > 7) {
+        std::ofstream o(argv[7], std::ios::binary);
+        o.write("MLGPURS2", 8);
+        o.write(reinterpret_cast<const char*>(&p.n), 8);
+        wr(o, r.frame); wr(o, r.sensor); wr(o, r.interaction); wr(o, r.n_int); wr(o, r.err);
+        wr(o, r.tof); wr(o, r.x); wr(o, r.y); wr(o, r.cx); wr(o, r.cy); wr(o, r.acc);
+        if (!o) return 1;
+    }
     return 0;
 }
 
