@@ -63,7 +63,12 @@ enum mlgpu_kind {
 
 enum mlgpu_rot_mode {
     MLGPU_ROT_XYZ = 0,   /* rot[0..2] = Tait-Bryan x, y, z angles (Rot3)   */
-    MLGPU_ROT_AXIS = 1   /* rot[0..2] = axis, rot[3] = angle              */
+    MLGPU_ROT_AXIS = 1,  /* rot[0..2] = axis, rot[3] = angle              */
+    MLGPU_ROT_AXIS_STORED = 2 /* rot[0..2] = the axis as a Rot3 holds it
+                           (Rot3::rot_axis: already divided by its norm by
+                           Rot3::set, Rot3.cpp:25-33), rot[3] = angle; only
+                           HomTra3's own normalisation follows
+                           (HomTra3.cpp:67-70) */
 };
 
 /* surface slots, same order as the SurfaceEntity members */
@@ -294,6 +299,47 @@ int mlgpu_propagate_noalloc(const mlgpu_scene* scene, int32_t world_frame,
                             const mlgpu_photons* photons, int64_t n,
                             const mlgpu_config* config,
                             mlgpu_results* results, void* stream);
+
+/* One entry of a photon's history after its production: the arguments of
+ * the Intersection that RayForPropagation::push_back_intersection_and_
+ * interaction appends (merlict/Intersection.h:24-29, RayForPropagation.h:
+ * 47-50) and the interaction.  A photon absorbed in void has the void
+ * intersection of RayAndFrame.cpp:160-167: frame -1, position the world
+ * point 1e4 m along the ray, normal the ray direction, distance 1e4. */
+typedef struct mlgpu_bounce {
+    double position[3];   /* Intersection::position_in_object_frame        */
+    double normal[3];     /* surface normal in the object frame            */
+    double distance;      /* Intersection::distance_to_ray_support         */
+    int32_t frame;        /* frame of the intersection, -1 void            */
+    uint8_t interaction;  /* mlgpu_interaction                             */
+    uint8_t from_outside; /* Intersection::from_outside_to_inside          */
+    uint8_t reserved[2];
+} mlgpu_bounce;
+
+/* Device arrays of the histories of n photons. */
+typedef struct mlgpu_history {
+    mlgpu_bounce* bounces; /* [n * capacity]: photon i's entries from
+                              i * capacity on, in order                     */
+    int32_t capacity;      /* >= config->max_num_interactions_per_photon    */
+    int32_t* num_bounces;  /* [n] entries written; a photon with an error
+                              keeps those before the bounce that failed     */
+    double* final_ray;     /* [n * 6] or NULL: support and direction of the
+                              photon's last ray (world frame), as
+                              Ray::support / Ray::direction read after
+                              merlict's propagation                         */
+} mlgpu_history;
+
+/* mlgpu_propagate that also records every photon's full history, so a
+ * caller can rebuild merlict::Photon objects bounce for bounce (the drop-in
+ * of propagate_photons_in_frame_with_config needs them: Photon::
+ * time_of_flight and Sensor::assign_photon read the history).  Same results
+ * and Philox streams as mlgpu_propagate; one lane per photon.  Asynchronous
+ * on `stream`. */
+int mlgpu_propagate_history(const mlgpu_scene* scene, int32_t world_frame,
+                            const mlgpu_photons* photons, int64_t n,
+                            const mlgpu_config* config,
+                            mlgpu_results* results,
+                            const mlgpu_history* history, void* stream);
 
 /* ------------------------------------------------------------ photon sources
  * Synthetic photons drawn on the GPU from Philox4x32-10 stream 1 of

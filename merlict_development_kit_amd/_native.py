@@ -192,6 +192,16 @@ class Tuning(C.Structure):
     ]
 
 
+class History(C.Structure):
+    _fields_ = [("bounces", C.c_void_p), ("capacity", C.c_int32), ("num_bounces", C.c_void_p),
+                ("final_ray", C.c_void_p)]
+
+
+# numpy layout of mlgpu_bounce (64 bytes)
+BOUNCE_FIELDS = [("position", "<f8", (3,)), ("normal", "<f8", (3,)), ("distance", "<f8"), ("frame", "<i4"),
+                 ("interaction", "u1"), ("from_outside", "u1"), ("reserved", "u1", (2,))]
+
+
 class Hits(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in ("frame", "distance", "position_x", "position_y", "position_z")]
 
@@ -221,6 +231,8 @@ SIGNATURES = {
                                   C.POINTER(Results), C.c_void_p]),
     "mlgpu_propagate_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64,
                                           C.POINTER(Config), C.POINTER(Results), C.c_void_p]),
+    "mlgpu_propagate_history": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Photons), C.c_int64,
+                                          C.POINTER(Config), C.POINTER(Results), C.POINTER(History), C.c_void_p]),
     "mlgpu_generate_photons": (C.c_int, [C.POINTER(Source), C.c_int64, C.c_uint64, C.c_uint64,
                                          C.POINTER(PhotonBuffers), C.c_void_p]),
     "mlgpu_propagate_bunches": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Bunch), C.c_int32, C.POINTER(Config),

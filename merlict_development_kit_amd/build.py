@@ -76,10 +76,13 @@ def build_library(verbose: bool = False) -> Path:
 
 
 def build_oracle() -> None:
-    """Build oracle/liboracle.so and, when /root/reference exists, oracle/_ref."""
+    """Build oracle/liboracle.so and, when /root/reference exists, oracle/_ref:
+    the reference build, and the reference-side GPU binding (facade/) with its
+    check against the reference (needs libmlgpu.so, so after build_library)."""
     _run(["make", "-C", str(ROOT / "oracle"), "-j8", "oracle"])
     if Path("/root/reference").is_dir():
         _run(["make", "-C", str(ROOT / "oracle"), "-j8", "ref"])
+        _run(["make", "-C", str(ROOT / "oracle"), "facade"])
 
 
 def build(verbose: bool = False) -> None:

@@ -563,6 +563,26 @@ int mlgpu_propagate_noalloc(const mlgpu_scene* s, int32_t world_frame, const mlg
     return MLGPU_OK;
 }
 
+int mlgpu_propagate_history(const mlgpu_scene* s, int32_t world_frame, const mlgpu_photons* photons, int64_t n,
+                            const mlgpu_config* cfg, mlgpu_results* r, const mlgpu_history* h, void* stream) {
+    if (!s || !cfg || !r || !h || n < 0) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
+    if (n == 0) return MLGPU_OK;
+    if (!photons_ok(photons)) { set_last_error("null photon array"); return MLGPU_ERR_ARGUMENT; }
+    int rc;
+    if ((rc = check_results(r)) || (rc = check_world(s, world_frame)) || (rc = check_config(cfg))) return rc;
+    if (!h->bounces || !h->num_bounces) { set_last_error("null history array"); return MLGPU_ERR_ARGUMENT; }
+    if (h->capacity < 1 || static_cast<uint32_t>(h->capacity) < cfg->max_num_interactions_per_photon) {
+        set_last_error("history capacity below max_num_interactions_per_photon");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (n > INT32_MAX) { set_last_error("at most 2^31 - 1 photons per call"); return MLGPU_ERR_ARGUMENT; }
+    const hipError_t e = mlgpu::launch_propagate_history(
+        s->dev, s->scope_of_frame[world_frame], *photons, n, cfg->max_num_interactions_per_photon, cfg->seed,
+        cfg->photon_id_offset, *r, *h, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "propagate launch");
+    return MLGPU_OK;
+}
+
 int mlgpu_propagate_bunches(const mlgpu_scene* s, int32_t world_frame, const mlgpu_bunch* bunches,
                             int32_t num_bunches, const mlgpu_config* cfg, const mlgpu_histogram* hist,
                             mlgpu_results* results, void* stream) {

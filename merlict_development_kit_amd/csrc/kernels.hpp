@@ -61,6 +61,11 @@ hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const ml
                             uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                             const mlgpu_results& out, hipStream_t stream, const int32_t* ids = nullptr);
 
+// the fused schedule recording every photon's history (capacity >= max_interactions)
+hipError_t launch_propagate_history(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in, int64_t n,
+                                    uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
+                                    const mlgpu_results& out, const mlgpu_history& hist, hipStream_t stream);
+
 // n (and seed, id_offset) size the grids; with cp, the kernels take the
 // photon count, seed and id offset from *cp (n is then the capacity)
 hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,

@@ -50,6 +50,22 @@ __global__ __launch_bounds__(kBlock) void propagate_kernel(const DeviceScene sc,
     propagate_photon(sc, world_scope, in, i, max_interactions, seed, photon_id(id_offset, ids, i), out, stk);
 }
 
+// The fused schedule with every photon's history recorded (one lane per
+// photon; the entries of photon i from i * capacity on).
+__global__ __launch_bounds__(kBlock) void propagate_history_kernel(const DeviceScene sc, const int32_t world_scope,
+                                                                   const mlgpu_photons in, const int64_t n,
+                                                                   const uint32_t max_interactions,
+                                                                   const uint64_t seed, const uint64_t id_offset,
+                                                                   const mlgpu_results out,
+                                                                   const mlgpu_history hist) {
+    MLGPU_LANE_STACK(stk);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    propagate_photon_history(sc, world_scope, in, i, max_interactions, seed, photon_id(id_offset, nullptr, i), out,
+                             stk, hist.bounces + i * hist.capacity, hist.num_bounces + i,
+                             hist.final_ray ? hist.final_ray + 6 * i : nullptr);
+}
+
 // ------------------------------------------------------- wavefront schedule
 // Per bounce level: trace the rays of the live photons (traversal + FP64 leaf
 // solves, hit record (leaf, t) per queue slot), then shade them (physics);
@@ -344,6 +360,16 @@ hipError_t launch_propagate(const DeviceScene& sc, int32_t world_scope, const ml
     const int64_t grid = (n + block - 1) / block;
     hipLaunchKernelGGL(propagate_kernel, dim3(static_cast<uint32_t>(grid)), dim3(block), 0, stream, sc, world_scope,
                        in, n, max_interactions, seed, id_offset, out, ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_propagate_history(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in, int64_t n,
+                                    uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
+                                    const mlgpu_results& out, const mlgpu_history& hist, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int64_t grid = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(propagate_history_kernel, dim3(static_cast<uint32_t>(grid)), dim3(kBlock), 0, stream, sc,
+                       world_scope, in, n, max_interactions, seed, id_offset, out, hist);
     return hipGetLastError();
 }
 

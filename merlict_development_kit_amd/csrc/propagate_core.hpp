@@ -511,22 +511,49 @@ MLGPU_HD inline void photon_init(PhotonState& st, const mlgpu_photons& in, const
     st.out_cy = 0.0;
 }
 
+// The history entry RayForPropagation::push_back_intersection_and_interaction
+// appends (RayForPropagation.cpp): the Intersection's constructor arguments
+// (Intersection.cpp:14-27) and the interaction.
+MLGPU_HD inline void record_bounce(mlgpu_bounce* const hist, const uint32_t k, const D3& position, const D3& normal,
+                                   const double distance, const int32_t frame, const uint8_t interaction,
+                                   const bool from_outside) {
+    mlgpu_bounce& b = hist[k];
+    b.position[0] = position.x;
+    b.position[1] = position.y;
+    b.position[2] = position.z;
+    b.normal[0] = normal.x;
+    b.normal[1] = normal.y;
+    b.normal[2] = normal.z;
+    b.distance = distance;
+    b.frame = frame;
+    b.interaction = interaction;
+    b.from_outside = from_outside ? 1 : 0;
+    b.reserved[0] = 0;
+    b.reserved[1] = 0;
+}
+
 // One pass of Propagator::work_on_first_causal_intersection and what follows
 // (merlict/PhotonAndFrame.cpp:23-124) for the closest hit (leaf, t) of the
 // current ray (leaf < 0: none).  Returns true when the photon propagates on.
+// hist (optional): the photon's history entries after the production; the
+// bounce is written to hist[n_hist - 1] when it is pushed.
 MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const int32_t leaf, const double t,
-                                   const double lambda, Rng& rng, const uint32_t max_interactions) {
+                                   const double lambda, Rng& rng, const uint32_t max_interactions,
+                                   mlgpu_bounce* const hist = nullptr) {
     if (leaf < 0) {
         // no hit: absorbed in void at 1e4 m, n = 1 on that leg
         double nv;
         if (!func_eval(sc, sc.default_refraction, lambda, nv)) { st.err = MLGPU_PHOTON_OUT_OF_TABLE; return false; }
         st.tof += nv * kVoidDistance / kC;
         st.dist += kVoidDistance;
+        const D3 e = at(st.s, st.d, kVoidDistance);
+        // the void intersection of CausalIntersection::calculate_closest_intersection
+        // (RayAndFrame.cpp:160-167): world point at 1e4 m, normal = direction
+        if (hist) record_bounce(hist, st.n_hist - 1, e, st.d, kVoidDistance, -1, MLGPU_ABSORPTION_IN_VOID, false);
         st.n_hist++;
         st.last_interaction = MLGPU_ABSORPTION_IN_VOID;
         st.last_frame = -1;
         st.last_sensor = -1;
-        const D3 e = at(st.s, st.d, kVoidDistance);
         st.out_x = e.x;
         st.out_y = e.y;
         const D3 inc = normalized(D3{e.x - st.s.x, e.y - st.s.y, e.z - st.s.z});
@@ -632,6 +659,7 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
     // push_back_intersection_and_interaction; Photon::time_of_flight (Photon.cpp:44-56)
     st.tof += n_from * t / kC;
     st.dist += t;
+    if (hist) record_bounce(hist, st.n_hist - 1, hp, n, t, ph.frame, interaction, outside_in);
     st.n_hist++;
     st.last_interaction = interaction;
     st.last_frame = ph.frame;
@@ -700,6 +728,38 @@ MLGPU_HD inline void propagate_photon(const DeviceScene& sc, const int32_t world
         go = photon_bounce(sc, st, c.leaf, c.t, lambda, rng, max_interactions);
     }
     photon_finish(st, out, i);
+}
+
+// propagate_photon that also writes the photon's history: its bounces to
+// bounces[0, num_bounces), the count, and the last ray (support, direction;
+// RayForPropagation keeps the ray of the last leg) to final_ray[6] if given.
+MLGPU_HD inline void propagate_photon_history(const DeviceScene& sc, const int32_t world_scope,
+                                              const mlgpu_photons& in, const int64_t i,
+                                              const uint32_t max_interactions, const uint64_t seed,
+                                              const uint64_t id, const mlgpu_results& out, const LaneStack stk,
+                                              mlgpu_bounce* const bounces, int32_t* const num_bounces,
+                                              double* const final_ray) {
+    PhotonState st;
+    photon_init(st, in, i, world_scope);
+    const double lambda = in.wavelength[i];
+    Rng rng;
+    rng.init(seed, id, 0u);
+    bool go = st.err == MLGPU_PHOTON_OK && st.n_hist <= max_interactions;
+    while (go) {
+        Closest c;
+        first_hit(sc, st.root, st.s, st.d, c, stk);
+        go = photon_bounce(sc, st, c.leaf, c.t, lambda, rng, max_interactions, bounces);
+    }
+    photon_finish(st, out, i);
+    *num_bounces = static_cast<int32_t>(st.n_hist) - 1;
+    if (final_ray) {
+        final_ray[0] = st.s.x;
+        final_ray[1] = st.s.y;
+        final_ray[2] = st.s.z;
+        final_ray[3] = st.d.x;
+        final_ray[4] = st.d.y;
+        final_ray[5] = st.d.z;
+    }
 }
 
 }  // namespace mlgpu

@@ -59,7 +59,10 @@ M34 pose(const double* pos, int mode, const double* rot) {
         R.T[2][2] = cx * cy;
     } else {
         double ax = rot[0], ay = rot[1], az = rot[2];
-        for (int k = 0; k < 2; k++) {
+        // Rot3::set divides the axis by its norm, HomTra3 once more; a stored
+        // axis has had the first division already
+        const int divisions = mode == MLGPU_ROT_AXIS_STORED ? 1 : 2;
+        for (int k = 0; k < divisions; k++) {
             const double nn = std::sqrt(ax * ax + ay * ay + az * az);
             ax /= nn; ay /= nn; az /= nn;
         }

@@ -29,7 +29,8 @@ void validate_desc(const mlgpu_scene_desc& d) {
             throw std::invalid_argument("frame " + std::to_string(i) + ": mother must precede child");
         if (d.kind[i] < 0 || d.kind[i] >= MLGPU_NUM_KINDS)
             throw std::invalid_argument("frame " + std::to_string(i) + ": unknown kind");
-        if (d.rot_mode[i] != MLGPU_ROT_XYZ && d.rot_mode[i] != MLGPU_ROT_AXIS)
+        if (d.rot_mode[i] != MLGPU_ROT_XYZ && d.rot_mode[i] != MLGPU_ROT_AXIS &&
+            d.rot_mode[i] != MLGPU_ROT_AXIS_STORED)
             throw std::invalid_argument("frame " + std::to_string(i) + ": unknown rotation mode");
         if (!finite3(d.pos + 3 * i)) throw std::invalid_argument("frame " + std::to_string(i) + ": position not finite");
         for (int s = 0; s < MLGPU_NUM_SURFACE_SLOTS; s++) {

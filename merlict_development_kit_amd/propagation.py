@@ -188,6 +188,30 @@ class GpuScene:
                                         _stream_ptr(stream)), "mlgpu_propagate")
         return out
 
+    def propagate_history(self, photons: PhotonBatch, config: PropagationConfig = PropagationConfig(),
+                          world_frame: int = 0, stream: Optional[torch.cuda.Stream] = None):
+        """propagate() that also records each photon's history after its
+        production (mlgpu_propagate_history): returns (result, history) with
+        history = {"bounces": uint8 (n, capacity, 64) mlgpu_bounce records
+        (decode_bounces reads them), "num_bounces": int32 (n),
+        "final_ray": float64 (n, 6) last support and direction}."""
+        n = len(photons)
+        if photons.device.type != "cuda":
+            raise ValueError("photons must be on the GPU")
+        dev = photons.device
+        cap = max(1, int(config.max_num_interactions_per_photon))
+        out = PropagationResult.empty(n, dev)
+        hist = {
+            "bounces": torch.empty((n, cap, 64), dtype=torch.uint8, device=dev),
+            "num_bounces": torch.empty(n, dtype=torch.int32, device=dev),
+            "final_ray": torch.empty((n, 6), dtype=torch.float64, device=dev),
+        }
+        h = N.History(hist["bounces"].data_ptr(), cap, hist["num_bounces"].data_ptr(), hist["final_ray"].data_ptr())
+        ph, rs, cfg = photons.c_struct(), out.c_struct(), config.c_struct()
+        N.check(N.lib().mlgpu_propagate_history(self._h, world_frame, C.byref(ph), n, C.byref(cfg), C.byref(rs),
+                                                C.byref(h), _stream_ptr(stream)), "mlgpu_propagate_history")
+        return out, hist
+
     def count_errors(self, result: PropagationResult, stream=None) -> int:
         rs = result.c_struct()
         k = C.c_int64()
@@ -228,3 +252,14 @@ def propagate_photons_in_frame_with_config(photons: PhotonBatch, world: GpuScene
             raise ValueError("photon wavelength must be > 0")
         raise IndexError("photon wavelength outside a surface function's range")
     return res
+
+
+def decode_bounces(bounces: torch.Tensor):
+    """mlgpu_bounce records (uint8, last dimension 64) as a numpy structured
+    array with fields position (3), normal (3), distance, frame, interaction,
+    from_outside."""
+    import numpy as np
+    raw = bounces.detach().cpu().contiguous().numpy()
+    if raw.shape[-1] != 64:
+        raise ValueError("expected 64-byte mlgpu_bounce records")
+    return raw.view(np.dtype(N.BOUNCE_FIELDS))[..., 0]

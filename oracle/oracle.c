@@ -82,8 +82,9 @@ static homtra ht_from_pos_rot(v3 pos, int rot_mode, const double* rot) {
         R.T[2][1] = -sinRx * cosRy;
         R.T[2][2] = cosRx * cosRy;
     } else {
-        axis = dvd(axis, norm(axis));  /* Rot3::set normalises          */
-        axis = dvd(axis, norm(axis));  /* set_rotation_..._rot_axis again */
+        if (rot_mode != MLGPU_ROT_AXIS_STORED)
+            axis = dvd(axis, norm(axis)); /* Rot3::set normalises          */
+        axis = dvd(axis, norm(axis));     /* set_rotation_..._rot_axis again */
         const double x = axis.x, y = axis.y, z = axis.z;
         const double sinR = sin(angle), cosR = cos(angle);
         R.T[0][0] = cosR + x * x * (1. - cosR);
