@@ -24,7 +24,7 @@
 #include "merlict/SurfaceEntity.h"
 #include "merlict/scenery/primitive/primitive.h"
 
-#include "../../include/mlgpu.h"
+#include "mlgpu.h"
 
 namespace merlict {
 namespace gpu {
