@@ -345,15 +345,23 @@ Box local_box(int kind, const double* p, const double* prm, double radius) {
     case MLGPU_KIND_SPHERE_CAP_RECT: {
         // hits satisfy |p| <= bounding radius and lie on the sphere
         // x^2 + y^2 + z^2 = 2 R z, so 0 <= z = |p|^2 / (2 R) <= r^2 / (2 R)
-        const double xy = kind == MLGPU_KIND_SPHERE_CAP_RECT ? std::max(prm[1], prm[2]) : p[1];
-        e[0] = e[1] = std::min(radius, xy);
+        // x, y: the hexagon's corners (+-r_o, 0), (+-r_o / 2, +-r_i), the
+        // rectangle's half widths, the circle's radius
+        const double ex = kind == MLGPU_KIND_SPHERE_CAP_RECT ? prm[1] : p[1];
+        const double ey = kind == MLGPU_KIND_SPHERE_CAP_RECT ? prm[2]
+                          : kind == MLGPU_KIND_SPHERE_CAP_HEX ? prm[1] : p[1];
+        e[0] = std::min(radius, ex);
+        e[1] = std::min(radius, ey);
         z0 = 0.0;
         z1 = std::min(radius, radius * radius / (2.0 * p[0]));
         break;
     }
     case MLGPU_KIND_PLANE: e[0] = prm[1]; e[1] = prm[2]; break;
-    case MLGPU_KIND_DISC:
     case MLGPU_KIND_HEX_PLANE:
+        e[0] = radius;  // corners (+-r_o, 0), (+-r_o / 2, +-r_i)
+        e[1] = prm[1];
+        break;
+    case MLGPU_KIND_DISC:
     case MLGPU_KIND_ANNULUS:
     case MLGPU_KIND_PLANE_DUAL_SPHERICAL:
         e[0] = e[1] = radius;
@@ -373,6 +381,44 @@ Box local_box(int kind, const double* p, const double* prm, double radius) {
     b.grow(lo);
     b.grow(hi);
     return b;
+}
+
+// Corners of a polygon in the object frame's plane z = 0 that holds every
+// point where a flat leaf can report a hit; the number of corners, 0 for a
+// leaf that is not flat.  Triangle: its corners; plane: the rectangle;
+// hexagon: HexagonalPrismZ's corners (+-r_o, 0), (+-r_o / 2, +-r_i)
+// (HexagonalPrismZ.cpp:14-29); disc and annulus: the 16-gon around the
+// outer circle.
+int flat_outline(int kind, const double* p, double (*v)[2]) {
+    switch (kind) {
+    case MLGPU_KIND_TRIANGLE:
+        for (int c = 0; c < 3; c++) { v[c][0] = p[2 * c]; v[c][1] = p[2 * c + 1]; }
+        return 3;
+    case MLGPU_KIND_PLANE: {
+        const double hx = 0.5 * p[0], hy = 0.5 * p[1];
+        const double xs[4] = {-hx, hx, hx, -hx}, ys[4] = {-hy, -hy, hy, hy};
+        for (int c = 0; c < 4; c++) { v[c][0] = xs[c]; v[c][1] = ys[c]; }
+        return 4;
+    }
+    case MLGPU_KIND_HEX_PLANE: {
+        const double ro = p[0], ri = p[0] * std::cos(M_PI / 6.0);
+        const double xs[6] = {ro, 0.5 * ro, -0.5 * ro, -ro, -0.5 * ro, 0.5 * ro};
+        const double ys[6] = {0.0, ri, ri, 0.0, -ri, -ri};
+        for (int c = 0; c < 6; c++) { v[c][0] = xs[c]; v[c][1] = ys[c]; }
+        return 6;
+    }
+    case MLGPU_KIND_DISC:
+    case MLGPU_KIND_ANNULUS: {
+        const double r = p[0] / std::cos(M_PI / 16.0);
+        for (int c = 0; c < 16; c++) {
+            v[c][0] = r * std::cos(c * M_PI / 8.0);
+            v[c][1] = r * std::sin(c * M_PI / 8.0);
+        }
+        return 16;
+    }
+    default:
+        return 0;
+    }
 }
 
 struct BNode {
@@ -416,7 +462,8 @@ class Builder {
             const int32_t r = scope_bnode_.at(f);
             int32_t entry;
             if (internal(r)) {
-                entry = pair_of_.at(r);
+                auto it = pair_of_.find(r);
+                entry = it != pair_of_.end() ? it->second : emit_pair(r, 1);
             } else {
                 entry = new_pair(1);
                 set_child(entry, 0, r, child_code(r, 2));
@@ -467,7 +514,18 @@ class Builder {
         const double* p = d_.params + MLGPU_NUM_PARAMS * f;
         const double radius = leaf_params(d_.kind[f], p, prm);
         const Box lb = local_box(d_.kind[f], p, prm, radius);
-        if (!lb.empty()) {
+        double outline[16][2];
+        const int nv = flat_outline(d_.kind[f], p, outline);
+        if (nv > 0) {
+            // a flat leaf: the world box of its outline, much tighter than
+            // that of its object-frame box when the leaf is slanted
+            const M34& m = t2w_[f];
+            for (int c = 0; c < nv; c++) {
+                double w[3];
+                for (int i = 0; i < 3; i++) w[i] = m.T[i][0] * outline[c][0] + m.T[i][1] * outline[c][1] + m.T[i][3];
+                b.box.grow(w);
+            }
+        } else if (!lb.empty()) {
             const M34& m = t2w_[f];
             for (int c = 0; c < 8; c++) {
                 const double q[3] = {(c & 1) ? lb.hi[0] : lb.lo[0], (c & 2) ? lb.hi[1] : lb.lo[1],
@@ -482,10 +540,14 @@ class Builder {
         return static_cast<int32_t>(bnodes_.size()) - 1;
     }
 
+    // the leaves under frame f, each as an item of f's tree; a restriction
+    // frame below f gets a tree of its own as well, for the propagation
+    // restricted to it, while f's tree holds its leaves one by one (so the
+    // boxes of f's tree are not bounded by the restricted sub-assemblies)
     void collect(int32_t f, std::vector<int32_t>* items) {
         for (int32_t c : children_[f]) {
-            if (scope_[c]) items->push_back(build_scope(c));
-            else if (children_[c].empty()) {
+            if (scope_[c]) build_scope(c);
+            if (children_[c].empty()) {
                 // a frame without a surface never reports a hit (Frame.cpp:172-179)
                 if (d_.kind[c] != MLGPU_KIND_FRAME) items->push_back(make_leaf(c));
             } else {
