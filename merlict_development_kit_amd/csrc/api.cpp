@@ -23,7 +23,7 @@ struct mlgpu_scene {
     int64_t device_bytes = 0;
     int device = 0;
     int num_cus = 0;
-    mlgpu_tuning tuning{MLGPU_SCHEDULE_AUTO, 24, 32};
+    mlgpu_tuning tuning{MLGPU_SCHEDULE_AUTO, 32, 32};
     // wavefront workspace, grown on demand.  Calls that use it are ordered
     // on the device: a call on another stream than the previous user first
     // waits for the event recorded after that user's kernels.
@@ -218,17 +218,8 @@ mlgpu::Workspace carve_workspace(void* mem, int64_t capacity) {
     Carver c{static_cast<char*>(mem)};
     mlgpu::Workspace w;
     w.cap = capacity;
-    w.sx = c.take<double>(cap);
-    w.sy = c.take<double>(cap);
-    w.sz = c.take<double>(cap);
-    w.dx = c.take<double>(cap);
-    w.dy = c.take<double>(cap);
-    w.dz = c.take<double>(cap);
-    w.tof = c.take<double>(cap);
-    w.dist = c.take<double>(cap);
-    w.root = c.take<int32_t>(cap);
-    w.n_hist = c.take<uint32_t>(cap);
-    w.draws = c.take<uint32_t>(cap);
+    w.ray = c.take<mlgpu::PhotonRay>(cap);
+    w.sums = c.take<mlgpu::PhotonSums>(cap);
     w.hit_leaf = c.take<int32_t>(cap);
     w.hit_t = c.take<double>(cap);
     w.tr_cur = c.take<int32_t>(cap);

@@ -10,12 +10,27 @@ namespace mlgpu {
 
 // Device workspace of the wavefront schedule.  Photon state is indexed by
 // photon (length cap), trace records by queue slot (length cap).
+// State of a live photon between bounce levels, in two records so that a
+// gathered access touches one or two cache lines instead of one per field:
+// the ray (what a trace reads) and the running sums (what the shading adds).
+struct alignas(64) PhotonRay {
+    double sx, sy, sz;
+    double dx, dy, dz;
+    int32_t root;     // scope the next trace runs in
+    uint32_t n_hist;  // history length so far
+    uint32_t draws;   // Philox draws consumed so far
+    int32_t pad;
+};
+
+struct alignas(16) PhotonSums {
+    double tof;   // running time of flight
+    double dist;  // running accumulated distance
+};
+
 struct Workspace {
     int64_t cap;
-    double *sx, *sy, *sz, *dx, *dy, *dz, *tof, *dist;
-    int32_t* root;
-    uint32_t* n_hist;
-    uint32_t* draws;
+    PhotonRay* ray;
+    PhotonSums* sums;
     int32_t* hit_leaf;   // best leaf so far per slot
     double* hit_t;       // its distance
     int32_t* tr_cur;     // traversal position of a deferred slot
@@ -44,7 +59,7 @@ constexpr int kNumCounts = 2 * kDeferredCounts;
 // bytes of workspace for cap photons (each array 256-byte aligned)
 constexpr size_t workspace_bytes(int64_t cap) {
     const size_t m = static_cast<size_t>(cap);
-    const size_t per = 8 * 8 + 4 + 4 + 4 + 4 + 8 + 4 + 4 + 4 * kMaxTreeDepth + 4 + 4 + 4;
+    const size_t per = sizeof(PhotonRay) + sizeof(PhotonSums) + 4 + 8 + 4 + 4 + 4 * kMaxTreeDepth + 4 + 4 + 4;
     return m * per + 4 * kNumCounts + 40 * 256;
 }
 

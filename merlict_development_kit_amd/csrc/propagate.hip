@@ -34,6 +34,20 @@ __device__ __forceinline__ uint64_t photon_id(const uint64_t id_offset, const in
     return id_offset + static_cast<uint64_t>(ids ? static_cast<int64_t>(ids[i]) : i);
 }
 
+// The state of a photon that propagates on, for the next level.
+__device__ __forceinline__ void store_state(const Workspace& ws, const int64_t i, const PhotonState& st,
+                                            const uint32_t draws) {
+    PhotonRay pr;
+    pr.sx = st.s.x; pr.sy = st.s.y; pr.sz = st.s.z;
+    pr.dx = st.d.x; pr.dy = st.d.y; pr.dz = st.d.z;
+    pr.root = st.root;
+    pr.n_hist = st.n_hist;
+    pr.draws = draws;
+    pr.pad = 0;
+    ws.ray[i] = pr;
+    ws.sums[i] = PhotonSums{st.tof, st.dist};
+}
+
 // The per-lane traversal stacks of a block: column threadIdx.x.
 #define MLGPU_LANE_STACK(name)                                          \
     __shared__ int32_t name##_mem[kMaxTreeDepth * kBlock];              \
@@ -122,11 +136,21 @@ __global__ void wf_reset_kernel(int32_t* counts, int n) {
     for (int i = static_cast<int>(threadIdx.x); i < n; i += static_cast<int>(blockDim.x)) counts[i] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void wf_init_kernel(const int32_t world_scope, const mlgpu_photons in,
-                                                         const int64_t n_arg, const uint32_t max_interactions,
-                                                         const Workspace ws, const mlgpu_results out,
-                                                         const CallParams* __restrict__ cp) {
+// Level 0 in one pass: every photon is read from the input, traced from its
+// production point and shaded; the photons that propagate on are written to
+// the workspace and queued for level 1.  Level-0 traces are short (the
+// first surface in front of the source), so they need neither the budget
+// nor the hit records of the later levels.
+__global__ __launch_bounds__(kBlock) void wf_first_kernel(const DeviceScene sc, const int32_t world_scope,
+                                                          const mlgpu_photons in, const int64_t n_arg,
+                                                          const uint32_t max_interactions, const uint64_t seed_arg,
+                                                          const uint64_t id_offset_arg, const Workspace ws,
+                                                          const mlgpu_results out, const CallParams* __restrict__ cp,
+                                                          const int32_t* __restrict__ ids) {
     const int64_t n = cp ? cp->n : n_arg;
+    const uint64_t seed = cp ? cp->seed : seed_arg;
+    const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
+    MLGPU_LANE_STACK(stk);
     __shared__ TileQueue<kTile> q;
     if (threadIdx.x == 0) q.n = 0;
     __syncthreads();
@@ -135,26 +159,26 @@ __global__ __launch_bounds__(kBlock) void wf_init_kernel(const int32_t world_sco
         #pragma unroll 1
         for (int k = 0; k < kTile; k += kBlock) {
             const int64_t i = tile + k + threadIdx.x;
-            bool live = false;
+            bool go = false;
             if (i < n) {
                 PhotonState st;
                 photon_init(st, in, i, world_scope);
-                live = st.err == MLGPU_PHOTON_OK && st.n_hist <= max_interactions;
-                if (live) {
-                    ws.sx[i] = st.s.x; ws.sy[i] = st.s.y; ws.sz[i] = st.s.z;
-                    ws.dx[i] = st.d.x; ws.dy[i] = st.d.y; ws.dz[i] = st.d.z;
-                    ws.tof[i] = 0.0;
-                    ws.dist[i] = 0.0;
-                    ws.root[i] = st.root;
-                    ws.n_hist[i] = st.n_hist;
-                    ws.draws[i] = 0u;
+                Rng rng;
+                rng.init(seed, photon_id(id_offset, ids, i), 0u);
+                if (st.err == MLGPU_PHOTON_OK && st.n_hist <= max_interactions) {
+                    Closest c;
+                    first_hit(sc, st.root, st.s, st.d, c, stk);
+                    go = photon_bounce(sc, st, c.leaf, c.t, in.wavelength[i], rng, max_interactions);
+                }
+                if (go) {
+                    store_state(ws, i, st, rng.next);
                 } else {
                     photon_finish(st, out, i);
                 }
             }
-            tile_push(q, live, static_cast<int32_t>(i));
+            tile_push(q, go, static_cast<int32_t>(i));
         }
-        tile_flush(q, ws.queue[0], ws.counts);
+        tile_flush(q, ws.queue[1], ws.counts + 1);
     }
 }
 
@@ -171,9 +195,10 @@ __global__ __launch_bounds__(kBlock, 5) void wf_trace_kernel(const DeviceScene s
     for (int32_t slot = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); slot < count;
          slot += static_cast<int32_t>(gridDim.x * blockDim.x)) {
         const int32_t i = queue[slot];
-        const D3 s{ws.sx[i], ws.sy[i], ws.sz[i]};
-        const D3 d{ws.dx[i], ws.dy[i], ws.dz[i]};
-        const int32_t scope = ws.root[i];
+        const PhotonRay pr = ws.ray[i];
+        const D3 s{pr.sx, pr.sy, pr.sz};
+        const D3 d{pr.dx, pr.dy, pr.dz};
+        const int32_t scope = pr.root;
         RayF r;
         ray_setup(sc, scope, s, d, r);
         TraceState ts;
@@ -218,10 +243,11 @@ __global__ __launch_bounds__(kBlock) void wf_resume_kernel(const DeviceScene sc,
          j += static_cast<int32_t>(gridDim.x * blockDim.x)) {
         const int32_t slot = ws.deferred[j];
         const int32_t i = queue[slot];
-        const D3 s{ws.sx[i], ws.sy[i], ws.sz[i]};
-        const D3 d{ws.dx[i], ws.dy[i], ws.dz[i]};
+        const PhotonRay pr = ws.ray[i];
+        const D3 s{pr.sx, pr.sy, pr.sz};
+        const D3 d{pr.dx, pr.dy, pr.dz};
         RayF r;
-        ray_setup(sc, ws.root[i], s, d, r);
+        ray_setup(sc, pr.root, s, d, r);
         TraceState ts;
         ts.cur = ws.tr_cur[slot];
         ts.depth = ws.tr_depth[slot];
@@ -258,26 +284,22 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
             int32_t i = 0;
             if (slot < count) {
                 i = queue[slot];
+                const PhotonRay pr = ws.ray[i];
+                const PhotonSums ps = ws.sums[i];
                 PhotonState st;
-                st.s = D3{ws.sx[i], ws.sy[i], ws.sz[i]};
-                st.d = D3{ws.dx[i], ws.dy[i], ws.dz[i]};
-                st.tof = ws.tof[i];
-                st.dist = ws.dist[i];
-                st.root = ws.root[i];
-                st.n_hist = ws.n_hist[i];
+                st.s = D3{pr.sx, pr.sy, pr.sz};
+                st.d = D3{pr.dx, pr.dy, pr.dz};
+                st.tof = ps.tof;
+                st.dist = ps.dist;
+                st.root = pr.root;
+                st.n_hist = pr.n_hist;
                 st.err = MLGPU_PHOTON_OK;
                 Rng rng;
-                rng.init(seed, photon_id(id_offset, ids, i), ws.draws[i]);
+                rng.init(seed, photon_id(id_offset, ids, i), pr.draws);
                 go = photon_bounce(sc, st, ws.hit_leaf[slot], ws.hit_t[slot], in.wavelength[i], rng,
                                    max_interactions);
                 if (go) {
-                    ws.sx[i] = st.s.x; ws.sy[i] = st.s.y; ws.sz[i] = st.s.z;
-                    ws.dx[i] = st.d.x; ws.dy[i] = st.d.y; ws.dz[i] = st.d.z;
-                    ws.tof[i] = st.tof;
-                    ws.dist[i] = st.dist;
-                    ws.root[i] = st.root;
-                    ws.n_hist[i] = st.n_hist;
-                    ws.draws[i] = rng.next;
+                    store_state(ws, i, st, rng.next);
                 } else {
                     photon_finish(st, out, i);
                 }
@@ -388,9 +410,9 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
     const int shade_grid = static_cast<int>(shade_tiles < grid_blocks ? shade_tiles : grid_blocks);
     const int64_t blocks = (n + kBlock - 1) / kBlock;
     const int grid = static_cast<int>(blocks < grid_blocks ? blocks : grid_blocks);
-    hipLaunchKernelGGL(wf_init_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, world_scope, in, n,
-                       max_interactions, ws, out, cp);
-    for (int level = 0; level < static_cast<int>(max_interactions); level++) {
+    hipLaunchKernelGGL(wf_first_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, sc, world_scope, in, n,
+                       max_interactions, seed, id_offset, ws, out, cp, ids);
+    for (int level = 1; level < static_cast<int>(max_interactions); level++) {
         const int32_t budget = trace_budget > 0 ? trace_budget : INT32_MAX;
         hipLaunchKernelGGL(wf_trace_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, ws, level, budget);
         if (trace_budget > 0) {
