@@ -252,18 +252,24 @@ enum mlgpu_schedule {
  *                  pass of a level; the unfinished rays resume, packed, in a
  *                  second pass.  <= 0: one pass.
  *   blocks_per_cu  WAVEFRONT: workgroups per compute unit of the grid-stride
- *                  kernels, 1 to 64. */
+ *                  kernels, 1 to 64.
+ *   tail_level     WAVEFRONT: from this bounce level on (>= 1) a photon's
+ *                  remaining bounces run in one lane, its state kept in
+ *                  registers; <= 0: every level is a trace pass and a shading
+ *                  pass of its own. */
 typedef struct mlgpu_tuning {
     int32_t schedule;
     int32_t trace_budget;
     int32_t blocks_per_cu;
+    int32_t tail_level;
 } mlgpu_tuning;
 int mlgpu_scene_set_tuning(mlgpu_scene* scene, const mlgpu_tuning* tuning);
 int mlgpu_scene_get_tuning(const mlgpu_scene* scene, mlgpu_tuning* tuning);
 
 /* In-library autotuner.  Times mlgpu_propagate of the given photons (device
  * arrays) under each candidate tuning: FUSED, and WAVEFRONT with trace
- * budget in {0, 12, 16, 24, 32, 48} x blocks per CU in {8, 16, 24, 32, 48};
+ * budget in {0, 12, 16, 24, 32, 48} x blocks per CU in {8, 16, 24, 32, 48}
+ * x tail level in {0, 1, 2};
  * one warm-up call and `repeats` (>= 1) timed calls each, between HIP events
  * on `stream`.  Keeps the fastest on the scene and returns it in *best and
  * its mean time per call in *best_ms (either may be NULL).  The results of

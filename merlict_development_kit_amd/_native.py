@@ -189,6 +189,7 @@ class Tuning(C.Structure):
         ("schedule", C.c_int32),
         ("trace_budget", C.c_int32),
         ("blocks_per_cu", C.c_int32),
+        ("tail_level", C.c_int32),
     ]
 
 

@@ -14,6 +14,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -60,15 +61,18 @@ def build_library(verbose: bool = False) -> Path:
     hipcc = _hipcc()
     objdir = ROOT / "_build" / "mlgpu"
     objdir.mkdir(parents=True, exist_ok=True)
-    objs = []
-    for src in SOURCES:
+    def compile_one(src: str) -> str:
         obj = objdir / (src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmd = [hipcc, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)]
-        out = _run(cmd)
+        out = _run([hipcc, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)])
         if verbose and out.strip():
             print(out)
-        objs.append(str(obj))
+        return str(obj)
+
+    # the translation units are independent: compile them side by side
+    workers = max(1, min(8, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        objs = list(pool.map(compile_one, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB)

@@ -23,7 +23,7 @@ struct mlgpu_scene {
     int64_t device_bytes = 0;
     int device = 0;
     int num_cus = 0;
-    mlgpu_tuning tuning{MLGPU_SCHEDULE_AUTO, 32, 32};
+    mlgpu_tuning tuning{MLGPU_SCHEDULE_AUTO, 32, 32, 0};
     // wavefront workspace, grown on demand.  Calls that use it are ordered
     // on the device: a call on another stream than the previous user first
     // waits for the event recorded after that user's kernels.
@@ -244,7 +244,7 @@ int launch_wavefront(mlgpu_scene* ms, int32_t scope, const mlgpu_photons& ph, in
     const mlgpu::Workspace w = carve_workspace(ms->ws, ms->ws_capacity);
     const hipError_t e = mlgpu::launch_propagate_wavefront(
         ms->dev, scope, ph, n, cfg.max_num_interactions_per_photon, cfg.seed, cfg.photon_id_offset, r, w,
-        ms->tuning.blocks_per_cu * ms->num_cus, ms->tuning.trace_budget, st, nullptr, ids);
+        ms->tuning.blocks_per_cu * ms->num_cus, ms->tuning.trace_budget, ms->tuning.tail_level, st, nullptr, ids);
     if (e != hipSuccess) return hip_fail(e, "propagate launch");
     return MLGPU_OK;
 }
@@ -465,9 +465,12 @@ int mlgpu_scene_autotune(mlgpu_scene* s, int32_t world_frame, const mlgpu_photon
     e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
     std::vector<mlgpu_tuning> cands;
-    cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_FUSED, s->tuning.trace_budget, s->tuning.blocks_per_cu});
+    cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_FUSED, s->tuning.trace_budget, s->tuning.blocks_per_cu,
+                                  s->tuning.tail_level});
     for (int32_t budget : {0, 12, 16, 24, 32, 48})
-        for (int32_t bpc : {8, 16, 24, 32, 48}) cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_WAVEFRONT, budget, bpc});
+        for (int32_t bpc : {8, 16, 24, 32, 48})
+            for (int32_t tail : {0, 1, 2})
+                cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_WAVEFRONT, budget, bpc, tail});
     const mlgpu_tuning saved = s->tuning;
     mlgpu_tuning chosen = saved;
     double chosen_ms = -1.0;
@@ -549,7 +552,7 @@ int mlgpu_propagate_noalloc(const mlgpu_scene* s, int32_t world_frame, const mlg
     e = mlgpu::launch_propagate_wavefront(s->dev, s->scope_of_frame[world_frame], *photons, n,
                                           cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset, *r,
                                           w, s->tuning.blocks_per_cu * s->num_cus, s->tuning.trace_budget,
-                                          static_cast<hipStream_t>(stream));
+                                          s->tuning.tail_level, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "propagate launch");
     return MLGPU_OK;
 }
@@ -941,7 +944,7 @@ int mlgpu_bunch_plan_create(const mlgpu_scene* s, int32_t world_frame, int32_t m
         le = mlgpu::launch_propagate_wavefront(s->dev, s->scope_of_frame[world_frame], ph, max_photons,
                                                p->max_interactions, cfg->seed, cfg->photon_id_offset, p->results, ws,
                                                s->tuning.blocks_per_cu * s->num_cus, s->tuning.trace_budget,
-                                               p->capture, params);
+                                               s->tuning.tail_level, p->capture, params);
     if (le == hipSuccess && hist) le = mlgpu::launch_histogram(p->results, max_photons, *hist, s->num_cus, p->capture,
                                                                 params);
     e = hipStreamEndCapture(p->capture, &p->graph);

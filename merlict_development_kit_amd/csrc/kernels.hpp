@@ -86,8 +86,8 @@ hipError_t launch_propagate_history(const DeviceScene& sc, int32_t world_scope, 
 hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& out, const Workspace& ws, int grid_blocks,
-                                      int32_t trace_budget, hipStream_t stream, const CallParams* cp = nullptr,
-                                      const int32_t* ids = nullptr);
+                                      int32_t trace_budget, int32_t tail_level, hipStream_t stream,
+                                      const CallParams* cp = nullptr, const int32_t* ids = nullptr);
 
 hipError_t launch_first_hit(const DeviceScene& sc, int32_t root_scope, const mlgpu_photons& rays, int64_t n,
                             const mlgpu_hits& out, hipStream_t stream);

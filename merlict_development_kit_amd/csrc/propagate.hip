@@ -310,6 +310,50 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
     }
 }
 
+// The last stage of a pass: the photons of a level are shaded with their
+// closest hits and then propagated to the end one lane per photon, trace and
+// bounce in turn, with no further round trip of their state through the
+// workspace.  After the mirror-to-camera level the remaining bounces are
+// short traces inside one light guide's scope, so finishing them in the lane
+// costs little divergence while every level it saves is one gathered read
+// and one write of the photon's state.
+__global__ __launch_bounds__(kBlock) void wf_tail_kernel(const DeviceScene sc, const mlgpu_photons in,
+                                                         const Workspace ws, const int level,
+                                                         const uint32_t max_interactions, const uint64_t seed_arg,
+                                                         const uint64_t id_offset_arg, const mlgpu_results out,
+                                                         const CallParams* __restrict__ cp,
+                                                         const int32_t* __restrict__ ids) {
+    MLGPU_LANE_STACK(stk);
+    const uint64_t seed = cp ? cp->seed : seed_arg;
+    const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
+    const int32_t count = ws.counts[level];
+    const int32_t* queue = ws.queue[level & 1];
+    for (int32_t slot = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); slot < count;
+         slot += static_cast<int32_t>(gridDim.x * blockDim.x)) {
+        const int32_t i = queue[slot];
+        const PhotonRay pr = ws.ray[i];
+        const PhotonSums ps = ws.sums[i];
+        PhotonState st;
+        st.s = D3{pr.sx, pr.sy, pr.sz};
+        st.d = D3{pr.dx, pr.dy, pr.dz};
+        st.tof = ps.tof;
+        st.dist = ps.dist;
+        st.root = pr.root;
+        st.n_hist = pr.n_hist;
+        st.err = MLGPU_PHOTON_OK;
+        const double lambda = in.wavelength[i];
+        Rng rng;
+        rng.init(seed, photon_id(id_offset, ids, i), pr.draws);
+        bool go = photon_bounce(sc, st, ws.hit_leaf[slot], ws.hit_t[slot], lambda, rng, max_interactions);
+        while (go) {
+            Closest c;
+            first_hit(sc, st.root, st.s, st.d, c, stk);
+            go = photon_bounce(sc, st, c.leaf, c.t, lambda, rng, max_interactions);
+        }
+        photon_finish(st, out, i);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void first_hit_kernel(const DeviceScene sc, const int32_t root_scope,
                                                            const mlgpu_photons rays, const int64_t n,
                                                            const mlgpu_hits out) {
@@ -398,8 +442,8 @@ hipError_t launch_propagate_history(const DeviceScene& sc, int32_t world_scope, 
 hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
                                       const mlgpu_results& out, const Workspace& ws, int grid_blocks,
-                                      int32_t trace_budget, hipStream_t stream, const CallParams* cp,
-                                      const int32_t* ids) {
+                                      int32_t trace_budget, int32_t tail_level, hipStream_t stream,
+                                      const CallParams* cp, const int32_t* ids) {
     if (n <= 0) return hipSuccess;
     const int levels = static_cast<int>(max_interactions) + 1;
     if (levels + 1 > kDeferredCounts) return hipErrorInvalidValue;
@@ -418,6 +462,11 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
         if (trace_budget > 0) {
             hipLaunchKernelGGL(wf_defer_kernel, dim3(tile_grid), dim3(kBlock), 0, stream, ws, level);
             hipLaunchKernelGGL(wf_resume_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, ws, level);
+        }
+        if (tail_level > 0 && level >= tail_level) {
+            hipLaunchKernelGGL(wf_tail_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, in, ws, level,
+                               max_interactions, seed, id_offset, out, cp, ids);
+            break;
         }
         hipLaunchKernelGGL(wf_shade_kernel, dim3(shade_grid), dim3(kBlock), 0, stream, sc, in, ws, level,
                            max_interactions, seed, id_offset, out, cp, ids);

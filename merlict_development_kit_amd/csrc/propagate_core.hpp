@@ -314,6 +314,16 @@ MLGPU_HD inline bool slab(const float* lo, const float* hi, const float idx, con
     return tmin <= tmax;
 }
 
+// 1 / x within 1 ulp (v_rcp_f32 on the GPU): the box tests only cull, and
+// the boxes are inflated by far more than its rounding moves an entry point
+MLGPU_HD inline float rcp_approx(const float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // FP32 test of the oriented box of a leaf for the ray o + t d: does the ray
 // meet it within [0, limit]?
 MLGPU_HD inline bool leaf_box_hit(const LeafBox& b, const float ox, const float oy, const float oz, const float dx,
@@ -324,7 +334,7 @@ MLGPU_HD inline bool leaf_box_hit(const LeafBox& b, const float ox, const float 
         const float* r = b.m + 3 * k;
         const float lo = fmaf(r[0], px, fmaf(r[1], py, r[2] * pz));
         const float ld = fmaf(r[0], dx, fmaf(r[1], dy, r[2] * dz));
-        const float inv = 1.0f / (fabsf(ld) > 1e-30f ? ld : copysignf(1e-30f, ld));
+        const float inv = rcp_approx(fabsf(ld) > 1e-30f ? ld : copysignf(1e-30f, ld));
         const float t1 = (-b.h[k] - lo) * inv, t2 = (b.h[k] - lo) * inv;
         tmin = fmaxf(tmin, fminf(t1, t2));
         tmax = fminf(tmax, fmaxf(t1, t2));
@@ -387,6 +397,21 @@ MLGPU_HD inline void trace_begin(const DeviceScene& sc, const int32_t scope, Tra
     ts.best.t = HUGE_VAL;
 }
 
+// Leaf `leaf` as a candidate: its oriented box is entered before the best
+// hit so far and its FP64 solve gives the closest hit; an exact tie goes to
+// the earlier candidate in merlict's order.
+MLGPU_HD inline void try_leaf(const DeviceScene& sc, const RayF& r, const D3& s, const D3& d, const int32_t leaf,
+                              Closest& best, float& best_f) {
+    double t;
+    if (leaf_box_hit(sc.boxes[leaf], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, best_f) &&
+        intersect_leaf(sc, sc.geo + leaf, s, d, t) &&
+        (t < best.t || (t == best.t && sc.geo[leaf].rank < sc.geo[best.leaf].rank))) {
+        best.leaf = leaf;
+        best.t = t;
+        best_f = static_cast<float>(t - r.t_adv) * (1.0f + 1e-6f) + sc.eps;
+    }
+}
+
 // Runs the traversal on, for at most `budget` steps (node visits and leaf
 // tests) when Budgeted; returns true when it has finished.  A traversal
 // stopped early resumes from ts and the stack, and ends with the same
@@ -430,16 +455,7 @@ MLGPU_HD inline bool trace_run(const DeviceScene& sc, const RayF& r, const D3& s
                 continue;
             }
         } else {
-            const int32_t leaf = -1 - cur;
-            double t;
-            // the closest hit; an exact tie goes to the earlier candidate
-            if (leaf_box_hit(sc.boxes[leaf], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, best_f) &&
-                intersect_leaf(sc, sc.geo + leaf, s, d, t) &&
-                (t < ts.best.t || (t == ts.best.t && sc.geo[leaf].rank < sc.geo[ts.best.leaf].rank))) {
-                ts.best.leaf = leaf;
-                ts.best.t = t;
-                best_f = static_cast<float>(t - r.t_adv) * (1.0f + 1e-6f) + sc.eps;
-            }
+            try_leaf(sc, r, s, d, -1 - cur, ts.best, best_f);
         }
         // next stack entry still in front of the best hit; a deferred leaf's
         // box is tested again first

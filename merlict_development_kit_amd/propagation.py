@@ -139,20 +139,24 @@ class GpuScene:
     _SCHEDULES = {"auto": N.SCHEDULE_AUTO, "fused": N.SCHEDULE_FUSED, "wavefront": N.SCHEDULE_WAVEFRONT}
 
     def tuning(self) -> dict:
-        """Launch parameters (mlgpu_tuning): schedule, trace_budget, blocks_per_cu."""
+        """Launch parameters (mlgpu_tuning): schedule, trace_budget, blocks_per_cu, tail_level."""
         t = N.Tuning()
         N.check(N.lib().mlgpu_scene_get_tuning(self._h, C.byref(t)), "mlgpu_scene_get_tuning")
         names = {v: k for k, v in self._SCHEDULES.items()}
-        return {"schedule": names[t.schedule], "trace_budget": t.trace_budget, "blocks_per_cu": t.blocks_per_cu}
+        return {"schedule": names[t.schedule], "trace_budget": t.trace_budget, "blocks_per_cu": t.blocks_per_cu,
+                "tail_level": t.tail_level}
 
     def set_tuning(self, schedule: Optional[str] = None, trace_budget: Optional[int] = None,
-                   blocks_per_cu: Optional[int] = None) -> None:
+                   blocks_per_cu: Optional[int] = None, tail_level: Optional[int] = None) -> None:
         """Change launch parameters; results do not depend on them.  schedule is
-        'auto', 'fused' (one lane per photon) or 'wavefront' (trace/shade per bounce)."""
+        'auto', 'fused' (one lane per photon) or 'wavefront' (trace/shade per
+        bounce level); tail_level (wavefront) is the level from which a
+        photon's remaining bounces run in one lane (<= 0: never)."""
         cur = self.tuning()
         t = N.Tuning(self._SCHEDULES[schedule if schedule is not None else cur["schedule"]],
                      trace_budget if trace_budget is not None else cur["trace_budget"],
-                     blocks_per_cu if blocks_per_cu is not None else cur["blocks_per_cu"])
+                     blocks_per_cu if blocks_per_cu is not None else cur["blocks_per_cu"],
+                     tail_level if tail_level is not None else cur["tail_level"])
         N.check(N.lib().mlgpu_scene_set_tuning(self._h, C.byref(t)), "mlgpu_scene_set_tuning")
 
     def set_schedule(self, schedule: str) -> None:

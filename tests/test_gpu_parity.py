@@ -112,17 +112,20 @@ def test_gpu_plenoscope_matches_oracle(cuda, schedule, oracle_threads):
 def test_gpu_schedules_agree_bitwise(cuda, max_int):
     """Launch parameters never change results: fused vs wavefront, and
     wavefront traces deferred after 1, 7 or 24 steps (their state saved and
-    resumed) or never."""
+    resumed) or never, with the last bounces run per lane from level 1, 2
+    or 3 or not at all."""
     gs = GpuScene(SCENES["fact_lg"]())
     ph = sources.fact_bunch(300_000, seed=8, device=cuda)
     ph.wavelength[:1000] = 150e-9  # out of the tables: error records
     gs.set_tuning(schedule="fused")
     ref = gs.propagate(ph, PropagationConfig(max_int, seed=5)).numpy()
-    for budget, blocks in ((24, 8), (1, 8), (7, 3), (0, 8)):
-        gs.set_tuning(schedule="wavefront", trace_budget=budget, blocks_per_cu=blocks)
+    for budget, blocks, tail in ((24, 8, 0), (1, 8, 0), (7, 3, 0), (0, 8, 0), (32, 32, 1), (7, 3, 2), (0, 8, 1),
+                                 (5, 16, 3)):
+        gs.set_tuning(schedule="wavefront", trace_budget=budget, blocks_per_cu=blocks, tail_level=tail)
         res = gs.propagate(ph, PropagationConfig(max_int, seed=5)).numpy()
         for k in ref:
-            np.testing.assert_array_equal(ref[k], res[k], err_msg=f"{k} (budget {budget}, {blocks} blocks/CU)")
+            np.testing.assert_array_equal(ref[k], res[k],
+                                          err_msg=f"{k} (budget {budget}, {blocks} blocks/CU, tail level {tail})")
 
 
 def test_gpu_autotune_picks_a_candidate_and_keeps_results(cuda):
@@ -139,6 +142,7 @@ def test_gpu_autotune_picks_a_candidate_and_keeps_results(cuda):
     if tuning["schedule"] == "wavefront":
         assert tuning["trace_budget"] in (0, 12, 16, 24, 32, 48)
         assert tuning["blocks_per_cu"] in (8, 16, 24, 32, 48)
+        assert tuning["tail_level"] in (0, 1, 2)
     else:
         assert tuning["schedule"] == "fused"
     res = gs.propagate(ph, PropagationConfig(5, seed=3)).numpy()
@@ -150,7 +154,7 @@ def test_gpu_autotune_picks_a_candidate_and_keeps_results(cuda):
 
 def test_gpu_tuning_is_validated(cuda):
     gs = GpuScene(SCENES["lens"]())
-    assert gs.tuning() == {"schedule": "auto", "trace_budget": 32, "blocks_per_cu": 32}
+    assert gs.tuning() == {"schedule": "auto", "trace_budget": 32, "blocks_per_cu": 32, "tail_level": 0}
     with pytest.raises(MlgpuError):
         gs.set_tuning(blocks_per_cu=0)
     with pytest.raises(MlgpuError):
