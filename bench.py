@@ -778,7 +778,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None,
                     help="host threads of the CPU baseline (default: every usable core of this process)")
-    ap.add_argument("--schedule", default="auto", choices=["auto", "fused", "wavefront"])
+    ap.add_argument("--schedule", default="auto", choices=["auto", "fused", "wavefront", "persistent"])
+    ap.add_argument("--shade-lanes", type=int, default=None)
     ap.add_argument("--trace-budget", type=int, default=None)
     ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--autotune", action="store_true",
@@ -812,7 +813,8 @@ def main() -> int:
         del src, dst
     desc = SceneDescription.fact(light_guides=not args.no_light_guides)
     scene = GpuScene(desc)
-    scene.set_tuning(schedule=args.schedule, trace_budget=args.trace_budget, blocks_per_cu=args.blocks_per_cu)
+    scene.set_tuning(schedule=args.schedule, trace_budget=args.trace_budget, blocks_per_cu=args.blocks_per_cu,
+                     shade_lanes=args.shade_lanes)
     n = args.photons
     photons = sources.fact_bunch().generate(n, seed=2024, device=dev)
     out = PropagationResult.empty(n, dev)
@@ -842,7 +844,8 @@ def main() -> int:
     value = n * args.steps / wall
     achieved = n * ALG_BYTES_PER_PHOTON / (ms_kernel * 1e-3)
     default_workload = (n == 10_000_000 and not args.no_light_guides and args.trace_budget is None
-                        and args.blocks_per_cu is None and args.schedule == "auto" and not args.autotune)
+                        and args.blocks_per_cu is None and args.shade_lanes is None and args.schedule == "auto"
+                        and not args.autotune)
     line = {
         "metric": METRIC,
         "value": value,

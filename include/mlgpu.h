@@ -239,11 +239,16 @@ int mlgpu_scene_stats(const mlgpu_scene* scene, int64_t* num_nodes,
  *              compacted queue of live photons (workspace owned by the scene,
  *              about 210 B per photon; calls sharing a scene are serialised
  *              on it);
+ *   PERSISTENT a grid of resident waves; a lane carries one photon through
+ *              all its bounces with its state in registers and, when the
+ *              photon ends, takes the next one, so waves stay full; traces
+ *              advance in quanta and the lanes of a wave shade together;
  *   AUTO       WAVEFRONT from 65,536 photons on. */
 enum mlgpu_schedule {
     MLGPU_SCHEDULE_AUTO = 0,
     MLGPU_SCHEDULE_FUSED = 1,
-    MLGPU_SCHEDULE_WAVEFRONT = 2
+    MLGPU_SCHEDULE_WAVEFRONT = 2,
+    MLGPU_SCHEDULE_PERSISTENT = 3
 };
 
 /* Launch parameters of mlgpu_propagate; they never change results.
@@ -251,25 +256,34 @@ enum mlgpu_schedule {
  *   trace_budget   WAVEFRONT: traversal steps a ray gets in the first trace
  *                  pass of a level; the unfinished rays resume, packed, in a
  *                  second pass.  <= 0: one pass.
- *   blocks_per_cu  WAVEFRONT: workgroups per compute unit of the grid-stride
- *                  kernels, 1 to 64.
+ *                  PERSISTENT: traversal steps per quantum, between which a
+ *                  wave checks whether enough lanes wait to shade; <= 0:
+ *                  traces run to their end.
+ *   blocks_per_cu  WAVEFRONT, PERSISTENT: workgroups per compute unit of the
+ *                  grid-stride and persistent kernels, 1 to 64.
  *   tail_level     WAVEFRONT: from this bounce level on (>= 1) a photon's
  *                  remaining bounces run in one lane, its state kept in
  *                  registers; <= 0: every level is a trace pass and a shading
- *                  pass of its own. */
+ *                  pass of its own.
+ *   shade_lanes    PERSISTENT: lanes of a wave whose traces have ended before
+ *                  the wave shades them (or all of its busy lanes, if fewer),
+ *                  1 to 64. */
 typedef struct mlgpu_tuning {
     int32_t schedule;
     int32_t trace_budget;
     int32_t blocks_per_cu;
     int32_t tail_level;
+    int32_t shade_lanes;
 } mlgpu_tuning;
 int mlgpu_scene_set_tuning(mlgpu_scene* scene, const mlgpu_tuning* tuning);
 int mlgpu_scene_get_tuning(const mlgpu_scene* scene, mlgpu_tuning* tuning);
 
 /* In-library autotuner.  Times mlgpu_propagate of the given photons (device
- * arrays) under each candidate tuning: FUSED, and WAVEFRONT with trace
+ * arrays) under each candidate tuning: FUSED; WAVEFRONT with trace
  * budget in {0, 12, 16, 24, 32, 48} x blocks per CU in {8, 16, 24, 32, 48}
- * x tail level in {0, 1, 2};
+ * x tail level in {0, 1, 2}; PERSISTENT with quantum (trace_budget) in
+ * {0, 8, 16, 32} x blocks per CU in {4, 8, 16} x shade lanes in
+ * {1, 16, 32, 48};
  * one warm-up call and `repeats` (>= 1) timed calls each, between HIP events
  * on `stream`.  Keeps the fastest on the scene and returns it in *best and
  * its mean time per call in *best_ms (either may be NULL).  The results of
@@ -451,8 +465,9 @@ int mlgpu_propagate_bunches(const mlgpu_scene* scene, int32_t world_frame,
                             void* stream);
 
 /* Bunch plans: the launch sequence of mlgpu_propagate_bunches (bunch table
- * up, photon draw, WAVEFRONT propagation with the scene's tuning at
- * creation, histogram into hist when it is not NULL) captured once in a
+ * up, photon draw, propagation with the scene's tuning at creation --
+ * the PERSISTENT work-queue kernel when that is the scene's schedule,
+ * WAVEFRONT otherwise --, histogram into hist when it is not NULL) captured once in a
  * hipGraph for batches of up to max_bunches bunches and max_photons photons,
  * and replayed per batch: the kernels read the batch's photon count, seed
  * and photon id offset from a device parameter block that each run uploads

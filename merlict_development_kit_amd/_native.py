@@ -42,6 +42,7 @@ REFRACTION_TO_INSIDE = 7
 SCHEDULE_AUTO = 0
 SCHEDULE_FUSED = 1
 SCHEDULE_WAVEFRONT = 2
+SCHEDULE_PERSISTENT = 3
 
 PHOTON_OK = 0
 PHOTON_BAD_WAVELENGTH = 1
@@ -190,6 +191,7 @@ class Tuning(C.Structure):
         ("trace_budget", C.c_int32),
         ("blocks_per_cu", C.c_int32),
         ("tail_level", C.c_int32),
+        ("shade_lanes", C.c_int32),
     ]
 
 

@@ -23,7 +23,7 @@ struct mlgpu_scene {
     int64_t device_bytes = 0;
     int device = 0;
     int num_cus = 0;
-    mlgpu_tuning tuning{MLGPU_SCHEDULE_AUTO, 32, 32, 0};
+    mlgpu_tuning tuning{MLGPU_SCHEDULE_AUTO, 32, 32, 0, 32};
     // wavefront workspace, grown on demand.  Calls that use it are ordered
     // on the device: a call on another stream than the previous user first
     // waits for the event recorded after that user's kernels.
@@ -232,19 +232,37 @@ mlgpu::Workspace carve_workspace(void* mem, int64_t capacity) {
     return w;
 }
 
-// Wavefront propagation; the caller holds ws_mutex and began shared use.
+// The schedules that use the scene's workspace (WAVEFRONT: photon state
+// and queues; PERSISTENT: only its counters) on workspace w, whose capacity
+// is at least n photons for WAVEFRONT.
+hipError_t launch_on_workspace(const mlgpu_scene* s, int schedule, int32_t scope, const mlgpu_photons& ph,
+                               int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
+                               const mlgpu_results& r, const mlgpu::Workspace& w, hipStream_t st,
+                               const mlgpu::CallParams* cp, const int32_t* ids) {
+    const mlgpu_tuning& t = s->tuning;
+    if (schedule == MLGPU_SCHEDULE_PERSISTENT)
+        return mlgpu::launch_propagate_persistent(s->dev, scope, ph, n, max_interactions, seed, id_offset, r, w.counts,
+                                                  t.blocks_per_cu * s->num_cus, t.trace_budget, t.shade_lanes, st, cp,
+                                                  ids);
+    return mlgpu::launch_propagate_wavefront(s->dev, scope, ph, n, max_interactions, seed, id_offset, r, w,
+                                             t.blocks_per_cu * s->num_cus, t.trace_budget, t.tail_level, st, cp, ids);
+}
+
+// WAVEFRONT or PERSISTENT propagation; the caller holds ws_mutex and began
+// shared use.
 int launch_wavefront(mlgpu_scene* ms, int32_t scope, const mlgpu_photons& ph, int64_t n, const mlgpu_config& cfg,
                      const mlgpu_results& r, hipStream_t st, const int32_t* ids = nullptr) {
-    if (ms->ws_capacity < n) {
-        const int64_t cap = n + n / 8;
+    const int schedule = schedule_for(ms, n);
+    const int64_t need = schedule == MLGPU_SCHEDULE_PERSISTENT ? 1 : n;
+    if (ms->ws_capacity < need) {
+        const int64_t cap = need + need / 8;
         const int rc = grow(ms, &ms->ws, mlgpu::workspace_bytes(cap));
         ms->ws_capacity = rc ? 0 : cap;
         if (rc) return rc;
     }
     const mlgpu::Workspace w = carve_workspace(ms->ws, ms->ws_capacity);
-    const hipError_t e = mlgpu::launch_propagate_wavefront(
-        ms->dev, scope, ph, n, cfg.max_num_interactions_per_photon, cfg.seed, cfg.photon_id_offset, r, w,
-        ms->tuning.blocks_per_cu * ms->num_cus, ms->tuning.trace_budget, ms->tuning.tail_level, st, nullptr, ids);
+    const hipError_t e = launch_on_workspace(ms, schedule, scope, ph, n, cfg.max_num_interactions_per_photon,
+                                             cfg.seed, cfg.photon_id_offset, r, w, st, nullptr, ids);
     if (e != hipSuccess) return hip_fail(e, "propagate launch");
     return MLGPU_OK;
 }
@@ -424,12 +442,16 @@ void mlgpu_scene_destroy(mlgpu_scene* s) {
 
 int mlgpu_scene_set_tuning(mlgpu_scene* s, const mlgpu_tuning* t) {
     if (!s || !t) { set_last_error("null argument"); return MLGPU_ERR_ARGUMENT; }
-    if (t->schedule < MLGPU_SCHEDULE_AUTO || t->schedule > MLGPU_SCHEDULE_WAVEFRONT) {
+    if (t->schedule < MLGPU_SCHEDULE_AUTO || t->schedule > MLGPU_SCHEDULE_PERSISTENT) {
         set_last_error("bad schedule");
         return MLGPU_ERR_ARGUMENT;
     }
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) {
         set_last_error("blocks_per_cu must be in [1, 64]");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (t->shade_lanes < 1 || t->shade_lanes > 64) {
+        set_last_error("shade_lanes must be in [1, 64]");
         return MLGPU_ERR_ARGUMENT;
     }
     s->tuning = *t;
@@ -466,11 +488,15 @@ int mlgpu_scene_autotune(mlgpu_scene* s, int32_t world_frame, const mlgpu_photon
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
     std::vector<mlgpu_tuning> cands;
     cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_FUSED, s->tuning.trace_budget, s->tuning.blocks_per_cu,
-                                  s->tuning.tail_level});
+                                  s->tuning.tail_level, s->tuning.shade_lanes});
     for (int32_t budget : {0, 12, 16, 24, 32, 48})
         for (int32_t bpc : {8, 16, 24, 32, 48})
             for (int32_t tail : {0, 1, 2})
-                cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_WAVEFRONT, budget, bpc, tail});
+                cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_WAVEFRONT, budget, bpc, tail, s->tuning.shade_lanes});
+    for (int32_t quantum : {0, 8, 16, 32})
+        for (int32_t bpc : {4, 8, 16})
+            for (int32_t lanes : {1, 16, 32, 48})
+                cands.push_back(mlgpu_tuning{MLGPU_SCHEDULE_PERSISTENT, quantum, bpc, s->tuning.tail_level, lanes});
     const mlgpu_tuning saved = s->tuning;
     mlgpu_tuning chosen = saved;
     double chosen_ms = -1.0;
@@ -549,10 +575,11 @@ int mlgpu_propagate_noalloc(const mlgpu_scene* s, int32_t world_frame, const mlg
     // the wavefront schedule on the library's static workspace; the kernels
     // clear their counters themselves, so nothing needs zeroing beforehand
     const mlgpu::Workspace w = carve_workspace(mem, mlgpu::kNoAllocMaxPhotons);
-    e = mlgpu::launch_propagate_wavefront(s->dev, s->scope_of_frame[world_frame], *photons, n,
-                                          cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset, *r,
-                                          w, s->tuning.blocks_per_cu * s->num_cus, s->tuning.trace_budget,
-                                          s->tuning.tail_level, static_cast<hipStream_t>(stream));
+    const int schedule = s->tuning.schedule == MLGPU_SCHEDULE_PERSISTENT ? MLGPU_SCHEDULE_PERSISTENT
+                                                                          : MLGPU_SCHEDULE_WAVEFRONT;
+    e = launch_on_workspace(s, schedule, s->scope_of_frame[world_frame], *photons, n,
+                            cfg->max_num_interactions_per_photon, cfg->seed, cfg->photon_id_offset, *r, w,
+                            static_cast<hipStream_t>(stream), nullptr, nullptr);
     if (e != hipSuccess) return hip_fail(e, "propagate launch");
     return MLGPU_OK;
 }
@@ -941,10 +968,11 @@ int mlgpu_bunch_plan_create(const mlgpu_scene* s, int32_t world_frame, int32_t m
     hipError_t le = hipMemcpyAsync(params, p->staging, p->head_bytes, hipMemcpyHostToDevice, p->capture);
     if (le == hipSuccess) le = mlgpu::launch_generate_bunches(table, max_bunches, max_photons, gen, p->capture, params);
     if (le == hipSuccess)
-        le = mlgpu::launch_propagate_wavefront(s->dev, s->scope_of_frame[world_frame], ph, max_photons,
-                                               p->max_interactions, cfg->seed, cfg->photon_id_offset, p->results, ws,
-                                               s->tuning.blocks_per_cu * s->num_cus, s->tuning.trace_budget,
-                                               s->tuning.tail_level, p->capture, params);
+        le = launch_on_workspace(s,
+                                 s->tuning.schedule == MLGPU_SCHEDULE_PERSISTENT ? MLGPU_SCHEDULE_PERSISTENT
+                                                                                 : MLGPU_SCHEDULE_WAVEFRONT,
+                                 s->scope_of_frame[world_frame], ph, max_photons, p->max_interactions, cfg->seed,
+                                 cfg->photon_id_offset, p->results, ws, p->capture, params, nullptr);
     if (le == hipSuccess && hist) le = mlgpu::launch_histogram(p->results, max_photons, *hist, s->num_cus, p->capture,
                                                                 params);
     e = hipStreamEndCapture(p->capture, &p->graph);

@@ -89,6 +89,15 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
                                       int32_t trace_budget, int32_t tail_level, hipStream_t stream,
                                       const CallParams* cp = nullptr, const int32_t* ids = nullptr);
 
+// The persistent schedule: counts is the workspace's counter block (its
+// last 64-bit word is the photon chunk counter); quantum <= 0: traces run
+// to their end before a shading round.
+hipError_t launch_propagate_persistent(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
+                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
+                                       const mlgpu_results& out, int32_t* counts, int grid_blocks, int32_t quantum,
+                                       int32_t shade_lanes, hipStream_t stream, const CallParams* cp = nullptr,
+                                       const int32_t* ids = nullptr);
+
 hipError_t launch_first_hit(const DeviceScene& sc, int32_t root_scope, const mlgpu_photons& rays, int64_t n,
                             const mlgpu_hits& out, hipStream_t stream);
 

@@ -354,6 +354,106 @@ __global__ __launch_bounds__(kBlock) void wf_tail_kernel(const DeviceScene sc, c
     }
 }
 
+// ----------------------------------------------------- persistent schedule
+// One lane per photon at a time, for all of its bounces, with the photon's
+// state in registers from its input to its result, so that the only photon
+// traffic is the input read and the result write.  A lane whose photon has
+// ended takes the next photon of its wave's chunk (kChunk consecutive
+// photons that a wave claims with one global atomic), so waves stay full
+// until the last chunks.  Traces advance in quanta of `quantum` traversal
+// steps; a lane whose trace has ended waits until `shade_lanes` lanes of its
+// wave (or every busy one) have, and then they run the bounce physics
+// together: the FP64 shading runs on filled waves while long traces (mirror
+// to camera, world scope) and short ones (inside one light guide) share a
+// wave.
+
+constexpr int kChunk = 256;
+
+__global__ __launch_bounds__(kBlock) void persistent_kernel(const DeviceScene sc, const int32_t world_scope,
+                                                            const mlgpu_photons in, const int64_t n_arg,
+                                                            const uint32_t max_interactions, const uint64_t seed_arg,
+                                                            const uint64_t id_offset_arg, const mlgpu_results out,
+                                                            unsigned long long* __restrict__ counter,
+                                                            const int32_t quantum, const int32_t shade_lanes,
+                                                            const CallParams* __restrict__ cp,
+                                                            const int32_t* __restrict__ ids) {
+    const int64_t n = cp ? cp->n : n_arg;
+    const uint64_t seed = cp ? cp->seed : seed_arg;
+    const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
+    MLGPU_LANE_STACK(stk);
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const uint64_t below = (1ull << lane) - 1ull;
+    // the wave's chunk [chunk_next, chunk_end); wave-uniform
+    int64_t chunk_next = 0, chunk_end = 0;
+    bool exhausted = false;
+    // this lane's photon
+    bool busy = false, traced = false;
+    int64_t i = 0;
+    double lambda = 0.0;
+    PhotonState st;
+    Rng rng;
+    TraceState ts;
+    for (;;) {
+        // idle lanes take photons until every lane is busy or no photon is left
+        for (;;) {
+            const uint64_t idle = __ballot(!busy);
+            if (idle == 0 || exhausted) break;
+            if (chunk_next >= chunk_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(counter, static_cast<unsigned long long>(kChunk));
+                base = __shfl(base, 0);
+                if (static_cast<int64_t>(base) >= n) {
+                    exhausted = true;
+                    break;
+                }
+                chunk_next = static_cast<int64_t>(base);
+                chunk_end = chunk_next + kChunk < n ? chunk_next + kChunk : n;
+            }
+            const int64_t want = __popcll(idle);
+            const int64_t take = want < chunk_end - chunk_next ? want : chunk_end - chunk_next;
+            const int64_t rank = __popcll(idle & below);
+            if (!busy && rank < take) {
+                i = chunk_next + rank;
+                photon_init(st, in, i, world_scope);
+                lambda = in.wavelength[i];
+                rng.init(seed, photon_id(id_offset, ids, i), 0u);
+                if (st.err == MLGPU_PHOTON_OK && st.n_hist <= max_interactions) {
+                    trace_begin(sc, st.root, ts);
+                    busy = true;
+                    traced = false;
+                } else {
+                    photon_finish(st, out, i);
+                }
+            }
+            chunk_next += take;
+        }
+        const uint64_t live = __ballot(busy);
+        if (live == 0) break;  // only once the photons are exhausted
+        const int live_n = __popcll(live);
+        const int need = shade_lanes < live_n ? shade_lanes : live_n;
+        {
+            // the float ray is rebuilt from the photon's state in every round
+            // rather than kept: it would otherwise hold registers through the
+            // shading of the other lanes
+            RayF r;
+            if (busy && !traced) ray_setup(sc, st.root, st.s, st.d, r);
+            for (;;) {
+                if (busy && !traced) traced = trace_run<true>(sc, r, st.s, st.d, ts, stk, quantum);
+                if (__popcll(__ballot(busy && traced)) >= need) break;
+            }
+        }
+        if (busy && traced) {
+            traced = false;
+            if (photon_bounce(sc, st, ts.best.leaf, ts.best.t, lambda, rng, max_interactions)) {
+                trace_begin(sc, st.root, ts);
+            } else {
+                photon_finish(st, out, i);
+                busy = false;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void first_hit_kernel(const DeviceScene sc, const int32_t root_scope,
                                                            const mlgpu_photons rays, const int64_t n,
                                                            const mlgpu_hits out) {
@@ -471,6 +571,26 @@ hipError_t launch_propagate_wavefront(const DeviceScene& sc, int32_t world_scope
         hipLaunchKernelGGL(wf_shade_kernel, dim3(shade_grid), dim3(kBlock), 0, stream, sc, in, ws, level,
                            max_interactions, seed, id_offset, out, cp, ids);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_propagate_persistent(const DeviceScene& sc, int32_t world_scope, const mlgpu_photons& in,
+                                       int64_t n, uint32_t max_interactions, uint64_t seed, uint64_t id_offset,
+                                       const mlgpu_results& out, int32_t* counts, int grid_blocks, int32_t quantum,
+                                       int32_t shade_lanes, hipStream_t stream, const CallParams* cp,
+                                       const int32_t* ids) {
+    if (n <= 0) return hipSuccess;
+    // the chunk counter: one 64-bit word at the end of the counts, cleared
+    // by a kernel so that the sequence can be captured in a graph
+    hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(256), 0, stream, counts, kNumCounts);
+    auto* counter = reinterpret_cast<unsigned long long*>(counts + kNumCounts - 2);
+    const int64_t waves_wanted = (n + kChunk - 1) / kChunk;
+    const int64_t blocks = (waves_wanted + kBlock / 64 - 1) / (kBlock / 64);
+    const int grid = static_cast<int>(blocks < grid_blocks ? blocks : grid_blocks);
+    const int32_t q = quantum > 0 ? quantum : INT32_MAX;
+    const int32_t lanes = shade_lanes < 1 ? 1 : (shade_lanes > 64 ? 64 : shade_lanes);
+    hipLaunchKernelGGL(persistent_kernel, dim3(grid), dim3(kBlock), 0, stream, sc, world_scope, in, n,
+                       max_interactions, seed, id_offset, out, counter, q, lanes, cp, ids);
     return hipGetLastError();
 }
 

@@ -507,7 +507,9 @@ struct PhotonState {
     uint8_t err;
     uint8_t last_interaction;
     int32_t last_frame, last_sensor;
-    double out_x, out_y, out_cx, out_cy;
+    // the result's x, y, cx, cy, rounded to float when the bounce computes
+    // them (the record stores floats)
+    float out_x, out_y, out_cx, out_cy;
 };
 
 MLGPU_HD inline void photon_init(PhotonState& st, const mlgpu_photons& in, const int64_t i, const int32_t world_scope) {
@@ -521,10 +523,10 @@ MLGPU_HD inline void photon_init(PhotonState& st, const mlgpu_photons& in, const
     st.last_interaction = MLGPU_PRODUCTION;
     st.last_frame = -1;
     st.last_sensor = -1;
-    st.out_x = st.s.x;
-    st.out_y = st.s.y;
-    st.out_cx = 0.0;
-    st.out_cy = 0.0;
+    st.out_x = static_cast<float>(st.s.x);
+    st.out_y = static_cast<float>(st.s.y);
+    st.out_cx = 0.0f;
+    st.out_cy = 0.0f;
 }
 
 // The history entry RayForPropagation::push_back_intersection_and_interaction
@@ -570,11 +572,11 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
         st.last_interaction = MLGPU_ABSORPTION_IN_VOID;
         st.last_frame = -1;
         st.last_sensor = -1;
-        st.out_x = e.x;
-        st.out_y = e.y;
+        st.out_x = static_cast<float>(e.x);
+        st.out_y = static_cast<float>(e.y);
         const D3 inc = normalized(D3{e.x - st.s.x, e.y - st.s.y, e.z - st.s.z});
-        st.out_cx = -1.0 * inc.x;
-        st.out_cy = -1.0 * inc.y;
+        st.out_cx = static_cast<float>(-1.0 * inc.x);
+        st.out_cy = static_cast<float>(-1.0 * inc.y);
         return false;
     }
     const LeafGeo& g = sc.geo[leaf];
@@ -680,14 +682,14 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
     st.last_interaction = interaction;
     st.last_frame = ph.frame;
     st.last_sensor = ph.sensor;
-    st.out_x = hp.x;
-    st.out_y = hp.y;
+    st.out_x = static_cast<float>(hp.x);
+    st.out_y = static_cast<float>(hp.y);
     {
         // final_intersection_incident_direction_wrt_frame (RayForPropagation.cpp:114-136)
         const D3 inc = normalized(D3{hit_world.x - st.s.x, hit_world.y - st.s.y, hit_world.z - st.s.z});
         const D3 loc = orient_inv(g, inc);
-        st.out_cx = -1.0 * loc.x;
-        st.out_cy = -1.0 * loc.y;
+        st.out_cx = static_cast<float>(-1.0 * loc.x);
+        st.out_cy = static_cast<float>(-1.0 * loc.y);
     }
     if (stop) return false;
     st.s = hit_world;
@@ -720,10 +722,10 @@ MLGPU_HD inline void photon_finish(const PhotonState& st, const mlgpu_results& o
     out.num_interactions[i] = static_cast<uint8_t>(st.n_hist);
     out.error[i] = st.err;
     out.time_of_flight[i] = st.tof;
-    out.x[i] = static_cast<float>(st.out_x);
-    out.y[i] = static_cast<float>(st.out_y);
-    out.cx[i] = static_cast<float>(st.out_cx);
-    out.cy[i] = static_cast<float>(st.out_cy);
+    out.x[i] = st.out_x;
+    out.y[i] = st.out_y;
+    out.cx[i] = st.out_cx;
+    out.cy[i] = st.out_cy;
     if (out.accumulated_distance) out.accumulated_distance[i] = st.dist;
 }
 

@@ -136,27 +136,33 @@ class GpuScene:
         """The mlgpu_scene handle."""
         return self._h
 
-    _SCHEDULES = {"auto": N.SCHEDULE_AUTO, "fused": N.SCHEDULE_FUSED, "wavefront": N.SCHEDULE_WAVEFRONT}
+    _SCHEDULES = {"auto": N.SCHEDULE_AUTO, "fused": N.SCHEDULE_FUSED, "wavefront": N.SCHEDULE_WAVEFRONT,
+                  "persistent": N.SCHEDULE_PERSISTENT}
 
     def tuning(self) -> dict:
-        """Launch parameters (mlgpu_tuning): schedule, trace_budget, blocks_per_cu, tail_level."""
+        """Launch parameters (mlgpu_tuning): schedule, trace_budget, blocks_per_cu, tail_level, shade_lanes."""
         t = N.Tuning()
         N.check(N.lib().mlgpu_scene_get_tuning(self._h, C.byref(t)), "mlgpu_scene_get_tuning")
         names = {v: k for k, v in self._SCHEDULES.items()}
         return {"schedule": names[t.schedule], "trace_budget": t.trace_budget, "blocks_per_cu": t.blocks_per_cu,
-                "tail_level": t.tail_level}
+                "tail_level": t.tail_level, "shade_lanes": t.shade_lanes}
 
     def set_tuning(self, schedule: Optional[str] = None, trace_budget: Optional[int] = None,
-                   blocks_per_cu: Optional[int] = None, tail_level: Optional[int] = None) -> None:
+                   blocks_per_cu: Optional[int] = None, tail_level: Optional[int] = None,
+                   shade_lanes: Optional[int] = None) -> None:
         """Change launch parameters; results do not depend on them.  schedule is
-        'auto', 'fused' (one lane per photon) or 'wavefront' (trace/shade per
-        bounce level); tail_level (wavefront) is the level from which a
-        photon's remaining bounces run in one lane (<= 0: never)."""
+        'auto', 'fused' (one lane per photon), 'wavefront' (trace/shade per
+        bounce level) or 'persistent' (resident waves, a lane takes the next
+        photon when its photon ends); tail_level (wavefront) is the level from
+        which a photon's remaining bounces run in one lane (<= 0: never);
+        shade_lanes (persistent) is how many lanes of a wave wait with ended
+        traces before the wave shades them."""
         cur = self.tuning()
         t = N.Tuning(self._SCHEDULES[schedule if schedule is not None else cur["schedule"]],
                      trace_budget if trace_budget is not None else cur["trace_budget"],
                      blocks_per_cu if blocks_per_cu is not None else cur["blocks_per_cu"],
-                     tail_level if tail_level is not None else cur["tail_level"])
+                     tail_level if tail_level is not None else cur["tail_level"],
+                     shade_lanes if shade_lanes is not None else cur["shade_lanes"])
         N.check(N.lib().mlgpu_scene_set_tuning(self._h, C.byref(t)), "mlgpu_scene_set_tuning")
 
     def set_schedule(self, schedule: str) -> None:

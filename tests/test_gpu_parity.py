@@ -78,7 +78,7 @@ def test_gpu_matches_reference_golden(cuda, golden, name, scene):
 
 
 @pytest.mark.parametrize("scene", ["lens", "fact", "fact_lg"])
-@pytest.mark.parametrize("schedule", ["fused", "wavefront"])
+@pytest.mark.parametrize("schedule", ["fused", "wavefront", "persistent"])
 def test_gpu_matches_oracle_many_photons(cuda, scene, schedule, oracle_threads):
     n = 200_000
     desc = SCENES[scene]()
@@ -91,7 +91,7 @@ def test_gpu_matches_oracle_many_photons(cuda, scene, schedule, oracle_threads):
     compare(res, ref, n, f"{scene} ({schedule}) vs oracle")
 
 
-@pytest.mark.parametrize("schedule", ["fused", "wavefront"])
+@pytest.mark.parametrize("schedule", ["fused", "wavefront", "persistent"])
 def test_gpu_plenoscope_matches_oracle(cuda, schedule, oracle_threads):
     """Plenoscope (635k leaves, 515,023 lixels): calibration photons from
     1e4 m and a bunch with a pointing offset, drawn by the GPU sources."""
@@ -126,6 +126,15 @@ def test_gpu_schedules_agree_bitwise(cuda, max_int):
         for k in ref:
             np.testing.assert_array_equal(ref[k], res[k],
                                           err_msg=f"{k} (budget {budget}, {blocks} blocks/CU, tail level {tail})")
+    # persistent: quanta of 1, 5 or 32 steps or whole traces, shading at 1 to
+    # 64 waiting lanes, grids from one block per CU
+    for quantum, blocks, lanes in ((16, 8, 32), (1, 1, 1), (5, 3, 64), (0, 8, 16), (32, 16, 48)):
+        gs.set_tuning(schedule="persistent", trace_budget=quantum, blocks_per_cu=blocks, shade_lanes=lanes)
+        res = gs.propagate(ph, PropagationConfig(max_int, seed=5)).numpy()
+        for k in ref:
+            np.testing.assert_array_equal(ref[k], res[k],
+                                          err_msg=f"{k} (persistent, quantum {quantum}, {blocks} blocks/CU, "
+                                                  f"{lanes} shade lanes)")
 
 
 def test_gpu_autotune_picks_a_candidate_and_keeps_results(cuda):
@@ -143,6 +152,10 @@ def test_gpu_autotune_picks_a_candidate_and_keeps_results(cuda):
         assert tuning["trace_budget"] in (0, 12, 16, 24, 32, 48)
         assert tuning["blocks_per_cu"] in (8, 16, 24, 32, 48)
         assert tuning["tail_level"] in (0, 1, 2)
+    elif tuning["schedule"] == "persistent":
+        assert tuning["trace_budget"] in (0, 8, 16, 32)
+        assert tuning["blocks_per_cu"] in (4, 8, 16)
+        assert tuning["shade_lanes"] in (1, 16, 32, 48)
     else:
         assert tuning["schedule"] == "fused"
     res = gs.propagate(ph, PropagationConfig(5, seed=3)).numpy()
@@ -154,11 +167,16 @@ def test_gpu_autotune_picks_a_candidate_and_keeps_results(cuda):
 
 def test_gpu_tuning_is_validated(cuda):
     gs = GpuScene(SCENES["lens"]())
-    assert gs.tuning() == {"schedule": "auto", "trace_budget": 32, "blocks_per_cu": 32, "tail_level": 0}
+    assert gs.tuning() == {"schedule": "auto", "trace_budget": 32, "blocks_per_cu": 32, "tail_level": 0,
+                           "shade_lanes": 32}
     with pytest.raises(MlgpuError):
         gs.set_tuning(blocks_per_cu=0)
     with pytest.raises(MlgpuError):
         gs.set_tuning(blocks_per_cu=65)
+    with pytest.raises(MlgpuError):
+        gs.set_tuning(shade_lanes=0)
+    with pytest.raises(MlgpuError):
+        gs.set_tuning(shade_lanes=65)
     gs.set_tuning(trace_budget=-1)
     assert gs.tuning()["trace_budget"] == -1
 

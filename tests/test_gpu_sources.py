@@ -90,7 +90,7 @@ def test_gpu_bunches_equal_separate_draws_and_oracle(cuda, oracle_threads):
                for k, m in enumerate((30_000, 0, 1, 77_777, 12_345))]
     cfg = PropagationConfig(5, seed=8, photon_id_offset=1000)
     binning = ArrivalBinning(1440, 16, 48e-9, 1e-9, jitter_sigma=1e-9, seed=2, photon_id_offset=5)
-    for schedule in ("fused", "wavefront"):
+    for schedule in ("fused", "wavefront", "persistent"):
         gs.set_tuning(schedule=schedule)
         counts, res = propagate_bunches(gs, bunches, cfg, binning, want_results=True, device=cuda)
         parts = [b.source.generate(b.num_photons, seed=b.seed, device=cuda) for b in bunches]
@@ -117,16 +117,19 @@ def test_gpu_bunches_equal_separate_draws_and_oracle(cuda, oracle_threads):
     np.testing.assert_array_equal(counts2.cpu().numpy(), counts.cpu().numpy())
 
 
-def test_gpu_bunch_plan_replays_match_eager(cuda):
+@pytest.mark.parametrize("schedule", ["wavefront", "persistent"])
+def test_gpu_bunch_plan_replays_match_eager(cuda, schedule):
     """A bunch plan (one hipGraph, replayed per batch with the batch's size,
     seed and id offset read from device memory) gives, replay after replay,
     the histograms and per-photon results of mlgpu_propagate_bunches on the
-    same batch; other work runs eagerly between the replays."""
+    same batch; other work runs eagerly between the replays.  The graph holds
+    the wavefront kernels or the one persistent work-queue kernel."""
     from merlict_development_kit_amd import (ArrivalBinning, Bunch, GpuScene, PropagationConfig, SceneDescription,
                                              propagate_bunches)
     from merlict_development_kit_amd._native import MlgpuError
     from merlict_development_kit_amd.bunches import BunchPlan
     gs = GpuScene(SceneDescription.fact(light_guides=True))
+    gs.set_tuning(schedule=schedule)
     binning = ArrivalBinning(1440, 16, 48e-9, 1e-9, jitter_sigma=1e-9, seed=2, photon_id_offset=5)
     plan = BunchPlan(gs, max_bunches=8, max_photons=300_000, config=PropagationConfig(5, seed=1), binning=binning,
                      device=cuda)

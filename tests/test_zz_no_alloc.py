@@ -30,8 +30,11 @@ def test_gpu_noalloc_propagation_equals_mlgpu_propagate(cuda):
     scene = GpuScene(SceneDescription.fact(light_guides=True))
     src = sources.fact_bunch()
     stream = torch.cuda.current_stream()
-    for call, (n, seed) in enumerate(((max_photons, 5), (100_000, 6))):
-        print(f"call {call}: {n} photons, seed {seed}", flush=True)
+    calls = [(schedule, n, seed) for schedule in ("wavefront", "persistent")
+             for n, seed in ((max_photons, 5), (100_000, 6))]
+    for call, (schedule, n, seed) in enumerate(calls):
+        scene.set_tuning(schedule=schedule)
+        print(f"call {call}: {schedule}, {n} photons, seed {seed}", flush=True)
         ph = src.generate(n, seed=seed, photon_id_offset=1000 * call, device=cuda)
         cfg = PropagationConfig(5, seed=seed + 10, photon_id_offset=77).c_struct()
         want, got = PropagationResult.empty(n, cuda), PropagationResult.empty(n, cuda)
