@@ -684,7 +684,7 @@ __global__ __launch_bounds__(256) void sample_kernel(const SampleArgs a, const i
 constexpr int kTileP = 128;
 constexpr int kHStride = kHidden + 8;
 constexpr int kFStride = 16 + 8;
-constexpr int kOStride = kOutPad + 4;
+constexpr int kOStride = kOut + 8;  // 65, odd
 
 typedef float float2v __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
@@ -707,7 +707,7 @@ __device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
 __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const int64_t n) {
     __shared__ __attribute__((aligned(16))) uint16_t H[kTileP * kHStride];
     __shared__ __attribute__((aligned(16))) uint16_t F16[kTileP * kFStride];
-    __shared__ float F[kTileP * kIn];
+    __shared__ float F[2][kTileP * kIn];  // features of the current and the next tile
     float* O = reinterpret_cast<float*>(H);
     const int t = static_cast<int>(threadIdx.x), lane = t & 63, w = t >> 6;
     const int lr = lane & 31, lk = (lane >> 5) * 8, lq = (lane >> 5) * 4;
@@ -721,8 +721,37 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
         fc_lds = feature_consts(a.s.dom);
     }
     __syncthreads();
-    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kTileP; base < n;
-         base += static_cast<int64_t>(gridDim.x) * kTileP) {
+    // features of the tile at `base` into Fb and F16, by one thread per photon
+    auto features = [&](const int64_t base, const int p, float* Fb) {
+        const int64_t i = base + p;
+        float* f = Fb + p * kIn;
+        if (i < n) {
+            if (a.features) {
+#pragma unroll
+                for (int j = 0; j < kIn; j++) f[j] = a.features[i * kIn + j];
+            } else {
+                const FeatureConsts fc = fc_lds;
+                photon_features(fc, a.ph, i, f);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kIn; j++) f[j] = 0.0f;
+        }
+        uint32_t* row = reinterpret_cast<uint32_t*>(F16 + p * kFStride);
+        row[0] = pack_bf16(f[0], f[1]);
+        row[1] = pack_bf16(f[2], f[3]);
+        row[2] = pack_bf16(f[4], 0.0f);
+#pragma unroll
+        for (int j = 3; j < 8; j++) row[j] = 0u;
+    };
+    // Waves 2 and 3 compute a tile's features while waves 0 and 1 draw the
+    // previous tile's photons (F double-buffered; F16 is free once layer 0
+    // has read it).
+    const int64_t step = static_cast<int64_t>(gridDim.x) * kTileP;
+    int buf = 0;
+    if (t >= kTileP && static_cast<int64_t>(blockIdx.x) * kTileP < n)
+        features(static_cast<int64_t>(blockIdx.x) * kTileP, t - kTileP, F[0]);
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kTileP; base < n; base += step, buf ^= 1) {
         // weight fragments of the first k-steps of the next layer, loaded
         // ahead of the barrier that precedes it (here layer 0: one k-step)
         bf16x8 pf0, pf1, pf2, pf3;
@@ -732,28 +761,6 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
             pf1 = P[64];
             pf2 = pf0;
             pf3 = pf1;
-        }
-        if (t < kTileP) {
-            const int64_t i = base + t;
-            float* f = F + t * kIn;
-            if (i < n) {
-                if (a.features) {
-#pragma unroll
-                    for (int j = 0; j < kIn; j++) f[j] = a.features[i * kIn + j];
-                } else {
-                    const FeatureConsts fc = fc_lds;
-                    photon_features(fc, a.ph, i, f);
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < kIn; j++) f[j] = 0.0f;
-            }
-            uint32_t* row = reinterpret_cast<uint32_t*>(F16 + t * kFStride);
-            row[0] = pack_bf16(f[0], f[1]);
-            row[1] = pack_bf16(f[2], f[3]);
-            row[2] = pack_bf16(f[4], 0.0f);
-#pragma unroll
-            for (int j = 3; j < 8; j++) row[j] = 0u;
         }
         __syncthreads();
         // layer 0 (K = 16, five of them non-zero) and the hidden layers.
@@ -919,18 +926,17 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
                 mma(wb, hb, false);
             }
             __syncthreads();
+            // an odd row stride: the draws read one photon's outputs per
+            // lane without LDS bank collisions
 #pragma unroll
             for (int g = 0; g < 4; g++) {
                 const int col = ot * 32 + g * 8 + lq;
 #pragma unroll
                 for (int p = 0; p < 2; p++) {
                     const int row = (pt0 + p) * 32 + lr;
-                    float4 z;
-                    z.x = acc[p][4 * g + 0];
-                    z.y = acc[p][4 * g + 1];
-                    z.z = acc[p][4 * g + 2];
-                    z.w = acc[p][4 * g + 3];
-                    *reinterpret_cast<float4*>(O + row * kOStride + col) = z;
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        if (col + e < kOut) O[row * kOStride + col + e] = acc[p][4 * g + e];
                 }
             }
             __syncthreads();
@@ -941,11 +947,15 @@ __global__ __launch_bounds__(256, 2) void fused_kernel(const FusedArgs a, const 
                 if (base + row < n) a.out[(base + row) * kOut + col] = O[row * kOStride + col];
             }
         }
-        if (a.sample && w < 2) {
-            const int64_t i = base + t;
-            const DrawConsts dk = dk_lds;
-            const bool fallback = i < n && sample_one(a.s, dk, F + t * kIn, O + t * kOStride, i);
-            append_fallback(a.s, fallback, i);
+        if (t < kTileP) {
+            if (a.sample) {
+                const int64_t i = base + t;
+                const DrawConsts dk = dk_lds;
+                const bool fallback = i < n && sample_one(a.s, dk, F[buf] + t * kIn, O + t * kOStride, i);
+                append_fallback(a.s, fallback, i);
+            }
+        } else if (base + step < n) {
+            features(base + step, t - kTileP, F[buf ^ 1]);
         }
         __syncthreads();
     }
