@@ -58,7 +58,11 @@ enum mlgpu_kind {
     MLGPU_KIND_CYLINDER = 9,            /* p0 radius, p1 length (along z)   */
     MLGPU_KIND_TRIANGLE = 10,           /* p0..p5 Ax Ay Bx By Cx Cy         */
     MLGPU_KIND_SPHERE = 11,             /* p0 radius                        */
-    MLGPU_NUM_KINDS = 12
+    MLGPU_KIND_ELLIPTICAL_CAP_HEX = 12, /* p0 focal length, p1 ratio of the
+                                           x to the y curvature radius, p2
+                                           hex rotation (kept, unused as in
+                                           merlict), p3 outer hex radius    */
+    MLGPU_NUM_KINDS = 13
 };
 
 enum mlgpu_rot_mode {

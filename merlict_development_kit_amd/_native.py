@@ -27,6 +27,7 @@ KIND = {
     "Cylinder": 9,
     "Triangle": 10,
     "Sphere": 11,
+    "EllipticalCapWithHexagonalBound": 12,
 }
 
 # merlict::Interaction (merlict/RayForPropagation.h:14-28)

@@ -218,6 +218,37 @@ MLGPU_HD inline bool intersect_leaf(const DeviceScene& sc, const LeafGeo* __rest
         const D3 p = at(s, d, t);
         if (!(fabs(p.z) <= g.p[1])) return false;
         if (!differs(s, p)) return false;
+    } else if (kind == MLGPU_KIND_ELLIPTICAL_CAP_HEX) {
+        // EllipticalCapRayIntersectionEquation.cpp:8-64 (the ellipsoid
+        // x^2/A^2 + y^2/B^2 + (z - C)^2/C^2 = 1 through the origin; p0 = C,
+        // p2..p4 = 1/A^2, 1/B^2, 1/C^2) and add_causeal_intersection with
+        // the hexagonal bound, as for the spherical caps
+        const double C = g.p[0], iAA = g.p[2], iBB = g.p[3], iCC = g.p[4];
+        const double qa = (d.x * d.x) * iAA + (d.y * d.y) * iBB + (d.z * d.z) * iCC;
+        const double qb = 2.0 * ((s.x * d.x) * iAA + (s.y * d.y) * iBB + d.z * (s.z - C) * iCC);
+        const double qc = (s.x * s.x) * iAA + (s.y * s.y) * iBB + (s.z * s.z - 2.0 * s.z * C) * iCC;
+        const double P = qb / qa, Q = qc / qa;
+        const double p2 = 0.5 * P;
+        const double inner = p2 * p2 - Q;
+        if (!(inner >= 0.0)) return false;
+        const double sq = sqrt(inner);
+        const double minus = -p2 - sq, plus = -p2 + sq;
+        const D3 pi = at(s, d, plus), mi = at(s, d, minus);
+        const bool p_in = hex_inside(sc, g.p[1], pi), m_in = hex_inside(sc, g.p[1], mi);
+        const double rb = g.p[7];
+        const bool p_c = plus > 0.0 && dot(pi, pi) <= rb * rb;
+        const bool m_c = minus > 0.0 && dot(mi, mi) <= rb * rb;
+        bool ok = false;
+        t = minus;
+        if (p_in && m_in) {
+            if (p_c && m_c) { t = (minus >= plus) ? plus : minus; ok = true; }
+            else if (p_c) { t = plus; ok = true; }
+            else if (m_c) { t = minus; ok = true; }
+        } else if (p_in && p_c) { t = plus; ok = true; }
+        else if (m_in && m_c) { t = minus; ok = true; }
+        if (!ok) return false;
+        const D3 p = at(s, d, t);
+        if (!differs(s, p)) return false;
     } else if (kind == MLGPU_KIND_SPHERE) {
         // Sphere.cpp:62-105
         const double sd = dot(s, d), d2 = dot(d, d), ss = dot(s, s);
@@ -280,6 +311,14 @@ MLGPU_HD inline D3 leaf_normal(const LeafGeo& g, const D3& p) {
     }
     if (kind == MLGPU_KIND_CYLINDER) return normalized(D3{p.x, p.y, 0.0});  // Cylinder.cpp:157-160
     if (kind == MLGPU_KIND_SPHERE) return normalized(p);                    // Sphere.cpp:80-81
+    if (kind == MLGPU_KIND_ELLIPTICAL_CAP_HEX) {
+        // EllipticalCapRayIntersectionEquation.cpp:72-99, as merlict writes
+        // it: the factor takes sqrt(1 - x^2/A^2 - y^2/B^2) (not its inverse)
+        // and both slopes use 1/A^2
+        const double C = g.p[0], iAA = g.p[2], iBB = g.p[3];
+        const double f = C * 0.5 * sqrt(1.0 - (p.x * p.x) * iAA - (p.y * p.y) * iBB);
+        return normalized(D3{f * (-2.0 * p.x * iAA), f * (-2.0 * p.y * iAA), 1.0});
+    }
     return D3{0.0, 0.0, 1.0};
 }
 

@@ -311,6 +311,23 @@ double leaf_params(int kind, const double* p, double* out) {
     case MLGPU_KIND_SPHERE:
         out[0] = p[0];
         return p[0];
+    case MLGPU_KIND_ELLIPTICAL_CAP_HEX: {
+        // EllipticalCapWithHexagonalBound.cpp:10-52: the radii of curvature,
+        // the outer hex radius restricted to the z one before the bound
+        // takes it, the enclosing sphere of the restricted hexagon
+        const double mean = p[0] * 0.5;
+        const double X = mean * p[1], Y = mean / p[1], Z = 0.5 * (X + Y);
+        const double r = p[3] > Z ? Z : p[3];
+        const double radius = std::hypot(r, Z - std::sqrt(Z * Z - r * r));
+        out[0] = Z;
+        out[1] = r * std::cos(M_PI / 6.0);
+        out[2] = 1.0 / (X * X);
+        out[3] = 1.0 / (Y * Y);
+        out[4] = 1.0 / (Z * Z);
+        out[5] = r;
+        out[7] = radius;
+        return radius;
+    }
     default:
         return 0.0;
     }
@@ -373,6 +390,15 @@ Box local_box(int kind, const double* p, const double* prm, double radius) {
         b.grow(hi);
         return b;
     }
+    case MLGPU_KIND_ELLIPTICAL_CAP_HEX:
+        // hits lie within the enclosing sphere, inside the hexagon (corners
+        // (+-r_o, 0), (+-r_o / 2, +-r_i)) and on the ellipsoid, whose points
+        // have 0 <= z <= 2 C
+        e[0] = std::min(radius, prm[5]);
+        e[1] = std::min(radius, prm[1]);
+        z0 = 0.0;
+        z1 = std::min(radius, 2.0 * prm[0]);
+        break;
     case MLGPU_KIND_CYLINDER: e[0] = e[1] = p[0]; z0 = -0.5 * p[1]; z1 = 0.5 * p[1]; break;
     case MLGPU_KIND_SPHERE: e[0] = e[1] = p[0]; z0 = -p[0]; z1 = p[0]; break;
     default: return b;  // a frame without surface is never hit

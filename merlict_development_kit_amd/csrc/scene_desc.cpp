@@ -1,5 +1,6 @@
 // Owned scene descriptions: validation, binary file I/O and the C ABI around
 // them (include/mlgpu.h).
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -74,6 +75,11 @@ void validate_desc(const mlgpu_scene_desc& d) {
         case MLGPU_KIND_CYLINDER:
             if (!(p[0] > 0.0) || !(p[1] > 0.0))
                 throw std::invalid_argument("frame " + std::to_string(i) + ": cylinder radius/length must be > 0");
+            break;
+        case MLGPU_KIND_ELLIPTICAL_CAP_HEX:
+            if (!(p[0] > 0.0) || !(p[1] > 0.0) || !(p[3] > 0.0) || !std::isfinite(p[2]))
+                throw std::invalid_argument("frame " + std::to_string(i) +
+                                            ": focal length, radii ratio and outer hex radius must be > 0");
             break;
         default: break;
         }

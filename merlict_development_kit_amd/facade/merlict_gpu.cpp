@@ -64,6 +64,10 @@ MLGPU_EXPOSE(TriangleA, Triangle, Vec2, A);
 MLGPU_EXPOSE(TriangleB, Triangle, Vec2, B);
 MLGPU_EXPOSE(TriangleC, Triangle, Vec2, C);
 MLGPU_EXPOSE(SphereRadius, Sphere, double, radius);
+MLGPU_EXPOSE(EllipFocal, EllipticalCapWithHexagonalBound, double, focal_length);
+MLGPU_EXPOSE(EllipRatio, EllipticalCapWithHexagonalBound, double, ratio_x_to_y_curvature_radius);
+MLGPU_EXPOSE(EllipRotation, EllipticalCapWithHexagonalBound, double, hex_bound_rotation);
+MLGPU_EXPOSE(EllipHexRadius, EllipticalCapWithHexagonalBound, double, outer_hex_radius);
 MLGPU_EXPOSE(RaySupport, Ray, Vec3, support_);
 MLGPU_EXPOSE(RayDirection, Ray, Vec3, direction_);
 
@@ -206,6 +210,16 @@ int32_t primitive_of(const Frame* f, double* p) {
     if (t == typeid(Sphere)) {
         p[0] = static_cast<const Sphere&>(*f).*member(SphereRadius());
         return MLGPU_KIND_SPHERE;
+    }
+    if (t == typeid(EllipticalCapWithHexagonalBound)) {
+        // the stored outer radius is already restricted to the z curvature
+        // radius; restricting it again changes nothing
+        const auto& s = static_cast<const EllipticalCapWithHexagonalBound&>(*f);
+        p[0] = s.*member(EllipFocal());
+        p[1] = s.*member(EllipRatio());
+        p[2] = s.*member(EllipRotation());
+        p[3] = s.*member(EllipHexRadius());
+        return MLGPU_KIND_ELLIPTICAL_CAP_HEX;
     }
     // frames that are never intersected themselves: plain frames, surface
     // entities without a geometry of their own, and merlict's composites,

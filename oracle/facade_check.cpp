@@ -25,8 +25,9 @@
 //       binding raises std::out_of_range as Func1::evaluate does; the photons
 //       before it are propagated, those after it untouched.
 //   unsupported
-//       an EllipticalCapWithHexagonalBound in the tree: std::invalid_argument
-//       before any GPU work (runs without a GPU).
+//       a surface class of the caller's own (derived from a merlict
+//       primitive) in the tree: std::invalid_argument before any GPU work
+//       (runs without a GPU).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -274,10 +275,19 @@ void build_api_scene(ApiScene* a, bool unsupported) {
     ball->outer_reflection = sc.functions.get("half");
     ball->set_radius(0.04);
 
+    ml::EllipticalCapWithHexagonalBound* ell = sc.root.add<ml::EllipticalCapWithHexagonalBound>();
+    ell->set_name_pos_rot("elliptic", ml::Vec3(-0.1, -0.08, 1.3), ml::Rot3(0.05, -0.1, 0.2));
+    ell->outer_reflection = sc.functions.get("half");
+    ell->inner_reflection = sc.functions.get("half");
+    ell->set_focalLength_radiiRatio_hexAngel_hexRadius(1.0, 1.2, 0.3, 0.08);
+
     if (unsupported) {
-        ml::EllipticalCapWithHexagonalBound* cap = sc.root.add<ml::EllipticalCapWithHexagonalBound>();
-        cap->set_name_pos_rot("elliptic", ml::Vec3(0.0, 0.0, 2.0), ml::Rot3(0.0, 0.0, 0.0));
-        cap->set_focalLength_radiiRatio_hexAngel_hexRadius(1.0, 1.2, 0.0, 0.1);
+        // a class of the caller's own: its intersections are not merlict's
+        // (its exact type is not one the binding knows), so it is refused
+        struct OwnCap : public ml::SphereCapWithHexagonalBound {};
+        OwnCap* cap = sc.root.add<OwnCap>();
+        cap->set_name_pos_rot("own_cap", ml::Vec3(0.0, 0.0, 2.0), ml::Rot3(0.0, 0.0, 0.0));
+        cap->set_curvature_radius_and_outer_hex_radius(1.0, 0.1);
     }
 
     // 48 sensor cells at z = 1.5 (more than 16 children: merlict clusters
@@ -381,7 +391,7 @@ int cmd_unsupported() {
         what = "other";
         msg = e.what();
     }
-    const bool names_frame = msg.find("elliptic") != std::string::npos;
+    const bool names_frame = msg.find("own_cap") != std::string::npos;
     std::printf("{\"mode\": \"unsupported\", \"gpu\": \"%s\", \"names_frame\": %s, \"num_interactions\": %u}\n",
                 what.c_str(), names_frame ? "true" : "false", ph[0].num_interactions());
     return 0;

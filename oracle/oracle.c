@@ -202,6 +202,7 @@ typedef struct {
     double cyl_r2_inner;
     double half_x, half_y;  /* RectangularPrismZ                     */
     double ds_dist, ds_r;   /* DualSphericalPrismZ                   */
+    double ell_C, ell_iAA, ell_iBB, ell_iCC;  /* EllipticalCapWithHexagonalBound */
 } oframe;
 
 struct oracle_scene {
@@ -315,6 +316,22 @@ static void primitive_init(oframe* f) {
     case MLGPU_KIND_SPHERE:               /* Sphere.cpp:13-21 */
         f->radius = p[0];
         break;
+    case MLGPU_KIND_ELLIPTICAL_CAP_HEX: { /* EllipticalCapWithHexagonalBound.cpp:10-52 */
+        /* p: focal length, x/y curvature ratio, hex rotation (unused), outer hex radius */
+        const double mean_curvature_radius = p[0] * 0.5;
+        const double X = mean_curvature_radius * p[1];
+        const double Y = mean_curvature_radius / p[1];
+        const double Z = 0.5 * (X + Y);
+        double outer = p[3];
+        f->ell_C = Z;
+        f->ell_iAA = 1.0 / (X * X);
+        f->ell_iBB = 1.0 / (Y * Y);
+        f->ell_iCC = 1.0 / (Z * Z);
+        if (outer > Z) outer = Z;             /* restricted before hexBounds takes it */
+        f->hex_inner = outer * cos(M_PI / 6.0);
+        f->radius = hypot(outer, Z - sqrt(Z * Z - outer * outer));
+        break;
+    }
     default:
         f->radius = 0.0;
     }
@@ -583,7 +600,8 @@ static int hex_inside(double inner, v3 v) {
 
 static int bound_inside(const oframe* f, v3 v) {
     switch (f->kind) {
-    case MLGPU_KIND_SPHERE_CAP_HEX: return hex_inside(f->hex_inner, v);
+    case MLGPU_KIND_SPHERE_CAP_HEX:
+    case MLGPU_KIND_ELLIPTICAL_CAP_HEX: return hex_inside(f->hex_inner, v);
     case MLGPU_KIND_SPHERE_CAP_CYL: return v.x * v.x + v.y * v.y < f->cyl_r2;
     case MLGPU_KIND_SPHERE_CAP_RECT: return fabs(v.x) < f->half_x && fabs(v.y) < f->half_y;
     default: return 0;
@@ -622,6 +640,52 @@ static int intersect_cap(const oframe* f, int fi, v3 s, v3 d, isect* out) {
     v3 n = sub(V(0.0, 0.0, R), p);
     n = dvd(n, norm(n));
     isect_make(out, fi, p, n, t, d);
+    return 1;
+}
+
+/* the causal solution of SurfaceWithOuterPrismBound::add_causeal_intersection
+ * (SurfaceWithOuterPrismBound.cpp:7-70) among the two roots of a surface
+ * equation: 1 and *t set when one is inside the bound and the enclosing sphere */
+static int causal_root(const oframe* f, v3 s, v3 d, double minus, double plus, double* t_out) {
+    const v3 pi = add(s, mul(d, plus)), mi = add(s, mul(d, minus));
+    const int p_in = bound_inside(f, pi), m_in = bound_inside(f, mi);
+    const int p_c = plus > 0.0 && norm_le(pi, f->radius);
+    const int m_c = minus > 0.0 && norm_le(mi, f->radius);
+    if (p_in && m_in) {
+        if (p_c && m_c) { *t_out = (minus >= plus) ? plus : minus; return 1; }
+        if (p_c) { *t_out = plus; return 1; }
+        if (m_c) { *t_out = minus; return 1; }
+        return 0;
+    }
+    if (p_in && p_c) { *t_out = plus; return 1; }
+    if (m_in && m_c) { *t_out = minus; return 1; }
+    return 0;
+}
+
+/* EllipticalCapRayIntersectionEquation.cpp:8-99 (the ellipsoid
+ * x^2/A^2 + y^2/B^2 + (z-C)^2/C^2 = 1; its normal as merlict computes it:
+ * factor C/2 sqrt(1 - x^2/A^2 - y^2/B^2), both slopes with 1/A^2) +
+ * add_causeal_intersection with the hexagonal bound */
+static int intersect_elliptical_cap(const oframe* f, int fi, v3 s, v3 d, isect* out) {
+    const double C = f->ell_C, iAA = f->ell_iAA, iBB = f->ell_iBB, iCC = f->ell_iCC;
+    const double a = (d.x * d.x) * iAA + (d.y * d.y) * iBB + (d.z * d.z) * iCC;
+    const double b = 2.0 * ((s.x * d.x) * iAA + (s.y * d.y) * iBB + d.z * (s.z - C) * iCC);
+    const double c = (s.x * s.x) * iAA + (s.y * s.y) * iBB + (s.z * s.z - 2.0 * s.z * C) * iCC;
+    const double p_over_2 = 0.5 * (b / a);
+    const double inner = p_over_2 * p_over_2 - c / a;
+    double t;
+    if (!(inner >= 0.0)) return 0;
+    {
+        const double sq = sqrt(inner);
+        if (!causal_root(f, s, d, -p_over_2 - sq, -p_over_2 + sq, &t)) return 0;
+    }
+    const v3 p = add(s, mul(d, t));
+    if (!vec_neq(s, p)) return 0;
+    {
+        const double factor = C * 0.5 * sqrt(1.0 - (p.x * p.x) * iAA - (p.y * p.y) * iBB);
+        const v3 n = normalized(V(factor * (-2.0 * p.x * iAA), factor * (-2.0 * p.y * iAA), 1.0));
+        isect_make(out, fi, p, n, t, d);
+    }
     return 1;
 }
 
@@ -720,7 +784,8 @@ static int intersect_leaf(const oframe* f, int fi, v3 s, v3 d, isect* out) {
         switch (f->kind) {
         case MLGPU_KIND_SPHERE_CAP_HEX:
         case MLGPU_KIND_SPHERE_CAP_CYL:
-        case MLGPU_KIND_SPHERE_CAP_RECT: g_work->leaf_cap++; break;
+        case MLGPU_KIND_SPHERE_CAP_RECT:
+        case MLGPU_KIND_ELLIPTICAL_CAP_HEX: g_work->leaf_cap++; break;
         case MLGPU_KIND_CYLINDER: g_work->leaf_cylinder++; break;
         case MLGPU_KIND_SPHERE: g_work->leaf_sphere++; break;
         default: g_work->leaf_planar++;
@@ -738,6 +803,7 @@ static int intersect_leaf(const oframe* f, int fi, v3 s, v3 d, isect* out) {
     case MLGPU_KIND_TRIANGLE: return intersect_planar(f, fi, s, d, out);
     case MLGPU_KIND_CYLINDER: return intersect_cylinder(f, fi, s, d, out);
     case MLGPU_KIND_SPHERE: return intersect_sphere(f, fi, s, d, out);
+    case MLGPU_KIND_ELLIPTICAL_CAP_HEX: return intersect_elliptical_cap(f, fi, s, d, out);
     default: return 0;
     }
 }

@@ -161,6 +161,7 @@ static void build(const Desc& d, RefScene* rs) {
         case 9: { auto* s = add_surface<ml::Cylinder>(m, d, i, rs); f = s; break; }
         case 10: { auto* s = add_surface<ml::Triangle>(m, d, i, rs); f = s; break; }
         case 11: { auto* s = add_surface<ml::Sphere>(m, d, i, rs); f = s; break; }
+        case 12: { auto* s = add_surface<ml::EllipticalCapWithHexagonalBound>(m, d, i, rs); f = s; break; }
         default: throw std::runtime_error("unknown kind");
         }
         const ml::Rot3 rot = d.rot_mode[i] == 0 ? ml::Rot3(d.rot[4 * i], d.rot[4 * i + 1], d.rot[4 * i + 2])
@@ -179,6 +180,10 @@ static void build(const Desc& d, RefScene* rs) {
         case 9: static_cast<ml::Cylinder*>(f)->set_radius_and_length(p[0], p[1]); break;
         case 10: static_cast<ml::Triangle*>(f)->set_corners_in_xy_plane(p[0], p[1], p[2], p[3], p[4], p[5]); break;
         case 11: static_cast<ml::Sphere*>(f)->set_radius(p[0]); break;
+        case 12:
+            static_cast<ml::EllipticalCapWithHexagonalBound*>(f)->set_focalLength_radiiRatio_hexAngel_hexRadius(
+                p[0], p[1], p[2], p[3]);
+            break;
         default: break;
         }
         rs->frames[i] = f;

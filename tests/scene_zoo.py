@@ -1,7 +1,9 @@
 """A scene holding every primitive kind, for first-hit and propagation
 checks of the kinds the telescope scenes do not use: caps with hexagonal,
 circular and rectangular bounds, plane, disc, hexagonal plane, annulus,
-dual-spherical plane, cylinder, triangle and sphere, in both rotation modes,
+dual-spherical plane, cylinder, triangle, sphere and elliptical caps with
+hexagonal bounds (one whose outer radius is restricted to its z curvature
+radius), in both rotation modes,
 a nested frame, a group of more than 16 children (merlict's octant
 clustering) and coplanar overlapping surfaces (exact distance ties)."""
 from __future__ import annotations
@@ -10,7 +12,7 @@ import numpy as np
 
 from merlict_development_kit_amd import SceneDescription
 
-FRAME, CAP_HEX, CAP_CYL, CAP_RECT, PLANE, DISC, HEX, ANNULUS, DUAL, CYL, TRI, SPHERE = range(12)
+FRAME, CAP_HEX, CAP_CYL, CAP_RECT, PLANE, DISC, HEX, ANNULUS, DUAL, CYL, TRI, SPHERE, ELLIP_CAP = range(13)
 
 
 def zoo() -> SceneDescription:
@@ -35,6 +37,10 @@ def zoo() -> SceneDescription:
     add(root, (2.0, 2.0, 0.0), CYL, (0.1, 0.5), rot=(1.2, 0.0, 0.0), surface=mirror)
     add(root, (0.0, 3.0, 0.0), TRI, (-0.2, -0.1, 0.25, -0.15, 0.0, 0.3), rot=(0.1, -0.1, 0.7))
     add(root, (1.0, 3.0, 0.0), SPHERE, (0.2,), surface=glass)
+    # elliptical caps: focal length, x/y curvature ratio, hex rotation, outer
+    # hex radius; the second one's 0.5 m exceeds its z curvature radius
+    add(root, (1.0, 4.0, 0.0), ELLIP_CAP, (4.0, 1.2, 0.3, 0.3), rot=(0.1, -0.2, 0.3), surface=glass)
+    add(root, (2.0, 4.0, 0.0), ELLIP_CAP, (0.8, 1.3, 0.0, 0.5), rot=(0.0, 1.0, 0.0, 0.4), mode=1, surface=mirror)
     nest = add(root, (2.0, 3.0, 0.2), FRAME, rot=(0.0, 0.0, 0.5))
     add(nest, (0.05, 0.0, 0.0), DISC, (0.15,))
     add(nest, (-0.05, 0.0, -0.1), SPHERE, (0.08,), surface=glass)
@@ -59,7 +65,8 @@ def zoo() -> SceneDescription:
 def probe_rays(n: int, seed: int) -> dict:
     """Rays from all around towards random points of the zoo."""
     rng = np.random.default_rng(seed)
-    centres = np.array([(x, y, 0.0) for y in range(4) for x in range(3)] + [(3.2, 1.5, 0.0), (-1.0, 1.5, -0.2)])
+    centres = np.array([(x, y, 0.0) for y in range(4) for x in range(3)] + [(3.2, 1.5, 0.0), (-1.0, 1.5, -0.2)] +
+                       [(1.0, 4.0, 0.0), (2.0, 4.0, 0.2)])
     target = centres[rng.integers(0, len(centres), n)] + rng.uniform(-1, 1, (n, 3)) * (0.4, 0.4, 0.15)
     u = rng.normal(size=(n, 3))
     u /= np.linalg.norm(u, axis=1, keepdims=True)

@@ -67,7 +67,7 @@ def test_oracle_zoo_first_hits_match_reference(golden):
     np.testing.assert_array_equal(h["distance"][hit], g["out_distance"][hit])
     np.testing.assert_array_equal(h["position"][hit], g["out_position"][hit])
     kinds = scene_zoo.zoo().arrays()["kind"][g["out_frame"][hit]]
-    assert set(np.unique(kinds)) == set(range(1, 12))  # every surface kind is hit
+    assert set(np.unique(kinds)) == set(range(1, 13))  # every surface kind is hit
 
 
 def test_oracle_tree_matches_reference(golden):
