@@ -601,6 +601,16 @@ typedef struct mlgpu_hex_cells {
     double inner_radius;
 } mlgpu_hex_cells;
 
+/* Factor (> 0, default 1) on the training loss's gradient with respect to
+ * the survival logit s, in mlgpu_surrogate_gradients and
+ * mlgpu_surrogate_train_step; the loss value is unchanged.  The survival's
+ * dependence on the wavelength (the mirror reflectivity, a few per cent)
+ * is a small signal next to the mixture's position terms, and the shared
+ * layers learn it only when its gradient is scaled up (10 in
+ * surrogate.train).  A stationary point of the survival's own parameters
+ * is the same for every factor.  No counterpart in the reference. */
+int mlgpu_surrogate_set_survival_scale(mlgpu_surrogate* s, float scale);
+
 /* The sensor cells of the surrogate's draws (mlgpu_propagate_hybrid: a
  * drawn position off the cells is a photon that misses the sensors).  The
  * model's probability of a detection on cell c is sigmoid(s) times the

@@ -268,6 +268,7 @@ SIGNATURES = {
     "mlgpu_surrogate_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p,
                                              C.c_void_p]),
     "mlgpu_surrogate_set_region": (C.c_int, [C.c_void_p, C.POINTER(SurrogateDomain), C.POINTER(HexCells)]),
+    "mlgpu_surrogate_set_survival_scale": (C.c_int, [C.c_void_p, C.c_float]),
     "mlgpu_surrogate_prepare": (C.c_int, [C.POINTER(SurrogateDomain), C.POINTER(Photons), C.POINTER(Results),
                                           C.c_void_p, C.POINTER(Results), C.c_void_p, C.c_int64, C.c_void_p,
                                           C.c_void_p, C.c_void_p]),

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void mixture_loss_kernel(const float* __restri
             const float s = o[0];
             // survival: binary cross entropy of sigmoid(s)
             loss = fmaxf(s, 0.0f) + log1pf(expf(-fabsf(s))) - d * s;
-            g[0] = 1.0f / (1.0f + expf(-s)) - d;
+            g[0] = reg.survival_scale * (1.0f / (1.0f + expf(-s)) - d);
             if (d > 0.5f) {
                 const float y[3] = {targets[b * 4 + 1], targets[b * 4 + 2], targets[b * 4 + 3]};
                 float wmax = -INFINITY;
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void mixture_loss_kernel(const float* __restri
                 const float sg = 1.0f / (1.0f + expf(-s)), om = 1.0f / (1.0f + expf(s));
                 const float val = fmaxf(om + sg * (1.0f - Z), 1e-30f);
                 loss = -logf(val);
-                g[0] = sg * om * Z / val;
+                g[0] = reg.survival_scale * sg * om * Z / val;
                 const float gz = -sg / val;
                 for (int j = 1; j < kOutPad; j++) g[j] = gz * dz[j];
             }

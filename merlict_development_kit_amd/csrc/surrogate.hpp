@@ -82,6 +82,7 @@ struct LossRegion {
     mlgpu_hex_cells cells;
     const float* features;  // B x kIn fp32, for the photons' image points
     uint32_t rotation_key;  // varies the quadrature's rotation between steps
+    float survival_scale;   // factor on the gradient wrt the survival logit
 };
 
 // Loss of the outputs (B x kOutPad fp32) for targets (B x 4: detected,

@@ -24,6 +24,9 @@ struct mlgpu_surrogate {
     int64_t step = 0;
     bool fp8 = false;  // inference precision of layers 1.. (mlgpu_surrogate_set_inference)
     std::mutex mutex;  // guards the batch workspace, the step counter and the region
+    // factor on the loss gradient wrt the survival logit
+    // (mlgpu_surrogate_set_survival_scale)
+    float survival_scale = 1.0f;
     // truncation region of the training loss (mlgpu_surrogate_set_region)
     bool region_on = false;
     mlgpu_surrogate_domain region_dom{};
@@ -149,6 +152,7 @@ hipError_t backward(mlgpu_surrogate* s, const float* features, const float* targ
     reg.cells = s->region_cells;
     reg.features = features;
     reg.rotation_key = static_cast<uint32_t>(static_cast<uint64_t>(s->step) * 0x9E3779B97F4A7C15ull >> 32);
+    reg.survival_scale = s->survival_scale;
     hipError_t e = hipMemsetAsync(s->loss, 0, sizeof(float), st);
     if (e == hipSuccess) e = forward(s, rows, st);
     if (e == hipSuccess) e = launch_mixture_loss(s->out, targets, n, rows, reg, s->gout, s->loss, st);
@@ -417,6 +421,17 @@ int mlgpu_surrogate_train_step(mlgpu_surrogate* s, const float* features, const 
     if (e == hipSuccess) e = launch_pad_weights(s->params, s->pw, st);
     if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, s->loss, 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "surrogate train step");
+    return MLGPU_OK;
+}
+
+int mlgpu_surrogate_set_survival_scale(mlgpu_surrogate* s, float scale) {
+    if (!s) { set_last_error("null surrogate"); return MLGPU_ERR_ARGUMENT; }
+    if (!(scale > 0.0f) || !std::isfinite(scale)) {
+        set_last_error("survival scale must be finite and > 0");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(s->mutex);
+    s->survival_scale = scale;
     return MLGPU_OK;
 }
 

@@ -99,6 +99,12 @@ class Surrogate:
                                                   _ptr(loss), _stream(stream)), "mlgpu_surrogate_gradients")
         return loss / len(features), g
 
+    def set_survival_scale(self, scale: float) -> None:
+        """Factor on the loss gradient wrt the survival logit in training
+        (mlgpu_surrogate_set_survival_scale; the loss value is unchanged)."""
+        N.check(N.lib().mlgpu_surrogate_set_survival_scale(self._h, float(scale)),
+                "mlgpu_surrogate_set_survival_scale")
+
     def train_step(self, features: torch.Tensor, targets: torch.Tensor, learning_rate: float,
                    loss: torch.Tensor = None, stream=None) -> None:
         """One Adam step; `loss` (1-element float32 on the GPU) receives the
@@ -289,7 +295,7 @@ def prepare(domain: SurrogateDomain, photons: PhotonBatch, result: PropagationRe
 def train(scene: GpuScene, desc: SceneDescription, surrogate: Surrogate, domain: SurrogateDomain, n_photons: int,
           height: float = 10.0, batch: int = 16384, learning_rate: float = 1e-3, final_learning_rate: float = 3e-5,
           seed: int = 0, chunk: int = 1 << 21, max_interactions: int = 5, device="cuda", log_every: int = 0,
-          cells: "HexCells" = None, focal_desc: SceneDescription = None):
+          cells: "HexCells" = None, focal_desc: SceneDescription = None, survival_scale: float = 10.0):
     """Train on n_photons drawn from the domain and propagated by the plain
     propagator (Philox seed `seed`, ids 0 .. n_photons - 1), Adam with a
     cosine learning-rate decay.  With `cells`, lost photons are explained by
@@ -297,9 +303,13 @@ def train(scene: GpuScene, desc: SceneDescription, surrogate: Surrogate, domain:
     hybrid's draws treat them.  With `focal_desc` (the scene with a sensor at
     the focal plane, e.g. SceneDescription.fact(focal_plane=True)) the same
     photons are propagated through it too and the position targets are
-    their focal-plane crossings.  Returns the domain (its time offset and
-    scale set from the data when they were unset) and the loss history
-    (mean loss per batch, every log_every steps)."""
+    their focal-plane crossings.  The gradient wrt the survival logit is
+    scaled by `survival_scale` (Surrogate.set_survival_scale): unscaled, the
+    shared layers barely learn the survival's few-per-cent dependence on the
+    wavelength (the mirror's reflectivity) next to the position terms.
+    Returns the domain (its time offset and scale set from the data when
+    they were unset) and the loss history (mean loss per batch, every
+    log_every steps)."""
     poses = torch.tensor(desc.frame_poses().reshape(-1), dtype=torch.float64, device=device)
     src = domain.source(height)
     cfg = PropagationConfig(max_interactions, seed=seed)
@@ -329,6 +339,7 @@ def train(scene: GpuScene, desc: SceneDescription, surrogate: Surrogate, domain:
                          time_scale=float(max(tof.std().item(), 1e-12)))
     if cells is not None:
         surrogate.set_region(domain, cells)
+    surrogate.set_survival_scale(survival_scale)
     steps = max(1, n_photons // batch)
     step = 0
     history = []

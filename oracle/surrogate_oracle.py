@@ -110,14 +110,15 @@ def forward_fp8(params: np.ndarray, features: np.ndarray) -> np.ndarray:
     raise AssertionError
 
 
-def mixture_loss(out: np.ndarray, targets: np.ndarray):
-    """Per-photon loss and its gradient wrt the 57 outputs."""
+def mixture_loss(out: np.ndarray, targets: np.ndarray, survival_scale: float = 1.0):
+    """Per-photon loss and its gradient wrt the 57 outputs, the gradient wrt
+    the survival logit times survival_scale."""
     n = len(out)
     s = out[:, 0]
     d = targets[:, 0]
     loss = np.maximum(s, 0) + np.log1p(np.exp(-np.abs(s))) - d * s
     g = np.zeros_like(out)
-    g[:, 0] = 1.0 / (1.0 + np.exp(-s)) - d
+    g[:, 0] = survival_scale * (1.0 / (1.0 + np.exp(-s)) - d)
     w = out[:, 1::7][:, :N_COMP]
     mu = np.stack([out[:, 2 + j::7][:, :N_COMP] for j in range(3)], axis=2)      # n x K x 3
     raw = np.stack([out[:, 5 + j::7][:, :N_COMP] for j in range(3)], axis=2)
@@ -142,11 +143,14 @@ def mixture_loss(out: np.ndarray, targets: np.ndarray):
     return loss, g
 
 
-def loss_and_gradients(params: np.ndarray, features: np.ndarray, targets: np.ndarray, bf16_storage: bool = True):
-    """Mean loss over the batch and its gradient wrt the flat parameters."""
+def loss_and_gradients(params: np.ndarray, features: np.ndarray, targets: np.ndarray, bf16_storage: bool = True,
+                       survival_scale: float = 1.0):
+    """Mean loss over the batch and its gradient wrt the flat parameters
+    (with survival_scale != 1: the backpropagation of the outputs' gradient
+    whose survival component is scaled)."""
     n = len(features)
     out, hs = forward(params, features, bf16_storage)
-    loss, g = mixture_loss(out, targets)
+    loss, g = mixture_loss(out, targets, survival_scale)
     q = bf16 if bf16_storage else (lambda a: np.asarray(a, dtype=np.float64))
     layers = unpack(np.asarray(params, dtype=np.float64))
     grads = [None] * len(layers)

@@ -66,6 +66,44 @@ def test_oracle_gradients_match_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
 
 
+def test_oracle_survival_scale_is_the_gradient_of_a_weighted_loss():
+    """survival_scale k leaves the loss value and every output gradient but
+    the survival logit's alone; the parameter gradient is then that of the
+    loss with its survival cross entropy weighted by k (finite differences)."""
+    p = _params(4)
+    f, t = _batch(6, 5)
+    k = 4.0
+    l1, g1 = SO.loss_and_gradients(p, f, t, bf16_storage=False)
+    lk, gk = SO.loss_and_gradients(p, f, t, bf16_storage=False, survival_scale=k)
+    assert lk == l1
+    out, _ = SO.forward(p, f, False)
+    _, o1 = SO.mixture_loss(out, t)
+    _, ok = SO.mixture_loss(out, t, survival_scale=k)
+    np.testing.assert_array_equal(ok[:, 1:], o1[:, 1:])
+    np.testing.assert_allclose(ok[:, 0], k * o1[:, 0], rtol=1e-15)
+
+    def weighted(pp):
+        o, _ = SO.forward(pp, f, False)
+        loss, _ = SO.mixture_loss(o, t)
+        s, d = o[:, 0], t[:, 0]
+        bce = np.maximum(s, 0) + np.log1p(np.exp(-np.abs(s))) - d * s
+        return (loss + (k - 1.0) * bce).mean()
+
+    rng = np.random.default_rng(8)
+    o = 0
+    idx = []
+    for out_n, inp in SO.SIZES:
+        idx += list(o + rng.integers(0, out_n * inp, 4)) + list(o + out_n * inp + rng.integers(0, out_n, 2))
+        o += out_n * inp + out_n
+    for i in idx:
+        e = 1e-6 * max(1.0, abs(p[i]))
+        pp, pm = p.copy(), p.copy()
+        pp[i] += e
+        pm[i] -= e
+        fd = (weighted(pp) - weighted(pm)) / (2 * e)
+        assert abs(fd - gk[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, gk[i])
+
+
 def test_oracle_bf16_rounding():
     # bf16 keeps 7 mantissa bits: neighbours of 1 are 1 and 1 + 2^-7
     x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -7 + 2 ** -9, -3.14159, 0.0, 1.0 + 2 ** -8 + 2 ** -10,
@@ -226,20 +264,23 @@ def test_gpu_surrogate_fp8_forward_matches_oracle(cuda):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [512, 1000, 4096])
-def test_gpu_surrogate_gradients_match_oracle(cuda, n):
+@pytest.mark.parametrize("n,survival_scale", [(512, 1.0), (1000, 1.0), (4096, 1.0), (1000, 10.0)])
+def test_gpu_surrogate_gradients_match_oracle(cuda, n, survival_scale):
     """bf16 matrix-core backpropagation vs FP64 backpropagation of the same
     bf16-rounded network: every layer's weight and bias gradient agrees in
-    direction (cosine > 0.999) and size (relative L2 error < 3 %)."""
+    direction (cosine > 0.999) and size (relative L2 error < 3 %), also with
+    the survival logit's gradient scaled."""
     from merlict_development_kit_amd.surrogate import Surrogate
     s = Surrogate(seed=6)
     p = _params(9)
     s.set_params(torch.tensor(p, dtype=torch.float32, device=cuda))
+    s.set_survival_scale(survival_scale)
     f, t = _batch(n, 10)
     loss, g = s.gradients(torch.tensor(f, dtype=torch.float32, device=cuda),
                           torch.tensor(t, dtype=torch.float32, device=cuda))
     g = g.cpu().numpy().astype(np.float64)
-    want_loss, want = SO.loss_and_gradients(p.astype(np.float32).astype(np.float64), f.astype(np.float32), t)
+    want_loss, want = SO.loss_and_gradients(p.astype(np.float32).astype(np.float64), f.astype(np.float32), t,
+                                            survival_scale=survival_scale)
     assert abs(loss.item() - want_loss) < 1e-2 * abs(want_loss)
     o = 0
     for li, (out, inp) in enumerate(SO.SIZES):
@@ -441,8 +482,10 @@ def test_gpu_trained_surrogate_gains_over_untrained(cuda):
     cells = SG.HexCells(desc, cuda)
     dom = SG.fact_domain()
     trained = SG.Surrogate(seed=1)
+    # the survival gradient unscaled: a network trained this briefly with it
+    # scaled keeps survival details that the fp8 inference loses
     dom, _ = SG.train(gs, desc, trained, dom, 100_000_000, seed=3, cells=cells,
-                      focal_desc=SceneDescription.fact(focal_plane=True))
+                      focal_desc=SceneDescription.fact(focal_plane=True), survival_scale=1.0)
     untrained = SG.Surrogate(seed=1)
     n, passes = 10_000_000, 8
     ph = dom.source(10.0).generate(n, seed=99, device=cuda)
@@ -473,3 +516,47 @@ def test_gpu_trained_surrogate_gains_over_untrained(cuda):
     assert tt < 0.6 * tu
     # the fp8 inference of the same network keeps what training bought
     assert p8 < 0.02 and p8 < pt + 0.01 and t8 < 0.6 * tu
+
+
+@pytest.mark.gpu
+def test_gpu_survival_scale_lets_survival_follow_the_wavelength(cuda):
+    """The FACT mirror's reflectivity makes the detected fraction vary by a
+    few per cent with the wavelength.  Trained with the survival logit's
+    gradient unscaled, the network's mean survival hardly moves between
+    wavelength bands; scaled by 10 (surrogate.train's default) it follows
+    the plain propagator's detected fraction band by band."""
+    import math
+    from merlict_development_kit_amd import GpuScene, PropagationConfig, SceneDescription
+    from merlict_development_kit_amd import surrogate as SG
+    from merlict_development_kit_amd.sources import FLIP, Source
+    desc = SceneDescription.fact(light_guides=True)
+    gs = GpuScene(desc)
+    cells = SG.HexCells(desc, cuda)
+    poses = torch.tensor(desc.frame_poses().reshape(-1), dtype=torch.float64, device=cuda)
+    bands = ((280e-9, 320e-9), (320e-9, 380e-9), (380e-9, 450e-9), (450e-9, 500e-9), (500e-9, 600e-9),
+             (600e-9, 700e-9))
+    plain = []
+    deviation = {}
+    for scale in (1.0, 10.0):
+        sur = SG.Surrogate(seed=1)
+        dom, _ = SG.train(gs, desc, sur, SG.fact_domain(), 200_000_000, seed=3, cells=cells,
+                          focal_desc=SceneDescription.fact(focal_plane=True), batch=65536, learning_rate=4e-3,
+                          survival_scale=scale)
+        dev = []
+        for k, (lo, hi) in enumerate(bands):
+            src = Source("disc", "cap", 2.25, math.radians(2.0), lo, hi, 0.0, FLIP, (0.0, 0.0, 10.0),
+                         spectrum="uniform")
+            ph = src.generate(1_000_000, seed=5 + k, photon_id_offset=10 ** 11, device=cuda)
+            res = gs.propagate(ph, PropagationConfig(5, seed=11))
+            if len(plain) < len(bands):
+                plain.append((res.sensor_id >= 0).double().mean().item())
+            f, _ = SG.prepare(dom, ph, res, poses)
+            survival = torch.sigmoid(sur.forward(f)[:, 0].double()).mean().item()
+            dev.append(survival - plain[k])
+        deviation[scale] = np.abs(dev)
+        print(f"survival scale {scale}: detected fraction per band {np.round(plain, 4)}, "
+              f"mean survival - detected {np.round(dev, 4)}")
+    spread = max(plain) - min(plain)
+    assert spread > 0.03  # the reflectivity's effect that the survival has to learn
+    assert deviation[10.0].max() < 0.5 * deviation[1.0].max()
+    assert deviation[10.0].max() < 0.02
