@@ -189,9 +189,9 @@ struct mlgpu_event_tape {
     }
 };
 
-extern "C" {
+namespace {
 
-int mlgpu_event_tape_open(const char* path, mlgpu_event_tape** out) {
+int event_tape_open(const char* path, mlgpu_event_tape** out) {
     if (!path || !out) {
         mlgpu::set_last_error("null argument");
         return MLGPU_ERR_ARGUMENT;
@@ -229,20 +229,7 @@ int mlgpu_event_tape_open(const char* path, mlgpu_event_tape** out) {
     return MLGPU_OK;
 }
 
-void mlgpu_event_tape_close(mlgpu_event_tape* t) { delete t; }
-
-int mlgpu_event_tape_run_header(const mlgpu_event_tape* t, float* raw) {
-    if (!t || !raw) {
-        mlgpu::set_last_error("null argument");
-        return MLGPU_ERR_ARGUMENT;
-    }
-    std::memcpy(raw, t->run_header, sizeof(t->run_header));
-    return MLGPU_OK;
-}
-
-int mlgpu_event_tape_has_events(const mlgpu_event_tape* t) { return t && t->has_events ? 1 : 0; }
-
-int mlgpu_event_tape_next_event(mlgpu_event_tape* t, float* header, int64_t* num_bunches) {
+int event_tape_next_event(mlgpu_event_tape* t, float* header, int64_t* num_bunches) {
     if (!t) {
         mlgpu::set_last_error("null tape");
         return MLGPU_ERR_ARGUMENT;
@@ -259,6 +246,43 @@ int mlgpu_event_tape_next_event(mlgpu_event_tape* t, float* header, int64_t* num
     if (header) std::memcpy(header, t->header, sizeof(t->header));
     if (num_bunches) *num_bunches = static_cast<int64_t>(t->bunches.size() / 8);
     return MLGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// A damaged file must not end the caller: allocation failures and other
+// exceptions become format errors at the C boundary.
+int mlgpu_event_tape_open(const char* path, mlgpu_event_tape** out) {
+    try {
+        return event_tape_open(path, out);
+    } catch (const std::exception& e) {
+        mlgpu::set_last_error(std::string("event tape: ") + e.what());
+        return MLGPU_ERR_FORMAT;
+    }
+}
+
+void mlgpu_event_tape_close(mlgpu_event_tape* t) { delete t; }
+
+int mlgpu_event_tape_run_header(const mlgpu_event_tape* t, float* raw) {
+    if (!t || !raw) {
+        mlgpu::set_last_error("null argument");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    std::memcpy(raw, t->run_header, sizeof(t->run_header));
+    return MLGPU_OK;
+}
+
+int mlgpu_event_tape_has_events(const mlgpu_event_tape* t) { return t && t->has_events ? 1 : 0; }
+
+int mlgpu_event_tape_next_event(mlgpu_event_tape* t, float* header, int64_t* num_bunches) {
+    try {
+        return event_tape_next_event(t, header, num_bunches);
+    } catch (const std::exception& e) {
+        mlgpu::set_last_error(std::string("event tape: ") + e.what());
+        return MLGPU_ERR_FORMAT;
+    }
 }
 
 int mlgpu_event_tape_event_bunches(const mlgpu_event_tape* t, float* bunches) {

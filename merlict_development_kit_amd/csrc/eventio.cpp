@@ -171,6 +171,11 @@ bool read_offsets(std::istream& f, const Head& h, std::vector<float>& out, std::
         err = "bad telescope offsets";
         return false;
     }
+    // the object must hold at least the two arrays its count announces
+    if (static_cast<int64_t>(narray) * 8 > h.length - 8) {
+        err = "telescope offsets longer than their object";
+        return false;
+    }
     const size_t na = static_cast<size_t>(narray);
     std::vector<float> x(na), y(na), w(na, 1.0f);
     if (!read_bytes(f, x.data(), static_cast<std::streamsize>(na * 4)) ||
@@ -217,6 +222,11 @@ Rd read_bunches(std::istream& f, std::vector<float>& out, std::string& err) {
         return Rd::bad;
     }
     const bool compact = h.version / 1000 == 1;
+    // nb bunches of 8 values (2 or 4 bytes each) after the 12-byte head
+    if (static_cast<int64_t>(nb) * 8 * (compact ? 2 : 4) > h.length - 12) {
+        err = "photon bunches longer than their object";
+        return Rd::bad;
+    }
     const size_t n = static_cast<size_t>(nb) * 8;
     out.resize(n);
     if (compact) {
@@ -305,9 +315,9 @@ int fail(const std::string& what) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int mlgpu_eventio_open(const char* path, mlgpu_eventio_run** out) {
+int eventio_open(const char* path, mlgpu_eventio_run** out) {
     if (!path || !out) {
         set_last_error("null argument");
         return MLGPU_ERR_ARGUMENT;
@@ -327,6 +337,10 @@ int mlgpu_eventio_open(const char* path, mlgpu_eventio_run** out) {
     if (ok) {
         r = get_head(run->f, 1212, h, err);
         ok = r == Rd::ok;
+        if (ok && (h.length < 0 || h.length > (int64_t{1} << 26))) {
+            err = "input card longer than 64 MiB";
+            ok = false;
+        }
         if (ok) {
             run->input_card.resize(static_cast<size_t>(h.length));
             ok = h.length == 0 || read_bytes(run->f, &run->input_card[0], static_cast<std::streamsize>(h.length));
@@ -345,6 +359,40 @@ int mlgpu_eventio_open(const char* path, mlgpu_eventio_run** out) {
     }
     *out = run;
     return MLGPU_OK;
+}
+
+int eventio_next_event(mlgpu_eventio_run* run, float* header, int32_t* num_offsets, int64_t* num_bunches) {
+    if (!run) {
+        set_last_error("null run");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    if (run->run_end_found) {
+        set_last_error("eventio: no event left in the run");
+        return MLGPU_ERR_ARGUMENT;
+    }
+    std::memcpy(run->header, run->next_header, sizeof(run->header));
+    run->offsets = run->next_offsets;
+    run->bunches.swap(run->next_bunches);
+    std::string err;
+    if (!advance(run, err)) return fail(err);
+    if (header) std::memcpy(header, run->header, sizeof(run->header));
+    if (num_offsets) *num_offsets = static_cast<int32_t>(run->offsets.size() / 4);
+    if (num_bunches) *num_bunches = static_cast<int64_t>(run->bunches.size() / 8);
+    return MLGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// A damaged file must not end the caller: allocation failures and other
+// exceptions become format errors at the C boundary.
+int mlgpu_eventio_open(const char* path, mlgpu_eventio_run** out) {
+    try {
+        return eventio_open(path, out);
+    } catch (const std::exception& e) {
+        return fail(std::string("eventio: ") + e.what());
+    }
 }
 
 void mlgpu_eventio_close(mlgpu_eventio_run* run) { delete run; }
@@ -374,23 +422,11 @@ int mlgpu_eventio_run_info(const mlgpu_eventio_run* run, float* telescope_positi
 int mlgpu_eventio_has_events(const mlgpu_eventio_run* run) { return run && !run->run_end_found ? 1 : 0; }
 
 int mlgpu_eventio_next_event(mlgpu_eventio_run* run, float* header, int32_t* num_offsets, int64_t* num_bunches) {
-    if (!run) {
-        set_last_error("null run");
-        return MLGPU_ERR_ARGUMENT;
+    try {
+        return eventio_next_event(run, header, num_offsets, num_bunches);
+    } catch (const std::exception& e) {
+        return fail(std::string("eventio: ") + e.what());
     }
-    if (run->run_end_found) {
-        set_last_error("eventio: no event left in the run");
-        return MLGPU_ERR_ARGUMENT;
-    }
-    std::memcpy(run->header, run->next_header, sizeof(run->header));
-    run->offsets = run->next_offsets;
-    run->bunches.swap(run->next_bunches);
-    std::string err;
-    if (!advance(run, err)) return fail(err);
-    if (header) std::memcpy(header, run->header, sizeof(run->header));
-    if (num_offsets) *num_offsets = static_cast<int32_t>(run->offsets.size() / 4);
-    if (num_bunches) *num_bunches = static_cast<int64_t>(run->bunches.size() / 8);
-    return MLGPU_OK;
 }
 
 int mlgpu_eventio_event_data(const mlgpu_eventio_run* run, float* offsets, float* bunches) {

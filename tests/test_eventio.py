@@ -103,6 +103,42 @@ def test_eventio_errors(tmp_path):
         _run(cut)
 
 
+def _first_bunch_block(raw: bytes) -> int:
+    """Byte offset of the bunch count of the first photon-bunch object (type
+    1205, a sub-object: type/version word, id, length, then array and
+    telescope int16, photons float32, bunch count int32), found by its
+    self-consistent length."""
+    import struct
+    for o in range(0, len(raw) - 24, 4):
+        word, _, length = struct.unpack_from("<iii", raw, o)
+        if word & 0xFFFF != 1205:
+            continue
+        nb = struct.unpack_from("<i", raw, o + 20)[0]
+        elem = 2 if (word >> 20) // 1000 == 1 else 4
+        if 0 < nb and length == 12 + 8 * elem * nb:
+            return o + 20
+    raise AssertionError("no photon-bunch object in the fixture")
+
+
+def test_eventio_rejects_counts_beyond_their_object(tmp_path):
+    """A bunch or offset count that the object's length cannot hold is a
+    format error before anything is allocated (a damaged count of 2^31 - 1
+    bunches would otherwise ask for 64 GiB)."""
+    import struct
+    from merlict_development_kit_amd._native import MlgpuError
+    raw = bytearray(GOLD.read_bytes())
+    at = _first_bunch_block(bytes(raw))
+    for count in (2 ** 31 - 1, struct.unpack_from("<i", raw, at)[0] + 1):
+        bad = bytearray(raw)
+        struct.pack_into("<i", bad, at, count)
+        path = tmp_path / f"count_{count}.dat"
+        path.write_bytes(bytes(bad))
+        with pytest.raises(MlgpuError, match="longer than their object"):
+            run = _run(path)
+            while run.has_events():
+                run.next_event()
+
+
 def test_eventio_matches_reference_reader():
     """Every bunch of every event, in order, equals what the reference's
     eventio::Run reads (bit for bit: the compact int16 bunches scaled by
