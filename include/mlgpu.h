@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-#define MLGPU_ABI_VERSION 1
+#define MLGPU_ABI_VERSION 2  /* 2: mlgpu_tuning.shade_lanes, the persistent schedule */
 
 /* ---------------------------------------------------------------- status */
 enum mlgpu_status {

@@ -9,6 +9,8 @@ import ctypes as C
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libmlgpu.so"
+# the MLGPU_ABI_VERSION of include/mlgpu.h these bindings follow
+ABI_VERSION = 2
 
 MLGPU_OK = 0
 MLGPU_NUM_PARAMS = 6
@@ -329,6 +331,9 @@ def lib() -> C.CDLL:
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
+        if handle.mlgpu_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {handle.mlgpu_abi_version()}, these bindings "
+                               f"expect {ABI_VERSION}; rebuild it")
         _lib = handle
     return _lib
 

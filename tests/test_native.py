@@ -29,7 +29,7 @@ def test_every_declared_function_is_exported():
 
 
 def test_abi_version():
-    assert N.lib().mlgpu_abi_version() == 1
+    assert N.lib().mlgpu_abi_version() == 2
 
 
 def test_desc_roundtrip(tmp_path):
