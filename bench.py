@@ -786,6 +786,7 @@ def main() -> int:
     ap.add_argument("--shade-lanes", type=int, default=None)
     ap.add_argument("--trace-budget", type=int, default=None)
     ap.add_argument("--blocks-per-cu", type=int, default=None)
+    ap.add_argument("--tail-level", type=int, default=None)
     ap.add_argument("--autotune", action="store_true",
                     help="run the library's autotuner on the bunch before timing (mlgpu_scene_autotune)")
     ap.add_argument("--no-hybrid", action="store_true")
@@ -818,7 +819,7 @@ def main() -> int:
     desc = SceneDescription.fact(light_guides=not args.no_light_guides)
     scene = GpuScene(desc)
     scene.set_tuning(schedule=args.schedule, trace_budget=args.trace_budget, blocks_per_cu=args.blocks_per_cu,
-                     shade_lanes=args.shade_lanes)
+                     tail_level=args.tail_level, shade_lanes=args.shade_lanes)
     n = args.photons
     photons = sources.fact_bunch().generate(n, seed=2024, device=dev)
     out = PropagationResult.empty(n, dev)
@@ -849,6 +850,7 @@ def main() -> int:
     achieved = n * ALG_BYTES_PER_PHOTON / (ms_kernel * 1e-3)
     default_workload = (n == 10_000_000 and not args.no_light_guides and args.trace_budget is None
                         and args.blocks_per_cu is None and args.shade_lanes is None and args.schedule == "auto"
+                        and args.tail_level is None
                         and not args.autotune)
     line = {
         "metric": METRIC,

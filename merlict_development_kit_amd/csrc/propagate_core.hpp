@@ -99,6 +99,61 @@ MLGPU_HD inline bool func_eval(const DeviceScene& sc, int32_t f, double x, doubl
     return true;
 }
 
+// func_eval of N functions at the same x with their table reads
+// interleaved: the heads, then the buckets, then the points of all N are
+// read together, so the lookups wait on three rounds of memory instead of
+// three per function.  ok[i] is false where x lies outside function i's
+// range; y[i] is then unset.  Each y[i] is the value func_eval gives.
+template <int N>
+MLGPU_HD inline void func_eval_n(const DeviceScene& sc, const int32_t (&f)[N], const double x, double (&y)[N],
+                                 bool (&ok)[N]) {
+    FuncHead h[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) h[i] = sc.funcs[f[i]];
+    bool table[N];
+    int32_t k[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        ok[i] = x >= h[i].lower && x < h[i].upper;
+        table[i] = ok[i] && (h[i].flags & (FUNC_DEFAULT_ABSORPTION | FUNC_DEFAULT_REFRACTION |
+                                           FUNC_DEFAULT_REFLECTION)) == 0;
+        k[i] = 0;
+        if (table[i]) {
+            int32_t b = static_cast<int32_t>((x - h[i].lower) * h[i].scale);
+            b = b < 0 ? 0 : (b >= h[i].n_buckets ? h[i].n_buckets - 1 : b);
+            k[i] = sc.buckets[h[i].bucket_start + b];
+        }
+    }
+    FuncPoint a[N];
+    double xn[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        if (!table[i]) continue;
+        const FuncPoint* pt = sc.points + h[i].start;
+        a[i] = pt[k[i]];
+        xn[i] = k[i] + 1 < h[i].n ? pt[k[i] + 1].x : HUGE_VAL;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        if (h[i].flags & FUNC_DEFAULT_ABSORPTION) { y[i] = HUGE_VAL; continue; }
+        if (h[i].flags & FUNC_DEFAULT_REFRACTION) { y[i] = 1.0; continue; }
+        if (h[i].flags & FUNC_DEFAULT_REFLECTION) { y[i] = 0.0; continue; }
+        if (!table[i]) continue;
+        // the bucket's segment, moved down or up until it holds x, as func_eval
+        // does (x is sorted strictly, so at most one of the two moves happens)
+        const FuncPoint* pt = sc.points + h[i].start;
+        int32_t kk = k[i];
+        if (kk > 0 && a[i].x > x) {
+            do { kk--; } while (kk > 0 && pt[kk].x > x);
+            a[i] = pt[kk];
+        } else if (kk + 2 < h[i].n && xn[i] <= x) {
+            do { kk++; } while (kk + 2 < h[i].n && pt[kk + 1].x <= x);
+            a[i] = pt[kk];
+        }
+        y[i] = a[i].slope * x + a[i].y - a[i].slope * a[i].x;
+    }
+}
+
 // ------------------------------------------------------------- transforms
 struct D3 {
     double x, y, z;
@@ -633,29 +688,46 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
     bool stop = true;
     D3 nd = st.d;
     int32_t new_root = st.root;
+    // the functions a bounce can need, looked up in two pairs (absorption and
+    // the index it comes from; reflection and the index it would enter); a
+    // lookup outside its table is an error only where merlict evaluates it
+    double fv[2];
+    bool fok[2];
+    {
+        const int32_t fns[2] = {ph.surf[outside_in ? MLGPU_OUTER_ABSORPTION : MLGPU_INNER_ABSORPTION],
+                                ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION]};
+        func_eval_n(sc, fns, lambda, fv, fok);
+    }
     // absorbed_in_medium_before_reaching_surface (PhotonAndFrame.cpp:32-38)
-    double ell;
-    if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_ABSORPTION : MLGPU_INNER_ABSORPTION], lambda, ell)) {
+    if (!fok[0]) {
         st.err = MLGPU_PHOTON_OUT_OF_TABLE;
         return false;
     }
+    const double ell = fv[0];
     // the refractive index on the side the photon comes from: Fresnel's n1
     // and the index of the leg's time of flight (Photon.cpp:44-56)
-    double n_from;
-    if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFRACTION : MLGPU_INNER_REFRACTION], lambda, n_from)) {
+    if (!fok[1]) {
         st.err = MLGPU_PHOTON_OUT_OF_TABLE;
         return false;
     }
+    const double n_from = fv[1];
     const double survival = exp(-t / ell);
     if (rng.uniform() > survival) {
         interaction = MLGPU_ABSORPTION_IN_VOID;
     } else {
         // interact_with_object (PhotonAndFrame.cpp:40-48)
-        double refl;
-        if (!func_eval(sc, ph.surf[outside_in ? MLGPU_OUTER_REFLECTION : MLGPU_INNER_REFLECTION], lambda, refl)) {
+        double gv[2];
+        bool gok[2];
+        {
+            const int32_t fns[2] = {ph.surf[outside_in ? MLGPU_OUTER_REFLECTION : MLGPU_INNER_REFLECTION],
+                                    ph.surf[outside_in ? MLGPU_INNER_REFRACTION : MLGPU_OUTER_REFRACTION]};
+            func_eval_n(sc, fns, lambda, gv, gok);
+        }
+        if (!gok[0]) {
             st.err = MLGPU_PHOTON_OUT_OF_TABLE;
             return false;
         }
+        const double refl = gv[0];
         bool mirror = false;
         if (refl >= rng.uniform()) {
             interaction = MLGPU_REFLECTION_ON_SURFACE;
@@ -667,12 +739,11 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
                 interaction = MLGPU_ABSORPTION_ON_SURFACE;
             } else {
                 // fresnel_refraction_and_reflection (PhotonAndFrame.cpp:75-87), Fresnel.cpp:7-38
-                double n_to;
-                const int32_t f_to = ph.surf[outside_in ? MLGPU_INNER_REFRACTION : MLGPU_OUTER_REFRACTION];
-                if (!func_eval(sc, f_to, lambda, n_to)) {
+                if (!gok[1]) {
                     st.err = MLGPU_PHOTON_OUT_OF_TABLE;
                     return false;
                 }
+                const double n_to = gv[1];
                 const D3 nrm = outside_in ? n : D3{n.x * -1.0, n.y * -1.0, n.z * -1.0};
                 const double cosI = dot(nrm, ld) * (-1.0);
                 const double eta = n_from / n_to;
