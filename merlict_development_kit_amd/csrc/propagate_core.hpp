@@ -711,7 +711,9 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
         return false;
     }
     const double n_from = fv[1];
-    const double survival = exp(-t / ell);
+    // an infinite absorption length (merlict's default) survives exactly:
+    // exp(-t / inf) = exp(-0) = 1
+    const double survival = ell == HUGE_VAL ? 1.0 : exp(-t / ell);
     if (rng.uniform() > survival) {
         interaction = MLGPU_ABSORPTION_IN_VOID;
     } else {
@@ -792,9 +794,13 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
     st.last_interaction = interaction;
     st.last_frame = ph.frame;
     st.last_sensor = ph.sensor;
-    st.out_x = static_cast<float>(hp.x);
-    st.out_y = static_cast<float>(hp.y);
-    {
+    // Propagator::propagate (PhotonAndFrame.cpp:13-21)
+    const bool go = !stop && st.n_hist <= max_interactions;
+    if (!go) {
+        // the final record, from the photon's last bounce (a bounce it goes
+        // on from is overwritten by the next one)
+        st.out_x = static_cast<float>(hp.x);
+        st.out_y = static_cast<float>(hp.y);
         // final_intersection_incident_direction_wrt_frame (RayForPropagation.cpp:114-136)
         const D3 inc = normalized(D3{hit_world.x - st.s.x, hit_world.y - st.s.y, hit_world.z - st.s.z});
         const D3 loc = orient_inv(g, inc);
@@ -805,8 +811,7 @@ MLGPU_HD inline bool photon_bounce(const DeviceScene& sc, PhotonState& st, const
     st.s = hit_world;
     st.d = nd;
     st.root = new_root;
-    // Propagator::propagate (PhotonAndFrame.cpp:13-21)
-    return st.n_hist <= max_interactions;
+    return go;
 }
 
 MLGPU_HD inline void photon_finish(const PhotonState& st, const mlgpu_results& out, const int64_t i) {
