@@ -416,6 +416,9 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
     s->dev.eps = static_cast<float>(hs.eps);
     s->dev.hex_sin = hs.hex_sin;
     s->dev.hex_cos = hs.hex_cos;
+    s->dev.num_funcs = static_cast<int32_t>(hs.funcs.size());
+    s->dev.num_points = static_cast<int32_t>(hs.points.size());
+    s->dev.num_buckets = static_cast<int32_t>(hs.buckets.size());
     s->scope_of_frame = hs.scope_of_frame;
     s->num_nodes = static_cast<int64_t>(hs.pairs.size());
     s->num_leaves = static_cast<int64_t>(hs.geo.size());

@@ -142,6 +142,7 @@ struct DeviceScene {
     int32_t default_refraction;  // function index of merlict's DEFAULT_REFRACTION
     double hex_sin, hex_cos;     // HexagonalPrismZ unit-vector components
     float eps;                   // box inflation (absolute, metres)
+    int32_t num_funcs, num_points, num_buckets;  // lengths of the function tables
 };
 
 }  // namespace mlgpu

@@ -53,11 +53,45 @@ __device__ __forceinline__ void store_state(const Workspace& ws, const int64_t i
     __shared__ int32_t name##_mem[kMaxTreeDepth * kBlock];              \
     const LaneStack name { name##_mem + threadIdx.x, kBlock }
 
-__global__ __launch_bounds__(kBlock) void propagate_kernel(const DeviceScene sc, const int32_t world_scope,
+// The shading kernels copy the scene's function tables (heads, points,
+// buckets) into LDS when they fit, so the three dependent reads of each
+// Func1 lookup are LDS reads instead of round trips to the L2; the FACT
+// telescope's tables take 9 KB.  Block-uniform: every thread calls it once,
+// before any other barrier.
+constexpr int kLdsFunctionBytes = 12 * 1024;
+
+__device__ __forceinline__ DeviceScene stage_functions(const DeviceScene& sc, unsigned char* lds) {
+    const int hb = sc.num_funcs * static_cast<int>(sizeof(FuncHead));
+    const int pb = sc.num_points * static_cast<int>(sizeof(FuncPoint));
+    const int bb = sc.num_buckets * static_cast<int>(sizeof(int32_t));
+    if (hb + pb + bb > kLdsFunctionBytes) return sc;
+    static_assert(sizeof(FuncHead) % 16 == 0 && sizeof(FuncPoint) % 16 == 0, "16-byte copies");
+    const uint4* fh = reinterpret_cast<const uint4*>(sc.funcs);
+    const uint4* fp = reinterpret_cast<const uint4*>(sc.points);
+    uint4* lh = reinterpret_cast<uint4*>(lds);
+    uint4* lp = reinterpret_cast<uint4*>(lds + hb);
+    int32_t* lb = reinterpret_cast<int32_t*>(lds + hb + pb);
+    for (int k = static_cast<int>(threadIdx.x); k < hb / 16; k += kBlock) lh[k] = fh[k];
+    for (int k = static_cast<int>(threadIdx.x); k < pb / 16; k += kBlock) lp[k] = fp[k];
+    for (int k = static_cast<int>(threadIdx.x); k < sc.num_buckets; k += kBlock) lb[k] = sc.buckets[k];
+    __syncthreads();
+    DeviceScene l = sc;
+    l.funcs = reinterpret_cast<const FuncHead*>(lds);
+    l.points = reinterpret_cast<const FuncPoint*>(lds + hb);
+    l.buckets = lb;
+    return l;
+}
+
+#define MLGPU_STAGE_FUNCTIONS(sc, sc_arg)                                   \
+    __shared__ __align__(16) unsigned char sc##_fn_lds[kLdsFunctionBytes]; \
+    const DeviceScene sc = stage_functions(sc_arg, sc##_fn_lds)
+
+__global__ __launch_bounds__(kBlock) void propagate_kernel(const DeviceScene sc_arg, const int32_t world_scope,
                                                            const mlgpu_photons in, const int64_t n,
                                                            const uint32_t max_interactions, const uint64_t seed,
                                                            const uint64_t id_offset, const mlgpu_results out,
                                                            const int32_t* __restrict__ ids) {
+    MLGPU_STAGE_FUNCTIONS(sc, sc_arg);
     MLGPU_LANE_STACK(stk);
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -66,12 +100,13 @@ __global__ __launch_bounds__(kBlock) void propagate_kernel(const DeviceScene sc,
 
 // The fused schedule with every photon's history recorded (one lane per
 // photon; the entries of photon i from i * capacity on).
-__global__ __launch_bounds__(kBlock) void propagate_history_kernel(const DeviceScene sc, const int32_t world_scope,
+__global__ __launch_bounds__(kBlock) void propagate_history_kernel(const DeviceScene sc_arg, const int32_t world_scope,
                                                                    const mlgpu_photons in, const int64_t n,
                                                                    const uint32_t max_interactions,
                                                                    const uint64_t seed, const uint64_t id_offset,
                                                                    const mlgpu_results out,
                                                                    const mlgpu_history hist) {
+    MLGPU_STAGE_FUNCTIONS(sc, sc_arg);
     MLGPU_LANE_STACK(stk);
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -141,7 +176,7 @@ __global__ void wf_reset_kernel(int32_t* counts, int n) {
 // the workspace and queued for level 1.  Level-0 traces are short (the
 // first surface in front of the source), so they need neither the budget
 // nor the hit records of the later levels.
-__global__ __launch_bounds__(kBlock) void wf_first_kernel(const DeviceScene sc, const int32_t world_scope,
+__global__ __launch_bounds__(kBlock) void wf_first_kernel(const DeviceScene sc_arg, const int32_t world_scope,
                                                           const mlgpu_photons in, const int64_t n_arg,
                                                           const uint32_t max_interactions, const uint64_t seed_arg,
                                                           const uint64_t id_offset_arg, const Workspace ws,
@@ -150,6 +185,7 @@ __global__ __launch_bounds__(kBlock) void wf_first_kernel(const DeviceScene sc, 
     const int64_t n = cp ? cp->n : n_arg;
     const uint64_t seed = cp ? cp->seed : seed_arg;
     const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
+    MLGPU_STAGE_FUNCTIONS(sc, sc_arg);
     MLGPU_LANE_STACK(stk);
     __shared__ TileQueue<kTile> q;
     if (threadIdx.x == 0) q.n = 0;
@@ -261,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void wf_resume_kernel(const DeviceScene sc,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, const mlgpu_photons in,
+__global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc_arg, const mlgpu_photons in,
                                                           const Workspace ws, const int level,
                                                           const uint32_t max_interactions, const uint64_t seed_arg,
                                                           const uint64_t id_offset_arg, const mlgpu_results out,
@@ -269,6 +305,7 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
                                                           const int32_t* __restrict__ ids) {
     const uint64_t seed = cp ? cp->seed : seed_arg;
     const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
+    MLGPU_STAGE_FUNCTIONS(sc, sc_arg);
     __shared__ TileQueue<kShadeTile> q;
     if (threadIdx.x == 0) q.n = 0;
     __syncthreads();
@@ -317,12 +354,13 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc, 
 // short traces inside one light guide's scope, so finishing them in the lane
 // costs little divergence while every level it saves is one gathered read
 // and one write of the photon's state.
-__global__ __launch_bounds__(kBlock) void wf_tail_kernel(const DeviceScene sc, const mlgpu_photons in,
+__global__ __launch_bounds__(kBlock) void wf_tail_kernel(const DeviceScene sc_arg, const mlgpu_photons in,
                                                          const Workspace ws, const int level,
                                                          const uint32_t max_interactions, const uint64_t seed_arg,
                                                          const uint64_t id_offset_arg, const mlgpu_results out,
                                                          const CallParams* __restrict__ cp,
                                                          const int32_t* __restrict__ ids) {
+    MLGPU_STAGE_FUNCTIONS(sc, sc_arg);
     MLGPU_LANE_STACK(stk);
     const uint64_t seed = cp ? cp->seed : seed_arg;
     const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
@@ -369,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void wf_tail_kernel(const DeviceScene sc, c
 
 constexpr int kChunk = 256;
 
-__global__ __launch_bounds__(kBlock) void persistent_kernel(const DeviceScene sc, const int32_t world_scope,
+__global__ __launch_bounds__(kBlock) void persistent_kernel(const DeviceScene sc_arg, const int32_t world_scope,
                                                             const mlgpu_photons in, const int64_t n_arg,
                                                             const uint32_t max_interactions, const uint64_t seed_arg,
                                                             const uint64_t id_offset_arg, const mlgpu_results out,
@@ -380,6 +418,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(const DeviceScene sc
     const int64_t n = cp ? cp->n : n_arg;
     const uint64_t seed = cp ? cp->seed : seed_arg;
     const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
+    MLGPU_STAGE_FUNCTIONS(sc, sc_arg);
     MLGPU_LANE_STACK(stk);
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const uint64_t below = (1ull << lane) - 1ull;
