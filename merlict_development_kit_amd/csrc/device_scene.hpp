@@ -2,7 +2,7 @@
 // HIP kernels).
 //
 // Traversal hierarchy: a binary tree of axis-aligned boxes whose nodes
-// store the boxes of both children, so one 64-byte fetch tests two boxes;
+// store the boxes of both children, so one 32-byte fetch tests two boxes;
 // traversal visits the nearer child first and keeps the farther one on a
 // short per-lane stack.  Every frame that can be a propagation root (the
 // world and each restriction frame, merlict/SurfaceEntity.cpp:59-65) is a
@@ -23,17 +23,21 @@ namespace mlgpu {
 // other negative value the leaf -1 - code.
 constexpr int32_t kNoChild = INT32_MIN;
 
-// 64 B per node: the world-frame boxes of both children (inflated by the
-// float rounding bound) and their codes.
-struct alignas(16) PairNode {
-    float lo0[3];
+// 32 B per node: the world-frame boxes of both children (inflated by the
+// float rounding bound), quantised to one byte per face on a grid of the
+// node's own, and their codes.  Axis a of the grid has origin o_a, a float
+// whose low 8 mantissa bits are zero, kept in the high 24 bits of g[a], and
+// step 2^(e_a - 127), e_a in the low 8 bits; the face of grid index k lies
+// at fmaf(k, step, o) (one rounding).  Byte 6 c + 3 f + a of q is child c's
+// lower (f = 0) or upper (f = 1) face on axis a.  The builder rounds lower
+// faces down and upper faces up past that rounding, so a quantised box
+// holds its child's float box; half the bytes of float faces, which is half
+// of what a node visit costs the vector memory pipeline.
+struct alignas(32) PairNode {
+    uint32_t g[3];
+    uint32_t q[3];
     int32_t c0;
-    float hi0[3];
     int32_t c1;
-    float lo1[3];
-    int32_t pad0;
-    float hi1[3];
-    int32_t pad1;
 };
 
 // A propagation root: the box of all its leaves and the node to start at.

@@ -445,6 +445,10 @@ struct RayF {
     float idx, idy, idz;
     float oix, oiy, oiz;
     double t_adv;
+    // byte selectors (perm_bytes) of a node's first child's faces the ray
+    // enters (near) and leaves (far) on x, y, z: the lower face where the
+    // direction component is positive, the upper one where it is negative
+    uint32_t sel_near, sel_far;
 };
 
 MLGPU_HD inline void ray_setup(const DeviceScene& sc, const int32_t scope, const D3& s, const D3& d, RayF& r) {
@@ -474,6 +478,61 @@ MLGPU_HD inline void ray_setup(const DeviceScene& sc, const int32_t scope, const
     r.oix = r.ox * r.idx;
     r.oiy = r.oy * r.idy;
     r.oiz = r.oz * r.idz;
+    // face byte 3 f + a of a child: lower (f = 0) or upper (f = 1) face on axis a
+    const uint32_t sx = r.idx < 0.0f ? 1u : 0u, sy = r.idy < 0.0f ? 1u : 0u, sz = r.idz < 0.0f ? 1u : 0u;
+    r.sel_near = (3u * sx) | (3u * sy + 1u) << 8 | (3u * sz + 2u) << 16;
+    r.sel_far = (3u * (1u - sx)) | (3u * (1u - sy) + 1u) << 8 | (3u * (1u - sz) + 2u) << 16;
+}
+
+// Bytes of the 8-byte value (hi, lo) picked by sel: byte j of the result is
+// byte (sel >> 8 j) & 7 of it (v_perm_b32 for selectors below 8).
+MLGPU_HD inline uint32_t perm_bytes(const uint32_t hi, const uint32_t lo, const uint32_t sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+    const uint64_t v = static_cast<uint64_t>(hi) << 32 | lo;
+    uint32_t out = 0;
+    for (int j = 0; j < 4; j++) out |= static_cast<uint32_t>((v >> (8 * ((sel >> (8 * j)) & 7u))) & 0xFFu) << (8 * j);
+    return out;
+#endif
+}
+
+// Slab test of a node's child boxes on its grid, in the ray's terms: a face
+// of grid index k on axis a is crossed at t = k A_a + B_a, A_a = step_a / d_a
+// (exact: a power of two times 1/d_a) and B_a = o_a / d_a - o_ray,a / d_a
+// (one rounding more than a float face would take, far inside the boxes'
+// inflation).  The near and far faces of each axis come from the ray's
+// byte selectors, so no per-axis min / max is needed.
+struct NodeRay {
+    float ax, ay, az, bx, by, bz;
+};
+
+MLGPU_HD inline NodeRay node_ray(const PairNode& nd, const RayF& r) {
+    NodeRay nr;
+    const float ox = __builtin_bit_cast(float, nd.g[0] & 0xFFFFFF00u);
+    const float oy = __builtin_bit_cast(float, nd.g[1] & 0xFFFFFF00u);
+    const float oz = __builtin_bit_cast(float, nd.g[2] & 0xFFFFFF00u);
+    nr.ax = __builtin_bit_cast(float, (nd.g[0] & 0xFFu) << 23) * r.idx;
+    nr.ay = __builtin_bit_cast(float, (nd.g[1] & 0xFFu) << 23) * r.idy;
+    nr.az = __builtin_bit_cast(float, (nd.g[2] & 0xFFu) << 23) * r.idz;
+    nr.bx = fmaf(ox, r.idx, -r.oix);
+    nr.by = fmaf(oy, r.idy, -r.oiy);
+    nr.bz = fmaf(oz, r.idz, -r.oiz);
+    return nr;
+}
+
+// near / far: the child's face bytes x, y, z in bytes 0-2 (perm_bytes)
+MLGPU_HD inline bool node_child_hit(const NodeRay& nr, const uint32_t near, const uint32_t far, const float limit,
+                                    float& tmin) {
+    const float nx = fmaf(static_cast<float>(near & 0xFFu), nr.ax, nr.bx);
+    const float ny = fmaf(static_cast<float>((near >> 8) & 0xFFu), nr.ay, nr.by);
+    const float nz = fmaf(static_cast<float>((near >> 16) & 0xFFu), nr.az, nr.bz);
+    const float fx = fmaf(static_cast<float>(far & 0xFFu), nr.ax, nr.bx);
+    const float fy = fmaf(static_cast<float>((far >> 8) & 0xFFu), nr.ay, nr.by);
+    const float fz = fmaf(static_cast<float>((far >> 16) & 0xFFu), nr.az, nr.bz);
+    tmin = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
+    const float tmax = fminf(fminf(fx, fy), fminf(fz, limit));
+    return tmin <= tmax;
 }
 
 // Where a traversal stands: the next node or leaf code to visit (kNoChild:
@@ -533,9 +592,14 @@ MLGPU_HD inline bool trace_run(const DeviceScene& sc, const RayF& r, const D3& s
         }
         if (cur >= 0) {
             const PairNode nd = sc.pairs[cur];
+            const NodeRay nr = node_ray(nd, r);
             float n0, n1;
-            const bool h0 = slab(nd.lo0, nd.hi0, r.idx, r.idy, r.idz, r.oix, r.oiy, r.oiz, best_f, n0) && nd.c0 != kNoChild;
-            const bool h1 = slab(nd.lo1, nd.hi1, r.idx, r.idy, r.idz, r.oix, r.oiy, r.oiz, best_f, n1) && nd.c1 != kNoChild;
+            const bool h0 = node_child_hit(nr, perm_bytes(nd.q[1], nd.q[0], r.sel_near),
+                                           perm_bytes(nd.q[1], nd.q[0], r.sel_far), best_f, n0) &&
+                            nd.c0 != kNoChild;
+            const bool h1 = node_child_hit(nr, perm_bytes(nd.q[2], nd.q[1], r.sel_near + 0x020202u),
+                                           perm_bytes(nd.q[2], nd.q[1], r.sel_far + 0x020202u), best_f, n1) &&
+                            nd.c1 != kNoChild;
             if (h0 && h1) {
                 const bool near0 = n0 <= n1;
                 const int32_t far = near0 ? nd.c1 : nd.c0;
@@ -562,11 +626,14 @@ MLGPU_HD inline bool trace_run(const DeviceScene& sc, const RayF& r, const D3& s
                 break;
             }
             const int32_t p = (-1 - e) & (kDeferredLeaf - 1);
-            const PairNode& nd = sc.pairs[p >> 1];
+            const PairNode nd = sc.pairs[p >> 1];
             const bool second = (p & 1) != 0;
+            const NodeRay nr = node_ray(nd, r);
+            const uint32_t w_hi = second ? nd.q[2] : nd.q[1], w_lo = second ? nd.q[1] : nd.q[0];
+            const uint32_t shift = second ? 0x020202u : 0u;
             float tn;
-            if (slab(second ? nd.lo1 : nd.lo0, second ? nd.hi1 : nd.hi0, r.idx, r.idy, r.idz, r.oix, r.oiy, r.oiz,
-                     best_f, tn)) {
+            if (node_child_hit(nr, perm_bytes(w_hi, w_lo, r.sel_near + shift), perm_bytes(w_hi, w_lo, r.sel_far + shift),
+                               best_f, tn)) {
                 cur = second ? nd.c1 : nd.c0;
                 break;
             }

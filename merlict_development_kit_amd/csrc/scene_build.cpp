@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <string>
 #include <stdexcept>
@@ -847,6 +848,63 @@ class Builder {
         hs_->hex_cos = std::cos(2. / 3. * M_PI);
     }
 
+    // The node's grid and its children's faces on it (PairNode): per axis
+    // the origin is the children's lowest face rounded down to a float with
+    // 15 mantissa bits, the step the least power of two that spans their
+    // highest face in 254 steps; faces are rounded outwards on the grid and
+    // moved one more step where the kernel's fmaf would round them back
+    // inside the float box.
+    static void quantize_pair(const float (&lo)[2][3], const float (&hi)[2][3], const std::array<bool, 2> live,
+                              PairNode& p) {
+        auto bits = [](float x) { uint32_t u; std::memcpy(&u, &x, 4); return u; };
+        auto from = [](uint32_t u) { float x; std::memcpy(&x, &u, 4); return x; };
+        uint8_t q[12] = {};
+        for (int a = 0; a < 3; a++) {
+            float ulo = HUGE_VALF, uhi = -HUGE_VALF;
+            for (int c = 0; c < 2; c++)
+                if (live[c]) {
+                    ulo = std::min(ulo, lo[c][a]);
+                    uhi = std::max(uhi, hi[c][a]);
+                }
+            if (!(ulo <= uhi)) ulo = uhi = 0.0f;  // no child: any grid
+            // origin: ulo with its low 8 mantissa bits cleared, rounded down
+            uint32_t ob = bits(ulo);
+            if ((ob & 0xFFu) != 0u) ob = (ob & 0x80000000u) ? (ob | 0xFFu) + 1u : ob;
+            ob &= 0xFFFFFF00u;
+            const float o = from(ob);
+            if (!(static_cast<double>(o) <= ulo)) throw std::logic_error("pair node grid origin above its boxes");
+            // step 2^k: o + 254 * 2^k >= uhi
+            int k = -126;
+            const double span = static_cast<double>(uhi) - static_cast<double>(o);
+            if (span > 0.0) {
+                int ex;
+                std::frexp(span / 254.0, &ex);  // span / 254 < 2^ex
+                k = std::max(-126, ex - 1);
+                while (static_cast<double>(o) + 254.0 * std::ldexp(1.0, k) < uhi) k++;
+            }
+            if (k > 127) throw std::invalid_argument("scene box too large for the node grid");
+            const float step = from(static_cast<uint32_t>(k + 127) << 23);
+            p.g[a] = ob | static_cast<uint32_t>(k + 127);
+            for (int c = 0; c < 2; c++) {
+                if (!live[c]) continue;
+                double fl = std::floor((static_cast<double>(lo[c][a]) - o) / step);
+                double fh = std::ceil((static_cast<double>(hi[c][a]) - o) / step);
+                int kl = static_cast<int>(std::min(255.0, std::max(0.0, fl)));
+                int kh = static_cast<int>(std::min(255.0, std::max(0.0, fh)));
+                while (kl > 0 && std::fmaf(static_cast<float>(kl), step, o) > lo[c][a]) kl--;
+                while (kh < 255 && std::fmaf(static_cast<float>(kh), step, o) < hi[c][a]) kh++;
+                if (std::fmaf(static_cast<float>(kl), step, o) > lo[c][a] ||
+                    std::fmaf(static_cast<float>(kh), step, o) < hi[c][a])
+                    throw std::logic_error("pair node face off its grid");
+                q[6 * c + a] = static_cast<uint8_t>(kl);
+                q[6 * c + 3 + a] = static_cast<uint8_t>(kh);
+            }
+        }
+        for (int w = 0; w < 3; w++)
+            p.q[w] = static_cast<uint32_t>(q[4 * w]) | static_cast<uint32_t>(q[4 * w + 1]) << 8 |
+                     static_cast<uint32_t>(q[4 * w + 2]) << 16 | static_cast<uint32_t>(q[4 * w + 3]) << 24;
+    }
+
     // Round boxes outwards to float and inflate them so the FP32 slab test
     // stays conservative: the ray origin is rounded to float and the slab
     // arithmetic rounds, both bounded by a few ulp of the scene's extent.
@@ -873,8 +931,10 @@ class Builder {
         };
         for (size_t i = 0; i < hs_->pairs.size(); i++) {
             PairNode& p = hs_->pairs[i];
-            put(pair_box_[i][0], p.lo0, p.hi0);
-            put(pair_box_[i][1], p.lo1, p.hi1);
+            float lo[2][3], hi[2][3];
+            put(pair_box_[i][0], lo[0], hi[0]);
+            put(pair_box_[i][1], lo[1], hi[1]);
+            quantize_pair(lo, hi, {!pair_box_[i][0].empty(), !pair_box_[i][1].empty()}, p);
         }
         for (size_t i = 0; i < hs_->scopes.size(); i++) put(scope_box_[i], hs_->scopes[i].lo, hs_->scopes[i].hi);
         // oriented leaf boxes: half extents grow by the same bound, rounded up
