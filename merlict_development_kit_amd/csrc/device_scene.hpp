@@ -81,15 +81,18 @@ struct alignas(16) LeafGeo {
     int32_t rank;
 };
 
-// Float oriented box of a leaf (64 B): a point x of the world lies in it when
+// Float oriented box of a leaf (48 B): a point x of the world lies in it when
 // |m (x - c)|_k <= h_k for k = x, y, z.  It holds every point where the leaf
-// can report a hit, inflated like the tree's boxes, so a ray that misses it
-// (or meets it only behind the best hit so far) needs no FP64 solve.
+// can report a hit, inflated like the tree's boxes and, because the rotation
+// m is kept in half precision, by the most that rounding can move a point of
+// the box (the row's largest rounding error times the box's diagonal), so a
+// ray that misses it (or meets it only behind the best hit so far) needs no
+// FP64 solve.
 struct alignas(16) LeafBox {
-    float m[9];  // world -> object rotation, row major
-    float c[3];  // world centre
-    float h[3];  // half extents
-    float pad;
+    float c[3];     // world centre
+    float h[3];     // half extents
+    uint32_t m[5];  // world -> object rotation, row major: 9 IEEE halves, two per word, low half first
+    uint32_t pad;
 };
 
 // Physics of a leaf, read once per bounce (48 B).

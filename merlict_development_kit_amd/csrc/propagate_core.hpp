@@ -420,12 +420,26 @@ MLGPU_HD inline float rcp_approx(const float x) {
 
 // FP32 test of the oriented box of a leaf for the ray o + t d: does the ray
 // meet it within [0, limit]?
+// Entry j of a leaf box's rotation (LeafBox::m).
+MLGPU_HD inline float box_rotation(const LeafBox& b, const int j) {
+    const uint32_t bits = (b.m[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(bits)));
+#else
+    const int e = static_cast<int>((bits >> 10) & 31u);
+    const float mant = static_cast<float>(bits & 1023u);
+    const float mag = e == 0 ? std::ldexp(mant, -24) : std::ldexp(1024.0f + mant, e - 25);
+    return (bits & 0x8000u) ? -mag : mag;
+#endif
+}
+
 MLGPU_HD inline bool leaf_box_hit(const LeafBox& b, const float ox, const float oy, const float oz, const float dx,
                                   const float dy, const float dz, const float limit) {
     const float px = ox - b.c[0], py = oy - b.c[1], pz = oz - b.c[2];
     float tmin = 0.0f, tmax = limit;
+#pragma unroll
     for (int k = 0; k < 3; k++) {
-        const float* r = b.m + 3 * k;
+        const float r[3] = {box_rotation(b, 3 * k), box_rotation(b, 3 * k + 1), box_rotation(b, 3 * k + 2)};
         const float lo = fmaf(r[0], px, fmaf(r[1], py, r[2] * pz));
         const float ld = fmaf(r[0], dx, fmaf(r[1], dy, r[2] * dz));
         const float inv = rcp_approx(fabsf(ld) > 1e-30f ? ld : copysignf(1e-30f, ld));

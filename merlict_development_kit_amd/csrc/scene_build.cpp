@@ -848,6 +848,30 @@ class Builder {
         hs_->hex_cos = std::cos(2. / 3. * M_PI);
     }
 
+    // IEEE half nearest to v (|v| <= 1, ties to even) and its value.
+    static uint16_t to_half(const double v) {
+        const uint16_t sign = std::signbit(v) ? 0x8000u : 0u;
+        const double a = std::fabs(v);
+        if (a == 0.0) return sign;
+        int e;
+        std::frexp(a, &e);  // a in [2^(e-1), 2^e)
+        int E = std::max(e - 1, -14);
+        double q = std::nearbyint(a / std::ldexp(1.0, E - 10));
+        if (q >= 2048.0) {
+            q = 1024.0;
+            E++;
+        }
+        if (q < 1024.0) return static_cast<uint16_t>(sign | static_cast<uint16_t>(q));  // subnormal
+        if (E > 15) throw std::logic_error("leaf box rotation entry out of half range");
+        return static_cast<uint16_t>(sign | static_cast<uint16_t>((E + 15) << 10) | static_cast<uint16_t>(q - 1024.0));
+    }
+    static double from_half(const uint16_t h) {
+        const int e = (h >> 10) & 31;
+        const double mant = h & 1023;
+        const double mag = e == 0 ? std::ldexp(mant, -24) : std::ldexp(1024.0 + mant, e - 25);
+        return (h & 0x8000u) ? -mag : mag;
+    }
+
     // The node's grid and its children's faces on it (PairNode): per axis
     // the origin is the children's lowest face rounded down to a float with
     // 15 mantissa bits, the step the least power of two that spans their
@@ -942,19 +966,30 @@ class Builder {
         for (size_t i = 0; i < leaf_obb_.size(); i++) {
             const auto& ob = leaf_obb_[i];
             LeafBox& lb = hs_->boxes[i];
-            for (int k = 0; k < 9; k++) lb.m[k] = static_cast<float>(ob[k]);
+            uint16_t mh[10] = {};
+            double row_err[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k < 9; k++) {
+                mh[k] = to_half(ob[k]);
+                row_err[k / 3] = std::max(row_err[k / 3], std::fabs(from_half(mh[k]) - ob[k]));
+            }
+            for (int w = 0; w < 5; w++) lb.m[w] = static_cast<uint32_t>(mh[2 * w]) | static_cast<uint32_t>(mh[2 * w + 1]) << 16;
+            // |(m~ - m)(x - c)|_k <= row_err_k |x - c|_1 <= row_err_k sqrt(3) |h|_2 for x in the box
+            double diag = 0.0;
+            for (int k = 0; k < 3; k++)
+                if (ob[12 + k] >= 0.0) diag += (ob[12 + k] + eps) * (ob[12 + k] + eps);
+            diag = std::sqrt(3.0 * diag) * (1.0 + 1e-6);
             for (int k = 0; k < 3; k++) {
                 lb.c[k] = static_cast<float>(ob[9 + k]);
                 if (ob[12 + k] < 0.0) {
                     lb.h[k] = -1.0f;
                     continue;
                 }
-                const double h = ob[12 + k] + eps;
+                const double h = ob[12 + k] + eps + row_err[k] * diag;
                 float hf = static_cast<float>(h);
                 if (static_cast<double>(hf) < h) hf = std::nextafter(hf, HUGE_VALF);
                 lb.h[k] = hf;
             }
-            lb.pad = 0.0f;
+            lb.pad = 0u;
         }
     }
 };
