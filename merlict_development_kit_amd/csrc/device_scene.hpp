@@ -66,7 +66,7 @@ enum LeafKind : int32_t {
     LK_SPHERE = 11,
 };
 
-// Geometry of a leaf (208 B), read by every leaf test.
+// Geometry of a leaf (192 B), read by every leaf test.
 //   rot    frame2world rotation, row major (HomTra3 T[i][j], j < 3)
 //   t      frame2world translation (T[i][3])
 //   ti     R^T t, the constant of HomTra3::position_inverse
