@@ -16,10 +16,10 @@ namespace mlgpu {
 struct alignas(64) PhotonRay {
     double sx, sy, sz;
     double dx, dy, dz;
-    int32_t root;     // scope the next trace runs in
-    uint32_t n_hist;  // history length so far
-    uint32_t draws;   // Philox draws consumed so far
-    int32_t pad;
+    double wavelength;    // the photon's, so the shading reads no input array
+    int32_t root;         // scope the next trace runs in
+    uint32_t hist_draws;  // history length so far (bits 0-7; the wavefront
+                          // runs at most 255 levels) | Philox draws consumed << 8
 };
 
 struct alignas(16) PhotonSums {
@@ -55,6 +55,9 @@ struct CallParams {
 
 constexpr int kDeferredCounts = 256;
 constexpr int kNumCounts = 2 * kDeferredCounts;
+// a wavefront pass runs at most kDeferredCounts - 1 levels, so the history
+// length of a live photon fits PhotonRay::hist_draws' 8 bits
+static_assert(kDeferredCounts <= 256, "PhotonRay keeps the history length in 8 bits");
 
 // bytes of workspace for cap photons (each array 256-byte aligned)
 constexpr size_t workspace_bytes(int64_t cap) {

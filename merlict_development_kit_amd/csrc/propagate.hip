@@ -36,16 +36,28 @@ __device__ __forceinline__ uint64_t photon_id(const uint64_t id_offset, const in
 
 // The state of a photon that propagates on, for the next level.
 __device__ __forceinline__ void store_state(const Workspace& ws, const int64_t i, const PhotonState& st,
-                                            const uint32_t draws) {
+                                            const uint32_t draws, const double wavelength) {
     PhotonRay pr;
     pr.sx = st.s.x; pr.sy = st.s.y; pr.sz = st.s.z;
     pr.dx = st.d.x; pr.dy = st.d.y; pr.dz = st.d.z;
+    pr.wavelength = wavelength;
     pr.root = st.root;
-    pr.n_hist = st.n_hist;
-    pr.draws = draws;
-    pr.pad = 0;
+    pr.hist_draws = (st.n_hist & 0xFFu) | draws << 8;
     ws.ray[i] = pr;
     ws.sums[i] = PhotonSums{st.tof, st.dist};
+}
+
+// The state of a photon from its records, with its draw count.
+__device__ __forceinline__ void load_state(const PhotonRay& pr, const PhotonSums& ps, PhotonState& st,
+                                           uint32_t& draws) {
+    st.s = D3{pr.sx, pr.sy, pr.sz};
+    st.d = D3{pr.dx, pr.dy, pr.dz};
+    st.tof = ps.tof;
+    st.dist = ps.dist;
+    st.root = pr.root;
+    st.n_hist = pr.hist_draws & 0xFFu;
+    st.err = MLGPU_PHOTON_OK;
+    draws = pr.hist_draws >> 8;
 }
 
 // The per-lane traversal stacks of a block: column threadIdx.x.
@@ -207,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void wf_first_kernel(const DeviceScene sc_a
                     go = photon_bounce(sc, st, c.leaf, c.t, in.wavelength[i], rng, max_interactions);
                 }
                 if (go) {
-                    store_state(ws, i, st, rng.next);
+                    store_state(ws, i, st, rng.next, in.wavelength[i]);
                 } else {
                     photon_finish(st, out, i);
                 }
@@ -322,21 +334,14 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(const DeviceScene sc_a
             if (slot < count) {
                 i = queue[slot];
                 const PhotonRay pr = ws.ray[i];
-                const PhotonSums ps = ws.sums[i];
                 PhotonState st;
-                st.s = D3{pr.sx, pr.sy, pr.sz};
-                st.d = D3{pr.dx, pr.dy, pr.dz};
-                st.tof = ps.tof;
-                st.dist = ps.dist;
-                st.root = pr.root;
-                st.n_hist = pr.n_hist;
-                st.err = MLGPU_PHOTON_OK;
+                uint32_t draws;
+                load_state(pr, ws.sums[i], st, draws);
                 Rng rng;
-                rng.init(seed, photon_id(id_offset, ids, i), pr.draws);
-                go = photon_bounce(sc, st, ws.hit_leaf[slot], ws.hit_t[slot], in.wavelength[i], rng,
-                                   max_interactions);
+                rng.init(seed, photon_id(id_offset, ids, i), draws);
+                go = photon_bounce(sc, st, ws.hit_leaf[slot], ws.hit_t[slot], pr.wavelength, rng, max_interactions);
                 if (go) {
-                    store_state(ws, i, st, rng.next);
+                    store_state(ws, i, st, rng.next, pr.wavelength);
                 } else {
                     photon_finish(st, out, i);
                 }
@@ -370,18 +375,12 @@ __global__ __launch_bounds__(kBlock) void wf_tail_kernel(const DeviceScene sc_ar
          slot += static_cast<int32_t>(gridDim.x * blockDim.x)) {
         const int32_t i = queue[slot];
         const PhotonRay pr = ws.ray[i];
-        const PhotonSums ps = ws.sums[i];
         PhotonState st;
-        st.s = D3{pr.sx, pr.sy, pr.sz};
-        st.d = D3{pr.dx, pr.dy, pr.dz};
-        st.tof = ps.tof;
-        st.dist = ps.dist;
-        st.root = pr.root;
-        st.n_hist = pr.n_hist;
-        st.err = MLGPU_PHOTON_OK;
-        const double lambda = in.wavelength[i];
+        uint32_t draws;
+        load_state(pr, ws.sums[i], st, draws);
+        const double lambda = pr.wavelength;
         Rng rng;
-        rng.init(seed, photon_id(id_offset, ids, i), pr.draws);
+        rng.init(seed, photon_id(id_offset, ids, i), draws);
         bool go = photon_bounce(sc, st, ws.hit_leaf[slot], ws.hit_t[slot], lambda, rng, max_interactions);
         while (go) {
             Closest c;
