@@ -95,15 +95,16 @@ struct alignas(16) LeafBox {
     uint32_t pad;
 };
 
-// Physics of a leaf, read once per bounce (48 B).
+// Physics of a leaf, read once per bounce (32 B).
 struct alignas(16) LeafPhys {
-    int32_t surf[6];       // function index per surface slot (defaults resolved)
+    int16_t surf[6];       // function index per surface slot (defaults resolved; a scene has at most 32,767)
     int32_t allowed_scope; // scope of the restriction frame, -1 none
     int32_t root_scope;    // scope of the tree root
     int32_t frame;         // description frame index
     int32_t sensor;        // sensor id or -1
-    int32_t pad0, pad1;
+    int32_t pad;
 };
+constexpr int32_t kMaxFunctions = 32767;
 
 // Func1 table: points sorted by x with the slope merlict computes before
 // sorting (merlict/function/Func1.cpp:25-36), and a bucket index over

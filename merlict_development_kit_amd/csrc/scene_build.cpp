@@ -746,13 +746,14 @@ class Builder {
                 else if (s <= MLGPU_INNER_REFRACTION) fn = def_refr_;
                 else fn = def_abs_;
             }
-            ph.surf[s] = fn;
+            if (fn > kMaxFunctions) throw std::invalid_argument("scene has more functions than a leaf can index");
+            ph.surf[s] = static_cast<int16_t>(fn);
         }
         ph.allowed_scope = -1;
         ph.root_scope = -1;
         ph.frame = f;
         ph.sensor = d_.sensor_id ? d_.sensor_id[f] : -1;
-        ph.pad0 = ph.pad1 = 0;
+        ph.pad = 0;
         hs_->geo.push_back(g);
         hs_->phys.push_back(ph);
         {
