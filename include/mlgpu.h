@@ -230,7 +230,9 @@ typedef struct mlgpu_scene mlgpu_scene;
 
 /* Build the traversal hierarchy on the host and upload it to the current
  * device.  Replaces init_tree_based_on_mother_child_relations()
- * (merlict/Frame.cpp:125-130) for the GPU. */
+ * (merlict/Frame.cpp:125-130) for the GPU.  MLGPU_ERR_SCENE when the scene
+ * exceeds the device layout: a tree deeper than 28 levels or more than
+ * 32,767 functions. */
 int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out);
 void mlgpu_scene_destroy(mlgpu_scene* scene);
 /* hierarchy statistics: nodes, leaves, device bytes */
