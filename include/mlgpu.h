@@ -257,7 +257,8 @@ enum mlgpu_schedule {
     MLGPU_SCHEDULE_PERSISTENT = 3
 };
 
-/* Launch parameters of mlgpu_propagate; they never change results.
+/* Launch parameters of mlgpu_propagate; they never change results.  A new
+ * scene starts with {AUTO, 32, 48, 0, 32}.
  *   schedule       an mlgpu_schedule
  *   trace_budget   WAVEFRONT: traversal steps a ray gets in the first trace
  *                  pass of a level; the unfinished rays resume, packed, in a

@@ -23,7 +23,7 @@ struct mlgpu_scene {
     int64_t device_bytes = 0;
     int device = 0;
     int num_cus = 0;
-    mlgpu_tuning tuning{MLGPU_SCHEDULE_AUTO, 32, 32, 0, 32};
+    mlgpu_tuning tuning{MLGPU_SCHEDULE_AUTO, 32, 48, 0, 32};
     // wavefront workspace, grown on demand.  Calls that use it are ordered
     // on the device: a call on another stream than the previous user first
     // waits for the event recorded after that user's kernels.

@@ -167,7 +167,7 @@ def test_gpu_autotune_picks_a_candidate_and_keeps_results(cuda):
 
 def test_gpu_tuning_is_validated(cuda):
     gs = GpuScene(SCENES["lens"]())
-    assert gs.tuning() == {"schedule": "auto", "trace_budget": 32, "blocks_per_cu": 32, "tail_level": 0,
+    assert gs.tuning() == {"schedule": "auto", "trace_budget": 32, "blocks_per_cu": 48, "tail_level": 0,
                            "shade_lanes": 32}
     with pytest.raises(MlgpuError):
         gs.set_tuning(blocks_per_cu=0)
