@@ -15,7 +15,9 @@ from merlict_development_kit_amd import SceneDescription
 FRAME, CAP_HEX, CAP_CYL, CAP_RECT, PLANE, DISC, HEX, ANNULUS, DUAL, CYL, TRI, SPHERE, ELLIP_CAP = range(13)
 
 
-def zoo() -> SceneDescription:
+def zoo(functions=None) -> SceneDescription:
+    """The primitive zoo; `functions` replaces its two tables (function 0:
+    the mirrors' reflectivity, 1: the glass' refractive index)."""
     rows = []  # mother, pos, rot_mode, rot, kind, params, surface
 
     def add(mother, pos, kind, params=(), rot=(0, 0, 0, 0), mode=0, surface=None):
@@ -59,7 +61,7 @@ def zoo() -> SceneDescription:
         mother=[r[0] for r in rows], pos=[r[1] for r in rows], rot_mode=[r[2] for r in rows],
         rot=[list(r[3]) + [0.0] * (4 - len(r[3])) for r in rows], kind=[r[4] for r in rows],
         params=[r[5] for r in rows], surface=[r[6] for r in rows], sensor_id=list(range(n)),
-        functions=[((200e-9, 1200e-9), (0.9, 0.9)), ((200e-9, 1200e-9), (1.5, 1.5))])
+        functions=functions or [((200e-9, 1200e-9), (0.9, 0.9)), ((200e-9, 1200e-9), (1.5, 1.5))])
 
 
 def probe_rays(n: int, seed: int) -> dict:

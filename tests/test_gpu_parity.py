@@ -91,6 +91,26 @@ def test_gpu_matches_oracle_many_photons(cuda, scene, schedule, oracle_threads):
     compare(res, ref, n, f"{scene} ({schedule}) vs oracle")
 
 
+def test_gpu_large_function_tables_match_oracle(cuda, oracle_threads):
+    """Function tables too large for the shading kernels' LDS copy (two of
+    1,500 points: 96 KB against 12 KB) are read from device memory instead;
+    the results are the oracle's under every schedule, wavelengths spread
+    over the tables."""
+    n = 100_000
+    x = np.linspace(200e-9, 1200e-9, 1500)
+    desc = scene_zoo.zoo(functions=[(x, 0.5 + 0.4 * np.sin(x * 3e7)), (x, 1.45 + 0.05 * np.cos(x * 2e7))])
+    rays = scene_zoo.probe_rays(n, 17)
+    rays["wavelength"] = np.random.default_rng(3).uniform(250e-9, 1150e-9, n)
+    ph = PhotonBatch.from_numpy(rays)
+    ref = O.OracleScene(desc).propagate(ph.numpy(), seed=5, threads=oracle_threads)
+    print(f"large tables: {int((ref['final_frame'] >= 0).sum())} of {n} photons end on a surface", flush=True)
+    for schedule in ("fused", "wavefront", "persistent"):
+        gs = GpuScene(desc)
+        gs.set_tuning(schedule=schedule)
+        res = gs.propagate(ph.to(cuda), PropagationConfig(5, seed=5)).numpy()
+        compare(res, ref, n, f"zoo with 1,500-point tables ({schedule}) vs oracle")
+
+
 @pytest.mark.parametrize("schedule", ["fused", "wavefront", "persistent"])
 def test_gpu_plenoscope_matches_oracle(cuda, schedule, oracle_threads):
     """Plenoscope (635k leaves, 515,023 lixels): calibration photons from
