@@ -419,6 +419,8 @@ int mlgpu_scene_create(const mlgpu_scene_desc* desc, mlgpu_scene** out) {
     s->dev.num_funcs = static_cast<int32_t>(hs.funcs.size());
     s->dev.num_points = static_cast<int32_t>(hs.points.size());
     s->dev.num_buckets = static_cast<int32_t>(hs.buckets.size());
+    s->dev.hot_count = hs.hot_count;
+    s->dev.num_hot = 0;
     s->scope_of_frame = hs.scope_of_frame;
     s->num_nodes = static_cast<int64_t>(hs.pairs.size());
     s->num_leaves = static_cast<int64_t>(hs.geo.size());

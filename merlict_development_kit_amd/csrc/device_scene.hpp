@@ -151,6 +151,15 @@ struct DeviceScene {
     double hex_sin, hex_cos;     // HexagonalPrismZ unit-vector components
     float eps;                   // box inflation (absolute, metres)
     int32_t num_funcs, num_points, num_buckets;  // lengths of the function tables
+    // The first hot_count nodes are the top of the world scope's tree,
+    // breadth first.  A kernel that copies them into LDS sets num_hot to
+    // hot_count; as uploaded, num_hot is 0.
+    int32_t hot_count;
+    int32_t num_hot;
 };
+
+// Nodes at the top of the world scope's tree that the traversal kernels keep
+// in LDS (4 KB).
+constexpr int kHotNodes = 128;
 
 }  // namespace mlgpu

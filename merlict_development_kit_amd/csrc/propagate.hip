@@ -94,6 +94,25 @@ __device__ __forceinline__ DeviceScene stage_functions(const DeviceScene& sc, un
     return l;
 }
 
+// The top of the world scope's tree (the scene's first hot_count nodes)
+// copied into LDS for the level-0 kernel, whose traces (source to the first
+// surface) stay in it: reading it from LDS spares the vector memory
+// pipeline, which the per-lane node gathers keep busy.  Block-uniform, like
+// stage_functions.
+__device__ __forceinline__ DeviceScene stage_hot(const DeviceScene& sc) {
+    const int m = sc.hot_count < kHotNodes ? sc.hot_count : kHotNodes;
+    static_assert(sizeof(PairNode) == 32, "two 16-byte copies per node");
+    const uint4* src = reinterpret_cast<const uint4*>(sc.pairs);
+    uint4* dst = reinterpret_cast<uint4*>(hot_nodes);
+    for (int k = static_cast<int>(threadIdx.x); k < 2 * m; k += kBlock) dst[k] = src[k];
+    __syncthreads();
+    DeviceScene l = sc;
+    l.num_hot = m;
+    return l;
+}
+
+#define MLGPU_STAGE_HOT(sc, sc_in) const DeviceScene sc = stage_hot(sc_in)
+
 #define MLGPU_STAGE_FUNCTIONS(sc, sc_arg)                                   \
     __shared__ __align__(16) unsigned char sc##_fn_lds[kLdsFunctionBytes]; \
     const DeviceScene sc = stage_functions(sc_arg, sc##_fn_lds)
@@ -197,7 +216,8 @@ __global__ __launch_bounds__(kBlock) void wf_first_kernel(const DeviceScene sc_a
     const int64_t n = cp ? cp->n : n_arg;
     const uint64_t seed = cp ? cp->seed : seed_arg;
     const uint64_t id_offset = cp ? cp->id_offset : id_offset_arg;
-    MLGPU_STAGE_FUNCTIONS(sc, sc_arg);
+    MLGPU_STAGE_FUNCTIONS(scf, sc_arg);
+    MLGPU_STAGE_HOT(sc, scf);
     MLGPU_LANE_STACK(stk);
     __shared__ TileQueue<kTile> q;
     if (threadIdx.x == 0) q.n = 0;
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void wf_first_kernel(const DeviceScene sc_a
                 rng.init(seed, photon_id(id_offset, ids, i), 0u);
                 if (st.err == MLGPU_PHOTON_OK && st.n_hist <= max_interactions) {
                     Closest c;
-                    first_hit(sc, st.root, st.s, st.d, c, stk);
+                    first_hit<true>(sc, st.root, st.s, st.d, c, stk);
                     go = photon_bounce(sc, st, c.leaf, c.t, in.wavelength[i], rng, max_interactions);
                 }
                 if (go) {
@@ -235,8 +255,8 @@ __global__ __launch_bounds__(kBlock) void wf_first_kernel(const DeviceScene sc_a
 // traversal state (tr_depth >= 0 marks them) and go on in wf_resume_kernel,
 // packed into full waves by wf_defer_kernel.  No barriers here: a block
 // barrier would make every wave wait for the slowest of its block.
-__global__ __launch_bounds__(kBlock, 5) void wf_trace_kernel(const DeviceScene sc, const Workspace ws, const int level,
-                                                             const int32_t budget) {
+__global__ __launch_bounds__(kBlock, 5) void wf_trace_kernel(const DeviceScene sc, const Workspace ws,
+                                                             const int level, const int32_t budget) {
     MLGPU_LANE_STACK(stk);
     const int32_t count = ws.counts[level];
     const int32_t* queue = ws.queue[level & 1];
@@ -283,7 +303,8 @@ __global__ __launch_bounds__(kBlock) void wf_defer_kernel(const Workspace ws, co
 }
 
 // Second pass: the deferred traversals of a level, run to the end.
-__global__ __launch_bounds__(kBlock) void wf_resume_kernel(const DeviceScene sc, const Workspace ws, const int level) {
+__global__ __launch_bounds__(kBlock) void wf_resume_kernel(const DeviceScene sc, const Workspace ws,
+                                                           const int level) {
     MLGPU_LANE_STACK(stk);
     const int32_t count = ws.counts[kDeferredCounts + level];
     const int32_t* queue = ws.queue[level & 1];

@@ -579,6 +579,30 @@ MLGPU_HD inline void try_leaf(const DeviceScene& sc, const RayF& r, const D3& s,
     }
 }
 
+#if defined(__HIPCC__)
+// The top of the world scope's tree in LDS, copied there by the kernels that
+// set DeviceScene::num_hot (one copy per block).
+static __shared__ PairNode hot_nodes[kHotNodes];
+#endif
+
+// Node k; with Hot, from the LDS copy of the top of the tree that the
+// calling kernel made (stage_hot).  Only the level-0 kernel gains from the
+// copy: its traces stay in the top of the tree, while the deeper traces run
+// slower with the extra branch in their node fetch.
+template <bool Hot>
+MLGPU_HD inline PairNode fetch_node(const DeviceScene& sc, const int32_t k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (Hot && k < sc.num_hot) {
+        const PairNode n = hot_nodes[k];
+        // keeps this LDS read apart from the global read below: merged, the
+        // two become one generic (flat) load through a selected address
+        __builtin_amdgcn_sched_barrier(0);
+        return n;
+    }
+#endif
+    return sc.pairs[k];
+}
+
 // Runs the traversal on, for at most `budget` steps (node visits and leaf
 // tests) when Budgeted; returns true when it has finished.  A traversal
 // stopped early resumes from ts and the stack, and ends with the same
@@ -590,7 +614,7 @@ MLGPU_HD inline void try_leaf(const DeviceScene& sc, const RayF& r, const D3& s,
 // the traversal order.
 // Nearer children are visited first so that the best hit shrinks early and
 // prunes more boxes.
-template <bool Budgeted>
+template <bool Budgeted, bool Hot = false>
 MLGPU_HD inline bool trace_run(const DeviceScene& sc, const RayF& r, const D3& s, const D3& d, TraceState& ts,
                                const LaneStack stk, int32_t budget) {
     // best distance relative to the float origin, plus margin
@@ -605,7 +629,7 @@ MLGPU_HD inline bool trace_run(const DeviceScene& sc, const RayF& r, const D3& s
             return false;
         }
         if (cur >= 0) {
-            const PairNode nd = sc.pairs[cur];
+            const PairNode nd = fetch_node<Hot>(sc, cur);
             const NodeRay nr = node_ray(nd, r);
             float n0, n1;
             const bool h0 = node_child_hit(nr, perm_bytes(nd.q[1], nd.q[0], r.sel_near),
@@ -640,7 +664,8 @@ MLGPU_HD inline bool trace_run(const DeviceScene& sc, const RayF& r, const D3& s
                 break;
             }
             const int32_t p = (-1 - e) & (kDeferredLeaf - 1);
-            const PairNode nd = sc.pairs[p >> 1];
+            const int32_t node = p >> 1;
+            const PairNode nd = fetch_node<Hot>(sc, node);
             const bool second = (p & 1) != 0;
             const NodeRay nr = node_ray(nd, r);
             const uint32_t w_hi = second ? nd.q[2] : nd.q[1], w_lo = second ? nd.q[1] : nd.q[0];
@@ -660,13 +685,14 @@ MLGPU_HD inline bool trace_run(const DeviceScene& sc, const RayF& r, const D3& s
 }
 
 // First intersection within scope `scope`.
+template <bool Hot = false>
 MLGPU_HD inline void first_hit(const DeviceScene& sc, const int32_t scope, const D3& s, const D3& d, Closest& best,
                                const LaneStack stk) {
     RayF r;
     ray_setup(sc, scope, s, d, r);
     TraceState ts;
     trace_begin(sc, scope, ts);
-    trace_run<false>(sc, r, s, d, ts, stk, 0);
+    trace_run<false, Hot>(sc, r, s, d, ts, stk, 0);
     best = ts.best;
 }
 

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <numeric>
 #include <string>
 #include <stdexcept>
@@ -517,6 +518,7 @@ class Builder {
             ph.allowed_scope = a >= 0 ? hs_->scope_of_frame[a] : -1;
             ph.root_scope = hs_->root_scope;
         }
+        hot_first();
         inflate();
     }
 
@@ -847,6 +849,43 @@ class Builder {
         // HexagonalPrismZ unit vectors V, W = (+-sin(2 pi/3), cos(2 pi/3)) (HexagonalPrismZ.h:10-16)
         hs_->hex_sin = std::sin(2. / 3. * M_PI);
         hs_->hex_cos = std::cos(2. / 3. * M_PI);
+    }
+
+    // Moves the top kHotNodes nodes of the world scope's tree (breadth first)
+    // to the front of the node array, where the traversal kernels copy them
+    // into LDS; every other node keeps its order.
+    void hot_first() {
+        const size_t m = hs_->pairs.size();
+        std::vector<int32_t> order;
+        order.reserve(m);
+        std::vector<char> taken(m, 0);
+        std::deque<int32_t> q{hs_->scopes[hs_->root_scope].entry};
+        while (!q.empty() && order.size() < static_cast<size_t>(kHotNodes)) {
+            const int32_t c = q.front();
+            q.pop_front();
+            if (c < 0 || taken[c]) continue;
+            taken[c] = 1;
+            order.push_back(c);
+            q.push_back(hs_->pairs[c].c0);
+            q.push_back(hs_->pairs[c].c1);
+        }
+        hs_->hot_count = static_cast<int32_t>(order.size());
+        for (size_t i = 0; i < m; i++)
+            if (!taken[i]) order.push_back(static_cast<int32_t>(i));
+        std::vector<int32_t> new_of(m);
+        for (size_t k = 0; k < m; k++) new_of[order[k]] = static_cast<int32_t>(k);
+        std::vector<PairNode> pairs(m);
+        std::vector<std::array<Box, 2>> boxes(m);
+        for (size_t k = 0; k < m; k++) {
+            pairs[k] = hs_->pairs[order[k]];
+            boxes[k] = pair_box_[order[k]];
+            if (pairs[k].c0 >= 0) pairs[k].c0 = new_of[pairs[k].c0];
+            if (pairs[k].c1 >= 0) pairs[k].c1 = new_of[pairs[k].c1];
+        }
+        for (Scope& sc : hs_->scopes) sc.entry = new_of[sc.entry];
+        hs_->pairs.swap(pairs);
+        pair_box_.swap(boxes);
+        for (auto& kv : pair_of_) kv.second = new_of[kv.second];
     }
 
     // IEEE half nearest to v (|v| <= 1, ties to even) and its value.

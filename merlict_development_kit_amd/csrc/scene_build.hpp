@@ -14,6 +14,7 @@ namespace mlgpu {
 struct HostScene {
     std::vector<PairNode> pairs;
     std::vector<Scope> scopes;
+    int32_t hot_count = 0;                           // nodes moved to the front (kHotNodes at most)
     int32_t tree_depth = 0;                          // longest chain of pair nodes
     std::vector<LeafGeo> geo;
     std::vector<LeafBox> boxes;
