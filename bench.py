@@ -32,12 +32,12 @@ HBM_PEAK = 8.0e12
 # HBM bytes per photon of the default workload (FACT with light guides, 1e7
 # photons, default tuning), rocprofv3 FETCH_SIZE + WRITE_SIZE of one propagation
 # pass scaled by a 1 GiB copy: profiles/propagation_pass_fact_lg_1e7_counters.md
-TRAFFIC_BYTES_PER_PHOTON = 834.0
+TRAFFIC_BYTES_PER_PHOTON = 835.0
 # VALU work of the same pass (profiles/propagation_pass_fact_lg_1e7_valu.md):
 # lane-FLOPs issued per photon and VALU wave instructions per photon
 VALU_FP32_FLOPS_PER_PHOTON = 4006.0
 VALU_FP64_FLOPS_PER_PHOTON = 2554.0
-VALU_INSTS_PER_PHOTON = 197.3
+VALU_INSTS_PER_PHOTON = 197.6
 FP32_PEAK, FP64_PEAK = 157.3e12, 78.6e12
 SIMDS, CLOCK_HZ = 256 * 4, 2.4e9
 IN_BYTES = 7 * 8            # support xyz, direction xyz, wavelength (float64)
@@ -890,9 +890,9 @@ def main() -> int:
             "interface_bytes_per_photon": IN_BYTES + OUT_BYTES,
             "interface_frac": n * (IN_BYTES + OUT_BYTES) / (ms_kernel * 1e-3) / HBM_PEAK,
             "measured_limiter": "the vector memory pipeline that serves per-lane gathers: its address and data "
-                                "units are busy 75 % / 93 % of the trace kernel's cycles and 79 % / 83 % of the "
-                                "shading kernel's (TA_TA_BUSY, TD_TD_BUSY); HBM carries 834 B per photon (18 % of "
-                                "peak) and VALU issue is at 54 % "
+                                "units are busy 75 % / 92 % of the trace kernel's cycles and 79 % / 83 % of the "
+                                "shading kernel's (TA_TA_BUSY, TD_TD_BUSY); HBM carries 835 B per photon (18 % of "
+                                "peak) and VALU issue is at 55 % "
                                 "(profiles/propagation_pass_fact_lg_1e7_counters.md)",
         },
     }
